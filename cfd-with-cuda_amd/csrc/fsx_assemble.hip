@@ -1,0 +1,325 @@
+// fsx_assemble.hip -- element-level kernels: static assembly (step0), the
+// numeric Z = sum_d G_d^T diag(MdOrigInv) G_d, and the matrix-free
+// convection residual.
+//
+// Element kernels run one launch per colour of a race-free element colouring,
+// so the scatter-add into CSR values and into R1 needs no atomics and the
+// result is bitwise reproducible run to run.
+#include "fsx_device.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fsx {
+
+// Jacobian of the trilinear (hex8) map at one Gauss point, its cofactor
+// inverse and determinant, in the arithmetic of calcJacob (BCH:2553-2590):
+// J[i][j] = sum_m dSp[m][i] x_m[j].
+__device__ __forceinline__ void jacobian(const double (*ec)[3], const double* __restrict__ dSp_k, double iJ[3][3],
+                                         double* det) {
+  double J[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double s = 0;
+#pragma unroll
+      for (int m = 0; m < 8; m++) s = s + dSp_k[m * 3 + i] * ec[m][j];
+      J[i][j] = s;
+    }
+  iJ[0][0] =   J[1][1]*J[2][2] - J[2][1]*J[1][2];
+  iJ[0][1] = -(J[0][1]*J[2][2] - J[0][2]*J[2][1]);
+  iJ[0][2] =   J[0][1]*J[1][2] - J[1][1]*J[0][2];
+  iJ[1][0] = -(J[1][0]*J[2][2] - J[1][2]*J[2][0]);
+  iJ[1][1] =   J[2][2]*J[0][0] - J[2][0]*J[0][2];
+  iJ[1][2] = -(J[1][2]*J[0][0] - J[1][0]*J[0][2]);
+  iJ[2][0] =   J[1][0]*J[2][1] - J[2][0]*J[1][1];
+  iJ[2][1] = -(J[2][1]*J[0][0] - J[2][0]*J[0][1]);
+  iJ[2][2] =   J[1][1]*J[0][0] - J[1][0]*J[0][1];
+  const double d = J[0][0]*(J[1][1]*J[2][2] - J[2][1]*J[1][2]) +
+                   J[0][1]*(J[1][2]*J[2][0] - J[1][0]*J[2][2]) +
+                   J[0][2]*(J[1][0]*J[2][1] - J[1][1]*J[2][0]);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) iJ[i][j] = iJ[i][j] / d;
+  *det = d;
+}
+
+// ------------------------------------------------------------ step0
+// One wavefront per element of the current colour. Per element (BCH:3176-3231):
+//   Me_ij = sum_k Sv_ki Sv_kj f_k,  Ke_ij = nu sum_k gradSv_i . gradSv_j f_k,
+//   Ge_d,ij = -(1/rho) sum_k Sp_kj d_d Sv_ki f_k,  f_k = detJ_k w_k,
+// scattered into the CSR values through a binary search of the sorted row.
+constexpr int kMaxNGP = 27;
+
+__global__ __launch_bounds__(kWave) void k_assemble(int ncolor_elems, const int* __restrict__ elems,
+                                                    const int* __restrict__ L27, const double* __restrict__ xc,
+                                                    ShapeTables tab, int NGP, double nu, double inv_rho,
+                                                    const int* __restrict__ Mrow, const int* __restrict__ Mcol,
+                                                    double* __restrict__ Mv, double* __restrict__ Kv,
+                                                    const int* __restrict__ Grow, const int* __restrict__ Gcol,
+                                                    double* __restrict__ Gv) {
+  __shared__ double gd[kMaxNGP * 81];   // [k][j][d]
+  __shared__ double fac[kMaxNGP];
+  __shared__ double ec[8][3];
+  __shared__ int L[27];
+  const int t = threadIdx.x;
+  const int e = elems[blockIdx.x];
+  if (t < 27) L[t] = L27[(size_t)e * 27 + t];
+  __syncthreads();
+  if (t < 24) ec[t / 3][t % 3] = xc[3 * (size_t)L[t / 3] + t % 3];
+  __syncthreads();
+  if (t < NGP) {
+    double iJ[3][3], det;
+    jacobian(ec, tab.dSp + t * 24, iJ, &det);
+    for (int j = 0; j < 27; j++) {
+      const double* ds = tab.dSv + (t * 27 + j) * 3;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        double s = 0;
+#pragma unroll
+        for (int m = 0; m < 3; m++) s = s + iJ[i][m] * ds[m];
+        gd[(t * 27 + j) * 3 + i] = s;
+      }
+    }
+    fac[t] = det * tab.w[t];
+  }
+  __syncthreads();
+  for (int idx = t; idx < 729; idx += kWave) {
+    const int i = idx / 27, j = idx % 27;
+    double me = 0.0, ke = 0.0;
+    for (int k = 0; k < NGP; k++) {
+      const double f = fac[k];
+      const double* gi = gd + (k * 27 + i) * 3;
+      const double* gj = gd + (k * 27 + j) * 3;
+      me = me + tab.Sv[k * 27 + i] * tab.Sv[k * 27 + j] * f;
+      ke = ke + nu * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]) * f;
+    }
+    const int r = L[i];
+    const int pos = find_col(Mcol, Mrow[r], Mrow[r + 1], L[j]);
+    Mv[pos] += me;
+    Kv[pos] += ke;
+  }
+  for (int idx = t; idx < 216; idx += kWave) {
+    const int i = idx / 8, j = idx % 8;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int k = 0; k < NGP; k++) {
+      const double f = fac[k];
+      const double* gi = gd + (k * 27 + i) * 3;
+      const double sp = tab.Sp[k * 8 + j];
+      g0 = g0 - inv_rho * sp * gi[0] * f;
+      g1 = g1 - inv_rho * sp * gi[1] * f;
+      g2 = g2 - inv_rho * sp * gi[2] * f;
+    }
+    const int r = L[i];
+    const int pos = find_col(Gcol, Grow[r], Grow[r + 1], L[j]);
+    Gv[3 * (size_t)pos] += g0;
+    Gv[3 * (size_t)pos + 1] += g1;
+    Gv[3 * (size_t)pos + 2] += g2;
+  }
+}
+
+// Lumped mass: Md = row sums of M (BCH:3262-3266), MdOrigInv = 1/Md
+// (BCH:3287-3289), MdInv = 1/Md with BC rows 1 (BCH:3292-3295, 4121-4135).
+__global__ void k_lump(int NN, const int* __restrict__ Mrow, const double* __restrict__ Mv,
+                       const unsigned char* __restrict__ bc, double* __restrict__ Md, double* __restrict__ MdOrigInv,
+                       double* __restrict__ MdInv) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < NN; r += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = Mrow[r]; k < Mrow[r + 1]; k++) s += Mv[k];
+    Md[r] = s;
+    MdOrigInv[r] = 1.0 / s;
+    MdInv[r] = bc[r] ? 1.0 : 1.0 / s;
+  }
+}
+
+// Values of G in the transposed pattern.
+__global__ void k_gt_values(long long nnz, const int* __restrict__ GTpos, const double* __restrict__ G,
+                            double* __restrict__ GT) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nnz; t += (long long)gridDim.x * blockDim.x) {
+    const size_t k = GTpos[t];
+    GT[3 * t] = G[3 * k]; GT[3 * t + 1] = G[3 * k + 1]; GT[3 * t + 2] = G[3 * k + 2];
+  }
+}
+
+int launch_assemble_static(fsx_ctx* ctx) {
+  FSX_TRY(launch_fill(ctx, ctx->M, 0.0, ctx->nnzM));
+  FSX_TRY(launch_fill(ctx, ctx->K, 0.0, ctx->nnzM));
+  FSX_TRY(launch_fill(ctx, ctx->G, 0.0, 3 * ctx->nnzG));
+  const int ncol = (int)ctx->color_ptr.size() - 1;
+  for (int c = 0; c < ncol; c++) {
+    const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_assemble, dim3(n), dim3(kWave), 0, ctx->stream, n, ctx->color_order + ctx->color_ptr[c], ctx->L27,
+                       ctx->xc, ctx->tab, ctx->NGP, ctx->viscosity, 1.0 / ctx->density, ctx->Mrow, ctx->Mcol, ctx->M, ctx->K,
+                       ctx->Grow, ctx->Gcol, ctx->G);
+    FSX_TRY(launch_check(ctx, "k_assemble"));
+  }
+  const int g = std::max(1, std::min((ctx->NN + kBlock - 1) / kBlock, 4096));
+  hipLaunchKernelGGL(k_lump, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->M, ctx->bcflag, ctx->Md,
+                     ctx->MdOrigInv, ctx->MdInv);
+  FSX_TRY(launch_check(ctx, "k_lump"));
+  hipLaunchKernelGGL(k_gt_values, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzG, ctx->GTpos, ctx->G, ctx->GT);
+  return launch_check(ctx, "k_gt_values");
+}
+
+// ------------------------------------------------------------ Z
+// One wavefront per pressure row p. The row's structural columns sit in LDS;
+// for each velocity row i of G^T(p) in ascending order the lanes take the
+// entries (i, q) of G row i, find q in the Z row and add
+// G_c(i,p) * (G_c(i,q) * MdOrigInv[i]) to the c-th partial sum. Rows of G
+// have distinct columns, so no two lanes of one step hit the same slot; the
+// steps are ordered by a barrier. Z = (Z1 + Z2) + Z3 reproduces the value
+// order of three cs_multiply calls and two value-wise adds (BCH:3384-3457).
+__global__ __launch_bounds__(kWave) void k_build_z(int NNp, const int* __restrict__ Zrow, const int* __restrict__ Zcol,
+                                                   double* __restrict__ Z, const int* __restrict__ GTrow,
+                                                   const int* __restrict__ GTcol, const double* __restrict__ GT,
+                                                   const int* __restrict__ Grow, const int* __restrict__ Gcol,
+                                                   const double* __restrict__ G, const double* __restrict__ MdOrigInv) {
+  __shared__ int zc[kZRowMax];
+  __shared__ double z1[kZRowMax], z2[kZRowMax], z3[kZRowMax];
+  const int p = blockIdx.x;
+  const int t = threadIdx.x;
+  const int zb = Zrow[p], zn = Zrow[p + 1] - zb;
+  for (int s = t; s < zn; s += kWave) { zc[s] = Zcol[zb + s]; z1[s] = 0.0; z2[s] = 0.0; z3[s] = 0.0; }
+  __syncthreads();
+  for (int k = GTrow[p]; k < GTrow[p + 1]; k++) {
+    const int i = GTcol[k];
+    const double a0 = GT[3 * (size_t)k], a1 = GT[3 * (size_t)k + 1], a2 = GT[3 * (size_t)k + 2];
+    const double m = MdOrigInv[i];
+    const int gb = Grow[i], ge = Grow[i + 1];
+    for (int g = gb + t; g < ge; g += kWave) {
+      const int s = find_col(zc, 0, zn, Gcol[g]);
+      z1[s] += a0 * (G[3 * (size_t)g] * m);
+      z2[s] += a1 * (G[3 * (size_t)g + 1] * m);
+      z3[s] += a2 * (G[3 * (size_t)g + 2] * m);
+    }
+    __syncthreads();
+  }
+  for (int s = t; s < zn; s += kWave) Z[zb + s] = (z1[s] + z2[s]) + z3[s];
+}
+
+// Penalty pin (BCH:4167-4177, only for node > 0), Jacobi inverse diagonal
+// and the FP32 copy used by the mixed-precision inner solve.
+__global__ void k_z_finish(int NNp, const int* __restrict__ Zrow, const int* __restrict__ Zcol, double* __restrict__ Z,
+                           float* __restrict__ Zf, double* __restrict__ dinv, float* __restrict__ dinvf, int zeroP) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < NNp; p += gridDim.x * blockDim.x) {
+    double d = 0.0;
+    for (int k = Zrow[p]; k < Zrow[p + 1]; k++) {
+      double v = Z[k];
+      if (Zcol[k] == p) {
+        if (zeroP > 0 && p == zeroP) { v = v * 1000.0; Z[k] = v; }
+        d = v;
+      }
+      Zf[k] = (float)v;
+    }
+    dinv[p] = 1.0 / d;
+    dinvf[p] = (float)(1.0 / d);
+  }
+}
+
+int launch_build_Z(fsx_ctx* ctx) {
+  hipLaunchKernelGGL(k_build_z, dim3(ctx->NNp), dim3(kWave), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z,
+                     ctx->GTrow, ctx->GTcol, ctx->GT, ctx->Grow, ctx->Gcol, ctx->G, ctx->MdOrigInv);
+  FSX_TRY(launch_check(ctx, "k_build_z"));
+  const int g = std::max(1, std::min((ctx->NNp + kBlock - 1) / kBlock, 4096));
+  hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z, ctx->Zf,
+                     ctx->Zdinv, ctx->Zdinvf, ctx->zeroP);
+  return launch_check(ctx, "k_z_finish");
+}
+
+// ------------------------------------------------------------ convection
+// R1 -= A(Un) UnpHalf_prev element by element, matrix free (BCH:3560-3654):
+//   R1e_c,i = sum_k Sv_ki f_k (u0_k . grad uprev_c(k)),
+//   u0_k = sum_j Sv_kj Un_j,  grad uprev_c(k) = J_k^-1 sum_j dSv_j(k) uprev_c,j,
+// which equals sum_j Ae_ij uprev_c,j of the reference's explicit Ae. One
+// wavefront per element, four elements per block; the shape tables are staged
+// in LDS once per block.
+constexpr int kConvElems = 4;
+
+__global__ __launch_bounds__(kWave * kConvElems) void k_convection(int ncolor_elems, const int* __restrict__ elems,
+                                                                   const int* __restrict__ L27,
+                                                                   const double* __restrict__ xc, ShapeTables tab, int NGP,
+                                                                   int NN, const double* __restrict__ Un,
+                                                                   const double* __restrict__ Uprev, double* __restrict__ R1) {
+  __shared__ double sSv[kMaxNGP * 27];
+  __shared__ double sdSv[kMaxNGP * 81];
+  __shared__ double sdSp[kMaxNGP * 24];
+  __shared__ double u0n[kConvElems][3][27], upn[kConvElems][3][27], conv[kConvElems][3][kMaxNGP];
+  __shared__ double ec[kConvElems][8][3];
+  __shared__ int L[kConvElems][27];
+  const int t = threadIdx.x;
+  const int w = t / kWave, lane = t % kWave;
+  for (int i = t; i < NGP * 27; i += blockDim.x) sSv[i] = tab.Sv[i];
+  for (int i = t; i < NGP * 81; i += blockDim.x) sdSv[i] = tab.dSv[i];
+  for (int i = t; i < NGP * 24; i += blockDim.x) sdSp[i] = tab.dSp[i];
+  const int slot = blockIdx.x * kConvElems + w;
+  const bool active = slot < ncolor_elems;
+  const int e = active ? elems[slot] : 0;
+  if (active && lane < 27) {
+    const int n = L27[(size_t)e * 27 + lane];
+    L[w][lane] = n;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      u0n[w][c][lane] = Un[(size_t)c * NN + n];
+      upn[w][c][lane] = Uprev[(size_t)c * NN + n];
+    }
+  }
+  __syncthreads();
+  if (active && lane < 24) ec[w][lane / 3][lane % 3] = xc[3 * (size_t)L[w][lane / 3] + lane % 3];
+  __syncthreads();
+  if (active && lane < NGP) {
+    const int k = lane;
+    double iJ[3][3], det;
+    jacobian(ec[w], sdSp + k * 24, iJ, &det);
+    const double f = det * tab.w[k];
+    double u0[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int j = 0; j < 27; j++) {
+      const double sv = sSv[k * 27 + j];
+      const double d0 = sdSv[(k * 27 + j) * 3], d1 = sdSv[(k * 27 + j) * 3 + 1], d2 = sdSv[(k * 27 + j) * 3 + 2];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        u0[c] += sv * u0n[w][c][j];
+        const double up = upn[w][c][j];
+        gr[c][0] += d0 * up; gr[c][1] += d1 * up; gr[c][2] += d2 * up;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double g[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) g[d] = iJ[d][0] * gr[c][0] + iJ[d][1] * gr[c][1] + iJ[d][2] * gr[c][2];
+      conv[w][c][k] = f * (u0[0] * g[0] + u0[1] * g[1] + u0[2] * g[2]);
+    }
+  }
+  __syncthreads();
+  if (active && lane < 27) {
+    const int i = lane;
+    double r[3] = {0, 0, 0};
+    for (int k = 0; k < NGP; k++) {
+      const double sv = sSv[k * 27 + i];
+#pragma unroll
+      for (int c = 0; c < 3; c++) r[c] += sv * conv[w][c][k];
+    }
+    const int n = L[w][i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) R1[(size_t)c * NN + n] -= r[c];
+  }
+}
+
+int launch_convection(fsx_ctx* ctx) {
+  FSX_TRY(launch_fill(ctx, ctx->R1, 0.0, 3LL * ctx->NN));                  // BCH:3541-3545
+  const int ncol = (int)ctx->color_ptr.size() - 1;
+  for (int c = 0; c < ncol; c++) {
+    const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
+    if (n == 0) continue;
+    const int g = (n + kConvElems - 1) / kConvElems;
+    hipLaunchKernelGGL(k_convection, dim3(g), dim3(kWave * kConvElems), 0, ctx->stream, n,
+                       ctx->color_order + ctx->color_ptr[c], ctx->L27, ctx->xc, ctx->tab, ctx->NGP, ctx->NN, ctx->Un,
+                       ctx->UnpHalf_prev, ctx->R1);
+  }
+  return launch_check(ctx, "k_convection");
+}
+
+}  // namespace fsx

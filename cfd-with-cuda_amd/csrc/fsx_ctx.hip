@@ -1,0 +1,593 @@
+// fsx_ctx.hip -- context lifecycle, uploads/downloads and the step-level
+// C-ABI of the fractional-step core (include/fsx.h). The kernels live in
+// fsx_assemble.hip (element work) and fsx_solve.hip (sparse, solver, steps).
+#include "fsx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+namespace fsx {
+
+int dev_alloc(fsx_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    ctx->err = "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return FSX_ENOMEM;
+  }
+  ctx->allocs.push_back(*p);
+  ctx->bytes += bytes;
+  return FSX_OK;
+}
+
+void dev_free(fsx_ctx* ctx, void* p) {
+  if (!p) return;
+  auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
+  if (it != ctx->allocs.end()) {
+    ctx->allocs.erase(it);
+    (void)hipFree(p);
+  }
+}
+
+int fail(fsx_ctx* ctx, int code, const std::string& msg) {
+  ctx->err = msg;
+  return code;
+}
+
+int launch_check(fsx_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, FSX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FSX_OK;
+}
+
+int row_grid(long long nrows, int rows_per_block) {
+  long long g = (nrows + rows_per_block - 1) / rows_per_block;
+  if (g > kMaxGrid) g = kMaxGrid;
+  if (g < 1) g = 1;
+  g = (g + 7) / 8 * 8;   // a multiple of the 8 XCDs
+  return (int)g;
+}
+
+// Parametric position (-1, 0, +1) of each hex27 node (BCH:2312-2343) and the
+// 1-D quadratic Lagrange factors used there: (x^2-x)/2, 1-x^2, (x^2+x)/2.
+static const int kNodePos[27][3] = {
+  {-1,-1,-1},{1,-1,-1},{1,1,-1},{-1,1,-1},{-1,-1,1},{1,-1,1},{1,1,1},{-1,1,1},
+  {0,-1,-1},{1,0,-1},{0,1,-1},{-1,0,-1},
+  {-1,-1,0},{1,-1,0},{1,1,0},{-1,1,0},
+  {0,-1,1},{1,0,1},{0,1,1},{-1,0,1},
+  {0,0,-1},{0,-1,0},{1,0,0},{0,1,0},{-1,0,0},{0,0,1},{0,0,0}};
+static double lag(int a, double x) { return a < 0 ? 0.5 * (x*x - x) : (a > 0 ? 0.5 * (x*x + x) : (1 - x*x)); }
+static double dlag(int a, double x) { return a < 0 ? 0.5 * (2*x - 1) : (a > 0 ? 0.5 * (2*x + 1) : (-2*x)); }
+
+// Gauss rule (BCH:2166-2208; NGP 27 = tensor 3-point rule, xi fastest) and
+// shape tables (BCH:2215-2488) in the GP-major layout of ShapeTables.
+static void build_tables(int NGP, std::vector<double>& out) {
+  std::vector<double> gp(NGP * 3, 0.0), gw(NGP, 0.0);
+  if (NGP == 1) {
+    gw[0] = 4.0;
+  } else if (NGP == 8) {
+    const double a = std::sqrt(1. / 3);
+    for (int k = 0; k < 8; k++) {
+      gp[3*k] = (k & 1) ? a : -a; gp[3*k+1] = (k & 2) ? a : -a; gp[3*k+2] = (k & 4) ? a : -a; gw[k] = 1.0;
+    }
+  } else {
+    const double p[3] = {-std::sqrt(0.6), 0.0, std::sqrt(0.6)};
+    const double w[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    for (int kz = 0; kz < 3; kz++) for (int ky = 0; ky < 3; ky++) for (int kx = 0; kx < 3; kx++) {
+      int k = kx + 3*ky + 9*kz;
+      gp[3*k] = p[kx]; gp[3*k+1] = p[ky]; gp[3*k+2] = p[kz]; gw[k] = w[kx] * w[ky] * w[kz];
+    }
+  }
+  out.assign((size_t)NGP * (27 + 81 + 8 + 24 + 1), 0.0);
+  double* Sv = out.data();
+  double* dSv = Sv + NGP * 27;
+  double* Sp = dSv + NGP * 81;
+  double* dSp = Sp + NGP * 8;
+  double* w = dSp + NGP * 24;
+  for (int k = 0; k < NGP; k++) {
+    double x = gp[3*k], y = gp[3*k+1], z = gp[3*k+2];
+    for (int i = 0; i < 27; i++) {
+      int a = kNodePos[i][0], b = kNodePos[i][1], c = kNodePos[i][2];
+      Sv[k*27 + i] = lag(a, x) * lag(b, y) * lag(c, z);
+      dSv[(k*27 + i)*3 + 0] = dlag(a, x) * lag(b, y) * lag(c, z);
+      dSv[(k*27 + i)*3 + 1] = lag(a, x) * dlag(b, y) * lag(c, z);
+      dSv[(k*27 + i)*3 + 2] = lag(a, x) * lag(b, y) * dlag(c, z);
+    }
+    for (int j = 0; j < 8; j++) {
+      double sx = kNodePos[j][0], sy = kNodePos[j][1], sz = kNodePos[j][2];
+      Sp[k*8 + j] = 0.125 * (1 + sx*x) * (1 + sy*y) * (1 + sz*z);
+      dSp[(k*8 + j)*3 + 0] = 0.125 * sx * (1 + sy*y) * (1 + sz*z);
+      dSp[(k*8 + j)*3 + 1] = 0.125 * (1 + sx*x) * sy * (1 + sz*z);
+      dSp[(k*8 + j)*3 + 2] = 0.125 * (1 + sx*x) * (1 + sy*y) * sz;
+    }
+    w[k] = gw[k];
+  }
+}
+
+}  // namespace fsx
+
+using namespace fsx;
+
+extern "C" {
+
+int fsx_create(int device, fsx_ctx** out) {
+  if (!out) return FSX_EINVAL;
+  *out = nullptr;
+  auto* ctx = new fsx_ctx();
+  ctx->device = device;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0 || device < 0 || device >= n) {
+    delete ctx;
+    return FSX_EHIP;
+  }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return FSX_EHIP;
+  }
+  int rc = alloc(ctx, &ctx->partials, 8 * (size_t)kMaxGrid);
+  if (rc == FSX_OK) rc = alloc(ctx, &ctx->sc, 1);
+  if (rc == FSX_OK) rc = alloc(ctx, &ctx->st, 1);
+  if (rc == FSX_OK) rc = alloc(ctx, &ctx->d_mon, 4);
+  if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_sc, sizeof(SolveScalars)) != hipSuccess) rc = FSX_ENOMEM;
+  if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_st, sizeof(StepScalars)) != hipSuccess) rc = FSX_ENOMEM;
+  if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_mon, 4 * sizeof(double)) != hipSuccess) rc = FSX_ENOMEM;
+  if (rc == FSX_OK && (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)) rc = FSX_EHIP;
+  if (rc == FSX_OK) {
+    if (hipMemsetAsync(ctx->sc, 0, sizeof(SolveScalars), ctx->stream) != hipSuccess ||
+        hipMemsetAsync(ctx->st, 0, sizeof(StepScalars), ctx->stream) != hipSuccess)
+      rc = FSX_EHIP;
+  }
+  if (rc != FSX_OK) {
+    fsx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return FSX_OK;
+}
+
+void fsx_destroy(fsx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->allocs) (void)hipFree(p);
+  if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
+  if (ctx->h_st) (void)hipHostFree(ctx->h_st);
+  if (ctx->h_mon) (void)hipHostFree(ctx->h_mon);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* fsx_last_error(const fsx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+size_t fsx_device_bytes(const fsx_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, double dt) {
+  if (!ctx) return FSX_EINVAL;
+  if (NGP != 1 && NGP != 8 && NGP != 27) return fail(ctx, FSX_EINVAL, "NGP must be 1, 8 or 27");
+  if (!(density > 0) || !(viscosity >= 0) || !(dt > 0)) return fail(ctx, FSX_EINVAL, "density, viscosity, dt");
+  if (ctx->assembled && NGP != ctx->NGP) return fail(ctx, FSX_ESTATE, "NGP cannot change after assembly");
+  std::vector<double> t;
+  build_tables(NGP, t);
+  if (ctx->tab_mem) dev_free(ctx, ctx->tab_mem);
+  FSX_TRY(alloc(ctx, &ctx->tab_mem, t.size()));
+  FSX_HIP(hipMemcpy(ctx->tab_mem, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  ctx->tab.Sv = ctx->tab_mem;
+  ctx->tab.dSv = ctx->tab.Sv + NGP * 27;
+  ctx->tab.Sp = ctx->tab.dSv + NGP * 81;
+  ctx->tab.dSp = ctx->tab.Sp + NGP * 8;
+  ctx->tab.w = ctx->tab.dSp + NGP * 24;
+  ctx->NGP = NGP; ctx->density = density; ctx->viscosity = viscosity; ctx->dt = dt;
+  ctx->have_params = true;
+  return FSX_OK;
+}
+
+int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit) {
+  if (!ctx) return FSX_EINVAL;
+  if (mode < FSX_SOLVER_FP64_JACOBI || mode > FSX_SOLVER_LEARNED) return fail(ctx, FSX_EINVAL, "unknown solver mode");
+  if (mode == FSX_SOLVER_LEARNED) return fail(ctx, FSX_EINVAL, "learned preconditioner not available in this build");
+  if (!(rtol >= 0) || maxit < 1) return fail(ctx, FSX_EINVAL, "rtol >= 0 and maxit >= 1 required");
+  ctx->solver_mode = mode; ctx->rtol = rtol; ctx->maxit = maxit;
+  return FSX_OK;
+}
+
+int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, const double* corner_coord,
+                    int ncolors, const int* color_ptr, const int* color_order) {
+  if (!ctx || NE <= 0 || NN <= 0 || NNp <= 0 || !LtoGnode || !corner_coord || ncolors <= 0 || !color_ptr || !color_order)
+    return ctx ? fail(ctx, FSX_EINVAL, "fsx_upload_mesh: bad arguments") : FSX_EINVAL;
+  if (ctx->have_mesh) return fail(ctx, FSX_ESTATE, "mesh already uploaded");
+  // host-side validation: ids in range, corners are pressure nodes, the
+  // colour table is a permutation of the elements
+  for (size_t i = 0; i < (size_t)NE * 27; i++) {
+    int n = LtoGnode[i];
+    if (n < 0 || n >= NN || ((i % 27) < 8 && n >= NNp)) return fail(ctx, FSX_EINVAL, "LtoGnode entry out of range");
+  }
+  if (color_ptr[0] != 0 || color_ptr[ncolors] != NE) return fail(ctx, FSX_EINVAL, "colour offsets");
+  {
+    std::vector<char> seen(NE, 0);
+    for (int i = 0; i < NE; i++) {
+      int e = color_order[i];
+      if (e < 0 || e >= NE || seen[e]) return fail(ctx, FSX_EINVAL, "colour order is not a permutation");
+      seen[e] = 1;
+    }
+    // no two elements of one colour share a node
+    std::vector<int> owner(NN, -1);
+    for (int c = 0; c < ncolors; c++) {
+      if (color_ptr[c + 1] < color_ptr[c]) return fail(ctx, FSX_EINVAL, "colour offsets");
+      for (int i = color_ptr[c]; i < color_ptr[c + 1]; i++) {
+        int e = color_order[i];
+        for (int j = 0; j < 27; j++) {
+          int n = LtoGnode[(size_t)e*27 + j];
+          if (owner[n] == c) return fail(ctx, FSX_EINVAL, "two elements of one colour share a node");
+          owner[n] = c;
+        }
+      }
+    }
+  }
+  ctx->NE = NE; ctx->NN = NN; ctx->NNp = NNp;
+  FSX_TRY(alloc(ctx, &ctx->L27, (size_t)NE * 27));
+  FSX_TRY(alloc(ctx, &ctx->xc, (size_t)NNp * 3));
+  FSX_TRY(alloc(ctx, &ctx->color_order, (size_t)NE));
+  FSX_HIP(hipMemcpy(ctx->L27, LtoGnode, (size_t)NE * 27 * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->xc, corner_coord, (size_t)NNp * 3 * sizeof(double), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->color_order, color_order, (size_t)NE * sizeof(int), hipMemcpyHostToDevice));
+  ctx->color_ptr.assign(color_ptr, color_ptr + ncolors + 1);
+  // state vectors
+  const size_t n3 = (size_t)3 * NN;
+  for (double** v : {&ctx->Un, &ctx->Unp1, &ctx->Unp1_prev, &ctx->UnpHalf, &ctx->UnpHalf_prev, &ctx->Acc, &ctx->Acc_prev,
+                     &ctx->KAcc, &ctx->R1})
+    FSX_TRY(alloc(ctx, v, n3));
+  for (double** v : {&ctx->Pn, &ctx->Pnp1, &ctx->Pnp1_prev, &ctx->Pdot, &ctx->R2, &ctx->wr, &ctx->wp, &ctx->wq, &ctx->wx})
+    FSX_TRY(alloc(ctx, v, (size_t)NNp));
+  for (float** v : {&ctx->fr, &ctx->fp, &ctx->fq, &ctx->fx}) FSX_TRY(alloc(ctx, v, (size_t)NNp));
+  FSX_TRY(alloc(ctx, &ctx->MdInv, (size_t)NN));
+  FSX_TRY(alloc(ctx, &ctx->MdOrigInv, (size_t)NN));
+  FSX_TRY(alloc(ctx, &ctx->Md, (size_t)NN));
+  FSX_TRY(alloc(ctx, &ctx->bcflag, (size_t)NN));
+  FSX_HIP(hipMemsetAsync(ctx->bcflag, 0, NN, ctx->stream));
+  ctx->have_mesh = true;
+  return FSX_OK;
+}
+
+int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const int* col) {
+  if (!ctx || !rowptr || !col) return ctx ? fail(ctx, FSX_EINVAL, "fsx_upload_csr: null") : FSX_EINVAL;
+  if (!ctx->have_mesh) return fail(ctx, FSX_ESTATE, "upload the mesh first");
+  int ncols;
+  if (kind == FSX_CSR_M) { if (nrows != ctx->NN) return fail(ctx, FSX_EINVAL, "M rows != NN"); ncols = ctx->NN; }
+  else if (kind == FSX_CSR_G) { if (nrows != ctx->NN) return fail(ctx, FSX_EINVAL, "G rows != NN"); ncols = ctx->NNp; }
+  else if (kind == FSX_CSR_Z) { if (nrows != ctx->NNp) return fail(ctx, FSX_EINVAL, "Z rows != NNp"); ncols = ctx->NNp; }
+  else return fail(ctx, FSX_EINVAL, "unknown CSR kind");
+  if (rowptr[0] != 0) return fail(ctx, FSX_EINVAL, "rowptr[0] != 0");
+  int rmax = 0;
+  for (int r = 0; r < nrows; r++) {
+    if (rowptr[r + 1] < rowptr[r]) return fail(ctx, FSX_EINVAL, "rowptr not monotone");
+    rmax = std::max(rmax, rowptr[r + 1] - rowptr[r]);
+    for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+      if (col[k] < 0 || col[k] >= ncols) return fail(ctx, FSX_EINVAL, "column out of range");
+      if (k > rowptr[r] && col[k] <= col[k - 1]) return fail(ctx, FSX_EINVAL, "columns not sorted/unique");
+    }
+  }
+  const long long nnz = rowptr[nrows];
+  int* drow; int* dcol;
+  FSX_TRY(alloc(ctx, &drow, (size_t)nrows + 1));
+  FSX_TRY(alloc(ctx, &dcol, (size_t)nnz));
+  FSX_HIP(hipMemcpy(drow, rowptr, ((size_t)nrows + 1) * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(dcol, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+  if (kind == FSX_CSR_M) {
+    ctx->Mrow = drow; ctx->Mcol = dcol; ctx->nnzM = nnz;
+    FSX_TRY(alloc(ctx, &ctx->K, (size_t)nnz));
+    FSX_TRY(alloc(ctx, &ctx->M, (size_t)nnz));
+    ctx->have_M = true;
+  } else if (kind == FSX_CSR_G) {
+    ctx->Grow = drow; ctx->Gcol = dcol; ctx->nnzG = nnz;
+    FSX_TRY(alloc(ctx, &ctx->G, (size_t)nnz * 3));
+    // transposed pattern: rows = pressure nodes, entries in ascending velocity row
+    const int NNp = ctx->NNp;
+    std::vector<int> trow(NNp + 1, 0), tcol(nnz), tpos(nnz);
+    for (long long k = 0; k < nnz; k++) trow[col[k] + 1]++;
+    for (int p = 0; p < NNp; p++) trow[p + 1] += trow[p];
+    std::vector<int> next(trow.begin(), trow.end() - 1);
+    for (int r = 0; r < nrows; r++)
+      for (int k = rowptr[r]; k < rowptr[r + 1]; k++) { int p = col[k]; tcol[next[p]] = r; tpos[next[p]] = k; next[p]++; }
+    FSX_TRY(alloc(ctx, &ctx->GTrow, (size_t)NNp + 1));
+    FSX_TRY(alloc(ctx, &ctx->GTcol, (size_t)nnz));
+    FSX_TRY(alloc(ctx, &ctx->GTpos, (size_t)nnz));
+    FSX_TRY(alloc(ctx, &ctx->GT, (size_t)nnz * 3));
+    FSX_HIP(hipMemcpy(ctx->GTrow, trow.data(), ((size_t)NNp + 1) * sizeof(int), hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(ctx->GTcol, tcol.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(ctx->GTpos, tpos.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+    ctx->have_G = true;
+  } else {
+    if (rmax > kZRowMax) return fail(ctx, FSX_EINVAL, "a Z row is longer than the Z-build kernel supports");
+    for (int r = 0; r < nrows; r++) {
+      bool diag = false;
+      for (int k = rowptr[r]; k < rowptr[r + 1]; k++) diag |= (col[k] == r);
+      if (!diag) return fail(ctx, FSX_EINVAL, "Z pattern lacks a diagonal entry");
+    }
+    ctx->Zrow = drow; ctx->Zcol = dcol; ctx->nnzZ = nnz; ctx->zrow_max = rmax;
+    FSX_TRY(alloc(ctx, &ctx->Z, (size_t)nnz));
+    FSX_TRY(alloc(ctx, &ctx->Zf, (size_t)nnz));
+    FSX_TRY(alloc(ctx, &ctx->Zdinv, (size_t)nrows));
+    FSX_TRY(alloc(ctx, &ctx->Zdinvf, (size_t)nrows));
+    ctx->have_Z = true;
+  }
+  return FSX_OK;
+}
+
+int fsx_upload_bc(fsx_ctx* ctx, int n, const int* nodes, const double* values, int zero_p_node) {
+  if (!ctx || n < 0 || (n > 0 && (!nodes || !values))) return ctx ? fail(ctx, FSX_EINVAL, "fsx_upload_bc") : FSX_EINVAL;
+  if (!ctx->have_mesh) return fail(ctx, FSX_ESTATE, "upload the mesh first");
+  if (zero_p_node >= ctx->NNp) return fail(ctx, FSX_EINVAL, "zero-pressure node out of range");
+  std::vector<unsigned char> flag(ctx->NN, 0);
+  for (int i = 0; i < n; i++) {
+    if (nodes[i] < 0 || nodes[i] >= ctx->NN) return fail(ctx, FSX_EINVAL, "BC node out of range");
+    flag[nodes[i]] = 1;
+  }
+  if (ctx->bcnodes) { dev_free(ctx, ctx->bcnodes); dev_free(ctx, ctx->bcvals); }
+  FSX_TRY(alloc(ctx, &ctx->bcnodes, (size_t)std::max(n, 1)));
+  FSX_TRY(alloc(ctx, &ctx->bcvals, (size_t)std::max(n, 1) * 3));
+  if (n > 0) {
+    FSX_HIP(hipMemcpy(ctx->bcnodes, nodes, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(ctx->bcvals, values, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
+  }
+  FSX_HIP(hipMemcpy(ctx->bcflag, flag.data(), (size_t)ctx->NN, hipMemcpyHostToDevice));
+  ctx->nbc = n;
+  ctx->zeroP = zero_p_node;
+  ctx->have_bc = true;
+  return FSX_OK;
+}
+
+int fsx_assemble_static(fsx_ctx* ctx) {
+  if (!ctx) return FSX_EINVAL;
+  if (!(ctx->have_mesh && ctx->have_M && ctx->have_G && ctx->have_bc && ctx->have_params))
+    return fail(ctx, FSX_ESTATE, "fsx_assemble_static needs mesh, M and G patterns, BCs and parameters");
+  FSX_TRY(launch_assemble_static(ctx));
+  ctx->assembled = true;
+  return FSX_OK;
+}
+
+int fsx_build_Z(fsx_ctx* ctx) {
+  if (!ctx) return FSX_EINVAL;
+  if (!ctx->assembled || !ctx->have_Z) return fail(ctx, FSX_ESTATE, "fsx_build_Z needs assembly and the Z pattern");
+  FSX_TRY(launch_build_Z(ctx));
+  ctx->z_built = true;
+  return FSX_OK;
+}
+
+int fsx_init_state(fsx_ctx* ctx) {
+  if (!ctx) return FSX_EINVAL;
+  if (!ctx->have_mesh || !ctx->have_bc) return fail(ctx, FSX_ESTATE, "fsx_init_state needs mesh and BCs");
+  const long long n3 = 3LL * ctx->NN;
+  for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
+                    ctx->KAcc, ctx->R1})
+    FSX_TRY(launch_fill(ctx, v, 0.0, n3));
+  for (double* v : {ctx->Pn, ctx->Pnp1, ctx->Pnp1_prev, ctx->Pdot, ctx->R2}) FSX_TRY(launch_fill(ctx, v, 0.0, ctx->NNp));
+  // applyBC_initial (BCH:4062-4107): Un at BC nodes <- BC values
+  if (ctx->nbc > 0) {
+    std::vector<int> nodes(ctx->nbc);
+    std::vector<double> vals((size_t)ctx->nbc * 3), un((size_t)n3, 0.0);
+    FSX_HIP(hipMemcpy(nodes.data(), ctx->bcnodes, nodes.size() * sizeof(int), hipMemcpyDeviceToHost));
+    FSX_HIP(hipMemcpy(vals.data(), ctx->bcvals, vals.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < ctx->nbc; i++)
+      for (int c = 0; c < 3; c++) un[(size_t)c * ctx->NN + nodes[i]] = vals[(size_t)i*3 + c];
+    FSX_HIP(hipStreamSynchronize(ctx->stream));
+    FSX_HIP(hipMemcpy(ctx->Un, un.data(), un.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  ctx->state_ready = true;
+  return FSX_OK;
+}
+
+static int need_step(fsx_ctx* ctx) {
+  if (!ctx) return FSX_EINVAL;
+  if (!ctx->z_built || !ctx->state_ready) return fail(ctx, FSX_ESTATE, "assemble, build Z and init the state first");
+  return FSX_OK;
+}
+
+int fsx_begin_step(fsx_ctx* ctx) {
+  FSX_TRY(need_step(ctx));
+  FSX_TRY(launch_copy(ctx, ctx->UnpHalf_prev, ctx->Un, 3LL * ctx->NN));    // BCH:2872-2876
+  FSX_TRY(launch_fill(ctx, ctx->Acc_prev, 0.0, 3LL * ctx->NN));
+  FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pn, ctx->NNp));
+  return FSX_OK;
+}
+
+int fsx_convection(fsx_ctx* ctx) { FSX_TRY(need_step(ctx)); return launch_convection(ctx); }
+int fsx_step1(fsx_ctx* ctx, int iter) { FSX_TRY(need_step(ctx)); return launch_step1(ctx, iter); }
+
+int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres) {
+  FSX_TRY(need_step(ctx));
+  FSX_TRY(launch_step2_rhs(ctx, iter));
+  int it = 0;
+  double rel = 0.0;
+  int rc = pressure_solve(ctx, ctx->R2, ctx->Pdot, &it, &rel);
+  if (rc != FSX_OK && rc != FSX_ENOCONV) return rc;
+  if (pcg_iters) *pcg_iters = it;
+  if (relres) *relres = rel;
+  FSX_TRY(launch_pnp1(ctx));
+  return rc;
+}
+
+int fsx_step3(fsx_ctx* ctx, int iter) { FSX_TRY(need_step(ctx)); return launch_step3(ctx, iter); }
+
+int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_u, double* rel_p) {
+  FSX_TRY(need_step(ctx));
+  FSX_HIP(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(StepScalars), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  double nu = std::sqrt(ctx->h_st->ud) / std::sqrt(ctx->h_st->uu);        // BCH:2936-2958
+  double np = std::sqrt(ctx->h_st->pd) / std::sqrt(ctx->h_st->pp);
+  if (rel_u) *rel_u = nu;
+  if (rel_p) *rel_p = np;
+  if (converged) *converged = (nu < tol && np < tol) ? 1 : 0;
+  return FSX_OK;
+}
+
+int fsx_next_iteration(fsx_ctx* ctx) {
+  FSX_TRY(need_step(ctx));
+  // BCH:2964-2972 as pointer swaps: every "current" buffer is fully
+  // rewritten before it is read again in the next iteration
+  std::swap(ctx->UnpHalf_prev, ctx->UnpHalf);
+  std::swap(ctx->Unp1_prev, ctx->Unp1);
+  std::swap(ctx->Acc_prev, ctx->Acc);
+  std::swap(ctx->Pnp1_prev, ctx->Pnp1);
+  return FSX_OK;
+}
+
+int fsx_kacc(fsx_ctx* ctx) { FSX_TRY(need_step(ctx)); return launch_kacc(ctx); }
+
+int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
+  FSX_TRY(need_step(ctx));
+  FSX_TRY(launch_maxacc(ctx));
+  FSX_HIP(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(StepScalars), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  double m = ctx->h_st->maxacc;
+  if (max_acc) *max_acc = m;
+  if (steady) *steady = (m > converge) ? 0 : 1;                            // BCH:3059-3061
+  std::swap(ctx->Un, ctx->Unp1);                                           // BCH:3070-3077
+  std::swap(ctx->Pn, ctx->Pnp1);
+  return FSX_OK;
+}
+
+int fsx_monitor(fsx_ctx* ctx, int node, double* out4) {
+  if (!ctx || !out4) return FSX_EINVAL;
+  if (node < 0 || node >= ctx->NNp) return fail(ctx, FSX_EINVAL, "monitor node out of range");
+  for (int c = 0; c < 3; c++)
+    FSX_HIP(hipMemcpyAsync(ctx->h_mon + c, ctx->Un + (size_t)c * ctx->NN + node, sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+  FSX_HIP(hipMemcpyAsync(ctx->h_mon + 3, ctx->Pn + node, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(out4, ctx->h_mon, 4 * sizeof(double));
+  return FSX_OK;
+}
+
+static int vec_info(fsx_ctx* ctx, int id, double** p, size_t* n, int* kind) {
+  // kind 0: plain, 1: NN vector replicated x3, 2: component c of G/GT (c = id - FSX_VAL_G1)
+  const size_t n3 = (size_t)3 * ctx->NN;
+  *kind = 0;
+  switch (id) {
+    case FSX_VEC_UN: *p = ctx->Un; *n = n3; break;
+    case FSX_VEC_UNP1: *p = ctx->Unp1; *n = n3; break;
+    case FSX_VEC_UNP1_PREV: *p = ctx->Unp1_prev; *n = n3; break;
+    case FSX_VEC_UNPHALF: *p = ctx->UnpHalf; *n = n3; break;
+    case FSX_VEC_UNPHALF_PREV: *p = ctx->UnpHalf_prev; *n = n3; break;
+    case FSX_VEC_ACC: *p = ctx->Acc; *n = n3; break;
+    case FSX_VEC_KACC: *p = ctx->KAcc; *n = n3; break;
+    case FSX_VEC_R1: *p = ctx->R1; *n = n3; break;
+    case FSX_VEC_PN: *p = ctx->Pn; *n = ctx->NNp; break;
+    case FSX_VEC_PNP1: *p = ctx->Pnp1; *n = ctx->NNp; break;
+    case FSX_VEC_PNP1_PREV: *p = ctx->Pnp1_prev; *n = ctx->NNp; break;
+    case FSX_VEC_PDOT: *p = ctx->Pdot; *n = ctx->NNp; break;
+    case FSX_VEC_R2: *p = ctx->R2; *n = ctx->NNp; break;
+    case FSX_VEC_MDINV: *p = ctx->MdInv; *n = n3; *kind = 1; break;
+    case FSX_VEC_MDORIGINV: *p = ctx->MdOrigInv; *n = n3; *kind = 1; break;
+    case FSX_VAL_K: *p = ctx->K; *n = ctx->nnzM; break;
+    case FSX_VAL_M: *p = ctx->M; *n = ctx->nnzM; break;
+    case FSX_VAL_G1: case FSX_VAL_G2: case FSX_VAL_G3: *p = ctx->G; *n = ctx->nnzG; *kind = 2; break;
+    case FSX_VAL_Z: *p = ctx->Z; *n = ctx->nnzZ; break;
+    default: return fail(ctx, FSX_EINVAL, "unknown vector id");
+  }
+  if (!*p) return fail(ctx, FSX_ESTATE, "vector not allocated yet");
+  return FSX_OK;
+}
+
+int fsx_download(fsx_ctx* ctx, int id, double* host, size_t count) {
+  if (!ctx || !host) return FSX_EINVAL;
+  double* p; size_t n; int kind;
+  FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
+  if (count != n) return fail(ctx, FSX_EINVAL, "fsx_download: count mismatch (expected " + std::to_string(n) + ")");
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  if (kind == 0) {
+    FSX_HIP(hipMemcpy(host, p, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (kind == 1) {
+    const size_t NN = ctx->NN;
+    FSX_HIP(hipMemcpy(host, p, NN * sizeof(double), hipMemcpyDeviceToHost));
+    std::memcpy(host + NN, host, NN * sizeof(double));
+    std::memcpy(host + 2 * NN, host, NN * sizeof(double));
+  } else {
+    std::vector<double> g(n * 3);
+    FSX_HIP(hipMemcpy(g.data(), p, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int c = id - FSX_VAL_G1;
+    for (size_t k = 0; k < n; k++) host[k] = g[3*k + c];
+  }
+  return FSX_OK;
+}
+
+int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
+  if (!ctx || !host) return FSX_EINVAL;
+  double* p; size_t n; int kind;
+  FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
+  if (kind != 0) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
+  if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
+  return FSX_OK;
+}
+
+int fsx_sync(fsx_ctx* ctx) {
+  if (!ctx) return FSX_EINVAL;
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  return FSX_OK;
+}
+
+int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  if (!ctx || !b || !x) return FSX_EINVAL;
+  if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  FSX_HIP(hipMemcpyAsync(ctx->R2, b, (size_t)ctx->NNp * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  int it = 0;
+  double rel = 0;
+  int rc = pressure_solve(ctx, ctx->R2, ctx->Pdot, &it, &rel);
+  if (rc != FSX_OK && rc != FSX_ENOCONV) return rc;
+  FSX_HIP(hipMemcpyAsync(x, ctx->Pdot, (size_t)ctx->NNp * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  return rc;
+}
+
+int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, double beta) {
+  if (!ctx || !x || !y) return FSX_EINVAL;
+  if (!ctx->assembled) return fail(ctx, FSX_ESTATE, "assemble first");
+  if (which == 4 && !ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  size_t nx, ny;
+  if (which == 0) { nx = ny = ctx->NN; }
+  else if (which >= 1 && which <= 3) { nx = ctx->NNp; ny = ctx->NN; }
+  else if (which == 4) { nx = ny = ctx->NNp; }
+  else if (which >= 5 && which <= 7) { nx = ctx->NN; ny = ctx->NNp; }
+  else return fail(ctx, FSX_EINVAL, "fsx_spmv: unknown operator");
+  double *dx, *dy;
+  FSX_TRY(alloc(ctx, &dx, nx));
+  FSX_TRY(alloc(ctx, &dy, ny));
+  int rc = FSX_OK;
+  if (hipMemcpy(dx, x, nx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dy, y, ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(ctx, FSX_EHIP, "fsx_spmv copy");
+  if (rc == FSX_OK) rc = spmv_host_op(ctx, which, dx, dy, alpha, beta);
+  if (rc == FSX_OK) {
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipMemcpy(y, dy, ny * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(ctx, FSX_EHIP, "fsx_spmv copy back");
+  }
+  dev_free(ctx, dx);
+  dev_free(ctx, dy);
+  ctx->bytes -= (nx + ny) * sizeof(double);
+  return rc;
+}
+
+int fsx_timing_reset(fsx_ctx* ctx, int enable) {
+  if (!ctx) return FSX_EINVAL;
+  ctx->timing = enable != 0;
+  ctx->tm = fsx_timing{};
+  return FSX_OK;
+}
+
+int fsx_timing_get(fsx_ctx* ctx, fsx_timing* out) {
+  if (!ctx || !out) return FSX_EINVAL;
+  *out = ctx->tm;
+  return FSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Internal definitions shared by the HIP translation units of libfsx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/fsx.h"
+
+namespace fsx {
+
+constexpr int kWave = 64;         // CDNA wavefront width
+constexpr int kBlock = 256;       // default block size for row kernels
+constexpr int kMaxGrid = 2048;    // row kernels: 256 CUs x 8 blocks, a multiple of 8 XCDs
+constexpr int kZRowMax = 1024;    // longest Z row handled by the Z-build kernel
+
+// Device-resident scalars of one Krylov solve. Written by the last block of a
+// reduction kernel, read by the following launches; never by the host while
+// kernels run.
+struct SolveScalars {
+  double rz, pq, rr, bb, alpha, beta, rtol2bb;
+  double rz_new;
+  int iters, done, maxit, pad;
+  unsigned int cnt[4];            // last-block counters, reset by the last block
+};
+
+// Scalars of the time step (norms for the convergence checks, max |dU|/dt).
+struct StepScalars {
+  double uu, ud, pp, pd, maxacc;
+  unsigned int cnt[4];
+};
+
+// Shape-function tables on the device, GP-major (k = Gauss point):
+//   Sv[k*27 + i], dSv[(k*27 + i)*3 + m], Sp[k*8 + j], dSp[(k*8 + j)*3 + m], w[k]
+struct ShapeTables {
+  double* Sv;
+  double* dSv;
+  double* Sp;
+  double* dSp;
+  double* w;
+};
+
+}  // namespace fsx
+
+struct fsx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // sizes and parameters
+  int NE = 0, NN = 0, NNp = 0, NGP = 27;
+  double density = 1.0, viscosity = 0.01, dt = 0.01;
+  int solver_mode = FSX_SOLVER_FP64_JACOBI;
+  double rtol = 1e-8;
+  int maxit = 5000;
+  bool have_mesh = false, have_M = false, have_G = false, have_Z = false, have_bc = false, have_params = false;
+  bool assembled = false, z_built = false, state_ready = false;
+  // mesh (device)
+  int* L27 = nullptr;            // NE*27
+  double* xc = nullptr;          // NNp*3 corner coordinates
+  int* color_order = nullptr;    // NE
+  std::vector<int> color_ptr;    // host copy, ncolors+1
+  // CSR patterns and values (device)
+  long long nnzM = 0, nnzG = 0, nnzZ = 0;
+  int* Mrow = nullptr; int* Mcol = nullptr;
+  double* K = nullptr; double* M = nullptr;
+  int* Grow = nullptr; int* Gcol = nullptr;
+  double* G = nullptr;           // nnzG*3, (G1, G2, G3) interleaved per nonzero
+  int* GTrow = nullptr; int* GTcol = nullptr; int* GTpos = nullptr;
+  double* GT = nullptr;          // nnzG*3, transposed pattern
+  int* Zrow = nullptr; int* Zcol = nullptr;
+  double* Z = nullptr; double* Zdinv = nullptr;
+  float* Zf = nullptr; float* Zdinvf = nullptr;
+  int zrow_max = 0;
+  // lumped mass and BCs (device, NN-long; the reference replicates them x3)
+  double* MdInv = nullptr; double* MdOrigInv = nullptr; double* Md = nullptr;
+  unsigned char* bcflag = nullptr;
+  int nbc = 0; int* bcnodes = nullptr; double* bcvals = nullptr;
+  int zeroP = -1;
+  // state (device)
+  double *Un = nullptr, *Unp1 = nullptr, *Unp1_prev = nullptr, *UnpHalf = nullptr, *UnpHalf_prev = nullptr;
+  double *Acc = nullptr, *Acc_prev = nullptr, *KAcc = nullptr, *R1 = nullptr;
+  double *Pn = nullptr, *Pnp1 = nullptr, *Pnp1_prev = nullptr, *Pdot = nullptr, *R2 = nullptr;
+  // solver work (device)
+  double *wr = nullptr, *wp = nullptr, *wq = nullptr, *wx = nullptr;    // FP64 PCG / IR outer
+  float *fr = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 inner PCG
+  double* partials = nullptr;    // 8 * kMaxGrid
+  fsx::SolveScalars* sc = nullptr;
+  fsx::StepScalars* st = nullptr;
+  fsx::SolveScalars* h_sc = nullptr;   // pinned host mirrors
+  fsx::StepScalars* h_st = nullptr;
+  double* h_mon = nullptr;
+  double* d_mon = nullptr;
+  fsx::ShapeTables tab{};
+  double* tab_mem = nullptr;
+  // accounting
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  fsx_timing tm{};
+  int last_iters = 0;
+  double last_relres = 0.0;
+};
+
+// error helpers (host)
+#define FSX_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      ctx->err = std::string(#call) + " -> " + hipGetErrorString(_e);                  \
+      return FSX_EHIP;                                                                  \
+    }                                                                                   \
+  } while (0)
+
+#define FSX_TRY(call)                 \
+  do {                                \
+    int _rc = (call);                 \
+    if (_rc != FSX_OK) return _rc;    \
+  } while (0)
+
+namespace fsx {
+int dev_alloc(fsx_ctx* ctx, void** p, size_t bytes);
+template <class T>
+int alloc(fsx_ctx* ctx, T** p, size_t n) {
+  return dev_alloc(ctx, reinterpret_cast<void**>(p), n * sizeof(T));
+}
+void dev_free(fsx_ctx* ctx, void* p);
+int fail(fsx_ctx* ctx, int code, const std::string& msg);
+int launch_check(fsx_ctx* ctx, const char* what);
+int row_grid(long long nrows, int rows_per_block);
+
+// kernel entry points implemented in the .hip files (host-side launchers)
+int launch_assemble_static(fsx_ctx* ctx);
+int launch_build_Z(fsx_ctx* ctx);
+int launch_convection(fsx_ctx* ctx);
+int launch_step1(fsx_ctx* ctx, int iter);
+int launch_step2_rhs(fsx_ctx* ctx, int iter);
+int launch_pnp1(fsx_ctx* ctx);
+int launch_step3(fsx_ctx* ctx, int iter);
+int launch_kacc(fsx_ctx* ctx);
+int launch_maxacc(fsx_ctx* ctx);
+int launch_fill(fsx_ctx* ctx, double* p, double v, long long n);
+int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n);
+int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
+int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta);
+}  // namespace fsx

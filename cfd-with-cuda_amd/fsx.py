@@ -1,0 +1,424 @@
+"""Python bindings (ctypes) of libfsx, the MI355X fractional-step core.
+
+The library is built in-tree by ``make -C cfd-with-cuda_amd`` into
+``cfd-with-cuda_amd/_build/libfsx.so``. Loading fails loudly when it is
+missing: there is no fallback path. All numerics run in the library (HIP
+kernels for gfx950 behind the C-ABI of include/fsx.h, host topology and the
+time-loop driver of include/fsx_host.h).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfsx.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTATE, FSX_ENOCONV, FSX_ENOMEM, FSX_ENUMERIC = range(7)
+CSR_M, CSR_G, CSR_Z = 0, 1, 2
+SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
+
+VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, R1=7, Pn=8, Pnp1=9,
+           Pnp1_prev=10, Pdot=11, R2=12, MdInv=13, MdOrigInv=14, K=15, M=16, G1=17, G2=18, G3=19, Z=20)
+
+# Exported C symbols declared in include/fsx.h and include/fsx_host.h
+DEVICE_SYMBOLS = (
+    "fsx_create", "fsx_destroy", "fsx_last_error", "fsx_device_bytes", "fsx_upload_mesh", "fsx_upload_csr",
+    "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
+    "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
+    "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
+    "fsx_solve_pressure", "fsx_spmv", "fsx_timing_reset", "fsx_timing_get",
+)
+HOST_SYMBOLS = (
+    "fsxh_mesh_cavity", "fsxh_mesh_duct", "fsxh_mesh_read_inp", "fsxh_mesh_write_inp", "fsxh_mesh_free",
+    "fsxh_mesh_info", "fsxh_mesh_params", "fsxh_mesh_set_params", "fsxh_mesh_get", "fsxh_mesh_profile",
+    "fsxh_topology", "fsxh_topo_free", "fsxh_topo_info", "fsxh_topo_len", "fsxh_topo_get",
+    "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_ctx", "fsxd_topo", "fsxd_error",
+    "fsxd_setup_times",
+)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("NGP", ctypes.c_int), ("maxIter", ctypes.c_int),
+                ("tolerance", ctypes.c_double), ("converge", ctypes.c_double), ("solver_mode", ctypes.c_int),
+                ("pcg_rtol", ctypes.c_double), ("pcg_maxit", ctypes.c_int), ("timing", ctypes.c_int)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int), ("inner_iters", ctypes.c_int), ("pcg_iters", ctypes.c_int),
+                ("steady", ctypes.c_int), ("time", ctypes.c_double), ("max_acc", ctypes.c_double),
+                ("mon", ctypes.c_double * 4), ("wall_s", ctypes.c_double), ("pcg_relres", ctypes.c_double)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("pcg_ms", ctypes.c_double), ("pcg_spmv_ms", ctypes.c_double),
+                ("pcg_spmv_launches", ctypes.c_longlong), ("other_ms", ctypes.c_double),
+                ("pcg_iters", ctypes.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libfsx.so not built: {LIB_PATH} (run `make -C cfd-with-cuda_amd` or "
+                           f"__graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "fsx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "fsx_destroy": ([_vp], None),
+        "fsx_last_error": ([_vp], ctypes.c_char_p),
+        "fsx_device_bytes": ([_vp], ctypes.c_size_t),
+        "fsx_upload_mesh": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _D, ctypes.c_int, _I, _I], ctypes.c_int),
+        "fsx_upload_csr": ([_vp, ctypes.c_int, ctypes.c_int, _I, _I], ctypes.c_int),
+        "fsx_upload_bc": ([_vp, ctypes.c_int, _I, _D, ctypes.c_int], ctypes.c_int),
+        "fsx_set_params": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "fsx_set_solver": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "fsx_assemble_static": ([_vp], ctypes.c_int),
+        "fsx_build_Z": ([_vp], ctypes.c_int),
+        "fsx_init_state": ([_vp], ctypes.c_int),
+        "fsx_begin_step": ([_vp], ctypes.c_int),
+        "fsx_convection": ([_vp], ctypes.c_int),
+        "fsx_step1": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_step2": ([_vp, ctypes.c_int, _I, _D], ctypes.c_int),
+        "fsx_step3": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_check_convergence": ([_vp, ctypes.c_double, _I, _D, _D], ctypes.c_int),
+        "fsx_next_iteration": ([_vp], ctypes.c_int),
+        "fsx_kacc": ([_vp], ctypes.c_int),
+        "fsx_end_step": ([_vp, ctypes.c_double, _I, _D], ctypes.c_int),
+        "fsx_monitor": ([_vp, ctypes.c_int, _D], ctypes.c_int),
+        "fsx_download": ([_vp, ctypes.c_int, _D, ctypes.c_size_t], ctypes.c_int),
+        "fsx_upload_vec": ([_vp, ctypes.c_int, _D, ctypes.c_size_t], ctypes.c_int),
+        "fsx_sync": ([_vp], ctypes.c_int),
+        "fsx_solve_pressure": ([_vp, _D, _D, _I, _D], ctypes.c_int),
+        "fsx_spmv": ([_vp, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "fsx_timing_reset": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_timing_get": ([_vp, ctypes.POINTER(Timing)], ctypes.c_int),
+        "fsxh_mesh_cavity": ([ctypes.c_int, ctypes.c_double, ctypes.c_double], _vp),
+        "fsxh_mesh_duct": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int], _vp),
+        "fsxh_mesh_read_inp": ([ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        "fsxh_mesh_write_inp": ([_vp, ctypes.c_char_p], ctypes.c_int),
+        "fsxh_mesh_free": ([_vp], None),
+        "fsxh_mesh_info": ([_vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "fsxh_mesh_params": ([_vp, _D], ctypes.c_int),
+        "fsxh_mesh_set_params": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "fsxh_mesh_get": ([_vp, ctypes.c_char_p, _vp], ctypes.c_int),
+        "fsxh_mesh_profile": ([_vp, _D], ctypes.c_int),
+        "fsxh_topology": ([_vp, ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        "fsxh_topo_free": ([_vp], None),
+        "fsxh_topo_info": ([_vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "fsxh_topo_len": ([_vp, ctypes.c_char_p], ctypes.c_longlong),
+        "fsxh_topo_get": ([_vp, ctypes.c_char_p, _vp], ctypes.c_int),
+        "fsxd_default_options": ([ctypes.POINTER(Options)], None),
+        "fsxd_create": ([_vp, ctypes.POINTER(Options), ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        "fsxd_free": ([_vp], None),
+        "fsxd_run": ([_vp, ctypes.c_int, ctypes.POINTER(StepStats)], ctypes.c_int),
+        "fsxd_ctx": ([_vp], _vp),
+        "fsxd_topo": ([_vp], _vp),
+        "fsxd_error": ([_vp], ctypes.c_char_p),
+        "fsxd_setup_times": ([_vp, _D], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class FsxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fsx error {code}: {msg}")
+        self.code = code
+
+
+# ------------------------------------------------------------------ mesh
+class Mesh:
+    """Mesh input (the content of a reference .inp file)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise FsxError(FSX_EINVAL, "mesh construction failed")
+        self.L = lib()
+        self.h = _vp(handle)
+
+    @classmethod
+    def cavity(cls, N, cluster=0.0, lid=1.0):
+        return cls(lib().fsxh_mesh_cavity(N, cluster, lid))
+
+    @classmethod
+    def duct(cls, n=16, n_in=40, n_arc=40, n_out=80, L_in=5.0, R=2.0, L_out=10.0, peak=1.0, seed=12345,
+             permute=True):
+        return cls(lib().fsxh_mesh_duct(n, n_in, n_arc, n_out, L_in, R, L_out, peak, seed, 1 if permute else 0))
+
+    @classmethod
+    def read_inp(cls, path):
+        out = _vp()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().fsxh_mesh_read_inp(path.encode(), ctypes.byref(out), err, 512)
+        if rc != FSX_OK:
+            raise FsxError(rc, err.value.decode())
+        return cls(out.value)
+
+    def write_inp(self, path):
+        rc = self.L.fsxh_mesh_write_inp(self.h, path.encode())
+        if rc != FSX_OK:
+            raise FsxError(rc, "write_inp")
+
+    def close(self):
+        if self.h:
+            self.L.fsxh_mesh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = (ctypes.c_longlong * 8)()
+        self.L.fsxh_mesh_info(self.h, out)
+        return dict(zip(("NE", "NCN", "nVelFaces", "nBC", "zeroP", "NGP", "maxIter", "has_profile"), list(out)))
+
+    def params(self):
+        out = np.zeros(10)
+        self.L.fsxh_mesh_params(self.h, _dp(out))
+        keys = ("dt", "t_ini", "t_final", "tolerance", "converge", "density", "viscosity", "mon_x", "mon_y", "mon_z")
+        return dict(zip(keys, out.tolist()))
+
+    def set_params(self, NGP=27, dt=0.01, maxIter=3, tolerance=1e-3, converge=1e-6, density=1.0, viscosity=0.01):
+        rc = self.L.fsxh_mesh_set_params(self.h, NGP, dt, maxIter, tolerance, converge, density, viscosity)
+        if rc != FSX_OK:
+            raise FsxError(rc, "set_params")
+        return self
+
+    def profile(self):
+        out = np.zeros(4)
+        self.L.fsxh_mesh_profile(self.h, _dp(out))
+        return dict(bc=int(out[0]), peak=out[1], y0=out[2], z0=out[3])
+
+    def arrays(self):
+        i = self.info()
+        coord = np.empty((i["NCN"], 3))
+        conn = np.empty((i["NE"], 8), dtype=np.int32)
+        faces = np.empty((i["nVelFaces"], 3), dtype=np.int32)
+        bcstr = np.empty((i["nBC"], 3))
+        for name, a in (("coord", coord), ("conn", conn), ("faces", faces), ("BCstr", bcstr)):
+            self.L.fsxh_mesh_get(self.h, name.encode(), a.ctypes.data_as(_vp))
+        p = self.params()
+        return dict(coord=coord, conn=conn, faces=faces, BCstr=bcstr, zeroP=i["zeroP"],
+                    mon=np.array([p["mon_x"], p["mon_y"], p["mon_z"]]))
+
+
+class Topology:
+    INT_ARRAYS = ("LtoGnode", "Mrow", "Mcol", "Grow", "Gcol", "Zrow", "Zcol", "BCnodes", "BCwhich", "color_ptr",
+                  "color_order")
+
+    def __init__(self, mesh=None, handle=None, owner=True):
+        self.L = lib()
+        self.owner = owner
+        if handle is not None:
+            self.h = _vp(handle)
+        else:
+            out = _vp()
+            err = ctypes.create_string_buffer(512)
+            rc = self.L.fsxh_topology(mesh.h, ctypes.byref(out), err, 512)
+            if rc != FSX_OK:
+                raise FsxError(rc, err.value.decode())
+            self.h = out
+
+    def close(self):
+        if self.h and self.owner:
+            self.L.fsxh_topo_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = (ctypes.c_longlong * 11)()
+        self.L.fsxh_topo_info(self.h, out)
+        keys = ("NE", "NCN", "NN", "NNp", "nnzM", "nnzG", "nnzZ", "nBCnodes", "ncolors", "monPoint", "zeroP")
+        return dict(zip(keys, list(out)))
+
+    def get(self, name):
+        n = self.L.fsxh_topo_len(self.h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        a = np.empty(n, dtype=np.float64 if name in ("coord", "BCvalues") else np.int32)
+        self.L.fsxh_topo_get(self.h, name.encode(), a.ctypes.data_as(_vp))
+        return a
+
+
+# ------------------------------------------------------------------ solver
+def default_options(**kw):
+    o = Options()
+    lib().fsxd_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Solver:
+    """Driver: topology + device context + the reference's time loop."""
+
+    def __init__(self, mesh, **opts):
+        self.L = lib()
+        self.opt = default_options(**opts)
+        out = _vp()
+        err = ctypes.create_string_buffer(1024)
+        rc = self.L.fsxd_create(mesh.h, ctypes.byref(self.opt), ctypes.byref(out), err, 1024)
+        if rc != FSX_OK:
+            raise FsxError(rc, err.value.decode())
+        self.h = out
+        self.ctx = _vp(self.L.fsxd_ctx(self.h))
+        self.topo = Topology(handle=self.L.fsxd_topo(self.h), owner=False)
+        self.sizes = self.topo.info()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fsxd_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def err(self):
+        return self.L.fsx_last_error(self.ctx).decode()
+
+    def check(self, rc, what=""):
+        if rc != FSX_OK:
+            raise FsxError(rc, f"{what}: {self.err()}")
+
+    def run(self, nsteps):
+        stats = (StepStats * max(1, nsteps))()
+        n = self.L.fsxd_run(self.h, nsteps, stats)
+        if n < 0:
+            raise FsxError(-n, self.L.fsxd_error(self.h).decode())
+        return [dict(step=s.step, inner=s.inner_iters, pcg=s.pcg_iters, steady=s.steady, time=s.time,
+                     max_acc=s.max_acc, mon=list(s.mon), wall_s=s.wall_s, relres=s.pcg_relres) for s in stats[:n]]
+
+    def setup_times(self):
+        out = np.zeros(3)
+        self.L.fsxd_setup_times(self.h, _dp(out))
+        return dict(topology=out[0], upload=out[1], assembly=out[2])
+
+    def vec_len(self, name):
+        s = self.sizes
+        if name in ("Pn", "Pnp1", "Pnp1_prev", "Pdot", "R2"):
+            return s["NNp"]
+        if name in ("K", "M"):
+            return s["nnzM"]
+        if name in ("G1", "G2", "G3"):
+            return s["nnzG"]
+        if name == "Z":
+            return s["nnzZ"]
+        return 3 * s["NN"]
+
+    def download(self, name):
+        n = self.vec_len(name)
+        a = np.empty(n)
+        self.check(self.L.fsx_download(self.ctx, VEC[name], _dp(a), n), f"download {name}")
+        return a
+
+    def upload(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self.check(self.L.fsx_upload_vec(self.ctx, VEC[name], _dp(a), a.size), f"upload {name}")
+
+    def solve_pressure(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.L.fsx_solve_pressure(self.ctx, _dp(b), _dp(x), ctypes.byref(it), ctypes.byref(rel))
+        if rc not in (FSX_OK, FSX_ENOCONV):
+            self.check(rc, "solve_pressure")
+        return x, it.value, rel.value, rc
+
+    def spmv(self, which, x, y=None, alpha=1.0, beta=0.0):
+        code = {"K": 0, "G1": 1, "G2": 2, "G3": 3, "Z": 4, "G1T": 5, "G2T": 6, "G3T": 7}[which]
+        s = self.sizes
+        ny = s["NNp"] if which in ("Z", "G1T", "G2T", "G3T") else s["NN"]
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(ny) if y is None else np.array(y, dtype=np.float64)
+        self.check(self.L.fsx_spmv(self.ctx, code, _dp(x), _dp(y), alpha, beta), "spmv")
+        return y
+
+    def set_solver(self, mode, rtol, maxit):
+        self.check(self.L.fsx_set_solver(self.ctx, mode, rtol, maxit), "set_solver")
+
+    # single-stage calls (the fsx_* C-ABI, for stage-by-stage parity tests)
+    def begin_step(self):
+        self.check(self.L.fsx_begin_step(self.ctx), "begin_step")
+
+    def convection(self):
+        self.check(self.L.fsx_convection(self.ctx), "convection")
+
+    def step1(self, it):
+        self.check(self.L.fsx_step1(self.ctx, it), "step1")
+
+    def step2(self, it):
+        n = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.L.fsx_step2(self.ctx, it, ctypes.byref(n), ctypes.byref(rel))
+        if rc not in (FSX_OK, FSX_ENOCONV):
+            self.check(rc, "step2")
+        return n.value, rel.value
+
+    def step3(self, it):
+        self.check(self.L.fsx_step3(self.ctx, it), "step3")
+
+    def check_convergence(self, tol):
+        c = ctypes.c_int()
+        ru = ctypes.c_double()
+        rp = ctypes.c_double()
+        self.check(self.L.fsx_check_convergence(self.ctx, tol, ctypes.byref(c), ctypes.byref(ru), ctypes.byref(rp)))
+        return bool(c.value), ru.value, rp.value
+
+    def next_iteration(self):
+        self.check(self.L.fsx_next_iteration(self.ctx))
+
+    def kacc(self):
+        self.check(self.L.fsx_kacc(self.ctx))
+
+    def end_step(self, converge):
+        s = ctypes.c_int()
+        m = ctypes.c_double()
+        self.check(self.L.fsx_end_step(self.ctx, converge, ctypes.byref(s), ctypes.byref(m)))
+        return bool(s.value), m.value
+
+    def timing_reset(self, enable=True):
+        self.check(self.L.fsx_timing_reset(self.ctx, 1 if enable else 0))
+
+    def timing(self):
+        t = Timing()
+        self.check(self.L.fsx_timing_get(self.ctx, ctypes.byref(t)))
+        return dict(pcg_ms=t.pcg_ms, pcg_spmv_ms=t.pcg_spmv_ms, pcg_spmv_launches=t.pcg_spmv_launches,
+                    other_ms=t.other_ms, pcg_iters=t.pcg_iters)
+
+    def sync(self):
+        self.check(self.L.fsx_sync(self.ctx))

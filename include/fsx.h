@@ -1,0 +1,184 @@
+/* fsx.h -- C-ABI of the MI355X-native fractional-step core (device side).
+ *
+ * This is the drop-in boundary that replaces the reference's GPU module,
+ * fractionalStep/Cpp/CUDAcodes.cu (CU), whose C++-linkage functions the
+ * reference driver fractionalStep/Cpp/blascoCodinaHuerta.cpp (BCH) calls
+ * (prototypes BCH:365-377). Each entry point below names the reference
+ * function it replaces. Data crosses as plain pointers and sizes; the device
+ * state lives in an opaque context instead of ~60 globals (BCH:141-322,
+ * CU:47-97). Every call returns FSX_OK (0) or an error code; the message is
+ * available from fsx_last_error(). Nothing blocks on stdin or exits.
+ *
+ * Conventions (the reference's layouts, BCH:1327-1403, 1807-1811):
+ *   velocity vectors are [u(0..NN-1) | v | w], FP64;
+ *   pressure is indexed by corner node (NNp = NCN), FP64;
+ *   CSR row pointers and columns are int32, 0-based, columns sorted.
+ * One context per host thread; all work is queued on the context's stream.
+ */
+#ifndef FSX_H
+#define FSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fsx_ctx fsx_ctx;
+
+enum fsx_status {
+  FSX_OK = 0,
+  FSX_EINVAL = 1,   /* bad argument or size */
+  FSX_EHIP = 2,     /* HIP runtime error */
+  FSX_ESTATE = 3,   /* call out of order (e.g. step before assembly) */
+  FSX_ENOCONV = 4,  /* linear solver hit maxit before rtol */
+  FSX_ENOMEM = 5,   /* device allocation failed */
+  FSX_ENUMERIC = 6  /* NaN/Inf met in a solver */
+};
+
+/* CSR kinds for fsx_upload_csr */
+enum fsx_csr_kind {
+  FSX_CSR_M = 0,   /* NN x NN pattern shared by M, K, A  (setupSparseM BCH:1675-1921) */
+  FSX_CSR_G = 1,   /* NN x NNp pattern shared by G1..G3  (setupSparseG BCH:1928-2159) */
+  FSX_CSR_Z = 2    /* NNp x NNp structural pattern of G^T G (calculateZ BCH:3322-3479) */
+};
+
+/* Vector / array ids for fsx_download and fsx_upload_vec. Sizes in doubles. */
+enum fsx_vec_id {
+  FSX_VEC_UN = 0,            /* 3NN */
+  FSX_VEC_UNP1 = 1,          /* 3NN */
+  FSX_VEC_UNP1_PREV = 2,     /* 3NN */
+  FSX_VEC_UNPHALF = 3,       /* 3NN */
+  FSX_VEC_UNPHALF_PREV = 4,  /* 3NN */
+  FSX_VEC_ACC = 5,           /* 3NN */
+  FSX_VEC_KACC = 6,          /* 3NN */
+  FSX_VEC_R1 = 7,            /* 3NN (after fsx_convection: -A*UnpHalf_prev; after fsx_step1: full R1) */
+  FSX_VEC_PN = 8,            /* NNp */
+  FSX_VEC_PNP1 = 9,          /* NNp */
+  FSX_VEC_PNP1_PREV = 10,    /* NNp */
+  FSX_VEC_PDOT = 11,         /* NNp */
+  FSX_VEC_R2 = 12,           /* NNp */
+  FSX_VEC_MDINV = 13,        /* 3NN (replicated x3 like BCH:3268-3270) */
+  FSX_VEC_MDORIGINV = 14,    /* 3NN */
+  FSX_VAL_K = 15,            /* nnz(M) values of K */
+  FSX_VAL_M = 16,            /* nnz(M) values of the consistent mass M */
+  FSX_VAL_G1 = 17,           /* nnz(G) */
+  FSX_VAL_G2 = 18,
+  FSX_VAL_G3 = 19,
+  FSX_VAL_Z = 20,            /* nnz(Z), penalty applied */
+  FSX_VEC_COUNT = 21
+};
+
+/* Pressure-solver modes for fsx_set_solver (CUSP_CG_solver CU:418-451 and
+ * MKL_CG_solver BCH:3981-4055 are the FP64 Jacobi form). */
+enum fsx_solver_mode {
+  FSX_SOLVER_FP64_JACOBI = 0,  /* FP64 Jacobi-PCG, x0 = 0, ||r|| <= rtol ||b|| */
+  FSX_SOLVER_MIXED_IR = 1,     /* FP32 inner Jacobi-PCG, FP64 residual + refinement, same stop test */
+  FSX_SOLVER_LEARNED = 2       /* flexible PCG with the learned preconditioner, Jacobi-PCG fallback */
+};
+
+/* ---------------------------------------------------------------- lifecycle */
+/* replaces selectCUDAdevice (CU:126-154) and the cuSPARSE/cuBLAS setup (CU:171-178) */
+int fsx_create(int device, fsx_ctx** out);
+void fsx_destroy(fsx_ctx* ctx);
+const char* fsx_last_error(const fsx_ctx* ctx);
+/* device memory currently held by the context, in bytes */
+size_t fsx_device_bytes(const fsx_ctx* ctx);
+
+/* ---------------------------------------------------------------- uploads
+ * Together these replace initializeAndAllocateGPU (CU:161-274). Host arrays
+ * are caller-owned and copied. */
+/* LtoGnode: NE*27 global node ids per element in the reference's local order
+ * (BCH:2312-2343); corner_coord: NNp*3 corner coordinates (BCH:582-585).
+ * color_order: NE element ids grouped by colour, color_ptr: ncolors+1 offsets;
+ * no two elements of one colour may share a node (checked on the device side
+ * inputs by the host library; see fsxh_color_elements). */
+int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, const double* corner_coord,
+                    int ncolors, const int* color_ptr, const int* color_order);
+/* rowptr has nrows+1 entries, col rowptr[nrows] entries. */
+int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const int* col);
+/* velocity BC nodes (sorted, BCvelNodes of BCH:1549-1573) with their u,v,w
+ * values (n*3), and the zero-pressure node (BCH:705-706; <= 0 disables the
+ * pin, reference quirk BCH:4172). */
+int fsx_upload_bc(fsx_ctx* ctx, int n, const int* nodes, const double* values, int zero_p_node);
+/* NGP in {1, 8, 27}; density and viscosity as in BCH:559-560; dt BCH:551. */
+int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, double dt);
+/* pressure solver: mode (fsx_solver_mode), relative tolerance, iteration cap */
+int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit);
+
+/* ---------------------------------------------------------------- assembly */
+/* step0 values (BCH:3127-3315): K = nu*sum grad.grad, M, G_d = -(1/rho) sum
+ * Sp d_d Sv, lumped Md, MdOrigInv, MdInv with BC rows 1 (BCH:4121-4135). */
+int fsx_assemble_static(fsx_ctx* ctx);
+/* Z = sum_d G_d^T diag(MdOrigInv) G_d on the uploaded Z pattern, penalty
+ * x1000 at the zero-pressure node; replaces calculateZ_CUSP (CU:281-411). */
+int fsx_build_Z(fsx_ctx* ctx);
+/* zero state + initial velocity BCs: initializeAndAllocate (BCH:2718-2808)
+ * and applyBC_initial (BCH:4062-4107) */
+int fsx_init_state(fsx_ctx* ctx);
+
+/* ---------------------------------------------------------------- time step
+ * The driver calls these in the order of timeLoop (BCH:2859-3118). */
+/* UnpHalf_prev <- Un, Pnp1_prev <- Pn, Acc_prev <- 0   (BCH:2865-2881) */
+int fsx_begin_step(fsx_ctx* ctx);
+/* R1 <- -A(Un) * UnpHalf_prev, matrix free; replaces calculateMatrixAGPU
+ * (CU:906-939) / calculateMatrixA (BCH:3520-3685) */
+int fsx_convection(fsx_ctx* ctx);
+/* R1 -= K UnpHalf_prev + G Pn; BC rows 0; UnpHalf = Un + dt R1 MdInv.
+ * replaces step1GPUpart (CU:946-973) */
+int fsx_step1(fsx_ctx* ctx, int iter);
+/* R2 = G^T (UnpHalf/dt^2 - [iter>1] MdOrigInv KAcc); R2[p0] = 0; solve
+ * Z Pdot = R2; Pnp1 = Pn + dt Pdot. replaces step2GPU (CU:980-1022) and
+ * CUSP_CG_solver (CU:418-451). pcg_iters/relres may be NULL. Returns
+ * FSX_ENOCONV if the solver stopped at maxit (the solution is still used,
+ * as the reference does). */
+int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres);
+/* R3 = -dt G Pdot - [iter>1] dt KAcc; BC rows 0; Acc = R3 MdInv;
+ * Unp1 = UnpHalf + dt Acc. replaces step3GPU (CU:1029-1060) */
+int fsx_step3(fsx_ctx* ctx, int iter);
+/* ||Unp1-Unp1_prev||/||Unp1|| and ||Pnp1-Pnp1_prev||/||Pnp1||, converged =
+ * both < tol (NaN counts as not converged). replaces checkConvergenceGPU
+ * (CU:1084-1121) */
+int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_u, double* rel_p);
+/* UnpHalf_prev <- UnpHalf, Unp1_prev <- Unp1, Acc_prev <- Acc,
+ * Pnp1_prev <- Pnp1 (BCH:2964-2972, CU via BCH:2925-2932) */
+int fsx_next_iteration(fsx_ctx* ctx);
+/* KAcc = K Acc_prev; replaces calculate_KtimesAcc_prevGPU (CU:1067-1077) */
+int fsx_kacc(fsx_ctx* ctx);
+/* maxAcc = max |Unp1 - Un| / dt (BCH:3048-3062; the reference GPU variant
+ * omits the /dt, CU:99-118), steady = maxAcc <= converge; then Un <- Unp1,
+ * Pn <- Pnp1 (BCH:3070-3077). replaces checkConvergenceInTimeGPU
+ * (CU:1151-1178) */
+int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc);
+/* the four monitor values u,v,w,p of Un/Pn at one node, without copying the
+ * whole field (replaces the full D2H of printMonitorDataGPU, CU:1128-1144) */
+int fsx_monitor(fsx_ctx* ctx, int node, double* out4);
+
+/* ---------------------------------------------------------------- data */
+int fsx_download(fsx_ctx* ctx, int id, double* host, size_t count);
+int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count);
+int fsx_sync(fsx_ctx* ctx);
+/* solve Z x = b on the context's Z with the current solver settings; b and
+ * x are host arrays of NNp doubles (a test and tooling entry point). */
+int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
+/* y = alpha * op(v) + beta * y on the context's matrices for host vectors:
+ * which = 0 K (NN), 1..3 G_d (NN <- NNp), 4 Z (NNp), 5..7 G_d^T (NNp <- NN).
+ * Mirrors the reference's mkl_dcsrmv / cusparseDcsrmv calls. */
+int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, double beta);
+
+/* kernel time accounting (device time of the last timed region, ms) */
+typedef struct fsx_timing {
+  double pcg_ms;         /* pressure solve kernels */
+  double pcg_spmv_ms;    /* of which the Z SpMV kernel */
+  long long pcg_spmv_launches;
+  double other_ms;       /* everything else of a step */
+  long long pcg_iters;
+} fsx_timing;
+int fsx_timing_reset(fsx_ctx* ctx, int enable);
+int fsx_timing_get(fsx_ctx* ctx, fsx_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSX_H */

@@ -1,0 +1,115 @@
+/* fsx_host.h -- host C++ side of the fractional-step core, C-ABI.
+ *
+ * Two layers sit on the host, both calling only the device C-ABI in fsx.h:
+ *
+ *  fsxh_*  mesh input and topology: what the reference's setup chain does
+ *          before the time loop (main, BCH:383-521): readInputFile
+ *          (BCH:528-725), setupNonCornerNodes (BCH:954-1320), setupLtoGdof
+ *          (BCH:1327-1403), determineVelBCnodes (BCH:1410-1580),
+ *          findMonitorPoint (BCH:1644-1668), setupSparseM/G (BCH:1675-2159),
+ *          the structural pattern of Z (BCH:3322-3479) and a race-free element
+ *          colouring (the role of setupMeshColoring, BCH:853-947). The mesh
+ *          generators follow the reference's cube and channel generators
+ *          (oldFiles/meshGenerators&Converters/HexaMeshGeneratorIn*.m).
+ *  fsxd_*  the driver: timeLoop (BCH:2815-3120) over the fsx_* device calls,
+ *          in the reference's call order.
+ *
+ * BCH = fractionalStep/Cpp/blascoCodinaHuerta.cpp.
+ */
+#ifndef FSX_HOST_H
+#define FSX_HOST_H
+
+#include <stdint.h>
+#include "fsx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fsxh_mesh fsxh_mesh;
+typedef struct fsxh_topo fsxh_topo;
+typedef struct fsxd_solver fsxd_solver;
+
+/* ------------------------------------------------------------ mesh input */
+/* Lid-driven cavity on the unit cube, N^3 hex27/hex8 elements, optional sinh
+ * clustering (0 = uniform), lid velocity (lid_u, 0, 0) on z = 1. Node,
+ * element and BC-face order of the reference cube generator. */
+fsxh_mesh* fsxh_mesh_cavity(int N, double cluster, double lid_u);
+/* 90-degree bent square duct: n x n cross-section elements on [0,1]^2,
+ * n_in elements over a straight inlet of length L_in, n_arc over a bend of
+ * centreline radius R, n_out over a straight outlet of length L_out.
+ * Parabolic inlet u = peak*16 s(1-s) t(1-t); no-slip walls; natural outlet
+ * with the zero-pressure node at the outlet centre. permute != 0 shuffles
+ * corner-node and element numbering with std::mt19937_64(seed). */
+fsxh_mesh* fsxh_mesh_duct(int n, int n_in, int n_arc, int n_out, double L_in, double R, double L_out, double peak,
+                          uint64_t seed, int permute);
+/* .inp reader/writer in the layout read by BCH:528-725 */
+int fsxh_mesh_read_inp(const char* path, fsxh_mesh** out, char* err, int errlen);
+int fsxh_mesh_write_inp(const fsxh_mesh* m, const char* path);
+void fsxh_mesh_free(fsxh_mesh* m);
+/* info: NE, NCN, nVelFaces, nBC, zeroP (0-based), NGP, maxIter, has_profile */
+int fsxh_mesh_info(const fsxh_mesh* m, long long* out8);
+/* params: dt, t_ini, t_final, tolerance, converge, density, viscosity, mon x,y,z */
+int fsxh_mesh_params(const fsxh_mesh* m, double* out9);
+int fsxh_mesh_set_params(fsxh_mesh* m, int NGP, double dt, int maxIter, double tolerance, double converge,
+                         double density, double viscosity);
+/* arrays: "coord" (NCN*3 f64), "conn" (NE*8 i32, 0-based), "faces" (nVelFaces*3
+ * i32, 0-based elem/face/bc), "BCstr" (nBC*3 f64) */
+int fsxh_mesh_get(const fsxh_mesh* m, const char* name, void* dst);
+/* parabolic inlet profile: bc id, peak, lower corner (y0, z0) of the inlet */
+int fsxh_mesh_profile(const fsxh_mesh* m, double* out4);
+
+/* ------------------------------------------------------------ topology */
+int fsxh_topology(const fsxh_mesh* m, fsxh_topo** out, char* err, int errlen);
+void fsxh_topo_free(fsxh_topo* t);
+/* sizes: NE, NCN, NN, NNp, nnzM, nnzG, nnzZ, nBCnodes, ncolors, monPoint, zeroP */
+int fsxh_topo_info(const fsxh_topo* t, long long* out11);
+/* int arrays: "LtoGnode" (NE*27), "Mrow","Mcol","Grow","Gcol","Zrow","Zcol",
+ * "BCnodes","BCwhich","color_ptr","color_order" ; f64: "coord" (NN*3),
+ * "BCvalues" (nBCnodes*3) */
+long long fsxh_topo_len(const fsxh_topo* t, const char* name);
+int fsxh_topo_get(const fsxh_topo* t, const char* name, void* dst);
+
+/* ------------------------------------------------------------ driver */
+typedef struct fsxd_options {
+  int device;
+  int NGP;            /* 1, 8 or 27 */
+  int maxIter;        /* inner iterations per step (BCH:553) */
+  double tolerance;   /* inner convergence (BCH:554) */
+  double converge;    /* steady-state criterion (BCH:555) */
+  int solver_mode;    /* fsx_solver_mode */
+  double pcg_rtol;
+  int pcg_maxit;
+  int timing;         /* 1: record per-kernel device times */
+} fsxd_options;
+
+typedef struct fsxd_step_stats {
+  int step;           /* timeN */
+  int inner_iters;
+  int pcg_iters;      /* summed over the step's solves */
+  int steady;
+  double time;        /* timeT */
+  double max_acc;
+  double mon[4];      /* u, v, w, p at the monitor node */
+  double wall_s;      /* host wall time of the step (device synchronised) */
+  double pcg_relres;  /* of the last solve */
+} fsxd_step_stats;
+
+void fsxd_default_options(fsxd_options* o);
+/* Builds the topology, creates the device context, uploads, assembles
+ * (step0 + Z) and initialises the state. */
+int fsxd_create(const fsxh_mesh* m, const fsxd_options* o, fsxd_solver** out, char* err, int errlen);
+void fsxd_free(fsxd_solver* s);
+/* Runs up to nsteps time steps (fewer if steady state is reached, as the
+ * reference does); returns the number of steps done or a negative status. */
+int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats);
+fsx_ctx* fsxd_ctx(fsxd_solver* s);
+const fsxh_topo* fsxd_topo(fsxd_solver* s);
+const char* fsxd_error(fsxd_solver* s);
+/* setup wall times (s): topology, upload, assembly (incl. Z) */
+int fsxd_setup_times(fsxd_solver* s, double* out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSX_HOST_H */
