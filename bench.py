@@ -1,0 +1,185 @@
+"""Benchmark: timesteps/s of the fractional-step hot path on one MI355X.
+
+Workload: lid-driven cavity on the unit cube, 64^3 hex27-velocity /
+hex8-pressure elements (129^3 velocity nodes, 65^3 pressure nodes), 27-point
+Gauss quadrature, Re = 1000 (viscosity 1/1000 with lid speed 1), dt = 0.0025,
+3 inner iterations per step at tolerance 1e-3, pressure solved to a relative
+residual of 1e-8. Inputs are synthetic (generated mesh, zero initial fields
+plus the lid BC) and resident in HBM before timing starts.
+
+One "step" is one time step of the reference's timeLoop (BCH:2859-3118):
+convection residual, step1, step2 (assembly of R2 + the pressure solve),
+step3, convergence checks, K*Acc, for up to 3 inner iterations. The same
+trajectory is run with the plain FP64 Jacobi-PCG and with the mixed-precision
+solver (FP32 inner PCG, FP64 refinement residual); `value` is the headline
+(mixed) mode, the FP64 rate and the pressure-solve speed-up sit beside it.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cfd-with-cuda_amd"))
+
+METRIC = "timesteps/s, 64^3 hex27 cavity: learned+mixed vs plain FP64 PCG at 1e-8 residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_mode(solver, mode, steps, warmup, rtol, maxit):
+    import numpy as np
+    solver.reset(mode=mode, rtol=rtol, maxit=maxit)
+    solver.timing_reset(True)
+    if warmup:
+        solver.run(warmup)
+    solver.timing_reset(True)
+    solver.sync()
+    t0 = time.perf_counter()
+    st = solver.run(steps)
+    solver.sync()
+    t1 = time.perf_counter()
+    tm = solver.timing()
+    assert len(st) == steps, "steady state reached early"
+    return dict(
+        wall_s=t1 - t0, steps_per_s=steps / (t1 - t0), pcg_iters=sum(s["pcg"] for s in st),
+        inner_iters=sum(s["inner"] for s in st), max_relres=max(s["relres"] for s in st),
+        solve_ms=tm["pcg_ms"], Un=solver.download("Un"), Pn=solver.download("Pn"),
+        mon=st[-1]["mon"], per_step=[dict(pcg=s["pcg"], inner=s["inner"], wall_s=s["wall_s"]) for s in st],
+    )
+
+
+def cpu_baseline(mesh, args, gpu_pcg_per_step, gpu_inner_per_step):
+    """The CPU oracle on the host cores, on a bounded sample of the same 64^3
+    workload: setup (untimed), then one inner iteration (step1, step2 with the
+    PCG capped at `cpu_pcg_iters` iterations, step3) timed, extrapolated to a
+    full step with the GPU run's measured inner and PCG iteration counts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import fsx_oracle
+    if not os.path.exists(fsx_oracle._LIB_PATH):
+        fsx_oracle.build()
+    cores = len(os.sched_getaffinity(0))
+    fsx_oracle.set_threads(cores)
+    t0 = time.perf_counter()
+    o = fsx_oracle.Oracle()
+    o.set_mesh(mesh.arrays())
+    o.set_params(NGP=args.ngp, viscosity=1.0 / args.re, dt=args.dt, maxIter=3, tolerance=1e-3, converge=1e-6,
+                 pcg_rtol=1e-30, pcg_maxit=args.cpu_pcg_iters)
+    o.setup()
+    t_setup = time.perf_counter() - t0
+    o.set_d("UnpHalf_prev", o.d("Un"))
+    t0 = time.perf_counter()
+    o.step1(1)
+    t1 = time.perf_counter()
+    o.step2(1)
+    t2 = time.perf_counter()
+    o.step3(1)
+    t3 = time.perf_counter()
+    tm = o.timings()
+    t_pcg_iter = tm["pcg_s"] / max(1, tm["pcg_iters"])
+    t_other = (t1 - t0) + (t2 - t1 - tm["pcg_s"]) + (t3 - t2)
+    t_step = gpu_inner_per_step * t_other + gpu_pcg_per_step * t_pcg_iter
+    o.close()
+    return dict(value=1.0 / t_step, unit="timesteps/s", cores=cores, kind="port",
+                sample=(f"64^3 cavity on the CPU oracle (C++/OpenMP, {cores} threads): one inner iteration timed "
+                        f"(step1 {t1 - t0:.2f} s, step2 non-solve {t2 - t1 - tm['pcg_s']:.2f} s, step3 {t3 - t2:.2f} s) "
+                        f"and {tm['pcg_iters']} FP64 Jacobi-PCG iterations ({t_pcg_iter * 1e3:.1f} ms each), "
+                        f"extrapolated to {gpu_inner_per_step:.2f} inner iterations and {gpu_pcg_per_step:.1f} PCG "
+                        f"iterations per step as measured on the GPU; setup {t_setup:.1f} s untimed"),
+                t_step_s=t_step)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--re", type=float, default=1000.0)
+    ap.add_argument("--dt", type=float, default=0.0025)
+    ap.add_argument("--ngp", type=int, default=27)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--maxit", type=int, default=20000)
+    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-pcg-iters", type=int, default=30)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the path is single-GPU, "
+              f"see the sharding design in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import fsx
+
+    mesh = fsx.Mesh.cavity(args.n).set_params(NGP=args.ngp, dt=args.dt, maxIter=3, tolerance=1e-3, converge=1e-6,
+                                              viscosity=1.0 / args.re)
+    log(f"setup {args.n}^3 cavity ...")
+    t0 = time.perf_counter()
+    solver = fsx.Solver(mesh, device=0, NGP=args.ngp, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=args.rtol,
+                        pcg_maxit=args.maxit, solver_mode=0)
+    log(f"setup done in {time.perf_counter() - t0:.1f} s: {solver.setup_times()} sizes {solver.sizes}")
+    modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED}
+    res = {}
+    for name in args.modes.split(","):
+        log(f"mode {name}: {args.warmup} warmup + {args.steps} timed steps")
+        res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit)
+        r = res[name]
+        log(f"  {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms, "
+            f"max relres {r['max_relres']:.2e}")
+    head = "mixed" if "mixed" in res else list(res)[0]
+    prec = 1 if head == "mixed" else 0
+    ms, bytes_per = solver.bench_spmv(precision=prec, reps=100)
+    ms64, bytes64 = solver.bench_spmv(precision=0, reps=100)
+    achieved = bytes_per / (ms * 1e-3) / 1e9
+    h = res[head]
+    out = {
+        "metric": METRIC,
+        "value": round(h["steps_per_s"], 4),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * h["wall_s"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {args.n}^3 hex27/hex8, NGP {args.ngp}, Re {args.re:g}, "
+                               f"dt {args.dt:g}, 3 inner iterations, pressure rtol {args.rtol:g}",
+                   "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
+                   "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": f"k_pcg_spmv<{'float' if prec else 'double'},16>",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_per, "launch_ms": round(ms, 5)},
+        "modes": {k: {"steps_per_s": round(v["steps_per_s"], 4), "pcg_iters": v["pcg_iters"],
+                      "inner_iters": v["inner_iters"], "solve_ms": round(v["solve_ms"], 2),
+                      "max_relres": v["max_relres"]} for k, v in res.items()},
+        "spmv_fp64": {"launch_ms": round(ms64, 5), "achieved_GBs": round(bytes64 / (ms64 * 1e-3) / 1e9, 1)},
+    }
+    if "fp64" in res and head != "fp64":
+        f = res["fp64"]
+        out["pressure_solve_speedup"] = round(f["solve_ms"] / h["solve_ms"], 3)
+        out["vs_fp64_steps"] = round(h["steps_per_s"] / f["steps_per_s"], 3)
+        out["velocity_rel_err_vs_fp64"] = float(np.linalg.norm(h["Un"] - f["Un"]) / np.linalg.norm(f["Un"]))
+        out["pressure_rel_err_vs_fp64"] = float(np.linalg.norm(h["Pn"] - f["Pn"]) / np.linalg.norm(f["Pn"]))
+    if not args.no_cpu:
+        base = res.get("fp64", h)
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(mesh, args, base["pcg_iters"] / args.steps,
+                                           base["inner_iters"] / args.steps)
+    else:
+        out["cpu_baseline"] = None
+    solver.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -429,12 +429,14 @@ int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_
 
 int fsx_next_iteration(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
-  // BCH:2964-2972 as pointer swaps: every "current" buffer is fully
-  // rewritten before it is read again in the next iteration
+  // BCH:2964-2972. UnpHalf and Acc are rewritten by step1/step3 before they
+  // are read again, so their "prev" copies are pointer swaps. Unp1 and Pnp1
+  // must keep their values: when this was the last inner iteration the time
+  // loop still moves them into Un and Pn (BCH:3070-3077), so they are copied.
   std::swap(ctx->UnpHalf_prev, ctx->UnpHalf);
-  std::swap(ctx->Unp1_prev, ctx->Unp1);
   std::swap(ctx->Acc_prev, ctx->Acc);
-  std::swap(ctx->Pnp1_prev, ctx->Pnp1);
+  FSX_TRY(launch_copy(ctx, ctx->Unp1_prev, ctx->Unp1, 3LL * ctx->NN));
+  FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pnp1, ctx->NNp));
   return FSX_OK;
 }
 

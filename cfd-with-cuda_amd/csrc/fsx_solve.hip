@@ -460,6 +460,40 @@ static int pcg_run(fsx_ctx* ctx, const T* val, const T* dinv, const double* b, T
   return FSX_OK;
 }
 
+// ------------------------------------------------------------ SpMV timing
+__global__ void k_arm(SolveScalars* sc) {
+  sc->done = 0;
+  sc->rz = 1.0;
+}
+
+template <class T>
+__global__ void k_fill_t(T* __restrict__ p, T v, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <class T>
+static int bench_spmv_t(fsx_ctx* ctx, const T* val, T* p, T* q, int reps, double* ms) {
+  constexpr int LPR = 16;
+  const int n = ctx->NNp;
+  hipLaunchKernelGGL(k_fill_t<T>, dim3(1024), dim3(kBlock), 0, ctx->stream, p, (T)1, n);
+  hipLaunchKernelGGL(k_arm, dim3(1), dim3(1), 0, ctx->stream, ctx->sc);
+  const int g_rows = row_grid(n, kBlock / LPR);
+  for (int w = 0; w < 3; w++)
+    hipLaunchKernelGGL((k_pcg_spmv<T, LPR>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, val, p, q,
+                       ctx->partials, ctx->sc);
+  FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL((k_pcg_spmv<T, LPR>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, val, p, q,
+                       ctx->partials, ctx->sc);
+  FSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  FSX_HIP(hipEventSynchronize(ctx->ev1));
+  FSX_TRY(launch_check(ctx, "k_pcg_spmv (timing)"));
+  float t = 0;
+  FSX_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  *ms = t / reps;
+  return FSX_OK;
+}
+
 // ------------------------------------------------------------ mixed precision
 __global__ void k_to_float(int n, const double* __restrict__ a, float* __restrict__ b) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
@@ -551,3 +585,22 @@ int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double*
 }
 
 }  // namespace fsx
+
+extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch, double* bytes_per_launch) {
+  using namespace fsx;
+  if (!ctx || reps < 1 || !ms_per_launch) return FSX_EINVAL;
+  if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  double ms = 0;
+  const double nnz = (double)ctx->nnzZ, n = ctx->NNp;
+  double bytes;
+  if (precision == 0) {
+    FSX_TRY(bench_spmv_t<double>(ctx, ctx->Z, ctx->wp, ctx->wq, reps, &ms));
+    bytes = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
+  } else {
+    FSX_TRY(bench_spmv_t<float>(ctx, ctx->Zf, ctx->fp, ctx->fq, reps, &ms));
+    bytes = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
+  }
+  *ms_per_launch = ms;
+  if (bytes_per_launch) *bytes_per_launch = bytes;
+  return FSX_OK;
+}

@@ -74,6 +74,7 @@ extern "C" int fsxd_create(const fsxh_mesh* m, const fsxd_options* o, fsxd_solve
   if (rc != FSX_OK) return fail(rc);
   s->dt = m->dt;
   s->t = m->t_ini;
+  s->t0 = m->t_ini;
   *out = s.release();
   return FSX_OK;
 }
@@ -131,6 +132,15 @@ extern "C" int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats) {
     if (steady) { s->steady = 1; return n + 1; }
   }
   return nsteps;
+}
+
+extern "C" int fsxd_reset(fsxd_solver* s) {
+  DRV_CHECK(fsx_init_state(s->ctx));
+  DRV_CHECK(fsx_sync(s->ctx));
+  s->timeN = 0;
+  s->t = s->t0;
+  s->steady = 0;
+  return FSX_OK;
 }
 
 extern "C" fsx_ctx* fsxd_ctx(fsxd_solver* s) { return s->ctx; }

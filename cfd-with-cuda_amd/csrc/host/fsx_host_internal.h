@@ -41,7 +41,7 @@ struct fsxd_solver {
   fsxh_topo* topo = nullptr;
   fsx_ctx* ctx = nullptr;
   fsxd_options opt{};
-  double dt = 0.0, t = 0.0;
+  double dt = 0.0, t = 0.0, t0 = 0.0;
   int timeN = 0;
   int steady = 0;
   double t_topo = 0, t_upload = 0, t_assemble = 0;

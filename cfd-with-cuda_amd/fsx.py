@@ -31,13 +31,13 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_spmv", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_timing_reset", "fsx_timing_get",
 )
 HOST_SYMBOLS = (
     "fsxh_mesh_cavity", "fsxh_mesh_duct", "fsxh_mesh_read_inp", "fsxh_mesh_write_inp", "fsxh_mesh_free",
     "fsxh_mesh_info", "fsxh_mesh_params", "fsxh_mesh_set_params", "fsxh_mesh_get", "fsxh_mesh_profile",
     "fsxh_topology", "fsxh_topo_free", "fsxh_topo_info", "fsxh_topo_len", "fsxh_topo_get",
-    "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_ctx", "fsxd_topo", "fsxd_error",
+    "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_reset", "fsxd_ctx", "fsxd_topo", "fsxd_error",
     "fsxd_setup_times",
 )
 
@@ -99,6 +99,7 @@ def lib():
         "fsx_sync": ([_vp], ctypes.c_int),
         "fsx_solve_pressure": ([_vp, _D, _D, _I, _D], ctypes.c_int),
         "fsx_spmv": ([_vp, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_timing_reset": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_timing_get": ([_vp, ctypes.POINTER(Timing)], ctypes.c_int),
         "fsxh_mesh_cavity": ([ctypes.c_int, ctypes.c_double, ctypes.c_double], _vp),
@@ -122,6 +123,7 @@ def lib():
         "fsxd_create": ([_vp, ctypes.POINTER(Options), ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
         "fsxd_free": ([_vp], None),
         "fsxd_run": ([_vp, ctypes.c_int, ctypes.POINTER(StepStats)], ctypes.c_int),
+        "fsxd_reset": ([_vp], ctypes.c_int),
         "fsxd_ctx": ([_vp], _vp),
         "fsxd_topo": ([_vp], _vp),
         "fsxd_error": ([_vp], ctypes.c_char_p),
@@ -321,6 +323,21 @@ class Solver:
             raise FsxError(-n, self.L.fsxd_error(self.h).decode())
         return [dict(step=s.step, inner=s.inner_iters, pcg=s.pcg_iters, steady=s.steady, time=s.time,
                      max_acc=s.max_acc, mon=list(s.mon), wall_s=s.wall_s, relres=s.pcg_relres) for s in stats[:n]]
+
+    def reset(self, mode=None, rtol=None, maxit=None):
+        """Back to the initial fields; optionally switch the pressure solver."""
+        rc = self.L.fsxd_reset(self.h)
+        if rc != FSX_OK:
+            raise FsxError(rc, self.L.fsxd_error(self.h).decode())
+        if mode is not None:
+            self.set_solver(mode, self.opt.pcg_rtol if rtol is None else rtol,
+                            self.opt.pcg_maxit if maxit is None else maxit)
+
+    def bench_spmv(self, precision=0, reps=50):
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        self.check(self.L.fsx_bench_spmv(self.ctx, precision, reps, ctypes.byref(ms), ctypes.byref(by)), "bench_spmv")
+        return ms.value, by.value
 
     def setup_times(self):
         out = np.zeros(3)

@@ -142,7 +142,8 @@ int fsx_step3(fsx_ctx* ctx, int iter);
  * (CU:1084-1121) */
 int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_u, double* rel_p);
 /* UnpHalf_prev <- UnpHalf, Unp1_prev <- Unp1, Acc_prev <- Acc,
- * Pnp1_prev <- Pnp1 (BCH:2964-2972, CU via BCH:2925-2932) */
+ * Pnp1_prev <- Pnp1 (BCH:2964-2972, CU via BCH:2925-2932); Unp1 and Pnp1
+ * keep their values (they become Un, Pn after the last iteration). */
 int fsx_next_iteration(fsx_ctx* ctx);
 /* KAcc = K Acc_prev; replaces calculate_KtimesAcc_prevGPU (CU:1067-1077) */
 int fsx_kacc(fsx_ctx* ctx);
@@ -166,6 +167,15 @@ int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
  * which = 0 K (NN), 1..3 G_d (NN <- NNp), 4 Z (NNp), 5..7 G_d^T (NNp <- NN).
  * Mirrors the reference's mkl_dcsrmv / cusparseDcsrmv calls. */
 int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, double beta);
+
+/* Times `reps` back-to-back launches of the pressure solver's SpMV kernel
+ * (q = Z p with its fused p.q reduction, the dominant kernel of a step) on the
+ * context's stream with HIP events. precision 0 = FP64 Z, 1 = FP32 Z (the
+ * mixed-precision inner solve). Returns the mean device time per launch in
+ * ms and the algorithmic HBM bytes of one launch: values + column indices +
+ * row pointers + one read of p + one write of q. No reference counterpart
+ * (the reference prints wall times only, BCH:4489-4518). */
+int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch, double* bytes_per_launch);
 
 /* kernel time accounting (device time of the last timed region, ms) */
 typedef struct fsx_timing {

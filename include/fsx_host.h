@@ -103,6 +103,9 @@ void fsxd_free(fsxd_solver* s);
 /* Runs up to nsteps time steps (fewer if steady state is reached, as the
  * reference does); returns the number of steps done or a negative status. */
 int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats);
+/* Back to t_ini and the initial fields (fsx_init_state), keeping the
+ * assembled matrices; the solver mode may be changed with fsx_set_solver. */
+int fsxd_reset(fsxd_solver* s);
 fsx_ctx* fsxd_ctx(fsxd_solver* s);
 const fsxh_topo* fsxd_topo(fsxd_solver* s);
 const char* fsxd_error(fsxd_solver* s);

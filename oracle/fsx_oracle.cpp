@@ -840,7 +840,7 @@ int timeStep(Case& c) {
     for (int d = 0; d < 3; d++)                                          // BCH:2981-3035
       csrmv(c.Mrow, c.Mcol, c.Kval.data(), c.Acc_prev.data() + (size_t)d*NN, c.KAcc.data() + (size_t)d*NN, NN, 1.0, 0.0);
   }
-  c.last_inner_iters = iter;
+  c.last_inner_iters = std::min(iter, c.maxIter);
   double oneOverdt = 1.0000000000000000 / c.dt;                         // BCH:3048-3062
   c.maxAcc = 0.0;
   for (size_t i = 0; i < (size_t)3*NN; i++) {
