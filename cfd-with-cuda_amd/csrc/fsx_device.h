@@ -138,6 +138,36 @@ __device__ inline bool grid_max(double v, double* partials, unsigned int* counte
   return true;
 }
 
+// Streaming loads: CSR values and column indices are read once per launch,
+// so they are loaded non-temporally and leave L2 to the gathered vectors.
+template <class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// Partial dot products of one CSR row for the LPR lanes of a row group: lane
+// l takes entries start+l, start+l+LPR, ... in rounds of U entries whose loads
+// are all issued before the first FMA (memory-level parallelism instead of a
+// dependent load chain). Out-of-row slots are clamped to a valid index and
+// their products selected away. `g(c, v, acc)` consumes one (column, value).
+template <int LPR, int U, class V, class G>
+__device__ __forceinline__ void row_foreach(const int* __restrict__ ci, const V* __restrict__ val, int vstride,
+                                            int start, int end, int l, G&& g) {
+  for (int base = start + l; base < end; base += U * LPR) {
+    int c[U];
+    V v[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int k = base + j * LPR;
+      const int kk = k < end ? k : end - 1;
+      c[j] = ld_stream(ci + kk);
+      v[j] = k < end ? ld_stream(val + (size_t)vstride * kk) : V(0);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) g(c[j], v[j], (base + j * LPR) < end);
+  }
+}
+
 // Position of `c` in the sorted columns col[lo, hi); the entry must exist.
 __device__ __forceinline__ int find_col(const int* __restrict__ col, int lo, int hi, int c) {
   while (hi - lo > 1) {

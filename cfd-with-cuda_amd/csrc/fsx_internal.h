@@ -22,7 +22,8 @@ constexpr int kZRowMax = 1024;    // longest Z row handled by the Z-build kernel
 // kernels run.
 struct SolveScalars {
   double rz, pq, rr, bb, alpha, beta, rtol2bb;
-  double rz_new;
+  double rr_anchor;   // mixed solver: FP64 residual^2 at the last reliable update
+  double rr_hi;       // mixed solver: latest FP64 residual^2
   int iters, done, maxit, pad;
   unsigned int cnt[4];            // last-block counters, reset by the last block
 };
@@ -84,8 +85,8 @@ struct fsx_ctx {
   double *Acc = nullptr, *Acc_prev = nullptr, *KAcc = nullptr, *R1 = nullptr;
   double *Pn = nullptr, *Pnp1 = nullptr, *Pnp1_prev = nullptr, *Pdot = nullptr, *R2 = nullptr;
   // solver work (device)
-  double *wr = nullptr, *wp = nullptr, *wq = nullptr, *wx = nullptr;    // FP64 PCG / IR outer
-  float *fr = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 inner PCG
+  double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wq2 = nullptr, *wq = nullptr;   // FP64 PCG (p double-buffered)
+  float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fp2 = nullptr, *fq = nullptr, *fx = nullptr;   // FP32 PCG
   double* partials = nullptr;    // 8 * kMaxGrid
   fsx::SolveScalars* sc = nullptr;
   fsx::StepScalars* st = nullptr;
@@ -102,6 +103,7 @@ struct fsx_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   fsx_timing tm{};
   int last_iters = 0;
+  int last_updates = 0;
   double last_relres = 0.0;
 };
 

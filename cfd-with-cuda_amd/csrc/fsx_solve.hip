@@ -1,11 +1,13 @@
 // fsx_solve.hip -- CSR row kernels of the time step (fused SpMV + elementwise
-// epilogues), deterministic reductions and the Jacobi-PCG pressure solver.
+// epilogues), deterministic reductions and the Jacobi-PCG pressure solvers.
 //
-// Every row kernel maps a group of LPR lanes to one CSR row: the group reads
-// the row's nonzeros contiguously (coalesced), gathers x through the column
-// indices (L2-resident for the local meshes here), and reduces with xor
-// shuffles inside the wavefront. Blocks own contiguous row ranges and are
-// renumbered so that each XCD walks one contiguous eighth of the rows.
+// Row kernels map a group of LPR lanes to one CSR row. Each lane walks the
+// row in rounds of U entries whose column/value loads are issued together
+// (non-temporal: they stream once) before the dependent gathers and FMAs, so
+// a wavefront keeps several HBM requests in flight instead of one dependent
+// chain. Row pointers of the next pass are prefetched. Blocks own contiguous
+// row ranges and are renumbered so that each XCD walks one contiguous eighth
+// of the rows (its gathers then stay in its own L2).
 #include "fsx_device.h"
 
 #include <algorithm>
@@ -35,10 +37,29 @@ int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n) {
   return launch_check(ctx, "k_copy");
 }
 
+// Row-group iteration scaffold: calls body(r, rs, re) for every row of the
+// block's range (r >= r1 marks an idle group in the last pass, rs = re = 0),
+// with the next pass's row pointers loaded ahead.
+template <int LPR, class Body>
+__device__ __forceinline__ void for_rows(long long nrows, const int* __restrict__ rp, Body&& body) {
+  constexpr int RPB = kBlock / LPR;
+  long long r0, r1;
+  block_rows(nrows, xcd_block(), gridDim.x, RPB, &r0, &r1);
+  const int grp = threadIdx.x / LPR;
+  long long r = r0 + grp;
+  int rs = r < r1 ? rp[r] : 0, re = r < r1 ? rp[r + 1] : 0;
+  for (long long base = r0; base < r1; base += RPB) {
+    const long long rn = base + RPB + grp;
+    const int rsn = rn < r1 ? rp[rn] : 0, ren = rn < r1 ? rp[rn + 1] : 0;
+    body(base + grp, r1, rs, re);
+    rs = rsn; re = ren;
+  }
+}
+
 // ------------------------------------------------------------ step 1
 // R1 <- (R1 - K UnpHalf_prev) - G Pn; BC rows 0; UnpHalf = Un + dt R1 MdInv.
 // (BCH:3746-3783; the convection part of R1 is already in R1.)
-template <int LPR>
+template <int LPR, int UK, int UG>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, const int* __restrict__ Mcol,
                                                   const double* __restrict__ K, const int* __restrict__ Grow,
                                                   const int* __restrict__ Gcol, const double* __restrict__ G,
@@ -46,24 +67,41 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
                                                   const double* __restrict__ Un, double* __restrict__ R1,
                                                   const double* __restrict__ MdInv, const unsigned char* __restrict__ bc,
                                                   double* __restrict__ UnpHalf, double dt) {
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(NN, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long r = base + grp;
+  const int l = threadIdx.x % LPR;
+  const size_t n1 = NN, n2 = 2 * (size_t)NN;
+  for_rows<LPR>(NN, Mrow, [&](long long r, long long r1, int rs, int re) {
     double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0;
-    if (r < r1) {
-      const int e = Mrow[r + 1];
-      for (int k = Mrow[r] + l; k < e; k += LPR) {
-        const int c = Mcol[k];
-        const double a = K[k];
-        k0 += a * Uprev[c]; k1 += a * Uprev[NN + c]; k2 += a * Uprev[2 * (size_t)NN + c];
+    for (int base = rs + l; base < re; base += UK * LPR) {
+      int c[UK];
+      double a[UK];
+#pragma unroll
+      for (int j = 0; j < UK; j++) {
+        const int k = base + j * LPR, kk = k < re ? k : re - 1;
+        c[j] = ld_stream(Mcol + kk);
+        a[j] = k < re ? ld_stream(K + kk) : 0.0;
       }
-      const int eg = Grow[r + 1];
-      for (int k = Grow[r] + l; k < eg; k += LPR) {
-        const double p = Pn[Gcol[k]];
-        g0 += G[3 * (size_t)k] * p; g1 += G[3 * (size_t)k + 1] * p; g2 += G[3 * (size_t)k + 2] * p;
+#pragma unroll
+      for (int j = 0; j < UK; j++) {
+        k0 += a[j] * Uprev[c[j]]; k1 += a[j] * Uprev[n1 + c[j]]; k2 += a[j] * Uprev[n2 + c[j]];
+      }
+    }
+    const int gs = r < r1 ? Grow[r] : 0, ge = r < r1 ? Grow[r + 1] : 0;
+    for (int base = gs + l; base < ge; base += UG * LPR) {
+      int c[UG];
+      double a0[UG], a1[UG], a2[UG];
+#pragma unroll
+      for (int j = 0; j < UG; j++) {
+        const int k = base + j * LPR, kk = k < ge ? k : ge - 1;
+        c[j] = ld_stream(Gcol + kk);
+        const bool ok = k < ge;
+        a0[j] = ok ? ld_stream(G + 3 * (size_t)kk) : 0.0;
+        a1[j] = ok ? ld_stream(G + 3 * (size_t)kk + 1) : 0.0;
+        a2[j] = ok ? ld_stream(G + 3 * (size_t)kk + 2) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < UG; j++) {
+        const double p = Pn[c[j]];
+        g0 += a0[j] * p; g1 += a1[j] * p; g2 += a2[j] * p;
       }
     }
     k0 = group_sum<LPR>(k0); k1 = group_sum<LPR>(k1); k2 = group_sum<LPR>(k2);
@@ -77,13 +115,13 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
       R1[i] = R;
       UnpHalf[i] = Un[i] + dt * R * MdInv[r];
     }
-  }
+  });
 }
 
 int launch_step1(fsx_ctx* ctx, int iter) {
   (void)iter;
   constexpr int LPR = 16;
-  hipLaunchKernelGGL(k_step1<LPR>, dim3(row_grid(ctx->NN, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+  hipLaunchKernelGGL((k_step1<LPR, 4, 2>), dim3(row_grid(ctx->NN, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                      ctx->Mrow, ctx->Mcol, ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un,
                      ctx->R1, ctx->MdInv, ctx->bcflag, ctx->UnpHalf, ctx->dt);
   return launch_check(ctx, "k_step1");
@@ -91,24 +129,25 @@ int launch_step1(fsx_ctx* ctx, int iter) {
 
 // ------------------------------------------------------------ K * Acc_prev
 // KAcc_c = K Acc_prev_c, c = 0..2, one pass over K (BCH:2981-3035)
-template <int LPR>
+template <int LPR, int U>
 __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ Mrow, const int* __restrict__ Mcol,
                                                const double* __restrict__ K, const double* __restrict__ x,
                                                double* __restrict__ y, double alpha, double beta) {
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(NN, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long r = base + grp;
+  const int l = threadIdx.x % LPR;
+  const size_t n1 = NN, n2 = 2 * (size_t)NN;
+  for_rows<LPR>(NN, Mrow, [&](long long r, long long r1, int rs, int re) {
     double k0 = 0, k1 = 0, k2 = 0;
-    if (r < r1) {
-      const int e = Mrow[r + 1];
-      for (int k = Mrow[r] + l; k < e; k += LPR) {
-        const int c = Mcol[k];
-        const double a = K[k];
-        k0 += a * x[c]; k1 += a * x[NN + c]; k2 += a * x[2 * (size_t)NN + c];
+    for (int base = rs + l; base < re; base += U * LPR) {
+      int c[U];
+      double a[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = base + j * LPR, kk = k < re ? k : re - 1;
+        c[j] = ld_stream(Mcol + kk);
+        a[j] = k < re ? ld_stream(K + kk) : 0.0;
       }
+#pragma unroll
+      for (int j = 0; j < U; j++) { k0 += a[j] * x[c[j]]; k1 += a[j] * x[n1 + c[j]]; k2 += a[j] * x[n2 + c[j]]; }
     }
     k0 = group_sum<LPR>(k0); k1 = group_sum<LPR>(k1); k2 = group_sum<LPR>(k2);
     if (r < r1 && l < 3) {
@@ -116,13 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
       const size_t i = (size_t)l * NN + r;
       y[i] = (beta == 0.0 ? 0.0 : beta * y[i]) + alpha * kk;
     }
-  }
+  });
 }
 
 int launch_kacc(fsx_ctx* ctx) {
   constexpr int LPR = 16;
-  hipLaunchKernelGGL(k_k3<LPR>, dim3(row_grid(ctx->NN, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,
-                     ctx->Mcol, ctx->K, ctx->Acc_prev, ctx->KAcc, 1.0, 0.0);
+  hipLaunchKernelGGL((k_k3<LPR, 4>), dim3(row_grid(ctx->NN, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                     ctx->Mrow, ctx->Mcol, ctx->K, ctx->Acc_prev, ctx->KAcc, 1.0, 0.0);
   return launch_check(ctx, "k_k3");
 }
 
@@ -130,40 +169,48 @@ int launch_kacc(fsx_ctx* ctx) {
 // R2[p] = sum_c G_c^T d_c, d_c = UnpHalf_c / dt^2 - [iter>1] MdOrigInv KAcc_c,
 // summed as (c0 + c1) + c2 like the three csrmv calls of BCH:3864-3868;
 // R2[p0] = 0 for p0 > 0 (BCH:4180-4184).
-template <int LPR>
+template <int LPR, int U>
 __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int* __restrict__ GTrow,
                                                       const int* __restrict__ GTcol, const double* __restrict__ GT,
                                                       const double* __restrict__ UnpHalf, const double* __restrict__ MdOrigInv,
                                                       const double* __restrict__ KAcc, int use_kacc, double o2,
                                                       int zeroP, double* __restrict__ R2) {
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(NNp, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long p = base + grp;
+  const int l = threadIdx.x % LPR;
+  const size_t n1 = NN, n2 = 2 * (size_t)NN;
+  for_rows<LPR>(NNp, GTrow, [&](long long p, long long r1, int rs, int re) {
     double s0 = 0, s1 = 0, s2 = 0;
-    if (p < r1) {
-      const int e = GTrow[p + 1];
-      for (int k = GTrow[p] + l; k < e; k += LPR) {
-        const int i = GTcol[k];
-        double d0 = o2 * UnpHalf[i], d1 = o2 * UnpHalf[NN + i], d2 = o2 * UnpHalf[2 * (size_t)NN + i];
+    for (int base = rs + l; base < re; base += U * LPR) {
+      int c[U];
+      double a0[U], a1[U], a2[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = base + j * LPR, kk = k < re ? k : re - 1;
+        c[j] = ld_stream(GTcol + kk);
+        const bool ok = k < re;
+        a0[j] = ok ? ld_stream(GT + 3 * (size_t)kk) : 0.0;
+        a1[j] = ok ? ld_stream(GT + 3 * (size_t)kk + 1) : 0.0;
+        a2[j] = ok ? ld_stream(GT + 3 * (size_t)kk + 2) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int i = c[j];
+        double d0 = o2 * UnpHalf[i], d1 = o2 * UnpHalf[n1 + i], d2 = o2 * UnpHalf[n2 + i];
         if (use_kacc) {
           const double m = MdOrigInv[i];
-          d0 = d0 - m * KAcc[i]; d1 = d1 - m * KAcc[NN + i]; d2 = d2 - m * KAcc[2 * (size_t)NN + i];
+          d0 = d0 - m * KAcc[i]; d1 = d1 - m * KAcc[n1 + i]; d2 = d2 - m * KAcc[n2 + i];
         }
-        s0 += GT[3 * (size_t)k] * d0; s1 += GT[3 * (size_t)k + 1] * d1; s2 += GT[3 * (size_t)k + 2] * d2;
+        s0 += a0[j] * d0; s1 += a1[j] * d1; s2 += a2[j] * d2;
       }
     }
     s0 = group_sum<LPR>(s0); s1 = group_sum<LPR>(s1); s2 = group_sum<LPR>(s2);
     if (p < r1 && l == 0) R2[p] = (zeroP > 0 && p == zeroP) ? 0.0 : (s0 + s1) + s2;
-  }
+  });
 }
 
 int launch_step2_rhs(fsx_ctx* ctx, int iter) {
-  constexpr int LPR = 32;
+  constexpr int LPR = 16;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
-  hipLaunchKernelGGL(k_step2_rhs<LPR>, dim3(row_grid(ctx->NNp, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
+  hipLaunchKernelGGL((k_step2_rhs<LPR, 8>), dim3(row_grid(ctx->NNp, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
                      ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GT, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc, iter != 1 ? 1 : 0,
                      o2, ctx->zeroP, ctx->R2);
   return launch_check(ctx, "k_step2_rhs");
@@ -197,7 +244,7 @@ int launch_pnp1(fsx_ctx* ctx) {
 // R3_c = -dt G_c Pdot - [iter>1] dt KAcc_c; BC rows 0; Acc = R3 MdInv;
 // Unp1 = UnpHalf + dt Acc (BCH:3912-3974), plus the velocity norms of the
 // convergence check (BCH:2936-2943).
-template <int LPR>
+template <int LPR, int U>
 __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict__ Grow, const int* __restrict__ Gcol,
                                                   const double* __restrict__ G, const double* __restrict__ Pdot,
                                                   const double* __restrict__ KAcc, int use_kacc,
@@ -205,19 +252,26 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
                                                   const double* __restrict__ UnpHalf, const double* __restrict__ Uprev1,
                                                   double* __restrict__ Acc, double* __restrict__ Unp1, double dt,
                                                   double* partials, StepScalars* st) {
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(NN, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  const int l = threadIdx.x % LPR;
   double v[2] = {0.0, 0.0};
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long r = base + grp;
+  for_rows<LPR>(NN, Grow, [&](long long r, long long r1, int rs, int re) {
     double g0 = 0, g1 = 0, g2 = 0;
-    if (r < r1) {
-      const int eg = Grow[r + 1];
-      for (int k = Grow[r] + l; k < eg; k += LPR) {
-        const double p = Pdot[Gcol[k]];
-        g0 += G[3 * (size_t)k] * p; g1 += G[3 * (size_t)k + 1] * p; g2 += G[3 * (size_t)k + 2] * p;
+    for (int base = rs + l; base < re; base += U * LPR) {
+      int c[U];
+      double a0[U], a1[U], a2[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = base + j * LPR, kk = k < re ? k : re - 1;
+        c[j] = ld_stream(Gcol + kk);
+        const bool ok = k < re;
+        a0[j] = ok ? ld_stream(G + 3 * (size_t)kk) : 0.0;
+        a1[j] = ok ? ld_stream(G + 3 * (size_t)kk + 1) : 0.0;
+        a2[j] = ok ? ld_stream(G + 3 * (size_t)kk + 2) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const double p = Pdot[c[j]];
+        g0 += a0[j] * p; g1 += a1[j] * p; g2 += a2[j] * p;
       }
     }
     g0 = group_sum<LPR>(g0); g1 = group_sum<LPR>(g1); g2 = group_sum<LPR>(g2);
@@ -234,16 +288,16 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
       const double d = u - Uprev1[i];
       v[0] += u * u; v[1] += d * d;
     }
-  }
+  });
   double out[2];
   if (grid_sum<2>(v, partials, &st->cnt[1], out) && threadIdx.x == 0) { st->uu = out[0]; st->ud = out[1]; }
 }
 
 int launch_step3(fsx_ctx* ctx, int iter) {
   constexpr int LPR = 4;
-  hipLaunchKernelGGL(k_step3<LPR>, dim3(row_grid(ctx->NN, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
-                     ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv, ctx->UnpHalf,
-                     ctx->Unp1_prev, ctx->Acc, ctx->Unp1, ctx->dt, ctx->partials, ctx->st);
+  hipLaunchKernelGGL((k_step3<LPR, 4>), dim3(row_grid(ctx->NN, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                     ctx->Grow, ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
+                     ctx->UnpHalf, ctx->Unp1_prev, ctx->Acc, ctx->Unp1, ctx->dt, ctx->partials, ctx->st);
   return launch_check(ctx, "k_step3");
 }
 
@@ -272,18 +326,13 @@ template <int LPR>
 __global__ __launch_bounds__(kBlock) void k_csrmv(int nrows, const int* __restrict__ rp, const int* __restrict__ ci,
                                                   const double* __restrict__ v, int vs, const double* __restrict__ x,
                                                   double* __restrict__ y, double alpha, double beta) {
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(nrows, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long r = base + grp;
+  const int l = threadIdx.x % LPR;
+  for_rows<LPR>(nrows, rp, [&](long long r, long long r1, int rs, int re) {
     double s = 0;
-    if (r < r1)
-      for (int k = rp[r] + l; k < rp[r + 1]; k += LPR) s += v[(size_t)vs * k] * x[ci[k]];
+    for (int k = rs + l; k < re; k += LPR) s += v[(size_t)vs * k] * x[ci[k]];
     s = group_sum<LPR>(s);
     if (r < r1 && l == 0) y[r] = (beta == 0.0 ? 0.0 : beta * y[r]) + alpha * s;
-  }
+  });
 }
 
 int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta) {
@@ -307,28 +356,37 @@ int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double a
 // ------------------------------------------------------------ Jacobi-PCG
 // Textbook preconditioned CG with x0 = 0 (BCH:3999-4004), Jacobi
 // preconditioner (ipar[10] = 1, BCH:4017) and the stop test
-// ||r||_2 <= rtol ||b||_2 checked after each update. Three kernels per
-// iteration, all scalars stay on the device; launches after convergence
-// return at once (done != 0), so the host only polls every few iterations.
-// T is the storage/arithmetic type of Z and the vectors (double, or float
-// for the inner solve of mixed precision); dot products accumulate in FP64.
+// ||r||_2 <= rtol ||b||_2 after each update. Two kernels per iteration:
+//   A: p_new = z + beta p_old (own row), q = Z p_new with the gathered
+//      z + beta p_old, p_new . q  ->  alpha = rz / pq
+//   B: x += alpha p_new, r -= alpha q, z = dinv r, r.r, r.z  ->  stop test,
+//      beta = rz_new / rz
+// p is double-buffered (A reads p_old at neighbours while rows write p_new).
+// All scalars stay on the device; launches after the stop test return at
+// once (done != 0), so the host only polls every few iterations.
+// T is the type of Z and the vectors (double, or float for the inner solve
+// of the mixed-precision solver); dot products accumulate in FP64.
+constexpr int kPcgLPR = 16;
+constexpr int kPcgU = 8;
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __restrict__ b, const T* __restrict__ dinv,
-                                                     T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
-                                                     double rtol, int maxit, double* partials, SolveScalars* sc) {
+                                                     T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                     T* __restrict__ p_old, double rtol, int maxit, double* partials,
+                                                     SolveScalars* sc) {
   long long r0, r1;
   block_rows(n, xcd_block(), gridDim.x, kBlock, &r0, &r1);
   double v[2] = {0.0, 0.0};
   for (long long i = r0 + threadIdx.x; i < r1; i += kBlock) {
     const T bi = (T)b[i];
-    const T z = dinv[i] * bi;
-    x[i] = (T)0; r[i] = bi; p[i] = z;
-    v[0] += (double)bi * (double)z;
+    const T zi = dinv[i] * bi;
+    x[i] = (T)0; r[i] = bi; z[i] = zi; p_old[i] = (T)0;
+    v[0] += (double)bi * (double)zi;
     v[1] += (double)bi * (double)bi;
   }
   double out[2];
   if (grid_sum<2>(v, partials, &sc->cnt[0], out) && threadIdx.x == 0) {
-    sc->rz = out[0]; sc->bb = out[1]; sc->rr = out[1];
+    sc->rz = out[0]; sc->bb = out[1]; sc->rr = out[1]; sc->rr_anchor = out[1];
     sc->rtol2bb = rtol * rtol * out[1];
     sc->iters = 0; sc->maxit = maxit;
     sc->done = (out[1] == 0.0) ? 1 : 0;
@@ -336,30 +394,40 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __rest
   }
 }
 
-// q = Z p and p.q; alpha = rz / pq
-template <class T, int LPR>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                     const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
-                                                     double* partials, SolveScalars* sc) {
+template <class T, int LPR, int U>
+__global__ __launch_bounds__(kBlock) void k_pcg_a(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const T* __restrict__ val, const T* __restrict__ z, T* p0, T* p1,
+                                                  T* __restrict__ q, double* partials, SolveScalars* sc) {
   if (sc->done) return;
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(n, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  const T be = (T)sc->beta;
+  // iteration i (0-based, = iterations executed so far) reads p_(i%2), writes the other
+  const bool odd = sc->iters & 1;
+  const T* __restrict__ p_old = odd ? p1 : p0;
+  T* __restrict__ p_new = odd ? p0 : p1;
+  const int l = threadIdx.x % LPR;
   double v[1] = {0.0};
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long r = base + grp;
+  for_rows<LPR>(n, rp, [&](long long r, long long r1, int rs, int re) {
     T s = 0;
-    if (r < r1) {
-      const int e = rp[r + 1];
-      for (int k = rp[r] + l; k < e; k += LPR) s += val[k] * p[ci[k]];
+    for (int base = rs + l; base < re; base += U * LPR) {
+      int c[U];
+      T a[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = base + j * LPR, kk = k < re ? k : re - 1;
+        c[j] = ld_stream(ci + kk);
+        a[j] = k < re ? ld_stream(val + kk) : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) s += a[j] * (z[c[j]] + be * p_old[c[j]]);
     }
     if constexpr (sizeof(T) == 8) s = group_sum<LPR>(s); else s = group_sumf<LPR>(s);
     if (r < r1 && l == 0) {
+      const T pn = z[r] + be * p_old[r];
+      p_new[r] = pn;
       q[r] = s;
-      v[0] += (double)p[r] * (double)s;
+      v[0] += (double)pn * (double)s;
     }
-  }
+  });
   double out[1];
   if (grid_sum<1>(v, partials, &sc->cnt[1], out) && threadIdx.x == 0) {
     sc->pq = out[0];
@@ -367,12 +435,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
   }
 }
 
-// x += alpha p; r -= alpha q; z = dinv r; rr, rz_new; convergence / beta
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x, T* __restrict__ r,
-                                                       const T* __restrict__ p, const T* __restrict__ q,
-                                                       const T* __restrict__ dinv, double* partials, SolveScalars* sc) {
+// kind 0: plain stop test. kind 1 (FP32 inner solve of the mixed solver):
+// the recurrence residual reaching the target (done = 1) or dropping by
+// delta^2 since the last FP64 residual (done = 5) pauses for a reliable
+// update by the host.
+template <class T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_pcg_b(int n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                  const T* p0, const T* p1, const T* __restrict__ q,
+                                                  const T* __restrict__ dinv, double delta2, double* partials,
+                                                  SolveScalars* sc) {
   if (sc->done) return;
+  const T* __restrict__ p = (sc->iters & 1) ? p0 : p1;   // written by this iteration's k_pcg_a
   long long r0, r1;
   block_rows(n, xcd_block(), gridDim.x, kBlock, &r0, &r1);
   const T a = (T)sc->alpha;
@@ -381,89 +454,225 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
     x[i] += a * p[i];
     const T ri = r[i] - a * q[i];
     r[i] = ri;
-    const T z = dinv[i] * ri;
+    const T zi = dinv[i] * ri;
+    z[i] = zi;
     v[0] += (double)ri * (double)ri;
-    v[1] += (double)ri * (double)z;
+    v[1] += (double)ri * (double)zi;
   }
   double out[2];
   if (grid_sum<2>(v, partials, &sc->cnt[2], out) && threadIdx.x == 0) {
     sc->iters += 1;
     sc->rr = out[0];
-    if (!(out[0] == out[0]) || !(out[1] == out[1]) || !(sc->alpha == sc->alpha)) sc->done = 3;   // NaN
-    else if (out[0] <= sc->rtol2bb) sc->done = 1;
-    else if (sc->iters >= sc->maxit) sc->done = 2;
-    else {
+    const bool bad = !(out[0] == out[0]) || !(out[1] == out[1]) || !(sc->alpha == sc->alpha);
+    if (!bad) {
       sc->beta = out[1] / sc->rz;
       sc->rz = out[1];
     }
+    if (bad) sc->done = 3;
+    else if (out[0] <= sc->rtol2bb) sc->done = 1;
+    else if (KIND == 1 && out[0] <= delta2 * sc->rr_anchor) sc->done = 5;
+    else if (sc->iters >= sc->maxit) sc->done = 2;
   }
 }
 
-// p = dinv r + beta p
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, const T* __restrict__ r, const T* __restrict__ dinv,
-                                                        T* __restrict__ p, const SolveScalars* sc) {
-  if (sc->done) return;
-  long long r0, r1;
-  block_rows(n, xcd_block(), gridDim.x, kBlock, &r0, &r1);
-  const T be = (T)sc->beta;
-  for (long long i = r0 + threadIdx.x; i < r1; i += kBlock) p[i] = dinv[i] * r[i] + be * p[i];
+static void pcg_launch_iteration(fsx_ctx* ctx, const T* val, const T* dinv, T* x, T* r, T* z, T* p0, T* p1, T* q,
+                                 int kind, double delta2) {
+  const int n = ctx->NNp;
+  const int g_rows = row_grid(n, kBlock / kPcgLPR), g_vec = row_grid(n, kBlock);
+  hipLaunchKernelGGL((k_pcg_a<T, kPcgLPR, kPcgU>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol,
+                     val, z, p0, p1, q, ctx->partials, ctx->sc);
+  if (kind == 0)
+    hipLaunchKernelGGL((k_pcg_b<T, 0>), dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, x, r, z, p0, p1, q, dinv, delta2,
+                       ctx->partials, ctx->sc);
+  else
+    hipLaunchKernelGGL((k_pcg_b<T, 1>), dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, x, r, z, p0, p1, q, dinv, delta2,
+                       ctx->partials, ctx->sc);
 }
 
-// Runs one Jacobi-PCG solve of Z x = b to completion. b is FP64 (converted to
-// T on load); x is the T-typed solution. Returns the iterations and the
-// recurrence residual ||r||/||b||.
+// Enqueues iterations in chunks (the first sized from the previous solve)
+// and polls the device scalars after each chunk until done != 0. Launches
+// after a stop or a pause do nothing, and the p double buffer is chosen on
+// the device from the executed iteration count, so no-op launches cannot
+// put it out of step.
 template <class T>
-static int pcg_run(fsx_ctx* ctx, const T* val, const T* dinv, const double* b, T* x, T* r, T* p, T* q, double rtol,
-                   int maxit, int* iters_out, double* relres_out, int* status_out) {
-  constexpr int LPR = 16;
-  const int n = ctx->NNp;
-  const int g_rows = row_grid(n, kBlock / LPR), g_vec = row_grid(n, kBlock);
-  if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
-  hipLaunchKernelGGL(k_pcg_init<T>, dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, b, dinv, x, r, p, rtol, maxit,
-                     ctx->partials, ctx->sc);
-  FSX_TRY(launch_check(ctx, "k_pcg_init"));
+static int pcg_iterate(fsx_ctx* ctx, const T* val, const T* dinv, T* x, T* r, T* z, T* p0, T* p1, T* q, int kind,
+                       double delta2, int first_chunk, int maxit) {
   int issued = 0;
-  int chunk = ctx->last_iters > 16 ? (int)(0.9 * ctx->last_iters) : 16;
-  long long spmv_launches = 0;
+  int chunk = first_chunk > 8 ? first_chunk : 8;
   for (;;) {
-    for (int j = 0; j < chunk; j++) {
-      hipLaunchKernelGGL((k_pcg_spmv<T, LPR>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, val, p,
-                         q, ctx->partials, ctx->sc);
-      hipLaunchKernelGGL(k_pcg_update<T>, dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, x, r, p, q, dinv, ctx->partials,
-                         ctx->sc);
-      hipLaunchKernelGGL(k_pcg_pupdate<T>, dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, r, dinv, p, ctx->sc);
-    }
-    spmv_launches += chunk;
+    for (int j = 0; j < chunk; j++) pcg_launch_iteration<T>(ctx, val, dinv, x, r, z, p0, p1, q, kind, delta2);
     issued += chunk;
     FSX_TRY(launch_check(ctx, "pcg iteration"));
     FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
     FSX_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->h_sc->done) break;
     if (issued > maxit + 64) return fail(ctx, FSX_ESTATE, "PCG did not terminate");
-    chunk = 16;
+    chunk = 8;
   }
-  if (ctx->timing) {
-    (void)hipEventRecord(ctx->ev1, ctx->stream);
-    (void)hipEventSynchronize(ctx->ev1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    ctx->tm.pcg_ms += ms;
-    ctx->tm.pcg_spmv_launches += ctx->h_sc->iters;
-    ctx->tm.pcg_iters += ctx->h_sc->iters;
-  }
-  *iters_out = ctx->h_sc->iters;
-  *relres_out = ctx->h_sc->bb > 0 ? std::sqrt(ctx->h_sc->rr / ctx->h_sc->bb) : 0.0;
-  *status_out = ctx->h_sc->done;
-  ctx->last_iters = ctx->h_sc->iters;
-  (void)spmv_launches;
   return FSX_OK;
+}
+
+template <class T>
+static void timing_begin(fsx_ctx* ctx) {
+  if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+}
+static void timing_end(fsx_ctx* ctx, int iters) {
+  if (!ctx->timing) return;
+  (void)hipEventRecord(ctx->ev1, ctx->stream);
+  (void)hipEventSynchronize(ctx->ev1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  ctx->tm.pcg_ms += ms;
+  ctx->tm.pcg_spmv_launches += iters;
+  ctx->tm.pcg_iters += iters;
+}
+
+static int solve_fp64(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  const int n = ctx->NNp;
+  timing_begin<double>(ctx);
+  hipLaunchKernelGGL(k_pcg_init<double>, dim3(row_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, b, ctx->Zdinv, x,
+                     ctx->wr, ctx->wz, ctx->wp, ctx->rtol, ctx->maxit, ctx->partials, ctx->sc);
+  FSX_TRY(launch_check(ctx, "k_pcg_init"));
+  FSX_TRY(pcg_iterate<double>(ctx, ctx->Z, ctx->Zdinv, x, ctx->wr, ctx->wz, ctx->wp, ctx->wq2, ctx->wq, 0, 0.0,
+                              (int)(0.85 * ctx->last_iters), ctx->maxit));
+  const SolveScalars& s = *ctx->h_sc;
+  timing_end(ctx, s.iters);
+  *iters = s.iters;
+  *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  ctx->last_iters = s.iters;
+  if (s.done == 3) return fail(ctx, FSX_ENUMERIC, "PCG met NaN/Inf");
+  if (s.done == 2) { ctx->err = "PCG reached maxit"; return FSX_ENOCONV; }
+  return FSX_OK;
+}
+
+// ------------------------------------------------------------ mixed precision
+// FP32 Jacobi-PCG with reliable updates: the CG recurrence (Z, x, r, z, p, q)
+// runs in FP32; whenever its residual has dropped by delta since the last
+// anchor, or reaches the target, the FP32 iterate is folded into the FP64
+// solution x_hi, the residual is recomputed in FP64 as b - Z x_hi (the
+// refinement residual), the FP64 stop test ||b - Z x_hi|| <= rtol ||b|| is
+// applied, and the FP32 residual is replaced by the rounded FP64 one. The
+// search direction is kept, so the Krylov information is not lost as in a
+// restarted refinement.
+__global__ __launch_bounds__(kBlock) void k_mx_fold(int n, double* __restrict__ xh, float* __restrict__ xl) {
+  long long r0, r1;
+  block_rows(n, xcd_block(), gridDim.x, kBlock, &r0, &r1);
+  for (long long i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    xh[i] += (double)xl[i];
+    xl[i] = 0.0f;
+  }
+}
+
+// rh = b - Z xh (FP64), ||rh||^2 -> final stop test or new anchor
+template <int LPR, int U>
+__global__ __launch_bounds__(kBlock) void k_mx_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                        const double* __restrict__ val, const double* __restrict__ xh,
+                                                        const double* __restrict__ b, double* __restrict__ rh,
+                                                        double* partials, SolveScalars* sc) {
+  const int l = threadIdx.x % LPR;
+  double v[1] = {0.0};
+  for_rows<LPR>(n, rp, [&](long long i, long long r1, int rs, int re) {
+    double s = 0;
+    for (int base = rs + l; base < re; base += U * LPR) {
+      int c[U];
+      double a[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = base + j * LPR, kk = k < re ? k : re - 1;
+        c[j] = ld_stream(ci + kk);
+        a[j] = k < re ? ld_stream(val + kk) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) s += a[j] * xh[c[j]];
+    }
+    s = group_sum<LPR>(s);
+    if (i < r1 && l == 0) {
+      const double ri = b[i] - s;
+      rh[i] = ri;
+      v[0] += ri * ri;
+    }
+  });
+  double out[1];
+  if (grid_sum<1>(v, partials, &sc->cnt[3], out) && threadIdx.x == 0) {
+    sc->rr_hi = out[0];
+    if (!(out[0] == out[0])) sc->done = 3;
+    else if (out[0] <= sc->rtol2bb) sc->done = 1;
+    else { sc->done = 0; sc->rr_anchor = out[0]; }
+  }
+}
+
+// r = float(rh), z = dinv r, rz recomputed
+__global__ __launch_bounds__(kBlock) void k_mx_replace(int n, const double* __restrict__ rh, const float* __restrict__ dinv,
+                                                       float* __restrict__ r, float* __restrict__ z, double* partials,
+                                                       SolveScalars* sc) {
+  long long r0, r1;
+  block_rows(n, xcd_block(), gridDim.x, kBlock, &r0, &r1);
+  double v[1] = {0.0};
+  for (long long i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    const float ri = (float)rh[i];
+    const float zi = dinv[i] * ri;
+    r[i] = ri; z[i] = zi;
+    v[0] += (double)ri * (double)zi;
+  }
+  double out[1];
+  if (grid_sum<1>(v, partials, &sc->cnt[0], out) && threadIdx.x == 0) sc->rz = out[0];
+}
+
+static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  const int n = ctx->NNp;
+  const double delta = 1e-2;
+  timing_begin<float>(ctx);
+  FSX_TRY(launch_fill(ctx, x, 0.0, n));
+  hipLaunchKernelGGL(k_pcg_init<float>, dim3(row_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, b, ctx->Zdinvf, ctx->fx,
+                     ctx->fr, ctx->fz, ctx->fp, ctx->rtol, ctx->maxit, ctx->partials, ctx->sc);
+  FSX_TRY(launch_check(ctx, "k_pcg_init<float>"));
+  int guess = (int)(0.5 * ctx->last_iters);
+  int status = 0, updates = 0;
+  for (;;) {
+    FSX_TRY(pcg_iterate<float>(ctx, ctx->Zf, ctx->Zdinvf, ctx->fx, ctx->fr, ctx->fz, ctx->fp, ctx->fp2, ctx->fq, 1,
+                               delta * delta, guess, ctx->maxit));
+    const int done = ctx->h_sc->done;
+    if (done == 3) return fail(ctx, FSX_ENUMERIC, "mixed PCG met NaN/Inf");
+    // fold the FP32 iterate into x and take the FP64 residual
+    hipLaunchKernelGGL(k_mx_fold, dim3(row_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, x, ctx->fx);
+    hipLaunchKernelGGL((k_mx_residual<kPcgLPR, kPcgU>), dim3(row_grid(n, kBlock / kPcgLPR)), dim3(kBlock), 0, ctx->stream,
+                       n, ctx->Zrow, ctx->Zcol, ctx->Z, x, b, ctx->wr, ctx->partials, ctx->sc);
+    FSX_TRY(launch_check(ctx, "mixed reliable update"));
+    FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+    FSX_HIP(hipStreamSynchronize(ctx->stream));
+    updates++;
+    if (ctx->h_sc->done == 3) return fail(ctx, FSX_ENUMERIC, "mixed residual met NaN/Inf");
+    if (ctx->h_sc->done == 1) break;
+    if (done == 2 || ctx->h_sc->iters >= ctx->maxit) { status = 2; break; }
+    hipLaunchKernelGGL(k_mx_replace, dim3(row_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, ctx->wr, ctx->Zdinvf,
+                       ctx->fr, ctx->fz, ctx->partials, ctx->sc);
+    FSX_TRY(launch_check(ctx, "k_mx_replace"));
+    guess = 8;
+  }
+  const SolveScalars& s = *ctx->h_sc;
+  timing_end(ctx, s.iters);
+  *iters = s.iters;
+  *relres = s.bb > 0 ? std::sqrt(s.rr_hi / s.bb) : 0.0;
+  ctx->last_iters = s.iters;
+  ctx->last_updates = updates;
+  if (status == 2) { ctx->err = "mixed-precision PCG reached maxit"; return FSX_ENOCONV; }
+  return FSX_OK;
+}
+
+int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  int rc = ctx->solver_mode == FSX_SOLVER_MIXED_IR ? solve_mixed(ctx, b, x, iters, relres)
+                                                     : solve_fp64(ctx, b, x, iters, relres);
+  ctx->last_relres = *relres;
+  return rc;
 }
 
 // ------------------------------------------------------------ SpMV timing
 __global__ void k_arm(SolveScalars* sc) {
   sc->done = 0;
+  sc->iters = 0;
   sc->rz = 1.0;
+  sc->beta = 0.0;
 }
 
 template <class T>
@@ -472,115 +681,25 @@ __global__ void k_fill_t(T* __restrict__ p, T v, int n) {
 }
 
 template <class T>
-static int bench_spmv_t(fsx_ctx* ctx, const T* val, T* p, T* q, int reps, double* ms) {
-  constexpr int LPR = 16;
+static int bench_spmv_t(fsx_ctx* ctx, const T* val, T* z, T* pa, T* pb, T* q, int reps, double* ms) {
   const int n = ctx->NNp;
-  hipLaunchKernelGGL(k_fill_t<T>, dim3(1024), dim3(kBlock), 0, ctx->stream, p, (T)1, n);
+  hipLaunchKernelGGL(k_fill_t<T>, dim3(1024), dim3(kBlock), 0, ctx->stream, z, (T)1, n);
+  hipLaunchKernelGGL(k_fill_t<T>, dim3(1024), dim3(kBlock), 0, ctx->stream, pa, (T)0, n);
   hipLaunchKernelGGL(k_arm, dim3(1), dim3(1), 0, ctx->stream, ctx->sc);
-  const int g_rows = row_grid(n, kBlock / LPR);
+  const int g_rows = row_grid(n, kBlock / kPcgLPR);
   for (int w = 0; w < 3; w++)
-    hipLaunchKernelGGL((k_pcg_spmv<T, LPR>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, val, p, q,
-                       ctx->partials, ctx->sc);
+    hipLaunchKernelGGL((k_pcg_a<T, kPcgLPR, kPcgU>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol,
+                       val, z, pa, pb, q, ctx->partials, ctx->sc);
   FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int r = 0; r < reps; r++)
-    hipLaunchKernelGGL((k_pcg_spmv<T, LPR>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, val, p, q,
-                       ctx->partials, ctx->sc);
+    hipLaunchKernelGGL((k_pcg_a<T, kPcgLPR, kPcgU>), dim3(g_rows), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol,
+                       val, z, pa, pb, q, ctx->partials, ctx->sc);
   FSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   FSX_HIP(hipEventSynchronize(ctx->ev1));
-  FSX_TRY(launch_check(ctx, "k_pcg_spmv (timing)"));
+  FSX_TRY(launch_check(ctx, "k_pcg_a (timing)"));
   float t = 0;
   FSX_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
   *ms = t / reps;
-  return FSX_OK;
-}
-
-// ------------------------------------------------------------ mixed precision
-__global__ void k_to_float(int n, const double* __restrict__ a, float* __restrict__ b) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
-}
-__global__ void k_add_float(int n, double* __restrict__ x, const float* __restrict__ d) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += (double)d[i];
-}
-
-// r = b - Z x and ||r||^2 (FP64), the refinement residual.
-template <int LPR>
-__global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                     const double* __restrict__ val, const double* __restrict__ x,
-                                                     const double* __restrict__ b, double* __restrict__ r,
-                                                     double* partials, SolveScalars* sc) {
-  constexpr int RPB = kBlock / LPR;
-  long long r0, r1;
-  block_rows(n, xcd_block(), gridDim.x, RPB, &r0, &r1);
-  const int grp = threadIdx.x / LPR, l = threadIdx.x % LPR;
-  double v[1] = {0.0};
-  for (long long base = r0; base < r1; base += RPB) {
-    const long long i = base + grp;
-    double s = 0;
-    if (i < r1)
-      for (int k = rp[i] + l; k < rp[i + 1]; k += LPR) s += val[k] * x[ci[k]];
-    s = group_sum<LPR>(s);
-    if (i < r1 && l == 0) {
-      const double ri = b[i] - s;
-      r[i] = ri;
-      v[0] += ri * ri;
-    }
-  }
-  double out[1];
-  if (grid_sum<1>(v, partials, &sc->cnt[3], out) && threadIdx.x == 0) sc->rr = out[0];
-}
-
-int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
-  const int n = ctx->NNp;
-  int status = 0;
-  if (ctx->solver_mode == FSX_SOLVER_FP64_JACOBI) {
-    FSX_TRY(pcg_run<double>(ctx, ctx->Z, ctx->Zdinv, b, x, ctx->wr, ctx->wp, ctx->wq, ctx->rtol, ctx->maxit, iters, relres,
-                            &status));
-    if (status == 3) return fail(ctx, FSX_ENUMERIC, "PCG met NaN/Inf");
-    ctx->last_relres = *relres;
-    if (status == 2) { ctx->err = "PCG reached maxit"; return FSX_ENOCONV; }
-    return FSX_OK;
-  }
-  // FSX_SOLVER_MIXED_IR: FP32 inner Jacobi-PCG, FP64 true residual, FP64 correction
-  constexpr int LPR = 16;
-  const double eta = 1e-3;            // inner relative tolerance
-  const int g_vec = std::max(1, std::min((n + kBlock - 1) / kBlock, 4096));
-  FSX_TRY(launch_fill(ctx, x, 0.0, n));
-  FSX_TRY(launch_copy(ctx, ctx->wr, b, n));
-  // ||b||^2 via the residual kernel with x = 0
-  hipLaunchKernelGGL(k_residual<LPR>, dim3(row_grid(n, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol,
-                     ctx->Z, x, b, ctx->wr, ctx->partials, ctx->sc);
-  FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-  FSX_HIP(hipStreamSynchronize(ctx->stream));
-  const double bb = ctx->h_sc->rr;
-  double rr = bb;
-  int total = 0, outer = 0;
-  const int saved_last = ctx->last_iters;
-  while (bb > 0.0 && rr > ctx->rtol * ctx->rtol * bb) {
-    if (total >= ctx->maxit || outer >= 100) { status = 2; break; }
-    hipLaunchKernelGGL(k_to_float, dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, ctx->wr, ctx->fr);
-    // inner solve: Zf d = r  (d in fx); the inner b is the FP64 residual
-    int it = 0, st = 0;
-    double rel = 0;
-    ctx->last_iters = saved_last > 0 ? saved_last / 2 : 0;
-    FSX_TRY(pcg_run<float>(ctx, ctx->Zf, ctx->Zdinvf, ctx->wr, ctx->fx, ctx->fr, ctx->fp, ctx->fq, eta, ctx->maxit - total,
-                           &it, &rel, &st));
-    if (st == 3) return fail(ctx, FSX_ENUMERIC, "inner PCG met NaN/Inf");
-    total += it;
-    hipLaunchKernelGGL(k_add_float, dim3(g_vec), dim3(kBlock), 0, ctx->stream, n, x, ctx->fx);
-    hipLaunchKernelGGL(k_residual<LPR>, dim3(row_grid(n, kBlock / LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow,
-                       ctx->Zcol, ctx->Z, x, b, ctx->wr, ctx->partials, ctx->sc);
-    FSX_TRY(launch_check(ctx, "mixed refinement"));
-    FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-    FSX_HIP(hipStreamSynchronize(ctx->stream));
-    rr = ctx->h_sc->rr;
-    if (!(rr == rr)) return fail(ctx, FSX_ENUMERIC, "refinement residual is NaN");
-    outer++;
-  }
-  *iters = total;
-  *relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-  ctx->last_iters = total;
-  ctx->last_relres = *relres;
-  if (status == 2) { ctx->err = "mixed-precision solve reached maxit"; return FSX_ENOCONV; }
   return FSX_OK;
 }
 
@@ -593,12 +712,14 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
   double ms = 0;
   const double nnz = (double)ctx->nnzZ, n = ctx->NNp;
   double bytes;
+  // algorithmic bytes: values + columns + row pointers, z and p_old read
+  // once, p_new and q written once
   if (precision == 0) {
-    FSX_TRY(bench_spmv_t<double>(ctx, ctx->Z, ctx->wp, ctx->wq, reps, &ms));
-    bytes = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
+    FSX_TRY(bench_spmv_t<double>(ctx, ctx->Z, ctx->wz, ctx->wp, ctx->wq2, ctx->wq, reps, &ms));
+    bytes = 12.0 * nnz + 4.0 * (n + 1) + 32.0 * n;
   } else {
-    FSX_TRY(bench_spmv_t<float>(ctx, ctx->Zf, ctx->fp, ctx->fq, reps, &ms));
-    bytes = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
+    FSX_TRY(bench_spmv_t<float>(ctx, ctx->Zf, ctx->fz, ctx->fp, ctx->fp2, ctx->fq, reps, &ms));
+    bytes = 8.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
   }
   *ms_per_launch = ms;
   if (bytes_per_launch) *bytes_per_launch = bytes;

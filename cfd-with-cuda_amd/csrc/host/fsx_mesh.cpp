@@ -40,7 +40,9 @@ static std::vector<double> axis_nodes(int n_el, double L, double cluster) {
 using namespace fsx;
 
 extern "C" fsxh_mesh* fsxh_mesh_cavity(int N, double cluster, double lid_u) {
-  if (N < 1) return nullptr;
+  // the mirrored sinh clustering of GEN:100-131 needs an even element count
+  // ("Select an even number for proper clustering", GEN:41)
+  if (N < 1 || (cluster != 0.0 && N % 2 != 0)) return nullptr;
   auto* m = new fsxh_mesh();
   const int nx = N, ny = N, nz = N, ncx = N + 1, ncy = N + 1, ncz = N + 1;
   auto dx = axis_nodes(nx, 1.0, cluster), dy = axis_nodes(ny, 1.0, cluster), dz = axis_nodes(nz, 1.0, cluster);

@@ -17,6 +17,8 @@ import numpy as np
 
 def axis(n_el, L, cluster):
     nc = n_el + 1
+    if cluster != 0.0 and n_el % 2:
+        raise ValueError("sinh clustering needs an even element count (GEN:41)")
     if cluster == 0.0:
         return np.array([(i * L) / (nc - 1) for i in range(nc)], dtype=np.float64)
     mx = np.sinh(cluster)
