@@ -243,10 +243,10 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
   for (double** v : {&ctx->Un, &ctx->Unp1, &ctx->Unp1_prev, &ctx->UnpHalf, &ctx->UnpHalf_prev, &ctx->Acc, &ctx->Acc_prev,
                      &ctx->KAcc, &ctx->R1})
     FSX_TRY(alloc(ctx, v, n3));
-  for (double** v : {&ctx->Pn, &ctx->Pnp1, &ctx->Pnp1_prev, &ctx->Pdot, &ctx->R2, &ctx->wr, &ctx->wz, &ctx->wp, &ctx->wq2,
-                     &ctx->wq})
+  for (double** v : {&ctx->Pn, &ctx->Pnp1, &ctx->Pnp1_prev, &ctx->Pdot, &ctx->R2, &ctx->wr, &ctx->wz, &ctx->wp, &ctx->wq,
+                     &ctx->wh})
     FSX_TRY(alloc(ctx, v, (size_t)NNp));
-  for (float** v : {&ctx->fr, &ctx->fz, &ctx->fp, &ctx->fp2, &ctx->fq, &ctx->fx}) FSX_TRY(alloc(ctx, v, (size_t)NNp));
+  for (float** v : {&ctx->fr, &ctx->fz, &ctx->fp, &ctx->fq, &ctx->fx}) FSX_TRY(alloc(ctx, v, (size_t)NNp));
   FSX_TRY(alloc(ctx, &ctx->MdInv, (size_t)NN));
   FSX_TRY(alloc(ctx, &ctx->MdOrigInv, (size_t)NN));
   FSX_TRY(alloc(ctx, &ctx->Md, (size_t)NN));

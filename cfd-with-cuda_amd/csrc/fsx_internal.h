@@ -17,21 +17,24 @@ constexpr int kBlock = 256;       // default block size for row kernels
 constexpr int kMaxGrid = 2048;    // row kernels: 256 CUs x 8 blocks, a multiple of 8 XCDs
 constexpr int kZRowMax = 1024;    // longest Z row handled by the Z-build kernel
 
-// Device-resident scalars of one Krylov solve. Written by the last block of a
-// reduction kernel, read by the following launches; never by the host while
-// kernels run.
+// Device-resident scalars of one Krylov solve, written by block 0 of a kernel
+// and read by later launches. Values that a kernel both reads (all blocks)
+// and updates (block 0) are versioned by the parity of the iteration index j
+// that the host passes in, so no block can see a value of the same launch.
 struct SolveScalars {
-  double rz, pq, rr, bb, alpha, beta, rtol2bb;
+  double rz_v[2];     // r.z entering iteration j sits in rz_v[j & 1]
+  double rr, bb, rtol2bb;
   double rr_anchor;   // mixed solver: FP64 residual^2 at the last reliable update
   double rr_hi;       // mixed solver: latest FP64 residual^2
-  int iters, done, maxit, pad;
-  unsigned int cnt[4];            // last-block counters, reset by the last block
+  int iters;          // completed iterations
+  int stop;           // 0 running, 1 converged, 2 maxit, 3 NaN/Inf, 5 paused for a reliable update
+  int maxit, pad;
+  int active_v[2];    // iteration j ran its update kernel (stop was 0 then)
 };
 
 // Scalars of the time step (norms for the convergence checks, max |dU|/dt).
 struct StepScalars {
   double uu, ud, pp, pd, maxacc;
-  unsigned int cnt[4];
 };
 
 // Shape-function tables on the device, GP-major (k = Gauss point):
@@ -85,9 +88,9 @@ struct fsx_ctx {
   double *Acc = nullptr, *Acc_prev = nullptr, *KAcc = nullptr, *R1 = nullptr;
   double *Pn = nullptr, *Pnp1 = nullptr, *Pnp1_prev = nullptr, *Pdot = nullptr, *R2 = nullptr;
   // solver work (device)
-  double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wq2 = nullptr, *wq = nullptr;   // FP64 PCG (p double-buffered)
-  float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fp2 = nullptr, *fq = nullptr, *fx = nullptr;   // FP32 PCG
-  double* partials = nullptr;    // 8 * kMaxGrid
+  double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wq = nullptr, *wh = nullptr;    // FP64 PCG (+ FP64 residual)
+  float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 PCG
+  double* partials = nullptr;    // 8 * kMaxGrid, per-block partial sums of the reduction kernels
   fsx::SolveScalars* sc = nullptr;
   fsx::StepScalars* st = nullptr;
   fsx::SolveScalars* h_sc = nullptr;   // pinned host mirrors
