@@ -155,7 +155,7 @@ def main():
                                f"dt {args.dt:g}, 3 inner iterations, pressure rtol {args.rtol:g}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": f"k_pcg_spmv<{'float' if prec else 'double'},16>",
+        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<float,32,4>" if prec else "k_pcg_spmv<double,16,8>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": bytes_per, "launch_ms": round(ms, 5)},
