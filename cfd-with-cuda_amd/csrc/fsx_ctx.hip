@@ -154,6 +154,7 @@ void fsx_destroy(fsx_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  fsx::unet_free(ctx);
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
   if (ctx->h_st) (void)hipHostFree(ctx->h_st);
@@ -191,7 +192,7 @@ int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, doub
 int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit) {
   if (!ctx) return FSX_EINVAL;
   if (mode < FSX_SOLVER_FP64_JACOBI || mode > FSX_SOLVER_LEARNED) return fail(ctx, FSX_EINVAL, "unknown solver mode");
-  if (mode == FSX_SOLVER_LEARNED) return fail(ctx, FSX_EINVAL, "learned preconditioner not available in this build");
+  if (mode == FSX_SOLVER_LEARNED && !ctx->unet) return fail(ctx, FSX_ESTATE, "learned mode needs fsx_unet_setup first");
   if (!(rtol >= 0) || maxit < 1) return fail(ctx, FSX_EINVAL, "rtol >= 0 and maxit >= 1 required");
   ctx->solver_mode = mode; ctx->rtol = rtol; ctx->maxit = maxit;
   return FSX_OK;

@@ -108,6 +108,12 @@ struct fsx_ctx {
   int last_iters = 0;
   int last_updates = 0;
   double last_relres = 0.0;
+  // learned preconditioner
+  struct fsx_unet* unet = nullptr;
+  int learned_maxit = 400;       // flexible-CG iterations before the Jacobi fallback
+  int last_learned_iters = 0;
+  int last_fallback = 0;
+  void (*capture_hook)(fsx_ctx*, const double* b, const double* x) = nullptr;
 };
 
 // error helpers (host)
@@ -151,4 +157,7 @@ int launch_fill(fsx_ctx* ctx, double* p, double v, long long n);
 int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n);
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
 int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta);
+int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);
+int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi);
+void unet_free(fsx_ctx* ctx);
 }  // namespace fsx

@@ -700,10 +700,69 @@ static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
   return FSX_OK;
 }
 
+// ------------------------------------------------------------ launchers used by the learned solver
+int solve_mixed_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  return solve_mixed(ctx, b, x, iters, relres);
+}
+int solve_fp64_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  return solve_fp64(ctx, b, x, iters, relres);
+}
+
+int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q) {
+  constexpr int LPR = PcgCfg<float>::LPR, U = PcgCfg<float>::U;
+  hipLaunchKernelGGL((k_pcg_spmv<float, LPR, U>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
+                     ctx->Zrow, ctx->Zcol, ctx->Zf, p, q, ctx->sc);
+  return FSX_OK;
+}
+
+int launch_pcg_dot_f32(fsx_ctx* ctx, const float* p, const float* q) {
+  hipLaunchKernelGGL(k_pcg_dot<float>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, p, q, ctx->partials, ctx->sc);
+  return FSX_OK;
+}
+
+__global__ void k_add_d(int n, double* __restrict__ x, const double* __restrict__ d) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += d[i];
+}
+
+int launch_add(fsx_ctx* ctx, double* x, const double* d, int n) {
+  hipLaunchKernelGGL(k_add_d, dim3(ew_grid(n)), dim3(kBlock), 0, ctx->stream, n, x, d);
+  return launch_check(ctx, "k_add_d");
+}
+
+// wh = b - Z x (FP64), ||wh||^2 and the FP64 stop test of k_mx_check; reads
+// the scalars back. With fold, first x += x_lo (FP32 iterate) and x_lo = 0.
+static int residual_check(fsx_ctx* ctx, double* x, const double* b, bool fold, int* stop, double* rr_hi) {
+  const int n = ctx->NNp;
+  if (fold) hipLaunchKernelGGL(k_mx_fold, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, x, ctx->fx);
+  hipLaunchKernelGGL((k_residual<PcgCfg<double>::LPR, PcgCfg<double>::U>), dim3(rows_grid(n, PcgCfg<double>::LPR)),
+                     dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, ctx->Z, x, b, ctx->wh);
+  hipLaunchKernelGGL(k_sumsq, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->partials);
+  hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc);
+  FSX_TRY(launch_check(ctx, "residual check"));
+  FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  *stop = ctx->h_sc->stop;
+  *rr_hi = ctx->h_sc->rr_hi;
+  return FSX_OK;
+}
+
+int launch_fold_residual(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi) {
+  return residual_check(ctx, x, b, true, stop, rr_hi);
+}
+int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi) {
+  return residual_check(ctx, x, b, false, stop, rr_hi);
+}
+
+int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres, int truncate_at);
+
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
-  int rc = ctx->solver_mode == FSX_SOLVER_MIXED_IR ? solve_mixed(ctx, b, x, iters, relres)
-                                                     : solve_fp64(ctx, b, x, iters, relres);
+  int rc;
+  if (ctx->solver_mode == FSX_SOLVER_LEARNED && ctx->unet) rc = solve_learned(ctx, b, x, iters, relres, 0);
+  else if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)
+    rc = solve_mixed(ctx, b, x, iters, relres);
+  else rc = solve_fp64(ctx, b, x, iters, relres);
   ctx->last_relres = *relres;
+  if (ctx->capture_hook && rc == FSX_OK) ctx->capture_hook(ctx, b, x);
   return rc;
 }
 

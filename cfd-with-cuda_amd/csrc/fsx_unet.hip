@@ -1,0 +1,1278 @@
+// fsx_unet.hip -- learned pressure preconditioner: a symmetric multilevel
+// 3-D convolutional network with its own BF16-MFMA forward, transpose and
+// weight-gradient kernels, Adam training on the GPU, and training pairs
+// produced by the solver itself.
+//
+// Preconditioner z = dinv r + gain s J^T J (r / s), s = rms(r) (so
+// M(cr) = c M(r)). J maps the pressure grid gf to features on three coarser
+// node-aligned grids (G0 = every other node of gf, G1 of G0, G2 of G1;
+// 65^3 -> 33^3 -> 17^3 -> 9^3):
+//   u  = B0 R x            full-weighting restriction gf -> G0      (1 ch)
+//   f0 = C0 u              conv 1 -> 16 at G0                       (VALU)
+//   h1 = B1 C1 f0          conv_s2 16 -> 32, G0 -> G1               (MFMA)
+//   f1 = C2 h1             conv 32 -> 32 at G1
+//   h3 = B2 C3 f1          conv_s2 32 -> 64, G1 -> G2
+//   f2 = C4 h3             conv 64 -> 64 at G2
+// and J^T J x = R^T B0 C0^T (f0 + C1^T B1 C2^T (f1 + C3^T B2 C4^T f2)), the
+// transpose pass running the same convolutions with transposed weights
+// (a U-net whose decoder is tied to its encoder). B_l zeroes the boundary
+// nodes of grid l: the pressure operator penalises boundary values (its
+// mass term), so the slow modes that the coarse levels must carry vanish
+// there. The correction is symmetric positive semidefinite by construction,
+// so M is SPD and the flexible CG keeps descent. With one channel, identity
+// and 1/sqrt(2) x full-weighting weights reproduce the additive multilevel
+// (BPX) preconditioner sum_l 2^-(l-1) P_l P_l^T; the network starts there
+// and is trained from it. Convolutions are 3x3x3, linear, without bias; weights
+// are BF16 for the MFMA operands with FP32 master copies; activations are
+// FP32 and rounded to BF16 where they feed an MFMA; accumulation is FP32.
+//
+// MFMA mapping (v_mfma_f32_16x16x32_bf16, layout checked on gfx950): a conv
+// is an implicit GEMM out[voxel][co] = sum_k A[voxel][k] W[k][co] with
+// k = tap * Cin + ci. A wavefront owns 32 output voxels x Cout; for each
+// 32-wide k chunk every lane gathers 8 contiguous channels (16 bytes) of one
+// neighbour voxel, zero outside the grid.
+#include "fsx_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace fsx {
+
+struct Grid3 {
+  int nx, ny, nz;
+  __host__ __device__ long long n() const { return (long long)nx * ny * nz; }
+};
+
+__device__ __forceinline__ __bf16 to_bf16(float f) {
+  // round to nearest even (finite inputs)
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  unsigned short h = (unsigned short)(u >> 16);
+  return __builtin_bit_cast(__bf16, h);
+}
+__device__ __forceinline__ float from_bf16(__bf16 b) {
+  return __uint_as_float(((unsigned)__builtin_bit_cast(unsigned short, b)) << 16);
+}
+
+// Input coordinate of tap t (t = dx + 3 dy + 9 dz, offsets in {-1,0,1}) for
+// output node (i,j,k). MODE 0: o + d; 1: o - d; 2: 2o + d (stride-2 down);
+// 3: (o - d) / 2 when even (transpose of MODE 2). Returns -1 outside.
+template <int MODE>
+__device__ __forceinline__ long long tap_input(int i, int j, int k, int t, const Grid3& gi) {
+  const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+  int a, b, c;
+  if (MODE == 0) { a = i + dx; b = j + dy; c = k + dz; }
+  else if (MODE == 1) { a = i - dx; b = j - dy; c = k - dz; }
+  else if (MODE == 2) { a = 2 * i + dx; b = 2 * j + dy; c = 2 * k + dz; }
+  else {
+    a = i - dx; b = j - dy; c = k - dz;
+    if ((a | b | c) & 1) return -1;
+    a >>= 1; b >>= 1; c >>= 1;
+  }
+  if (a < 0 || b < 0 || c < 0 || a >= gi.nx || b >= gi.ny || c >= gi.nz) return -1;
+  return ((long long)c * gi.ny + b) * gi.nx + a;
+}
+
+__host__ __device__ __forceinline__ bool on_boundary(long long v, const Grid3& g) {
+  const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  return i == 0 || j == 0 || k == 0 || i == g.nx - 1 || j == g.ny - 1 || k == g.nz - 1;
+}
+
+// ------------------------------------------------------------ MFMA conv
+// out32[v][co] = sum_k A[v][k] W[co][k] (+ res[v][co]); zero at the boundary
+// nodes of go when mask is set; optional BF16 copy. W is [COUT][KP] BF16,
+// KP = 27 * CIN rounded up to 32 (zero padded).
+constexpr int kConvVox = 32;   // output voxels per wavefront
+constexpr int kConvWaves = 4;
+
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(kWave * kConvWaves) void k_conv_mfma(Grid3 gi, Grid3 go, const __bf16* __restrict__ in,
+                                                                   const __bf16* __restrict__ W, float* __restrict__ out32,
+                                                                   __bf16* __restrict__ out16, const float* __restrict__ res,
+                                                                   int mask, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  constexpr int KP = (27 * CIN + 31) / 32 * 32;
+  constexpr int NCT = COUT / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long nout = go.n();
+  const long long vbase = ((long long)blockIdx.x * kConvWaves + wave) * kConvVox;
+  const int grp = lane >> 4, col = lane & 15;
+  int oi[2], oj[2], ok[2];
+  bool ov[2];
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++) {
+    const long long v = vbase + rt * 16 + col;
+    ov[rt] = v < nout;
+    const long long vv = ov[rt] ? v : 0;
+    oi[rt] = (int)(vv % go.nx);
+    oj[rt] = (int)((vv / go.nx) % go.ny);
+    ok[rt] = (int)(vv / ((long long)go.nx * go.ny));
+  }
+  f32x4 acc[2][NCT];
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = {};
+  for (int c0 = 0; c0 < KP; c0 += 32) {
+    const int kk0 = c0 + grp * 8;
+    const int t = kk0 / CIN, ci0 = kk0 % CIN;
+    bf16x8 a[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++) {
+      long long src = -1;
+      if (t < 27 && ov[rt]) src = tap_input<MODE>(oi[rt], oj[rt], ok[rt], t, gi);
+      a[rt] = src >= 0 ? *reinterpret_cast<const bf16x8*>(in + src * CIN + ci0) : zero8;
+    }
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + (size_t)(ct * 16 + col) * KP + kk0);
+#pragma unroll
+      for (int rt = 0; rt < 2; rt++) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], b, acc[rt][ct], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const long long v = vbase + rt * 16 + grp * 4 + r;
+        if (v < nout) {
+          const size_t o = (size_t)v * COUT + ct * 16 + col;
+          float val = acc[rt][ct][r];
+          if (res) val += res[o];
+          if (mask && on_boundary(v, go)) val = 0.0f;
+          out32[o] = val;
+          if (out16) out16[o] = to_bf16(val);
+        }
+      }
+}
+
+// ------------------------------------------------------------ 1-channel layers (VALU)
+// a_in[v][co] = (1/s) sum_t Win[t][co] x[v + d_t]   (Win FP32 values of BF16 weights)
+__global__ __launch_bounds__(kBlock) void k_conv_in(Grid3 g, const float* __restrict__ x, const float* __restrict__ Win,
+                                                    float inv_s_host, const double* __restrict__ rr_partials, long long nrms,
+                                                    float* __restrict__ out32, __bf16* __restrict__ out16,
+                                                    const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  __shared__ float w[27 * 16];   // [t][co]
+  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i % 16) * 32 + i / 16];
+  float inv_s = inv_s_host;
+  if (rr_partials) {   // s = rms of the residual (nrms entries) from the solver's r.r partials
+    double rr[1];
+    read_partials<1>(rr_partials, rr);
+    inv_s = rr[0] > 0.0 ? (float)(1.0 / std::sqrt(rr[0] / (double)nrms)) : 0.0f;
+  }
+  __syncthreads();
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.n()) return;
+  const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  float acc[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) acc[c] = 0.f;
+  for (int t = 0; t < 27; t++) {
+    const long long s = tap_input<0>(i, j, k, t, g);
+    if (s < 0) continue;
+    const float xv = x[s] * inv_s;
+#pragma unroll
+    for (int c = 0; c < 16; c++) acc[c] += w[t * 16 + c] * xv;
+  }
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    out32[v * 16 + c] = acc[c];
+    if (out16) out16[v * 16 + c] = to_bf16(acc[c]);
+  }
+}
+
+// transpose of k_conv_in, scaled and masked:
+// v[o] = gain * s * sum_t sum_c Win[c][t] g[o - d_t][c], 0 at the boundary of g
+__global__ __launch_bounds__(kBlock) void k_conv_in_T(Grid3 g, const float* __restrict__ a, const float* __restrict__ Win,
+                                                      float s_host, const double* __restrict__ rr_partials, long long nrms,
+                                                      float gain, float* __restrict__ y, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  __shared__ float w[27 * 16];   // [t][c]
+  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i % 16) * 32 + i / 16];
+  float s = s_host;
+  if (rr_partials) {
+    double rr[1];
+    read_partials<1>(rr_partials, rr);
+    s = (float)std::sqrt(rr[0] / (double)nrms);
+  }
+  __syncthreads();
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.n()) return;
+  if (on_boundary(v, g)) { y[v] = 0.0f; return; }
+  const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  float acc = 0.f;
+  for (int t = 0; t < 27; t++) {
+    const long long src = tap_input<1>(i, j, k, t, g);
+    if (src < 0) continue;
+    const float4* ap = reinterpret_cast<const float4*>(a + src * 16);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float4 av = ap[q];
+      acc += w[t * 16 + 4 * q] * av.x + w[t * 16 + 4 * q + 1] * av.y + w[t * 16 + 4 * q + 2] * av.z + w[t * 16 + 4 * q + 3] * av.w;
+    }
+  }
+  y[v] = gain * s * acc;
+}
+
+// ------------------------------------------------------------ up-sampling
+// fine[f][c] = beta*fine + skip[f][c] + sum of node-aligned trilinear weights * coarse
+template <int C>
+__global__ __launch_bounds__(kBlock) void k_upsample(Grid3 gc, Grid3 gf, const float* __restrict__ coarse,
+                                                     const float* __restrict__ skip, float* __restrict__ out32,
+                                                     __bf16* __restrict__ out16, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= gf.n() * C) return;
+  const int c = (int)(idx % C);
+  const long long f = idx / C;
+  const int i = (int)(f % gf.nx), j = (int)((f / gf.nx) % gf.ny), k = (int)(f / ((long long)gf.nx * gf.ny));
+  const int i0 = i >> 1, i1 = (i + 1) >> 1, j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
+  const float wi = (i & 1) ? 0.5f : 1.0f, wj = (j & 1) ? 0.5f : 1.0f, wk = (k & 1) ? 0.5f : 1.0f;
+  auto at = [&](int a, int b, int cc) { return coarse[(((long long)cc * gc.ny + b) * gc.nx + a) * C + c]; };
+  float v = at(i0, j0, k0);
+  if (i & 1) v += at(i1, j0, k0);
+  float s = wi * v;
+  if (j & 1) { float v2 = at(i0, j1, k0); if (i & 1) v2 += at(i1, j1, k0); s += wi * v2; }
+  s *= wj;
+  if (k & 1) {
+    float v3 = at(i0, j0, k1); if (i & 1) v3 += at(i1, j0, k1);
+    float s3 = wi * v3;
+    if (j & 1) { float v4 = at(i0, j1, k1); if (i & 1) v4 += at(i1, j1, k1); s3 += wi * v4; }
+    s += wj * s3;
+  }
+  s *= wk;
+  const float val = s + (skip ? skip[idx] : 0.0f);
+  out32[idx] = val;
+  if (out16) out16[idx] = to_bf16(val);
+}
+
+// adjoint of the up-sampling (restriction): coarse[u][c] = sum over fine
+// neighbours f = 2u + e, e in {-1,0,1}^3, of w(e) fine[f][c], w = prod (1 or 1/2)
+template <int C>
+__global__ __launch_bounds__(kBlock) void k_upsample_adj(Grid3 gf, Grid3 gc, const float* __restrict__ fine,
+                                                         float* __restrict__ out32, __bf16* __restrict__ out16, int mask,
+                                                         const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= gc.n() * C) return;
+  const int c = (int)(idx % C);
+  const long long u = idx / C;
+  if (mask && on_boundary(u, gc)) {
+    out32[idx] = 0.0f;
+    if (out16) out16[idx] = to_bf16(0.0f);
+    return;
+  }
+  const int i = (int)(u % gc.nx), j = (int)((u / gc.nx) % gc.ny), k = (int)(u / ((long long)gc.nx * gc.ny));
+  float s = 0.f;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int fz = 2 * k + dz;
+    if (fz < 0 || fz >= gf.nz) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int fy = 2 * j + dy;
+      if (fy < 0 || fy >= gf.ny) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        const int fx = 2 * i + dx;
+        if (fx < 0 || fx >= gf.nx) continue;
+        const float w = (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f);
+        s += w * fine[(((long long)fz * gf.ny + fy) * gf.nx + fx) * C + c];
+      }
+    }
+  }
+  out32[idx] = s;
+  if (out16) out16[idx] = to_bf16(s);
+}
+
+// ------------------------------------------------------------ weight gradients
+// part[chunk][co][t*CIN + ci] = sum over the chunk's output voxels v of
+//   in[tap_input(v, t)][ci] * gout[v][co]
+// (FP32 activations and gradients). grid = (kWChunks, 27).
+constexpr int kWChunks = 64;
+constexpr int kWBatch = 32;
+
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_wgrad(Grid3 gi, Grid3 go, const float* __restrict__ in,
+                                                  const float* __restrict__ gout, float* __restrict__ part) {
+  constexpr int P = CIN * COUT;
+  constexpr int PPT = (P + kBlock - 1) / kBlock;
+  __shared__ float sa[kWBatch][CIN];
+  __shared__ float sg[kWBatch][COUT];
+  const int t = blockIdx.y;
+  const long long nout = go.n();
+  const long long per = (nout + kWChunks - 1) / kWChunks;
+  const long long v0 = blockIdx.x * per, v1 = std::min(nout, v0 + per);
+  float acc[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; q++) acc[q] = 0.f;
+  for (long long vb = v0; vb < v1; vb += kWBatch) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kWBatch * CIN; e += kBlock) {
+      const int b = e / CIN, ci = e % CIN;
+      const long long v = vb + b;
+      float val = 0.f;
+      if (v < v1) {
+        const int i = (int)(v % go.nx), j = (int)((v / go.nx) % go.ny), k = (int)(v / ((long long)go.nx * go.ny));
+        const long long src = tap_input<MODE>(i, j, k, t, gi);
+        if (src >= 0) val = in[src * CIN + ci];
+      }
+      sa[b][ci] = val;
+    }
+    for (int e = threadIdx.x; e < kWBatch * COUT; e += kBlock) {
+      const int b = e / COUT, co = e % COUT;
+      const long long v = vb + b;
+      sg[b][co] = v < v1 ? gout[v * COUT + co] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+      const int pidx = threadIdx.x + q * kBlock;
+      if (pidx < P) {
+        const int co = pidx / CIN, ci = pidx % CIN;
+        float s = 0.f;
+#pragma unroll 8
+        for (int b = 0; b < kWBatch; b++) s += sa[b][ci] * sg[b][co];
+        acc[q] += s;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    const int pidx = threadIdx.x + q * kBlock;
+    if (pidx < P) {
+      const int co = pidx / CIN, ci = pidx % CIN;
+      part[((size_t)blockIdx.x * COUT + co) * (27 * CIN) + t * CIN + ci] = acc[q];
+    }
+  }
+}
+
+// grad[co][k] (+)= sum_chunk part[chunk][co][k] in chunk order; grad has row stride KP
+__global__ void k_wgrad_reduce(int cout, int k27, int kp, const float* __restrict__ part, float* __restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= cout * k27) return;
+  const int co = e / k27, k = e % k27;
+  float s = 0.f;
+  for (int c = 0; c < kWChunks; c++) s += part[((size_t)c * cout + co) * k27 + k];
+  grad[(size_t)co * kp + k] += s;
+}
+
+// ------------------------------------------------------------ optimizer, weights
+// Adam on the FP32 master weights; refreshes the BF16 copy.
+__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, __bf16* __restrict__ w16, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi; v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (std::sqrt(vi / c2) + eps);
+  w[i] = wi;
+  if (w16) w16[i] = to_bf16(wi);
+}
+
+// BF16 copy and the transposed copy used by backward data:
+//   WT[ci][t*COUT + co] = W[co][t*CIN + ci]
+__global__ void k_weights_refresh(int cin, int cout, int kp, int kpt, const float* __restrict__ w32,
+                                  __bf16* __restrict__ w16, __bf16* __restrict__ wt16, float* __restrict__ wr) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= cout * kp) return;
+  const int co = e / kp, k = e % kp;
+  const float v = k < 27 * cin ? w32[e] : 0.f;
+  const __bf16 b = to_bf16(v);
+  w16[e] = b;
+  if (wr) wr[e] = from_bf16(b);   // rounded FP32 values for the VALU layers
+  if (k < 27 * cin && wt16) {
+    const int t = k / cin, ci = k % cin;
+    wt16[(size_t)ci * kpt + t * cout + co] = b;
+  }
+}
+
+__global__ void k_scale_f(long long n, float* __restrict__ a, float s) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) a[i] *= s;
+}
+
+// relative squared error of one sample and its gradient:
+// L = sum (y - t)^2 / sum t^2 ; gy = 2 (y - t) / sum t^2   (tt = sum t^2 precomputed)
+__global__ __launch_bounds__(kBlock) void k_loss(int n, const float* __restrict__ y, const float* __restrict__ tgt, double tt,
+                                                 float* __restrict__ gy, double* __restrict__ partials) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const float d = y[i] - tgt[i];
+    gy[i] = (float)(2.0 * d / tt);
+    v[0] += (double)d * d;
+  }
+  write_partials<1>(v, partials);
+}
+
+// energy-norm loss of one sample: d = y - t (in gy on entry), q = Z d;
+// L = d.q / tt, gy = 2 q / tt (gradient w.r.t. the network output y)
+__global__ __launch_bounds__(kBlock) void k_eloss(int n, const float* __restrict__ q, double tt, float* __restrict__ gy,
+                                                  double* __restrict__ partials) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const float d = gy[i];
+    v[0] += (double)d * (double)q[i];
+    gy[i] = (float)(2.0 * q[i] / tt);
+  }
+  write_partials<1>(v, partials);
+}
+
+__global__ void k_sub_f(int n, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ c) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = a[i] - b[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_f(int n, const float* __restrict__ a, const float* __restrict__ b,
+                                                  double* __restrict__ partials) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += (double)a[i] * (double)b[i];
+  write_partials<1>(v, partials);
+}
+
+// y = Z x on the FP32 copy of Z; 32 lanes per row (Z rows hold up to 125
+// entries on hex meshes)
+__global__ __launch_bounds__(kBlock) void k_spmv_f(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const float* __restrict__ val, const float* __restrict__ x,
+                                                   float* __restrict__ y) {
+  constexpr int LPR = 32;
+  const int l = threadIdx.x % LPR;
+  const long long r = (long long)blockIdx.x * (kBlock / LPR) + threadIdx.x / LPR;
+  float s = 0.f;
+  if (r < n)
+    for (int k = rp[r] + l; k < rp[r + 1]; k += LPR) s += val[k] * x[ci[k]];
+  s = group_sumf<LPR>(s);
+  if (r < n && l == 0) y[r] = s;
+}
+
+}  // namespace fsx
+
+// ====================================================================== host side
+using namespace fsx;
+
+namespace {
+
+struct Layer {
+  int cin, cout, mode;      // mode 0 stride 1, 2 stride-2 down
+  Grid3 gi, go;
+  int kp, kpt;              // row strides of W [cout][kp] and WT [cin][kpt]
+  size_t off;               // offset of this layer's weights in the flat FP32 parameter array
+  __bf16* w16 = nullptr;
+  __bf16* wt16 = nullptr;
+  float* wr = nullptr;      // BF16-rounded FP32 copy (VALU layers)
+};
+
+// Activations of one application of J^T J: encoder values (u, f0, h1, f1,
+// h3, f2) and transpose-pass values (g3 = B2 C4^T f2, g2 = f1 + C3^T g3,
+// g1 = B1 C2^T g2, g0 = f0 + C1^T g1, v = B0 C0^T g0 scaled). The transpose
+// values are the gradients of <J x, G> with respect to the encoder outputs
+// for G = (f0, f1, f2), which the weight gradient uses.
+struct Acts {
+  float *u = nullptr, *f0 = nullptr, *h1 = nullptr, *f1 = nullptr, *h3 = nullptr, *f2 = nullptr;
+  float *g3 = nullptr, *g2 = nullptr, *g1 = nullptr, *g0 = nullptr, *v = nullptr;
+  __bf16 *f0b = nullptr, *h1b = nullptr, *f1b = nullptr, *h3b = nullptr, *f2b = nullptr, *g3b = nullptr, *g2b = nullptr,
+         *g1b = nullptr;
+};
+
+}  // namespace
+
+struct fsx_unet {
+  Grid3 gf{};               // pressure grid
+  Grid3 g0{}, g1{}, g2{};   // coarse levels
+  std::vector<Layer> L;     // C0 .. C4
+  size_t nparam = 0;        // padded parameter count (rows of stride kp)
+  float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  float *x = nullptr, *y = nullptr, *tgt = nullptr, *gy = nullptr, *zq = nullptr;   // pressure grid
+  Acts A, B;                // A: the preconditioner / the sample; B: the loss gradient (training)
+  float* wpart = nullptr;
+  // training set (normalised pairs, FP32)
+  int ncap = 0, cap_max = 0;
+  double *cap_b = nullptr, *cap_x = nullptr;     // captured (R2, Pdot) pairs
+  int nsamp = 0, samp_max = 0;
+  float *s_in = nullptr, *s_tgt = nullptr;
+  std::vector<double> s_tt;   // per-pair loss normaliser (error energy)
+  int adam_t = 0;
+  float gain = 1.0f;        // fixed output gain of the network
+  double* tmp_x = nullptr;  // truncated iterate while preparing samples
+};
+
+namespace {
+
+Grid3 coarse_of(Grid3 g) { return Grid3{(g.nx + 1) / 2, (g.ny + 1) / 2, (g.nz + 1) / 2}; }
+
+unsigned blocks_for(long long n, int per) { return (unsigned)std::max(1LL, (n + per - 1) / per); }
+
+template <int CIN, int COUT, int MODE>
+void conv_launch(fsx_ctx* ctx, Grid3 gi, Grid3 go, const __bf16* in, const __bf16* W, float* out32, __bf16* out16,
+                 const float* res, int mask, const SolveScalars* sc) {
+  const long long per = (long long)kConvVox * kConvWaves;
+  hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, MODE>), dim3(blocks_for(go.n(), (int)per)), dim3(kWave * kConvWaves), 0,
+                     ctx->stream, gi, go, in, W, out32, out16, res, mask, sc);
+}
+
+// dispatch on (cin, cout, mode): the encoder convolutions (modes 0, 2) and
+// their transposes (mode 1 for stride 1, 3 for stride 2, with WT)
+int conv(fsx_ctx* ctx, int cin, int cout, int mode, Grid3 gi, Grid3 go, const __bf16* in, const __bf16* W, float* out32,
+         __bf16* out16, const float* res, int mask, const SolveScalars* sc) {
+#define CONV_CASE(A, B, M) \
+  if (cin == A && cout == B && mode == M) { conv_launch<A, B, M>(ctx, gi, go, in, W, out32, out16, res, mask, sc); return FSX_OK; }
+  CONV_CASE(16, 32, 2) CONV_CASE(32, 32, 0) CONV_CASE(32, 64, 2) CONV_CASE(64, 64, 0)
+  CONV_CASE(64, 64, 1) CONV_CASE(64, 32, 3) CONV_CASE(32, 32, 1) CONV_CASE(32, 16, 3)
+#undef CONV_CASE
+  return fail(ctx, FSX_EINVAL, "conv shape not instantiated");
+}
+
+int wgrad(fsx_ctx* ctx, int cin, int cout, int mode, Grid3 gi, Grid3 go, const float* in, const float* gout, float* part) {
+#define WG_CASE(A, B, M) \
+  if (cin == A && cout == B && mode == M) { \
+    hipLaunchKernelGGL((k_wgrad<A, B, M>), dim3(kWChunks, 27), dim3(kBlock), 0, ctx->stream, gi, go, in, gout, part); \
+    return FSX_OK; }
+  WG_CASE(1, 16, 0) WG_CASE(16, 32, 2) WG_CASE(32, 32, 0) WG_CASE(32, 64, 2) WG_CASE(64, 64, 0)
+#undef WG_CASE
+  return fail(ctx, FSX_EINVAL, "wgrad shape not instantiated");
+}
+
+int refresh_weights(fsx_ctx* ctx, fsx_unet* u) {
+  for (Layer& l : u->L) {
+    const int n = l.cout * l.kp;
+    hipLaunchKernelGGL(k_weights_refresh, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, l.cin, l.cout, l.kp, l.kpt,
+                       u->w + l.off, l.w16, l.wt16, l.wr);
+  }
+  return launch_check(ctx, "k_weights_refresh");
+}
+
+template <class T>
+int ualloc(fsx_ctx* ctx, T** p, size_t n) { return alloc(ctx, p, n); }
+
+// One application on activation set S. x: FP32 on gf; y = gain s P v on gf
+// (skipped when y is null: training only needs the transpose-pass values).
+// s = rms(x) from the r.r partials when given, else inv_s / s as passed.
+int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, float inv_s, float s, const double* rr_partials,
+              const SolveScalars* sc) {
+  const Layer* L = u->L.data();
+  const long long nf = u->gf.n(), n0 = u->g0.n();
+  hipLaunchKernelGGL(k_upsample_adj<1>, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->gf, u->g0, x, S.u,
+                     nullptr, 1, sc);
+  hipLaunchKernelGGL(k_conv_in, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.u, L[0].wr, inv_s,
+                     rr_partials, nf, S.f0, S.f0b, sc);
+  FSX_TRY(conv(ctx, 16, 32, 2, u->g0, u->g1, S.f0b, L[1].w16, S.h1, S.h1b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 32, 32, 0, u->g1, u->g1, S.h1b, L[2].w16, S.f1, S.f1b, nullptr, 0, sc));
+  FSX_TRY(conv(ctx, 32, 64, 2, u->g1, u->g2, S.f1b, L[3].w16, S.h3, S.h3b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 64, 64, 0, u->g2, u->g2, S.h3b, L[4].w16, S.f2, S.f2b, nullptr, 0, sc));
+  // transpose pass
+  FSX_TRY(conv(ctx, 64, 64, 1, u->g2, u->g2, S.f2b, L[4].wt16, S.g3, S.g3b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 64, 32, 3, u->g2, u->g1, S.g3b, L[3].wt16, S.g2, S.g2b, S.f1, 0, sc));
+  FSX_TRY(conv(ctx, 32, 32, 1, u->g1, u->g1, S.g2b, L[2].wt16, S.g1, S.g1b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 0, sc));
+  if (y) {
+    hipLaunchKernelGGL(k_conv_in_T, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.g0, L[0].wr, s,
+                       rr_partials, nf, u->gain, S.v, sc);
+    hipLaunchKernelGGL(k_upsample<1>, dim3(blocks_for(nf, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, u->gf, S.v, nullptr,
+                       y, nullptr, sc);
+  }
+  return launch_check(ctx, "network application");
+}
+
+int forward(fsx_ctx* ctx, fsx_unet* u, const float* x, float* y, float inv_s, float s, const double* rr_partials,
+            const SolveScalars* sc) {
+  return apply_net(ctx, u, u->A, x, y, inv_s, s, rr_partials, sc);
+}
+
+// Parameter gradient of <gy, y> after forward() on set A (accumulates into
+// u->g): with y = gain J^T J x, <gy, y> = gain <J gy, J x>, whose gradient is
+// the encoder backpropagation at input x with output gradients J (gain gy)
+// plus the one at input gain gy with output gradients J x. Both sets of
+// output-side gradients are transpose-pass values: set A holds those of x,
+// set B (the network applied to gain gy) those of gain gy.
+int backward(fsx_ctx* ctx, fsx_unet* u) {
+  hipLaunchKernelGGL(k_scale_f, dim3(256), dim3(kBlock), 0, ctx->stream, (long long)u->gf.n(), u->gy, u->gain);
+  FSX_TRY(apply_net(ctx, u, u->B, u->gy, nullptr, 1.0f, 1.0f, nullptr, nullptr));
+  const Layer* L = u->L.data();
+  auto wg = [&](int li, const float* inA, const float* gB, const float* inB, const float* gA) -> int {
+    const Layer& l = L[li];
+    const int n = l.cout * 27 * l.cin;
+    FSX_TRY(wgrad(ctx, l.cin, l.cout, l.mode, l.gi, l.go, inA, gB, u->wpart));
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, l.cout, 27 * l.cin, l.kp,
+                       u->wpart, u->g + l.off);
+    FSX_TRY(wgrad(ctx, l.cin, l.cout, l.mode, l.gi, l.go, inB, gA, u->wpart));
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, l.cout, 27 * l.cin, l.kp,
+                       u->wpart, u->g + l.off);
+    return launch_check(ctx, "wgrad");
+  };
+  Acts &A = u->A, &B = u->B;
+  FSX_TRY(wg(4, A.h3, B.f2, B.h3, A.f2));
+  FSX_TRY(wg(3, A.f1, B.g3, B.f1, A.g3));
+  FSX_TRY(wg(2, A.h1, B.g2, B.h1, A.g2));
+  FSX_TRY(wg(1, A.f0, B.g1, B.f0, A.g1));
+  FSX_TRY(wg(0, A.u, B.g0, B.u, A.g0));
+  return FSX_OK;
+}
+
+int alloc_acts(fsx_ctx* ctx, fsx_unet* u, Acts& S) {
+  const size_t n0 = u->g0.n(), n1 = u->g1.n(), n2 = u->g2.n();
+  FSX_TRY(ualloc(ctx, &S.u, n0)); FSX_TRY(ualloc(ctx, &S.v, n0));
+  FSX_TRY(ualloc(ctx, &S.f0, n0 * 16)); FSX_TRY(ualloc(ctx, &S.f0b, n0 * 16)); FSX_TRY(ualloc(ctx, &S.g0, n0 * 16));
+  FSX_TRY(ualloc(ctx, &S.h1, n1 * 32)); FSX_TRY(ualloc(ctx, &S.h1b, n1 * 32));
+  FSX_TRY(ualloc(ctx, &S.f1, n1 * 32)); FSX_TRY(ualloc(ctx, &S.f1b, n1 * 32));
+  FSX_TRY(ualloc(ctx, &S.g2, n1 * 32)); FSX_TRY(ualloc(ctx, &S.g2b, n1 * 32));
+  FSX_TRY(ualloc(ctx, &S.g1, n1 * 32)); FSX_TRY(ualloc(ctx, &S.g1b, n1 * 32));
+  FSX_TRY(ualloc(ctx, &S.h3, n2 * 64)); FSX_TRY(ualloc(ctx, &S.h3b, n2 * 64));
+  FSX_TRY(ualloc(ctx, &S.f2, n2 * 64)); FSX_TRY(ualloc(ctx, &S.f2b, n2 * 64));
+  FSX_TRY(ualloc(ctx, &S.g3, n2 * 64)); FSX_TRY(ualloc(ctx, &S.g3b, n2 * 64));
+  return FSX_OK;
+}
+
+// Initial weights: channel 0 of every layer carries the additive multilevel
+// preconditioner (C0, C2, C4: centre tap 1; C1, C3: full weighting scaled by
+// 1/sqrt(2)), so J^T J = sum_l 2^-(l-1) P_l P_l^T over the three coarse
+// levels; the other entries are small seeded noise (1/10 of a fan-in-scaled
+// normal) so that training can use them; gain = median(dinv) / 4.
+void init_weights(const std::vector<Layer>& L, size_t nparam, unsigned long long seed, std::vector<float>& w) {
+  w.assign(nparam, 0.0f);
+  std::mt19937_64 rng(seed);
+  const float a = (float)std::sqrt(0.5);
+  for (size_t li = 0; li < L.size(); li++) {
+    const Layer& l = L[li];
+    std::normal_distribution<float> nd(0.0f, 0.1f * (float)std::sqrt(1.0 / (27.0 * l.cin)));
+    for (int co = 0; co < l.cout; co++)
+      for (int k = 0; k < 27 * l.cin; k++) w[l.off + (size_t)co * l.kp + k] = nd(rng);
+    for (int t = 0; t < 27; t++) {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+      float v;
+      if (l.mode == 2) v = a * (dx ? 0.5f : 1.0f) * (dy ? 0.5f : 1.0f) * (dz ? 0.5f : 1.0f);
+      else v = (t == 13) ? 1.0f : 0.0f;
+      w[l.off + (size_t)t * l.cin] = v;   // co = 0, ci = 0
+    }
+  }
+}
+
+int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fsx_unet** out) {
+  if ((nx - 1) % 8 || (ny - 1) % 8 || (nz - 1) % 8 || nx < 9 || ny < 9 || nz < 9)
+    return fail(ctx, FSX_EINVAL, "pressure grid needs 8m+1 nodes (m >= 1) per axis for three node-aligned coarsenings");
+  if ((long long)nx * ny * nz != ctx->NNp) return fail(ctx, FSX_EINVAL, "U-net grid size != NNp");
+  auto u = std::make_unique<fsx_unet>();
+  u->gf = Grid3{nx, ny, nz};
+  u->g0 = coarse_of(u->gf);
+  u->g1 = coarse_of(u->g0);
+  u->g2 = coarse_of(u->g1);
+  struct Spec { int cin, cout, mode; Grid3 gi, go; };
+  const Spec spec[5] = {{1, 16, 0, u->g0, u->g0}, {16, 32, 2, u->g0, u->g1}, {32, 32, 0, u->g1, u->g1},
+                        {32, 64, 2, u->g1, u->g2}, {64, 64, 0, u->g2, u->g2}};
+  size_t off = 0;
+  for (const Spec& sp : spec) {
+    Layer l;
+    l.cin = sp.cin; l.cout = sp.cout; l.mode = sp.mode; l.gi = sp.gi; l.go = sp.go;
+    l.kp = (27 * sp.cin + 31) / 32 * 32;
+    l.kpt = (27 * sp.cout + 31) / 32 * 32;
+    l.off = off;
+    off += (size_t)l.cout * l.kp;
+    u->L.push_back(l);
+  }
+  u->nparam = off;
+  FSX_TRY(ualloc(ctx, &u->w, off)); FSX_TRY(ualloc(ctx, &u->g, off));
+  FSX_TRY(ualloc(ctx, &u->m, off)); FSX_TRY(ualloc(ctx, &u->v, off));
+  for (Layer& l : u->L) {
+    FSX_TRY(ualloc(ctx, &l.w16, (size_t)l.cout * l.kp));
+    FSX_TRY(ualloc(ctx, &l.wt16, (size_t)l.cin * l.kpt + 8));
+    FSX_TRY(ualloc(ctx, &l.wr, (size_t)l.cout * l.kp));
+    FSX_HIP(hipMemsetAsync(l.wt16, 0, ((size_t)l.cin * l.kpt + 8) * sizeof(__bf16), ctx->stream));
+  }
+  std::vector<float> w;
+  init_weights(u->L, off, seed, w);
+  FSX_HIP(hipMemcpy(u->w, w.data(), off * sizeof(float), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemsetAsync(u->m, 0, off * sizeof(float), ctx->stream));
+  FSX_HIP(hipMemsetAsync(u->v, 0, off * sizeof(float), ctx->stream));
+  FSX_HIP(hipMemsetAsync(u->g, 0, off * sizeof(float), ctx->stream));
+  // gain: a quarter of the median Jacobi scaling
+  {
+    std::vector<double> dinv(ctx->NNp);
+    FSX_HIP(hipMemcpy(dinv.data(), ctx->Zdinv, dinv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::nth_element(dinv.begin(), dinv.begin() + dinv.size() / 2, dinv.end());
+    u->gain = (float)(0.25 * dinv[dinv.size() / 2]);
+  }
+  const size_t nf = u->gf.n();
+  for (float** p : {&u->x, &u->y, &u->tgt, &u->gy, &u->zq}) FSX_TRY(ualloc(ctx, p, nf));
+  FSX_TRY(alloc_acts(ctx, u.get(), u->A));
+  FSX_TRY(alloc_acts(ctx, u.get(), u->B));
+  FSX_TRY(ualloc(ctx, &u->wpart, (size_t)kWChunks * 64 * 27 * 64));
+  FSX_TRY(refresh_weights(ctx, u.get()));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  *out = u.release();
+  return FSX_OK;
+}
+
+}  // namespace
+
+// ====================================================================== learned solve
+// Flexible CG in the FCG(1) form (Notay 2000): alpha = r.p / p.Zp, the new
+// direction Z-orthogonalised against the previous one, beta = -z.Zp / p.Zp.
+// Both stay exact when the preconditioner varies from one application to the
+// next (BF16 rounding, a scale taken from the residual) and when a reliable
+// update replaces r. Partial-sum slots (kRedGrid doubles each):
+//   0 p.q, 1 r.p (k_fpcg_dot); 2 r.r (k_fpcg_update, k_fpcg_load_r);
+//   3 r.z, 4 z.q (k_fpcg_z).
+namespace fsx {
+
+constexpr int kSlotRR = 2;
+
+__global__ __launch_bounds__(kBlock) void k_fpcg_dot(int n, const float* __restrict__ r, const float* __restrict__ p,
+                                                     const float* __restrict__ q, double* __restrict__ partials,
+                                                     const SolveScalars* __restrict__ sc) {
+  if (sc->stop) return;
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    v[0] += (double)p[i] * (double)q[i];
+    v[1] += (double)r[i] * (double)p[i];
+  }
+  write_partials<2>(v, partials);
+}
+
+// alpha = r.p / p.q (0 when p is not a descent direction, which
+// k_fpcg_direction then reports); x += alpha p, r -= alpha q; r.r partials
+__global__ __launch_bounds__(kBlock) void k_fpcg_update(int j, int n, float* __restrict__ x, float* __restrict__ r,
+                                                        const float* __restrict__ p, const float* __restrict__ q,
+                                                        double* __restrict__ partials, SolveScalars* sc) {
+  const int stop = sc->stop;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->active_v[j & 1] = stop ? 0 : 1;
+  if (stop) return;
+  double s[2];
+  read_partials<2>(partials, s);
+  const float a = (s[0] > 0.0 && s[1] > 0.0) ? (float)(s[1] / s[0]) : 0.0f;
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    x[i] += a * p[i];
+    const float ri = r[i] - a * q[i];
+    r[i] = ri;
+    v[0] += (double)ri * (double)ri;
+  }
+  write_partials<1>(v, partials + kSlotRR * kRedGrid);
+}
+
+// z = dinv r + y (Jacobi plus the network's correction); r.z and z.q partials
+__global__ __launch_bounds__(kBlock) void k_fpcg_z(int n, const float* __restrict__ r, const float* __restrict__ dinv,
+                                                   const float* __restrict__ y, const float* __restrict__ q,
+                                                   float* __restrict__ z, double* __restrict__ partials,
+                                                   const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;   // sc is null on a start, which always runs
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const float zi = dinv[i] * r[i] + y[i];
+    z[i] = zi;
+    v[0] += (double)r[i] * (double)zi;
+    v[1] += (double)zi * (double)q[i];
+  }
+  write_partials<2>(v, partials + 3 * kRedGrid);
+}
+
+// p = z - (z.q / p.q) p; stop tests (block 0): 3 NaN/Inf, 6 loss of descent
+// (p.Zp, r.p or r.z not positive), 1 converged (FP32 residual; the FP64
+// check decides), 5 pause for a reliable update, 2 iteration cap.
+__global__ __launch_bounds__(kBlock) void k_fpcg_direction(int j, int n, const float* __restrict__ z, float* __restrict__ p,
+                                                           const double* __restrict__ partials, double delta2,
+                                                           SolveScalars* sc) {
+  if (!sc->active_v[j & 1]) return;
+  double s[5];
+  read_partials<5>(partials, s);
+  const double pq = s[0], rp = s[1], rr = s[2], rz = s[3], zq = s[4];
+  const bool bad = !(pq == pq) || !(rp == rp) || !(rr == rr) || !(rz == rz) || !(zq == zq);
+  const bool lost = !(pq > 0.0) || !(rp > 0.0) || !(rz > 0.0);
+  const float be = (bad || lost) ? 0.f : (float)(-zq / pq);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) p[i] = z[i] + be * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int it = sc->iters + 1;
+    sc->iters = it;
+    sc->rr = rr;
+    if (bad) sc->stop = 3;
+    else if (lost) sc->stop = 6;
+    else if (rr <= sc->rtol2bb) sc->stop = 1;
+    else if (rr <= delta2 * sc->rr_anchor) sc->stop = 5;
+    else if (it >= sc->maxit) sc->stop = 2;
+  }
+}
+
+__global__ void k_copy_ff(int n, const float* __restrict__ a, float* __restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+// r = float(rh), x_lo = 0, r.r partials
+__global__ __launch_bounds__(kBlock) void k_fpcg_load_r(int n, const double* __restrict__ rh, float* __restrict__ r,
+                                                        float* __restrict__ x, double* __restrict__ partials) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const float ri = (float)rh[i];
+    r[i] = ri;
+    x[i] = 0.0f;
+    v[0] += (double)ri * (double)ri;
+  }
+  write_partials<1>(v, partials + kSlotRR * kRedGrid);
+}
+
+__global__ void k_zero_f(int n, float* __restrict__ a) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sumsq_u(int n, const double* __restrict__ a, double* __restrict__ partials) {
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += a[i] * a[i];
+  write_partials<1>(v, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_learned_init(const double* __restrict__ partials, SolveScalars* sc, double rtol,
+                                                         int maxit) {
+  double bb[1];
+  read_partials<1>(partials, bb);
+  if (threadIdx.x == 0) {
+    sc->bb = bb[0]; sc->rr = bb[0]; sc->rr_anchor = bb[0]; sc->rr_hi = bb[0];
+    sc->rtol2bb = rtol * rtol * bb[0];
+    sc->iters = 0; sc->maxit = maxit;
+    sc->stop = bb[0] == 0.0 ? 1 : 0;
+    sc->active_v[0] = 0; sc->active_v[1] = 0;
+  }
+}
+
+// training sample from (b, x*, x_k): r = b - Z x_k (in rh, |r|^2 partials in
+// slot 0), s = rms(r); in = r / s; t = (x* - x_k - dinv r) / s; slot 1 gets
+// the error energy (x* - x_k).r / s^2, the loss normaliser of the sample
+__global__ __launch_bounds__(kBlock) void k_make_sample(int n, const double* __restrict__ rh, const double* __restrict__ xs,
+                                                        const double* __restrict__ xk, const double* __restrict__ dinv,
+                                                        double* __restrict__ partials, float* __restrict__ in,
+                                                        float* __restrict__ tgt) {
+  double rr[1];
+  read_partials<1>(partials, rr);
+  const double inv_s = rr[0] > 0.0 ? 1.0 / std::sqrt(rr[0] / n) : 0.0;
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const double r = rh[i], e = xs[i] - xk[i];
+    in[i] = (float)(r * inv_s);
+    tgt[i] = (float)((e - dinv[i] * r) * inv_s);
+    v[0] += e * r * inv_s * inv_s;
+  }
+  write_partials<1>(v, partials + kRedGrid);
+}
+
+}  // namespace fsx
+
+namespace {
+
+// z = dinv r + U(r) for the current FP32 residual r (its r.r in slot 2)
+int precondition(fsx_ctx* ctx, fsx_unet* u, const SolveScalars* sc) {
+  FSX_TRY(forward(ctx, u, ctx->fr, u->y, 0.f, 0.f, ctx->partials + kSlotRR * kRedGrid, sc));
+  hipLaunchKernelGGL(k_fpcg_z, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->fr, ctx->Zdinvf, u->y, ctx->fq,
+                     ctx->fz, ctx->partials, sc);
+  return launch_check(ctx, "precondition");
+}
+
+}  // namespace
+
+namespace fsx {
+
+int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q);
+int launch_fold_residual(fsx_ctx* ctx, double* x, const double* b, int* stop_out, double* rr_hi_out);
+int solve_mixed_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
+
+// Flexible PCG preconditioned by Jacobi plus the U-net correction, in FP32
+// with FP64 reliable updates exactly like the mixed solver (b - Z x in FP64
+// decides convergence). If the flexible recurrence loses descent, meets the
+// iteration cap or NaN, the remaining FP64 residual is handed to the mixed
+// Jacobi-PCG, which then also decides the final residual. truncate_at > 0
+// instead stops after that many flexible iterations (or at a loss of
+// descent) and returns the iterate without the fallback (sample generation).
+int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres, int truncate_at) {
+  fsx_unet* u = ctx->unet;
+  const int n = ctx->NNp;
+  const double delta = 1e-2;
+  const int max_learned = truncate_at > 0 ? truncate_at : std::min(ctx->maxit, ctx->learned_maxit);
+  FSX_TRY(launch_fill(ctx, x, 0.0, n));
+  FSX_TRY(launch_copy(ctx, ctx->wh, b, n));
+  hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->partials);
+  hipLaunchKernelGGL(k_learned_init, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, max_learned);
+  FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->h_sc->stop == 1) { *iters = 0; *relres = 0.0; return FSX_OK; }   // b = 0
+  // start: r = b, z = M r, p = z
+  hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx, ctx->partials);
+  hipLaunchKernelGGL(k_zero_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fq);
+  FSX_TRY(precondition(ctx, u, nullptr));
+  hipLaunchKernelGGL(k_copy_ff, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fz, ctx->fp);
+  FSX_TRY(launch_check(ctx, "fpcg start"));
+  int j = 0;
+  int guess = std::max(8, std::min(max_learned, (int)(0.9 * ctx->last_learned_iters)));
+  int fallback = 0;
+  for (;;) {
+    int issued = 0;
+    for (;;) {
+      for (int c = 0; c < guess; c++, j++) {
+        FSX_TRY(launch_pcg_spmv_f32(ctx, ctx->fp, ctx->fq));
+        hipLaunchKernelGGL(k_fpcg_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->fr, ctx->fp, ctx->fq,
+                           ctx->partials, ctx->sc);
+        hipLaunchKernelGGL(k_fpcg_update, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, ctx->fx, ctx->fr, ctx->fp,
+                           ctx->fq, ctx->partials, ctx->sc);
+        FSX_TRY(precondition(ctx, u, ctx->sc));
+        hipLaunchKernelGGL(k_fpcg_direction, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, ctx->fz, ctx->fp,
+                           ctx->partials, delta * delta, ctx->sc);
+      }
+      issued += guess;
+      FSX_TRY(launch_check(ctx, "fpcg iteration"));
+      FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
+      FSX_HIP(hipStreamSynchronize(ctx->stream));
+      if (ctx->h_sc->stop) break;
+      if (issued > max_learned + 64) return fail(ctx, FSX_ESTATE, "FPCG did not terminate");
+      guess = 8;
+    }
+    const int stop = ctx->h_sc->stop;
+    j = ctx->h_sc->iters;
+    // reliable update: fold x_lo into x, FP64 residual wh = b - Z x, FP64 stop test
+    int st = 0;
+    double rr_hi = 0;
+    FSX_TRY(launch_fold_residual(ctx, x, b, &st, &rr_hi));
+    if (st == 3 || stop == 3) return fail(ctx, FSX_ENUMERIC, "learned solve met NaN/Inf");
+    if (st == 1) break;                                     // converged in FP64
+    if (stop == 6 || stop == 2 || st == 2) { fallback = stop == 6 ? 6 : 2; break; }
+    // replace r by the FP64 residual and keep p: the next alpha uses r.p
+    hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx,
+                       ctx->partials);
+    guess = 8;
+  }
+  const int learned_iters = ctx->h_sc->iters;
+  const double bb = ctx->h_sc->bb, rh = ctx->h_sc->rr_hi;
+  if (truncate_at > 0) {
+    *iters = learned_iters;
+    *relres = bb > 0 ? std::sqrt(rh / bb) : 0.0;
+    return FSX_OK;
+  }
+  ctx->last_learned_iters = learned_iters;
+  ctx->last_fallback = fallback;
+  if (fallback) {
+    // Jacobi-PCG (mixed) on the remaining FP64 residual wh: x += Z^-1 wh,
+    // to the same absolute threshold rtol ||b||
+    int it2 = 0;
+    double rel2 = 0;
+    const double saved_rtol = ctx->rtol;
+    ctx->rtol = rh > 0 ? std::min(1.0, saved_rtol * std::sqrt(bb / rh)) : saved_rtol;
+    FSX_TRY(launch_copy(ctx, ctx->wr, ctx->wh, n));        // the mixed solver overwrites wh
+    const int rc = solve_mixed_entry(ctx, ctx->wr, ctx->wz, &it2, &rel2);
+    ctx->rtol = saved_rtol;
+    if (rc != FSX_OK && rc != FSX_ENOCONV) return rc;
+    FSX_TRY(launch_add(ctx, x, ctx->wz, n));
+    int st = 0;
+    double rr2 = 0;
+    FSX_TRY(launch_residual_only(ctx, x, b, &st, &rr2));
+    *iters = learned_iters + it2;
+    *relres = bb > 0 ? std::sqrt(rr2 / bb) : 0.0;
+    ctx->last_iters = *iters;
+    if (st != 1) { ctx->err = "learned solve and Jacobi fallback reached maxit"; return FSX_ENOCONV; }
+    return FSX_OK;
+  }
+  *iters = learned_iters;
+  *relres = bb > 0 ? std::sqrt(rh / bb) : 0.0;
+  ctx->last_iters = learned_iters;
+  return FSX_OK;
+}
+
+int solve_fp64_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
+
+void unet_free(fsx_ctx* ctx) {
+  delete ctx->unet;   // device buffers belong to the context's allocation list
+  ctx->unet = nullptr;
+  ctx->capture_hook = nullptr;
+}
+
+
+}  // namespace fsx
+
+namespace {
+
+// stores (R2, Pdot) of every converged pressure solve while capacity lasts
+void capture_pair(fsx_ctx* ctx, const double* b, const double* x) {
+  fsx_unet* u = ctx->unet;
+  if (!u || u->ncap >= u->cap_max) return;
+  const size_t n = ctx->NNp;
+  (void)hipMemcpyAsync(u->cap_b + (size_t)u->ncap * n, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+  (void)hipMemcpyAsync(u->cap_x + (size_t)u->ncap * n, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+  u->ncap++;
+}
+
+int sum_partials_host(fsx_ctx* ctx, double* out, int slot = 0) {
+  double h[kRedGrid];
+  FSX_HIP(hipMemcpyAsync(h, ctx->partials + (size_t)slot * kRedGrid, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  double s = 0;
+  for (int i = 0; i < kRedGrid; i++) s += h[i];
+  *out = s;
+  return FSX_OK;
+}
+
+int need_unet(fsx_ctx* ctx) {
+  if (!ctx) return FSX_EINVAL;
+  if (!ctx->unet) return fail(ctx, FSX_ESTATE, "call fsx_unet_setup first");
+  return FSX_OK;
+}
+
+void spmv_f(fsx_ctx* ctx, const float* x, float* y) {
+  hipLaunchKernelGGL(k_spmv_f, dim3(blocks_for(ctx->NNp, kBlock / 32)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow,
+                     ctx->Zcol, ctx->Zf, x, y);
+}
+
+// One training sample: forward, loss, output gradient, backward (accumulates
+// u->g). kind 0: relative L2 loss |y - t|^2 / tt with tt = |t|^2; kind 1:
+// energy-norm loss (y - t)' Z (y - t) / tt with tt = t' Z t.
+int train_sample(fsx_ctx* ctx, fsx_unet* u, int kind, const float* x, const float* t, double tt, double* loss) {
+  const int n = ctx->NNp;
+  FSX_TRY(forward(ctx, u, x, u->y, 1.0f, 1.0f, nullptr, nullptr));
+  if (kind == 0) {
+    hipLaunchKernelGGL(k_loss, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, u->y, t, tt, u->gy, ctx->partials);
+  } else {
+    hipLaunchKernelGGL(k_sub_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, u->y, t, u->gy);
+    spmv_f(ctx, u->gy, u->zq);
+    hipLaunchKernelGGL(k_eloss, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, u->zq, tt, u->gy, ctx->partials);
+  }
+  FSX_TRY(launch_check(ctx, "unet loss"));
+  FSX_TRY(backward(ctx, u));
+  if (loss) {
+    double d2 = 0;
+    FSX_TRY(sum_partials_host(ctx, &d2));
+    *loss = d2 / tt;
+  }
+  return FSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsx_unet_setup(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed) {
+  if (!ctx) return FSX_EINVAL;
+  if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z before setting up the U-net");
+  if (ctx->unet) return fail(ctx, FSX_ESTATE, "U-net already set up");
+  fsx_unet* u = nullptr;
+  FSX_TRY(setup_unet(ctx, nx, ny, nz, seed, &u));
+  ctx->unet = u;
+  return FSX_OK;
+}
+
+int fsx_unet_params(fsx_ctx* ctx, long long* n) {
+  FSX_TRY(need_unet(ctx));
+  *n = (long long)ctx->unet->nparam;
+  return FSX_OK;
+}
+
+int fsx_unet_get_weights(fsx_ctx* ctx, float* host, size_t n) {
+  FSX_TRY(need_unet(ctx));
+  if (!host || n != ctx->unet->nparam) return fail(ctx, FSX_EINVAL, "weight count mismatch");
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_HIP(hipMemcpy(host, ctx->unet->w, n * sizeof(float), hipMemcpyDeviceToHost));
+  return FSX_OK;
+}
+
+int fsx_unet_set_weights(fsx_ctx* ctx, const float* host, size_t n) {
+  FSX_TRY(need_unet(ctx));
+  if (!host || n != ctx->unet->nparam) return fail(ctx, FSX_EINVAL, "weight count mismatch");
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_HIP(hipMemcpy(ctx->unet->w, host, n * sizeof(float), hipMemcpyHostToDevice));
+  FSX_TRY(refresh_weights(ctx, ctx->unet));
+  return fsx_sync(ctx);
+}
+
+int fsx_unet_gain(fsx_ctx* ctx, float set, float* get) {
+  FSX_TRY(need_unet(ctx));
+  if (set > 0.0f) ctx->unet->gain = set;
+  if (get) *get = ctx->unet->gain;
+  return FSX_OK;
+}
+
+int fsx_unet_apply(fsx_ctx* ctx, const float* x, float* y) {
+  FSX_TRY(need_unet(ctx));
+  fsx_unet* u = ctx->unet;
+  const size_t n = ctx->NNp;
+  FSX_HIP(hipMemcpyAsync(u->x, x, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  FSX_TRY(forward(ctx, u, u->x, u->y, 1.0f, 1.0f, nullptr, nullptr));
+  FSX_HIP(hipMemcpyAsync(y, u->y, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  return FSX_OK;
+}
+
+int fsx_unet_loss_grad(fsx_ctx* ctx, int kind, const float* x, const float* t, double* loss, float* grad) {
+  FSX_TRY(need_unet(ctx));
+  if (kind != 0 && kind != 1) return fail(ctx, FSX_EINVAL, "loss kind must be 0 (L2) or 1 (energy)");
+  fsx_unet* u = ctx->unet;
+  const size_t n = ctx->NNp;
+  FSX_HIP(hipMemcpyAsync(u->x, x, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  FSX_HIP(hipMemcpyAsync(u->tgt, t, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  double tt = 0;
+  if (kind == 0) {
+    for (size_t i = 0; i < n; i++) tt += (double)t[i] * t[i];
+  } else {
+    spmv_f(ctx, u->tgt, u->zq);
+    hipLaunchKernelGGL(k_dot_f, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, (int)n, u->tgt, u->zq, ctx->partials);
+    FSX_TRY(sum_partials_host(ctx, &tt));
+  }
+  if (!(tt > 0)) return fail(ctx, FSX_EINVAL, "target has no positive norm");
+  FSX_HIP(hipMemsetAsync(u->g, 0, u->nparam * sizeof(float), ctx->stream));
+  double l = 0;
+  FSX_TRY(train_sample(ctx, u, kind, u->x, u->tgt, tt, &l));
+  if (loss) *loss = l;
+  if (grad) FSX_HIP(hipMemcpy(grad, u->g, u->nparam * sizeof(float), hipMemcpyDeviceToHost));
+  return FSX_OK;
+}
+
+int fsx_unet_capture(fsx_ctx* ctx, int capacity) {
+  FSX_TRY(need_unet(ctx));
+  fsx_unet* u = ctx->unet;
+  if (capacity <= 0) { ctx->capture_hook = nullptr; return FSX_OK; }
+  if (capacity > u->cap_max) {
+    if (u->cap_b) { dev_free(ctx, u->cap_b); dev_free(ctx, u->cap_x); }
+    FSX_TRY(alloc(ctx, &u->cap_b, (size_t)capacity * ctx->NNp));
+    FSX_TRY(alloc(ctx, &u->cap_x, (size_t)capacity * ctx->NNp));
+    u->cap_max = capacity;
+    u->ncap = 0;
+  }
+  ctx->capture_hook = capture_pair;
+  return FSX_OK;
+}
+
+int fsx_unet_captured(fsx_ctx* ctx, int* n) {
+  FSX_TRY(need_unet(ctx));
+  *n = ctx->unet->ncap;
+  return FSX_OK;
+}
+
+// Training pairs from the captured solves: for each (b, x*) and each k in
+// ks, x_k = the k-th iterate from x = 0 of FP64 Jacobi-PCG, or (flag 1) of
+// the learned flexible PCG itself, so that the network also sees the
+// residuals it leaves; r = b - Z x_k; input r / rms(r), target
+// (x* - x_k - dinv r) / rms(r), loss normaliser (x* - x_k).r / rms(r)^2 (the
+// error energy). Pairs whose residual is below 1e-5 ||b|| are skipped (x*
+// itself is only that accurate). Flag 2 appends to the existing pairs,
+// otherwise they are replaced.
+int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsamples) {
+  FSX_TRY(need_unet(ctx));
+  fsx_unet* u = ctx->unet;
+  if (!ks || nk <= 0) return fail(ctx, FSX_EINVAL, "no truncation depths");
+  if (u->ncap == 0) return fail(ctx, FSX_ESTATE, "no captured solves");
+  const bool use_learned = flags & 1, append = (flags & 2) && u->nsamp > 0;
+  const int n = ctx->NNp;
+  const int start = append ? u->nsamp : 0;
+  const int total = start + u->ncap * nk;
+  if (total > u->samp_max) {
+    float *ni = nullptr, *nt = nullptr;
+    FSX_TRY(alloc(ctx, &ni, (size_t)total * n));
+    FSX_TRY(alloc(ctx, &nt, (size_t)total * n));
+    if (start > 0) {
+      FSX_HIP(hipMemcpyAsync(ni, u->s_in, (size_t)start * n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+      FSX_HIP(hipMemcpyAsync(nt, u->s_tgt, (size_t)start * n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    FSX_HIP(hipStreamSynchronize(ctx->stream));
+    if (u->s_in) { dev_free(ctx, u->s_in); dev_free(ctx, u->s_tgt); }
+    u->s_in = ni;
+    u->s_tgt = nt;
+    u->samp_max = total;
+  }
+  u->s_tt.resize(total, 0.0);
+  if (!u->tmp_x) FSX_TRY(alloc(ctx, &u->tmp_x, (size_t)n));
+  const double saved_rtol = ctx->rtol;
+  const int saved_maxit = ctx->maxit, saved_last = ctx->last_iters;
+  const int saved_li = ctx->last_learned_iters, saved_fb = ctx->last_fallback;
+  auto hook = ctx->capture_hook;
+  ctx->capture_hook = nullptr;
+  auto restore = [&]() {
+    ctx->rtol = saved_rtol; ctx->maxit = saved_maxit; ctx->last_iters = saved_last;
+    ctx->last_learned_iters = saved_li; ctx->last_fallback = saved_fb; ctx->capture_hook = hook;
+  };
+  int s = start;
+  for (int p = 0; p < u->ncap; p++) {
+    const double* b = u->cap_b + (size_t)p * n;
+    const double* xs = u->cap_x + (size_t)p * n;
+    double bb = 0;
+    hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->partials);
+    FSX_TRY(sum_partials_host(ctx, &bb));
+    if (!(bb > 0)) continue;
+    for (int q = 0; q < nk; q++) {
+      if (ks[q] <= 0) {
+        FSX_TRY(launch_fill(ctx, u->tmp_x, 0.0, n));
+      } else {
+        ctx->rtol = 0.0;
+        ctx->maxit = ks[q];
+        ctx->last_iters = ks[q];
+        int it = 0;
+        double rel = 0;
+        const int rc = use_learned ? solve_learned(ctx, b, u->tmp_x, &it, &rel, ks[q])
+                                   : solve_fp64_entry(ctx, b, u->tmp_x, &it, &rel);
+        if (rc != FSX_OK && rc != FSX_ENOCONV) { restore(); return rc; }
+      }
+      int st = 0;
+      double rr = 0;
+      FSX_TRY(launch_residual_only(ctx, u->tmp_x, b, &st, &rr));
+      // an iterate closer than 1e-5 ||b|| to the solution is compared with x*
+      // that is itself only good to the solver's tolerance: no information
+      if (!(rr >= 1e-10 * bb)) continue;
+      hipLaunchKernelGGL(k_make_sample, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, xs, u->tmp_x, ctx->Zdinv,
+                         ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
+      FSX_TRY(launch_check(ctx, "k_make_sample"));
+      FSX_TRY(sum_partials_host(ctx, &u->s_tt[s], 1));
+      if (!(u->s_tt[s] > 0)) { restore(); return fail(ctx, FSX_ENUMERIC, "a training sample has no error energy"); }
+      s++;
+    }
+  }
+  restore();
+  if (s == 0) return fail(ctx, FSX_ESTATE, "no informative training pairs (all truncated iterates converged)");
+  u->nsamp = s;
+  if (nsamples) *nsamples = s;
+  return fsx_sync(ctx);
+}
+
+// Adam on mini-batches drawn (seeded) from the prepared samples. Returns the
+// mean relative loss of the last min(50, steps) steps.
+int fsx_unet_train(fsx_ctx* ctx, int steps, int batch, float lr, unsigned long long seed, double* loss_out) {
+  FSX_TRY(need_unet(ctx));
+  fsx_unet* u = ctx->unet;
+  if (u->nsamp == 0) return fail(ctx, FSX_ESTATE, "no prepared samples");
+  if (steps < 1 || batch < 1 || !(lr > 0)) return fail(ctx, FSX_EINVAL, "steps, batch, lr");
+  const int n = ctx->NNp;
+  std::mt19937_64 rng(seed);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double tail = 0;
+  int tail_n = 0;
+  for (int step = 0; step < steps; step++) {
+    FSX_HIP(hipMemsetAsync(u->g, 0, u->nparam * sizeof(float), ctx->stream));
+    double lsum = 0;
+    for (int b = 0; b < batch; b++) {
+      const int idx = (int)(rng() % (unsigned long long)u->nsamp);
+      double l = 0;
+      FSX_TRY(train_sample(ctx, u, 1, u->s_in + (size_t)idx * n, u->s_tgt + (size_t)idx * n, u->s_tt[idx], &l));
+      lsum += l;
+    }
+    u->adam_t++;
+    const float c1 = 1.0f - std::pow(b1, (float)u->adam_t), c2 = 1.0f - std::pow(b2, (float)u->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for((long long)u->nparam, 256)), dim3(256), 0, ctx->stream, (int)u->nparam, u->w,
+                       u->g, u->m, u->v, nullptr, lr, b1, b2, eps, c1, c2);
+    FSX_TRY(refresh_weights(ctx, u));
+    if (step >= steps - 50) { tail += lsum / batch; tail_n++; }
+  }
+  if (loss_out) *loss_out = tail_n ? tail / tail_n : 0.0;
+  return fsx_sync(ctx);
+}
+
+int fsx_unet_limit(fsx_ctx* ctx, int learned_maxit) {
+  FSX_TRY(need_unet(ctx));
+  if (learned_maxit < 1) return fail(ctx, FSX_EINVAL, "learned_maxit >= 1 required");
+  ctx->learned_maxit = learned_maxit;
+  return FSX_OK;
+}
+
+int fsx_unet_stats(fsx_ctx* ctx, int* last_learned_iters, int* last_fallback) {
+  if (!ctx) return FSX_EINVAL;
+  if (last_learned_iters) *last_learned_iters = ctx->last_learned_iters;
+  if (last_fallback) *last_fallback = ctx->last_fallback;
+  return FSX_OK;
+}
+
+}  // extern "C"
+

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libfsx.so")
 
 _D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
@@ -32,6 +33,9 @@ DEVICE_SYMBOLS = (
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
     "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
+    "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
+    "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
 )
 HOST_SYMBOLS = (
     "fsxh_mesh_cavity", "fsxh_mesh_duct", "fsxh_mesh_read_inp", "fsxh_mesh_write_inp", "fsxh_mesh_free",
@@ -102,6 +106,19 @@ def lib():
         "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_timing_reset": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_timing_get": ([_vp, ctypes.POINTER(Timing)], ctypes.c_int),
+        "fsx_unet_setup": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64], ctypes.c_int),
+        "fsx_unet_params": ([_vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "fsx_unet_get_weights": ([_vp, _F, ctypes.c_size_t], ctypes.c_int),
+        "fsx_unet_set_weights": ([_vp, _F, ctypes.c_size_t], ctypes.c_int),
+        "fsx_unet_gain": ([_vp, ctypes.c_float, _F], ctypes.c_int),
+        "fsx_unet_apply": ([_vp, _F, _F], ctypes.c_int),
+        "fsx_unet_loss_grad": ([_vp, ctypes.c_int, _F, _F, _D, _F], ctypes.c_int),
+        "fsx_unet_capture": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_unet_captured": ([_vp, _I], ctypes.c_int),
+        "fsx_unet_prepare": ([_vp, _I, ctypes.c_int, ctypes.c_int, _I], ctypes.c_int),
+        "fsx_unet_train": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _D], ctypes.c_int),
+        "fsx_unet_limit": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_unet_stats": ([_vp, _I, _I], ctypes.c_int),
         "fsxh_mesh_cavity": ([ctypes.c_int, ctypes.c_double, ctypes.c_double], _vp),
         "fsxh_mesh_duct": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                             ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int], _vp),
@@ -135,6 +152,11 @@ def lib():
         f.restype = res
     _lib = L
     return L
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
 
 
 def _dp(a):
@@ -439,3 +461,71 @@ class Solver:
 
     def sync(self):
         self.check(self.L.fsx_sync(self.ctx))
+
+    # learned preconditioner (fsx_unet_* of include/fsx.h)
+    def unet_setup(self, grid, seed=0):
+        nx, ny, nz = grid
+        self.check(self.L.fsx_unet_setup(self.ctx, nx, ny, nz, seed), "unet_setup")
+        self.unet_grid = (nz, ny, nx)
+
+    def unet_nparams(self):
+        n = ctypes.c_longlong()
+        self.check(self.L.fsx_unet_params(self.ctx, ctypes.byref(n)), "unet_params")
+        return n.value
+
+    def unet_weights(self):
+        w = np.empty(self.unet_nparams(), dtype=np.float32)
+        self.check(self.L.fsx_unet_get_weights(self.ctx, _fp(w), w.size), "unet_get_weights")
+        return w
+
+    def unet_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self.check(self.L.fsx_unet_set_weights(self.ctx, _fp(w), w.size), "unet_set_weights")
+
+    def unet_gain(self, set_to=0.0):
+        g = ctypes.c_float()
+        self.check(self.L.fsx_unet_gain(self.ctx, set_to, ctypes.byref(g)), "unet_gain")
+        return g.value
+
+    def unet_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty_like(x)
+        self.check(self.L.fsx_unet_apply(self.ctx, _fp(x), _fp(y)), "unet_apply")
+        return y
+
+    def unet_loss_grad(self, x, t, kind=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.unet_nparams(), dtype=np.float32)
+        loss = ctypes.c_double()
+        self.check(self.L.fsx_unet_loss_grad(self.ctx, kind, _fp(x), _fp(t), ctypes.byref(loss), _fp(g)), "unet_loss_grad")
+        return loss.value, g
+
+    def unet_capture(self, capacity):
+        self.check(self.L.fsx_unet_capture(self.ctx, capacity), "unet_capture")
+
+    def unet_captured(self):
+        n = ctypes.c_int()
+        self.check(self.L.fsx_unet_captured(self.ctx, ctypes.byref(n)), "unet_captured")
+        return n.value
+
+    def unet_prepare(self, ks, learned=False, append=False):
+        k = np.ascontiguousarray(ks, dtype=np.int32)
+        n = ctypes.c_int()
+        flags = (1 if learned else 0) | (2 if append else 0)
+        self.check(self.L.fsx_unet_prepare(self.ctx, _ip(k), k.size, flags, ctypes.byref(n)), "unet_prepare")
+        return n.value
+
+    def unet_train(self, steps, batch=4, lr=1e-3, seed=0):
+        loss = ctypes.c_double()
+        self.check(self.L.fsx_unet_train(self.ctx, steps, batch, lr, seed, ctypes.byref(loss)), "unet_train")
+        return loss.value
+
+    def unet_limit(self, learned_maxit):
+        self.check(self.L.fsx_unet_limit(self.ctx, learned_maxit), "unet_limit")
+
+    def unet_stats(self):
+        it = ctypes.c_int()
+        fb = ctypes.c_int()
+        self.check(self.L.fsx_unet_stats(self.ctx, ctypes.byref(it), ctypes.byref(fb)), "unet_stats")
+        return it.value, fb.value

@@ -188,6 +188,51 @@ typedef struct fsx_timing {
 int fsx_timing_reset(fsx_ctx* ctx, int enable);
 int fsx_timing_get(fsx_ctx* ctx, fsx_timing* out);
 
+/* ---------------------------------------------------------------- learned preconditioner
+ * No counterpart in the reference (its only preconditioner is Jacobi,
+ * CU:418-451, BCH:3981-4055). A 3-level linear 3-D conv U-net (channels
+ * 16/32/64, 3x3x3 kernels, BF16 weights, FP32 accumulation) acting on the
+ * pressure grid; FSX_SOLVER_LEARNED runs flexible PCG with
+ * z = dinv r + U(r), falling back to Jacobi-PCG when the flexible recurrence
+ * stalls. The pressure nodes must be numbered as a structured nx*ny*nz grid,
+ * x fastest (the corner numbering of the structured generators), with
+ * nx, ny, nz = 4m+1. */
+int fsx_unet_setup(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed);
+/* number of FP32 parameters: layers in order in, d1, l1a, d2, l2a, l2b, l1b,
+ * l1c, out; layer weights [cout][kp] with kp = 27*cin rounded up to 32 and
+ * k = tap*cin + ci, tap = dx+1 + 3(dy+1) + 9(dz+1); padding entries are 0 */
+int fsx_unet_params(fsx_ctx* ctx, long long* n);
+int fsx_unet_get_weights(fsx_ctx* ctx, float* host, size_t n);
+int fsx_unet_set_weights(fsx_ctx* ctx, const float* host, size_t n);
+/* output gain g of y = g * s * U(x / s): set if set > 0, returned in *get */
+int fsx_unet_gain(fsx_ctx* ctx, float set, float* get);
+/* y = U(x) (s = 1) for host arrays of NNp floats */
+int fsx_unet_apply(fsx_ctx* ctx, const float* x, float* y);
+/* loss of y = gain * U(x) against t and its parameter gradient (nparam
+ * floats): kind 0 relative L2 |y - t|^2 / |t|^2, kind 1 relative energy norm
+ * (y - t)' Z (y - t) / t' Z t on the FP32 Z (the training loss) */
+int fsx_unet_loss_grad(fsx_ctx* ctx, int kind, const float* x, const float* t, double* loss, float* grad);
+/* keep the right-hand sides and solutions of the next `capacity` converged
+ * pressure solves (0 stops capturing) */
+int fsx_unet_capture(fsx_ctx* ctx, int capacity);
+int fsx_unet_captured(fsx_ctx* ctx, int* n);
+/* training pairs from the captured solves, one per truncation depth ks[q]:
+ * the residual of the ks[q]-th iterate from x = 0 of FP64 Jacobi-PCG (flags
+ * bit 0 clear) or of the learned flexible PCG (bit 0 set), with the error
+ * that remains as target; bit 1 appends to the pairs already held, otherwise
+ * they are replaced and the output gain is set */
+int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsamples);
+/* Adam on the energy-norm loss (y - t)' Z (y - t) normalised per pair by its
+ * error energy, seeded mini-batches of prepared pairs; returns the mean loss
+ * of the last 50 steps (the fraction of error energy the preconditioner
+ * leaves) */
+int fsx_unet_train(fsx_ctx* ctx, int steps, int batch, float lr, unsigned long long seed, double* loss);
+/* flexible-PCG iterations allowed before the Jacobi-PCG fallback (default 400) */
+int fsx_unet_limit(fsx_ctx* ctx, int learned_maxit);
+/* flexible-PCG iterations of the last learned solve and why it fell back to
+ * Jacobi-PCG: 0 it did not, 2 iteration cap, 6 loss of descent */
+int fsx_unet_stats(fsx_ctx* ctx, int* last_learned_iters, int* last_fallback);
+
 #ifdef __cplusplus
 }
 #endif

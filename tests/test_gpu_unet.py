@@ -1,0 +1,174 @@
+"""GPU checks of the learned pressure preconditioner (fsx_unet_* C-ABI).
+
+The U-net kernels are compared with the numpy oracle (oracle/unet_ref.py) in
+its device arithmetic model (BF16 weights and MFMA operands, FP32
+elsewhere). The oracle sums in float64 while the device sums in FP32 in
+another order, and a BF16 rounding of an intermediate value can land on the
+other side of a tie, so the bars are normwise relative errors:
+  * forward output: 2e-3;
+  * loss: 1e-3; parameter gradient, per layer: 1e-2.
+Scaling the input by 2 commutes with every rounding step, so that check is
+bit-exact. The learned solves must meet the same FP64 stop test as the
+Jacobi solver (||b - Z x|| <= 1e-8 ||b||, checked here with the FP64 Z).
+"""
+import numpy as np
+import pytest
+
+import unet_ref as U
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cavity_solver(fsx, N, ngp=8, mode=1, rtol=1e-8, steps=0, dt=0.01, viscosity=0.01):
+    mesh = fsx.Mesh.cavity(N)
+    mesh.set_params(NGP=ngp, dt=dt, maxIter=3, tolerance=1e-3, converge=1e-6, viscosity=viscosity)
+    s = fsx.Solver(mesh, device=0, NGP=ngp, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=rtol,
+                   pcg_maxit=20000, solver_mode=mode)
+    if steps:
+        s.run(steps)
+    return s
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_unet_forward_matches_oracle(fsx_lib, N):
+    say(f"unet forward: grid {N + 1}^3")
+    s = cavity_solver(fsx_lib, N)
+    g = (N + 1,) * 3
+    s.unet_setup(g, seed=11)
+    w = s.unet_weights().astype(np.float64)
+    x = np.random.default_rng(N).normal(size=(N + 1) ** 3).astype(np.float32)
+    y = s.unet_apply(x)
+    y_ref, _ = U.forward(x, w, g, exact=False)
+    e = rel(y, y_ref)
+    say(f"  rel err {e:.2e}")
+    assert e < 2e-3
+    # scaling by 2 commutes with every BF16/FP32 rounding: bit-exact
+    assert np.array_equal(s.unet_apply(2 * x), 2 * y)
+    # weights round-trip, and other weights change the output as the oracle says
+    w2 = U.init_weights(5).astype(np.float32)
+    s.unet_set_weights(w2)
+    assert np.array_equal(s.unet_weights(), w2)
+    y2 = s.unet_apply(x)
+    assert rel(y2, U.forward(x, w2.astype(np.float64), g, exact=False)[0]) < 2e-3
+    s.close()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_unet_gradient_matches_oracle(fsx_lib, kind):
+    say(f"unet loss gradient: grid 9^3, loss kind {kind}")
+    s = cavity_solver(fsx_lib, 8)
+    g = (9, 9, 9)
+    s.unet_setup(g, seed=3)
+    gain = float(np.float32(0.6))
+    s.unet_gain(gain)
+    w = s.unet_weights().astype(np.float64)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=729).astype(np.float32)
+    t = rng.normal(size=729).astype(np.float32)
+    loss, grad = s.unet_loss_grad(x, t, kind=kind)
+    if kind == 0:
+        loss_ref, grad_ref, _ = U.loss_grad(x, t, w, g, gain=gain, exact=False)
+    else:
+        # energy norm on the context's FP64 Z (the device trains on its FP32 copy)
+        y, cache = U.forward(x, w, g, gain=gain, exact=False)
+        t64 = t.astype(np.float64)
+        tt = float(t64 @ s.spmv("Z", t64))
+        zd = s.spmv("Z", y - t64)
+        loss_ref = float((y - t64) @ zd) / tt
+        grad_ref = U.backward(2.0 * zd / tt, cache)
+    say(f"  loss {loss:.6f} oracle {loss_ref:.6f}")
+    assert abs(loss - loss_ref) <= 1e-3 * loss_ref
+    offs, total = U.layer_offsets()
+    assert grad.size == total
+    for li, ((cin, cout, _), off) in enumerate(zip(U.CHANNELS, offs)):
+        n = cout * U.kpad(cin)
+        e = rel(grad[off:off + n], grad_ref[off:off + n])
+        say(f"  layer {U.LAYER_NAMES[li]}: rel err {e:.2e}")
+        assert e < 1e-2
+    s.close()
+
+
+def true_relres(s, b, x):
+    return float(np.linalg.norm(b - s.spmv("Z", x)) / np.linalg.norm(b))
+
+
+def test_learned_solve_trains_and_beats_jacobi(fsx_lib):
+    say("learned solve: cavity 16^3, NGP 8, capture 3 steps, train, solve a later right-hand side")
+    s = cavity_solver(fsx_lib, 16, steps=0)
+    s.unet_setup((17, 17, 17), seed=7)
+    s.unet_capture(64)
+    s.run(3)
+    ncap = s.unet_captured()
+    s.unet_capture(0)
+    s.run(1)
+    b = s.download("R2")
+    assert ncap >= 3
+    nsamp = s.unet_prepare([0, 2, 5, 10, 20, 40])
+    assert ncap <= nsamp <= 6 * ncap   # iterates converged below 1e-5 |b| are skipped
+    l0 = s.unet_train(1, batch=16, lr=1e-9, seed=1)
+    l1 = s.unet_train(400, batch=4, lr=1e-3, seed=2)
+    say(f"  captured {ncap}, samples {nsamp}, loss {l0:.4f} -> {l1:.4f}, gain {s.unet_gain():.3e}")
+    assert l1 < l0
+    s.set_solver(1, 1e-8, 20000)
+    xj, itj, relj, rc = s.solve_pressure(b)
+    assert rc == 0
+    s.set_solver(2, 1e-8, 20000)
+    xl, itl, rell, rc = s.solve_pressure(b)
+    lit, fb = s.unet_stats()
+    rj, rl = true_relres(s, b, xj), true_relres(s, b, xl)
+    say(f"  jacobi(mixed) {itj} it relres {rj:.2e}; learned {itl} it (flexible {lit}, fallback {fb}) relres {rl:.2e}")
+    assert rc == 0
+    assert rl <= 1.0001e-8
+    assert itl < itj
+    # both solutions solve the same system to 1e-8
+    assert rel(xl, xj) < 1e-5
+    s.close()
+
+
+def test_learned_fallback_and_zero_rhs(fsx_lib):
+    say("learned fallback: cavity 8^3, untrained network, flexible CG capped at 3 iterations")
+    s = cavity_solver(fsx_lib, 8, steps=1)
+    b = s.download("R2")
+    s.unet_setup((9, 9, 9), seed=2)
+    s.unet_limit(3)
+    s.set_solver(2, 1e-8, 20000)
+    x, it, r, rc = s.solve_pressure(b)
+    lit, fb = s.unet_stats()
+    rr = true_relres(s, b, x)
+    say(f"  flexible {lit} it, fallback {fb}, total {it} it, relres {rr:.2e}")
+    # the untrained network loses descent or hits the cap; Jacobi-PCG finishes
+    assert rc == 0 and fb in (2, 6) and 1 <= lit <= 3
+    assert rr <= 1.0001e-8
+    x0, it0, r0, rc0 = s.solve_pressure(np.zeros_like(b))
+    assert rc0 == 0 and it0 == 0 and not np.any(x0)
+    s.close()
+
+
+def test_learned_time_steps_match_jacobi(fsx_lib):
+    say("learned time steps: cavity 8^3, NGP 8, 3 steps learned vs Jacobi (rtol 1e-10)")
+    a = cavity_solver(fsx_lib, 8, rtol=1e-10, mode=0)
+    a.run(3)
+    ua, pa = a.download("Un"), a.download("Pn")
+    a.close()
+    s = cavity_solver(fsx_lib, 8, rtol=1e-10)
+    s.unet_setup((9, 9, 9), seed=4)
+    s.unet_capture(16)
+    s.run(2)
+    s.unet_capture(0)
+    s.unet_prepare([0, 3, 8])
+    s.unet_train(100, batch=4, lr=1e-3, seed=3)
+    s.reset(mode=2, rtol=1e-10, maxit=20000)
+    s.run(3)
+    ub, pb = s.download("Un"), s.download("Pn")
+    eu, ep = rel(ub, ua), rel(pb, pa)
+    say(f"  rel diff U {eu:.2e} P {ep:.2e}")
+    assert eu < 1e-6 and ep < 1e-6
+    s.close()
