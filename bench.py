@@ -10,9 +10,14 @@ plus the lid BC) and resident in HBM before timing starts.
 One "step" is one time step of the reference's timeLoop (BCH:2859-3118):
 convection residual, step1, step2 (assembly of R2 + the pressure solve),
 step3, convergence checks, K*Acc, for up to 3 inner iterations. The same
-trajectory is run with the plain FP64 Jacobi-PCG and with the mixed-precision
-solver (FP32 inner PCG, FP64 refinement residual); `value` is the headline
-(mixed) mode, the FP64 rate and the pressure-solve speed-up sit beside it.
+trajectory is run with the plain FP64 Jacobi-PCG, with the mixed-precision
+solver (FP32 inner PCG, FP64 refinement residual) and with learned+mixed
+(flexible PCG in FP32 preconditioned by Jacobi plus the trained multilevel
+network, FP64 reliable updates and stop test, Jacobi-PCG fallback); `value`
+is the headline (learned+mixed) mode, the other rates, the pressure-solve
+speed-up over FP64 and the field errors against the FP64 run sit beside it.
+The network is set up, fed with the solver's own pressure solves and
+trained before the timed region (train_s in the output).
 
 Prints one JSON line.
 """
@@ -53,6 +58,27 @@ def run_mode(solver, mode, steps, warmup, rtol, maxit):
         solve_ms=tm["pcg_ms"], Un=solver.download("Un"), Pn=solver.download("Pn"),
         mon=st[-1]["mon"], per_step=[dict(pcg=s["pcg"], inner=s["inner"], wall_s=s["wall_s"]) for s in st],
     )
+
+
+def train_learned(solver, args):
+    """Untimed preparation of the learned mode: capture the pressure solves of
+    a few mixed-precision steps, build training pairs from truncated
+    Jacobi-PCG iterates, train with Adam on the GPU."""
+    import fsx
+    t0 = time.perf_counter()
+    solver.reset(mode=fsx.SOLVER_MIXED, rtol=args.rtol, maxit=args.maxit)
+    n1 = args.n + 1
+    solver.unet_setup((n1, n1, n1), seed=args.seed)
+    solver.unet_capture(64)
+    solver.run(args.train_capture_steps)
+    ncap = solver.unet_captured()
+    solver.unet_capture(0)
+    nsamp = solver.unet_prepare([0, 5, 20, 50, 100])
+    loss0 = solver.unet_train(1, batch=8, lr=1e-12, seed=args.seed)
+    loss = solver.unet_train(args.train_iters, batch=4, lr=args.lr, seed=args.seed + 1)
+    solver.sync()
+    return dict(train_s=time.perf_counter() - t0, captured=ncap, samples=nsamp, loss_initial=loss0, loss_final=loss,
+                train_iters=args.train_iters, params=solver.unet_nparams())
 
 
 def cpu_baseline(mesh, args, gpu_pcg_per_step, gpu_inner_per_step):
@@ -106,7 +132,11 @@ def main():
     ap.add_argument("--ngp", type=int, default=27)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--maxit", type=int, default=20000)
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,mixed,learned")
+    ap.add_argument("--train-iters", type=int, default=400)
+    ap.add_argument("--train-capture-steps", type=int, default=3)
+    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=30)
     args = ap.parse_args()
@@ -124,16 +154,21 @@ def main():
     solver = fsx.Solver(mesh, device=0, NGP=args.ngp, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=args.rtol,
                         pcg_maxit=args.maxit, solver_mode=0)
     log(f"setup done in {time.perf_counter() - t0:.1f} s: {solver.setup_times()} sizes {solver.sizes}")
-    modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED}
+    modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED, "learned": fsx.SOLVER_LEARNED}
     res = {}
+    train = None
     for name in args.modes.split(","):
+        if name == "learned":
+            log("learned mode: capture, prepare and train the network (untimed) ...")
+            train = train_learned(solver, args)
+            log(f"  {train}")
         log(f"mode {name}: {args.warmup} warmup + {args.steps} timed steps")
         res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit)
         r = res[name]
         log(f"  {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms, "
             f"max relres {r['max_relres']:.2e}")
-    head = "mixed" if "mixed" in res else list(res)[0]
-    prec = 1 if head == "mixed" else 0
+    head = next((m for m in ("learned", "mixed") if m in res), list(res)[0])
+    prec = 1 if head in ("mixed", "learned") else 0
     ms, bytes_per = solver.bench_spmv(precision=prec, reps=100)
     ms64, bytes64 = solver.bench_spmv(precision=0, reps=100)
     achieved = bytes_per / (ms * 1e-3) / 1e9
@@ -149,7 +184,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "dtype": {"learned": "bf16+f32+f64", "mixed": "f32+f64"}.get(head, "f64"),
         "data": "synthetic",
         "config": {"workload": f"lid-driven cavity {args.n}^3 hex27/hex8, NGP {args.ngp}, Re {args.re:g}, "
                                f"dt {args.dt:g}, 3 inner iterations, pressure rtol {args.rtol:g}",
@@ -164,6 +199,8 @@ def main():
                       "max_relres": v["max_relres"]} for k, v in res.items()},
         "spmv_fp64": {"launch_ms": round(ms64, 5), "achieved_GBs": round(bytes64 / (ms64 * 1e-3) / 1e9, 1)},
     }
+    if train is not None:
+        out["learned_training"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in train.items()}
     if "fp64" in res and head != "fp64":
         f = res["fp64"]
         out["pressure_solve_speedup"] = round(f["solve_ms"] / h["solve_ms"], 3)

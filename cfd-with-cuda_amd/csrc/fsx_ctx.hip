@@ -38,6 +38,16 @@ int fail(fsx_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+void iter_graph_reset(IterGraph& g) {
+  for (int i = 0; i < 2; i++) {
+    if (g.exec[i]) (void)hipGraphExecDestroy(g.exec[i]);
+    if (g.graph[i]) (void)hipGraphDestroy(g.graph[i]);
+    g.exec[i] = nullptr;
+    g.graph[i] = nullptr;
+  }
+  g.key = 0;
+}
+
 int launch_check(fsx_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ctx, FSX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -155,6 +165,7 @@ void fsx_destroy(fsx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   fsx::unet_free(ctx);
+  for (fsx::IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) fsx::iter_graph_reset(*g);
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
   if (ctx->h_st) (void)hipHostFree(ctx->h_st);
@@ -186,6 +197,12 @@ int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, doub
   ctx->tab.w = ctx->tab.dSp + NGP * 24;
   ctx->NGP = NGP; ctx->density = density; ctx->viscosity = viscosity; ctx->dt = dt;
   ctx->have_params = true;
+  return FSX_OK;
+}
+
+int fsx_set_graphs(fsx_ctx* ctx, int enable) {
+  if (!ctx) return FSX_EINVAL;
+  ctx->use_graphs = enable != 0;
   return FSX_OK;
 }
 

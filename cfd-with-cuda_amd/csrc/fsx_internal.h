@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -46,6 +47,30 @@ struct ShapeTables {
   double* dSp;
   double* w;
 };
+
+// A block of solver iterations captured once as a HIP graph (one per
+// starting parity; kernels see the iteration index only through j & 1) and
+// replayed, since host launch cost otherwise dominates the short vector
+// kernels of an iteration. `key` names what the captured arguments depend
+// on beyond the context's own buffers; a new key recaptures.
+struct IterGraph {
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  unsigned long long key = 0;
+};
+constexpr int kGraphBlock = 2;   // iterations per captured graph (even)
+
+// Iterations expected before the next host decision (a reliable update every
+// delta of residual reduction, or convergence at rtol), from the previous
+// solve's count: chunks sized so, then topped up in small steps, keep the
+// no-op launches after a stop few.
+inline int segment_guess(int last_iters, double delta, double rtol, int floor_) {
+  if (last_iters <= 0) return floor_;
+  double f = 1.0;
+  if (delta > 0.0 && delta < 1.0 && rtol > 0.0 && rtol < delta) f = std::log(delta) / std::log(rtol);
+  const int g = (int)(0.9 * last_iters * f);
+  return g > floor_ ? g : floor_;
+}
 
 }  // namespace fsx
 
@@ -114,6 +139,9 @@ struct fsx_ctx {
   int last_learned_iters = 0;
   int last_fallback = 0;
   void (*capture_hook)(fsx_ctx*, const double* b, const double* x) = nullptr;
+  // iteration graphs: FP64 PCG, mixed inner PCG, learned flexible PCG
+  bool use_graphs = true;
+  fsx::IterGraph graph_fp64, graph_mixed, graph_learned;
 };
 
 // error helpers (host)
@@ -160,4 +188,35 @@ int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double a
 int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);
 int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi);
 void unet_free(fsx_ctx* ctx);
+void iter_graph_reset(IterGraph& g);
+
+// Enqueue iterations j .. j + count - 1 (rounded up to whole graphs when
+// graphs are on; iterations after the stop flag is set are no-ops).
+// issue(jj) enqueues one iteration on ctx->stream and returns a status.
+// *enqueued returns how many iterations were enqueued (the next start).
+template <class F>
+int run_iterations(fsx_ctx* ctx, IterGraph& G, unsigned long long key, int j, int count, int* enqueued, F&& issue) {
+  if (!ctx->use_graphs) {
+    for (int c = 0; c < count; c++) FSX_TRY(issue(j + c));
+    *enqueued = count;
+    return launch_check(ctx, "solver iterations");
+  }
+  if (G.key != key) { iter_graph_reset(G); G.key = key; }
+  const int par = j & 1;
+  if (!G.exec[par]) {
+    FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FSX_OK;
+    for (int i = 0; i < kGraphBlock && rc == FSX_OK; i++) rc = issue(par + i);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != FSX_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(ctx, FSX_EHIP, std::string("hipStreamEndCapture -> ") + hipGetErrorString(e));
+    G.graph[par] = g;
+    FSX_HIP(hipGraphInstantiate(&G.exec[par], g, nullptr, nullptr, 0));
+  }
+  int c = 0;
+  for (; c < count; c += kGraphBlock) FSX_HIP(hipGraphLaunch(G.exec[par], ctx->stream));
+  *enqueued = c;
+  return FSX_OK;
+}
 }  // namespace fsx

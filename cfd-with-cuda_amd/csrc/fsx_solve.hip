@@ -527,15 +527,22 @@ template <class T>
 static int pcg_iterate(fsx_ctx* ctx, int j0, const T* val, const T* dinv, T* x, T* r, T* z, T* p, T* q, int kind,
                        double delta2, int first_chunk, int maxit) {
   int j = j0;
-  int chunk = std::max(8, first_chunk);
+  int chunk = std::max(4, first_chunk);
+  IterGraph& G = sizeof(T) == 8 ? ctx->graph_fp64 : ctx->graph_mixed;
+  const unsigned long long key = (unsigned long long)(uintptr_t)x;   // the only caller-owned buffer
+  auto issue = [&](int jj) -> int {
+    pcg_launch_iteration<T>(ctx, jj, val, dinv, x, r, z, p, q, kind, delta2);
+    return FSX_OK;
+  };
   for (;;) {
-    for (int c = 0; c < chunk; c++, j++) pcg_launch_iteration<T>(ctx, j, val, dinv, x, r, z, p, q, kind, delta2);
-    FSX_TRY(launch_check(ctx, "pcg iteration"));
+    int done = 0;
+    FSX_TRY(run_iterations(ctx, G, key, j, chunk, &done, issue));
+    j += done;
     FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
     FSX_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->h_sc->stop) return FSX_OK;
     if (j - j0 > maxit + 64) return fail(ctx, FSX_ESTATE, "PCG did not terminate");
-    chunk = 8;
+    chunk = 4;
   }
 }
 
@@ -555,15 +562,13 @@ static void timing_end(fsx_ctx* ctx, int iters) {
 
 static int solve_fp64(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   const int n = ctx->NNp;
-  timing_begin(ctx);
   hipLaunchKernelGGL(k_pcg_init<double>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->Zdinv, x, ctx->wr,
                      ctx->wz, ctx->wp, ctx->partials);
   hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, ctx->maxit);
   FSX_TRY(launch_check(ctx, "k_pcg_init"));
   FSX_TRY(pcg_iterate<double>(ctx, 0, ctx->Z, ctx->Zdinv, x, ctx->wr, ctx->wz, ctx->wp, ctx->wq, 0, 0.0,
-                              (int)(0.9 * ctx->last_iters), ctx->maxit));
+                              segment_guess(std::min(ctx->last_iters, ctx->maxit), 1.0, ctx->rtol, 4), ctx->maxit));
   const SolveScalars& s = *ctx->h_sc;
-  timing_end(ctx, s.iters);
   *iters = s.iters;
   *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
   ctx->last_iters = s.iters;
@@ -657,13 +662,13 @@ __global__ __launch_bounds__(kBlock) void k_mx_replace_finish(int j, const doubl
 static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   const int n = ctx->NNp;
   const double delta = 1e-2;
-  timing_begin(ctx);
   FSX_TRY(launch_fill(ctx, x, 0.0, n));
   hipLaunchKernelGGL(k_pcg_init<float>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->Zdinvf, ctx->fx, ctx->fr,
                      ctx->fz, ctx->fp, ctx->partials);
   hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, ctx->maxit);
   FSX_TRY(launch_check(ctx, "k_pcg_init<float>"));
-  int j = 0, guess = (int)(0.45 * ctx->last_iters), status = 0, updates = 0;
+  const int seg = segment_guess(ctx->last_iters, delta, ctx->rtol, 4);
+  int j = 0, guess = seg, status = 0, updates = 0;
   for (;;) {
     FSX_TRY(pcg_iterate<float>(ctx, j, ctx->Zf, ctx->Zdinvf, ctx->fx, ctx->fr, ctx->fz, ctx->fp, ctx->fq, 1, delta * delta,
                                guess, ctx->maxit));
@@ -688,10 +693,9 @@ static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
                        ctx->partials);
     hipLaunchKernelGGL(k_mx_replace_finish, dim3(1), dim3(kBlock), 0, ctx->stream, j, ctx->partials, ctx->sc);
     FSX_TRY(launch_check(ctx, "k_mx_replace"));
-    guess = 8;
+    guess = seg;
   }
   const SolveScalars& s = *ctx->h_sc;
-  timing_end(ctx, s.iters);
   *iters = s.iters;
   *relres = s.bb > 0 ? std::sqrt(s.rr_hi / s.bb) : 0.0;
   ctx->last_iters = s.iters;
@@ -757,10 +761,12 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
 
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   int rc;
+  timing_begin(ctx);
   if (ctx->solver_mode == FSX_SOLVER_LEARNED && ctx->unet) rc = solve_learned(ctx, b, x, iters, relres, 0);
   else if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)
     rc = solve_mixed(ctx, b, x, iters, relres);
   else rc = solve_fp64(ctx, b, x, iters, relres);
+  if (rc == FSX_OK || rc == FSX_ENOCONV) timing_end(ctx, *iters);
   ctx->last_relres = *relres;
   if (ctx->capture_hook && rc == FSX_OK) ctx->capture_hook(ctx, b, x);
   return rc;

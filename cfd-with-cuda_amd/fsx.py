@@ -29,7 +29,7 @@ VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, 
 # Exported C symbols declared in include/fsx.h and include/fsx_host.h
 DEVICE_SYMBOLS = (
     "fsx_create", "fsx_destroy", "fsx_last_error", "fsx_device_bytes", "fsx_upload_mesh", "fsx_upload_csr",
-    "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
+    "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
     "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_timing_reset", "fsx_timing_get",
@@ -85,6 +85,7 @@ def lib():
         "fsx_upload_bc": ([_vp, ctypes.c_int, _I, _D, ctypes.c_int], ctypes.c_int),
         "fsx_set_params": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "fsx_set_solver": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "fsx_set_graphs": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_assemble_static": ([_vp], ctypes.c_int),
         "fsx_build_Z": ([_vp], ctypes.c_int),
         "fsx_init_state": ([_vp], ctypes.c_int),
@@ -409,6 +410,9 @@ class Solver:
 
     def set_solver(self, mode, rtol, maxit):
         self.check(self.L.fsx_set_solver(self.ctx, mode, rtol, maxit), "set_solver")
+
+    def set_graphs(self, enable):
+        self.check(self.L.fsx_set_graphs(self.ctx, 1 if enable else 0), "set_graphs")
 
     # single-stage calls (the fsx_* C-ABI, for stage-by-stage parity tests)
     def begin_step(self):

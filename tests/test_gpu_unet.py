@@ -46,7 +46,8 @@ def test_unet_forward_matches_oracle(fsx_lib, N):
     w = s.unet_weights().astype(np.float64)
     x = np.random.default_rng(N).normal(size=(N + 1) ** 3).astype(np.float32)
     y = s.unet_apply(x)
-    y_ref, _ = U.forward(x, w, g, exact=False)
+    gain = s.unet_gain()
+    y_ref, _ = U.forward(x, w, g, gain=gain, exact=False)
     e = rel(y, y_ref)
     say(f"  rel err {e:.2e}")
     assert e < 2e-3
@@ -57,7 +58,7 @@ def test_unet_forward_matches_oracle(fsx_lib, N):
     s.unet_set_weights(w2)
     assert np.array_equal(s.unet_weights(), w2)
     y2 = s.unet_apply(x)
-    assert rel(y2, U.forward(x, w2.astype(np.float64), g, exact=False)[0]) < 2e-3
+    assert rel(y2, U.forward(x, w2.astype(np.float64), g, gain=gain, exact=False)[0]) < 2e-3
     s.close()
 
 
@@ -171,4 +172,26 @@ def test_learned_time_steps_match_jacobi(fsx_lib):
     eu, ep = rel(ub, ua), rel(pb, pa)
     say(f"  rel diff U {eu:.2e} P {ep:.2e}")
     assert eu < 1e-6 and ep < 1e-6
+    s.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_graph_replay_matches_eager(fsx_lib, mode):
+    """Solver iterations replayed from captured HIP graphs give bit-identical
+    iterates to individual launches, on every replay (several graph launches
+    per solve, three solves in a row with eager work in between)."""
+    say(f"graph replay vs eager: cavity 16^3, solver mode {mode}")
+    s = cavity_solver(fsx_lib, 16, steps=1)
+    b = s.download("R2")
+    if mode == 2:
+        s.unet_setup((17, 17, 17), seed=5)
+    s.set_solver(mode, 1e-8, 20000)
+    s.set_graphs(False)
+    xe, ite, _, rce = s.solve_pressure(b)
+    s.set_graphs(True)
+    for rep in range(3):
+        xg, itg, _, rcg = s.solve_pressure(b)
+        assert rcg == rce == 0 and itg == ite, (rep, itg, ite)
+        assert np.array_equal(xg, xe), rep
+        s.spmv("Z", xg)   # eager work between replays
     s.close()
