@@ -60,6 +60,19 @@ def run_mode(solver, mode, steps, warmup, rtol, maxit):
     )
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    runs (FETCH_SIZE and WRITE_SIZE of bench.py --spmv-probe, each scaled by
+    the same counter on a copy of known size), or None when not measured."""
+    path = os.path.join(ROOT, "profiles", "pmc_spmv_cavity64.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get("fsx::" + kernel)
+    return None if k is None else round(k["hbm_bytes_per_launch"])
+
+
 def train_learned(solver, args):
     """Untimed preparation of the learned mode: capture the pressure solves of
     a few mixed-precision steps, build training pairs from truncated
@@ -137,6 +150,9 @@ def main():
     ap.add_argument("--train-capture-steps", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--spmv-probe", action="store_true",
+                    help="only set up the case, time the Z SpMV kernels and run a 256 MiB calibration copy "
+                         "(the workload of the rocprofv3 --pmc traffic runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=30)
     args = ap.parse_args()
@@ -154,6 +170,16 @@ def main():
     solver = fsx.Solver(mesh, device=0, NGP=args.ngp, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=args.rtol,
                         pcg_maxit=args.maxit, solver_mode=0)
     log(f"setup done in {time.perf_counter() - t0:.1f} s: {solver.setup_times()} sizes {solver.sizes}")
+    if args.spmv_probe:
+        ms32, b32 = solver.bench_spmv(precision=1, reps=20)
+        ms64, b64 = solver.bench_spmv(precision=0, reps=20)
+        copy_bytes = 256 << 20
+        msc = solver.bench_copy(copy_bytes, reps=5)
+        print(json.dumps({"spmv_f32": {"ms": ms32, "bytes": b32}, "spmv_f64": {"ms": ms64, "bytes": b64},
+                          "copy_bytes_each_way": copy_bytes, "copy_ms": msc,
+                          "copy_GBs": 2 * copy_bytes / (msc * 1e-3) / 1e9}), flush=True)
+        solver.close()
+        return
     modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED, "learned": fsx.SOLVER_LEARNED}
     res = {}
     train = None
@@ -192,7 +218,8 @@ def main():
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<float,32,4>" if prec else "k_pcg_spmv<double,16,8>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("k_pcg_spmv<float, 32, 4>" if prec else "k_pcg_spmv<double, 16, 8>"),
                      "algorithmic_bytes_per_launch": bytes_per, "launch_ms": round(ms, 5)},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 4), "pcg_iters": v["pcg_iters"],
                       "inner_iters": v["inner_iters"], "solve_ms": round(v["solve_ms"], 2),

@@ -823,3 +823,32 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
   if (bytes_per_launch) *bytes_per_launch = bytes;
   return FSX_OK;
 }
+
+// Times `reps` launches of a plain device copy of `bytes` (rounded down to
+// whole doubles) between two scratch buffers: the reference figure for HBM
+// bandwidth and the kernel of known traffic that calibrates the PMC byte
+// counters. No reference counterpart.
+extern "C" int fsx_bench_copy(fsx_ctx* ctx, size_t bytes, int reps, double* ms_per_launch) {
+  using namespace fsx;
+  if (!ctx || reps < 1 || bytes < 8 || !ms_per_launch) return FSX_EINVAL;
+  const long long n = (long long)(bytes / 8);
+  double *a = nullptr, *b = nullptr;
+  FSX_TRY(alloc(ctx, &a, (size_t)n));
+  int rc = alloc(ctx, &b, (size_t)n);
+  if (rc != FSX_OK) { dev_free(ctx, a); return rc; }
+  rc = launch_fill(ctx, a, 1.0, n);
+  for (int w = 0; w < 2 && rc == FSX_OK; w++) rc = launch_copy(ctx, b, a, n);
+  float t = 0;
+  if (rc == FSX_OK) {
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    for (int r = 0; r < reps && rc == FSX_OK; r++) rc = launch_copy(ctx, b, a, n);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    if (hipEventSynchronize(ctx->ev1) != hipSuccess) rc = fail(ctx, FSX_EHIP, "fsx_bench_copy: event sync");
+    (void)hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  dev_free(ctx, a);
+  dev_free(ctx, b);
+  *ms_per_launch = t / reps;
+  return rc;
+}

@@ -180,6 +180,11 @@ int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, 
  * row pointers + one read of p + one write of q. No reference counterpart
  * (the reference prints wall times only, BCH:4489-4518). */
 int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch, double* bytes_per_launch);
+/* Times `reps` launches of a plain device copy of `bytes` (read once, written
+ * once) between two scratch buffers: the HBM bandwidth reference and the
+ * known-traffic kernel that calibrates PMC byte counters. No reference
+ * counterpart. */
+int fsx_bench_copy(fsx_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
 
 /* kernel time accounting (device time of the last timed region, ms) */
 typedef struct fsx_timing {
