@@ -198,6 +198,11 @@ def main():
     ms, bytes_per = solver.bench_spmv(precision=prec, reps=100)
     ms64, bytes64 = solver.bench_spmv(precision=0, reps=100)
     achieved = bytes_per / (ms * 1e-3) / 1e9
+    npat, _ = solver.z_patterns()
+    if npat > 0:   # the row-pattern form runs 8 lanes x 16 loads per row in both precisions
+        kname = "k_pcg_spmv<float, 8, 16, true>" if prec else "k_pcg_spmv<double, 8, 16, true>"
+    else:
+        kname = "k_pcg_spmv<float, 32, 4, false>" if prec else "k_pcg_spmv<double, 16, 8, false>"
     h = res[head]
     out = {
         "metric": METRIC,
@@ -216,15 +221,15 @@ def main():
                                f"dt {args.dt:g}, 3 inner iterations, pressure rtol {args.rtol:g}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<float,32,4>" if prec else "k_pcg_spmv<double,16,8>",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("k_pcg_spmv<float, 32, 4>" if prec else "k_pcg_spmv<double, 16, 8>"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
                      "algorithmic_bytes_per_launch": bytes_per, "launch_ms": round(ms, 5)},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 4), "pcg_iters": v["pcg_iters"],
                       "inner_iters": v["inner_iters"], "solve_ms": round(v["solve_ms"], 2),
                       "max_relres": v["max_relres"]} for k, v in res.items()},
         "spmv_fp64": {"launch_ms": round(ms64, 5), "achieved_GBs": round(bytes64 / (ms64 * 1e-3) / 1e9, 1)},
+        "z_row_patterns": npat,
     }
     if train is not None:
         out["learned_training"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in train.items()}

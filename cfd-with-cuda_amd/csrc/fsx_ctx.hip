@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <utility>
 
 namespace fsx {
@@ -274,6 +275,42 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
   return FSX_OK;
 }
 
+// Dictionary of the distinct column-offset lists (col - row) of a CSR
+// pattern. Structured meshes have a handful (interior rows all share one),
+// so the SpMVs can read a 4-byte pattern id per row instead of a 4-byte
+// column per entry. Returns false when there are more than max_pat distinct
+// lists or they hold more than max_len offsets in total.
+static bool build_patterns(int nrows, const int* rowptr, const int* col, int max_pat, long long max_len,
+                           std::vector<int>& row_pat, std::vector<int>& pat_ptr, std::vector<int>& pat_off) {
+  std::unordered_map<unsigned long long, std::vector<int>> by_hash;
+  row_pat.assign(nrows, 0);
+  pat_ptr.assign(1, 0);
+  pat_off.clear();
+  for (int r = 0; r < nrows; r++) {
+    const int rs = rowptr[r], len = rowptr[r + 1] - rs;
+    unsigned long long h = 1469598103934665603ull ^ (unsigned long long)len;
+    for (int k = 0; k < len; k++) h = (h ^ (unsigned long long)(unsigned)(col[rs + k] - r)) * 1099511628211ull;
+    int id = -1;
+    auto& cand = by_hash[h];
+    for (int c : cand) {
+      const int cs = pat_ptr[c], clen = pat_ptr[c + 1] - cs;
+      if (clen != len) continue;
+      bool same = true;
+      for (int k = 0; k < len && same; k++) same = pat_off[cs + k] == col[rs + k] - r;
+      if (same) { id = c; break; }
+    }
+    if (id < 0) {
+      id = (int)pat_ptr.size() - 1;
+      if (id >= max_pat || (long long)pat_off.size() + len > max_len) return false;
+      for (int k = 0; k < len; k++) pat_off.push_back(col[rs + k] - r);
+      pat_ptr.push_back((int)pat_off.size());
+      cand.push_back(id);
+    }
+    row_pat[r] = id;
+  }
+  return true;
+}
+
 int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const int* col) {
   if (!ctx || !rowptr || !col) return ctx ? fail(ctx, FSX_EINVAL, "fsx_upload_csr: null") : FSX_EINVAL;
   if (!ctx->have_mesh) return fail(ctx, FSX_ESTATE, "upload the mesh first");
@@ -330,6 +367,18 @@ int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const i
       if (!diag) return fail(ctx, FSX_EINVAL, "Z pattern lacks a diagonal entry");
     }
     ctx->Zrow = drow; ctx->Zcol = dcol; ctx->nnzZ = nnz; ctx->zrow_max = rmax;
+    {
+      std::vector<int> rp, pp, po;
+      if (build_patterns(nrows, rowptr, col, 1 << 16, 1 << 22, rp, pp, po)) {
+        for (int& id : rp) id = pp[id];   // rows keep where their pattern starts
+        FSX_TRY(alloc(ctx, &ctx->Zrowpat, rp.size()));
+        FSX_TRY(alloc(ctx, &ctx->Zpatoff, std::max<size_t>(1, po.size())));
+        FSX_HIP(hipMemcpy(ctx->Zrowpat, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!po.empty()) FSX_HIP(hipMemcpy(ctx->Zpatoff, po.data(), po.size() * sizeof(int), hipMemcpyHostToDevice));
+        ctx->Znpat = (int)pp.size() - 1;
+        ctx->Zpatlen = (long long)po.size();
+      }
+    }
     FSX_TRY(alloc(ctx, &ctx->Z, (size_t)nnz));
     FSX_TRY(alloc(ctx, &ctx->Zf, (size_t)nnz));
     FSX_TRY(alloc(ctx, &ctx->Zdinv, (size_t)nrows));

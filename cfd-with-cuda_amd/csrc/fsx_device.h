@@ -86,6 +86,29 @@ __device__ __forceinline__ double block_max_all(double v) {
   return m;
 }
 
+// Column source of a CSR matrix: the explicit column array, or the
+// dictionary of row patterns (column = row + pat_off[row_pat[row] + k -
+// rowptr[row]], row_pat holding where the row's pattern starts in pat_off),
+// which replaces a 4-byte column per entry by 4 bytes per row.
+struct ColSrc {
+  const int* ci;
+  const int* row_pat;
+  const int* pat_off;
+};
+
+// per-row offset that turns an entry index kk into its pattern slot
+template <bool PAT>
+__device__ __forceinline__ int col_base(const ColSrc& cs, long long r, int rs) {
+  if constexpr (PAT) return cs.row_pat[r] - rs;
+  else return 0;
+}
+
+template <bool PAT>
+__device__ __forceinline__ int col_at(const ColSrc& cs, long long r, int base, int kk) {
+  if constexpr (PAT) return (int)r + cs.pat_off[base + kk];
+  else return cs.ci[kk];
+}
+
 // Position of `c` in the sorted columns col[lo, hi); the entry must exist.
 __device__ __forceinline__ int find_col(const int* __restrict__ col, int lo, int hi, int c) {
   while (hi - lo > 1) {

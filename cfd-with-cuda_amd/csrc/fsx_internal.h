@@ -102,6 +102,11 @@ struct fsx_ctx {
   int* Zrow = nullptr; int* Zcol = nullptr;
   double* Z = nullptr; double* Zdinv = nullptr;
   float* Zf = nullptr; float* Zdinvf = nullptr;
+  // Z columns as a dictionary of row patterns: col = row + Zpatoff[Zrowpat[row] + k - Zrow[row]]
+  // (Zrowpat holds where the row's pattern starts; null when the rows do not
+  // compress, and the SpMVs then read Zcol)
+  int* Zrowpat = nullptr; int* Zpatoff = nullptr;
+  int Znpat = 0; long long Zpatlen = 0;
   int zrow_max = 0;
   // lumped mass and BCs (device, NN-long; the reference replicates them x3)
   double* MdInv = nullptr; double* MdOrigInv = nullptr; double* Md = nullptr;

@@ -387,8 +387,8 @@ template <class T> struct PcgCfg;
 template <> struct PcgCfg<double> { static constexpr int LPR = 16, U = 8; };
 template <> struct PcgCfg<float> { static constexpr int LPR = 32, U = 4; };
 
-template <class T, int LPR, int U>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+template <class T, int LPR, int U, bool PAT>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restrict__ rp, ColSrc cs,
                                                      const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
                                                      const SolveScalars* __restrict__ sc) {
   if (sc->stop) return;
@@ -397,13 +397,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
   T s = 0;
   if (r < n) {
     const int rs = rp[r], re = rp[r + 1];
+    const int cb = col_base<PAT>(cs, r, rs);
     for (int base = rs + l; base < re; base += U * LPR) {
       int c[U];
       T a[U];
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
-        c[j] = ci[kk];
+        c[j] = col_at<PAT>(cs, r, cb, kk);
         a[j] = k < re ? val[kk] : T(0);
       }
 #pragma unroll
@@ -412,6 +413,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
   }
   if constexpr (sizeof(T) == 8) s = group_sum<LPR>(s); else s = group_sumf<LPR>(s);
   if (r < n && l == 0) q[r] = s;
+}
+
+// q = Z p and rh = b - Z xh on the context's Z, through the row-pattern
+// dictionary when Z has one
+static ColSrc z_cols(fsx_ctx* ctx) { return ColSrc{ctx->Zcol, ctx->Zrowpat, ctx->Zpatoff}; }
+
+template <class T>
+static void launch_zspmv(fsx_ctx* ctx, const T* val, const T* p, T* q) {
+  constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
+  const ColSrc cs = z_cols(ctx);
+  const int n = ctx->NNp;
+  // the dictionary form uses 8 lanes x 16 loads per row in both precisions
+  // (a sweep of 8x16, 16x8, 32x4, 64x2 on the 64^3 Z: 36.7, 56.2, 64.3,
+  // 82.0 us in FP32; 44.5, 45.5, 53.4, 80.9 us in FP64): more independent
+  // loads in flight per lane hide the per-row pattern lookup
+  if (cs.row_pat)
+    hipLaunchKernelGGL((k_pcg_spmv<T, 8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, cs,
+                       val, p, q, ctx->sc);
+  else
+    hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow,
+                       cs, val, p, q, ctx->sc);
 }
 
 template <class T>
@@ -507,8 +529,7 @@ static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* val, const T* din
                                  int kind, double delta2) {
   constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
   const int n = ctx->NNp;
-  hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow,
-                     ctx->Zcol, val, p, q, ctx->sc);
+  launch_zspmv<T>(ctx, val, p, q);
   hipLaunchKernelGGL(k_pcg_dot<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, p, q, ctx->partials, ctx->sc);
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, x, r, z, p, q, dinv,
                      ctx->partials, ctx->sc);
@@ -594,8 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_fold(int n, double* __restrict__ 
 }
 
 // rh = b - Z xh (FP64)
-template <int LPR, int U>
-__global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+template <int LPR, int U, bool PAT>
+__global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restrict__ rp, ColSrc cs,
                                                      const double* __restrict__ val, const double* __restrict__ xh,
                                                      const double* __restrict__ b, double* __restrict__ rh) {
   int l;
@@ -603,13 +624,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restric
   double s = 0;
   if (i < n) {
     const int rs = rp[i], re = rp[i + 1];
+    const int cb = col_base<PAT>(cs, i, rs);
     for (int base = rs + l; base < re; base += U * LPR) {
       int c[U];
       double a[U];
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
-        c[j] = ci[kk];
+        c[j] = col_at<PAT>(cs, i, cb, kk);
         a[j] = k < re ? val[kk] : 0.0;
       }
 #pragma unroll
@@ -618,6 +640,18 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restric
   }
   s = group_sum<LPR>(s);
   if (i < n && l == 0) rh[i] = b[i] - s;
+}
+
+static void launch_zresidual(fsx_ctx* ctx, const double* xh, const double* b, double* rh) {
+  constexpr int LPR = PcgCfg<double>::LPR, U = PcgCfg<double>::U;
+  const ColSrc cs = z_cols(ctx);
+  const int n = ctx->NNp;
+  if (cs.row_pat)
+    hipLaunchKernelGGL((k_residual<8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, cs, ctx->Z,
+                       xh, b, rh);
+  else
+    hipLaunchKernelGGL((k_residual<LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, cs,
+                       ctx->Z, xh, b, rh);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sumsq(int n, const double* __restrict__ a, double* __restrict__ partials) {
@@ -678,8 +712,7 @@ static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
     j = ctx->h_sc->iters;                            // next iteration index
     // reliable update: fold the FP32 iterate into x, FP64 residual and stop test
     hipLaunchKernelGGL(k_mx_fold, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, x, ctx->fx);
-    hipLaunchKernelGGL((k_residual<PcgCfg<double>::LPR, PcgCfg<double>::U>), dim3(rows_grid(n, PcgCfg<double>::LPR)),
-                       dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, ctx->Z, x, b, ctx->wh);
+    launch_zresidual(ctx, x, b, ctx->wh);
     hipLaunchKernelGGL(k_sumsq, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->partials);
     hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc);
     FSX_TRY(launch_check(ctx, "mixed reliable update"));
@@ -714,8 +747,7 @@ int solve_fp64_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, doubl
 
 int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q) {
   constexpr int LPR = PcgCfg<float>::LPR, U = PcgCfg<float>::U;
-  hipLaunchKernelGGL((k_pcg_spmv<float, LPR, U>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
-                     ctx->Zrow, ctx->Zcol, ctx->Zf, p, q, ctx->sc);
+  launch_zspmv<float>(ctx, ctx->Zf, p, q);
   return FSX_OK;
 }
 
@@ -738,8 +770,7 @@ int launch_add(fsx_ctx* ctx, double* x, const double* d, int n) {
 static int residual_check(fsx_ctx* ctx, double* x, const double* b, bool fold, int* stop, double* rr_hi) {
   const int n = ctx->NNp;
   if (fold) hipLaunchKernelGGL(k_mx_fold, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, x, ctx->fx);
-  hipLaunchKernelGGL((k_residual<PcgCfg<double>::LPR, PcgCfg<double>::U>), dim3(rows_grid(n, PcgCfg<double>::LPR)),
-                     dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, ctx->Zcol, ctx->Z, x, b, ctx->wh);
+  launch_zresidual(ctx, x, b, ctx->wh);
   hipLaunchKernelGGL(k_sumsq, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->partials);
   hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc);
   FSX_TRY(launch_check(ctx, "residual check"));
@@ -787,12 +818,10 @@ static int bench_spmv_t(fsx_ctx* ctx, const T* val, T* p, T* q, int reps, double
   hipLaunchKernelGGL(k_fill_t<T>, dim3(1024), dim3(kBlock), 0, ctx->stream, p, (T)1, n);
   hipLaunchKernelGGL(k_arm, dim3(1), dim3(1), 0, ctx->stream, ctx->sc);
   for (int w = 0; w < 3; w++)
-    hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow,
-                       ctx->Zcol, val, p, q, ctx->sc);
+    launch_zspmv<T>(ctx, val, p, q);
   FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int r = 0; r < reps; r++)
-    hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow,
-                       ctx->Zcol, val, p, q, ctx->sc);
+    launch_zspmv<T>(ctx, val, p, q);
   FSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   FSX_HIP(hipEventSynchronize(ctx->ev1));
   FSX_TRY(launch_check(ctx, "k_pcg_spmv (timing)"));
@@ -811,16 +840,26 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
   double ms = 0;
   const double nnz = (double)ctx->nnzZ, n = ctx->NNp;
   double bytes;
-  // algorithmic bytes: values + columns + row pointers, p read once, q written once
+  // algorithmic bytes: values + column indices (or, with the row-pattern
+  // dictionary, one pattern id per row plus the dictionary) + row pointers,
+  // p read once, q written once
+  const double cols = ctx->Zrowpat ? 4.0 * n + 4.0 * (ctx->Zpatlen + ctx->Znpat + 1) : 4.0 * nnz;
   if (precision == 0) {
     FSX_TRY(bench_spmv_t<double>(ctx, ctx->Z, ctx->wp, ctx->wq, reps, &ms));
-    bytes = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n;
+    bytes = 8.0 * nnz + cols + 4.0 * (n + 1) + 16.0 * n;
   } else {
     FSX_TRY(bench_spmv_t<float>(ctx, ctx->Zf, ctx->fp, ctx->fq, reps, &ms));
-    bytes = 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n;
+    bytes = 4.0 * nnz + cols + 4.0 * (n + 1) + 8.0 * n;
   }
   *ms_per_launch = ms;
   if (bytes_per_launch) *bytes_per_launch = bytes;
+  return FSX_OK;
+}
+
+extern "C" int fsx_z_patterns(fsx_ctx* ctx, int* npatterns, long long* offsets) {
+  if (!ctx) return FSX_EINVAL;
+  if (npatterns) *npatterns = ctx->Zrowpat ? ctx->Znpat : 0;
+  if (offsets) *offsets = ctx->Zrowpat ? ctx->Zpatlen : 0;
   return FSX_OK;
 }
 

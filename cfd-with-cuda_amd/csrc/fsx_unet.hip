@@ -3,8 +3,8 @@
 // weight-gradient kernels, Adam training on the GPU, and training pairs
 // produced by the solver itself.
 //
-// Preconditioner z = dinv r + gain s J^T J (r / s), s = rms(r) (so
-// M(cr) = c M(r)). J maps the pressure grid gf to features on three coarser
+// Preconditioner z = dinv r + gain J^T J r (linear, so M(cr) = c M(r); the
+// training pairs are normalised by rms(r)). J maps the pressure grid gf to features on three coarser
 // node-aligned grids (G0 = every other node of gf, G1 of G0, G2 of G1;
 // 65^3 -> 33^3 -> 17^3 -> 9^3):
 //   u  = B0 R x            full-weighting restriction gf -> G0      (1 ch)
@@ -98,7 +98,7 @@ __host__ __device__ __forceinline__ bool on_boundary(long long v, const Grid3& g
 constexpr int kConvVox = 16;   // output voxels per block
 
 template <int CIN>
-constexpr int conv_ks() { return ((27 * CIN + 31) / 32) >= 24 ? 8 : 4; }
+constexpr int conv_ks() { return ((27 * CIN + 31) / 32) >= 48 ? 16 : ((27 * CIN + 31) / 32) >= 24 ? 8 : 4; }
 
 template <int CIN, int COUT, int MODE>
 __global__ __launch_bounds__(kWave * conv_ks<CIN>()) void k_conv_mfma(Grid3 gi, Grid3 go, const __bf16* __restrict__ in,
@@ -158,85 +158,63 @@ __global__ __launch_bounds__(kWave * conv_ks<CIN>()) void k_conv_mfma(Grid3 gi, 
 }
 
 // ------------------------------------------------------------ 1-channel layers (VALU)
-// a_in[v][co] = (1/s) sum_t Win[t][co] x[v + d_t]   (Win FP32 values of BF16 weights)
+// a_in[v][co] = sum_t Win[co][t] x[v + d_t]   (Win FP32 values of BF16
+// weights). One thread per (voxel, channel): the 16 threads of a voxel share
+// its 27 inputs through the cache and write 64 contiguous bytes.
 __global__ __launch_bounds__(kBlock) void k_conv_in(Grid3 g, const float* __restrict__ x, const float* __restrict__ Win,
-                                                    float inv_s_host, const double* __restrict__ rr_partials, long long nrms,
                                                     float* __restrict__ out32, __bf16* __restrict__ out16,
                                                     const SolveScalars* __restrict__ sc) {
   if (sc && sc->stop) return;
-  __shared__ float w[27 * 16];   // [t][co]
-  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i % 16) * 32 + i / 16];
-  float inv_s = inv_s_host;
-  if (rr_partials) {   // s = rms of the residual (nrms entries) from the solver's r.r partials
-    double rr[1];
-    read_partials<1>(rr_partials, rr);
-    inv_s = rr[0] > 0.0 ? (float)(1.0 / std::sqrt(rr[0] / (double)nrms)) : 0.0f;
-  }
+  __shared__ float w[27 * 16];   // [t][co]: the 16 lanes of a voxel read 16 consecutive words
+  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i & 15) * 32 + (i >> 4)];
   __syncthreads();
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= g.n()) return;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.n() * 16) return;
+  const int c = (int)(idx & 15);
+  const long long v = idx >> 4;
   const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
-  float acc[16];
+  float acc = 0.f;
 #pragma unroll
-  for (int c = 0; c < 16; c++) acc[c] = 0.f;
   for (int t = 0; t < 27; t++) {
     const long long s = tap_input<0>(i, j, k, t, g);
-    if (s < 0) continue;
-    const float xv = x[s] * inv_s;
-#pragma unroll
-    for (int c = 0; c < 16; c++) acc[c] += w[t * 16 + c] * xv;
+    if (s >= 0) acc += w[t * 16 + c] * x[s];
   }
-#pragma unroll
-  for (int c = 0; c < 16; c++) {
-    out32[v * 16 + c] = acc[c];
-    if (out16) out16[v * 16 + c] = to_bf16(acc[c]);
-  }
+  out32[idx] = acc;
+  if (out16) out16[idx] = to_bf16(acc);
 }
 
 // transpose of k_conv_in, scaled and masked:
-// v[o] = gain * s * sum_t sum_c Win[c][t] g[o - d_t][c], 0 at the boundary of g
+// v[o] = gain * sum_t sum_c Win[c][t] g[o - d_t][c], 0 at the boundary of g.
+// 16 lanes per voxel (one per channel), summed with a shuffle tree.
 __global__ __launch_bounds__(kBlock) void k_conv_in_T(Grid3 g, const float* __restrict__ a, const float* __restrict__ Win,
-                                                      float s_host, const double* __restrict__ rr_partials, long long nrms,
                                                       float gain, float* __restrict__ y, const SolveScalars* __restrict__ sc) {
   if (sc && sc->stop) return;
-  __shared__ float w[27 * 16];   // [t][c]
-  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i % 16) * 32 + i / 16];
-  float s = s_host;
-  if (rr_partials) {
-    double rr[1];
-    read_partials<1>(rr_partials, rr);
-    s = (float)std::sqrt(rr[0] / (double)nrms);
-  }
+  __shared__ float w[27 * 16];   // [t][c], conflict-free across the 16 lanes of a voxel
+  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i & 15) * 32 + (i >> 4)];
   __syncthreads();
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= g.n()) return;
-  if (on_boundary(v, g)) { y[v] = 0.0f; return; }
-  const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long v = idx >> 4;
+  const int c = (int)(idx & 15);
+  const bool live = v < g.n() && !on_boundary(v < g.n() ? v : 0, g);
   float acc = 0.f;
-  for (int t = 0; t < 27; t++) {
-    const long long src = tap_input<1>(i, j, k, t, g);
-    if (src < 0) continue;
-    const float4* ap = reinterpret_cast<const float4*>(a + src * 16);
+  if (live) {
+    const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const float4 av = ap[q];
-      acc += w[t * 16 + 4 * q] * av.x + w[t * 16 + 4 * q + 1] * av.y + w[t * 16 + 4 * q + 2] * av.z + w[t * 16 + 4 * q + 3] * av.w;
+    for (int t = 0; t < 27; t++) {
+      const long long src = tap_input<1>(i, j, k, t, g);
+      if (src >= 0) acc += w[t * 16 + c] * a[src * 16 + c];
     }
   }
-  y[v] = gain * s * acc;
+  acc = group_sumf<16>(acc);
+  if (c == 0 && v < g.n()) y[v] = live ? gain * acc : 0.0f;
 }
 
 // ------------------------------------------------------------ up-sampling
 // fine[f][c] = beta*fine + skip[f][c] + sum of node-aligned trilinear weights * coarse
+// node-aligned trilinear value of channel c of a C-channel coarse field at fine node f
 template <int C>
-__global__ __launch_bounds__(kBlock) void k_upsample(Grid3 gc, Grid3 gf, const float* __restrict__ coarse,
-                                                     const float* __restrict__ skip, float* __restrict__ out32,
-                                                     __bf16* __restrict__ out16, const SolveScalars* __restrict__ sc) {
-  if (sc && sc->stop) return;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= gf.n() * C) return;
-  const int c = (int)(idx % C);
-  const long long f = idx / C;
+__device__ __forceinline__ float prolong_at(const Grid3& gc, const Grid3& gf, const float* __restrict__ coarse, long long f,
+                                            int c) {
   const int i = (int)(f % gf.nx), j = (int)((f / gf.nx) % gf.ny), k = (int)(f / ((long long)gf.nx * gf.ny));
   const int i0 = i >> 1, i1 = (i + 1) >> 1, j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
   const float wi = (i & 1) ? 0.5f : 1.0f, wj = (j & 1) ? 0.5f : 1.0f, wk = (k & 1) ? 0.5f : 1.0f;
@@ -252,8 +230,19 @@ __global__ __launch_bounds__(kBlock) void k_upsample(Grid3 gc, Grid3 gf, const f
     if (j & 1) { float v4 = at(i0, j1, k1); if (i & 1) v4 += at(i1, j1, k1); s3 += wi * v4; }
     s += wj * s3;
   }
-  s *= wk;
-  const float val = s + (skip ? skip[idx] : 0.0f);
+  return s * wk;
+}
+
+template <int C>
+__global__ __launch_bounds__(kBlock) void k_upsample(Grid3 gc, Grid3 gf, const float* __restrict__ coarse,
+                                                     const float* __restrict__ skip, float* __restrict__ out32,
+                                                     __bf16* __restrict__ out16, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= gf.n() * C) return;
+  const int c = (int)(idx % C);
+  const long long f = idx / C;
+  const float val = prolong_at<C>(gc, gf, coarse, f, c) + (skip ? skip[idx] : 0.0f);
   out32[idx] = val;
   if (out16) out16[idx] = to_bf16(val);
 }
@@ -292,6 +281,24 @@ __global__ __launch_bounds__(kBlock) void k_upsample_adj(Grid3 gf, Grid3 gc, con
   }
   out32[idx] = s;
   if (out16) out16[idx] = to_bf16(s);
+}
+
+// z = dinv r + P v (the network's coarse output prolonged, fused); r.z and
+// z.q partials (slots 3, 4) of the flexible CG
+__global__ __launch_bounds__(kBlock) void k_prolong_z(Grid3 gc, Grid3 gf, const float* __restrict__ v,
+                                                      const float* __restrict__ r, const float* __restrict__ dinv,
+                                                      const float* __restrict__ q, float* __restrict__ z,
+                                                      double* __restrict__ partials, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;   // sc is null on a start, which always runs
+  double acc[2] = {0.0, 0.0};
+  const long long n = gf.n();
+  for (long long f = blockIdx.x * kBlock + threadIdx.x; f < n; f += kRedGrid * kBlock) {
+    const float zi = dinv[f] * r[f] + prolong_at<1>(gc, gf, v, f, 0);
+    z[f] = zi;
+    acc[0] += (double)r[f] * (double)zi;
+    acc[1] += (double)zi * (double)q[f];
+  }
+  write_partials<2>(acc, partials + 3 * kRedGrid);
 }
 
 // ------------------------------------------------------------ weight gradients
@@ -553,17 +560,20 @@ int refresh_weights(fsx_ctx* ctx, fsx_unet* u) {
 template <class T>
 int ualloc(fsx_ctx* ctx, T** p, size_t n) { return alloc(ctx, p, n); }
 
-// One application on activation set S. x: FP32 on gf; y = gain s P v on gf
-// (skipped when y is null: training only needs the transpose-pass values).
-// s = rms(x) from the r.r partials when given, else inv_s / s as passed.
-int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, float inv_s, float s, const double* rr_partials,
-              const SolveScalars* sc) {
+// One application on activation set S, x FP32 on gf. Output modes:
+// kOutY: y = gain P v = gain J^T J x on gf; kOutZ: the solver's
+// z = dinv x + gain J^T J x with its r.z / z.q partials (fused into the
+// prolongation); kOutNone: stop after the transpose pass (training only
+// needs its values).
+enum { kOutNone = 0, kOutY = 1, kOutZ = 2 };
+
+int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, const SolveScalars* sc, int out = kOutY) {
   const Layer* L = u->L.data();
   const long long nf = u->gf.n(), n0 = u->g0.n();
   hipLaunchKernelGGL(k_upsample_adj<1>, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->gf, u->g0, x, S.u,
                      nullptr, 1, sc);
-  hipLaunchKernelGGL(k_conv_in, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.u, L[0].wr, inv_s,
-                     rr_partials, nf, S.f0, S.f0b, sc);
+  hipLaunchKernelGGL(k_conv_in, dim3(blocks_for(n0 * 16, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.u, L[0].wr,
+                     S.f0, S.f0b, sc);
   FSX_TRY(conv(ctx, 16, 32, 2, u->g0, u->g1, S.f0b, L[1].w16, S.h1, S.h1b, nullptr, 1, sc));
   FSX_TRY(conv(ctx, 32, 32, 0, u->g1, u->g1, S.h1b, L[2].w16, S.f1, S.f1b, nullptr, 0, sc));
   FSX_TRY(conv(ctx, 32, 64, 2, u->g1, u->g2, S.f1b, L[3].w16, S.h3, S.h3b, nullptr, 1, sc));
@@ -573,19 +583,21 @@ int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, floa
   FSX_TRY(conv(ctx, 64, 32, 3, u->g2, u->g1, S.g3b, L[3].wt16, S.g2, S.g2b, S.f1, 0, sc));
   FSX_TRY(conv(ctx, 32, 32, 1, u->g1, u->g1, S.g2b, L[2].wt16, S.g1, S.g1b, nullptr, 1, sc));
   FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 0, sc));
-  if (y) {
-    hipLaunchKernelGGL(k_conv_in_T, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.g0, L[0].wr, s,
-                       rr_partials, nf, u->gain, S.v, sc);
+  if (out != kOutNone) {
+    hipLaunchKernelGGL(k_conv_in_T, dim3(blocks_for(n0 * 16, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.g0, L[0].wr,
+                       u->gain, S.v, sc);
+  }
+  if (out == kOutY) {
     hipLaunchKernelGGL(k_upsample<1>, dim3(blocks_for(nf, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, u->gf, S.v, nullptr,
                        y, nullptr, sc);
+  } else if (out == kOutZ) {
+    hipLaunchKernelGGL(k_prolong_z, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, u->g0, u->gf, S.v, x, ctx->Zdinvf, ctx->fq,
+                       ctx->fz, ctx->partials, sc);
   }
   return launch_check(ctx, "network application");
 }
 
-int forward(fsx_ctx* ctx, fsx_unet* u, const float* x, float* y, float inv_s, float s, const double* rr_partials,
-            const SolveScalars* sc) {
-  return apply_net(ctx, u, u->A, x, y, inv_s, s, rr_partials, sc);
-}
+int forward(fsx_ctx* ctx, fsx_unet* u, const float* x, float* y) { return apply_net(ctx, u, u->A, x, y, nullptr); }
 
 // Parameter gradient of <gy, y> after forward() on set A (accumulates into
 // u->g): with y = gain J^T J x, <gy, y> = gain <J gy, J x>, whose gradient is
@@ -595,7 +607,7 @@ int forward(fsx_ctx* ctx, fsx_unet* u, const float* x, float* y, float inv_s, fl
 // set B (the network applied to gain gy) those of gain gy.
 int backward(fsx_ctx* ctx, fsx_unet* u) {
   hipLaunchKernelGGL(k_scale_f, dim3(256), dim3(kBlock), 0, ctx->stream, (long long)u->gf.n(), u->gy, u->gain);
-  FSX_TRY(apply_net(ctx, u, u->B, u->gy, nullptr, 1.0f, 1.0f, nullptr, nullptr));
+  FSX_TRY(apply_net(ctx, u, u->B, u->gy, nullptr, nullptr, kOutNone));
   const Layer* L = u->L.data();
   auto wg = [&](int li, const float* inA, const float* gB, const float* inB, const float* gA) -> int {
     const Layer& l = L[li];
@@ -719,7 +731,7 @@ int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fs
 // next (BF16 rounding, a scale taken from the residual) and when a reliable
 // update replaces r. Partial-sum slots (kRedGrid doubles each):
 //   0 p.q, 1 r.p (k_fpcg_dot); 2 r.r (k_fpcg_update, k_fpcg_load_r);
-//   3 r.z, 4 z.q (k_fpcg_z).
+//   3 r.z, 4 z.q (k_prolong_z).
 namespace fsx {
 
 constexpr int kSlotRR = 2;
@@ -755,22 +767,6 @@ __global__ __launch_bounds__(kBlock) void k_fpcg_update(int j, int n, float* __r
     v[0] += (double)ri * (double)ri;
   }
   write_partials<1>(v, partials + kSlotRR * kRedGrid);
-}
-
-// z = dinv r + y (Jacobi plus the network's correction); r.z and z.q partials
-__global__ __launch_bounds__(kBlock) void k_fpcg_z(int n, const float* __restrict__ r, const float* __restrict__ dinv,
-                                                   const float* __restrict__ y, const float* __restrict__ q,
-                                                   float* __restrict__ z, double* __restrict__ partials,
-                                                   const SolveScalars* __restrict__ sc) {
-  if (sc && sc->stop) return;   // sc is null on a start, which always runs
-  double v[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
-    const float zi = dinv[i] * r[i] + y[i];
-    z[i] = zi;
-    v[0] += (double)r[i] * (double)zi;
-    v[1] += (double)zi * (double)q[i];
-  }
-  write_partials<2>(v, partials + 3 * kRedGrid);
 }
 
 // p = z - (z.q / p.q) p; stop tests (block 0): 3 NaN/Inf, 6 loss of descent
@@ -863,11 +859,13 @@ __global__ __launch_bounds__(kBlock) void k_make_sample(int n, const double* __r
 
 namespace {
 
-// z = dinv r + U(r) for the current FP32 residual r (its r.r in slot 2)
+// z = dinv r + gain J^T J r for the current FP32 residual r, with the r.z
+// and z.q partials. J^T J is linear, so this equals gain s J^T J (r / s) for
+// any s; the residual is not rescaled here (BF16 has the FP32 exponent
+// range, so a residual reduced by 1e-8 stays far from underflow), while the
+// training pairs are normalised by rms(r).
 int precondition(fsx_ctx* ctx, fsx_unet* u, const SolveScalars* sc) {
-  FSX_TRY(forward(ctx, u, ctx->fr, u->y, 0.f, 0.f, ctx->partials + kSlotRR * kRedGrid, sc));
-  hipLaunchKernelGGL(k_fpcg_z, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->fr, ctx->Zdinvf, u->y, ctx->fq,
-                     ctx->fz, ctx->partials, sc);
+  FSX_TRY(apply_net(ctx, u, u->A, ctx->fr, nullptr, sc, kOutZ));
   return launch_check(ctx, "precondition");
 }
 
@@ -1035,7 +1033,7 @@ void spmv_f(fsx_ctx* ctx, const float* x, float* y) {
 // energy-norm loss (y - t)' Z (y - t) / tt with tt = t' Z t.
 int train_sample(fsx_ctx* ctx, fsx_unet* u, int kind, const float* x, const float* t, double tt, double* loss) {
   const int n = ctx->NNp;
-  FSX_TRY(forward(ctx, u, x, u->y, 1.0f, 1.0f, nullptr, nullptr));
+  FSX_TRY(forward(ctx, u, x, u->y));
   if (kind == 0) {
     hipLaunchKernelGGL(k_loss, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, u->y, t, tt, u->gy, ctx->partials);
   } else {
@@ -1102,7 +1100,7 @@ int fsx_unet_apply(fsx_ctx* ctx, const float* x, float* y) {
   fsx_unet* u = ctx->unet;
   const size_t n = ctx->NNp;
   FSX_HIP(hipMemcpyAsync(u->x, x, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  FSX_TRY(forward(ctx, u, u->x, u->y, 1.0f, 1.0f, nullptr, nullptr));
+  FSX_TRY(forward(ctx, u, u->x, u->y));
   FSX_HIP(hipMemcpyAsync(y, u->y, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   return FSX_OK;

@@ -32,7 +32,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_copy", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_copy", "fsx_z_patterns", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
@@ -106,6 +106,7 @@ def lib():
         "fsx_spmv": ([_vp, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_bench_copy": ([_vp, ctypes.c_size_t, ctypes.c_int, _D], ctypes.c_int),
+        "fsx_z_patterns": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "fsx_timing_reset": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_timing_get": ([_vp, ctypes.POINTER(Timing)], ctypes.c_int),
         "fsx_unet_setup": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64], ctypes.c_int),
@@ -362,6 +363,12 @@ class Solver:
         by = ctypes.c_double()
         self.check(self.L.fsx_bench_spmv(self.ctx, precision, reps, ctypes.byref(ms), ctypes.byref(by)), "bench_spmv")
         return ms.value, by.value
+
+    def z_patterns(self):
+        n = ctypes.c_int()
+        ln = ctypes.c_longlong()
+        self.check(self.L.fsx_z_patterns(self.ctx, ctypes.byref(n), ctypes.byref(ln)), "z_patterns")
+        return n.value, ln.value
 
     def bench_copy(self, nbytes, reps=20):
         ms = ctypes.c_double()

@@ -173,13 +173,19 @@ int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
 int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, double beta);
 
 /* Times `reps` back-to-back launches of the pressure solver's SpMV kernel
- * (q = Z p with its fused p.q reduction, the dominant kernel of a step) on the
- * context's stream with HIP events. precision 0 = FP64 Z, 1 = FP32 Z (the
- * mixed-precision inner solve). Returns the mean device time per launch in
- * ms and the algorithmic HBM bytes of one launch: values + column indices +
- * row pointers + one read of p + one write of q. No reference counterpart
- * (the reference prints wall times only, BCH:4489-4518). */
+ * (q = Z p, the dominant kernel of a step) on the context's stream with HIP
+ * events. precision 0 = FP64 Z, 1 = FP32 Z (the mixed-precision and learned
+ * inner solves). Returns the mean device time per launch in ms and the
+ * algorithmic HBM bytes of one launch: values + column indices (or the
+ * row-pattern ids and dictionary) + row pointers + one read of p + one write
+ * of q. No reference counterpart (the reference prints wall times only,
+ * BCH:4489-4518). */
 int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch, double* bytes_per_launch);
+/* Size of the dictionary of Z row patterns (distinct column-offset lists)
+ * that the Z SpMVs read instead of a column index per entry; 0 patterns when
+ * Z did not compress and the SpMVs read the CSR columns. No reference
+ * counterpart (the reference's Z is plain CSR, CU:281-411). */
+int fsx_z_patterns(fsx_ctx* ctx, int* npatterns, long long* offsets);
 /* Times `reps` launches of a plain device copy of `bytes` (read once, written
  * once) between two scratch buffers: the HBM bandwidth reference and the
  * known-traffic kernel that calibrates PMC byte counters. No reference
