@@ -56,21 +56,25 @@ __device__ __forceinline__ void block_sum_all(double (&v)[N], double* red) {
 // fixed order, so all blocks agree bit for bit and runs are reproducible.
 constexpr int kRedGrid = 256;
 
-template <int N>
+// BS: the block size the kernel is launched with (the LDS scratch is sized
+// for BS / 64 wavefronts).
+template <int N, int BS = kBlock>
 __device__ __forceinline__ void write_partials(double (&v)[N], double* partials) {
-  __shared__ double red[N * (kBlock / kWave)];
+  __shared__ double red[N * (BS / kWave)];
   block_sum_all<N>(v, red);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int n = 0; n < N; n++) partials[n * kRedGrid + blockIdx.x] = v[n];
 }
 
-// partials[n * kRedGrid + b], b < kRedGrid, summed in a fixed order; blockDim.x == kRedGrid.
-template <int N>
+// partials[n * kRedGrid + b], b < kRedGrid, summed in a fixed order by a
+// block of BS >= kRedGrid threads (threads past kRedGrid add exact zeros, so
+// the sum does not depend on BS).
+template <int N, int BS = kBlock>
 __device__ __forceinline__ void read_partials(const double* partials, double (&out)[N]) {
-  __shared__ double red[N * (kBlock / kWave)];
+  __shared__ double red[N * (BS / kWave)];
 #pragma unroll
-  for (int n = 0; n < N; n++) out[n] = partials[n * kRedGrid + threadIdx.x];
+  for (int n = 0; n < N; n++) out[n] = threadIdx.x < kRedGrid ? partials[n * kRedGrid + threadIdx.x] : 0.0;
   block_sum_all<N>(out, red);
 }
 

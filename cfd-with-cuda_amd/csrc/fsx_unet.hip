@@ -4,27 +4,26 @@
 // produced by the solver itself.
 //
 // Preconditioner z = dinv r + gain J^T J r (linear, so M(cr) = c M(r); the
-// training pairs are normalised by rms(r)). J maps the pressure grid gf to features on three coarser
-// node-aligned grids (G0 = every other node of gf, G1 of G0, G2 of G1;
-// 65^3 -> 33^3 -> 17^3 -> 9^3):
-//   u  = B0 R x            full-weighting restriction gf -> G0      (1 ch)
-//   f0 = C0 u              conv 1 -> 16 at G0                       (VALU)
-//   h1 = B1 C1 f0          conv_s2 16 -> 32, G0 -> G1               (MFMA)
-//   f1 = C2 h1             conv 32 -> 32 at G1
-//   h3 = B2 C3 f1          conv_s2 32 -> 64, G1 -> G2
-//   f2 = C4 h3             conv 64 -> 64 at G2
-// and J^T J x = R^T B0 C0^T (f0 + C1^T B1 C2^T (f1 + C3^T B2 C4^T f2)), the
-// transpose pass running the same convolutions with transposed weights
-// (a U-net whose decoder is tied to its encoder). B_l zeroes the boundary
-// nodes of grid l: the pressure operator penalises boundary values (its
-// mass term), so the slow modes that the coarse levels must carry vanish
-// there. The correction is symmetric positive semidefinite by construction,
-// so M is SPD and the flexible CG keeps descent. With one channel, identity
-// and 1/sqrt(2) x full-weighting weights reproduce the additive multilevel
-// (BPX) preconditioner sum_l 2^-(l-1) P_l P_l^T; the network starts there
-// and is trained from it. Convolutions are 3x3x3, linear, without bias; weights
-// are BF16 for the MFMA operands with FP32 master copies; activations are
-// FP32 and rounded to BF16 where they feed an MFMA; accumulation is FP32.
+// training pairs are normalised by rms(r)). J maps the pressure grid gf to
+// features on three coarser node-aligned grids (G0 = every other node of gf,
+// G1 of G0, G2 of G1; 65^3 -> 33^3 -> 17^3 -> 9^3), one stride-2 3x3x3
+// convolution per level:
+//   f0 = B0 C0 r           gf -> G0, 1 -> 16 channels       (VALU)
+//   f1 = B1 C1 f0          G0 -> G1, 16 -> 32               (MFMA)
+//   f2 = B2 C2 f1          G1 -> G2, 32 -> 64               (MFMA)
+// and J^T J r = C0^T B0 (f0 + C1^T B1 (f1 + C2^T f2)), the transpose pass
+// running the same convolutions with transposed weights (a U-net whose
+// decoder is tied to its encoder; the last step is fused with z). B_l zeroes
+// the boundary nodes of grid l: the pressure operator penalises boundary
+// values (its mass term), so the slow modes that the coarse levels must
+// carry vanish there. The correction is symmetric positive semidefinite by
+// construction, so M is SPD and the flexible CG keeps descent. With one
+// channel, full-weighting weights (C0) and 1/sqrt(2) x full weighting (C1,
+// C2) reproduce the additive multilevel (BPX) preconditioner
+// sum_l 2^-(l-1) P_l P_l^T; the network starts there and is trained from it.
+// Convolutions are linear, without bias; weights are BF16 for the MFMA
+// operands with FP32 master copies; activations are FP32 and rounded to BF16
+// where they feed an MFMA; accumulation is FP32.
 //
 // MFMA mapping (v_mfma_f32_16x16x32_bf16, layout checked on gfx950): a conv
 // is an implicit GEMM out[voxel][co] = sum_k A[voxel][k] W[k][co] with
@@ -48,6 +47,15 @@ struct Grid3 {
   int nx, ny, nz;
   __host__ __device__ long long n() const { return (long long)nx * ny * nz; }
 };
+
+// (i, j, k) of node v, x fastest. The grids hold far fewer than 2^31 nodes,
+// so two 32-bit divisions do (64-bit integer division is a long emulated
+// sequence on the GPU).
+__device__ __forceinline__ void node_ijk(long long v, const Grid3& g, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)v, nx = (unsigned)g.nx, nxy = nx * (unsigned)g.ny;
+  const unsigned kk = u / nxy, rem = u - kk * nxy, jj = rem / nx;
+  k = (int)kk; j = (int)jj; i = (int)(rem - jj * nx);
+}
 
 __device__ __forceinline__ __bf16 to_bf16(float f) {
   // round to nearest even (finite inputs)
@@ -76,11 +84,12 @@ __device__ __forceinline__ long long tap_input(int i, int j, int k, int t, const
     a >>= 1; b >>= 1; c >>= 1;
   }
   if (a < 0 || b < 0 || c < 0 || a >= gi.nx || b >= gi.ny || c >= gi.nz) return -1;
-  return ((long long)c * gi.ny + b) * gi.nx + a;
+  return (long long)(((unsigned)c * (unsigned)gi.ny + (unsigned)b) * (unsigned)gi.nx + (unsigned)a);
 }
 
 __host__ __device__ __forceinline__ bool on_boundary(long long v, const Grid3& g) {
-  const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  int i, j, k;
+  node_ijk(v, g, i, j, k);
   return i == 0 || j == 0 || k == 0 || i == g.nx - 1 || j == g.ny - 1 || k == g.nz - 1;
 }
 
@@ -119,7 +128,8 @@ __global__ __launch_bounds__(kWave * conv_ks<CIN>()) void k_conv_mfma(Grid3 gi, 
   const long long va = vbase + col;   // A-operand row of this lane
   const bool ov = va < nout;
   const long long vv = ov ? va : 0;
-  const int oi = (int)(vv % go.nx), oj = (int)((vv / go.nx) % go.ny), ok = (int)(vv / ((long long)go.nx * go.ny));
+  int oi, oj, ok;
+  node_ijk(vv, go, oi, oj, ok);
   f32x4 acc[NCT];
 #pragma unroll
   for (int ct = 0; ct < NCT; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -157,148 +167,177 @@ __global__ __launch_bounds__(kWave * conv_ks<CIN>()) void k_conv_mfma(Grid3 gi, 
   }
 }
 
-// ------------------------------------------------------------ 1-channel layers (VALU)
-// a_in[v][co] = sum_t Win[co][t] x[v + d_t]   (Win FP32 values of BF16
-// weights). One thread per (voxel, channel): the 16 threads of a voxel share
-// its 27 inputs through the cache and write 64 contiguous bytes.
-__global__ __launch_bounds__(kBlock) void k_conv_in(Grid3 g, const float* __restrict__ x, const float* __restrict__ Win,
-                                                    float* __restrict__ out32, __bf16* __restrict__ out16,
-                                                    const SolveScalars* __restrict__ sc) {
+// Transposed stride-2 convolution (the MODE 3 form of k_conv_mfma), tiled by
+// output parity class: output node (i, j, k) receives only the taps with
+// d == parity per axis (d = 0 at an even index, d = +-1 at an odd one), so a
+// block owns 16 nodes of one class (blockIdx.y) and walks only that class's
+// 1 to 8 taps instead of all 27. Same epilogue as k_conv_mfma.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(kWave * 4) void k_conv_mfma_t(Grid3 gi, Grid3 go, const __bf16* __restrict__ in,
+                                                           const __bf16* __restrict__ W, float* __restrict__ out32,
+                                                           __bf16* __restrict__ out16, const float* __restrict__ res,
+                                                           int mask, const SolveScalars* __restrict__ sc) {
   if (sc && sc->stop) return;
-  __shared__ float w[27 * 16];   // [t][co]: the 16 lanes of a voxel read 16 consecutive words
-  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i & 15) * 32 + (i >> 4)];
+  constexpr int KS = 4;
+  constexpr int KP = (27 * CIN + 31) / 32 * 32;
+  constexpr int NCT = COUT / 16;
+  constexpr int RS = COUT + 1;
+  __shared__ float red[KS][kConvVox][RS];
+  const int cls = blockIdx.y;
+  const int pi = cls & 1, pj = (cls >> 1) & 1, pk = cls >> 2;
+  // nodes of this class: i = 2 ic + pi, ic < cx, ...
+  const int cx = (go.nx - pi + 1) >> 1, cy = (go.ny - pj + 1) >> 1, cz = (go.nz - pk + 1) >> 1;
+  const long long ncls = (long long)cx * cy * cz;
+  const long long vbase = (long long)blockIdx.x * kConvVox;
+  if (vbase >= ncls) return;   // uniform over the block
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane >> 4, col = lane & 15;
+  const int ntx = 1 + pi, nty = 1 + pj, ntz = 1 + pk;
+  const int nch = ntx * nty * ntz * CIN / 32;
+  // this lane's A-operand row
+  const long long va = vbase + col;
+  const bool ov = va < ncls;
+  int oi = 0, oj = 0, ok = 0;
+  if (ov) {
+    const unsigned u = (unsigned)va, cxy = (unsigned)cx * (unsigned)cy;
+    const unsigned kc = u / cxy, rem = u - kc * cxy, jc = rem / (unsigned)cx, ic = rem - jc * (unsigned)cx;
+    oi = 2 * (int)ic + pi; oj = 2 * (int)jc + pj; ok = 2 * (int)kc + pk;
+  }
+  f32x4 acc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = {};
+#pragma unroll 2
+  for (int c = wave; c < nch; c += KS) {
+    const int kl = c * 32 + grp * 8;           // k within this class's taps
+    const int tl = kl / CIN, ci0 = kl % CIN;   // class-local tap, channel
+    const int ax = tl % ntx, ay = (tl / ntx) % nty, az = tl / (ntx * nty);
+    const int dx = pi ? (ax ? -1 : 1) : 0, dy = pj ? (ay ? -1 : 1) : 0, dz = pk ? (az ? -1 : 1) : 0;
+    const int t = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+    bf16x8 a = zero8;
+    if (ov) {
+      const int sx = (oi - dx) >> 1, sy = (oj - dy) >> 1, sz = (ok - dz) >> 1;
+      if (sx >= 0 && sy >= 0 && sz >= 0 && sx < gi.nx && sy < gi.ny && sz < gi.nz) {
+        const unsigned src = ((unsigned)sz * (unsigned)gi.ny + (unsigned)sy) * (unsigned)gi.nx + (unsigned)sx;
+        a = *reinterpret_cast<const bf16x8*>(in + (size_t)src * CIN + ci0);
+      }
+    }
+    const int kg = t * CIN + ci0;              // k in the weight layout
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + (size_t)(ct * 16 + col) * KP + kg);
+      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[ct], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) red[wave][grp * 4 + r][ct * 16 + col] = acc[ct][r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < kConvVox * COUT; e += KS * kWave) {
+    const int row = e / COUT, co = e % COUT;
+    const long long vl = vbase + row;
+    if (vl >= ncls) continue;
+    const unsigned u = (unsigned)vl, cxy = (unsigned)cx * (unsigned)cy;
+    const unsigned kc = u / cxy, rem = u - kc * cxy, jc = rem / (unsigned)cx, ic = rem - jc * (unsigned)cx;
+    const long long v = ((long long)(2 * (int)kc + pk) * go.ny + (2 * (int)jc + pj)) * go.nx + (2 * (int)ic + pi);
+    float val = 0.f;
+#pragma unroll
+    for (int w = 0; w < KS; w++) val += red[w][row][co];
+    const size_t o = (size_t)v * COUT + co;
+    if (res) val += res[o];
+    if (mask && on_boundary(v, go)) val = 0.0f;
+    out32[o] = val;
+    if (out16) out16[o] = to_bf16(val);
+  }
+}
+
+// ------------------------------------------------------------ pressure-grid layers (VALU)
+// f0[o][c] = B0(o) sum_t W0[c][t] x[2o + d_t]: the first layer, a stride-2
+// conv from the pressure grid gf (1 channel) to G0 (16 channels), zero at
+// the boundary nodes of G0; FP32 and a BF16 copy for the next (MFMA) layer.
+// One thread per (node, channel); weights in LDS as [t][c], so the 16 lanes
+// of a node read 16 consecutive words.
+__global__ __launch_bounds__(kBlock) void k_conv0(Grid3 gf, Grid3 g0, const float* __restrict__ x,
+                                                  const float* __restrict__ W0, float* __restrict__ out32,
+                                                  __bf16* __restrict__ out16, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  __shared__ float w[27 * 16];
+  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = W0[(i & 15) * 32 + (i >> 4)];
   __syncthreads();
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= g.n() * 16) return;
+  if (idx >= g0.n() * 16) return;
   const int c = (int)(idx & 15);
   const long long v = idx >> 4;
-  const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  int i, j, k;
+  node_ijk(v, g0, i, j, k);
   float acc = 0.f;
+  if (!(i == 0 || j == 0 || k == 0 || i == g0.nx - 1 || j == g0.ny - 1 || k == g0.nz - 1)) {
 #pragma unroll
-  for (int t = 0; t < 27; t++) {
-    const long long s = tap_input<0>(i, j, k, t, g);
-    if (s >= 0) acc += w[t * 16 + c] * x[s];
+    for (int t = 0; t < 27; t++) {
+      const long long s = tap_input<2>(i, j, k, t, gf);
+      if (s >= 0) acc += w[t * 16 + c] * x[s];
+    }
   }
   out32[idx] = acc;
   if (out16) out16[idx] = to_bf16(acc);
 }
 
-// transpose of k_conv_in, scaled and masked:
-// v[o] = gain * sum_t sum_c Win[c][t] g[o - d_t][c], 0 at the boundary of g.
-// 16 lanes per voxel (one per channel), summed with a shuffle tree.
-__global__ __launch_bounds__(kBlock) void k_conv_in_T(Grid3 g, const float* __restrict__ a, const float* __restrict__ Win,
-                                                      float gain, float* __restrict__ y, const SolveScalars* __restrict__ sc) {
-  if (sc && sc->stop) return;
-  __shared__ float w[27 * 16];   // [t][c], conflict-free across the 16 lanes of a voxel
-  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = Win[(i & 15) * 32 + (i >> 4)];
+// The last layer, fused with the solver's use of it: for each pressure node
+// f, y = gain sum_{o,t: 2o + d_t = f} sum_c W0[c][t] g0[o][c] (the transpose
+// of k_conv0; g0 is already zero on the boundary of G0). With z set:
+// z = dinv r + y and the r.z, z.q partials (slots 3, 4) of the flexible CG;
+// otherwise y is stored. Grid: kRedGrid blocks of kWideBlock threads,
+// grid-stride (about one node per thread on 65^3).
+constexpr int kWideBlock = 1024;
+
+__global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const float* __restrict__ g0v,
+                                                   const float* __restrict__ W0, float gain, float* __restrict__ y,
+                                                   const float* __restrict__ r, const float* __restrict__ dinv,
+                                                   const float* __restrict__ q, float* __restrict__ z,
+                                                   double* __restrict__ partials, const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;   // sc is null on a start, which always runs
+  __shared__ float w[27 * 16];
+  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = W0[(i & 15) * 32 + (i >> 4)];
   __syncthreads();
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long v = idx >> 4;
-  const int c = (int)(idx & 15);
-  const bool live = v < g.n() && !on_boundary(v < g.n() ? v : 0, g);
-  float acc = 0.f;
-  if (live) {
-    const int i = (int)(v % g.nx), j = (int)((v / g.nx) % g.ny), k = (int)(v / ((long long)g.nx * g.ny));
+  double acc2[2] = {0.0, 0.0};
+  const long long n = gf.n();
+  for (long long f = blockIdx.x * kWideBlock + threadIdx.x; f < n; f += (long long)kRedGrid * kWideBlock) {
+    int i, j, k;
+    node_ijk(f, gf, i, j, k);
+    // coarse parents o = (f - d) / 2 per axis: one (d = 0) at an even index,
+    // two (d = +1, -1) at an odd one; at most 8 taps in all
+    const int ni = 1 + (i & 1), nj = 1 + (j & 1), nk = 1 + (k & 1);
+    float acc = 0.f;
+    for (int c = 0; c < nk; c++) {
+      const int dz = (k & 1) ? (c ? -1 : 1) : 0, oz = (k - dz) >> 1;
+      for (int b = 0; b < nj; b++) {
+        const int dy = (j & 1) ? (b ? -1 : 1) : 0, oy = (j - dy) >> 1;
+        for (int a = 0; a < ni; a++) {
+          const int dx = (i & 1) ? (a ? -1 : 1) : 0, ox = (i - dx) >> 1;
+          const int t = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+          const unsigned o = ((unsigned)oz * (unsigned)g0.ny + (unsigned)oy) * (unsigned)g0.nx + (unsigned)ox;
+          const float4* gp = reinterpret_cast<const float4*>(g0v + (size_t)o * 16);
+          const float* wt = w + t * 16;
 #pragma unroll
-    for (int t = 0; t < 27; t++) {
-      const long long src = tap_input<1>(i, j, k, t, g);
-      if (src >= 0) acc += w[t * 16 + c] * a[src * 16 + c];
-    }
-  }
-  acc = group_sumf<16>(acc);
-  if (c == 0 && v < g.n()) y[v] = live ? gain * acc : 0.0f;
-}
-
-// ------------------------------------------------------------ up-sampling
-// fine[f][c] = beta*fine + skip[f][c] + sum of node-aligned trilinear weights * coarse
-// node-aligned trilinear value of channel c of a C-channel coarse field at fine node f
-template <int C>
-__device__ __forceinline__ float prolong_at(const Grid3& gc, const Grid3& gf, const float* __restrict__ coarse, long long f,
-                                            int c) {
-  const int i = (int)(f % gf.nx), j = (int)((f / gf.nx) % gf.ny), k = (int)(f / ((long long)gf.nx * gf.ny));
-  const int i0 = i >> 1, i1 = (i + 1) >> 1, j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
-  const float wi = (i & 1) ? 0.5f : 1.0f, wj = (j & 1) ? 0.5f : 1.0f, wk = (k & 1) ? 0.5f : 1.0f;
-  auto at = [&](int a, int b, int cc) { return coarse[(((long long)cc * gc.ny + b) * gc.nx + a) * C + c]; };
-  float v = at(i0, j0, k0);
-  if (i & 1) v += at(i1, j0, k0);
-  float s = wi * v;
-  if (j & 1) { float v2 = at(i0, j1, k0); if (i & 1) v2 += at(i1, j1, k0); s += wi * v2; }
-  s *= wj;
-  if (k & 1) {
-    float v3 = at(i0, j0, k1); if (i & 1) v3 += at(i1, j0, k1);
-    float s3 = wi * v3;
-    if (j & 1) { float v4 = at(i0, j1, k1); if (i & 1) v4 += at(i1, j1, k1); s3 += wi * v4; }
-    s += wj * s3;
-  }
-  return s * wk;
-}
-
-template <int C>
-__global__ __launch_bounds__(kBlock) void k_upsample(Grid3 gc, Grid3 gf, const float* __restrict__ coarse,
-                                                     const float* __restrict__ skip, float* __restrict__ out32,
-                                                     __bf16* __restrict__ out16, const SolveScalars* __restrict__ sc) {
-  if (sc && sc->stop) return;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= gf.n() * C) return;
-  const int c = (int)(idx % C);
-  const long long f = idx / C;
-  const float val = prolong_at<C>(gc, gf, coarse, f, c) + (skip ? skip[idx] : 0.0f);
-  out32[idx] = val;
-  if (out16) out16[idx] = to_bf16(val);
-}
-
-// adjoint of the up-sampling (restriction): coarse[u][c] = sum over fine
-// neighbours f = 2u + e, e in {-1,0,1}^3, of w(e) fine[f][c], w = prod (1 or 1/2)
-template <int C>
-__global__ __launch_bounds__(kBlock) void k_upsample_adj(Grid3 gf, Grid3 gc, const float* __restrict__ fine,
-                                                         float* __restrict__ out32, __bf16* __restrict__ out16, int mask,
-                                                         const SolveScalars* __restrict__ sc) {
-  if (sc && sc->stop) return;
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= gc.n() * C) return;
-  const int c = (int)(idx % C);
-  const long long u = idx / C;
-  if (mask && on_boundary(u, gc)) {
-    out32[idx] = 0.0f;
-    if (out16) out16[idx] = to_bf16(0.0f);
-    return;
-  }
-  const int i = (int)(u % gc.nx), j = (int)((u / gc.nx) % gc.ny), k = (int)(u / ((long long)gc.nx * gc.ny));
-  float s = 0.f;
-  for (int dz = -1; dz <= 1; dz++) {
-    const int fz = 2 * k + dz;
-    if (fz < 0 || fz >= gf.nz) continue;
-    for (int dy = -1; dy <= 1; dy++) {
-      const int fy = 2 * j + dy;
-      if (fy < 0 || fy >= gf.ny) continue;
-      for (int dx = -1; dx <= 1; dx++) {
-        const int fx = 2 * i + dx;
-        if (fx < 0 || fx >= gf.nx) continue;
-        const float w = (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f);
-        s += w * fine[(((long long)fz * gf.ny + fy) * gf.nx + fx) * C + c];
+          for (int qd = 0; qd < 4; qd++) {
+            const float4 gv = gp[qd];
+            acc += wt[4 * qd] * gv.x + wt[4 * qd + 1] * gv.y + wt[4 * qd + 2] * gv.z + wt[4 * qd + 3] * gv.w;
+          }
+        }
       }
     }
+    const float yv = gain * acc;
+    if (z) {
+      const float zi = dinv[f] * r[f] + yv;
+      z[f] = zi;
+      acc2[0] += (double)r[f] * (double)zi;
+      acc2[1] += (double)zi * (double)q[f];
+    } else {
+      y[f] = yv;
+    }
   }
-  out32[idx] = s;
-  if (out16) out16[idx] = to_bf16(s);
-}
-
-// z = dinv r + P v (the network's coarse output prolonged, fused); r.z and
-// z.q partials (slots 3, 4) of the flexible CG
-__global__ __launch_bounds__(kBlock) void k_prolong_z(Grid3 gc, Grid3 gf, const float* __restrict__ v,
-                                                      const float* __restrict__ r, const float* __restrict__ dinv,
-                                                      const float* __restrict__ q, float* __restrict__ z,
-                                                      double* __restrict__ partials, const SolveScalars* __restrict__ sc) {
-  if (sc && sc->stop) return;   // sc is null on a start, which always runs
-  double acc[2] = {0.0, 0.0};
-  const long long n = gf.n();
-  for (long long f = blockIdx.x * kBlock + threadIdx.x; f < n; f += kRedGrid * kBlock) {
-    const float zi = dinv[f] * r[f] + prolong_at<1>(gc, gf, v, f, 0);
-    z[f] = zi;
-    acc[0] += (double)r[f] * (double)zi;
-    acc[1] += (double)zi * (double)q[f];
-  }
-  write_partials<2>(acc, partials + 3 * kRedGrid);
+  if (z) write_partials<2, kWideBlock>(acc2, partials + 3 * kRedGrid);
 }
 
 // ------------------------------------------------------------ weight gradients
@@ -329,7 +368,8 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(Grid3 gi, Grid3 go, const floa
       const long long v = vb + b;
       float val = 0.f;
       if (v < v1) {
-        const int i = (int)(v % go.nx), j = (int)((v / go.nx) % go.ny), k = (int)(v / ((long long)go.nx * go.ny));
+        int i, j, k;
+  node_ijk(v, go, i, j, k);
         const long long src = tap_input<MODE>(i, j, k, t, gi);
         if (src >= 0) val = in[src * CIN + ci];
       }
@@ -479,16 +519,13 @@ struct Layer {
   float* wr = nullptr;      // BF16-rounded FP32 copy (VALU layers)
 };
 
-// Activations of one application of J^T J: encoder values (u, f0, h1, f1,
-// h3, f2) and transpose-pass values (g3 = B2 C4^T f2, g2 = f1 + C3^T g3,
-// g1 = B1 C2^T g2, g0 = f0 + C1^T g1, v = B0 C0^T g0 scaled). The transpose
-// values are the gradients of <J x, G> with respect to the encoder outputs
-// for G = (f0, f1, f2), which the weight gradient uses.
+// Activations of one application of J^T J: encoder values f0, f1, f2 and the
+// transpose-pass values g1 = B1 (f1 + C2^T f2), g0 = B0 (f0 + C1^T g1). The
+// transpose values are the gradients of <J x, G> with respect to the encoder
+// outputs for G = (f0, f1, f2), which the weight gradient uses.
 struct Acts {
-  float *u = nullptr, *f0 = nullptr, *h1 = nullptr, *f1 = nullptr, *h3 = nullptr, *f2 = nullptr;
-  float *g3 = nullptr, *g2 = nullptr, *g1 = nullptr, *g0 = nullptr, *v = nullptr;
-  __bf16 *f0b = nullptr, *h1b = nullptr, *f1b = nullptr, *h3b = nullptr, *f2b = nullptr, *g3b = nullptr, *g2b = nullptr,
-         *g1b = nullptr;
+  float *f0 = nullptr, *f1 = nullptr, *f2 = nullptr, *g1 = nullptr, *g0 = nullptr;
+  __bf16 *f0b = nullptr, *f1b = nullptr, *f2b = nullptr, *g1b = nullptr;
 };
 
 }  // namespace
@@ -496,7 +533,7 @@ struct Acts {
 struct fsx_unet {
   Grid3 gf{};               // pressure grid
   Grid3 g0{}, g1{}, g2{};   // coarse levels
-  std::vector<Layer> L;     // C0 .. C4
+  std::vector<Layer> L;     // C0, C1, C2
   size_t nparam = 0;        // padded parameter count (rows of stride kp)
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   float *x = nullptr, *y = nullptr, *tgt = nullptr, *gy = nullptr, *zq = nullptr;   // pressure grid
@@ -522,18 +559,24 @@ unsigned blocks_for(long long n, int per) { return (unsigned)std::max(1LL, (n + 
 template <int CIN, int COUT, int MODE>
 void conv_launch(fsx_ctx* ctx, Grid3 gi, Grid3 go, const __bf16* in, const __bf16* W, float* out32, __bf16* out16,
                  const float* res, int mask, const SolveScalars* sc) {
-  hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, MODE>), dim3(blocks_for(go.n(), kConvVox)), dim3(kWave * conv_ks<CIN>()), 0,
-                     ctx->stream, gi, go, in, W, out32, out16, res, mask, sc);
+  if constexpr (MODE == 3) {
+    // class (0,0,0) is the largest: ceil(n/2) nodes per axis
+    const long long nmax = (long long)((go.nx + 1) / 2) * ((go.ny + 1) / 2) * ((go.nz + 1) / 2);
+    hipLaunchKernelGGL((k_conv_mfma_t<CIN, COUT>), dim3(blocks_for(nmax, kConvVox), 8), dim3(kWave * 4), 0, ctx->stream, gi, go,
+                       in, W, out32, out16, res, mask, sc);
+  } else {
+    hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, MODE>), dim3(blocks_for(go.n(), kConvVox)), dim3(kWave * conv_ks<CIN>()), 0,
+                       ctx->stream, gi, go, in, W, out32, out16, res, mask, sc);
+  }
 }
 
-// dispatch on (cin, cout, mode): the encoder convolutions (modes 0, 2) and
-// their transposes (mode 1 for stride 1, 3 for stride 2, with WT)
+// dispatch on (cin, cout, mode): the encoder convolutions (mode 2, stride 2)
+// and their transposes (mode 3, with WT)
 int conv(fsx_ctx* ctx, int cin, int cout, int mode, Grid3 gi, Grid3 go, const __bf16* in, const __bf16* W, float* out32,
          __bf16* out16, const float* res, int mask, const SolveScalars* sc) {
 #define CONV_CASE(A, B, M) \
   if (cin == A && cout == B && mode == M) { conv_launch<A, B, M>(ctx, gi, go, in, W, out32, out16, res, mask, sc); return FSX_OK; }
-  CONV_CASE(16, 32, 2) CONV_CASE(32, 32, 0) CONV_CASE(32, 64, 2) CONV_CASE(64, 64, 0)
-  CONV_CASE(64, 64, 1) CONV_CASE(64, 32, 3) CONV_CASE(32, 32, 1) CONV_CASE(32, 16, 3)
+  CONV_CASE(16, 32, 2) CONV_CASE(32, 64, 2) CONV_CASE(64, 32, 3) CONV_CASE(32, 16, 3)
 #undef CONV_CASE
   return fail(ctx, FSX_EINVAL, "conv shape not instantiated");
 }
@@ -543,7 +586,7 @@ int wgrad(fsx_ctx* ctx, int cin, int cout, int mode, Grid3 gi, Grid3 go, const f
   if (cin == A && cout == B && mode == M) { \
     hipLaunchKernelGGL((k_wgrad<A, B, M>), dim3(kWChunks, 27), dim3(kBlock), 0, ctx->stream, gi, go, in, gout, part); \
     return FSX_OK; }
-  WG_CASE(1, 16, 0) WG_CASE(16, 32, 2) WG_CASE(32, 32, 0) WG_CASE(32, 64, 2) WG_CASE(64, 64, 0)
+  WG_CASE(1, 16, 2) WG_CASE(16, 32, 2) WG_CASE(32, 64, 2)
 #undef WG_CASE
   return fail(ctx, FSX_EINVAL, "wgrad shape not instantiated");
 }
@@ -561,51 +604,39 @@ template <class T>
 int ualloc(fsx_ctx* ctx, T** p, size_t n) { return alloc(ctx, p, n); }
 
 // One application on activation set S, x FP32 on gf. Output modes:
-// kOutY: y = gain P v = gain J^T J x on gf; kOutZ: the solver's
-// z = dinv x + gain J^T J x with its r.z / z.q partials (fused into the
-// prolongation); kOutNone: stop after the transpose pass (training only
-// needs its values).
+// kOutY: y = gain J^T J x on gf; kOutZ: the solver's z = dinv x + gain J^T J x
+// with its r.z / z.q partials (fused into the last layer); kOutNone: stop
+// after the transpose pass (training only needs its values).
 enum { kOutNone = 0, kOutY = 1, kOutZ = 2 };
 
 int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, const SolveScalars* sc, int out = kOutY) {
   const Layer* L = u->L.data();
-  const long long nf = u->gf.n(), n0 = u->g0.n();
-  hipLaunchKernelGGL(k_upsample_adj<1>, dim3(blocks_for(n0, kBlock)), dim3(kBlock), 0, ctx->stream, u->gf, u->g0, x, S.u,
-                     nullptr, 1, sc);
-  hipLaunchKernelGGL(k_conv_in, dim3(blocks_for(n0 * 16, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.u, L[0].wr,
-                     S.f0, S.f0b, sc);
-  FSX_TRY(conv(ctx, 16, 32, 2, u->g0, u->g1, S.f0b, L[1].w16, S.h1, S.h1b, nullptr, 1, sc));
-  FSX_TRY(conv(ctx, 32, 32, 0, u->g1, u->g1, S.h1b, L[2].w16, S.f1, S.f1b, nullptr, 0, sc));
-  FSX_TRY(conv(ctx, 32, 64, 2, u->g1, u->g2, S.f1b, L[3].w16, S.h3, S.h3b, nullptr, 1, sc));
-  FSX_TRY(conv(ctx, 64, 64, 0, u->g2, u->g2, S.h3b, L[4].w16, S.f2, S.f2b, nullptr, 0, sc));
+  const long long n0 = u->g0.n();
+  hipLaunchKernelGGL(k_conv0, dim3(blocks_for(n0 * 16, kBlock)), dim3(kBlock), 0, ctx->stream, u->gf, u->g0, x, L[0].wr, S.f0,
+                     S.f0b, sc);
+  FSX_TRY(conv(ctx, 16, 32, 2, u->g0, u->g1, S.f0b, L[1].w16, S.f1, S.f1b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 32, 64, 2, u->g1, u->g2, S.f1b, L[2].w16, S.f2, S.f2b, nullptr, 1, sc));
   // transpose pass
-  FSX_TRY(conv(ctx, 64, 64, 1, u->g2, u->g2, S.f2b, L[4].wt16, S.g3, S.g3b, nullptr, 1, sc));
-  FSX_TRY(conv(ctx, 64, 32, 3, u->g2, u->g1, S.g3b, L[3].wt16, S.g2, S.g2b, S.f1, 0, sc));
-  FSX_TRY(conv(ctx, 32, 32, 1, u->g1, u->g1, S.g2b, L[2].wt16, S.g1, S.g1b, nullptr, 1, sc));
-  FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 0, sc));
-  if (out != kOutNone) {
-    hipLaunchKernelGGL(k_conv_in_T, dim3(blocks_for(n0 * 16, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, S.g0, L[0].wr,
-                       u->gain, S.v, sc);
-  }
-  if (out == kOutY) {
-    hipLaunchKernelGGL(k_upsample<1>, dim3(blocks_for(nf, kBlock)), dim3(kBlock), 0, ctx->stream, u->g0, u->gf, S.v, nullptr,
-                       y, nullptr, sc);
-  } else if (out == kOutZ) {
-    hipLaunchKernelGGL(k_prolong_z, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, u->g0, u->gf, S.v, x, ctx->Zdinvf, ctx->fq,
-                       ctx->fz, ctx->partials, sc);
-  }
+  FSX_TRY(conv(ctx, 64, 32, 3, u->g2, u->g1, S.f2b, L[2].wt16, S.g1, S.g1b, S.f1, 1, sc));
+  FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 1, sc));
+  if (out == kOutY)
+    hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, S.g0, L[0].wr, u->gain, y, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, sc);
+  else if (out == kOutZ)
+    hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, S.g0, L[0].wr, u->gain, nullptr,
+                       x, ctx->Zdinvf, ctx->fq, ctx->fz, ctx->partials, sc);
   return launch_check(ctx, "network application");
 }
 
 int forward(fsx_ctx* ctx, fsx_unet* u, const float* x, float* y) { return apply_net(ctx, u, u->A, x, y, nullptr); }
 
-// Parameter gradient of <gy, y> after forward() on set A (accumulates into
-// u->g): with y = gain J^T J x, <gy, y> = gain <J gy, J x>, whose gradient is
-// the encoder backpropagation at input x with output gradients J (gain gy)
-// plus the one at input gain gy with output gradients J x. Both sets of
-// output-side gradients are transpose-pass values: set A holds those of x,
-// set B (the network applied to gain gy) those of gain gy.
-int backward(fsx_ctx* ctx, fsx_unet* u) {
+// Parameter gradient of <gy, y> after forward() of x on set A (accumulates
+// into u->g): with y = gain J^T J x, <gy, y> = gain <J gy, J x>, whose
+// gradient is the encoder backpropagation at input x with output gradients
+// J (gain gy) plus the one at input gain gy with output gradients J x. Both
+// sets of output-side gradients are transpose-pass values: set A holds those
+// of x, set B (the network applied to gain gy) those of gain gy.
+int backward(fsx_ctx* ctx, fsx_unet* u, const float* x) {
   hipLaunchKernelGGL(k_scale_f, dim3(256), dim3(kBlock), 0, ctx->stream, (long long)u->gf.n(), u->gy, u->gain);
   FSX_TRY(apply_net(ctx, u, u->B, u->gy, nullptr, nullptr, kOutNone));
   const Layer* L = u->L.data();
@@ -621,48 +652,38 @@ int backward(fsx_ctx* ctx, fsx_unet* u) {
     return launch_check(ctx, "wgrad");
   };
   Acts &A = u->A, &B = u->B;
-  FSX_TRY(wg(4, A.h3, B.f2, B.h3, A.f2));
-  FSX_TRY(wg(3, A.f1, B.g3, B.f1, A.g3));
-  FSX_TRY(wg(2, A.h1, B.g2, B.h1, A.g2));
+  FSX_TRY(wg(2, A.f1, B.f2, B.f1, A.f2));
   FSX_TRY(wg(1, A.f0, B.g1, B.f0, A.g1));
-  FSX_TRY(wg(0, A.u, B.g0, B.u, A.g0));
+  FSX_TRY(wg(0, x, B.g0, u->gy, A.g0));
   return FSX_OK;
 }
 
 int alloc_acts(fsx_ctx* ctx, fsx_unet* u, Acts& S) {
   const size_t n0 = u->g0.n(), n1 = u->g1.n(), n2 = u->g2.n();
-  FSX_TRY(ualloc(ctx, &S.u, n0)); FSX_TRY(ualloc(ctx, &S.v, n0));
   FSX_TRY(ualloc(ctx, &S.f0, n0 * 16)); FSX_TRY(ualloc(ctx, &S.f0b, n0 * 16)); FSX_TRY(ualloc(ctx, &S.g0, n0 * 16));
-  FSX_TRY(ualloc(ctx, &S.h1, n1 * 32)); FSX_TRY(ualloc(ctx, &S.h1b, n1 * 32));
   FSX_TRY(ualloc(ctx, &S.f1, n1 * 32)); FSX_TRY(ualloc(ctx, &S.f1b, n1 * 32));
-  FSX_TRY(ualloc(ctx, &S.g2, n1 * 32)); FSX_TRY(ualloc(ctx, &S.g2b, n1 * 32));
   FSX_TRY(ualloc(ctx, &S.g1, n1 * 32)); FSX_TRY(ualloc(ctx, &S.g1b, n1 * 32));
-  FSX_TRY(ualloc(ctx, &S.h3, n2 * 64)); FSX_TRY(ualloc(ctx, &S.h3b, n2 * 64));
   FSX_TRY(ualloc(ctx, &S.f2, n2 * 64)); FSX_TRY(ualloc(ctx, &S.f2b, n2 * 64));
-  FSX_TRY(ualloc(ctx, &S.g3, n2 * 64)); FSX_TRY(ualloc(ctx, &S.g3b, n2 * 64));
   return FSX_OK;
 }
 
 // Initial weights: channel 0 of every layer carries the additive multilevel
-// preconditioner (C0, C2, C4: centre tap 1; C1, C3: full weighting scaled by
+// preconditioner (C0: full weighting; C1, C2: full weighting scaled by
 // 1/sqrt(2)), so J^T J = sum_l 2^-(l-1) P_l P_l^T over the three coarse
 // levels; the other entries are small seeded noise (1/10 of a fan-in-scaled
 // normal) so that training can use them; gain = median(dinv) / 4.
 void init_weights(const std::vector<Layer>& L, size_t nparam, unsigned long long seed, std::vector<float>& w) {
   w.assign(nparam, 0.0f);
   std::mt19937_64 rng(seed);
-  const float a = (float)std::sqrt(0.5);
   for (size_t li = 0; li < L.size(); li++) {
     const Layer& l = L[li];
     std::normal_distribution<float> nd(0.0f, 0.1f * (float)std::sqrt(1.0 / (27.0 * l.cin)));
     for (int co = 0; co < l.cout; co++)
       for (int k = 0; k < 27 * l.cin; k++) w[l.off + (size_t)co * l.kp + k] = nd(rng);
+    const float a = li == 0 ? 1.0f : (float)std::sqrt(0.5);
     for (int t = 0; t < 27; t++) {
       const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-      float v;
-      if (l.mode == 2) v = a * (dx ? 0.5f : 1.0f) * (dy ? 0.5f : 1.0f) * (dz ? 0.5f : 1.0f);
-      else v = (t == 13) ? 1.0f : 0.0f;
-      w[l.off + (size_t)t * l.cin] = v;   // co = 0, ci = 0
+      w[l.off + (size_t)t * l.cin] = a * (dx ? 0.5f : 1.0f) * (dy ? 0.5f : 1.0f) * (dz ? 0.5f : 1.0f);   // co = ci = 0
     }
   }
 }
@@ -677,8 +698,7 @@ int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fs
   u->g1 = coarse_of(u->g0);
   u->g2 = coarse_of(u->g1);
   struct Spec { int cin, cout, mode; Grid3 gi, go; };
-  const Spec spec[5] = {{1, 16, 0, u->g0, u->g0}, {16, 32, 2, u->g0, u->g1}, {32, 32, 0, u->g1, u->g1},
-                        {32, 64, 2, u->g1, u->g2}, {64, 64, 0, u->g2, u->g2}};
+  const Spec spec[3] = {{1, 16, 2, u->gf, u->g0}, {16, 32, 2, u->g0, u->g1}, {32, 64, 2, u->g1, u->g2}};
   size_t off = 0;
   for (const Spec& sp : spec) {
     Layer l;
@@ -731,7 +751,7 @@ int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fs
 // next (BF16 rounding, a scale taken from the residual) and when a reliable
 // update replaces r. Partial-sum slots (kRedGrid doubles each):
 //   0 p.q, 1 r.p (k_fpcg_dot); 2 r.r (k_fpcg_update, k_fpcg_load_r);
-//   3 r.z, 4 z.q (k_prolong_z).
+//   3 r.z, 4 z.q (k_conv0T).
 namespace fsx {
 
 constexpr int kSlotRR = 2;
@@ -1042,7 +1062,7 @@ int train_sample(fsx_ctx* ctx, fsx_unet* u, int kind, const float* x, const floa
     hipLaunchKernelGGL(k_eloss, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, u->zq, tt, u->gy, ctx->partials);
   }
   FSX_TRY(launch_check(ctx, "unet loss"));
-  FSX_TRY(backward(ctx, u));
+  FSX_TRY(backward(ctx, u, x));
   if (loss) {
     double d2 = 0;
     FSX_TRY(sum_partials_host(ctx, &d2));

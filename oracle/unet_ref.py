@@ -7,18 +7,15 @@ The reference has no learned preconditioner: its pressure solver is
 Jacobi-PCG only (CU:418-451 CUSP cg with diagonal preconditioner,
 BCH:3981-4055 MKL RCI CG with the Jacobi step). This module restates the
 network the GPU library trains and applies, so that its kernels can be
-checked. The preconditioner is z = dinv r + gain s J^T J (r / s) with
+checked. The preconditioner is z = dinv r + gain J^T J r with
 
-    u  = B0 R x            R full weighting, pressure grid -> G0
-    f0 = C0 u              conv 1 -> 16 at G0
-    h1 = B1 C1 f0          conv 16 -> 32, stride 2, G0 -> G1
-    f1 = C2 h1             conv 32 -> 32 at G1
-    h3 = B2 C3 f1          conv 32 -> 64, stride 2, G1 -> G2
-    f2 = C4 h3             conv 64 -> 64 at G2
-    J^T J x = R^T B0 C0^T (f0 + C1^T B1 C2^T (f1 + C3^T B2 C4^T f2))
+    f0 = B0 C0 x           conv 1 -> 16, stride 2, pressure grid -> G0
+    f1 = B1 C1 f0          conv 16 -> 32, stride 2, G0 -> G1
+    f2 = B2 C2 f1          conv 32 -> 64, stride 2, G1 -> G2
+    J^T J x = C0^T B0 (f0 + C1^T B1 (f1 + C2^T f2))
 
-B_l zeroes the boundary nodes of grid l, R^T is node-aligned trilinear
-prolongation. All convolutions are 3x3x3 with zero padding and no bias;
+B_l zeroes the boundary nodes of grid l. All convolutions are 3x3x3 with
+zero padding and no bias;
 grids are (nz, ny, nx) with x fastest, each coarse grid keeps every other
 node of the one above. Weights of a layer are stored [cout][kp] with
 k = tap * cin + ci, tap = (dx+1) + 3 (dy+1) + 9 (dz+1), kp = 27 cin rounded
@@ -43,8 +40,8 @@ from __future__ import annotations
 
 import numpy as np
 
-CHANNELS = [(1, 16, 1), (16, 32, 2), (32, 32, 1), (32, 64, 2), (64, 64, 1)]
-LAYER_NAMES = ["C0", "C1", "C2", "C3", "C4"]
+CHANNELS = [(1, 16, 2), (16, 32, 2), (32, 64, 2)]
+LAYER_NAMES = ["C0", "C1", "C2"]
 
 
 def kpad(cin):
@@ -95,20 +92,17 @@ def pack(mats):
 
 
 def init_weights(seed, noise=0.1):
-    """Channel 0 carries the additive multilevel preconditioner (C0, C2, C4:
-    centre tap 1; C1, C3: full weighting x 1/sqrt(2)), so that
+    """Channel 0 carries the additive multilevel preconditioner (C0: full
+    weighting; C1, C2: full weighting x 1/sqrt(2)), so that
     J^T J = sum_l 2^-(l-1) P_l P_l^T over the three coarse levels; the other
     entries are seeded noise of `noise` x a fan-in-scaled normal (the
     device's own generator is not restated)."""
     rng = np.random.default_rng(seed)
     mats = [rng.normal(0.0, noise * np.sqrt(1.0 / (27 * cin)), size=(27, cin, cout)) for cin, cout, _ in CHANNELS]
-    a = np.sqrt(0.5)
-    for M, (cin, cout, stride) in zip(mats, CHANNELS):
+    for li, M in enumerate(mats):
+        a = 1.0 if li == 0 else np.sqrt(0.5)
         for t, dx, dy, dz in _taps():
-            if stride == 2:
-                M[t, 0, 0] = a * (0.5 if dx else 1.0) * (0.5 if dy else 1.0) * (0.5 if dz else 1.0)
-            else:
-                M[t, 0, 0] = 1.0 if t == 13 else 0.0
+            M[t, 0, 0] = a * (0.5 if dx else 1.0) * (0.5 if dy else 1.0) * (0.5 if dz else 1.0)
     return pack(mats)
 
 
@@ -214,17 +208,12 @@ def apply_jtj(x3, W, grids, exact):
     the activations the device keeps (dict)"""
     gf, g0, g1, g2 = grids
     r, f32 = _models(exact)
-    A = {}
-    A["u"] = f32(mask(up_adjoint(x3, g0)))
-    A["f0"] = f32(conv(A["u"], W[0], 1, g0))
-    A["h1"] = f32(mask(conv(r(A["f0"]), W[1], 2, g1)))
-    A["f1"] = f32(conv(r(A["h1"]), W[2], 1, g1))
-    A["h3"] = f32(mask(conv(r(A["f1"]), W[3], 2, g2)))
-    A["f2"] = f32(conv(r(A["h3"]), W[4], 1, g2))
-    A["g3"] = f32(mask(conv_adjoint(r(A["f2"]), W[4], 1, g2)))
-    A["g2"] = f32(conv_adjoint(r(A["g3"]), W[3], 2, g1) + A["f1"])
-    A["g1"] = f32(mask(conv_adjoint(r(A["g2"]), W[2], 1, g1)))
-    A["g0"] = f32(conv_adjoint(r(A["g1"]), W[1], 2, g0) + A["f0"])
+    A = {"x": x3}
+    A["f0"] = f32(mask(conv(x3, W[0], 2, g0)))
+    A["f1"] = f32(mask(conv(r(A["f0"]), W[1], 2, g1)))
+    A["f2"] = f32(mask(conv(r(A["f1"]), W[2], 2, g2)))
+    A["g1"] = f32(mask(conv_adjoint(r(A["f2"]), W[2], 2, g1) + A["f1"]))
+    A["g0"] = f32(mask(conv_adjoint(r(A["g1"]), W[1], 2, g0) + A["f0"]))
     return A
 
 
@@ -240,8 +229,7 @@ def forward(x, w, shape, gain=1.0, exact=True):
     W = [r(M) for M in unpack(np.asarray(w, dtype=np.float64))]
     x3 = f32(np.asarray(x, dtype=np.float64).reshape(gf + (1,)))
     A = apply_jtj(x3, W, (gf, g0, g1, g2), exact)
-    v = f32(mask(gain * conv_adjoint(A["g0"], W[0], 1, g0)))
-    y = f32(up(v, gf))
+    y = f32(gain * conv_adjoint(A["g0"], W[0], 2, gf))
     return y.ravel(), {"A": A, "W": W, "grids": (gf, g0, g1, g2), "gain": gain, "exact": exact}
 
 
@@ -255,8 +243,8 @@ def backward(gy, c):
     gf = c["grids"][0]
     A = c["A"]
     B = apply_jtj(f32(c["gain"] * np.asarray(gy, dtype=np.float64).reshape(gf + (1,))), c["W"], c["grids"], c["exact"])
-    pairs = [("u", "g0", 1), ("f0", "g1", 2), ("h1", "g2", 1), ("f1", "g3", 2), ("h3", "f2", 1)]
-    dW = [conv_wgrad(A[i], B[o], st) + conv_wgrad(B[i], A[o], st) for i, o, st in pairs]
+    pairs = [("x", "g0"), ("f0", "g1"), ("f1", "f2")]
+    dW = [conv_wgrad(A[i], B[o], 2) + conv_wgrad(B[i], A[o], 2) for i, o in pairs]
     return pack(dW)
 
 
