@@ -275,40 +275,146 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
   return FSX_OK;
 }
 
-// Dictionary of the distinct column-offset lists (col - row) of a CSR
-// pattern. Structured meshes have a handful (interior rows all share one),
-// so the SpMVs can read a 4-byte pattern id per row instead of a 4-byte
-// column per entry. Returns false when there are more than max_pat distinct
-// lists or they hold more than max_len offsets in total.
-static bool build_patterns(int nrows, const int* rowptr, const int* col, int max_pat, long long max_len,
-                           std::vector<int>& row_pat, std::vector<int>& pat_ptr, std::vector<int>& pat_off) {
+// Dictionary of the distinct per-row lists of a CSR pattern: each row's
+// column offsets (col - row) and, when `vbits` is given, the 32-bit words of
+// its values. Structured meshes have a handful of lists (interior rows all
+// share one), so the SpMVs can read a pattern id per row instead of a
+// column (and value) per entry. Returns false when there are more than
+// max_pat distinct lists or they hold more than max_len entries in total.
+static bool build_row_dict(int nrows, const int* rowptr, const int* col, const unsigned* vbits, int max_pat,
+                           long long max_len, std::vector<int>& row_pat, std::vector<int>& pat_ptr,
+                           std::vector<int>& pat_off, std::vector<unsigned>* pat_val) {
   std::unordered_map<unsigned long long, std::vector<int>> by_hash;
   row_pat.assign(nrows, 0);
   pat_ptr.assign(1, 0);
   pat_off.clear();
+  if (pat_val) pat_val->clear();
   for (int r = 0; r < nrows; r++) {
     const int rs = rowptr[r], len = rowptr[r + 1] - rs;
     unsigned long long h = 1469598103934665603ull ^ (unsigned long long)len;
-    for (int k = 0; k < len; k++) h = (h ^ (unsigned long long)(unsigned)(col[rs + k] - r)) * 1099511628211ull;
+    for (int k = 0; k < len; k++) {
+      h = (h ^ (unsigned long long)(unsigned)(col[rs + k] - r)) * 1099511628211ull;
+      if (vbits) h = (h ^ (unsigned long long)vbits[rs + k]) * 1099511628211ull;
+    }
     int id = -1;
     auto& cand = by_hash[h];
     for (int c : cand) {
       const int cs = pat_ptr[c], clen = pat_ptr[c + 1] - cs;
       if (clen != len) continue;
       bool same = true;
-      for (int k = 0; k < len && same; k++) same = pat_off[cs + k] == col[rs + k] - r;
+      for (int k = 0; k < len && same; k++)
+        same = pat_off[cs + k] == col[rs + k] - r && (!vbits || (*pat_val)[cs + k] == vbits[rs + k]);
       if (same) { id = c; break; }
     }
     if (id < 0) {
       id = (int)pat_ptr.size() - 1;
       if (id >= max_pat || (long long)pat_off.size() + len > max_len) return false;
-      for (int k = 0; k < len; k++) pat_off.push_back(col[rs + k] - r);
+      for (int k = 0; k < len; k++) {
+        pat_off.push_back(col[rs + k] - r);
+        if (vbits) pat_val->push_back(vbits[rs + k]);
+      }
       pat_ptr.push_back((int)pat_off.size());
       cand.push_back(id);
     }
     row_pat[r] = id;
   }
   return true;
+}
+
+static bool build_patterns(int nrows, const int* rowptr, const int* col, int max_pat, long long max_len,
+                           std::vector<int>& row_pat, std::vector<int>& pat_ptr, std::vector<int>& pat_off) {
+  return build_row_dict(nrows, rowptr, col, nullptr, max_pat, max_len, row_pat, pat_ptr, pat_off, nullptr);
+}
+
+// The solvers' forms of the built Z (structural zeros dropped): FP64 rows
+// with a pattern dictionary, and FP32 row classes when they compress.
+// Runs once per fsx_build_Z on host copies of Z; captured solver graphs that
+// name the previous forms are dropped.
+static int build_z_forms(fsx_ctx* ctx) {
+  const int n = ctx->NNp;
+  const long long nnz = ctx->nnzZ;
+  std::vector<int> rp(n + 1), ci(nnz);
+  std::vector<double> v(nnz);
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_HIP(hipMemcpy(rp.data(), ctx->Zrow, (size_t)(n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(ci.data(), ctx->Zcol, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(v.data(), ctx->Z, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> rp2(n + 1, 0), ci2;
+  std::vector<double> v2;
+  std::vector<unsigned> f2;
+  ci2.reserve(nnz); v2.reserve(nnz); f2.reserve(nnz);
+  for (int r = 0; r < n; r++) {
+    double m = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; k++) m = std::max(m, std::fabs(v[k]));
+    const double thr = m * kPruneRel;
+    for (int k = rp[r]; k < rp[r + 1]; k++) {
+      if (!(std::fabs(v[k]) >= thr) && ci[k] != r) continue;
+      ci2.push_back(ci[k]);
+      v2.push_back(v[k]);
+      const float f = (float)v[k];
+      unsigned u;
+      std::memcpy(&u, &f, sizeof u);
+      f2.push_back(u);
+    }
+    rp2[r + 1] = (int)ci2.size();
+  }
+  for (fsx::IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) iter_graph_reset(*g);
+  for (void* p : {(void*)ctx->Zr_row, (void*)ctx->Zr_val, (void*)ctx->Zr_col, (void*)ctx->Zr_pat, (void*)ctx->Zr_off,
+                  (void*)ctx->Zc_ent, (void*)ctx->Zc_chunk, (void*)ctx->Zc_rows})
+    if (p) dev_free(ctx, p);
+  ctx->Zr_row = nullptr; ctx->Zr_val = nullptr; ctx->Zr_col = nullptr; ctx->Zr_pat = nullptr; ctx->Zr_off = nullptr;
+  ctx->Zc_ent = nullptr; ctx->Zc_chunk = nullptr; ctx->Zc_rows = nullptr;
+  ctx->Zr_npat = 0; ctx->Zr_patlen = 0; ctx->Zc_n = 0; ctx->Zc_len = 0; ctx->Zc_nchunk = 0;
+  const long long nnz2 = rp2[n];
+  ctx->nnzZr = nnz2;
+  FSX_TRY(alloc(ctx, &ctx->Zr_row, (size_t)n + 1));
+  FSX_TRY(alloc(ctx, &ctx->Zr_val, (size_t)std::max(1LL, nnz2)));
+  FSX_HIP(hipMemcpy(ctx->Zr_row, rp2.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->Zr_val, v2.data(), (size_t)nnz2 * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<int> rowpat, pp, po;
+  if (build_patterns(n, rp2.data(), ci2.data(), 1 << 16, 1 << 22, rowpat, pp, po)) {
+    for (int& id : rowpat) id = pp[id];
+    FSX_TRY(alloc(ctx, &ctx->Zr_pat, (size_t)n));
+    FSX_TRY(alloc(ctx, &ctx->Zr_off, std::max<size_t>(1, po.size())));
+    FSX_HIP(hipMemcpy(ctx->Zr_pat, rowpat.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    if (!po.empty()) FSX_HIP(hipMemcpy(ctx->Zr_off, po.data(), po.size() * sizeof(int), hipMemcpyHostToDevice));
+    ctx->Zr_npat = (int)pp.size() - 1;
+    ctx->Zr_patlen = (long long)po.size();
+  } else {
+    FSX_TRY(alloc(ctx, &ctx->Zr_col, (size_t)std::max(1LL, nnz2)));
+    FSX_HIP(hipMemcpy(ctx->Zr_col, ci2.data(), (size_t)nnz2 * sizeof(int), hipMemcpyHostToDevice));
+  }
+  std::vector<int> cls, cp, co;
+  std::vector<unsigned> cv;
+  if (build_row_dict(n, rp2.data(), ci2.data(), f2.data(), 65535, 1 << 20, cls, cp, co, &cv)) {
+    std::vector<int2> ent(co.size());
+    for (size_t k = 0; k < co.size(); k++) ent[k] = make_int2(co[k], (int)cv[k]);
+    FSX_TRY(alloc(ctx, &ctx->Zc_ent, std::max<size_t>(1, ent.size())));
+    if (!ent.empty()) FSX_HIP(hipMemcpy(ctx->Zc_ent, ent.data(), ent.size() * sizeof(int2), hipMemcpyHostToDevice));
+    ctx->Zc_n = (int)cp.size() - 1;
+    ctx->Zc_len = (long long)ent.size();
+    // launch plan: the rows of each class in ascending order, cut into
+    // chunks of <= 64 rows; chunks ordered by their first row
+    std::vector<std::vector<int>> members(cp.size() - 1);
+    for (int r = 0; r < n; r++) members[cls[r]].push_back(r);
+    std::vector<int4> chunks;
+    std::vector<int> rows;
+    rows.reserve(n);
+    for (size_t c = 0; c < members.size(); c++)
+      for (size_t s0 = 0; s0 < members[c].size(); s0 += 64) {
+        const int len = (int)std::min<size_t>(64, members[c].size() - s0);
+        chunks.push_back(make_int4((int)rows.size(), len, cp[c], cp[c + 1] - cp[c]));
+        rows.insert(rows.end(), members[c].begin() + s0, members[c].begin() + s0 + len);
+      }
+    std::stable_sort(chunks.begin(), chunks.end(),
+                     [&](const int4& a, const int4& b) { return rows[a.x] < rows[b.x]; });
+    FSX_TRY(alloc(ctx, &ctx->Zc_chunk, std::max<size_t>(1, chunks.size())));
+    FSX_TRY(alloc(ctx, &ctx->Zc_rows, (size_t)n));
+    FSX_HIP(hipMemcpy(ctx->Zc_chunk, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(ctx->Zc_rows, rows.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    ctx->Zc_nchunk = (int)chunks.size();
+  }
+  return FSX_OK;
 }
 
 int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const int* col) {
@@ -424,6 +530,7 @@ int fsx_build_Z(fsx_ctx* ctx) {
   if (!ctx) return FSX_EINVAL;
   if (!ctx->assembled || !ctx->have_Z) return fail(ctx, FSX_ESTATE, "fsx_build_Z needs assembly and the Z pattern");
   FSX_TRY(launch_build_Z(ctx));
+  FSX_TRY(build_z_forms(ctx));
   ctx->z_built = true;
   return FSX_OK;
 }
@@ -622,11 +729,13 @@ int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
 int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, double beta) {
   if (!ctx || !x || !y) return FSX_EINVAL;
   if (!ctx->assembled) return fail(ctx, FSX_ESTATE, "assemble first");
-  if (which == 4 && !ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  if ((which == 4 || which == 8 || which == 9) && !ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  if ((which == 8 || which == 9) && !(alpha == 1.0 && beta == 0.0))
+    return fail(ctx, FSX_EINVAL, "fsx_spmv: the solvers' Z forms take alpha = 1, beta = 0");
   size_t nx, ny;
   if (which == 0) { nx = ny = ctx->NN; }
   else if (which >= 1 && which <= 3) { nx = ctx->NNp; ny = ctx->NN; }
-  else if (which == 4) { nx = ny = ctx->NNp; }
+  else if (which == 4 || which == 8 || which == 9) { nx = ny = ctx->NNp; }
   else if (which >= 5 && which <= 7) { nx = ctx->NN; ny = ctx->NNp; }
   else return fail(ctx, FSX_EINVAL, "fsx_spmv: unknown operator");
   double *dx, *dy;

@@ -17,6 +17,12 @@ constexpr int kWave = 64;         // CDNA wavefront width
 constexpr int kBlock = 256;       // default block size for row kernels
 constexpr int kMaxGrid = 2048;    // row kernels: 256 CUs x 8 blocks, a multiple of 8 XCDs
 constexpr int kZRowMax = 1024;    // longest Z row handled by the Z-build kernel
+// Entries of Z whose magnitude is below this fraction (2^-46) of their row's
+// largest are structural zeros: couplings that vanish analytically and that
+// the triple product G^T M^-1 G leaves as cancellation residue (about 1e-37
+// on the cavity meshes, against 1.6e-6 for the smallest coupling that does
+// not vanish). The solvers' copies of Z drop them; the assembled Z keeps them.
+constexpr double kPruneRel = 1.4210854715202004e-14;
 
 // Device-resident scalars of one Krylov solve, written by block 0 of a kernel
 // and read by later launches. Values that a kernel both reads (all blocks)
@@ -108,6 +114,24 @@ struct fsx_ctx {
   int* Zrowpat = nullptr; int* Zpatoff = nullptr;
   int Znpat = 0; long long Zpatlen = 0;
   int zrow_max = 0;
+  // Z without its structural zeros (entries below kPruneRel of the row's
+  // largest magnitude), built by fsx_build_Z for the solvers:
+  //  * FP64 rows Zr_row / Zr_val, columns through a pattern dictionary
+  //    (Zr_pat, Zr_off as above) or, when the rows do not compress, Zr_col;
+  //  * the FP32 copy as row classes when its rows take few distinct lists of
+  //    (column offset, FP32 value): a class's entries sit in Zc_ent as
+  //    (offset, float bits); null otherwise (the FP32 solvers then read Zf
+  //    on the full pattern).
+  int* Zr_row = nullptr; double* Zr_val = nullptr; int* Zr_col = nullptr;
+  int* Zr_pat = nullptr; int* Zr_off = nullptr;
+  long long nnzZr = 0; int Zr_npat = 0; long long Zr_patlen = 0;
+  int2* Zc_ent = nullptr;
+  int Zc_n = 0; long long Zc_len = 0;
+  // the class rows' launch plan: the rows grouped by class (ascending within
+  // a class) in Zc_rows, cut into chunks of at most 64 rows of one class:
+  // (first index into Zc_rows, row count, class entry start, entry count)
+  int4* Zc_chunk = nullptr; int Zc_nchunk = 0;
+  int* Zc_rows = nullptr;
   // lumped mass and BCs (device, NN-long; the reference replicates them x3)
   double* MdInv = nullptr; double* MdOrigInv = nullptr; double* Md = nullptr;
   unsigned char* bcflag = nullptr;

@@ -352,24 +352,6 @@ __global__ __launch_bounds__(kBlock) void k_csrmv(int nrows, const int* __restri
   if (r < nrows && l == 0) y[r] = (beta == 0.0 ? 0.0 : beta * y[r]) + alpha * s;
 }
 
-int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta) {
-  constexpr int LPR = 16;
-  if (which == 0) {
-    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,
-                       ctx->Mcol, ctx->K, 1, dx, dy, alpha, beta);
-  } else if (which >= 1 && which <= 3) {
-    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
-                       ctx->Gcol, ctx->G + (which - 1), 3, dx, dy, alpha, beta);
-  } else if (which == 4) {
-    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow,
-                       ctx->Zcol, ctx->Z, 1, dx, dy, alpha, beta);
-  } else {
-    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->GTrow,
-                       ctx->GTcol, ctx->GT + (which - 5), 3, dx, dy, alpha, beta);
-  }
-  return launch_check(ctx, "k_csrmv");
-}
-
 // ------------------------------------------------------------ Jacobi-PCG
 // Textbook preconditioned CG with x0 = 0 (BCH:3999-4004), Jacobi
 // preconditioner (ipar[10] = 1, BCH:4017) and the stop test
@@ -391,7 +373,7 @@ template <class T, int LPR, int U, bool PAT>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restrict__ rp, ColSrc cs,
                                                      const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
                                                      const SolveScalars* __restrict__ sc) {
-  if (sc->stop) return;
+  if (sc && sc->stop) return;
   int l;
   const long long r = my_row<LPR>(&l);
   T s = 0;
@@ -415,25 +397,116 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
   if (r < n && l == 0) q[r] = s;
 }
 
-// q = Z p and rh = b - Z xh on the context's Z, through the row-pattern
-// dictionary when Z has one
-static ColSrc z_cols(fsx_ctx* ctx) { return ColSrc{ctx->Zcol, ctx->Zrowpat, ctx->Zpatoff}; }
+// q = Z p on the FP32 row classes: row r sums the entries of its class,
+// (offset o, value a) pairs, as a * p[r + o], in entry order. The class table
+// is a few tens of KB and stays in cache, so a launch moves little more than
+// the row list, p and q. One wavefront per chunk of up to 64 rows of one
+// class: the entries are wave-uniform and come through scalar loads, and
+// entry k's gathers p[r + o_k] follow the chunk's rows (contiguous for the
+// interior of a structured grid). With XCD, consecutive blocks of chunks go
+// to one XCD (blocks are dealt to the 8 XCDs round robin), so each XCD's L2
+// holds one slab of p instead of all of it.
+template <int U, bool XCD>
+__global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __restrict__ chunk,
+                                                      const int* __restrict__ rows, const int2* __restrict__ ent,
+                                                      const float* __restrict__ p, float* __restrict__ q,
+                                                      const SolveScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  int b = blockIdx.x;
+  if constexpr (XCD) b = (b % 8) * (gridDim.x / 8) + b / 8;   // gridDim.x is a multiple of 8
+  const int w = b * (kBlock / kWave) + threadIdx.x / kWave;
+  if (w >= nchunk) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int4 c = chunk[w];
+  const int rs = __builtin_amdgcn_readfirstlane(c.x), len = __builtin_amdgcn_readfirstlane(c.y);
+  const int es = __builtin_amdgcn_readfirstlane(c.z), el = __builtin_amdgcn_readfirstlane(c.w);
+  const int r = rows[rs + (lane < len ? lane : 0)];
+  const int2* __restrict__ e = ent + es;
+  float s = 0.0f;
+  int k = 0;
+  for (; k + U <= el; k += U) {
+    int2 t[U];
+    float v[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) t[j] = e[k + j];
+#pragma unroll
+    for (int j = 0; j < U; j++) v[j] = p[r + t[j].x];
+#pragma unroll
+    for (int j = 0; j < U; j++) s += __int_as_float(t[j].y) * v[j];
+  }
+  for (; k < el; k++) {
+    const int2 t = e[k];
+    s += __int_as_float(t.y) * p[r + t.x];
+  }
+  if (lane < len) q[r] = s;
+}
 
+// The solvers' Z (structural zeros dropped): FP64 rows through the pattern
+// dictionary when there is one
+static ColSrc zr_cols(fsx_ctx* ctx) { return ColSrc{ctx->Zr_col, ctx->Zr_pat, ctx->Zr_off}; }
+
+constexpr int kClsU = 8;
+
+// q = Z p: FP64 on the solvers' rows; FP32 on the row classes, or on the
+// FP32 copy of the full pattern when the rows do not compress
 template <class T>
-static void launch_zspmv(fsx_ctx* ctx, const T* val, const T* p, T* q) {
-  constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
-  const ColSrc cs = z_cols(ctx);
+static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc) {
   const int n = ctx->NNp;
+  if constexpr (sizeof(T) == 4) {
+    if (ctx->Zc_chunk) {
+      const int wpb = kBlock / kWave;
+      const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
+      hipLaunchKernelGGL((k_zcls_spmv<kClsU, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
+                         ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc);
+      return;
+    }
+  }
+  constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
+  const bool fp64 = sizeof(T) == 8;
+  const ColSrc cs = fp64 ? zr_cols(ctx) : ColSrc{ctx->Zcol, ctx->Zrowpat, ctx->Zpatoff};
+  const int* rp = fp64 ? ctx->Zr_row : ctx->Zrow;
+  const T* val = fp64 ? (const T*)(const void*)ctx->Zr_val : (const T*)(const void*)ctx->Zf;
   // the dictionary form uses 8 lanes x 16 loads per row in both precisions
   // (a sweep of 8x16, 16x8, 32x4, 64x2 on the 64^3 Z: 36.7, 56.2, 64.3,
   // 82.0 us in FP32; 44.5, 45.5, 53.4, 80.9 us in FP64): more independent
   // loads in flight per lane hide the per-row pattern lookup
   if (cs.row_pat)
-    hipLaunchKernelGGL((k_pcg_spmv<T, 8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, cs,
-                       val, p, q, ctx->sc);
+    hipLaunchKernelGGL((k_pcg_spmv<T, 8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, rp, cs, val,
+                       p, q, sc);
   else
-    hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow,
-                       cs, val, p, q, ctx->sc);
+    hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, rp, cs,
+                       val, p, q, sc);
+}
+
+__global__ void k_to_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+__global__ void k_to_f64(int n, const float* __restrict__ a, double* __restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (double)a[i];
+}
+
+int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta) {
+  constexpr int LPR = 16;
+  if (which == 0) {
+    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,
+                       ctx->Mcol, ctx->K, 1, dx, dy, alpha, beta);
+  } else if (which >= 1 && which <= 3) {
+    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
+                       ctx->Gcol, ctx->G + (which - 1), 3, dx, dy, alpha, beta);
+  } else if (which == 4) {
+    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow,
+                       ctx->Zcol, ctx->Z, 1, dx, dy, alpha, beta);
+  } else if (which == 8) {
+    launch_zspmv<double>(ctx, dx, dy, nullptr);
+  } else if (which == 9) {
+    hipLaunchKernelGGL(k_to_f32, dim3(ew_grid(ctx->NNp)), dim3(kBlock), 0, ctx->stream, ctx->NNp, dx, ctx->fp);
+    launch_zspmv<float>(ctx, ctx->fp, ctx->fq, nullptr);
+    hipLaunchKernelGGL(k_to_f64, dim3(ew_grid(ctx->NNp)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->fq, dy);
+  } else {
+    hipLaunchKernelGGL(k_csrmv<LPR>, dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->GTrow,
+                       ctx->GTcol, ctx->GT + (which - 5), 3, dx, dy, alpha, beta);
+  }
+  return launch_check(ctx, "k_csrmv");
 }
 
 template <class T>
@@ -525,11 +598,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_direction(int j, int n, const T*
 }
 
 template <class T>
-static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* val, const T* dinv, T* x, T* r, T* z, T* p, T* q,
+static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* dinv, T* x, T* r, T* z, T* p, T* q,
                                  int kind, double delta2) {
   constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
   const int n = ctx->NNp;
-  launch_zspmv<T>(ctx, val, p, q);
+  launch_zspmv<T>(ctx, p, q, ctx->sc);
   hipLaunchKernelGGL(k_pcg_dot<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, p, q, ctx->partials, ctx->sc);
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, x, r, z, p, q, dinv,
                      ctx->partials, ctx->sc);
@@ -545,14 +618,14 @@ static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* val, const T* din
 // previous solve) and polls the device scalars after each chunk until the
 // stop flag is set. Returns with *ctx->h_sc current.
 template <class T>
-static int pcg_iterate(fsx_ctx* ctx, int j0, const T* val, const T* dinv, T* x, T* r, T* z, T* p, T* q, int kind,
+static int pcg_iterate(fsx_ctx* ctx, int j0, const T* dinv, T* x, T* r, T* z, T* p, T* q, int kind,
                        double delta2, int first_chunk, int maxit) {
   int j = j0;
   int chunk = std::max(4, first_chunk);
   IterGraph& G = sizeof(T) == 8 ? ctx->graph_fp64 : ctx->graph_mixed;
   const unsigned long long key = (unsigned long long)(uintptr_t)x;   // the only caller-owned buffer
   auto issue = [&](int jj) -> int {
-    pcg_launch_iteration<T>(ctx, jj, val, dinv, x, r, z, p, q, kind, delta2);
+    pcg_launch_iteration<T>(ctx, jj, dinv, x, r, z, p, q, kind, delta2);
     return FSX_OK;
   };
   for (;;) {
@@ -587,7 +660,7 @@ static int solve_fp64(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
                      ctx->wz, ctx->wp, ctx->partials);
   hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, ctx->maxit);
   FSX_TRY(launch_check(ctx, "k_pcg_init"));
-  FSX_TRY(pcg_iterate<double>(ctx, 0, ctx->Z, ctx->Zdinv, x, ctx->wr, ctx->wz, ctx->wp, ctx->wq, 0, 0.0,
+  FSX_TRY(pcg_iterate<double>(ctx, 0, ctx->Zdinv, x, ctx->wr, ctx->wz, ctx->wp, ctx->wq, 0, 0.0,
                               segment_guess(std::min(ctx->last_iters, ctx->maxit), 1.0, ctx->rtol, 4), ctx->maxit));
   const SolveScalars& s = *ctx->h_sc;
   *iters = s.iters;
@@ -644,14 +717,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restric
 
 static void launch_zresidual(fsx_ctx* ctx, const double* xh, const double* b, double* rh) {
   constexpr int LPR = PcgCfg<double>::LPR, U = PcgCfg<double>::U;
-  const ColSrc cs = z_cols(ctx);
+  const ColSrc cs = zr_cols(ctx);
   const int n = ctx->NNp;
   if (cs.row_pat)
-    hipLaunchKernelGGL((k_residual<8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, cs, ctx->Z,
-                       xh, b, rh);
+    hipLaunchKernelGGL((k_residual<8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, ctx->Zr_row, cs,
+                       ctx->Zr_val, xh, b, rh);
   else
-    hipLaunchKernelGGL((k_residual<LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zrow, cs,
-                       ctx->Z, xh, b, rh);
+    hipLaunchKernelGGL((k_residual<LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, ctx->Zr_row,
+                       cs, ctx->Zr_val, xh, b, rh);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sumsq(int n, const double* __restrict__ a, double* __restrict__ partials) {
@@ -704,7 +777,7 @@ static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
   const int seg = segment_guess(ctx->last_iters, delta, ctx->rtol, 4);
   int j = 0, guess = seg, status = 0, updates = 0;
   for (;;) {
-    FSX_TRY(pcg_iterate<float>(ctx, j, ctx->Zf, ctx->Zdinvf, ctx->fx, ctx->fr, ctx->fz, ctx->fp, ctx->fq, 1, delta * delta,
+    FSX_TRY(pcg_iterate<float>(ctx, j, ctx->Zdinvf, ctx->fx, ctx->fr, ctx->fz, ctx->fp, ctx->fq, 1, delta * delta,
                                guess, ctx->maxit));
     const int stop = ctx->h_sc->stop;
     if (stop == 3) return fail(ctx, FSX_ENUMERIC, "mixed PCG met NaN/Inf");
@@ -745,9 +818,8 @@ int solve_fp64_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, doubl
   return solve_fp64(ctx, b, x, iters, relres);
 }
 
-int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q) {
-  constexpr int LPR = PcgCfg<float>::LPR, U = PcgCfg<float>::U;
-  launch_zspmv<float>(ctx, ctx->Zf, p, q);
+int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc) {
+  launch_zspmv<float>(ctx, p, q, sc);
   return FSX_OK;
 }
 
@@ -812,16 +884,15 @@ __global__ void k_fill_t(T* __restrict__ p, T v, int n) {
 }
 
 template <class T>
-static int bench_spmv_t(fsx_ctx* ctx, const T* val, T* p, T* q, int reps, double* ms) {
-  constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
+static int bench_spmv_t(fsx_ctx* ctx, T* p, T* q, int reps, double* ms) {
   const int n = ctx->NNp;
   hipLaunchKernelGGL(k_fill_t<T>, dim3(1024), dim3(kBlock), 0, ctx->stream, p, (T)1, n);
   hipLaunchKernelGGL(k_arm, dim3(1), dim3(1), 0, ctx->stream, ctx->sc);
   for (int w = 0; w < 3; w++)
-    launch_zspmv<T>(ctx, val, p, q);
+    launch_zspmv<T>(ctx, p, q, ctx->sc);
   FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int r = 0; r < reps; r++)
-    launch_zspmv<T>(ctx, val, p, q);
+    launch_zspmv<T>(ctx, p, q, ctx->sc);
   FSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   FSX_HIP(hipEventSynchronize(ctx->ev1));
   FSX_TRY(launch_check(ctx, "k_pcg_spmv (timing)"));
@@ -838,21 +909,42 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
   if (!ctx || reps < 1 || !ms_per_launch) return FSX_EINVAL;
   if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
   double ms = 0;
-  const double nnz = (double)ctx->nnzZ, n = ctx->NNp;
+  const double n = ctx->NNp;
   double bytes;
-  // algorithmic bytes: values + column indices (or, with the row-pattern
-  // dictionary, one pattern id per row plus the dictionary) + row pointers,
-  // p read once, q written once
-  const double cols = ctx->Zrowpat ? 4.0 * n + 4.0 * (ctx->Zpatlen + ctx->Znpat + 1) : 4.0 * nnz;
+  // algorithmic bytes: values and columns as the solvers store them, p read
+  // once, q written once.
+  //  FP64: values of the rows without structural zeros + column indices (or,
+  //  with the row-pattern dictionary, one pattern start per row plus the
+  //  dictionary) + row pointers.
+  //  FP32 row classes: the launch plan (16 bytes per chunk of rows, 4 per
+  //  row) + the class table; otherwise
+  //  the FP32 copy on the full pattern like the FP64 form.
   if (precision == 0) {
-    FSX_TRY(bench_spmv_t<double>(ctx, ctx->Z, ctx->wp, ctx->wq, reps, &ms));
+    FSX_TRY(bench_spmv_t<double>(ctx, ctx->wp, ctx->wq, reps, &ms));
+    const double nnz = (double)ctx->nnzZr;
+    const double cols = ctx->Zr_pat ? 4.0 * n + 4.0 * (ctx->Zr_patlen + ctx->Zr_npat + 1) : 4.0 * nnz;
     bytes = 8.0 * nnz + cols + 4.0 * (n + 1) + 16.0 * n;
   } else {
-    FSX_TRY(bench_spmv_t<float>(ctx, ctx->Zf, ctx->fp, ctx->fq, reps, &ms));
-    bytes = 4.0 * nnz + cols + 4.0 * (n + 1) + 8.0 * n;
+    FSX_TRY(bench_spmv_t<float>(ctx, ctx->fp, ctx->fq, reps, &ms));
+    if (ctx->Zc_chunk) {
+      bytes = 16.0 * ctx->Zc_nchunk + 4.0 * n + 8.0 * ctx->Zc_len + 8.0 * n;
+    } else {
+      const double nnz = (double)ctx->nnzZ;
+      const double cols = ctx->Zrowpat ? 4.0 * n + 4.0 * (ctx->Zpatlen + ctx->Znpat + 1) : 4.0 * nnz;
+      bytes = 4.0 * nnz + cols + 4.0 * (n + 1) + 8.0 * n;
+    }
   }
   *ms_per_launch = ms;
   if (bytes_per_launch) *bytes_per_launch = bytes;
+  return FSX_OK;
+}
+
+extern "C" int fsx_z_forms(fsx_ctx* ctx, long long* nnz_solver, int* nclasses, long long* class_entries) {
+  if (!ctx) return FSX_EINVAL;
+  if (!ctx->z_built) return fsx::fail(ctx, FSX_ESTATE, "build Z first");
+  if (nnz_solver) *nnz_solver = ctx->nnzZr;
+  if (nclasses) *nclasses = ctx->Zc_chunk ? ctx->Zc_n : 0;
+  if (class_entries) *class_entries = ctx->Zc_chunk ? ctx->Zc_len : 0;
   return FSX_OK;
 }
 

@@ -487,21 +487,6 @@ __global__ __launch_bounds__(kBlock) void k_dot_f(int n, const float* __restrict
   write_partials<1>(v, partials);
 }
 
-// y = Z x on the FP32 copy of Z; 32 lanes per row (Z rows hold up to 125
-// entries on hex meshes)
-__global__ __launch_bounds__(kBlock) void k_spmv_f(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                   const float* __restrict__ val, const float* __restrict__ x,
-                                                   float* __restrict__ y) {
-  constexpr int LPR = 32;
-  const int l = threadIdx.x % LPR;
-  const long long r = (long long)blockIdx.x * (kBlock / LPR) + threadIdx.x / LPR;
-  float s = 0.f;
-  if (r < n)
-    for (int k = rp[r] + l; k < rp[r + 1]; k += LPR) s += val[k] * x[ci[k]];
-  s = group_sumf<LPR>(s);
-  if (r < n && l == 0) y[r] = s;
-}
-
 }  // namespace fsx
 
 // ====================================================================== host side
@@ -893,7 +878,7 @@ int precondition(fsx_ctx* ctx, fsx_unet* u, const SolveScalars* sc) {
 
 namespace fsx {
 
-int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q);
+int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc);
 int launch_fold_residual(fsx_ctx* ctx, double* x, const double* b, int* stop_out, double* rr_hi_out);
 int solve_mixed_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
 
@@ -927,7 +912,7 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
   int guess = seg;
   int fallback = 0;
   auto issue = [&](int jj) -> int {
-    FSX_TRY(launch_pcg_spmv_f32(ctx, ctx->fp, ctx->fq));
+    FSX_TRY(launch_pcg_spmv_f32(ctx, ctx->fp, ctx->fq, ctx->sc));
     hipLaunchKernelGGL(k_fpcg_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->fr, ctx->fp, ctx->fq, ctx->partials,
                        ctx->sc);
     hipLaunchKernelGGL(k_fpcg_update, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, jj, n, ctx->fx, ctx->fr, ctx->fp,
@@ -1043,10 +1028,8 @@ int need_unet(fsx_ctx* ctx) {
   return FSX_OK;
 }
 
-void spmv_f(fsx_ctx* ctx, const float* x, float* y) {
-  hipLaunchKernelGGL(k_spmv_f, dim3(blocks_for(ctx->NNp, kBlock / 32)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow,
-                     ctx->Zcol, ctx->Zf, x, y);
-}
+// y = Z x in FP32 (the solvers' FP32 form of Z)
+void spmv_f(fsx_ctx* ctx, const float* x, float* y) { (void)launch_pcg_spmv_f32(ctx, x, y, nullptr); }
 
 // One training sample: forward, loss, output gradient, backward (accumulates
 // u->g). kind 0: relative L2 loss |y - t|^2 / tt with tt = |t|^2; kind 1:

@@ -32,7 +32,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_copy", "fsx_z_patterns", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_copy", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
@@ -107,6 +107,7 @@ def lib():
         "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_bench_copy": ([_vp, ctypes.c_size_t, ctypes.c_int, _D], ctypes.c_int),
         "fsx_z_patterns": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "fsx_z_forms": ([_vp, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "fsx_timing_reset": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_timing_get": ([_vp, ctypes.POINTER(Timing)], ctypes.c_int),
         "fsx_unet_setup": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64], ctypes.c_int),
@@ -370,6 +371,15 @@ class Solver:
         self.check(self.L.fsx_z_patterns(self.ctx, ctypes.byref(n), ctypes.byref(ln)), "z_patterns")
         return n.value, ln.value
 
+    def z_forms(self):
+        """(entries of Z kept by the solvers, FP32 row classes, class-table
+        entries); 0 classes when the FP32 copy did not compress"""
+        nz = ctypes.c_longlong()
+        nc = ctypes.c_int()
+        ln = ctypes.c_longlong()
+        self.check(self.L.fsx_z_forms(self.ctx, ctypes.byref(nz), ctypes.byref(nc), ctypes.byref(ln)), "z_forms")
+        return nz.value, nc.value, ln.value
+
     def bench_copy(self, nbytes, reps=20):
         ms = ctypes.c_double()
         self.check(self.L.fsx_bench_copy(self.ctx, nbytes, reps, ctypes.byref(ms)), "bench_copy")
@@ -413,9 +423,12 @@ class Solver:
         return x, it.value, rel.value, rc
 
     def spmv(self, which, x, y=None, alpha=1.0, beta=0.0):
-        code = {"K": 0, "G1": 1, "G2": 2, "G3": 3, "Z": 4, "G1T": 5, "G2T": 6, "G3T": 7}[which]
+        """which: K, G1..G3, Z, G1T..G3T; Zsolver64 / Zsolver32 apply Z as the
+        FP64 / FP32 solvers read it (alpha = 1, beta = 0 only)"""
+        code = {"K": 0, "G1": 1, "G2": 2, "G3": 3, "Z": 4, "G1T": 5, "G2T": 6, "G3T": 7,
+                "Zsolver64": 8, "Zsolver32": 9}[which]
         s = self.sizes
-        ny = s["NNp"] if which in ("Z", "G1T", "G2T", "G3T") else s["NN"]
+        ny = s["NNp"] if which in ("Z", "G1T", "G2T", "G3T", "Zsolver64", "Zsolver32") else s["NN"]
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(ny) if y is None else np.array(y, dtype=np.float64)
         self.check(self.L.fsx_spmv(self.ctx, code, _dp(x), _dp(y), alpha, beta), "spmv")

@@ -169,16 +169,20 @@ int fsx_sync(fsx_ctx* ctx);
 int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
 /* y = alpha * op(v) + beta * y on the context's matrices for host vectors:
  * which = 0 K (NN), 1..3 G_d (NN <- NNp), 4 Z (NNp), 5..7 G_d^T (NNp <- NN).
- * Mirrors the reference's mkl_dcsrmv / cusparseDcsrmv calls. */
+ * Mirrors the reference's mkl_dcsrmv / cusparseDcsrmv calls. which = 8 and
+ * 9 apply Z as the FP64 and the FP32 solvers read it (structural zeros
+ * dropped; FP32 rounds v, Z and y) with alpha = 1, beta = 0 only. */
 int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, double beta);
 
 /* Times `reps` back-to-back launches of the pressure solver's SpMV kernel
- * (q = Z p, the dominant kernel of a step) on the context's stream with HIP
- * events. precision 0 = FP64 Z, 1 = FP32 Z (the mixed-precision and learned
- * inner solves). Returns the mean device time per launch in ms and the
- * algorithmic HBM bytes of one launch: values + column indices (or the
- * row-pattern ids and dictionary) + row pointers + one read of p + one write
- * of q. No reference counterpart (the reference prints wall times only,
+ * (q = Z p) on the context's stream with HIP events. precision 0 = FP64 Z,
+ * 1 = FP32 Z (the mixed-precision and learned inner solves). Returns the
+ * mean device time per launch in ms and the algorithmic HBM bytes of one
+ * launch in the form the solvers read: FP64 values + column indices (or the
+ * row-pattern ids and dictionary) + row pointers; or, for FP32 row classes,
+ * the launch plan (a 16-byte record per chunk of rows of one class and a
+ * 4-byte row index per row) + the class table; plus one read of p and one
+ * write of q. No reference counterpart (the reference prints wall times only,
  * BCH:4489-4518). */
 int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch, double* bytes_per_launch);
 /* Size of the dictionary of Z row patterns (distinct column-offset lists)
@@ -186,6 +190,16 @@ int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch,
  * Z did not compress and the SpMVs read the CSR columns. No reference
  * counterpart (the reference's Z is plain CSR, CU:281-411). */
 int fsx_z_patterns(fsx_ctx* ctx, int* npatterns, long long* offsets);
+/* The solvers' forms of the built Z. Entries whose magnitude is below 2^-46
+ * of their row's largest are structural zeros (couplings that vanish
+ * analytically, left by G^T M^-1 G as cancellation residue near 1e-37) and
+ * are dropped; nnz_solver returns the entries kept. The FP32 copy is stored
+ * as row classes when its rows take at most 65535 distinct lists of (column
+ * offset, FP32 value) with at most 2^20 entries in total: nclasses and
+ * class_entries return their count and total length (0 when the FP32 solves
+ * read the full FP32 copy). No reference counterpart (the reference solves
+ * with the assembled CSR Z, CU:418-451, BCH:3981-4055). */
+int fsx_z_forms(fsx_ctx* ctx, long long* nnz_solver, int* nclasses, long long* class_entries);
 /* Times `reps` launches of a plain device copy of `bytes` (read once, written
  * once) between two scratch buffers: the HBM bandwidth reference and the
  * known-traffic kernel that calibrates PMC byte counters. No reference

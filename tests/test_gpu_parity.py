@@ -101,6 +101,44 @@ def test_pcg_solution(fsx_lib, oracle_lib, mode):
     s.close()
 
 
+@pytest.mark.parametrize("case", ["cavity16", "cavity6_sinh", "duct"])
+def test_solver_z_forms(fsx_lib, oracle_lib, case):
+    """Z as the solvers read it: structural zeros (below 2^-46 of the row's
+    largest magnitude) dropped, FP64 rows and FP32 row classes. The kept
+    count is recomputed from the oracle's Z; the products must match the
+    oracle's full Z to FP64 round-off (1e-14) and FP32 round-off (1e-6)."""
+    say(f"solver Z forms: {case}")
+    if case == "cavity16":
+        mesh = fsx_lib.Mesh.cavity(16)
+    elif case == "cavity6_sinh":
+        mesh = fsx_lib.Mesh.cavity(6, cluster=1.5)
+    else:
+        mesh = fsx_lib.Mesh.duct(n=4, n_in=4, n_arc=6, n_out=5, seed=3)
+    s, o = make_pair(fsx_lib, oracle_lib, mesh, NGP=27, viscosity=1.0 / 200)
+    import scipy.sparse as sp
+    n = o.sizes()["NNp"]
+    zr, zc, zv = o.i("Zrow"), o.i("Zcol"), o.d("Z")
+    Z = sp.csr_matrix((zv, zc, zr), shape=(n, n))
+    a = np.abs(zv)
+    rowmax = np.maximum.reduceat(a, zr[:-1])
+    rows = np.repeat(np.arange(n), np.diff(zr))
+    keep = (a >= np.repeat(rowmax, np.diff(zr)) * 2.0 ** -46) | (zc == rows)
+    nnz_solver, ncls, clen = s.z_forms()
+    assert nnz_solver == int(keep.sum()), (nnz_solver, int(keep.sum()))
+    if case == "cavity16":
+        assert nnz_solver < 0.8 * zv.size
+        assert 0 < ncls <= 200, ncls
+    rng = np.random.default_rng(5)
+    for trial in range(2):
+        x = rng.standard_normal(n) if trial == 0 else np.linspace(-1.0, 2.0, n)
+        ref = Z @ x
+        y64 = s.spmv("Zsolver64", x)
+        y32 = s.spmv("Zsolver32", x)
+        assert rel(y64, ref) < 1e-14, rel(y64, ref)
+        assert rel(y32, ref) < 1e-6, rel(y32, ref)
+    s.close()
+
+
 def test_stage_by_stage(fsx_lib, oracle_lib):
     """One inner iteration stage by stage from an identical non-trivial state."""
     say("stage by stage: cavity 5^3")
