@@ -4,6 +4,7 @@
 #include "fsx_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -119,6 +120,44 @@ static void build_tables(int NGP, std::vector<double>& out) {
   }
 }
 
+// One wavefront copies the solve scalars to host-coherent memory, then
+// releases the sequence number the host waits for.
+__global__ void k_publish(const SolveScalars* __restrict__ sc, Published* pub, unsigned seq) {
+  constexpr int nw = (int)(sizeof(SolveScalars) / sizeof(unsigned));
+  const unsigned* src = reinterpret_cast<const unsigned*>(sc);
+  volatile unsigned* dst = reinterpret_cast<volatile unsigned*>(&pub->s);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) *reinterpret_cast<volatile unsigned*>(&pub->seq) = seq;
+}
+
+// Waits for the scalars with a host spin on the published sequence number
+// (a copy plus hipStreamSynchronize costs tens of microseconds of idle GPU
+// per solver decision). The stream is queried every few thousand spins so
+// that a failed kernel or a lost publication ends the wait with an error.
+int read_scalars(fsx_ctx* ctx) {
+  static_assert(sizeof(SolveScalars) % sizeof(unsigned) == 0, "SolveScalars is copied as 32-bit words");
+  const unsigned seq = ++ctx->pub_seq;
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kWave), 0, ctx->stream, ctx->sc, ctx->d_pub, seq);
+  FSX_TRY(launch_check(ctx, "k_publish"));
+  volatile unsigned* flag = &ctx->h_pub->seq;
+  for (long spin = 1; *flag != seq; spin++) {
+    if ((spin & 4095) == 0) {
+      const hipError_t e = hipStreamQuery(ctx->stream);
+      if (e != hipSuccess && e != hipErrorNotReady)
+        return fail(ctx, FSX_EHIP, std::string("solver scalars: ") + hipGetErrorString(e));
+      if (e == hipSuccess) {
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+        if (*flag != seq) return fail(ctx, FSX_EHIP, "solver scalars: publication not visible after the stream drained");
+      }
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  std::memcpy(ctx->h_sc, const_cast<const SolveScalars*>(&ctx->h_pub->s), sizeof(SolveScalars));
+  return FSX_OK;
+}
+
 }  // namespace fsx
 
 using namespace fsx;
@@ -145,6 +184,13 @@ int fsx_create(int device, fsx_ctx** out) {
   if (rc == FSX_OK) rc = alloc(ctx, &ctx->st, 1);
   if (rc == FSX_OK) rc = alloc(ctx, &ctx->d_mon, 4);
   if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_sc, sizeof(SolveScalars)) != hipSuccess) rc = FSX_ENOMEM;
+  if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_pub, sizeof(Published), hipHostMallocMapped | hipHostMallocCoherent) !=
+                          hipSuccess)
+    rc = FSX_ENOMEM;
+  if (rc == FSX_OK) {
+    std::memset((void*)ctx->h_pub, 0, sizeof(Published));
+    if (hipHostGetDevicePointer((void**)&ctx->d_pub, ctx->h_pub, 0) != hipSuccess) rc = FSX_EHIP;
+  }
   if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_st, sizeof(StepScalars)) != hipSuccess) rc = FSX_ENOMEM;
   if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_mon, 4 * sizeof(double)) != hipSuccess) rc = FSX_ENOMEM;
   if (rc == FSX_OK && (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)) rc = FSX_EHIP;
@@ -169,6 +215,7 @@ void fsx_destroy(fsx_ctx* ctx) {
   for (fsx::IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) fsx::iter_graph_reset(*g);
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
+  if (ctx->h_pub) (void)hipHostFree(ctx->h_pub);
   if (ctx->h_st) (void)hipHostFree(ctx->h_st);
   if (ctx->h_mon) (void)hipHostFree(ctx->h_mon);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -39,6 +39,14 @@ struct SolveScalars {
   int active_v[2];    // iteration j ran its update kernel (stop was 0 then)
 };
 
+// SolveScalars as published to host-coherent pinned memory by k_publish:
+// the host spins on seq instead of a copy plus a stream synchronisation.
+struct Published {
+  SolveScalars s;
+  unsigned seq;
+  unsigned pad[3];
+};
+
 // Scalars of the time step (norms for the convergence checks, max |dU|/dt).
 struct StepScalars {
   double uu, ud, pp, pd, maxacc;
@@ -147,6 +155,9 @@ struct fsx_ctx {
   double* partials = nullptr;    // 8 * kMaxGrid, per-block partial sums of the reduction kernels
   fsx::SolveScalars* sc = nullptr;
   fsx::StepScalars* st = nullptr;
+  fsx::Published* h_pub = nullptr;     // host-coherent, written by k_publish
+  fsx::Published* d_pub = nullptr;     // its device address
+  unsigned pub_seq = 0;
   fsx::SolveScalars* h_sc = nullptr;   // pinned host mirrors
   fsx::StepScalars* h_st = nullptr;
   double* h_mon = nullptr;
@@ -217,6 +228,8 @@ int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double a
 int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);
 int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi);
 void unet_free(fsx_ctx* ctx);
+// *ctx->h_sc <- the solve scalars after all work enqueued so far
+int read_scalars(fsx_ctx* ctx);
 void iter_graph_reset(IterGraph& g);
 
 // Enqueue iterations j .. j + count - 1 (rounded up to whole graphs when

@@ -632,8 +632,7 @@ static int pcg_iterate(fsx_ctx* ctx, int j0, const T* dinv, T* x, T* r, T* z, T*
     int done = 0;
     FSX_TRY(run_iterations(ctx, G, key, j, chunk, &done, issue));
     j += done;
-    FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-    FSX_HIP(hipStreamSynchronize(ctx->stream));
+    FSX_TRY(read_scalars(ctx));
     if (ctx->h_sc->stop) return FSX_OK;
     if (j - j0 > maxit + 64) return fail(ctx, FSX_ESTATE, "PCG did not terminate");
     chunk = 4;
@@ -789,8 +788,7 @@ static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, dou
     hipLaunchKernelGGL(k_sumsq, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->partials);
     hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc);
     FSX_TRY(launch_check(ctx, "mixed reliable update"));
-    FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-    FSX_HIP(hipStreamSynchronize(ctx->stream));
+    FSX_TRY(read_scalars(ctx));
     updates++;
     if (ctx->h_sc->stop == 3) return fail(ctx, FSX_ENUMERIC, "mixed residual met NaN/Inf");
     if (ctx->h_sc->stop == 1) break;
@@ -846,8 +844,7 @@ static int residual_check(fsx_ctx* ctx, double* x, const double* b, bool fold, i
   hipLaunchKernelGGL(k_sumsq, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->partials);
   hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc);
   FSX_TRY(launch_check(ctx, "residual check"));
-  FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_TRY(read_scalars(ctx));
   *stop = ctx->h_sc->stop;
   *rr_hi = ctx->h_sc->rr_hi;
   return FSX_OK;

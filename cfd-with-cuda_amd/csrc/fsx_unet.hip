@@ -172,17 +172,24 @@ __global__ __launch_bounds__(kWave * conv_ks<CIN>()) void k_conv_mfma(Grid3 gi, 
 // d == parity per axis (d = 0 at an even index, d = +-1 at an odd one), so a
 // block owns 16 nodes of one class (blockIdx.y) and walks only that class's
 // 1 to 8 taps instead of all 27. Same epilogue as k_conv_mfma.
+//
+// With hout (COUT = 16, the layer that produces g0): also the first layer's
+// transpose taps h[v][t] = sum_c W0[c][t] g0[v][c] (t < 27, rows of 32), so
+// the final pressure-grid layer only sums the <= 8 taps that reach a node.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(kWave * 4) void k_conv_mfma_t(Grid3 gi, Grid3 go, const __bf16* __restrict__ in,
                                                            const __bf16* __restrict__ W, float* __restrict__ out32,
                                                            __bf16* __restrict__ out16, const float* __restrict__ res,
-                                                           int mask, const SolveScalars* __restrict__ sc) {
+                                                           int mask, const float* __restrict__ w0, float* __restrict__ hout,
+                                                           const SolveScalars* __restrict__ sc) {
   if (sc && sc->stop) return;
   constexpr int KS = 4;
   constexpr int KP = (27 * CIN + 31) / 32 * 32;
   constexpr int NCT = COUT / 16;
   constexpr int RS = COUT + 1;
   __shared__ float red[KS][kConvVox][RS];
+  __shared__ float fin[kConvVox][RS];
+  __shared__ float w0s[COUT == 16 ? 16 * 27 : 1];
   const int cls = blockIdx.y;
   const int pi = cls & 1, pj = (cls >> 1) & 1, pk = cls >> 2;
   // nodes of this class: i = 2 ic + pi, ic < cx, ...
@@ -249,6 +256,25 @@ __global__ __launch_bounds__(kWave * 4) void k_conv_mfma_t(Grid3 gi, Grid3 go, c
     if (mask && on_boundary(v, go)) val = 0.0f;
     out32[o] = val;
     if (out16) out16[o] = to_bf16(val);
+    fin[row][co] = val;
+  }
+  if constexpr (COUT == 16) {
+    if (hout) {
+      for (int e = threadIdx.x; e < 16 * 27; e += KS * kWave) w0s[e] = w0[(e / 27) * 32 + e % 27];
+      __syncthreads();
+      for (int e = threadIdx.x; e < kConvVox * 32; e += KS * kWave) {
+        const int row = e >> 5, t = e & 31;
+        const long long vl = vbase + row;
+        if (vl >= ncls || t >= 27) continue;
+        const unsigned u = (unsigned)vl, cxy = (unsigned)cx * (unsigned)cy;
+        const unsigned kc = u / cxy, rem = u - kc * cxy, jc = rem / (unsigned)cx, ic = rem - jc * (unsigned)cx;
+        const long long v = ((long long)(2 * (int)kc + pk) * go.ny + (2 * (int)jc + pj)) * go.nx + (2 * (int)ic + pi);
+        float h = 0.f;
+#pragma unroll
+        for (int c = 0; c < 16; c++) h += w0s[c * 27 + t] * fin[row][c];
+        hout[(size_t)v * 32 + t] = h;
+      }
+    }
   }
 }
 
@@ -273,69 +299,87 @@ __global__ __launch_bounds__(kBlock) void k_conv0(Grid3 gf, Grid3 g0, const floa
   node_ijk(v, g0, i, j, k);
   float acc = 0.f;
   if (!(i == 0 || j == 0 || k == 0 || i == g0.nx - 1 || j == g0.ny - 1 || k == g0.nz - 1)) {
+    // an interior node's taps 2o + d stay inside gf, so all 27 loads are
+    // issued together (no per-tap bounds test)
+    const int sx = 1, sy = gf.nx, sz = gf.nx * gf.ny;
+    const float* xc = x + ((2 * k) * sz + (2 * j) * sy + 2 * i);
+    float xv[27];
 #pragma unroll
-    for (int t = 0; t < 27; t++) {
-      const long long s = tap_input<2>(i, j, k, t, gf);
-      if (s >= 0) acc += w[t * 16 + c] * x[s];
-    }
+    for (int t = 0; t < 27; t++) xv[t] = xc[(t / 9 - 1) * sz + ((t / 3) % 3 - 1) * sy + (t % 3 - 1) * sx];
+#pragma unroll
+    for (int t = 0; t < 27; t++) acc += w[t * 16 + c] * xv[t];
   }
   out32[idx] = acc;
   if (out16) out16[idx] = to_bf16(acc);
 }
 
 // The last layer, fused with the solver's use of it: for each pressure node
-// f, y = gain sum_{o,t: 2o + d_t = f} sum_c W0[c][t] g0[o][c] (the transpose
-// of k_conv0; g0 is already zero on the boundary of G0). With z set:
-// z = dinv r + y and the r.z, z.q partials (slots 3, 4) of the flexible CG;
-// otherwise y is stored. Grid: kRedGrid blocks of kWideBlock threads,
-// grid-stride (about one node per thread on 65^3).
+// f, y = gain sum_{o,t: 2o + d_t = f} h[o][t], with h[o][t] = sum_c W0[c][t]
+// g0[o][c] from the previous layer (the transpose of k_conv0; g0 is already
+// zero on the boundary of G0). With z set: z = dinv r + y and the r.z, z.q
+// partials (slots 3, 4) of the flexible CG; otherwise y is stored. Grid:
+// kRedGrid blocks of kWideBlock threads, grid-stride (about one node per
+// thread on 65^3).
 constexpr int kWideBlock = 1024;
 
-__global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const float* __restrict__ g0v,
-                                                   const float* __restrict__ W0, float gain, float* __restrict__ y,
-                                                   const float* __restrict__ r, const float* __restrict__ dinv,
-                                                   const float* __restrict__ q, float* __restrict__ z,
-                                                   double* __restrict__ partials, const SolveScalars* __restrict__ sc) {
+__global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const float* __restrict__ h, float gain,
+                                                   float* __restrict__ y, const float* __restrict__ r,
+                                                   const float* __restrict__ dinv, const float* __restrict__ q,
+                                                   float* __restrict__ z, double* __restrict__ partials,
+                                                   const SolveScalars* __restrict__ sc) {
   if (sc && sc->stop) return;   // sc is null on a start, which always runs
-  __shared__ float w[27 * 16];
-  for (int i = threadIdx.x; i < 27 * 16; i += blockDim.x) w[i] = W0[(i & 15) * 32 + (i >> 4)];
-  __syncthreads();
   double acc2[2] = {0.0, 0.0};
-  const long long n = gf.n();
-  for (long long f = blockIdx.x * kWideBlock + threadIdx.x; f < n; f += (long long)kRedGrid * kWideBlock) {
+  // block b owns nodes [b per, (b + 1) per); a thread takes kNodes of them
+  // at a stride of the block size, all loads issued before the sums
+  constexpr int kNodes = 2;
+  const long long n = gf.n(), per = (n + kRedGrid - 1) / kRedGrid;
+  const long long lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  for (long long f0 = lo + threadIdx.x; f0 < hi; f0 += (long long)kNodes * kWideBlock) {
+  float hv[kNodes][8];
+  float rv[kNodes], dv[kNodes], qv[kNodes];
+  unsigned mk[kNodes];
+#pragma unroll
+  for (int u = 0; u < kNodes; u++) {
+    const long long fu = f0 + (long long)u * kWideBlock;
+    const long long f = fu < hi ? fu : f0;   // an idle slot repeats the first node, unused
     int i, j, k;
     node_ijk(f, gf, i, j, k);
     // coarse parents o = (f - d) / 2 per axis: one (d = 0) at an even index,
-    // two (d = +1, -1) at an odd one; at most 8 taps in all
-    const int ni = 1 + (i & 1), nj = 1 + (j & 1), nk = 1 + (k & 1);
-    float acc = 0.f;
-    for (int c = 0; c < nk; c++) {
-      const int dz = (k & 1) ? (c ? -1 : 1) : 0, oz = (k - dz) >> 1;
-      for (int b = 0; b < nj; b++) {
-        const int dy = (j & 1) ? (b ? -1 : 1) : 0, oy = (j - dy) >> 1;
-        for (int a = 0; a < ni; a++) {
-          const int dx = (i & 1) ? (a ? -1 : 1) : 0, ox = (i - dx) >> 1;
-          const int t = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-          const unsigned o = ((unsigned)oz * (unsigned)g0.ny + (unsigned)oy) * (unsigned)g0.nx + (unsigned)ox;
-          const float4* gp = reinterpret_cast<const float4*>(g0v + (size_t)o * 16);
-          const float* wt = w + t * 16;
+    // two (d = +1, -1) at an odd one. Every node walks 2 x 2 x 2 slots; the
+    // slots an even index does not have repeat its parent (a valid address)
+    // with weight 0, so the loads carry no branches and issue together.
+    const int dx0 = i & 1, dy0 = j & 1, dz0 = k & 1;   // slot 0: d = +1 when odd, else 0
+    const unsigned ox[2] = {(unsigned)(i - dx0) >> 1, (unsigned)(i + dx0) >> 1};
+    const unsigned oy[2] = {(unsigned)(j - dy0) >> 1, (unsigned)(j + dy0) >> 1};
+    const unsigned oz[2] = {(unsigned)(k - dz0) >> 1, (unsigned)(k + dz0) >> 1};
+    const int tx[2] = {dx0 + 1, 1 - dx0}, ty[2] = {dy0 + 1, 1 - dy0}, tz[2] = {dz0 + 1, 1 - dz0};
+    mk[u] = (unsigned)dx0 | ((unsigned)dy0 << 1) | ((unsigned)dz0 << 2);
 #pragma unroll
-          for (int qd = 0; qd < 4; qd++) {
-            const float4 gv = gp[qd];
-            acc += wt[4 * qd] * gv.x + wt[4 * qd + 1] * gv.y + wt[4 * qd + 2] * gv.z + wt[4 * qd + 3] * gv.w;
-          }
-        }
-      }
+    for (int s8 = 0; s8 < 8; s8++) {
+      const int a = s8 & 1, b = (s8 >> 1) & 1, c = s8 >> 2;
+      const unsigned o = (oz[c] * (unsigned)g0.ny + oy[b]) * (unsigned)g0.nx + ox[a];
+      hv[u][s8] = h[(size_t)o * 32 + (tx[a] + 3 * ty[b] + 9 * tz[c])];
     }
+    if (z) { rv[u] = r[f]; dv[u] = dinv[f]; qv[u] = q[f]; }
+  }
+#pragma unroll
+  for (int u = 0; u < kNodes; u++) {
+    const long long f = f0 + (long long)u * kWideBlock;
+    if (f >= hi) continue;
+    // slot s8 exists when each axis with a second parent (s8 bit set) is odd
+    float acc = 0.f;
+#pragma unroll
+    for (int s8 = 0; s8 < 8; s8++) acc += ((s8 & ~mk[u]) ? 0.f : 1.f) * hv[u][s8];
     const float yv = gain * acc;
     if (z) {
-      const float zi = dinv[f] * r[f] + yv;
+      const float zi = dv[u] * rv[u] + yv;
       z[f] = zi;
-      acc2[0] += (double)r[f] * (double)zi;
-      acc2[1] += (double)zi * (double)q[f];
+      acc2[0] += (double)rv[u] * (double)zi;
+      acc2[1] += (double)zi * (double)qv[u];
     } else {
       y[f] = yv;
     }
+  }
   }
   if (z) write_partials<2, kWideBlock>(acc2, partials + 3 * kRedGrid);
 }
@@ -524,6 +568,7 @@ struct fsx_unet {
   float *x = nullptr, *y = nullptr, *tgt = nullptr, *gy = nullptr, *zq = nullptr;   // pressure grid
   Acts A, B;                // A: the preconditioner / the sample; B: the loss gradient (training)
   float* wpart = nullptr;
+  float* h0 = nullptr;      // first layer's transpose taps on G0, 32 per node (27 used)
   // training set (normalised pairs, FP32)
   int ncap = 0, cap_max = 0;
   double *cap_b = nullptr, *cap_x = nullptr;     // captured (R2, Pdot) pairs
@@ -543,12 +588,12 @@ unsigned blocks_for(long long n, int per) { return (unsigned)std::max(1LL, (n + 
 
 template <int CIN, int COUT, int MODE>
 void conv_launch(fsx_ctx* ctx, Grid3 gi, Grid3 go, const __bf16* in, const __bf16* W, float* out32, __bf16* out16,
-                 const float* res, int mask, const SolveScalars* sc) {
+                 const float* res, int mask, const float* w0, float* hout, const SolveScalars* sc) {
   if constexpr (MODE == 3) {
     // class (0,0,0) is the largest: ceil(n/2) nodes per axis
     const long long nmax = (long long)((go.nx + 1) / 2) * ((go.ny + 1) / 2) * ((go.nz + 1) / 2);
     hipLaunchKernelGGL((k_conv_mfma_t<CIN, COUT>), dim3(blocks_for(nmax, kConvVox), 8), dim3(kWave * 4), 0, ctx->stream, gi, go,
-                       in, W, out32, out16, res, mask, sc);
+                       in, W, out32, out16, res, mask, w0, hout, sc);
   } else {
     hipLaunchKernelGGL((k_conv_mfma<CIN, COUT, MODE>), dim3(blocks_for(go.n(), kConvVox)), dim3(kWave * conv_ks<CIN>()), 0,
                        ctx->stream, gi, go, in, W, out32, out16, res, mask, sc);
@@ -556,11 +601,14 @@ void conv_launch(fsx_ctx* ctx, Grid3 gi, Grid3 go, const __bf16* in, const __bf1
 }
 
 // dispatch on (cin, cout, mode): the encoder convolutions (mode 2, stride 2)
-// and their transposes (mode 3, with WT)
+// and their transposes (mode 3, with WT); hout (the 32 -> 16 transpose only)
+// also receives the first layer's transpose taps (k_conv_mfma_t)
 int conv(fsx_ctx* ctx, int cin, int cout, int mode, Grid3 gi, Grid3 go, const __bf16* in, const __bf16* W, float* out32,
-         __bf16* out16, const float* res, int mask, const SolveScalars* sc) {
+         __bf16* out16, const float* res, int mask, const SolveScalars* sc, const float* w0 = nullptr,
+         float* hout = nullptr) {
+  if (hout && !(cin == 32 && cout == 16 && mode == 3)) return fail(ctx, FSX_EINVAL, "conv: taps output on another layer");
 #define CONV_CASE(A, B, M) \
-  if (cin == A && cout == B && mode == M) { conv_launch<A, B, M>(ctx, gi, go, in, W, out32, out16, res, mask, sc); return FSX_OK; }
+  if (cin == A && cout == B && mode == M) { conv_launch<A, B, M>(ctx, gi, go, in, W, out32, out16, res, mask, w0, hout, sc); return FSX_OK; }
   CONV_CASE(16, 32, 2) CONV_CASE(32, 64, 2) CONV_CASE(64, 32, 3) CONV_CASE(32, 16, 3)
 #undef CONV_CASE
   return fail(ctx, FSX_EINVAL, "conv shape not instantiated");
@@ -603,12 +651,13 @@ int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, cons
   FSX_TRY(conv(ctx, 32, 64, 2, u->g1, u->g2, S.f1b, L[2].w16, S.f2, S.f2b, nullptr, 1, sc));
   // transpose pass
   FSX_TRY(conv(ctx, 64, 32, 3, u->g2, u->g1, S.f2b, L[2].wt16, S.g1, S.g1b, S.f1, 1, sc));
-  FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 1, sc));
+  FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 1, sc, L[0].wr,
+               out == kOutNone ? nullptr : u->h0));
   if (out == kOutY)
-    hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, S.g0, L[0].wr, u->gain, y, nullptr,
+    hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, u->h0, u->gain, y, nullptr,
                        nullptr, nullptr, nullptr, nullptr, sc);
   else if (out == kOutZ)
-    hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, S.g0, L[0].wr, u->gain, nullptr,
+    hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, u->h0, u->gain, nullptr,
                        x, ctx->Zdinvf, ctx->fq, ctx->fz, ctx->partials, sc);
   return launch_check(ctx, "network application");
 }
@@ -721,6 +770,7 @@ int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fs
   FSX_TRY(alloc_acts(ctx, u.get(), u->A));
   FSX_TRY(alloc_acts(ctx, u.get(), u->B));
   FSX_TRY(ualloc(ctx, &u->wpart, (size_t)kWChunks * 64 * 27 * 64));
+  FSX_TRY(ualloc(ctx, &u->h0, (size_t)u->g0.n() * 32));
   FSX_TRY(refresh_weights(ctx, u.get()));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   *out = u.release();
@@ -898,8 +948,7 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
   FSX_TRY(launch_copy(ctx, ctx->wh, b, n));
   hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->partials);
   hipLaunchKernelGGL(k_learned_init, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, max_learned);
-  FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_TRY(read_scalars(ctx));
   if (ctx->h_sc->stop == 1) { *iters = 0; *relres = 0.0; return FSX_OK; }   // b = 0
   // start: r = b, z = M r, p = z
   hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx, ctx->partials);
@@ -933,8 +982,7 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
       FSX_TRY(run_iterations(ctx, ctx->graph_learned, key, j, guess, &done, issue));
       j += done;
       issued += done;
-      FSX_HIP(hipMemcpyAsync(ctx->h_sc, ctx->sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, ctx->stream));
-      FSX_HIP(hipStreamSynchronize(ctx->stream));
+      FSX_TRY(read_scalars(ctx));
       if (ctx->h_sc->stop) break;
       if (issued > max_learned + 64) return fail(ctx, FSX_ESTATE, "FPCG did not terminate");
       guess = 2;
