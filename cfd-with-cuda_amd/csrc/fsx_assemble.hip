@@ -233,89 +233,134 @@ int launch_build_Z(fsx_ctx* ctx) {
 // R1 -= A(Un) UnpHalf_prev element by element, matrix free (BCH:3560-3654):
 //   R1e_c,i = sum_k Sv_ki f_k (u0_k . grad uprev_c(k)),
 //   u0_k = sum_j Sv_kj Un_j,  grad uprev_c(k) = J_k^-1 sum_j dSv_j(k) uprev_c,j,
-// which equals sum_j Ae_ij uprev_c,j of the reference's explicit Ae. One
-// wavefront per element, four elements per block; the shape tables are staged
-// in LDS once per block.
-constexpr int kConvElems = 4;
+// which equals sum_j Ae_ij uprev_c,j of the reference's explicit Ae.
+//
+// Persistent form: a block stages the shape tables in LDS once, and each
+// half-wavefront (32 lanes) walks elements of the colour at a grid stride,
+// one element at a time (lane = Gauss point, then lane = node), so two
+// elements share a wavefront and 54 of its 64 lanes work. The next
+// element's node values and corner coordinates are loaded into registers
+// while the current one computes. LDS holds the tables packed per (Gauss
+// point, node) as (Sv, dSv/dxi, dSv/deta, dSv/dzeta) and the node values as
+// (Un_x, Un_y, Un_z, Uprev_x, Uprev_y, Uprev_z), so one step of the inner
+// sum reads five 16-byte words; the arithmetic and its order are those of
+// the per-element form.
+constexpr int kConvWaves = 4;      // 3 blocks per CU fit (140 VGPRs, 46 KB LDS); 8-wave blocks at 128 VGPRs spill and run slower
+constexpr int kConvHalves = 2 * kConvWaves;
 
-__global__ __launch_bounds__(kWave * kConvElems) void k_convection(int ncolor_elems, const int* __restrict__ elems,
+// orders LDS traffic between the lanes of one wavefront
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_elems, const int* __restrict__ elems,
                                                                    const int* __restrict__ L27,
                                                                    const double* __restrict__ xc, ShapeTables tab, int NGP,
                                                                    int NN, const double* __restrict__ Un,
                                                                    const double* __restrict__ Uprev, double* __restrict__ R1) {
-  __shared__ double sSv[kMaxNGP * 27];
-  __shared__ double sdSv[kMaxNGP * 81];
+  __shared__ double4 sT[kMaxNGP * 27];                 // (Sv, dSv_0, dSv_1, dSv_2) at [k][j]
   __shared__ double sdSp[kMaxNGP * 24];
-  __shared__ double u0n[kConvElems][3][27], upn[kConvElems][3][27], conv[kConvElems][3][kMaxNGP];
-  __shared__ double ec[kConvElems][8][3];
-  __shared__ int L[kConvElems][27];
+  __shared__ double sw[kMaxNGP];
+  __shared__ double2 nod[kConvHalves][27][3];          // (Un_x, Un_y), (Un_z, Up_x), (Up_y, Up_z)
+  __shared__ double conv[kConvHalves][3][kMaxNGP];
+  __shared__ double ec[kConvHalves][8][3];
   const int t = threadIdx.x;
-  const int w = t / kWave, lane = t % kWave;
-  for (int i = t; i < NGP * 27; i += blockDim.x) sSv[i] = tab.Sv[i];
-  for (int i = t; i < NGP * 81; i += blockDim.x) sdSv[i] = tab.dSv[i];
+  for (int i = t; i < NGP * 27; i += blockDim.x)
+    sT[i] = make_double4(tab.Sv[i], tab.dSv[3 * i], tab.dSv[3 * i + 1], tab.dSv[3 * i + 2]);
   for (int i = t; i < NGP * 24; i += blockDim.x) sdSp[i] = tab.dSp[i];
-  const int slot = blockIdx.x * kConvElems + w;
-  const bool active = slot < ncolor_elems;
-  const int e = active ? elems[slot] : 0;
-  if (active && lane < 27) {
-    const int n = L27[(size_t)e * 27 + lane];
-    L[w][lane] = n;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      u0n[w][c][lane] = Un[(size_t)c * NN + n];
-      upn[w][c][lane] = Uprev[(size_t)c * NN + n];
-    }
-  }
+  for (int i = t; i < NGP; i += blockDim.x) sw[i] = tab.w[i];
   __syncthreads();
-  if (active && lane < 24) ec[w][lane / 3][lane % 3] = xc[3 * (size_t)L[w][lane / 3] + lane % 3];
-  __syncthreads();
-  if (active && lane < NGP) {
-    const int k = lane;
-    double iJ[3][3], det;
-    jacobian(ec[w], sdSp + k * 24, iJ, &det);
-    const double f = det * tab.w[k];
-    double u0[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int j = 0; j < 27; j++) {
-      const double sv = sSv[k * 27 + j];
-      const double d0 = sdSv[(k * 27 + j) * 3], d1 = sdSv[(k * 27 + j) * 3 + 1], d2 = sdSv[(k * 27 + j) * 3 + 2];
+  const int lane = t % kWave, l = lane & 31;
+  const int hw = (t / kWave) * 2 + (lane >> 5);        // half-wave of this lane in the block
+  const int stride = gridDim.x * kConvHalves;
+  int slot = blockIdx.x * kConvHalves + hw;
+  // registers of the element being fetched: node id and values (l < 27),
+  // one corner coordinate (l < 24)
+  int nn = 0;
+  double un[3] = {0, 0, 0}, up[3] = {0, 0, 0}, xv = 0;
+  auto fetch = [&](int sl) {
+    const bool act = sl < ncolor_elems;
+    const int e = act ? elems[sl] : 0;
+    if (act && l < 27) {
+      nn = L27[(size_t)e * 27 + l];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        u0[c] += sv * u0n[w][c][j];
-        const double up = upn[w][c][j];
-        gr[c][0] += d0 * up; gr[c][1] += d1 * up; gr[c][2] += d2 * up;
+        un[c] = Un[(size_t)c * NN + nn];
+        up[c] = Uprev[(size_t)c * NN + nn];
       }
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      double g[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) g[d] = iJ[d][0] * gr[c][0] + iJ[d][1] * gr[c][1] + iJ[d][2] * gr[c][2];
-      conv[w][c][k] = f * (u0[0] * g[0] + u0[1] * g[1] + u0[2] * g[2]);
+    const int cn = __shfl(nn, (lane & 32) + (l < 24 ? l / 3 : 0), kWave);
+    if (act && l < 24) xv = xc[3 * (size_t)cn + l % 3];
+  };
+  fetch(slot);
+  for (; slot < ncolor_elems; slot += stride) {
+    // stage the fetched element, then start fetching the next one
+    const int node = nn;
+    if (l < 27) {
+      nod[hw][l][0] = make_double2(un[0], un[1]);
+      nod[hw][l][1] = make_double2(un[2], up[0]);
+      nod[hw][l][2] = make_double2(up[1], up[2]);
     }
-  }
-  __syncthreads();
-  if (active && lane < 27) {
-    const int i = lane;
-    double r[3] = {0, 0, 0};
-    for (int k = 0; k < NGP; k++) {
-      const double sv = sSv[k * 27 + i];
+    if (l < 24) ec[hw][l / 3][l % 3] = xv;
+    wave_sync();
+    fetch(slot + stride);
+    if (l < NGP) {
+      const int k = l;
+      double iJ[3][3], det;
+      jacobian(ec[hw], sdSp + k * 24, iJ, &det);
+      const double f = det * sw[k];
+      double u0[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int j = 0; j < 27; j++) {
+        const double4 T = sT[k * 27 + j];
+        const double2 a = nod[hw][j][0], b = nod[hw][j][1], c2 = nod[hw][j][2];
+        const double u0v[3] = {a.x, a.y, b.x}, upv[3] = {b.y, c2.x, c2.y};
 #pragma unroll
-      for (int c = 0; c < 3; c++) r[c] += sv * conv[w][c][k];
+        for (int c = 0; c < 3; c++) {
+          u0[c] += T.x * u0v[c];
+          gr[c][0] += T.y * upv[c]; gr[c][1] += T.z * upv[c]; gr[c][2] += T.w * upv[c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double g[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) g[d] = iJ[d][0] * gr[c][0] + iJ[d][1] * gr[c][1] + iJ[d][2] * gr[c][2];
+        conv[hw][c][k] = f * (u0[0] * g[0] + u0[1] * g[1] + u0[2] * g[2]);
+      }
     }
-    const int n = L[w][i];
+    wave_sync();
+    if (l < 27) {
+      const int i = l;
+      double r[3] = {0, 0, 0};
+      for (int k = 0; k < NGP; k++) {
+        const double sv = sT[k * 27 + i].x;
 #pragma unroll
-    for (int c = 0; c < 3; c++) R1[(size_t)c * NN + n] -= r[c];
+        for (int c = 0; c < 3; c++) r[c] += sv * conv[hw][c][k];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) R1[(size_t)c * NN + node] -= r[c];
+    }
+    wave_sync();   // the next element's staging overwrites this one's LDS
   }
 }
 
 int launch_convection(fsx_ctx* ctx) {
   FSX_TRY(launch_fill(ctx, ctx->R1, 0.0, 3LL * ctx->NN));                  // BCH:3541-3545
+  if (ctx->conv_grid == 0) {
+    // as many blocks as fit on the device at once (the kernel is persistent)
+    int per_cu = 0, cus = 0;
+    FSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_convection, kWave * kConvWaves, 0));
+    FSX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    ctx->conv_grid = std::max(1, per_cu * cus);
+  }
   const int ncol = (int)ctx->color_ptr.size() - 1;
   for (int c = 0; c < ncol; c++) {
     const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
     if (n == 0) continue;
-    const int g = (n + kConvElems - 1) / kConvElems;
-    hipLaunchKernelGGL(k_convection, dim3(g), dim3(kWave * kConvElems), 0, ctx->stream, n,
+    const int g = std::min(ctx->conv_grid, (n + kConvHalves - 1) / kConvHalves);
+    hipLaunchKernelGGL(k_convection, dim3(g), dim3(kWave * kConvWaves), 0, ctx->stream, n,
                        ctx->color_order + ctx->color_ptr[c], ctx->L27, ctx->xc, ctx->tab, ctx->NGP, ctx->NN, ctx->Un,
                        ctx->UnpHalf_prev, ctx->R1);
   }

@@ -163,6 +163,7 @@ struct fsx_ctx {
   double* h_mon = nullptr;
   double* d_mon = nullptr;
   fsx::ShapeTables tab{};
+  int conv_grid = 0;             // blocks of the persistent convection kernel (0: not yet queried)
   double* tab_mem = nullptr;
   // accounting
   std::vector<void*> allocs;
@@ -180,7 +181,7 @@ struct fsx_ctx {
   int last_fallback = 0;
   void (*capture_hook)(fsx_ctx*, const double* b, const double* x) = nullptr;
   // iteration graphs: FP64 PCG, mixed inner PCG, learned flexible PCG
-  bool use_graphs = true;
+  bool use_graphs = false;      // measured on par with individual launches on this path
   fsx::IterGraph graph_fp64, graph_mixed, graph_learned;
 };
 
