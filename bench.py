@@ -31,6 +31,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cfd-with-cuda_amd"))
 
 METRIC = "timesteps/s, 64^3 hex27 cavity: learned+mixed vs plain FP64 PCG at 1e-8 residual"
+# rocprofv3 names of the step's row kernels: step1 with K*Acc_prev fused
+# (inner iterations >= 2), step2 RHS, step3
+STEP_KERNELS = ("k_step1<8, 8, 2, true>", "k_step2_rhs<16, 8>", "k_step3<4, 4>")
+
+
+def zspmv_name(prec, npat, ncls):
+    """rocprofv3 name of the Z SpMV kernel the solvers run in this form."""
+    if prec and ncls > 0:
+        return "k_zcls_spmv<8, true>"
+    if npat > 0:   # the row-pattern form runs 8 lanes x 16 loads per row in both precisions
+        return "k_pcg_spmv<float, 8, 16, true>" if prec else "k_pcg_spmv<double, 8, 16, true>"
+    return "k_pcg_spmv<float, 32, 4, false>" if prec else "k_pcg_spmv<double, 16, 8, false>"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -62,9 +74,9 @@ def run_mode(solver, mode, steps, warmup, rtol, maxit):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    runs (FETCH_SIZE and WRITE_SIZE of bench.py --spmv-probe, each scaled by
+    runs (FETCH_SIZE and WRITE_SIZE of bench.py --probe, each scaled by
     the same counter on a copy of known size), or None when not measured."""
-    path = os.path.join(ROOT, "profiles", "pmc_spmv_cavity64.json")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_cavity64.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -150,9 +162,9 @@ def main():
     ap.add_argument("--train-capture-steps", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--spmv-probe", action="store_true",
-                    help="only set up the case, time the Z SpMV kernels and run a 256 MiB calibration copy "
-                         "(the workload of the rocprofv3 --pmc traffic runs)")
+    ap.add_argument("--probe", action="store_true",
+                    help="only set up the case, run one FP64 time step, time the Z SpMV kernels and run a 256 MiB "
+                         "calibration copy (the workload of the rocprofv3 --pmc traffic runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=30)
     args = ap.parse_args()
@@ -170,7 +182,11 @@ def main():
     solver = fsx.Solver(mesh, device=0, NGP=args.ngp, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=args.rtol,
                         pcg_maxit=args.maxit, solver_mode=0)
     log(f"setup done in {time.perf_counter() - t0:.1f} s: {solver.setup_times()} sizes {solver.sizes}")
-    if args.spmv_probe:
+    if args.probe:
+        solver.run(1)
+        solver.sync()
+        for kid in (fsx.KERNEL_STEP1, fsx.KERNEL_KACC, fsx.KERNEL_STEP2_RHS, fsx.KERNEL_STEP3):
+            solver.bench_kernel(kid, reps=3)
         ms32, b32 = solver.bench_spmv(precision=1, reps=20)
         ms64, b64 = solver.bench_spmv(precision=0, reps=20)
         copy_bytes = 256 << 20
@@ -194,15 +210,26 @@ def main():
         log(f"  {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms, "
             f"max relres {r['max_relres']:.2e}")
     head = next((m for m in ("learned", "mixed") if m in res), list(res)[0])
+    h = res[head]
     prec = 1 if head in ("mixed", "learned") else 0
+    # live per-launch device times (HIP events on the context's stream) of the
+    # step's row kernels and the headline solver's Z SpMV; the roofline entry
+    # is the one with the most device time per step in the headline mode
+    steps_inner = h["inner_iters"] / args.steps
+    kern = {}
+    for kid, kname, per_step in ((fsx.KERNEL_STEP1, STEP_KERNELS[0], steps_inner),
+                                 (fsx.KERNEL_STEP2_RHS, STEP_KERNELS[1], steps_inner),
+                                 (fsx.KERNEL_STEP3, STEP_KERNELS[2], steps_inner)):
+        ms_k, by_k = solver.bench_kernel(kid, reps=20)
+        kern[kname] = dict(launch_ms=ms_k, bytes=by_k, ms_per_step=ms_k * per_step)
     ms, bytes_per = solver.bench_spmv(precision=prec, reps=100)
     ms64, bytes64 = solver.bench_spmv(precision=0, reps=100)
-    achieved = bytes_per / (ms * 1e-3) / 1e9
     npat, _ = solver.z_patterns()
-    if npat > 0:   # the row-pattern form runs 8 lanes x 16 loads per row in both precisions
-        kname = "k_pcg_spmv<float, 8, 16, true>" if prec else "k_pcg_spmv<double, 8, 16, true>"
-    else:
-        kname = "k_pcg_spmv<float, 32, 4, false>" if prec else "k_pcg_spmv<double, 16, 8, false>"
+    _, ncls, _ = solver.z_forms()
+    zname = zspmv_name(prec, npat, ncls)
+    kern[zname] = dict(launch_ms=ms, bytes=bytes_per, ms_per_step=ms * h["pcg_iters"] / args.steps)
+    kname = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    achieved = kern[kname]["bytes"] / (kern[kname]["launch_ms"] * 1e-3) / 1e9
     h = res[head]
     out = {
         "metric": METRIC,
@@ -224,7 +251,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
-                     "algorithmic_bytes_per_launch": bytes_per, "launch_ms": round(ms, 5)},
+                     "algorithmic_bytes_per_launch": kern[kname]["bytes"],
+                     "launch_ms": round(kern[kname]["launch_ms"], 5)},
+        "kernels": {k: {"launch_ms": round(v["launch_ms"], 5), "algorithmic_bytes": v["bytes"],
+                        "achieved_GBs": round(v["bytes"] / (v["launch_ms"] * 1e-3) / 1e9, 1),
+                        "ms_per_step": round(v["ms_per_step"], 4), "traffic": measured_traffic(k)}
+                    for k, v in kern.items()},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 4), "pcg_iters": v["pcg_iters"],
                       "inner_iters": v["inner_iters"], "solve_ms": round(v["solve_ms"], 2),
                       "max_relres": v["max_relres"]} for k, v in res.items()},

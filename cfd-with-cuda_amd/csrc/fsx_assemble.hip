@@ -287,8 +287,8 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
       nn = L27[(size_t)e * 27 + l];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        un[c] = Un[(size_t)c * NN + nn];
-        up[c] = Uprev[(size_t)c * NN + nn];
+        un[c] = Un[3 * (size_t)nn + c];
+        up[c] = Uprev[3 * (size_t)nn + c];
       }
     }
     const int cn = __shfl(nn, (lane & 32) + (l < 24 ? l / 3 : 0), kWave);
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
         for (int c = 0; c < 3; c++) r[c] += sv * conv[hw][c][k];
       }
 #pragma unroll
-      for (int c = 0; c < 3; c++) R1[(size_t)c * NN + node] -= r[c];
+      for (int c = 0; c < 3; c++) R1[3 * (size_t)node + c] -= r[c];
     }
     wave_sync();   // the next element's staging overwrites this one's LDS
   }

@@ -586,6 +586,7 @@ int fsx_init_state(fsx_ctx* ctx) {
   if (!ctx) return FSX_EINVAL;
   if (!ctx->have_mesh || !ctx->have_bc) return fail(ctx, FSX_ESTATE, "fsx_init_state needs mesh and BCs");
   const long long n3 = 3LL * ctx->NN;
+  ctx->kacc_pending = false;
   for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
                     ctx->KAcc, ctx->R1})
     FSX_TRY(launch_fill(ctx, v, 0.0, n3));
@@ -597,7 +598,7 @@ int fsx_init_state(fsx_ctx* ctx) {
     FSX_HIP(hipMemcpy(nodes.data(), ctx->bcnodes, nodes.size() * sizeof(int), hipMemcpyDeviceToHost));
     FSX_HIP(hipMemcpy(vals.data(), ctx->bcvals, vals.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < ctx->nbc; i++)
-      for (int c = 0; c < 3; c++) un[(size_t)c * ctx->NN + nodes[i]] = vals[(size_t)i*3 + c];
+      for (int c = 0; c < 3; c++) un[3 * (size_t)nodes[i] + c] = vals[(size_t)i*3 + c];
     FSX_HIP(hipStreamSynchronize(ctx->stream));
     FSX_HIP(hipMemcpy(ctx->Un, un.data(), un.size() * sizeof(double), hipMemcpyHostToDevice));
   }
@@ -613,6 +614,7 @@ static int need_step(fsx_ctx* ctx) {
 
 int fsx_begin_step(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
+  FSX_TRY(settle_kacc(ctx));
   FSX_TRY(launch_copy(ctx, ctx->UnpHalf_prev, ctx->Un, 3LL * ctx->NN));    // BCH:2872-2876
   FSX_TRY(launch_fill(ctx, ctx->Acc_prev, 0.0, 3LL * ctx->NN));
   FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pn, ctx->NNp));
@@ -624,6 +626,7 @@ int fsx_step1(fsx_ctx* ctx, int iter) { FSX_TRY(need_step(ctx)); return launch_s
 
 int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres) {
   FSX_TRY(need_step(ctx));
+  FSX_TRY(settle_kacc(ctx));
   FSX_TRY(launch_step2_rhs(ctx, iter));
   int it = 0;
   double rel = 0.0;
@@ -635,7 +638,11 @@ int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres) {
   return rc;
 }
 
-int fsx_step3(fsx_ctx* ctx, int iter) { FSX_TRY(need_step(ctx)); return launch_step3(ctx, iter); }
+int fsx_step3(fsx_ctx* ctx, int iter) {
+  FSX_TRY(need_step(ctx));
+  FSX_TRY(settle_kacc(ctx));
+  return launch_step3(ctx, iter);
+}
 
 int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_u, double* rel_p) {
   FSX_TRY(need_step(ctx));
@@ -651,6 +658,7 @@ int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_
 
 int fsx_next_iteration(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
+  FSX_TRY(settle_kacc(ctx));
   // BCH:2964-2972. UnpHalf and Acc are rewritten by step1/step3 before they
   // are read again, so their "prev" copies are pointer swaps. Unp1 and Pnp1
   // must keep their values: when this was the last inner iteration the time
@@ -662,10 +670,18 @@ int fsx_next_iteration(fsx_ctx* ctx) {
   return FSX_OK;
 }
 
-int fsx_kacc(fsx_ctx* ctx) { FSX_TRY(need_step(ctx)); return launch_kacc(ctx); }
+// KAcc = K Acc_prev: recorded here and formed by the next fsx_step1 in its
+// pass over K (k_step1 with KACC); any other entry point that reads KAcc or
+// changes Acc_prev first forms it on its own (settle_kacc).
+int fsx_kacc(fsx_ctx* ctx) {
+  FSX_TRY(need_step(ctx));
+  ctx->kacc_pending = true;
+  return FSX_OK;
+}
 
 int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
   FSX_TRY(need_step(ctx));
+  FSX_TRY(settle_kacc(ctx));
   FSX_TRY(launch_maxacc(ctx));
   FSX_HIP(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(StepScalars), hipMemcpyDeviceToHost, ctx->stream));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -681,7 +697,7 @@ int fsx_monitor(fsx_ctx* ctx, int node, double* out4) {
   if (!ctx || !out4) return FSX_EINVAL;
   if (node < 0 || node >= ctx->NNp) return fail(ctx, FSX_EINVAL, "monitor node out of range");
   for (int c = 0; c < 3; c++)
-    FSX_HIP(hipMemcpyAsync(ctx->h_mon + c, ctx->Un + (size_t)c * ctx->NN + node, sizeof(double), hipMemcpyDeviceToHost,
+    FSX_HIP(hipMemcpyAsync(ctx->h_mon + c, ctx->Un + 3 * (size_t)node + c, sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
   FSX_HIP(hipMemcpyAsync(ctx->h_mon + 3, ctx->Pn + node, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -690,18 +706,19 @@ int fsx_monitor(fsx_ctx* ctx, int node, double* out4) {
 }
 
 static int vec_info(fsx_ctx* ctx, int id, double** p, size_t* n, int* kind) {
-  // kind 0: plain, 1: NN vector replicated x3, 2: component c of G/GT (c = id - FSX_VAL_G1)
+  // kind 0: plain, 1: NN vector replicated x3, 2: component c of G/GT (c = id - FSX_VAL_G1),
+  // 3: velocity vector, [u|v|w] at the boundary and node-major (u, v, w of a node together) on the device
   const size_t n3 = (size_t)3 * ctx->NN;
   *kind = 0;
   switch (id) {
-    case FSX_VEC_UN: *p = ctx->Un; *n = n3; break;
-    case FSX_VEC_UNP1: *p = ctx->Unp1; *n = n3; break;
-    case FSX_VEC_UNP1_PREV: *p = ctx->Unp1_prev; *n = n3; break;
-    case FSX_VEC_UNPHALF: *p = ctx->UnpHalf; *n = n3; break;
-    case FSX_VEC_UNPHALF_PREV: *p = ctx->UnpHalf_prev; *n = n3; break;
-    case FSX_VEC_ACC: *p = ctx->Acc; *n = n3; break;
-    case FSX_VEC_KACC: *p = ctx->KAcc; *n = n3; break;
-    case FSX_VEC_R1: *p = ctx->R1; *n = n3; break;
+    case FSX_VEC_UN: *p = ctx->Un; *n = n3; *kind = 3; break;
+    case FSX_VEC_UNP1: *p = ctx->Unp1; *n = n3; *kind = 3; break;
+    case FSX_VEC_UNP1_PREV: *p = ctx->Unp1_prev; *n = n3; *kind = 3; break;
+    case FSX_VEC_UNPHALF: *p = ctx->UnpHalf; *n = n3; *kind = 3; break;
+    case FSX_VEC_UNPHALF_PREV: *p = ctx->UnpHalf_prev; *n = n3; *kind = 3; break;
+    case FSX_VEC_ACC: *p = ctx->Acc; *n = n3; *kind = 3; break;
+    case FSX_VEC_KACC: *p = ctx->KAcc; *n = n3; *kind = 3; break;
+    case FSX_VEC_R1: *p = ctx->R1; *n = n3; *kind = 3; break;
     case FSX_VEC_PN: *p = ctx->Pn; *n = ctx->NNp; break;
     case FSX_VEC_PNP1: *p = ctx->Pnp1; *n = ctx->NNp; break;
     case FSX_VEC_PNP1_PREV: *p = ctx->Pnp1_prev; *n = ctx->NNp; break;
@@ -724,9 +741,16 @@ int fsx_download(fsx_ctx* ctx, int id, double* host, size_t count) {
   double* p; size_t n; int kind;
   FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_download: count mismatch (expected " + std::to_string(n) + ")");
+  FSX_TRY(settle_kacc(ctx));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   if (kind == 0) {
     FSX_HIP(hipMemcpy(host, p, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (kind == 3) {
+    const size_t NN = ctx->NN;
+    std::vector<double> t(n);
+    FSX_HIP(hipMemcpy(t.data(), p, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < NN; i++)
+      for (int c = 0; c < 3; c++) host[c * NN + i] = t[3 * i + c];
   } else if (kind == 1) {
     const size_t NN = ctx->NN;
     FSX_HIP(hipMemcpy(host, p, NN * sizeof(double), hipMemcpyDeviceToHost));
@@ -745,10 +769,19 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   if (!ctx || !host) return FSX_EINVAL;
   double* p; size_t n; int kind;
   FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
-  if (kind != 0) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
+  if (kind != 0 && kind != 3) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
+  FSX_TRY(settle_kacc(ctx));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
   FSX_HIP(hipStreamSynchronize(ctx->stream));
-  FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
+  if (kind == 3) {
+    const size_t NN = ctx->NN;
+    std::vector<double> t(n);
+    for (size_t i = 0; i < NN; i++)
+      for (int c = 0; c < 3; c++) t[3 * i + c] = host[c * NN + i];
+    FSX_HIP(hipMemcpy(p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
+  }
   return FSX_OK;
 }
 
@@ -761,6 +794,7 @@ int fsx_sync(fsx_ctx* ctx) {
 int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   if (!ctx || !b || !x) return FSX_EINVAL;
   if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  FSX_TRY(settle_kacc(ctx));
   FSX_HIP(hipMemcpyAsync(ctx->R2, b, (size_t)ctx->NNp * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   int it = 0;
   double rel = 0;
@@ -777,6 +811,7 @@ int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, 
   if (!ctx || !x || !y) return FSX_EINVAL;
   if (!ctx->assembled) return fail(ctx, FSX_ESTATE, "assemble first");
   if ((which == 4 || which == 8 || which == 9) && !ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  FSX_TRY(settle_kacc(ctx));
   if ((which == 8 || which == 9) && !(alpha == 1.0 && beta == 0.0))
     return fail(ctx, FSX_EINVAL, "fsx_spmv: the solvers' Z forms take alpha = 1, beta = 0");
   size_t nx, ny;

@@ -100,6 +100,7 @@ struct fsx_ctx {
   int maxit = 5000;
   bool have_mesh = false, have_M = false, have_G = false, have_Z = false, have_bc = false, have_params = false;
   bool assembled = false, z_built = false, state_ready = false;
+  bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_kacc)
   // mesh (device)
   int* L27 = nullptr;            // NE*27
   double* xc = nullptr;          // NNp*3 corner coordinates
@@ -145,7 +146,11 @@ struct fsx_ctx {
   unsigned char* bcflag = nullptr;
   int nbc = 0; int* bcnodes = nullptr; double* bcvals = nullptr;
   int zeroP = -1;
-  // state (device)
+  // state (device). Velocity-sized vectors (3NN) are node-major: u, v, w of
+  // node i at 3i, 3i+1, 3i+2, so the gathers of the K, G and G^T row kernels
+  // and of the element kernels fetch one 24-byte run per node instead of
+  // three scattered words. fsx_download / fsx_upload_vec convert to and from
+  // the reference's [u|v|w] layout.
   double *Un = nullptr, *Unp1 = nullptr, *Unp1_prev = nullptr, *UnpHalf = nullptr, *UnpHalf_prev = nullptr;
   double *Acc = nullptr, *Acc_prev = nullptr, *KAcc = nullptr, *R1 = nullptr;
   double *Pn = nullptr, *Pnp1 = nullptr, *Pnp1_prev = nullptr, *Pdot = nullptr, *R2 = nullptr;
@@ -221,6 +226,7 @@ int launch_step2_rhs(fsx_ctx* ctx, int iter);
 int launch_pnp1(fsx_ctx* ctx);
 int launch_step3(fsx_ctx* ctx, int iter);
 int launch_kacc(fsx_ctx* ctx);
+int settle_kacc(fsx_ctx* ctx);
 int launch_maxacc(fsx_ctx* ctx);
 int launch_fill(fsx_ctx* ctx, double* p, double v, long long n);
 int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n);

@@ -46,7 +46,12 @@ int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n) {
   return launch_check(ctx, "k_copy");
 }
 
-// Row of this lane group and the lane inside the group.
+// Row of this lane group and the lane inside the group. Blocks take rows in
+// launch order: workgroups are dealt to the 8 XCDs round robin, so all XCDs
+// sweep the same region of rows at once and share the x-gathers through the
+// Infinity Cache. (Handing each XCD its own contiguous slab of rows instead
+// measured slower on the 64^3 cavity: K*Acc 0.67 -> 1.00 ms, step3 0.32 ->
+// 0.48 ms.)
 template <int LPR>
 __device__ __forceinline__ long long my_row(int* lane) {
   *lane = threadIdx.x % LPR;
@@ -55,19 +60,23 @@ __device__ __forceinline__ long long my_row(int* lane) {
 
 // ------------------------------------------------------------ step 1
 // R1 <- (R1 - K UnpHalf_prev) - G Pn; BC rows 0; UnpHalf = Un + dt R1 MdInv.
-// (BCH:3746-3783; the convection part of R1 is already in R1.)
-template <int LPR, int UK, int UG>
+// (BCH:3746-3783; the convection part of R1 is already in R1.) With KACC the
+// same pass over K also forms KAcc = K Acc_prev (BCH:2981-3035), which the
+// reference computes at the end of the previous inner iteration: K's columns
+// and values are read once for six right-hand sides instead of twice for
+// three.
+template <int LPR, int UK, int UG, bool KACC>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, const int* __restrict__ Mcol,
                                                   const double* __restrict__ K, const int* __restrict__ Grow,
                                                   const int* __restrict__ Gcol, const double* __restrict__ G,
                                                   const double* __restrict__ Uprev, const double* __restrict__ Pn,
                                                   const double* __restrict__ Un, double* __restrict__ R1,
                                                   const double* __restrict__ MdInv, const unsigned char* __restrict__ bc,
-                                                  double* __restrict__ UnpHalf, double dt) {
+                                                  double* __restrict__ UnpHalf, double dt,
+                                                  const double* __restrict__ Accp, double* __restrict__ KAcc) {
   int l;
   const long long r = my_row<LPR>(&l);
-  const size_t n1 = NN, n2 = 2 * (size_t)NN;
-  double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0;
+  double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
   if (r < NN) {
     const int rs = Mrow[r], re = Mrow[r + 1];
     for (int base = rs + l; base < re; base += UK * LPR) {
@@ -81,48 +90,60 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
       }
 #pragma unroll
       for (int j = 0; j < UK; j++) {
-        k0 += a[j] * Uprev[c[j]]; k1 += a[j] * Uprev[n1 + c[j]]; k2 += a[j] * Uprev[n2 + c[j]];
+        const size_t x = 3 * (size_t)c[j];
+        k0 += a[j] * Uprev[x]; k1 += a[j] * Uprev[x + 1]; k2 += a[j] * Uprev[x + 2];
+        if constexpr (KACC) { a0 += a[j] * Accp[x]; a1 += a[j] * Accp[x + 1]; a2 += a[j] * Accp[x + 2]; }
       }
     }
     const int gs = Grow[r], ge = Grow[r + 1];
     for (int base = gs + l; base < ge; base += UG * LPR) {
       int c[UG];
-      double a0[UG], a1[UG], a2[UG];
+      double b0[UG], b1[UG], b2[UG];
 #pragma unroll
       for (int j = 0; j < UG; j++) {
         const int k = base + j * LPR, kk = k < ge ? k : ge - 1;
         c[j] = Gcol[kk];
         const bool ok = k < ge;
-        a0[j] = ok ? G[3 * (size_t)kk] : 0.0;
-        a1[j] = ok ? G[3 * (size_t)kk + 1] : 0.0;
-        a2[j] = ok ? G[3 * (size_t)kk + 2] : 0.0;
+        b0[j] = ok ? G[3 * (size_t)kk] : 0.0;
+        b1[j] = ok ? G[3 * (size_t)kk + 1] : 0.0;
+        b2[j] = ok ? G[3 * (size_t)kk + 2] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < UG; j++) {
         const double p = Pn[c[j]];
-        g0 += a0[j] * p; g1 += a1[j] * p; g2 += a2[j] * p;
+        g0 += b0[j] * p; g1 += b1[j] * p; g2 += b2[j] * p;
       }
     }
   }
   k0 = group_sum<LPR>(k0); k1 = group_sum<LPR>(k1); k2 = group_sum<LPR>(k2);
   g0 = group_sum<LPR>(g0); g1 = group_sum<LPR>(g1); g2 = group_sum<LPR>(g2);
+  if constexpr (KACC) { a0 = group_sum<LPR>(a0); a1 = group_sum<LPR>(a1); a2 = group_sum<LPR>(a2); }
   if (r < NN && l < 3) {
     const double kk = l == 0 ? k0 : (l == 1 ? k1 : k2);
     const double gg = l == 0 ? g0 : (l == 1 ? g1 : g2);
-    const size_t i = (size_t)l * NN + r;
+    const size_t i = 3 * (size_t)r + l;
     double R = (R1[i] - kk) - gg;
     if (bc[r]) R = 0.0;
     R1[i] = R;
     UnpHalf[i] = Un[i] + dt * R * MdInv[r];
+    if constexpr (KACC) KAcc[i] = l == 0 ? a0 : (l == 1 ? a1 : a2);
   }
 }
 
 int launch_step1(fsx_ctx* ctx, int iter) {
   (void)iter;
   constexpr int LPR = 8;   // 8 lanes x 8 K entries per row: 0.93 ms against 1.01 ms for 16 x 4 at 64^3
-  hipLaunchKernelGGL((k_step1<LPR, 8, 2>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
-                     ctx->Mrow, ctx->Mcol, ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un,
-                     ctx->R1, ctx->MdInv, ctx->bcflag, ctx->UnpHalf, ctx->dt);
+  const dim3 grid(rows_grid(ctx->NN, LPR));
+  if (ctx->kacc_pending) {
+    hipLaunchKernelGGL((k_step1<LPR, 8, 2, true>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
+                       ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,
+                       ctx->bcflag, ctx->UnpHalf, ctx->dt, ctx->Acc_prev, ctx->KAcc);
+    ctx->kacc_pending = false;
+  } else {
+    hipLaunchKernelGGL((k_step1<LPR, 8, 2, false>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
+                       ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,
+                       ctx->bcflag, ctx->UnpHalf, ctx->dt, nullptr, nullptr);
+  }
   return launch_check(ctx, "k_step1");
 }
 
@@ -134,7 +155,6 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
                                                double* __restrict__ y) {
   int l;
   const long long r = my_row<LPR>(&l);
-  const size_t n1 = NN, n2 = 2 * (size_t)NN;
   double k0 = 0, k1 = 0, k2 = 0;
   if (r < NN) {
     const int rs = Mrow[r], re = Mrow[r + 1];
@@ -148,19 +168,28 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
         a[j] = k < re ? K[kk] : 0.0;
       }
 #pragma unroll
-      for (int j = 0; j < U; j++) { k0 += a[j] * x[c[j]]; k1 += a[j] * x[n1 + c[j]]; k2 += a[j] * x[n2 + c[j]]; }
+      for (int j = 0; j < U; j++) {
+        const size_t xi = 3 * (size_t)c[j];
+        k0 += a[j] * x[xi]; k1 += a[j] * x[xi + 1]; k2 += a[j] * x[xi + 2];
+      }
     }
   }
   k0 = group_sum<LPR>(k0); k1 = group_sum<LPR>(k1); k2 = group_sum<LPR>(k2);
-  if (r < NN && l < 3) y[(size_t)l * NN + r] = l == 0 ? k0 : (l == 1 ? k1 : k2);
+  if (r < NN && l < 3) y[3 * (size_t)r + l] = l == 0 ? k0 : (l == 1 ? k1 : k2);
 }
 
 int launch_kacc(fsx_ctx* ctx) {
   constexpr int LPR = 16;
   hipLaunchKernelGGL((k_k3<LPR, 4>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,
                      ctx->Mcol, ctx->K, ctx->Acc_prev, ctx->KAcc);
+  ctx->kacc_pending = false;
   return launch_check(ctx, "k_k3");
 }
+
+// KAcc = K Acc_prev is deferred from fsx_kacc to the next step1, which forms
+// it in its own pass over K; every other entry point that could read KAcc or
+// change Acc_prev first settles a pending product with k_k3.
+int settle_kacc(fsx_ctx* ctx) { return ctx->kacc_pending ? launch_kacc(ctx) : FSX_OK; }
 
 // ------------------------------------------------------------ step 2 RHS
 // R2[p] = sum_c G_c^T d_c, d_c = UnpHalf_c / dt^2 - [iter>1] MdOrigInv KAcc_c,
@@ -174,7 +203,6 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
                                                       int zeroP, double* __restrict__ R2) {
   int l;
   const long long p = my_row<LPR>(&l);
-  const size_t n1 = NN, n2 = 2 * (size_t)NN;
   double s0 = 0, s1 = 0, s2 = 0;
   if (p < NNp) {
     const int rs = GTrow[p], re = GTrow[p + 1];
@@ -193,10 +221,11 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int i = c[j];
-        double d0 = o2 * UnpHalf[i], d1 = o2 * UnpHalf[n1 + i], d2 = o2 * UnpHalf[n2 + i];
+        const size_t x = 3 * (size_t)i;
+        double d0 = o2 * UnpHalf[x], d1 = o2 * UnpHalf[x + 1], d2 = o2 * UnpHalf[x + 2];
         if (use_kacc) {
           const double m = MdOrigInv[i];
-          d0 = d0 - m * KAcc[i]; d1 = d1 - m * KAcc[n1 + i]; d2 = d2 - m * KAcc[n2 + i];
+          d0 = d0 - m * KAcc[x]; d1 = d1 - m * KAcc[x + 1]; d2 = d2 - m * KAcc[x + 2];
         }
         s0 += a0[j] * d0; s1 += a1[j] * d1; s2 += a2[j] * d2;
       }
@@ -293,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
   g0 = group_sum<LPR>(g0); g1 = group_sum<LPR>(g1); g2 = group_sum<LPR>(g2);
   if (r < NN && l < 3) {
     const double gg = l == 0 ? g0 : (l == 1 ? g1 : g2);
-    const size_t i = (size_t)l * NN + r;
+    const size_t i = 3 * (size_t)r + l;
     double R = -dt * gg;
     if (use_kacc) R = R - dt * KAcc[i];
     if (bc[r]) R = 0.0;
@@ -303,11 +332,15 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
   }
 }
 
-int launch_step3(fsx_ctx* ctx, int iter) {
+static void launch_step3_rows(fsx_ctx* ctx, int iter) {
   constexpr int LPR = 4;
   hipLaunchKernelGGL((k_step3<LPR, 4>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
                      ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv, ctx->UnpHalf,
                      ctx->Acc, ctx->Unp1, ctx->dt);
+}
+
+int launch_step3(fsx_ctx* ctx, int iter) {
+  launch_step3_rows(ctx, iter);
   hipLaunchKernelGGL(k_vnorms, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, 3LL * ctx->NN, ctx->Unp1, ctx->Unp1_prev,
                      ctx->partials);
   hipLaunchKernelGGL(k_finish2, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, &ctx->st->uu);
@@ -932,6 +965,55 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
     }
   }
   *ms_per_launch = ms;
+  if (bytes_per_launch) *bytes_per_launch = bytes;
+  return FSX_OK;
+}
+
+// Times `reps` back-to-back launches of one row kernel of the time step on
+// the context's stream (HIP events) and returns its algorithmic HBM bytes:
+// every matrix entry, row pointer and vector element it touches counted once
+// (int32 indices, FP64 values and vectors). The step vectors are clobbered.
+extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_per_launch, double* bytes_per_launch) {
+  using namespace fsx;
+  if (!ctx || reps < 1 || !ms_per_launch) return FSX_EINVAL;
+  if (!ctx->K || !ctx->G || !ctx->GT) return fail(ctx, FSX_ESTATE, "fsx_bench_kernel: assemble first");
+  if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP3) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
+  const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG;
+  auto launch = [&]() {
+    switch (kernel) {
+      case FSX_KERNEL_STEP1: ctx->kacc_pending = true; (void)launch_step1(ctx, 2); break;
+      case FSX_KERNEL_KACC: (void)launch_kacc(ctx); break;
+      case FSX_KERNEL_STEP2_RHS: (void)launch_step2_rhs(ctx, 2); break;
+      default: launch_step3_rows(ctx, 2); break;
+    }
+  };
+  double bytes = 0;
+  switch (kernel) {
+    case FSX_KERNEL_STEP1:   // K (col + value), G (col + 3 values), Uprev, Pn, R1 r/w, Un, MdInv, bc, UnpHalf,
+                             // Acc_prev, KAcc
+      bytes = 12.0 * nK + 4.0 * (NN + 1) + 24.0 * NN + 28.0 * nG + 4.0 * (NN + 1) + 8.0 * NNp + 48.0 * NN + 24.0 * NN +
+              8.0 * NN + 1.0 * NN + 24.0 * NN + 24.0 * NN + 24.0 * NN;
+      break;
+    case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
+      bytes = 12.0 * nK + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
+      break;
+    case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2
+      bytes = 28.0 * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
+      break;
+    default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc, Unp1
+      bytes = 28.0 * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN + 24.0 * NN;
+      break;
+  }
+  FSX_TRY(settle_kacc(ctx));
+  for (int w = 0; w < 2; w++) launch();
+  FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  for (int r = 0; r < reps; r++) launch();
+  FSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  FSX_HIP(hipEventSynchronize(ctx->ev1));
+  FSX_TRY(launch_check(ctx, "fsx_bench_kernel"));
+  float t = 0;
+  FSX_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+  *ms_per_launch = t / reps;
   if (bytes_per_launch) *bytes_per_launch = bytes;
   return FSX_OK;
 }

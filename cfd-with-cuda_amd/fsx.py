@@ -22,6 +22,7 @@ _vp = ctypes.c_void_p
 FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTATE, FSX_ENOCONV, FSX_ENOMEM, FSX_ENUMERIC = range(7)
 CSR_M, CSR_G, CSR_Z = 0, 1, 2
 SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
+KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3 = 0, 1, 2, 3
 
 VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, R1=7, Pn=8, Pnp1=9,
            Pnp1_prev=10, Pdot=11, R2=12, MdInv=13, MdOrigInv=14, K=15, M=16, G1=17, G2=18, G3=19, Z=20)
@@ -32,7 +33,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_copy", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
@@ -105,6 +106,7 @@ def lib():
         "fsx_solve_pressure": ([_vp, _D, _D, _I, _D], ctypes.c_int),
         "fsx_spmv": ([_vp, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
+        "fsx_bench_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_bench_copy": ([_vp, ctypes.c_size_t, ctypes.c_int, _D], ctypes.c_int),
         "fsx_z_patterns": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "fsx_z_forms": ([_vp, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
@@ -363,6 +365,15 @@ class Solver:
         ms = ctypes.c_double()
         by = ctypes.c_double()
         self.check(self.L.fsx_bench_spmv(self.ctx, precision, reps, ctypes.byref(ms), ctypes.byref(by)), "bench_spmv")
+        return ms.value, by.value
+
+    def bench_kernel(self, kernel, reps=20):
+        """(ms per launch, algorithmic bytes per launch) of one step row kernel
+        (KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3); clobbers
+        the step vectors (reset() before stepping again)."""
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        self.check(self.L.fsx_bench_kernel(self.ctx, kernel, reps, ctypes.byref(ms), ctypes.byref(by)), "bench_kernel")
         return ms.value, by.value
 
     def z_patterns(self):

@@ -150,7 +150,10 @@ int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_
  * Pnp1_prev <- Pnp1 (BCH:2964-2972, CU via BCH:2925-2932); Unp1 and Pnp1
  * keep their values (they become Un, Pn after the last iteration). */
 int fsx_next_iteration(fsx_ctx* ctx);
-/* KAcc = K Acc_prev; replaces calculate_KtimesAcc_prevGPU (CU:1067-1077) */
+/* KAcc = K Acc_prev; replaces calculate_KtimesAcc_prevGPU (CU:1067-1077).
+ * The product is formed by the next fsx_step1 in the same pass over K as
+ * K UnpHalf_prev; any other call that reads KAcc or changes Acc_prev first
+ * forms it on its own, so the result is the same at every observable point. */
 int fsx_kacc(fsx_ctx* ctx);
 /* maxAcc = max |Unp1 - Un| / dt (BCH:3048-3062; the reference GPU variant
  * omits the /dt, CU:99-118), steady = maxAcc <= converge; then Un <- Unp1,
@@ -206,6 +209,22 @@ int fsx_z_forms(fsx_ctx* ctx, long long* nnz_solver, int* nclasses, long long* c
  * known-traffic kernel that calibrates PMC byte counters. No reference
  * counterpart. */
 int fsx_bench_copy(fsx_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
+
+/* Row kernels of the time step for fsx_bench_kernel */
+enum fsx_kernel_id {
+  FSX_KERNEL_STEP1 = 0,      /* K 3-RHS + G 3-block SpMV with the step1 update (step1GPUpart, CU:946-973) */
+  FSX_KERNEL_KACC = 1,       /* K 3-RHS SpMV (calculate_KtimesAcc_prevGPU, CU:1067-1077) */
+  FSX_KERNEL_STEP2_RHS = 2,  /* G^T 3-block SpMV forming R2 (step2GPU, CU:980-1011) */
+  FSX_KERNEL_STEP3 = 3       /* G 3-block SpMV with the step3 update (step3GPU, CU:1029-1060) */
+};
+/* Times `reps` back-to-back launches of one of those kernels (inner
+ * iteration >= 2 form, with KAcc) on the context's stream with HIP events:
+ * mean device time per launch in ms, and its algorithmic HBM bytes (every
+ * matrix entry, row pointer and vector element it touches counted once;
+ * int32 indices, FP64 values). Clobbers the step vectors: call
+ * fsx_init_state before stepping again. No reference counterpart (the
+ * reference prints wall times only, BCH:4489-4518). */
+int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_per_launch, double* bytes_per_launch);
 
 /* kernel time accounting (device time of the last timed region, ms) */
 typedef struct fsx_timing {
