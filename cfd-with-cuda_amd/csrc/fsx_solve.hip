@@ -132,15 +132,18 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
 
 int launch_step1(fsx_ctx* ctx, int iter) {
   (void)iter;
-  constexpr int LPR = 8;   // 8 lanes x 8 K entries per row: 0.93 ms against 1.01 ms for 16 x 4 at 64^3
+  // 8 lanes x 2 K entries in flight per row. Fused with K*Acc at 64^3
+  // (node-major vectors): 8x2 1.064 ms, 16x2 1.057, 8x4 1.077, 16x4 1.092,
+  // 8x8 1.150, 8x16 1.217, 16x8 1.224, 4x16 1.352, 32x2 1.287
+  constexpr int LPR = 8, UK = 2;
   const dim3 grid(rows_grid(ctx->NN, LPR));
   if (ctx->kacc_pending) {
-    hipLaunchKernelGGL((k_step1<LPR, 8, 2, true>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
+    hipLaunchKernelGGL((k_step1<LPR, UK, 2, true>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
                        ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,
                        ctx->bcflag, ctx->UnpHalf, ctx->dt, ctx->Acc_prev, ctx->KAcc);
     ctx->kacc_pending = false;
   } else {
-    hipLaunchKernelGGL((k_step1<LPR, 8, 2, false>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
+    hipLaunchKernelGGL((k_step1<LPR, UK, 2, false>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
                        ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,
                        ctx->bcflag, ctx->UnpHalf, ctx->dt, nullptr, nullptr);
   }
@@ -236,6 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
 }
 
 int launch_step2_rhs(fsx_ctx* ctx, int iter) {
+  // 64^3: 16x8 0.307 ms, 16x4 0.310, 32x4 0.316, 16x2 0.312, 8x4 0.344, 8x8 0.346, 64x2 0.372, 4x8 0.429
   constexpr int LPR = 16;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
   hipLaunchKernelGGL((k_step2_rhs<LPR, 8>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->NN,
@@ -333,8 +337,9 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
 }
 
 static void launch_step3_rows(fsx_ctx* ctx, int iter) {
-  constexpr int LPR = 4;
-  hipLaunchKernelGGL((k_step3<LPR, 4>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
+  // 64^3: 8x2 0.253 ms, 8x4 0.266, 4x2 0.290, 4x8 0.298, 4x4 0.307, 16x2 0.312, 8x8 0.319, 16x4 0.343
+  constexpr int LPR = 8;
+  hipLaunchKernelGGL((k_step3<LPR, 2>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
                      ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv, ctx->UnpHalf,
                      ctx->Acc, ctx->Unp1, ctx->dt);
 }
