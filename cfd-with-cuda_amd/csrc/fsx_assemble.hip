@@ -14,9 +14,9 @@ namespace fsx {
 
 // Jacobian of the trilinear (hex8) map at one Gauss point, its cofactor
 // inverse and determinant, in the arithmetic of calcJacob (BCH:2553-2590):
-// J[i][j] = sum_m dSp[m][i] x_m[j].
+// J[i][j] = sum_m dSp[m][i] x_m[j], dSp[m][i] read at dSp_k[(m * 3 + i) * stride].
 __device__ __forceinline__ void jacobian(const double (*ec)[3], const double* __restrict__ dSp_k, double iJ[3][3],
-                                         double* det) {
+                                         double* det, int stride = 1) {
   double J[3][3];
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -24,7 +24,7 @@ __device__ __forceinline__ void jacobian(const double (*ec)[3], const double* __
     for (int j = 0; j < 3; j++) {
       double s = 0;
 #pragma unroll
-      for (int m = 0; m < 8; m++) s = s + dSp_k[m * 3 + i] * ec[m][j];
+      for (int m = 0; m < 8; m++) s = s + dSp_k[(m * 3 + i) * stride] * ec[m][j];
       J[i][j] = s;
     }
   iJ[0][0] =   J[1][1]*J[2][2] - J[2][1]*J[1][2];
@@ -240,11 +240,14 @@ int launch_build_Z(fsx_ctx* ctx) {
 // one element at a time (lane = Gauss point, then lane = node), so two
 // elements share a wavefront and 54 of its 64 lanes work. The next
 // element's node values and corner coordinates are loaded into registers
-// while the current one computes. LDS holds the tables packed per (Gauss
-// point, node) as (Sv, dSv/dxi, dSv/deta, dSv/dzeta) and the node values as
+// while the current one computes. LDS holds the tables packed per (node,
+// Gauss point) as (Sv, dSv/dxi, dSv/deta, dSv/dzeta) and the node values as
 // (Un_x, Un_y, Un_z, Uprev_x, Uprev_y, Uprev_z), so one step of the inner
 // sum reads five 16-byte words; the arithmetic and its order are those of
-// the per-element form.
+// the per-element form. Each table a lane-indexed phase reads has that
+// lane's index fastest (Gauss point in the first phase, node in the second),
+// so the lanes of a wavefront read consecutive LDS words instead of a few
+// banks at a stride of 27 or 24 entries.
 constexpr int kConvWaves = 4;      // 3 blocks per CU fit (140 VGPRs, 46 KB LDS); 8-wave blocks at 128 VGPRs spill and run slower
 constexpr int kConvHalves = 2 * kConvWaves;
 
@@ -260,16 +263,23 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
                                                                    const double* __restrict__ xc, ShapeTables tab, int NGP,
                                                                    int NN, const double* __restrict__ Un,
                                                                    const double* __restrict__ Uprev, double* __restrict__ R1) {
-  __shared__ double4 sT[kMaxNGP * 27];                 // (Sv, dSv_0, dSv_1, dSv_2) at [k][j]
-  __shared__ double sdSp[kMaxNGP * 24];
+  __shared__ double4 sT[27 * kMaxNGP];                 // (Sv, dSv_0, dSv_1, dSv_2) at [j][k]
+  __shared__ double sSv[kMaxNGP * 27];                 // Sv at [k][i]
+  __shared__ double sdSp[24 * kMaxNGP];                // dSp at [m * 3 + i][k]
   __shared__ double sw[kMaxNGP];
   __shared__ double2 nod[kConvHalves][27][3];          // (Un_x, Un_y), (Un_z, Up_x), (Up_y, Up_z)
   __shared__ double conv[kConvHalves][3][kMaxNGP];
   __shared__ double ec[kConvHalves][8][3];
   const int t = threadIdx.x;
-  for (int i = t; i < NGP * 27; i += blockDim.x)
-    sT[i] = make_double4(tab.Sv[i], tab.dSv[3 * i], tab.dSv[3 * i + 1], tab.dSv[3 * i + 2]);
-  for (int i = t; i < NGP * 24; i += blockDim.x) sdSp[i] = tab.dSp[i];
+  for (int i = t; i < NGP * 27; i += blockDim.x) {   // i = k * 27 + j
+    const int k = i / 27, j = i - 27 * k;
+    sT[j * NGP + k] = make_double4(tab.Sv[i], tab.dSv[3 * i], tab.dSv[3 * i + 1], tab.dSv[3 * i + 2]);
+    sSv[i] = tab.Sv[i];
+  }
+  for (int i = t; i < NGP * 24; i += blockDim.x) {   // i = k * 24 + m * 3 + d
+    const int k = i / 24, m3 = i - 24 * k;
+    sdSp[m3 * NGP + k] = tab.dSp[i];
+  }
   for (int i = t; i < NGP; i += blockDim.x) sw[i] = tab.w[i];
   __syncthreads();
   const int lane = t % kWave, l = lane & 31;
@@ -309,11 +319,11 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
     if (l < NGP) {
       const int k = l;
       double iJ[3][3], det;
-      jacobian(ec[hw], sdSp + k * 24, iJ, &det);
+      jacobian(ec[hw], sdSp + k, iJ, &det, NGP);
       const double f = det * sw[k];
       double u0[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       for (int j = 0; j < 27; j++) {
-        const double4 T = sT[k * 27 + j];
+        const double4 T = sT[j * NGP + k];
         const double2 a = nod[hw][j][0], b = nod[hw][j][1], c2 = nod[hw][j][2];
         const double u0v[3] = {a.x, a.y, b.x}, upv[3] = {b.y, c2.x, c2.y};
 #pragma unroll
@@ -335,7 +345,7 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
       const int i = l;
       double r[3] = {0, 0, 0};
       for (int k = 0; k < NGP; k++) {
-        const double sv = sT[k * 27 + i].x;
+        const double sv = sSv[k * 27 + i];
 #pragma unroll
         for (int c = 0; c < 3; c++) r[c] += sv * conv[hw][c][k];
       }
