@@ -54,7 +54,8 @@ __device__ __forceinline__ void jacobian(const double (*ec)[3], const double* __
 constexpr int kMaxNGP = 27;
 
 __global__ __launch_bounds__(kWave) void k_assemble(int ncolor_elems, const int* __restrict__ elems,
-                                                    const int* __restrict__ L27, const double* __restrict__ xc,
+                                                    const int* __restrict__ L27, const int* __restrict__ P8,
+                                                    const double* __restrict__ xc,
                                                     ShapeTables tab, int NGP, double nu, double inv_rho,
                                                     const int* __restrict__ Mrow, const int* __restrict__ Mcol,
                                                     double* __restrict__ Mv, double* __restrict__ Kv,
@@ -63,12 +64,13 @@ __global__ __launch_bounds__(kWave) void k_assemble(int ncolor_elems, const int*
   __shared__ double gd[kMaxNGP * 81];   // [k][j][d]
   __shared__ double fac[kMaxNGP];
   __shared__ double ec[8][3];
-  __shared__ int L[27];
+  __shared__ int L[27], P[8];   // velocity ids (internal), pressure ids of the corners
   const int t = threadIdx.x;
   const int e = elems[blockIdx.x];
   if (t < 27) L[t] = L27[(size_t)e * 27 + t];
+  if (t < 8) P[t] = P8[(size_t)e * 8 + t];
   __syncthreads();
-  if (t < 24) ec[t / 3][t % 3] = xc[3 * (size_t)L[t / 3] + t % 3];
+  if (t < 24) ec[t / 3][t % 3] = xc[3 * (size_t)P[t / 3] + t % 3];
   __syncthreads();
   if (t < NGP) {
     double iJ[3][3], det;
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(kWave) void k_assemble(int ncolor_elems, const int*
       g2 = g2 - inv_rho * sp * gi[2] * f;
     }
     const int r = L[i];
-    const int pos = find_col(Gcol, Grow[r], Grow[r + 1], L[j]);
+    const int pos = find_col(Gcol, Grow[r], Grow[r + 1], P[j]);
     Gv[3 * (size_t)pos] += g0;
     Gv[3 * (size_t)pos + 1] += g1;
     Gv[3 * (size_t)pos + 2] += g2;
@@ -152,7 +154,7 @@ int launch_assemble_static(fsx_ctx* ctx) {
     const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
     if (n == 0) continue;
     hipLaunchKernelGGL(k_assemble, dim3(n), dim3(kWave), 0, ctx->stream, n, ctx->color_order + ctx->color_ptr[c], ctx->L27,
-                       ctx->xc, ctx->tab, ctx->NGP, ctx->viscosity, 1.0 / ctx->density, ctx->Mrow, ctx->Mcol, ctx->M, ctx->K,
+                       ctx->P8, ctx->xc, ctx->tab, ctx->NGP, ctx->viscosity, 1.0 / ctx->density, ctx->Mrow, ctx->Mcol, ctx->M, ctx->K,
                        ctx->Grow, ctx->Gcol, ctx->G);
     FSX_TRY(launch_check(ctx, "k_assemble"));
   }
@@ -260,6 +262,7 @@ __device__ __forceinline__ void wave_sync() {
 
 __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_elems, const int* __restrict__ elems,
                                                                    const int* __restrict__ L27,
+                                                                   const int* __restrict__ P8,
                                                                    const double* __restrict__ xc, ShapeTables tab, int NGP,
                                                                    int NN, const double* __restrict__ Un,
                                                                    const double* __restrict__ Uprev, double* __restrict__ R1) {
@@ -301,7 +304,9 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
         up[c] = Uprev[3 * (size_t)nn + c];
       }
     }
-    const int cn = __shfl(nn, (lane & 32) + (l < 24 ? l / 3 : 0), kWave);
+    int pn = 0;   // pressure id (row of xc) of corner l
+    if (act && l < 8) pn = P8[(size_t)e * 8 + l];
+    const int cn = __shfl(pn, (lane & 32) + (l < 24 ? l / 3 : 0), kWave);
     if (act && l < 24) xv = xc[3 * (size_t)cn + l % 3];
   };
   fetch(slot);
@@ -371,7 +376,7 @@ int launch_convection(fsx_ctx* ctx) {
     if (n == 0) continue;
     const int g = std::min(ctx->conv_grid, (n + kConvHalves - 1) / kConvHalves);
     hipLaunchKernelGGL(k_convection, dim3(g), dim3(kWave * kConvWaves), 0, ctx->stream, n,
-                       ctx->color_order + ctx->color_ptr[c], ctx->L27, ctx->xc, ctx->tab, ctx->NGP, ctx->NN, ctx->Un,
+                       ctx->color_order + ctx->color_ptr[c], ctx->L27, ctx->P8, ctx->xc, ctx->tab, ctx->NGP, ctx->NN, ctx->Un,
                        ctx->UnpHalf_prev, ctx->R1);
   }
   return launch_check(ctx, "k_convection");

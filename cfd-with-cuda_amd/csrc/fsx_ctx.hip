@@ -297,10 +297,59 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
     }
   }
   ctx->NE = NE; ctx->NN = NN; ctx->NNp = NNp;
-  FSX_TRY(alloc(ctx, &ctx->L27, (size_t)NE * 27));
+  // internal velocity numbering: the nodes in the order of their coordinates
+  // (z, y, x), each node placed by the trilinear map of the first element
+  // that holds it, coordinates compared on a grid of 1e-9 of the mesh extent.
+  // (Ordering each grid line even-ranked x first, so that a wavefront's rows
+  // have equal lengths, measured no faster.)
+  {
+    std::vector<double> X((size_t)NN * 3, 0.0);
+    std::vector<char> have(NN, 0);
+    for (int e = 0; e < NE; e++) {
+      const int* L = LtoGnode + (size_t)e * 27;
+      for (int j = 0; j < 27; j++) {
+        const int nd = L[j];
+        if (have[nd]) continue;
+        have[nd] = 1;
+        for (int m = 0; m < 8; m++) {
+          const double w = 0.125 * (1 + kNodePos[m][0] * kNodePos[j][0]) * (1 + kNodePos[m][1] * kNodePos[j][1]) *
+                           (1 + kNodePos[m][2] * kNodePos[j][2]);
+          for (int d = 0; d < 3; d++) X[3 * (size_t)nd + d] += w * corner_coord[3 * (size_t)L[m] + d];
+        }
+      }
+    }
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int nd = 0; nd < NN; nd++)
+      for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], X[3 * (size_t)nd + d]); hi[d] = std::max(hi[d], X[3 * (size_t)nd + d]); }
+    const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    const double h = ext > 0 ? ext * 1e-9 : 1.0;
+    struct Key { long long z, y, x; int id; };
+    std::vector<Key> key(NN);
+    for (int nd = 0; nd < NN; nd++)
+      key[nd] = Key{std::llround((X[3 * (size_t)nd + 2] - lo[2]) / h), std::llround((X[3 * (size_t)nd + 1] - lo[1]) / h),
+                    std::llround((X[3 * (size_t)nd] - lo[0]) / h), nd};
+    std::sort(key.begin(), key.end(), [](const Key& a, const Key& b) {
+      if (a.z != b.z) return a.z < b.z;
+      if (a.y != b.y) return a.y < b.y;
+      if (a.x != b.x) return a.x < b.x;
+      return a.id < b.id;
+    });
+    ctx->viperm.resize(NN);
+    ctx->vperm.resize(NN);
+    for (int k = 0; k < NN; k++) { ctx->viperm[k] = key[k].id; ctx->vperm[key[k].id] = k; }
+  }
+  {
+    std::vector<int> li((size_t)NE * 27), p8((size_t)NE * 8);
+    for (size_t i = 0; i < (size_t)NE * 27; i++) li[i] = ctx->vperm[LtoGnode[i]];
+    for (int e = 0; e < NE; e++)
+      for (int m = 0; m < 8; m++) p8[(size_t)e * 8 + m] = LtoGnode[(size_t)e * 27 + m];
+    FSX_TRY(alloc(ctx, &ctx->L27, (size_t)NE * 27));
+    FSX_TRY(alloc(ctx, &ctx->P8, (size_t)NE * 8));
+    FSX_HIP(hipMemcpy(ctx->L27, li.data(), li.size() * sizeof(int), hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(ctx->P8, p8.data(), p8.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   FSX_TRY(alloc(ctx, &ctx->xc, (size_t)NNp * 3));
   FSX_TRY(alloc(ctx, &ctx->color_order, (size_t)NE));
-  FSX_HIP(hipMemcpy(ctx->L27, LtoGnode, (size_t)NE * 27 * sizeof(int), hipMemcpyHostToDevice));
   FSX_HIP(hipMemcpy(ctx->xc, corner_coord, (size_t)NNp * 3 * sizeof(double), hipMemcpyHostToDevice));
   FSX_HIP(hipMemcpy(ctx->color_order, color_order, (size_t)NE * sizeof(int), hipMemcpyHostToDevice));
   ctx->color_ptr.assign(color_ptr, color_ptr + ncolors + 1);
@@ -483,6 +532,34 @@ int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const i
     }
   }
   const long long nnz = rowptr[nrows];
+  // M (rows and columns) and G (rows) go to the device in the internal
+  // velocity numbering, rows in internal order with sorted columns; the
+  // reference pattern is kept on the host for value downloads
+  std::vector<int> irow, icol;
+  if (kind == FSX_CSR_M || kind == FSX_CSR_G) {
+    const std::vector<int>& vp = ctx->vperm;
+    irow.assign((size_t)nrows + 1, 0);
+    for (int r = 0; r < nrows; r++) irow[vp[r] + 1] = rowptr[r + 1] - rowptr[r];
+    for (int r = 0; r < nrows; r++) irow[r + 1] += irow[r];
+    icol.resize((size_t)nnz);
+    for (int r = 0; r < nrows; r++) {
+      int* dst = icol.data() + irow[vp[r]];
+      const int len = rowptr[r + 1] - rowptr[r];
+      for (int k = 0; k < len; k++) dst[k] = kind == FSX_CSR_M ? vp[col[rowptr[r] + k]] : col[rowptr[r] + k];
+      if (kind == FSX_CSR_M) std::sort(dst, dst + len);
+    }
+    if (kind == FSX_CSR_M) {
+      ctx->refMrow.assign(rowptr, rowptr + nrows + 1);
+      ctx->refMcol.assign(col, col + nnz);
+      ctx->posM.clear();
+    } else {
+      ctx->refGrow.assign(rowptr, rowptr + nrows + 1);
+      ctx->refGcol.assign(col, col + nnz);
+      ctx->posG.clear();
+    }
+    rowptr = irow.data();
+    col = icol.data();
+  }
   int* drow; int* dcol;
   FSX_TRY(alloc(ctx, &drow, (size_t)nrows + 1));
   FSX_TRY(alloc(ctx, &dcol, (size_t)nnz));
@@ -492,6 +569,18 @@ int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const i
     ctx->Mrow = drow; ctx->Mcol = dcol; ctx->nnzM = nnz;
     FSX_TRY(alloc(ctx, &ctx->K, (size_t)nnz));
     FSX_TRY(alloc(ctx, &ctx->M, (size_t)nnz));
+    // K's columns as a dictionary of row patterns: in the coordinate order a
+    // structured mesh has a few hundred distinct rows
+    std::vector<int> rp, pp, po;
+    if (build_patterns(nrows, rowptr, col, 1 << 16, 1 << 22, rp, pp, po)) {
+      for (int& id : rp) id = pp[id];
+      FSX_TRY(alloc(ctx, &ctx->Mrowpat, rp.size()));
+      FSX_TRY(alloc(ctx, &ctx->Mpatoff, std::max<size_t>(1, po.size())));
+      FSX_HIP(hipMemcpy(ctx->Mrowpat, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice));
+      if (!po.empty()) FSX_HIP(hipMemcpy(ctx->Mpatoff, po.data(), po.size() * sizeof(int), hipMemcpyHostToDevice));
+      ctx->Mnpat = (int)pp.size() - 1;
+      ctx->Mpatlen = (long long)po.size();
+    }
     ctx->have_M = true;
   } else if (kind == FSX_CSR_G) {
     ctx->Grow = drow; ctx->Gcol = dcol; ctx->nnzG = nnz;
@@ -546,15 +635,17 @@ int fsx_upload_bc(fsx_ctx* ctx, int n, const int* nodes, const double* values, i
   if (!ctx->have_mesh) return fail(ctx, FSX_ESTATE, "upload the mesh first");
   if (zero_p_node >= ctx->NNp) return fail(ctx, FSX_EINVAL, "zero-pressure node out of range");
   std::vector<unsigned char> flag(ctx->NN, 0);
+  std::vector<int> inodes(std::max(n, 1));   // internal ids
   for (int i = 0; i < n; i++) {
     if (nodes[i] < 0 || nodes[i] >= ctx->NN) return fail(ctx, FSX_EINVAL, "BC node out of range");
-    flag[nodes[i]] = 1;
+    inodes[i] = ctx->vperm[nodes[i]];
+    flag[inodes[i]] = 1;
   }
   if (ctx->bcnodes) { dev_free(ctx, ctx->bcnodes); dev_free(ctx, ctx->bcvals); }
   FSX_TRY(alloc(ctx, &ctx->bcnodes, (size_t)std::max(n, 1)));
   FSX_TRY(alloc(ctx, &ctx->bcvals, (size_t)std::max(n, 1) * 3));
   if (n > 0) {
-    FSX_HIP(hipMemcpy(ctx->bcnodes, nodes, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(ctx->bcnodes, inodes.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
     FSX_HIP(hipMemcpy(ctx->bcvals, values, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
   }
   FSX_HIP(hipMemcpy(ctx->bcflag, flag.data(), (size_t)ctx->NN, hipMemcpyHostToDevice));
@@ -697,7 +788,7 @@ int fsx_monitor(fsx_ctx* ctx, int node, double* out4) {
   if (!ctx || !out4) return FSX_EINVAL;
   if (node < 0 || node >= ctx->NNp) return fail(ctx, FSX_EINVAL, "monitor node out of range");
   for (int c = 0; c < 3; c++)
-    FSX_HIP(hipMemcpyAsync(ctx->h_mon + c, ctx->Un + 3 * (size_t)node + c, sizeof(double), hipMemcpyDeviceToHost,
+    FSX_HIP(hipMemcpyAsync(ctx->h_mon + c, ctx->Un + 3 * (size_t)ctx->vperm[node] + c, sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
   FSX_HIP(hipMemcpyAsync(ctx->h_mon + 3, ctx->Pn + node, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -706,8 +797,9 @@ int fsx_monitor(fsx_ctx* ctx, int node, double* out4) {
 }
 
 static int vec_info(fsx_ctx* ctx, int id, double** p, size_t* n, int* kind) {
-  // kind 0: plain, 1: NN vector replicated x3, 2: component c of G/GT (c = id - FSX_VAL_G1),
-  // 3: velocity vector, [u|v|w] at the boundary and node-major (u, v, w of a node together) on the device
+  // kind 0: plain, 1: NN vector replicated x3, 2: component c of G (c = id - FSX_VAL_G1),
+  // 3: velocity vector, [u|v|w] at the boundary and node-major (u, v, w of a node together) on the device,
+  // 4: values on the M pattern; kinds 1-4 are in the internal velocity numbering on the device
   const size_t n3 = (size_t)3 * ctx->NN;
   *kind = 0;
   switch (id) {
@@ -726,13 +818,39 @@ static int vec_info(fsx_ctx* ctx, int id, double** p, size_t* n, int* kind) {
     case FSX_VEC_R2: *p = ctx->R2; *n = ctx->NNp; break;
     case FSX_VEC_MDINV: *p = ctx->MdInv; *n = n3; *kind = 1; break;
     case FSX_VEC_MDORIGINV: *p = ctx->MdOrigInv; *n = n3; *kind = 1; break;
-    case FSX_VAL_K: *p = ctx->K; *n = ctx->nnzM; break;
-    case FSX_VAL_M: *p = ctx->M; *n = ctx->nnzM; break;
+    case FSX_VAL_K: *p = ctx->K; *n = ctx->nnzM; *kind = 4; break;
+    case FSX_VAL_M: *p = ctx->M; *n = ctx->nnzM; *kind = 4; break;
     case FSX_VAL_G1: case FSX_VAL_G2: case FSX_VAL_G3: *p = ctx->G; *n = ctx->nnzG; *kind = 2; break;
     case FSX_VAL_Z: *p = ctx->Z; *n = ctx->nnzZ; break;
     default: return fail(ctx, FSX_EINVAL, "unknown vector id");
   }
   if (!*p) return fail(ctx, FSX_ESTATE, "vector not allocated yet");
+  return FSX_OK;
+}
+
+// Position in the device's (internal) CSR of every entry of an uploaded
+// reference pattern: kind M maps rows and columns through vperm, kind G only
+// the rows (its columns are pressure nodes).
+static int build_pos(fsx_ctx* ctx, bool is_m, std::vector<int>& pos) {
+  const std::vector<int>& rr = is_m ? ctx->refMrow : ctx->refGrow;
+  const std::vector<int>& rc = is_m ? ctx->refMcol : ctx->refGcol;
+  const int nrows = ctx->NN;
+  const size_t nnz = rc.size();
+  std::vector<int> irow((size_t)nrows + 1), icol(nnz);
+  FSX_HIP(hipMemcpy(irow.data(), is_m ? ctx->Mrow : ctx->Grow, irow.size() * sizeof(int), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(icol.data(), is_m ? ctx->Mcol : ctx->Gcol, nnz * sizeof(int), hipMemcpyDeviceToHost));
+  pos.resize(nnz);
+  for (int r = 0; r < nrows; r++) {
+    const int pr = ctx->vperm[r];
+    const int* lo = icol.data() + irow[pr];
+    const int* hi = icol.data() + irow[pr + 1];
+    for (int k = rr[r]; k < rr[r + 1]; k++) {
+      const int c = is_m ? ctx->vperm[rc[k]] : rc[k];
+      const int* f = std::lower_bound(lo, hi, c);
+      if (f == hi || *f != c) return fail(ctx, FSX_ESTATE, "internal pattern lacks an uploaded entry");
+      pos[k] = (int)(f - icol.data());
+    }
+  }
   return FSX_OK;
 }
 
@@ -743,24 +861,32 @@ int fsx_download(fsx_ctx* ctx, int id, double* host, size_t count) {
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_download: count mismatch (expected " + std::to_string(n) + ")");
   FSX_TRY(settle_kacc(ctx));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
+  const size_t NN = ctx->NN;
+  const std::vector<int>& ip = ctx->viperm;
   if (kind == 0) {
     FSX_HIP(hipMemcpy(host, p, n * sizeof(double), hipMemcpyDeviceToHost));
-  } else if (kind == 3) {
-    const size_t NN = ctx->NN;
+  } else if (kind == 3) {   // node-major, internal order -> [u|v|w], reference order
     std::vector<double> t(n);
     FSX_HIP(hipMemcpy(t.data(), p, n * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < NN; i++)
-      for (int c = 0; c < 3; c++) host[c * NN + i] = t[3 * i + c];
-  } else if (kind == 1) {
-    const size_t NN = ctx->NN;
-    FSX_HIP(hipMemcpy(host, p, NN * sizeof(double), hipMemcpyDeviceToHost));
+      for (int c = 0; c < 3; c++) host[c * NN + ip[i]] = t[3 * i + c];
+  } else if (kind == 1) {   // one NN vector in internal order, replicated x3
+    std::vector<double> t(NN);
+    FSX_HIP(hipMemcpy(t.data(), p, NN * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < NN; i++) host[ip[i]] = t[i];
     std::memcpy(host + NN, host, NN * sizeof(double));
     std::memcpy(host + 2 * NN, host, NN * sizeof(double));
-  } else {
+  } else if (kind == 4) {   // values on the M pattern, in the reference CSR order
+    if (ctx->posM.empty()) FSX_TRY(build_pos(ctx, true, ctx->posM));
+    std::vector<double> t(n);
+    FSX_HIP(hipMemcpy(t.data(), p, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; k++) host[k] = t[ctx->posM[k]];
+  } else {                  // component c of the interleaved G values, reference CSR order
+    if (ctx->posG.empty()) FSX_TRY(build_pos(ctx, false, ctx->posG));
     std::vector<double> g(n * 3);
     FSX_HIP(hipMemcpy(g.data(), p, g.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int c = id - FSX_VAL_G1;
-    for (size_t k = 0; k < n; k++) host[k] = g[3*k + c];
+    for (size_t k = 0; k < n; k++) host[k] = g[3 * (size_t)ctx->posG[k] + c];
   }
   return FSX_OK;
 }
@@ -769,7 +895,8 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   if (!ctx || !host) return FSX_EINVAL;
   double* p; size_t n; int kind;
   FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
-  if (kind != 0 && kind != 3) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
+  if (kind != 0 && kind != 3 && kind != 4)
+    return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
   FSX_TRY(settle_kacc(ctx));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
   FSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -777,7 +904,12 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
     const size_t NN = ctx->NN;
     std::vector<double> t(n);
     for (size_t i = 0; i < NN; i++)
-      for (int c = 0; c < 3; c++) t[3 * i + c] = host[c * NN + i];
+      for (int c = 0; c < 3; c++) t[3 * i + c] = host[c * NN + ctx->viperm[i]];
+    FSX_HIP(hipMemcpy(p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  } else if (kind == 4) {
+    if (ctx->posM.empty()) FSX_TRY(build_pos(ctx, true, ctx->posM));
+    std::vector<double> t(n);
+    for (size_t k = 0; k < n; k++) t[ctx->posM[k]] = host[k];
     FSX_HIP(hipMemcpy(p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
   } else {
     FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
@@ -824,13 +956,21 @@ int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, 
   FSX_TRY(alloc(ctx, &dx, nx));
   FSX_TRY(alloc(ctx, &dy, ny));
   int rc = FSX_OK;
-  if (hipMemcpy(dx, x, nx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dy, y, ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  // velocity-sized operands move between the reference and the internal numbering
+  const bool x_vel = which == 0 || (which >= 5 && which <= 7), y_vel = which >= 0 && which <= 3;
+  std::vector<double> hx(x, x + nx), hy(y, y + ny);
+  if (x_vel) for (size_t i = 0; i < nx; i++) hx[ctx->vperm[i]] = x[i];
+  if (y_vel) for (size_t i = 0; i < ny; i++) hy[ctx->vperm[i]] = y[i];
+  if (hipMemcpy(dx, hx.data(), nx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dy, hy.data(), ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(ctx, FSX_EHIP, "fsx_spmv copy");
   if (rc == FSX_OK) rc = spmv_host_op(ctx, which, dx, dy, alpha, beta);
   if (rc == FSX_OK) {
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess || hipMemcpy(y, dy, ny * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(hy.data(), dy, ny * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       rc = fail(ctx, FSX_EHIP, "fsx_spmv copy back");
+    else if (y_vel) for (size_t i = 0; i < ny; i++) y[i] = hy[ctx->vperm[i]];
+    else std::memcpy(y, hy.data(), ny * sizeof(double));
   }
   dev_free(ctx, dx);
   dev_free(ctx, dy);

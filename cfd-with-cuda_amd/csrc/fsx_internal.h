@@ -101,8 +101,21 @@ struct fsx_ctx {
   bool have_mesh = false, have_M = false, have_G = false, have_Z = false, have_bc = false, have_params = false;
   bool assembled = false, z_built = false, state_ready = false;
   bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_kacc)
+  // Internal velocity numbering. The device numbers the velocity nodes in
+  // the order of their coordinates (z, then y, then x), so that the nodes a
+  // row of K or G couples to sit close together in memory; the reference
+  // numbers corners, mid-edge, mid-face and centre nodes in four separate
+  // ranges (BCH:954-1320), which spreads every row's gathers over four
+  // distant regions. vperm maps a reference id to the internal one, viperm
+  // back. Everything at the boundary (patterns, vectors, values, BC nodes)
+  // stays in the reference numbering and is converted on upload/download.
+  std::vector<int> vperm, viperm;
+  // the uploaded (reference-numbered) M and G patterns, for value downloads
+  std::vector<int> refMrow, refMcol, refGrow, refGcol;
+  std::vector<int> posM, posG;   // reference CSR position -> internal one (built on first use)
   // mesh (device)
-  int* L27 = nullptr;            // NE*27
+  int* L27 = nullptr;            // NE*27, internal velocity ids
+  int* P8 = nullptr;             // NE*8, pressure ids of the corner nodes (the reference's LtoGnode[e][0..7])
   double* xc = nullptr;          // NNp*3 corner coordinates
   int* color_order = nullptr;    // NE
   std::vector<int> color_ptr;    // host copy, ncolors+1
@@ -110,6 +123,10 @@ struct fsx_ctx {
   long long nnzM = 0, nnzG = 0, nnzZ = 0;
   int* Mrow = nullptr; int* Mcol = nullptr;
   double* K = nullptr; double* M = nullptr;
+  // M/K columns as a dictionary of row patterns (as for Z below); null when
+  // the rows do not compress
+  int* Mrowpat = nullptr; int* Mpatoff = nullptr;
+  int Mnpat = 0; long long Mpatlen = 0;
   int* Grow = nullptr; int* Gcol = nullptr;
   double* G = nullptr;           // nnzG*3, (G1, G2, G3) interleaved per nonzero
   int* GTrow = nullptr; int* GTcol = nullptr; int* GTpos = nullptr;

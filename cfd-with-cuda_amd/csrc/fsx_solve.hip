@@ -65,8 +65,8 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // reference computes at the end of the previous inner iteration: K's columns
 // and values are read once for six right-hand sides instead of twice for
 // three.
-template <int LPR, int UK, int UG, bool KACC>
-__global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, const int* __restrict__ Mcol,
+template <int LPR, int UK, int UG, bool KACC, bool KPAT>
+__global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, ColSrc mcs,
                                                   const double* __restrict__ K, const int* __restrict__ Grow,
                                                   const int* __restrict__ Gcol, const double* __restrict__ G,
                                                   const double* __restrict__ Uprev, const double* __restrict__ Pn,
@@ -79,13 +79,14 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
   double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
   if (r < NN) {
     const int rs = Mrow[r], re = Mrow[r + 1];
+    const int cb = col_base<KPAT>(mcs, r, rs);
     for (int base = rs + l; base < re; base += UK * LPR) {
       int c[UK];
       double a[UK];
 #pragma unroll
       for (int j = 0; j < UK; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
-        c[j] = Mcol[kk];
+        c[j] = col_at<KPAT>(mcs, r, cb, kk);
         a[j] = k < re ? K[kk] : 0.0;
       }
 #pragma unroll
@@ -137,23 +138,26 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   // 8x8 1.150, 8x16 1.217, 16x8 1.224, 4x16 1.352, 32x2 1.287
   constexpr int LPR = 8, UK = 2;
   const dim3 grid(rows_grid(ctx->NN, LPR));
-  if (ctx->kacc_pending) {
-    hipLaunchKernelGGL((k_step1<LPR, UK, 2, true>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
-                       ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,
-                       ctx->bcflag, ctx->UnpHalf, ctx->dt, ctx->Acc_prev, ctx->KAcc);
-    ctx->kacc_pending = false;
+  const ColSrc mcs{ctx->Mcol, ctx->Mrowpat, ctx->Mpatoff};
+  const bool kacc = ctx->kacc_pending;
+  ctx->kacc_pending = false;
+#define STEP1_LAUNCH(KA, KP)                                                                                          \
+  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, ctx->K, \
+                     ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv, ctx->bcflag, \
+                     ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr)
+  if (mcs.row_pat) {
+    if (kacc) STEP1_LAUNCH(true, true); else STEP1_LAUNCH(false, true);
   } else {
-    hipLaunchKernelGGL((k_step1<LPR, UK, 2, false>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->Mcol,
-                       ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,
-                       ctx->bcflag, ctx->UnpHalf, ctx->dt, nullptr, nullptr);
+    if (kacc) STEP1_LAUNCH(true, false); else STEP1_LAUNCH(false, false);
   }
+#undef STEP1_LAUNCH
   return launch_check(ctx, "k_step1");
 }
 
 // ------------------------------------------------------------ K * Acc_prev
 // KAcc_c = K Acc_prev_c, c = 0..2, one pass over K (BCH:2981-3035)
-template <int LPR, int U>
-__global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ Mrow, const int* __restrict__ Mcol,
+template <int LPR, int U, bool KPAT>
+__global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ Mrow, ColSrc mcs,
                                                const double* __restrict__ K, const double* __restrict__ x,
                                                double* __restrict__ y) {
   int l;
@@ -161,13 +165,14 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
   double k0 = 0, k1 = 0, k2 = 0;
   if (r < NN) {
     const int rs = Mrow[r], re = Mrow[r + 1];
+    const int cb = col_base<KPAT>(mcs, r, rs);
     for (int base = rs + l; base < re; base += U * LPR) {
       int c[U];
       double a[U];
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
-        c[j] = Mcol[kk];
+        c[j] = col_at<KPAT>(mcs, r, cb, kk);
         a[j] = k < re ? K[kk] : 0.0;
       }
 #pragma unroll
@@ -183,8 +188,13 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
 
 int launch_kacc(fsx_ctx* ctx) {
   constexpr int LPR = 16;
-  hipLaunchKernelGGL((k_k3<LPR, 4>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,
-                     ctx->Mcol, ctx->K, ctx->Acc_prev, ctx->KAcc);
+  const ColSrc mcs{ctx->Mcol, ctx->Mrowpat, ctx->Mpatoff};
+  if (mcs.row_pat)
+    hipLaunchKernelGGL((k_k3<LPR, 4, true>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                       ctx->Mrow, mcs, ctx->K, ctx->Acc_prev, ctx->KAcc);
+  else
+    hipLaunchKernelGGL((k_k3<LPR, 4, false>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                       ctx->Mrow, mcs, ctx->K, ctx->Acc_prev, ctx->KAcc);
   ctx->kacc_pending = false;
   return launch_check(ctx, "k_k3");
 }
@@ -984,6 +994,8 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   if (!ctx->K || !ctx->G || !ctx->GT) return fail(ctx, FSX_ESTATE, "fsx_bench_kernel: assemble first");
   if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP3) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
   const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG;
+  // K's columns: a pattern start per row plus the dictionary, or a column per entry
+  const double kcols = ctx->Mrowpat ? 4.0 * NN + 4.0 * (double)ctx->Mpatlen : 4.0 * nK;
   auto launch = [&]() {
     switch (kernel) {
       case FSX_KERNEL_STEP1: ctx->kacc_pending = true; (void)launch_step1(ctx, 2); break;
@@ -996,11 +1008,11 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   switch (kernel) {
     case FSX_KERNEL_STEP1:   // K (col + value), G (col + 3 values), Uprev, Pn, R1 r/w, Un, MdInv, bc, UnpHalf,
                              // Acc_prev, KAcc
-      bytes = 12.0 * nK + 4.0 * (NN + 1) + 24.0 * NN + 28.0 * nG + 4.0 * (NN + 1) + 8.0 * NNp + 48.0 * NN + 24.0 * NN +
-              8.0 * NN + 1.0 * NN + 24.0 * NN + 24.0 * NN + 24.0 * NN;
+      bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 28.0 * nG + 4.0 * (NN + 1) + 8.0 * NNp + 48.0 * NN +
+              24.0 * NN + 8.0 * NN + 1.0 * NN + 24.0 * NN + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
-      bytes = 12.0 * nK + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
+      bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2
       bytes = 28.0 * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;

@@ -10,7 +10,9 @@
  * available from fsx_last_error(). Nothing blocks on stdin or exits.
  *
  * Conventions (the reference's layouts, BCH:1327-1403, 1807-1811):
- *   velocity vectors are [u(0..NN-1) | v | w], FP64;
+ *   velocity vectors are [u(0..NN-1) | v | w], FP64, in the reference's node
+ *   numbering (the device keeps its own node order and layout and converts
+ *   at every call that takes or returns velocity-sized data or M/G values);
  *   pressure is indexed by corner node (NNp = NCN), FP64;
  *   CSR row pointers and columns are int32, 0-based, columns sorted.
  * One context per host thread; all work is queued on the context's stream.
@@ -221,7 +223,9 @@ enum fsx_kernel_id {
  * iteration >= 2 form, with KAcc) on the context's stream with HIP events:
  * mean device time per launch in ms, and its algorithmic HBM bytes (every
  * matrix entry, row pointer and vector element it touches counted once;
- * int32 indices, FP64 values). Clobbers the step vectors: call
+ * FP64 values, int32 indices, K's columns as the kernels read them: one
+ * pattern start per row plus the pattern dictionary when K's rows compress,
+ * else a column per entry). Clobbers the step vectors: call
  * fsx_init_state before stepping again. No reference counterpart (the
  * reference prints wall times only, BCH:4489-4518). */
 int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_per_launch, double* bytes_per_launch);
