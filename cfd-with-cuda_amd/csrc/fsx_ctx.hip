@@ -651,6 +651,7 @@ int fsx_upload_bc(fsx_ctx* ctx, int n, const int* nodes, const double* values, i
   FSX_HIP(hipMemcpy(ctx->bcflag, flag.data(), (size_t)ctx->NN, hipMemcpyHostToDevice));
   ctx->nbc = n;
   ctx->zeroP = zero_p_node;
+  ctx->mass_dinv_ready = false;
   ctx->have_bc = true;
   return FSX_OK;
 }
@@ -661,6 +662,7 @@ int fsx_assemble_static(fsx_ctx* ctx) {
     return fail(ctx, FSX_ESTATE, "fsx_assemble_static needs mesh, M and G patterns, BCs and parameters");
   FSX_TRY(launch_assemble_static(ctx));
   ctx->assembled = true;
+  ctx->mass_dinv_ready = false;
   return FSX_OK;
 }
 

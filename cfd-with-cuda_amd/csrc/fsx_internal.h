@@ -101,6 +101,20 @@ struct fsx_ctx {
   bool have_mesh = false, have_M = false, have_G = false, have_Z = false, have_bc = false, have_params = false;
   bool assembled = false, z_built = false, state_ready = false;
   bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_kacc)
+  // velocity mass (fsx_set_mass): lumped (the reference) or consistent, M_bc
+  // solved by the 3-right-hand-side Jacobi-PCG of fsx_mass.hip
+  int mass_mode = FSX_MASS_LUMPED;
+  double mass_rtol = 1e-10;
+  int mass_maxit = 1000;
+  double* mass_w = nullptr;      // r, z, p, q (4 x 3NN)
+  double* mass_x = nullptr;      // 3NN, step1's solution
+  double* mass_b = nullptr;      // 3NN, step3's right-hand side R3
+  double* mass_dinv = nullptr;   // NN, 1 / M_ii, 0 at BC nodes
+  double* mass_sc = nullptr;     // device MassScalars
+  bool mass_dinv_ready = false;
+  int mass_last_iters = 0;
+  double mass_last_relres = 0.0;
+  long long mass_total_iters = 0;
   // Internal velocity numbering. The device numbers the velocity nodes in
   // the order of their coordinates (z, then y, then x), so that the nodes a
   // row of K or G couples to sit close together in memory; the reference
@@ -244,6 +258,8 @@ int launch_pnp1(fsx_ctx* ctx);
 int launch_step3(fsx_ctx* ctx, int iter);
 int launch_kacc(fsx_ctx* ctx);
 int settle_kacc(fsx_ctx* ctx);
+int mass_update_step1(fsx_ctx* ctx);
+int mass_update_step3(fsx_ctx* ctx);
 int launch_maxacc(fsx_ctx* ctx);
 int launch_fill(fsx_ctx* ctx, double* p, double v, long long n);
 int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n);

@@ -151,7 +151,9 @@ int launch_step1(fsx_ctx* ctx, int iter) {
     if (kacc) STEP1_LAUNCH(true, false); else STEP1_LAUNCH(false, false);
   }
 #undef STEP1_LAUNCH
-  return launch_check(ctx, "k_step1");
+  FSX_TRY(launch_check(ctx, "k_step1"));
+  if (ctx->mass_mode == FSX_MASS_CONSISTENT) return mass_update_step1(ctx);
+  return FSX_OK;
 }
 
 // ------------------------------------------------------------ K * Acc_prev
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
                                                   const double* __restrict__ KAcc, int use_kacc,
                                                   const unsigned char* __restrict__ bc, const double* __restrict__ MdInv,
                                                   const double* __restrict__ UnpHalf, double* __restrict__ Acc,
-                                                  double* __restrict__ Unp1, double dt) {
+                                                  double* __restrict__ Unp1, double dt, double* __restrict__ Rout) {
   int l;
   const long long r = my_row<LPR>(&l);
   double g0 = 0, g1 = 0, g2 = 0;
@@ -340,22 +342,30 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
     double R = -dt * gg;
     if (use_kacc) R = R - dt * KAcc[i];
     if (bc[r]) R = 0.0;
+    if (Rout) { Rout[i] = R; return; }   // consistent mass: the caller solves for Acc
     const double a = R * MdInv[r];
     Acc[i] = a;
     Unp1[i] = UnpHalf[i] + dt * a;
   }
 }
 
-static void launch_step3_rows(fsx_ctx* ctx, int iter) {
+static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
   // 64^3: 8x2 0.253 ms, 8x4 0.266, 4x2 0.290, 4x8 0.298, 4x4 0.307, 16x2 0.312, 8x8 0.319, 16x4 0.343
   constexpr int LPR = 8;
   hipLaunchKernelGGL((k_step3<LPR, 2>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
                      ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv, ctx->UnpHalf,
-                     ctx->Acc, ctx->Unp1, ctx->dt);
+                     ctx->Acc, ctx->Unp1, ctx->dt, Rout);
 }
 
 int launch_step3(fsx_ctx* ctx, int iter) {
-  launch_step3_rows(ctx, iter);
+  if (ctx->mass_mode == FSX_MASS_CONSISTENT) {
+    launch_step3_rows(ctx, iter, ctx->mass_b);
+    FSX_TRY(launch_check(ctx, "k_step3"));
+    const int rc = mass_update_step3(ctx);
+    if (rc != FSX_OK && rc != FSX_ENOCONV) return rc;
+  } else {
+    launch_step3_rows(ctx, iter);
+  }
   hipLaunchKernelGGL(k_vnorms, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, 3LL * ctx->NN, ctx->Unp1, ctx->Unp1_prev,
                      ctx->partials);
   hipLaunchKernelGGL(k_finish2, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, &ctx->st->uu);

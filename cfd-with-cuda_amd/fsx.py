@@ -23,6 +23,7 @@ FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTATE, FSX_ENOCONV, FSX_ENOMEM, FSX_ENUMERIC 
 CSR_M, CSR_G, CSR_Z = 0, 1, 2
 SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
 KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3 = 0, 1, 2, 3
+MASS_LUMPED, MASS_CONSISTENT = 0, 1
 
 VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, R1=7, Pn=8, Pnp1=9,
            Pnp1_prev=10, Pdot=11, R2=12, MdInv=13, MdOrigInv=14, K=15, M=16, G1=17, G2=18, G3=19, Z=20)
@@ -33,7 +34,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
@@ -107,6 +108,8 @@ def lib():
         "fsx_spmv": ([_vp, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_bench_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
+        "fsx_set_mass": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "fsx_mass_stats": ([_vp, _I, _D, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "fsx_bench_copy": ([_vp, ctypes.c_size_t, ctypes.c_int, _D], ctypes.c_int),
         "fsx_z_patterns": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "fsx_z_forms": ([_vp, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
@@ -366,6 +369,21 @@ class Solver:
         by = ctypes.c_double()
         self.check(self.L.fsx_bench_spmv(self.ctx, precision, reps, ctypes.byref(ms), ctypes.byref(by)), "bench_spmv")
         return ms.value, by.value
+
+    def set_mass(self, mode, rtol=1e-10, maxit=1000):
+        """Velocity mass of the step1/step3 updates: MASS_LUMPED (the
+        reference) or MASS_CONSISTENT (3-RHS Jacobi-PCG on M with BC rows and
+        columns replaced by the identity)."""
+        self.check(self.L.fsx_set_mass(self.ctx, mode, rtol, maxit), "set_mass")
+
+    def mass_stats(self):
+        """(iterations, relative residual) of the last mass solve and the
+        total mass iterations so far."""
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        tot = ctypes.c_longlong()
+        self.check(self.L.fsx_mass_stats(self.ctx, ctypes.byref(it), ctypes.byref(rel), ctypes.byref(tot)), "mass_stats")
+        return it.value, rel.value, tot.value
 
     def bench_kernel(self, kernel, reps=20):
         """(ms per launch, algorithmic bytes per launch) of one step row kernel

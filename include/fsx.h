@@ -80,6 +80,12 @@ enum fsx_solver_mode {
   FSX_SOLVER_LEARNED = 2       /* flexible PCG with the learned preconditioner, Jacobi-PCG fallback */
 };
 
+/* Velocity mass for fsx_set_mass */
+enum fsx_mass_mode {
+  FSX_MASS_LUMPED = 0,      /* row-sum lumped M, the reference's update (BCH:3262-3295, 3781-3783, 3960-3962) */
+  FSX_MASS_CONSISTENT = 1   /* consistent M, one 3-RHS Jacobi-PCG per update (no reference counterpart) */
+};
+
 /* ---------------------------------------------------------------- lifecycle */
 /* replaces selectCUDAdevice (CU:126-154) and the cuSPARSE/cuBLAS setup (CU:171-178) */
 int fsx_create(int device, fsx_ctx** out);
@@ -108,6 +114,21 @@ int fsx_upload_bc(fsx_ctx* ctx, int n, const int* nodes, const double* values, i
 int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, double dt);
 /* pressure solver: mode (fsx_solver_mode), relative tolerance, iteration cap */
 int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit);
+/* Velocity mass of the step1 and step3 updates. FSX_MASS_LUMPED (default) is
+ * the reference: UnpHalf = Un + dt R1 MdInv, Acc = R3 MdInv. With
+ * FSX_MASS_CONSISTENT the updates solve M_bc a = R1 and M_bc Acc = R3
+ * instead, M_bc being the assembled consistent mass with the rows and
+ * columns of BC nodes replaced by the identity; the three velocity
+ * components are solved together as one 3-right-hand-side Jacobi-PCG (one
+ * pass over M per iteration, per-component scalars and stop test
+ * ||r_c|| <= rtol ||b_c||, x0 = 0). The pressure operator Z keeps the lumped
+ * MdOrigInv. No reference counterpart (the reference assembles M, then lumps
+ * it, BCH:3195-3295). Needs the mesh uploaded. */
+int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit);
+/* iterations (largest over the components is the count of the block
+ * solve) and largest relative residual of the last mass solve, and the
+ * iterations of all mass solves so far; any pointer may be NULL */
+int fsx_mass_stats(fsx_ctx* ctx, int* last_iters, double* last_relres, long long* total_iters);
 /* enqueue solver iterations as replayed HIP graphs (1) or as individual
  * launches (0, the default: on the 64^3 cavity both run within 5% of each
  * other); results are identical. No reference counterpart (the CUSP and MKL

@@ -69,6 +69,15 @@ struct Case {
   // solver options (reference: rtol 1e-6 equivalent, maxit 1000)
   double pcg_rtol = 1e-8;
   int pcg_maxit = 1000;
+  // velocity mass: 0 lumped (the reference, BCH:3262-3295), 1 consistent M
+  // solved by Jacobi-PCG for the three components (no reference counterpart)
+  int mass_mode = 0;
+  double mass_rtol = 1e-10;
+  int mass_maxit = 1000;
+  std::vector<double> Mbc;        // M with BC rows and columns replaced by the identity
+  int last_mass_iters = 0;
+  double last_mass_relres = 0.0;
+  long total_mass_iters = 0;
   // ---- topology ----
   std::vector<int> elemsOfPres, nElemOfPres;   // NNp*10
   std::vector<int> neigh, nNeigh;              // NE*26
@@ -763,6 +772,32 @@ void convection(const Case& c, const double* Un, const double* Uprev, double* R1
   }
 }
 
+// Consistent-mass velocity update (no counterpart in the reference, which
+// lumps M, BCH:3262-3295): a_d = M_bc^-1 R_d for d = x, y, z with M_bc = M
+// whose BC rows and columns are replaced by identity rows and columns. R is
+// zero at BC nodes, so a is too, as with the lumped update (MdInv = 1 there).
+// Each component is an independent textbook Jacobi-PCG to mass_rtol.
+void mass_solve(Case& c, const double* R, double* a) {
+  const int NN = c.NN;
+  if (c.Mbc.empty()) {
+    std::vector<char> bc(NN, 0);
+    for (int n : c.BCnodes) bc[n] = 1;
+    c.Mbc = c.Mval;
+    for (int r = 0; r < NN; r++)
+      for (int k = c.Mrow[r]; k < c.Mrow[r+1]; k++)
+        if (bc[r] || bc[c.Mcol[k]]) c.Mbc[k] = (c.Mcol[k] == r) ? 1.0 : 0.0;
+  }
+  c.last_mass_iters = 0;
+  c.last_mass_relres = 0.0;
+  for (int d = 0; d < 3; d++) {
+    double rel = 0.0;
+    const int it = pcg(c.Mrow, c.Mcol, c.Mbc, NN, R + (size_t)d*NN, a + (size_t)d*NN, c.mass_rtol, c.mass_maxit, &rel);
+    c.last_mass_iters = std::max(c.last_mass_iters, it);
+    c.last_mass_relres = std::max(c.last_mass_relres, rel);
+    c.total_mass_iters += it;
+  }
+}
+
 // BCH:3692-3801
 void step1(Case& c, int iter) {
   (void)iter;
@@ -775,6 +810,12 @@ void step1(Case& c, int iter) {
   for (int d = 0; d < 3; d++)
     csrmv(c.Grow, c.Gcol, Gs[d]->data(), c.Pn.data(), c.R1.data() + (size_t)d*NN, NN, -1.0, 1.0);
   for (int n : c.BCnodes) { c.R1[n] = 0.0; c.R1[n + NN] = 0.0; c.R1[n + 2*(size_t)NN] = 0.0; }   // BCH:4137-4146
+  if (c.mass_mode == 1) {
+    std::vector<double> a((size_t)3*NN);
+    mass_solve(c, c.R1.data(), a.data());
+    for (size_t i = 0; i < (size_t)3*NN; i++) c.UnpHalf[i] = c.Un[i] + c.dt * a[i];
+    return;
+  }
   for (size_t i = 0; i < (size_t)3*NN; i++) c.UnpHalf[i] = c.Un[i] + c.dt * c.R1[i] * c.MdInv[i];
 }
 
@@ -807,7 +848,9 @@ void step3(Case& c, int iter) {
   if (iter != 1)
     for (size_t i = 0; i < (size_t)3*NN; i++) c.R3[i] = c.R3[i] - c.dt * c.KAcc[i];
   for (int n : c.BCnodes) { c.R3[n] = 0.0; c.R3[n + NN] = 0.0; c.R3[n + 2*(size_t)NN] = 0.0; }   // BCH:4196-4206
-  for (size_t i = 0; i < (size_t)3*NN; i++) c.Acc[i] = c.R3[i] * c.MdInv[i];
+  if (c.mass_mode == 1) mass_solve(c, c.R3.data(), c.Acc.data());
+  else
+    for (size_t i = 0; i < (size_t)3*NN; i++) c.Acc[i] = c.R3[i] * c.MdInv[i];
   for (size_t i = 0; i < (size_t)3*NN; i++) c.Unp1[i] = c.UnpHalf[i] + c.dt * c.Acc[i];
 }
 
@@ -998,6 +1041,20 @@ int orc_run(void* h, int nsteps, double* stats) {
 }
 
 int orc_step1(void* h, int iter) { step1(*(Case*)h, iter); return 0; }
+
+// velocity mass mode (0 lumped, 1 consistent with Jacobi-PCG per component)
+int orc_set_mass(void* h, int mode, double rtol, int maxit) {
+  Case& c = *(Case*)h;
+  if (mode < 0 || mode > 1 || !(rtol >= 0) || maxit < 1) { c.err = "orc_set_mass: bad arguments"; return 1; }
+  c.mass_mode = mode; c.mass_rtol = rtol; c.mass_maxit = maxit;
+  return 0;
+}
+// last mass solve: largest iteration count and relative residual over the components; total iterations
+int orc_mass_stats(void* h, double* out) {
+  Case& c = *(Case*)h;
+  out[0] = c.last_mass_iters; out[1] = c.last_mass_relres; out[2] = (double)c.total_mass_iters;
+  return 0;
+}
 int orc_step2(void* h, int iter) { step2(*(Case*)h, iter); return 0; }
 int orc_step3(void* h, int iter) { step3(*(Case*)h, iter); return 0; }
 int orc_kacc(void* h) {

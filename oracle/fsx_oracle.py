@@ -56,6 +56,8 @@ def lib():
         L.orc_pcg_csr.argtypes = [ctypes.c_int, _I, _I, _D, _D, _D, ctypes.c_double, ctypes.c_int, _D]
         L.orc_spmv.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double]
         L.orc_timings.argtypes = [ctypes.c_void_p, _D]
+        L.orc_set_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.orc_mass_stats.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
@@ -155,6 +157,18 @@ class Oracle:
         stats = np.zeros((nsteps, 8))
         done = self.L.orc_run(self.h, nsteps, _dp(stats))
         return stats[:done]
+
+    def set_mass(self, mode, rtol=1e-10, maxit=1000):
+        """Velocity mass: 0 lumped (the reference), 1 consistent M solved by
+        Jacobi-PCG per component."""
+        self._check(self.L.orc_set_mass(self.h, mode, rtol, maxit), "set_mass")
+
+    def mass_stats(self):
+        """(iterations, relative residual) of the last mass solve, largest over
+        the components, and the total iterations so far."""
+        out = np.zeros(3)
+        self.L.orc_mass_stats(self.h, _dp(out))
+        return int(out[0]), float(out[1]), int(out[2])
 
     def step1(self, it):
         self.L.orc_step1(self.h, it)

@@ -388,3 +388,44 @@ def test_step_invariants(oracle_lib):
         gp = o.spmv(g, o.d("Pdot"))
         dt = 0.01
         assert np.allclose(corr[free], -dt * dt * minv[free] * gp[free], rtol=1e-12, atol=1e-18)
+
+
+def test_consistent_mass_step(oracle_lib):
+    """Consistent-mass mode (no reference counterpart; the reference lumps M):
+    step1 sets UnpHalf = Un + dt a with M_bc a = R1 and step3 sets Acc with
+    M_bc Acc = R3, where M_bc is M with BC rows and columns replaced by the
+    identity. Checked against a dense direct solve built here from the
+    oracle's CSR M and BC list, and at the BC nodes (a = 0, as with the
+    lumped update)."""
+    o = oracle_lib.case_from_mesh(meshgen.cavity(3), NGP=27, pcg_rtol=1e-12)
+    o.set_mass(1, rtol=1e-13, maxit=500)
+    s = o.sizes()
+    NN = s["NN"]
+    dt = 0.01
+    M = csr_dense(o.i("Mrow"), o.i("Mcol"), o.d("M"), (NN, NN))
+    bcn = o.i("BCnodes")
+    Mbc = M.copy()
+    Mbc[bcn, :] = 0.0
+    Mbc[:, bcn] = 0.0
+    Mbc[bcn, bcn] = 1.0
+    o.set_d("UnpHalf_prev", o.d("Un"))
+    o.step1(1)
+    it, rel, _ = o.mass_stats()
+    assert 0 < it < 500 and rel <= 1e-13
+    R1 = o.d("R1")
+    a = (o.d("UnpHalf") - o.d("Un")) / dt
+    for c in range(3):
+        ad = np.linalg.solve(Mbc, R1[c * NN:(c + 1) * NN])
+        assert np.allclose(a[c * NN:(c + 1) * NN], ad, rtol=1e-9, atol=1e-9 * np.abs(ad).max())
+        assert not ad[bcn].any()
+    # the consistent update differs from the lumped one (M is not diagonal)
+    lumped = R1 * o.d("MdInv")
+    assert np.abs(a - lumped).max() > 1e-3 * np.abs(lumped).max()
+    o.step2(1)
+    o.step3(1)
+    R3 = o.d("R3")
+    acc = o.d("Acc")
+    for c in range(3):
+        ad = np.linalg.solve(Mbc, R3[c * NN:(c + 1) * NN])
+        assert np.allclose(acc[c * NN:(c + 1) * NN], ad, rtol=1e-9, atol=1e-9 * np.abs(ad).max())
+    assert np.allclose(o.d("Unp1"), o.d("UnpHalf") + dt * acc, rtol=0, atol=1e-15)
