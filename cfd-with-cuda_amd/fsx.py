@@ -44,7 +44,7 @@ HOST_SYMBOLS = (
     "fsxh_mesh_info", "fsxh_mesh_params", "fsxh_mesh_set_params", "fsxh_mesh_get", "fsxh_mesh_profile",
     "fsxh_topology", "fsxh_topo_free", "fsxh_topo_info", "fsxh_topo_len", "fsxh_topo_get",
     "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_reset", "fsxd_ctx", "fsxd_topo", "fsxd_error",
-    "fsxd_setup_times",
+    "fsxd_setup_times", "fsxh_write_tecplot", "fsxh_read_restart", "fsxd_write_tecplot", "fsxd_read_restart",
 )
 
 
@@ -154,6 +154,10 @@ def lib():
         "fsxd_topo": ([_vp], _vp),
         "fsxd_error": ([_vp], ctypes.c_char_p),
         "fsxd_setup_times": ([_vp, _D], ctypes.c_int),
+        "fsxh_write_tecplot": ([_vp, ctypes.c_char_p, _D, _D, ctypes.c_char_p], ctypes.c_int),
+        "fsxh_read_restart": ([_vp, ctypes.c_char_p, _D, _D], ctypes.c_int),
+        "fsxd_write_tecplot": ([_vp, ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
+        "fsxd_read_restart": ([_vp, ctypes.c_char_p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -295,6 +299,25 @@ class Topology:
         keys = ("NE", "NCN", "NN", "NNp", "nnzM", "nnzG", "nnzZ", "nBCnodes", "ncolors", "monPoint", "zeroP")
         return dict(zip(keys, list(out)))
 
+    def write_tecplot(self, path, Un, Pn, title=""):
+        """Fields ([u|v|w] and corner pressures) to a Tecplot file in the
+        reference's layout (createTecplot, BCH:4249-4482)."""
+        un = np.ascontiguousarray(Un, dtype=np.float64)
+        pn = np.ascontiguousarray(Pn, dtype=np.float64)
+        rc = self.L.fsxh_write_tecplot(self.h, title.encode(), _dp(un), _dp(pn), path.encode())
+        if rc != FSX_OK:
+            raise FsxError(rc, f"cannot write {path}")
+
+    def read_restart(self, path):
+        """(Un, Pn) from a file in that layout (readRestartFile, BCH:4214-4242)."""
+        i = self.info()
+        un = np.zeros(3 * i["NN"])
+        pn = np.zeros(i["NNp"])
+        rc = self.L.fsxh_read_restart(self.h, path.encode(), _dp(un), _dp(pn))
+        if rc != FSX_OK:
+            raise FsxError(rc, f"cannot read restart file {path}")
+        return un, pn
+
     def get(self, name):
         n = self.L.fsxh_topo_len(self.h, name.encode())
         if n < 0:
@@ -369,6 +392,18 @@ class Solver:
         by = ctypes.c_double()
         self.check(self.L.fsx_bench_spmv(self.ctx, precision, reps, ctypes.byref(ms), ctypes.byref(by)), "bench_spmv")
         return ms.value, by.value
+
+    def write_tecplot(self, path, title=""):
+        """Current Un, Pn to a Tecplot file (createTecplot, BCH:4249-4482)."""
+        rc = self.L.fsxd_write_tecplot(self.h, title.encode(), path.encode())
+        if rc != FSX_OK:
+            raise FsxError(rc, self.L.fsxd_error(self.h).decode())
+
+    def read_restart(self, path):
+        """Restart from a Tecplot file instead of the initial BCs (BCH:2795-2796)."""
+        rc = self.L.fsxd_read_restart(self.h, path.encode())
+        if rc != FSX_OK:
+            raise FsxError(rc, self.L.fsxd_error(self.h).decode())
 
     def set_mass(self, mode, rtol=1e-10, maxit=1000):
         """Velocity mass of the step1/step3 updates: MASS_LUMPED (the

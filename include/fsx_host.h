@@ -112,6 +112,21 @@ const char* fsxd_error(fsxd_solver* s);
 /* setup wall times (s): topology, upload, assembly (incl. Z) */
 int fsxd_setup_times(fsxd_solver* s, double* out3);
 
+/* ------------------------------------------------------------ output and restart
+ * The reference's Tecplot layout (createTecplot, BCH:4249-4482): title,
+ * variables and zone lines, NN lines "x y z u v w p" in %.11e with the
+ * pressure at non-corner nodes interpolated from the element corners (edge,
+ * face, element means), then 8 NE one-based sub-brick connectivity lines
+ * (each hex27 split into its 8 octants). The restart reader takes the same
+ * file back (readRestartFile, BCH:4214-4242). */
+int fsxh_write_tecplot(const fsxh_topo* t, const char* title, const double* Un, const double* Pn, const char* path);
+int fsxh_read_restart(const fsxh_topo* t, const char* path, double* Un, double* Pn);
+/* the driver's current Un, Pn to a Tecplot file */
+int fsxd_write_tecplot(fsxd_solver* s, const char* title, const char* path);
+/* start from a restart file instead of the initial BCs (BCH:2795-2796):
+ * fsxd_reset, then Un and Pn from the file */
+int fsxd_read_restart(fsxd_solver* s, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

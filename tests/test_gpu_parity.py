@@ -281,3 +281,27 @@ def test_consistent_mass_steps(fsx_lib, oracle_lib, case):
     s.run(3)
     assert rel(s.download("Un"), un) > 1e-6
     s.close()
+
+
+def test_tecplot_restart_continues_run(fsx_lib, tmp_path):
+    """Output and restart through the driver (createTecplot BCH:4249-4482,
+    readRestartFile BCH:2795-2796): a run of 2 + 2 steps and a run of 2
+    steps, written, restarted in a new solver and continued for 2 steps end
+    at the same fields to the 12 digits the file keeps (the restart carries
+    Un and Pn only, like the reference's)."""
+    say("tecplot restart: cavity 4^3, 2 + 2 steps")
+    mesh = fsx_lib.Mesh.cavity(4).set_params(NGP=27, dt=0.01)
+    path = str(tmp_path / "cavity4.dat")
+    a = fsx_lib.Solver(mesh, device=0, NGP=27, pcg_rtol=1e-12)
+    a.run(2)
+    a.write_tecplot(path, title="cavity4")
+    u2, p2 = a.download("Un"), a.download("Pn")
+    a.run(2)
+    b = fsx_lib.Solver(mesh, device=0, NGP=27, pcg_rtol=1e-12)
+    b.read_restart(path)
+    assert rel(b.download("Un"), u2) < 1e-11 and rel(b.download("Pn"), p2) < 1e-11
+    b.run(2)
+    assert rel(b.download("Un"), a.download("Un")) < 1e-9
+    assert rel(b.download("Pn"), a.download("Pn")) < 1e-8
+    a.close()
+    b.close()

@@ -192,3 +192,60 @@ def test_abi_exports_every_declared_symbol(fsx_lib):
     missing = [s for s in sorted(declared) if not hasattr(L, s)]
     assert not missing, missing
     assert set(fsx_lib.DEVICE_SYMBOLS + fsx_lib.HOST_SYMBOLS) <= declared
+
+
+# parametric position of each hex27 local node (BCH:2312-2343)
+_POS = np.array([(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1),
+                 (0, -1, -1), (1, 0, -1), (0, 1, -1), (-1, 0, -1),
+                 (-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0),
+                 (0, -1, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1),
+                 (0, 0, -1), (0, -1, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, 0, 1), (0, 0, 0)])
+
+
+def test_tecplot_layout_and_restart_roundtrip(fsx_lib, tmp_path):
+    """The Tecplot writer (createTecplot, BCH:4249-4482) and restart reader
+    (BCH:4214-4242): header lines, NN node lines and 8 NE sub-brick lines;
+    fields read back to the 12 significant digits written; the pressure at a
+    non-corner node is the mean of the element corners that share its
+    parametric coordinates on the axes where it sits at +-1 (edge: 2, face:
+    4, centre: 8); on a uniform cavity every sub-brick is an axis-aligned cube
+    of side h/2 listed in hex8 vertex order, and they fill the unit cube."""
+    N = 3
+    topo = fsx_lib.Topology(fsx_lib.Mesh.cavity(N))
+    info = topo.info()
+    NN, NNp, NE = info["NN"], info["NNp"], info["NE"]
+    rng = np.random.default_rng(4)
+    un, pn = rng.standard_normal(3 * NN), rng.standard_normal(NNp)
+    path = str(tmp_path / "cavity3.dat")
+    topo.write_tecplot(path, un, pn, title="cavity3")
+    lines = open(path).read().splitlines()
+    assert lines[0] == "TITLE = cavity3"
+    assert lines[1] == "VARIABLES = x,  y,  z,  u, v, w, p"
+    assert lines[2] == f"ZONE N={NN}, E={8 * NE}, F=FEPOINT, ET=BRICK"
+    assert len(lines) == 3 + NN + 8 * NE
+    nodes = np.array([[float(v) for v in ln.split()] for ln in lines[3:3 + NN]])
+    assert re.fullmatch(r"(-?\d\.\d{11}e[+-]\d\d ){6}-?\d\.\d{11}e[+-]\d\d", lines[3])
+    assert np.array_equal(nodes[:, :3], topo.get("coord").reshape(-1, 3).round(12))
+    un2, pn2 = topo.read_restart(path)
+    assert np.allclose(un2, un, rtol=1e-11, atol=0) and np.allclose(pn2, pn, rtol=1e-11, atol=0)
+    assert np.allclose(nodes[:NNp, 6], pn, rtol=1e-11, atol=0)
+    L = topo.get("LtoGnode").reshape(NE, 27)
+    p = nodes[:, 6]
+    for e in range(NE):
+        for j in range(8, 27):
+            on = _POS[j] != 0
+            corners = [m for m in range(8) if np.array_equal(_POS[m][on], _POS[j][on])]
+            assert len(corners) == 2 ** int((~on).sum())
+            assert np.isclose(p[L[e, j]], np.mean(pn[L[e, corners]]), rtol=1e-10, atol=1e-12)
+    conn = np.array([[int(v) for v in ln.split()] for ln in lines[3 + NN:]]) - 1
+    assert conn.min() >= 0 and conn.max() < NN
+    xyz = nodes[:, :3]
+    h2 = 0.5 / N
+    hex8 = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+    vol = 0.0
+    for b in conn:
+        v = xyz[b]
+        assert np.allclose(v, v[0] + h2 * hex8, atol=1e-12)
+        vol += h2 ** 3
+    assert np.isclose(vol, 1.0)
+    assert len({tuple(sorted(b)) for b in conn}) == 8 * NE
