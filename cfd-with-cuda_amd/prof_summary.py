@@ -81,25 +81,28 @@ def steps_main(argv):
 
 
 def counter_avgs(db, counter):
+    """kernel -> (mean counter value per dispatch, dispatches, largest value)"""
     rows = sqlite3.connect(db).execute(
-        "select name, avg(counter_value), count(*) from pmc_events where counter_name = ? group by name",
-        (counter,)).fetchall()
-    return {clean(n): (v, c) for n, v, c in rows}
+        "select name, avg(counter_value), count(*), max(counter_value) from pmc_events where counter_name = ? "
+        "group by name", (counter,)).fetchall()
+    return {clean(n): (v, c, m) for n, v, c, m in rows}
 
 
 def pmc_main(argv):
     import json
     fdb, wdb, copy_bytes, out_json, cmd = argv[0], argv[1], float(argv[2]), argv[3], argv[4]
     fetch, write = counter_avgs(fdb, "FETCH_SIZE"), counter_avgs(wdb, "WRITE_SIZE")
+    # the calibration copy is the largest k_copy dispatch (the time step's own
+    # vector copies are smaller), so its factor comes from the largest value
     copies = [k for k in fetch if k.endswith("k_copy")]
-    cal = max(copies, key=lambda k: fetch[k][0])
-    f_fac = copy_bytes / (fetch[cal][0] * 1024.0)
-    w_fac = copy_bytes / (write[cal][0] * 1024.0)
+    cal = max(copies, key=lambda k: fetch[k][2])
+    f_fac = copy_bytes / (fetch[cal][2] * 1024.0)
+    w_fac = copy_bytes / (write[cal][2] * 1024.0)
     out = {"command": cmd, "calibration_kernel": cal, "copy_bytes_each_way": copy_bytes,
            "fetch_factor": f_fac, "write_factor": w_fac, "kernels": {}}
     print(f"# HBM traffic per launch (rocprofv3 --pmc), 64^3 cavity: one time step and the Z SpMVs\n\nCommand: `{cmd}`\n")
     print(f"Calibration: `{cal}` copies {copy_bytes / 2**20:.0f} MiB each way; FETCH_SIZE reads "
-          f"{fetch[cal][0]:.0f} KiB, WRITE_SIZE {write[cal][0]:.0f} KiB; factors {f_fac:.4f} (fetch), "
+          f"{fetch[cal][2]:.0f} KiB, WRITE_SIZE {write[cal][2]:.0f} KiB; factors {f_fac:.4f} (fetch), "
           f"{w_fac:.4f} (write).\n")
     print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per launch (scaled) |")
     print("|---|---|---|---|---|")
