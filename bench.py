@@ -31,9 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cfd-with-cuda_amd"))
 
 METRIC = "timesteps/s, 64^3 hex27 cavity: learned+mixed vs plain FP64 PCG at 1e-8 residual"
-# rocprofv3 names of the step's row kernels: step1 with K*Acc_prev fused
+# rocprofv3 names of the step's row kernels: step1 with K*Acc_prev fused and G*Pn kept
 # (inner iterations >= 2), step2 RHS, step3
-STEP_KERNELS = ("k_step1<8, 2, 2, true, true>", "k_step2_rhs<16, 8>", "k_step3<8, 2>")
+STEP_KERNELS = ("k_step1<8, 2, 2, true, true, 2>", "k_step2_rhs<16, 8>", "k_step3<8, 2>")
 
 
 def zspmv_name(prec, npat, ncls):

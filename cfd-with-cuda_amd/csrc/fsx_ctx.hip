@@ -356,7 +356,7 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
   // state vectors
   const size_t n3 = (size_t)3 * NN;
   for (double** v : {&ctx->Un, &ctx->Unp1, &ctx->Unp1_prev, &ctx->UnpHalf, &ctx->UnpHalf_prev, &ctx->Acc, &ctx->Acc_prev,
-                     &ctx->KAcc, &ctx->R1})
+                     &ctx->KAcc, &ctx->R1, &ctx->GPn})
     FSX_TRY(alloc(ctx, v, n3));
   for (double** v : {&ctx->Pn, &ctx->Pnp1, &ctx->Pnp1_prev, &ctx->Pdot, &ctx->R2, &ctx->wr, &ctx->wz, &ctx->wp, &ctx->wq,
                      &ctx->wh})
@@ -662,6 +662,7 @@ int fsx_assemble_static(fsx_ctx* ctx) {
     return fail(ctx, FSX_ESTATE, "fsx_assemble_static needs mesh, M and G patterns, BCs and parameters");
   FSX_TRY(launch_assemble_static(ctx));
   ctx->assembled = true;
+  ctx->gpn_valid = false;
   ctx->mass_dinv_ready = false;
   return FSX_OK;
 }
@@ -680,6 +681,7 @@ int fsx_init_state(fsx_ctx* ctx) {
   if (!ctx->have_mesh || !ctx->have_bc) return fail(ctx, FSX_ESTATE, "fsx_init_state needs mesh and BCs");
   const long long n3 = 3LL * ctx->NN;
   ctx->kacc_pending = false;
+  ctx->gpn_valid = false;
   for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
                     ctx->KAcc, ctx->R1})
     FSX_TRY(launch_fill(ctx, v, 0.0, n3));
@@ -783,6 +785,7 @@ int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
   if (steady) *steady = (m > converge) ? 0 : 1;                            // BCH:3059-3061
   std::swap(ctx->Un, ctx->Unp1);                                           // BCH:3070-3077
   std::swap(ctx->Pn, ctx->Pnp1);
+  ctx->gpn_valid = false;                                                  // G Pn changes with Pn
   return FSX_OK;
 }
 
@@ -901,6 +904,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
     return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
   FSX_TRY(settle_kacc(ctx));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
+  ctx->gpn_valid = false;
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   if (kind == 3) {
     const size_t NN = ctx->NN;

@@ -101,6 +101,8 @@ struct fsx_ctx {
   bool have_mesh = false, have_M = false, have_G = false, have_Z = false, have_bc = false, have_params = false;
   bool assembled = false, z_built = false, state_ready = false;
   bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_kacc)
+  bool gpn_valid = false;        // GPn holds G Pn for the current Pn
+  double* GPn = nullptr;         // 3NN: G Pn, node-major, kept over the inner iterations of a step
   // velocity mass (fsx_set_mass): lumped (the reference) or consistent, M_bc
   // solved by the 3-right-hand-side Jacobi-PCG of fsx_mass.hip
   int mass_mode = FSX_MASS_LUMPED;

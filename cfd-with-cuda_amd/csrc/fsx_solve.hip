@@ -64,8 +64,10 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // same pass over K also forms KAcc = K Acc_prev (BCH:2981-3035), which the
 // reference computes at the end of the previous inner iteration: K's columns
 // and values are read once for six right-hand sides instead of twice for
-// three.
-template <int LPR, int UK, int UG, bool KACC, bool KPAT>
+// three. Pn is fixed over the inner iterations of a time step, so G Pn is the
+// same in each: GMODE 1 forms it and keeps it in GPn, GMODE 2 reads it back
+// (24 bytes per row instead of G's 28 per nonzero), GMODE 0 only forms it.
+template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, ColSrc mcs,
                                                   const double* __restrict__ K, const int* __restrict__ Grow,
                                                   const int* __restrict__ Gcol, const double* __restrict__ G,
@@ -73,7 +75,8 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
                                                   const double* __restrict__ Un, double* __restrict__ R1,
                                                   const double* __restrict__ MdInv, const unsigned char* __restrict__ bc,
                                                   double* __restrict__ UnpHalf, double dt,
-                                                  const double* __restrict__ Accp, double* __restrict__ KAcc) {
+                                                  const double* __restrict__ Accp, double* __restrict__ KAcc,
+                                                  double* __restrict__ GPn) {
   int l;
   const long long r = my_row<LPR>(&l);
   double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
@@ -96,33 +99,41 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
         if constexpr (KACC) { a0 += a[j] * Accp[x]; a1 += a[j] * Accp[x + 1]; a2 += a[j] * Accp[x + 2]; }
       }
     }
-    const int gs = Grow[r], ge = Grow[r + 1];
-    for (int base = gs + l; base < ge; base += UG * LPR) {
-      int c[UG];
-      double b0[UG], b1[UG], b2[UG];
+    if constexpr (GMODE != 2) {
+      const int gs = Grow[r], ge = Grow[r + 1];
+      for (int base = gs + l; base < ge; base += UG * LPR) {
+        int c[UG];
+        double b0[UG], b1[UG], b2[UG];
 #pragma unroll
-      for (int j = 0; j < UG; j++) {
-        const int k = base + j * LPR, kk = k < ge ? k : ge - 1;
-        c[j] = Gcol[kk];
-        const bool ok = k < ge;
-        b0[j] = ok ? G[3 * (size_t)kk] : 0.0;
-        b1[j] = ok ? G[3 * (size_t)kk + 1] : 0.0;
-        b2[j] = ok ? G[3 * (size_t)kk + 2] : 0.0;
-      }
+        for (int j = 0; j < UG; j++) {
+          const int k = base + j * LPR, kk = k < ge ? k : ge - 1;
+          c[j] = Gcol[kk];
+          const bool ok = k < ge;
+          b0[j] = ok ? G[3 * (size_t)kk] : 0.0;
+          b1[j] = ok ? G[3 * (size_t)kk + 1] : 0.0;
+          b2[j] = ok ? G[3 * (size_t)kk + 2] : 0.0;
+        }
 #pragma unroll
-      for (int j = 0; j < UG; j++) {
-        const double p = Pn[c[j]];
-        g0 += b0[j] * p; g1 += b1[j] * p; g2 += b2[j] * p;
+        for (int j = 0; j < UG; j++) {
+          const double p = Pn[c[j]];
+          g0 += b0[j] * p; g1 += b1[j] * p; g2 += b2[j] * p;
+        }
       }
     }
   }
   k0 = group_sum<LPR>(k0); k1 = group_sum<LPR>(k1); k2 = group_sum<LPR>(k2);
-  g0 = group_sum<LPR>(g0); g1 = group_sum<LPR>(g1); g2 = group_sum<LPR>(g2);
+  if constexpr (GMODE != 2) { g0 = group_sum<LPR>(g0); g1 = group_sum<LPR>(g1); g2 = group_sum<LPR>(g2); }
   if constexpr (KACC) { a0 = group_sum<LPR>(a0); a1 = group_sum<LPR>(a1); a2 = group_sum<LPR>(a2); }
   if (r < NN && l < 3) {
     const double kk = l == 0 ? k0 : (l == 1 ? k1 : k2);
-    const double gg = l == 0 ? g0 : (l == 1 ? g1 : g2);
     const size_t i = 3 * (size_t)r + l;
+    double gg;
+    if constexpr (GMODE == 2) {
+      gg = GPn[i];
+    } else {
+      gg = l == 0 ? g0 : (l == 1 ? g1 : g2);
+      if constexpr (GMODE == 1) GPn[i] = gg;
+    }
     double R = (R1[i] - kk) - gg;
     if (bc[r]) R = 0.0;
     R1[i] = R;
@@ -141,17 +152,24 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   const ColSrc mcs{ctx->Mcol, ctx->Mrowpat, ctx->Mpatoff};
   const bool kacc = ctx->kacc_pending;
   ctx->kacc_pending = false;
-#define STEP1_LAUNCH(KA, KP)                                                                                          \
-  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, ctx->K, \
-                     ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv, ctx->bcflag, \
-                     ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr)
+  // G Pn: formed and kept by the first step1 after Pn changed, read back after
+  const bool gload = ctx->gpn_valid;
+#define STEP1_LAUNCH(KA, KP, GM)                                                                                      \
+  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs,     \
+                     ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,   \
+                     ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr,       \
+                     ctx->GPn)
+#define STEP1_G(KA, KP) \
+  do { if (gload) STEP1_LAUNCH(KA, KP, 2); else STEP1_LAUNCH(KA, KP, 1); } while (0)
   if (mcs.row_pat) {
-    if (kacc) STEP1_LAUNCH(true, true); else STEP1_LAUNCH(false, true);
+    if (kacc) STEP1_G(true, true); else STEP1_G(false, true);
   } else {
-    if (kacc) STEP1_LAUNCH(true, false); else STEP1_LAUNCH(false, false);
+    if (kacc) STEP1_G(true, false); else STEP1_G(false, false);
   }
+#undef STEP1_G
 #undef STEP1_LAUNCH
   FSX_TRY(launch_check(ctx, "k_step1"));
+  ctx->gpn_valid = true;
   if (ctx->mass_mode == FSX_MASS_CONSISTENT) return mass_update_step1(ctx);
   return FSX_OK;
 }
@@ -1008,7 +1026,7 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   const double kcols = ctx->Mrowpat ? 4.0 * NN + 4.0 * (double)ctx->Mpatlen : 4.0 * nK;
   auto launch = [&]() {
     switch (kernel) {
-      case FSX_KERNEL_STEP1: ctx->kacc_pending = true; (void)launch_step1(ctx, 2); break;
+      case FSX_KERNEL_STEP1: ctx->kacc_pending = true; ctx->gpn_valid = true; (void)launch_step1(ctx, 2); break;
       case FSX_KERNEL_KACC: (void)launch_kacc(ctx); break;
       case FSX_KERNEL_STEP2_RHS: (void)launch_step2_rhs(ctx, 2); break;
       default: launch_step3_rows(ctx, 2); break;
@@ -1016,10 +1034,10 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   };
   double bytes = 0;
   switch (kernel) {
-    case FSX_KERNEL_STEP1:   // K (col + value), G (col + 3 values), Uprev, Pn, R1 r/w, Un, MdInv, bc, UnpHalf,
-                             // Acc_prev, KAcc
-      bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 28.0 * nG + 4.0 * (NN + 1) + 8.0 * NNp + 48.0 * NN +
-              24.0 * NN + 8.0 * NN + 1.0 * NN + 24.0 * NN + 24.0 * NN + 24.0 * NN;
+    case FSX_KERNEL_STEP1:   // K (col + value), G Pn (kept: 3 doubles per row), Uprev, R1 r/w, Un, MdInv, bc,
+                             // UnpHalf, Acc_prev, KAcc: the form of the second and later inner iterations
+      bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
+              24.0 * NN + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
       bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
