@@ -318,9 +318,11 @@ __global__ __launch_bounds__(kBlock) void k_conv0(Grid3 gf, Grid3 g0, const floa
 // g0[o][c] from the previous layer (the transpose of k_conv0; g0 is already
 // zero on the boundary of G0). With z set: z = dinv r + y and the r.z, z.q
 // partials (slots 3, 4) of the flexible CG; otherwise y is stored. Grid:
-// kRedGrid blocks of kWideBlock threads, grid-stride (about one node per
-// thread on 65^3).
-constexpr int kWideBlock = 1024;
+// kRedGrid blocks of kWideBlock threads, each block a contiguous range of
+// nodes at a stride of the block size (about two nodes per thread on 65^3).
+// Measured per application at 64^3 (threads x nodes in flight per thread):
+// 512 x 1 7.0 us, 512 x 2 7.5, 256 x 4 7.9, 256 x 2 8.2, 1024 x 2 10.3.
+constexpr int kWideBlock = 512;
 
 __global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const float* __restrict__ h, float gain,
                                                    float* __restrict__ y, const float* __restrict__ r,
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const
   double acc2[2] = {0.0, 0.0};
   // block b owns nodes [b per, (b + 1) per); a thread takes kNodes of them
   // at a stride of the block size, all loads issued before the sums
-  constexpr int kNodes = 2;
+  constexpr int kNodes = 1;
   const long long n = gf.n(), per = (n + kRedGrid - 1) / kRedGrid;
   const long long lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
   for (long long f0 = lo + threadIdx.x; f0 < hi; f0 += (long long)kNodes * kWideBlock) {
