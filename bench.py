@@ -31,9 +31,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cfd-with-cuda_amd"))
 
 METRIC = "timesteps/s, 64^3 hex27 cavity: learned+mixed vs plain FP64 PCG at 1e-8 residual"
-# rocprofv3 names of the step's row kernels: step1 with K*Acc_prev fused and G*Pn kept
-# (inner iterations >= 2), step2 RHS, step3
-STEP_KERNELS = ("k_step1<8, 2, 2, true, true, 2>", "k_step2_rhs<16, 8>", "k_step3<8, 2>")
+
+
+def step_kernels(vt):
+    """rocprofv3 names of the step's row kernels for matrix values of type vt
+    ("double", or "float" in the mixed forms): step1 with K*Acc_prev fused
+    and G*Pn kept (inner iterations >= 2), step2's right-hand side, step3."""
+    return (f"k_step1<8, 2, 2, true, true, 2, {vt}>", f"k_step2_rhs<16, 8, {vt}>", f"k_step3<8, 2, {vt}>")
 
 
 def zspmv_name(prec, npat, ncls):
@@ -50,9 +54,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_mode(solver, mode, steps, warmup, rtol, maxit):
+def run_mode(solver, mode, steps, warmup, rtol, maxit, matrix_bits=64):
     import numpy as np
     solver.reset(mode=mode, rtol=rtol, maxit=maxit)
+    solver.set_matrix_precision(matrix_bits)
     solver.timing_reset(True)
     if warmup:
         solver.run(warmup)
@@ -92,6 +97,7 @@ def train_learned(solver, args):
     import fsx
     t0 = time.perf_counter()
     solver.reset(mode=fsx.SOLVER_MIXED, rtol=args.rtol, maxit=args.maxit)
+    solver.set_matrix_precision(32)
     n1 = args.n + 1
     solver.unet_setup((n1, n1, n1), seed=args.seed)
     solver.unet_capture(64)
@@ -243,7 +249,8 @@ def main():
             train = train_learned(solver, args)
             log(f"  {train}")
         log(f"mode {name}: {args.warmup} warmup + {args.steps} timed steps")
-        res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit)
+        res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit,
+                             matrix_bits=64 if name == "fp64" else 32)
         r = res[name]
         log(f"  {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms, "
             f"max relres {r['max_relres']:.2e}")
@@ -254,10 +261,12 @@ def main():
     # step's row kernels and the headline solver's Z SpMV; the roofline entry
     # is the one with the most device time per step in the headline mode
     steps_inner = h["inner_iters"] / args.steps
+    solver.set_matrix_precision(32 if prec else 64)   # the headline mode's matrices
+    names = step_kernels("float" if prec else "double")
     kern = {}
-    for kid, kname, per_step in ((fsx.KERNEL_STEP1, STEP_KERNELS[0], steps_inner),
-                                 (fsx.KERNEL_STEP2_RHS, STEP_KERNELS[1], steps_inner),
-                                 (fsx.KERNEL_STEP3, STEP_KERNELS[2], steps_inner)):
+    for kid, kname, per_step in ((fsx.KERNEL_STEP1, names[0], steps_inner),
+                                 (fsx.KERNEL_STEP2_RHS, names[1], steps_inner),
+                                 (fsx.KERNEL_STEP3, names[2], steps_inner)):
         ms_k, by_k = solver.bench_kernel(kid, reps=20)
         kern[kname] = dict(launch_ms=ms_k, bytes=by_k, ms_per_step=ms_k * per_step)
     ms, bytes_per = solver.bench_spmv(precision=prec, reps=100)

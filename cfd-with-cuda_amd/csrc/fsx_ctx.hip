@@ -248,6 +248,14 @@ int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, doub
   return FSX_OK;
 }
 
+int fsx_set_matrix_precision(fsx_ctx* ctx, int bits) {
+  if (!ctx) return FSX_EINVAL;
+  if (bits != 32 && bits != 64) return fail(ctx, FSX_EINVAL, "fsx_set_matrix_precision: 32 or 64");
+  ctx->mat_bits = bits;
+  ctx->gpn_valid = false;   // G Pn is formed again from the values now in use
+  return refresh_matrix_copies(ctx);
+}
+
 int fsx_set_graphs(fsx_ctx* ctx, int enable) {
   if (!ctx) return FSX_EINVAL;
   ctx->use_graphs = enable != 0;
@@ -663,6 +671,7 @@ int fsx_assemble_static(fsx_ctx* ctx) {
   FSX_TRY(launch_assemble_static(ctx));
   ctx->assembled = true;
   ctx->gpn_valid = false;
+  FSX_TRY(refresh_matrix_copies(ctx));
   ctx->mass_dinv_ready = false;
   return FSX_OK;
 }
@@ -917,6 +926,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
     std::vector<double> t(n);
     for (size_t k = 0; k < n; k++) t[ctx->posM[k]] = host[k];
     FSX_HIP(hipMemcpy(p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    if (id == FSX_VAL_K) FSX_TRY(refresh_matrix_copies(ctx));
   } else {
     FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
   }

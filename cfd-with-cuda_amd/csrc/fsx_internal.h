@@ -139,6 +139,11 @@ struct fsx_ctx {
   long long nnzM = 0, nnzG = 0, nnzZ = 0;
   int* Mrow = nullptr; int* Mcol = nullptr;
   double* K = nullptr; double* M = nullptr;
+  // FP32 copies of K, G, G^T read by the step kernels when the matrix
+  // precision is 32 (fsx_set_matrix_precision); null at precision 64
+  float* Kf = nullptr; float* Gf = nullptr; float* GTf = nullptr;
+  float *Kf_mem = nullptr, *Gf_mem = nullptr, *GTf_mem = nullptr;   // their allocations
+  int mat_bits = 64;
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
@@ -260,6 +265,7 @@ int launch_pnp1(fsx_ctx* ctx);
 int launch_step3(fsx_ctx* ctx, int iter);
 int launch_kacc(fsx_ctx* ctx);
 int settle_kacc(fsx_ctx* ctx);
+int refresh_matrix_copies(fsx_ctx* ctx);
 int mass_update_step1(fsx_ctx* ctx);
 int mass_update_step3(fsx_ctx* ctx);
 int launch_maxacc(fsx_ctx* ctx);

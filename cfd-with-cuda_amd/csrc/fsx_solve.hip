@@ -67,10 +67,10 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // three. Pn is fixed over the inner iterations of a time step, so G Pn is the
 // same in each: GMODE 1 forms it and keeps it in GPn, GMODE 2 reads it back
 // (24 bytes per row instead of G's 28 per nonzero), GMODE 0 only forms it.
-template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE>
+template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE, class VT>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, ColSrc mcs,
-                                                  const double* __restrict__ K, const int* __restrict__ Grow,
-                                                  const int* __restrict__ Gcol, const double* __restrict__ G,
+                                                  const VT* __restrict__ K, const int* __restrict__ Grow,
+                                                  const int* __restrict__ Gcol, const VT* __restrict__ G,
                                                   const double* __restrict__ Uprev, const double* __restrict__ Pn,
                                                   const double* __restrict__ Un, double* __restrict__ R1,
                                                   const double* __restrict__ MdInv, const unsigned char* __restrict__ bc,
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
       for (int j = 0; j < UK; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
         c[j] = col_at<KPAT>(mcs, r, cb, kk);
-        a[j] = k < re ? K[kk] : 0.0;
+        a[j] = k < re ? (double)K[kk] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < UK; j++) {
@@ -109,9 +109,9 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
           const int k = base + j * LPR, kk = k < ge ? k : ge - 1;
           c[j] = Gcol[kk];
           const bool ok = k < ge;
-          b0[j] = ok ? G[3 * (size_t)kk] : 0.0;
-          b1[j] = ok ? G[3 * (size_t)kk + 1] : 0.0;
-          b2[j] = ok ? G[3 * (size_t)kk + 2] : 0.0;
+          b0[j] = ok ? (double)G[3 * (size_t)kk] : 0.0;
+          b1[j] = ok ? (double)G[3 * (size_t)kk + 1] : 0.0;
+          b2[j] = ok ? (double)G[3 * (size_t)kk + 2] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < UG; j++) {
@@ -154,11 +154,15 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   ctx->kacc_pending = false;
   // G Pn: formed and kept by the first step1 after Pn changed, read back after
   const bool gload = ctx->gpn_valid;
-#define STEP1_LAUNCH(KA, KP, GM)                                                                                      \
-  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs,     \
-                     ctx->K, ctx->Grow, ctx->Gcol, ctx->G, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,   \
+  const bool f32 = ctx->Kf != nullptr;   // FP32 matrix values (fsx_set_matrix_precision)
+#define STEP1_LAUNCH_T(KA, KP, GM, VT, KV, GV)                                                                        \
+  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, \
+                     KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,           \
                      ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr,       \
                      ctx->GPn)
+#define STEP1_LAUNCH(KA, KP, GM) \
+  do { if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, ctx->Kf, ctx->Gf); \
+       else STEP1_LAUNCH_T(KA, KP, GM, double, ctx->K, ctx->G); } while (0)
 #define STEP1_G(KA, KP) \
   do { if (gload) STEP1_LAUNCH(KA, KP, 2); else STEP1_LAUNCH(KA, KP, 1); } while (0)
   if (mcs.row_pat) {
@@ -168,17 +172,43 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   }
 #undef STEP1_G
 #undef STEP1_LAUNCH
+#undef STEP1_LAUNCH_T
   FSX_TRY(launch_check(ctx, "k_step1"));
   ctx->gpn_valid = true;
   if (ctx->mass_mode == FSX_MASS_CONSISTENT) return mass_update_step1(ctx);
   return FSX_OK;
 }
 
+// ------------------------------------------------------------ matrix precision
+__global__ void k_d2f(long long n, const double* __restrict__ a, float* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// The step kernels' FP32 copies of K, G and G^T (rounded to nearest) at
+// matrix precision 32, after assembly or a K upload; at precision 64 the
+// kernels read the FP64 values.
+int refresh_matrix_copies(fsx_ctx* ctx) {
+  ctx->Kf = ctx->Gf = ctx->GTf = nullptr;
+  if (ctx->mat_bits != 32 || !ctx->assembled) return FSX_OK;
+  if (!ctx->Kf_mem) {
+    FSX_TRY(alloc(ctx, &ctx->Kf_mem, (size_t)ctx->nnzM));
+    FSX_TRY(alloc(ctx, &ctx->Gf_mem, 3 * (size_t)ctx->nnzG));
+    FSX_TRY(alloc(ctx, &ctx->GTf_mem, 3 * (size_t)ctx->nnzG));
+  }
+  hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzM, ctx->K, ctx->Kf_mem);
+  hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->G, ctx->Gf_mem);
+  hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->GT, ctx->GTf_mem);
+  FSX_TRY(launch_check(ctx, "k_d2f"));
+  ctx->Kf = ctx->Kf_mem; ctx->Gf = ctx->Gf_mem; ctx->GTf = ctx->GTf_mem;
+  return FSX_OK;
+}
+
 // ------------------------------------------------------------ K * Acc_prev
 // KAcc_c = K Acc_prev_c, c = 0..2, one pass over K (BCH:2981-3035)
-template <int LPR, int U, bool KPAT>
+template <int LPR, int U, bool KPAT, class VT>
 __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ Mrow, ColSrc mcs,
-                                               const double* __restrict__ K, const double* __restrict__ x,
+                                               const VT* __restrict__ K, const double* __restrict__ x,
                                                double* __restrict__ y) {
   int l;
   const long long r = my_row<LPR>(&l);
@@ -193,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
       for (int j = 0; j < U; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
         c[j] = col_at<KPAT>(mcs, r, cb, kk);
-        a[j] = k < re ? K[kk] : 0.0;
+        a[j] = k < re ? (double)K[kk] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < U; j++) {
@@ -209,12 +239,22 @@ __global__ __launch_bounds__(kBlock) void k_k3(int NN, const int* __restrict__ M
 int launch_kacc(fsx_ctx* ctx) {
   constexpr int LPR = 16;
   const ColSrc mcs{ctx->Mcol, ctx->Mrowpat, ctx->Mpatoff};
-  if (mcs.row_pat)
-    hipLaunchKernelGGL((k_k3<LPR, 4, true>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
-                       ctx->Mrow, mcs, ctx->K, ctx->Acc_prev, ctx->KAcc);
-  else
-    hipLaunchKernelGGL((k_k3<LPR, 4, false>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
-                       ctx->Mrow, mcs, ctx->K, ctx->Acc_prev, ctx->KAcc);
+  const dim3 grid(rows_grid(ctx->NN, LPR));
+  if (ctx->Kf) {
+    if (mcs.row_pat)
+      hipLaunchKernelGGL((k_k3<LPR, 4, true, float>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, ctx->Kf,
+                         ctx->Acc_prev, ctx->KAcc);
+    else
+      hipLaunchKernelGGL((k_k3<LPR, 4, false, float>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, ctx->Kf,
+                         ctx->Acc_prev, ctx->KAcc);
+  } else {
+    if (mcs.row_pat)
+      hipLaunchKernelGGL((k_k3<LPR, 4, true, double>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, ctx->K,
+                         ctx->Acc_prev, ctx->KAcc);
+    else
+      hipLaunchKernelGGL((k_k3<LPR, 4, false, double>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, ctx->K,
+                         ctx->Acc_prev, ctx->KAcc);
+  }
   ctx->kacc_pending = false;
   return launch_check(ctx, "k_k3");
 }
@@ -228,9 +268,9 @@ int settle_kacc(fsx_ctx* ctx) { return ctx->kacc_pending ? launch_kacc(ctx) : FS
 // R2[p] = sum_c G_c^T d_c, d_c = UnpHalf_c / dt^2 - [iter>1] MdOrigInv KAcc_c,
 // summed as (c0 + c1) + c2 like the three csrmv calls of BCH:3864-3868;
 // R2[p0] = 0 for p0 > 0 (BCH:4180-4184).
-template <int LPR, int U>
+template <int LPR, int U, class VT>
 __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int* __restrict__ GTrow,
-                                                      const int* __restrict__ GTcol, const double* __restrict__ GT,
+                                                      const int* __restrict__ GTcol, const VT* __restrict__ GT,
                                                       const double* __restrict__ UnpHalf, const double* __restrict__ MdOrigInv,
                                                       const double* __restrict__ KAcc, int use_kacc, double o2,
                                                       int zeroP, double* __restrict__ R2) {
@@ -247,9 +287,9 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
         c[j] = GTcol[kk];
         const bool ok = k < re;
-        a0[j] = ok ? GT[3 * (size_t)kk] : 0.0;
-        a1[j] = ok ? GT[3 * (size_t)kk + 1] : 0.0;
-        a2[j] = ok ? GT[3 * (size_t)kk + 2] : 0.0;
+        a0[j] = ok ? (double)GT[3 * (size_t)kk] : 0.0;
+        a1[j] = ok ? (double)GT[3 * (size_t)kk + 1] : 0.0;
+        a2[j] = ok ? (double)GT[3 * (size_t)kk + 2] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < U; j++) {
@@ -272,9 +312,14 @@ int launch_step2_rhs(fsx_ctx* ctx, int iter) {
   // 64^3: 16x8 0.307 ms, 16x4 0.310, 32x4 0.316, 16x2 0.312, 8x4 0.344, 8x8 0.346, 64x2 0.372, 4x8 0.429
   constexpr int LPR = 16;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
-  hipLaunchKernelGGL((k_step2_rhs<LPR, 8>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->NN,
-                     ctx->GTrow, ctx->GTcol, ctx->GT, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc, iter != 1 ? 1 : 0, o2,
-                     ctx->zeroP, ctx->R2);
+  if (ctx->GTf)
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
+                       ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
+                       iter != 1 ? 1 : 0, o2, ctx->zeroP, ctx->R2);
+  else
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, double>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
+                       ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GT, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
+                       iter != 1 ? 1 : 0, o2, ctx->zeroP, ctx->R2);
   return launch_check(ctx, "k_step2_rhs");
 }
 
@@ -322,9 +367,9 @@ int launch_pnp1(fsx_ctx* ctx) {
 // ------------------------------------------------------------ step 3
 // R3_c = -dt G_c Pdot - [iter>1] dt KAcc_c; BC rows 0; Acc = R3 MdInv;
 // Unp1 = UnpHalf + dt Acc (BCH:3912-3974).
-template <int LPR, int U>
+template <int LPR, int U, class VT>
 __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict__ Grow, const int* __restrict__ Gcol,
-                                                  const double* __restrict__ G, const double* __restrict__ Pdot,
+                                                  const VT* __restrict__ G, const double* __restrict__ Pdot,
                                                   const double* __restrict__ KAcc, int use_kacc,
                                                   const unsigned char* __restrict__ bc, const double* __restrict__ MdInv,
                                                   const double* __restrict__ UnpHalf, double* __restrict__ Acc,
@@ -342,9 +387,9 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
         c[j] = Gcol[kk];
         const bool ok = k < re;
-        a0[j] = ok ? G[3 * (size_t)kk] : 0.0;
-        a1[j] = ok ? G[3 * (size_t)kk + 1] : 0.0;
-        a2[j] = ok ? G[3 * (size_t)kk + 2] : 0.0;
+        a0[j] = ok ? (double)G[3 * (size_t)kk] : 0.0;
+        a1[j] = ok ? (double)G[3 * (size_t)kk + 1] : 0.0;
+        a2[j] = ok ? (double)G[3 * (size_t)kk + 2] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < U; j++) {
@@ -370,9 +415,14 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
 static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
   // 64^3: 8x2 0.253 ms, 8x4 0.266, 4x2 0.290, 4x8 0.298, 4x4 0.307, 16x2 0.312, 8x8 0.319, 16x4 0.343
   constexpr int LPR = 8;
-  hipLaunchKernelGGL((k_step3<LPR, 2>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Grow,
-                     ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv, ctx->UnpHalf,
-                     ctx->Acc, ctx->Unp1, ctx->dt, Rout);
+  if (ctx->Gf)
+    hipLaunchKernelGGL((k_step3<LPR, 2, float>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                       ctx->Grow, ctx->Gcol, ctx->Gf, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout);
+  else
+    hipLaunchKernelGGL((k_step3<LPR, 2, double>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                       ctx->Grow, ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout);
 }
 
 int launch_step3(fsx_ctx* ctx, int iter) {
@@ -1022,6 +1072,7 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   if (!ctx->K || !ctx->G || !ctx->GT) return fail(ctx, FSX_ESTATE, "fsx_bench_kernel: assemble first");
   if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP3) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
   const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG;
+  const double vb = ctx->Kf ? 4.0 : 8.0;   // bytes per stored matrix value
   // K's columns: a pattern start per row plus the dictionary, or a column per entry
   const double kcols = ctx->Mrowpat ? 4.0 * NN + 4.0 * (double)ctx->Mpatlen : 4.0 * nK;
   auto launch = [&]() {
@@ -1036,17 +1087,18 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   switch (kernel) {
     case FSX_KERNEL_STEP1:   // K (col + value), G Pn (kept: 3 doubles per row), Uprev, R1 r/w, Un, MdInv, bc,
                              // UnpHalf, Acc_prev, KAcc: the form of the second and later inner iterations
-      bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
+      bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
               24.0 * NN + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
-      bytes = 8.0 * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
+      bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2
-      bytes = 28.0 * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
+      bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
       break;
     default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc, Unp1
-      bytes = 28.0 * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN + 24.0 * NN;
+      bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +
+              24.0 * NN;
       break;
   }
   FSX_TRY(settle_kacc(ctx));

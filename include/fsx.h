@@ -129,6 +129,14 @@ int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit);
  * solve) and largest relative residual of the last mass solve, and the
  * iterations of all mass solves so far; any pointer may be NULL */
 int fsx_mass_stats(fsx_ctx* ctx, int* last_iters, double* last_relres, long long* total_iters);
+/* Storage precision of K, G and G^T as the step kernels read them (step1,
+ * K*Acc, step2's right-hand side, step3): 64 (default, the reference) or 32,
+ * the mixed-precision forms' FP32 assembly: FP32 copies rounded to nearest
+ * from the FP64 assembly, products and sums still in FP64. Z, the pressure
+ * solve's FP64 residual and every vector keep FP64. Costs 4 bytes per K
+ * entry and 24 per G entry of device memory. No reference counterpart (the
+ * reference is FP64 throughout; its `real` type is unused, BCH:130-134). */
+int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
 /* enqueue solver iterations as replayed HIP graphs (1) or as individual
  * launches (0, the default: on the 64^3 cavity both run within 5% of each
  * other); results are identical. No reference counterpart (the CUSP and MKL

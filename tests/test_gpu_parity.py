@@ -334,3 +334,33 @@ def test_reference_sized_cases(fsx_lib, oracle_lib, case):
     for x, v in zip(st, so[:, 1]):
         assert abs(x["pcg"] - v) <= 0.02 * v + 2, (x["pcg"], v)
     s.close()
+
+
+def test_fp32_matrix_copies(fsx_lib, oracle_lib):
+    """Matrix precision 32 (the mixed forms' FP32 assembly: the step kernels
+    read K, G, G^T rounded to FP32, with FP64 products, sums and vectors)
+    against the oracle whose K and G values are rounded to FP32 the same way
+    after set-up (Z and the lumped masses stay FP64 in both): fields after
+    3 steps agree to 1e-9; and against the FP64 run, the stated accuracy of
+    the form: velocity within 1e-6, pressure within 1e-5."""
+    say("matrix precision 32: cavity 8^3, 3 steps")
+    s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(8), NGP=27, pcg_rtol=1e-12)
+    for name in ("K", "G1", "G2", "G3"):
+        o.set_d(name, o.d(name).astype(np.float32).astype(np.float64))
+    s.run(3)
+    u64, p64 = s.download("Un"), s.download("Pn")
+    s.reset(mode=0)
+    s.set_matrix_precision(32)
+    s.run(3)
+    o.run(3)
+    u32, p32 = s.download("Un"), s.download("Pn")
+    assert rel(u32, o.d("Un")) < 1e-9, rel(u32, o.d("Un"))
+    assert rel(p32, o.d("Pn")) < 1e-9, rel(p32, o.d("Pn"))
+    eu, ep = rel(u32, u64), rel(p32, p64)
+    say(f"  FP32 matrices vs FP64: velocity {eu:.2e}, pressure {ep:.2e}")
+    assert 0 < eu < 1e-6 and ep < 1e-5
+    s.set_matrix_precision(64)
+    s.reset(mode=0)
+    s.run(3)
+    assert np.array_equal(s.download("Un"), u64)
+    s.close()
