@@ -55,9 +55,12 @@ def log(msg):
 
 
 def run_mode(solver, mode, steps, warmup, rtol, maxit, matrix_bits=64):
+    """Warm-up then timed steps in one solver mode; matrix_bits 32 gives the
+    mixed forms' FP32 assembly (K, G, G^T copies and the convection)."""
     import numpy as np
     solver.reset(mode=mode, rtol=rtol, maxit=maxit)
     solver.set_matrix_precision(matrix_bits)
+    solver.set_convection_precision(matrix_bits)
     solver.timing_reset(True)
     if warmup:
         solver.run(warmup)
@@ -98,6 +101,7 @@ def train_learned(solver, args):
     t0 = time.perf_counter()
     solver.reset(mode=fsx.SOLVER_MIXED, rtol=args.rtol, maxit=args.maxit)
     solver.set_matrix_precision(32)
+    solver.set_convection_precision(32)
     n1 = args.n + 1
     solver.unet_setup((n1, n1, n1), seed=args.seed)
     solver.unet_capture(64)

@@ -260,30 +260,71 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// packed LDS records of the compute type CT
+template <class CT> struct alignas(4 * sizeof(CT)) CT4 { CT x, y, z, w; };
+template <class CT> struct alignas(2 * sizeof(CT)) CT2 { CT x, y; };
+
+// jacobian() in the compute type CT
+template <class CT>
+__device__ __forceinline__ void jacobian_ct(const CT (*ec)[3], const CT* __restrict__ dSp_k, CT iJ[3][3], CT* det,
+                                            int stride) {
+  CT J[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      CT s = 0;
+#pragma unroll
+      for (int m = 0; m < 8; m++) s = s + dSp_k[(m * 3 + i) * stride] * ec[m][j];
+      J[i][j] = s;
+    }
+  iJ[0][0] =   J[1][1]*J[2][2] - J[2][1]*J[1][2];
+  iJ[0][1] = -(J[0][1]*J[2][2] - J[0][2]*J[2][1]);
+  iJ[0][2] =   J[0][1]*J[1][2] - J[1][1]*J[0][2];
+  iJ[1][0] = -(J[1][0]*J[2][2] - J[1][2]*J[2][0]);
+  iJ[1][1] =   J[2][2]*J[0][0] - J[2][0]*J[0][2];
+  iJ[1][2] = -(J[1][2]*J[0][0] - J[1][0]*J[0][2]);
+  iJ[2][0] =   J[1][0]*J[2][1] - J[2][0]*J[1][1];
+  iJ[2][1] = -(J[2][1]*J[0][0] - J[2][0]*J[0][1]);
+  iJ[2][2] =   J[1][1]*J[0][0] - J[1][0]*J[0][1];
+  const CT d = J[0][0]*(J[1][1]*J[2][2] - J[2][1]*J[1][2]) +
+               J[0][1]*(J[1][2]*J[2][0] - J[1][0]*J[2][2]) +
+               J[0][2]*(J[1][0]*J[2][1] - J[1][1]*J[2][0]);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) iJ[i][j] = iJ[i][j] / d;
+  *det = d;
+}
+
+// CT is the arithmetic type: double (the reference's arithmetic) or float
+// (the mixed forms' FP32 assembly: tables, node values, Jacobians and sums in
+// FP32, the result added to the FP64 R1).
+template <class CT>
 __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_elems, const int* __restrict__ elems,
                                                                    const int* __restrict__ L27,
                                                                    const int* __restrict__ P8,
                                                                    const double* __restrict__ xc, ShapeTables tab, int NGP,
                                                                    int NN, const double* __restrict__ Un,
                                                                    const double* __restrict__ Uprev, double* __restrict__ R1) {
-  __shared__ double4 sT[27 * kMaxNGP];                 // (Sv, dSv_0, dSv_1, dSv_2) at [j][k]
-  __shared__ double sSv[kMaxNGP * 27];                 // Sv at [k][i]
-  __shared__ double sdSp[24 * kMaxNGP];                // dSp at [m * 3 + i][k]
-  __shared__ double sw[kMaxNGP];
-  __shared__ double2 nod[kConvHalves][27][3];          // (Un_x, Un_y), (Un_z, Up_x), (Up_y, Up_z)
-  __shared__ double conv[kConvHalves][3][kMaxNGP];
-  __shared__ double ec[kConvHalves][8][3];
+  __shared__ CT4<CT> sT[27 * kMaxNGP];                 // (Sv, dSv_0, dSv_1, dSv_2) at [j][k]
+  __shared__ CT sSv[kMaxNGP * 27];                     // Sv at [k][i]
+  __shared__ CT sdSp[24 * kMaxNGP];                    // dSp at [m * 3 + i][k]
+  __shared__ CT sw[kMaxNGP];
+  __shared__ CT2<CT> nod[kConvHalves][27][3];          // (Un_x, Un_y), (Un_z, Up_x), (Up_y, Up_z)
+  __shared__ CT conv[kConvHalves][3][kMaxNGP];
+  __shared__ CT ec[kConvHalves][8][3];
   const int t = threadIdx.x;
   for (int i = t; i < NGP * 27; i += blockDim.x) {   // i = k * 27 + j
     const int k = i / 27, j = i - 27 * k;
-    sT[j * NGP + k] = make_double4(tab.Sv[i], tab.dSv[3 * i], tab.dSv[3 * i + 1], tab.dSv[3 * i + 2]);
-    sSv[i] = tab.Sv[i];
+    sT[j * NGP + k] = CT4<CT>{(CT)tab.Sv[i], (CT)tab.dSv[3 * i], (CT)tab.dSv[3 * i + 1], (CT)tab.dSv[3 * i + 2]};
+    sSv[i] = (CT)tab.Sv[i];
   }
   for (int i = t; i < NGP * 24; i += blockDim.x) {   // i = k * 24 + m * 3 + d
     const int k = i / 24, m3 = i - 24 * k;
-    sdSp[m3 * NGP + k] = tab.dSp[i];
+    sdSp[m3 * NGP + k] = (CT)tab.dSp[i];
   }
-  for (int i = t; i < NGP; i += blockDim.x) sw[i] = tab.w[i];
+  for (int i = t; i < NGP; i += blockDim.x) sw[i] = (CT)tab.w[i];
   __syncthreads();
   const int lane = t % kWave, l = lane & 31;
   const int hw = (t / kWave) * 2 + (lane >> 5);        // half-wave of this lane in the block
@@ -292,7 +333,7 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
   // registers of the element being fetched: node id and values (l < 27),
   // one corner coordinate (l < 24)
   int nn = 0;
-  double un[3] = {0, 0, 0}, up[3] = {0, 0, 0}, xv = 0;
+  CT un[3] = {0, 0, 0}, up[3] = {0, 0, 0}, xv = 0;
   auto fetch = [&](int sl) {
     const bool act = sl < ncolor_elems;
     const int e = act ? elems[sl] : 0;
@@ -300,37 +341,37 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
       nn = L27[(size_t)e * 27 + l];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        un[c] = Un[3 * (size_t)nn + c];
-        up[c] = Uprev[3 * (size_t)nn + c];
+        un[c] = (CT)Un[3 * (size_t)nn + c];
+        up[c] = (CT)Uprev[3 * (size_t)nn + c];
       }
     }
     int pn = 0;   // pressure id (row of xc) of corner l
     if (act && l < 8) pn = P8[(size_t)e * 8 + l];
     const int cn = __shfl(pn, (lane & 32) + (l < 24 ? l / 3 : 0), kWave);
-    if (act && l < 24) xv = xc[3 * (size_t)cn + l % 3];
+    if (act && l < 24) xv = (CT)xc[3 * (size_t)cn + l % 3];
   };
   fetch(slot);
   for (; slot < ncolor_elems; slot += stride) {
     // stage the fetched element, then start fetching the next one
     const int node = nn;
     if (l < 27) {
-      nod[hw][l][0] = make_double2(un[0], un[1]);
-      nod[hw][l][1] = make_double2(un[2], up[0]);
-      nod[hw][l][2] = make_double2(up[1], up[2]);
+      nod[hw][l][0] = CT2<CT>{un[0], un[1]};
+      nod[hw][l][1] = CT2<CT>{un[2], up[0]};
+      nod[hw][l][2] = CT2<CT>{up[1], up[2]};
     }
     if (l < 24) ec[hw][l / 3][l % 3] = xv;
     wave_sync();
     fetch(slot + stride);
     if (l < NGP) {
       const int k = l;
-      double iJ[3][3], det;
-      jacobian(ec[hw], sdSp + k, iJ, &det, NGP);
-      const double f = det * sw[k];
-      double u0[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      CT iJ[3][3], det;
+      jacobian_ct<CT>(ec[hw], sdSp + k, iJ, &det, NGP);
+      const CT f = det * sw[k];
+      CT u0[3] = {0, 0, 0}, gr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       for (int j = 0; j < 27; j++) {
-        const double4 T = sT[j * NGP + k];
-        const double2 a = nod[hw][j][0], b = nod[hw][j][1], c2 = nod[hw][j][2];
-        const double u0v[3] = {a.x, a.y, b.x}, upv[3] = {b.y, c2.x, c2.y};
+        const CT4<CT> T = sT[j * NGP + k];
+        const CT2<CT> a = nod[hw][j][0], b = nod[hw][j][1], c2 = nod[hw][j][2];
+        const CT u0v[3] = {a.x, a.y, b.x}, upv[3] = {b.y, c2.x, c2.y};
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           u0[c] += T.x * u0v[c];
@@ -339,7 +380,7 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        double g[3];
+        CT g[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) g[d] = iJ[d][0] * gr[c][0] + iJ[d][1] * gr[c][1] + iJ[d][2] * gr[c][2];
         conv[hw][c][k] = f * (u0[0] * g[0] + u0[1] * g[1] + u0[2] * g[2]);
@@ -348,38 +389,45 @@ __global__ __launch_bounds__(kWave * kConvWaves) void k_convection(int ncolor_el
     wave_sync();
     if (l < 27) {
       const int i = l;
-      double r[3] = {0, 0, 0};
+      CT r[3] = {0, 0, 0};
       for (int k = 0; k < NGP; k++) {
-        const double sv = sSv[k * 27 + i];
+        const CT sv = sSv[k * 27 + i];
 #pragma unroll
         for (int c = 0; c < 3; c++) r[c] += sv * conv[hw][c][k];
       }
 #pragma unroll
-      for (int c = 0; c < 3; c++) R1[3 * (size_t)node + c] -= r[c];
+      for (int c = 0; c < 3; c++) R1[3 * (size_t)node + c] -= (double)r[c];
     }
     wave_sync();   // the next element's staging overwrites this one's LDS
   }
 }
 
-int launch_convection(fsx_ctx* ctx) {
-  FSX_TRY(launch_fill(ctx, ctx->R1, 0.0, 3LL * ctx->NN));                  // BCH:3541-3545
-  if (ctx->conv_grid == 0) {
+template <class CT>
+static int launch_convection_t(fsx_ctx* ctx, int* grid_cache) {
+  if (*grid_cache == 0) {
     // as many blocks as fit on the device at once (the kernel is persistent)
     int per_cu = 0, cus = 0;
-    FSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_convection, kWave * kConvWaves, 0));
+    FSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_convection<CT>, kWave * kConvWaves, 0));
     FSX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    ctx->conv_grid = std::max(1, per_cu * cus);
+    *grid_cache = std::max(1, per_cu * cus);
   }
   const int ncol = (int)ctx->color_ptr.size() - 1;
   for (int c = 0; c < ncol; c++) {
     const int n = ctx->color_ptr[c + 1] - ctx->color_ptr[c];
     if (n == 0) continue;
-    const int g = std::min(ctx->conv_grid, (n + kConvHalves - 1) / kConvHalves);
-    hipLaunchKernelGGL(k_convection, dim3(g), dim3(kWave * kConvWaves), 0, ctx->stream, n,
+    const int g = std::min(*grid_cache, (n + kConvHalves - 1) / kConvHalves);
+    hipLaunchKernelGGL(k_convection<CT>, dim3(g), dim3(kWave * kConvWaves), 0, ctx->stream, n,
                        ctx->color_order + ctx->color_ptr[c], ctx->L27, ctx->P8, ctx->xc, ctx->tab, ctx->NGP, ctx->NN, ctx->Un,
                        ctx->UnpHalf_prev, ctx->R1);
   }
   return launch_check(ctx, "k_convection");
+}
+
+// FP64 arithmetic, or FP32 (fsx_set_convection_precision, the mixed forms)
+int launch_convection(fsx_ctx* ctx) {
+  FSX_TRY(launch_fill(ctx, ctx->R1, 0.0, 3LL * ctx->NN));                  // BCH:3541-3545
+  if (ctx->conv_bits == 32) return launch_convection_t<float>(ctx, &ctx->conv_grid_f);
+  return launch_convection_t<double>(ctx, &ctx->conv_grid);
 }
 
 }  // namespace fsx

@@ -256,6 +256,13 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits) {
   return refresh_matrix_copies(ctx);
 }
 
+int fsx_set_convection_precision(fsx_ctx* ctx, int bits) {
+  if (!ctx) return FSX_EINVAL;
+  if (bits != 32 && bits != 64) return fail(ctx, FSX_EINVAL, "fsx_set_convection_precision: 32 or 64");
+  ctx->conv_bits = bits;
+  return FSX_OK;
+}
+
 int fsx_set_graphs(fsx_ctx* ctx, int enable) {
   if (!ctx) return FSX_EINVAL;
   ctx->use_graphs = enable != 0;

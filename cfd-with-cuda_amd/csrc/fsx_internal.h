@@ -144,6 +144,7 @@ struct fsx_ctx {
   float* Kf = nullptr; float* Gf = nullptr; float* GTf = nullptr;
   float *Kf_mem = nullptr, *Gf_mem = nullptr, *GTf_mem = nullptr;   // their allocations
   int mat_bits = 64;
+  int conv_bits = 64;            // arithmetic of the convection kernel (fsx_set_convection_precision)
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
@@ -207,6 +208,7 @@ struct fsx_ctx {
   double* d_mon = nullptr;
   fsx::ShapeTables tab{};
   int conv_grid = 0;             // blocks of the persistent convection kernel (0: not yet queried)
+  int conv_grid_f = 0;           // the same for the FP32 convection
   double* tab_mem = nullptr;
   // accounting
   std::vector<void*> allocs;

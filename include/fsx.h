@@ -137,6 +137,11 @@ int fsx_mass_stats(fsx_ctx* ctx, int* last_iters, double* last_relres, long long
  * entry and 24 per G entry of device memory. No reference counterpart (the
  * reference is FP64 throughout; its `real` type is unused, BCH:130-134). */
 int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
+/* Arithmetic of the matrix-free convection residual (fsx_convection): 64
+ * (default, the reference's) or 32, the mixed forms' FP32 element assembly:
+ * shape tables, node values, Jacobians and Gauss-point sums in FP32, the
+ * element result subtracted from the FP64 R1. No reference counterpart. */
+int fsx_set_convection_precision(fsx_ctx* ctx, int bits);
 /* enqueue solver iterations as replayed HIP graphs (1) or as individual
  * launches (0, the default: on the 64^3 cavity both run within 5% of each
  * other); results are identical. No reference counterpart (the CUSP and MKL
