@@ -4,14 +4,17 @@ Workload: lid-driven cavity on the unit cube, 64^3 hex27-velocity /
 hex8-pressure elements (129^3 velocity nodes, 65^3 pressure nodes), 27-point
 Gauss quadrature, Re = 1000 (viscosity 1/1000 with lid speed 1), dt = 0.0025,
 3 inner iterations per step at tolerance 1e-3, pressure solved to a relative
-residual of 1e-8. Inputs are synthetic (generated mesh, zero initial fields
-plus the lid BC) and resident in HBM before timing starts.
+residual of 1e-8 from the previous Pdot in every mode (the reference's GPU
+path warm-starts its CG the same way; --x0 zero gives its CPU path's cold
+start). Inputs are synthetic (generated mesh, zero initial fields plus the
+lid BC) and resident in HBM before timing starts.
 
 One "step" is one time step of the reference's timeLoop (BCH:2859-3118):
 convection residual, step1, step2 (assembly of R2 + the pressure solve),
 step3, convergence checks, K*Acc, for up to 3 inner iterations. The same
 trajectory is run with the plain FP64 Jacobi-PCG, with the mixed-precision
-solver (FP32 inner PCG, FP64 refinement residual) and with learned+mixed
+form (FP32 assembly of K, G, G^T and the convection, FP32 inner PCG, FP64
+refinement residual) and with learned+mixed
 (flexible PCG in FP32 preconditioned by Jacobi plus the trained multilevel
 network, FP64 reliable updates and stop test, Jacobi-PCG fallback); `value`
 is the headline (learned+mixed) mode, the other rates, the pressure-solve
@@ -54,11 +57,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_mode(solver, mode, steps, warmup, rtol, maxit, matrix_bits=64):
+X0_POLICY = {"zero": 0, "previous": 1}
+
+
+def run_mode(solver, mode, steps, warmup, rtol, maxit, matrix_bits=64, x0="zero"):
     """Warm-up then timed steps in one solver mode; matrix_bits 32 gives the
-    mixed forms' FP32 assembly (K, G, G^T copies and the convection)."""
+    mixed forms' FP32 assembly (K, G, G^T copies and the convection); x0 the
+    pressure solves' initial guess (zero, or the previous Pdot)."""
     import numpy as np
     solver.reset(mode=mode, rtol=rtol, maxit=maxit)
+    solver.set_solver_x0(X0_POLICY[x0])
     solver.set_matrix_precision(matrix_bits)
     solver.set_convection_precision(matrix_bits)
     solver.timing_reset(True)
@@ -208,6 +216,9 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="only set up the case, run one FP64 time step, time the Z SpMV kernels and run a 256 MiB "
                          "calibration copy (the workload of the rocprofv3 --pmc traffic runs)")
+    ap.add_argument("--x0", default="previous", choices=("zero", "previous"),
+                    help="initial guess of every pressure solve, in all modes: the previous Pdot (the reference's "
+                         "GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
     ap.add_argument("--workload", default="cavity", choices=("cavity", "duct"),
                     help="duct: the bent-duct case with consistent-mass updates, printed as a line of its own")
     ap.add_argument("--no-cpu", action="store_true")
@@ -254,7 +265,7 @@ def main():
             log(f"  {train}")
         log(f"mode {name}: {args.warmup} warmup + {args.steps} timed steps")
         res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit,
-                             matrix_bits=64 if name == "fp64" else 32)
+                             matrix_bits=64 if name == "fp64" else 32, x0=args.x0)
         r = res[name]
         log(f"  {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms, "
             f"max relres {r['max_relres']:.2e}")
@@ -296,7 +307,8 @@ def main():
         "dtype": {"learned": "bf16+f32+f64", "mixed": "f32+f64"}.get(head, "f64"),
         "data": "synthetic",
         "config": {"workload": f"lid-driven cavity {args.n}^3 hex27/hex8, NGP {args.ngp}, Re {args.re:g}, "
-                               f"dt {args.dt:g}, 3 inner iterations, pressure rtol {args.rtol:g}",
+                               f"dt {args.dt:g}, 3 inner iterations, pressure rtol {args.rtol:g}, "
+                               f"pressure x0 {args.x0}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname,

@@ -256,6 +256,13 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits) {
   return refresh_matrix_copies(ctx);
 }
 
+int fsx_set_solver_x0(fsx_ctx* ctx, int policy) {
+  if (!ctx) return FSX_EINVAL;
+  if (policy != FSX_X0_ZERO && policy != FSX_X0_PREVIOUS) return fail(ctx, FSX_EINVAL, "fsx_set_solver_x0: unknown policy");
+  ctx->x0_policy = policy;
+  return FSX_OK;
+}
+
 int fsx_set_convection_precision(fsx_ctx* ctx, int bits) {
   if (!ctx) return FSX_EINVAL;
   if (bits != 32 && bits != 64) return fail(ctx, FSX_EINVAL, "fsx_set_convection_precision: 32 or 64");
@@ -374,7 +381,7 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
                      &ctx->KAcc, &ctx->R1, &ctx->GPn})
     FSX_TRY(alloc(ctx, v, n3));
   for (double** v : {&ctx->Pn, &ctx->Pnp1, &ctx->Pnp1_prev, &ctx->Pdot, &ctx->R2, &ctx->wr, &ctx->wz, &ctx->wp, &ctx->wq,
-                     &ctx->wh})
+                     &ctx->wh, &ctx->w0r, &ctx->w0e})
     FSX_TRY(alloc(ctx, v, (size_t)NNp));
   for (float** v : {&ctx->fr, &ctx->fz, &ctx->fp, &ctx->fq, &ctx->fx}) FSX_TRY(alloc(ctx, v, (size_t)NNp));
   FSX_TRY(alloc(ctx, &ctx->MdInv, (size_t)NN));

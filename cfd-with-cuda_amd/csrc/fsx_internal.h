@@ -195,6 +195,8 @@ struct fsx_ctx {
   double *Pn = nullptr, *Pnp1 = nullptr, *Pnp1_prev = nullptr, *Pdot = nullptr, *R2 = nullptr;
   // solver work (device)
   double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wq = nullptr, *wh = nullptr;    // FP64 PCG (+ FP64 residual)
+  double *w0r = nullptr, *w0e = nullptr;   // warm start: b - Z x0 and the correction
+  int x0_policy = FSX_X0_ZERO;
   float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 PCG
   double* partials = nullptr;    // 8 * kMaxGrid, per-block partial sums of the reduction kernels
   fsx::SolveScalars* sc = nullptr;

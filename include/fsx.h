@@ -80,6 +80,12 @@ enum fsx_solver_mode {
   FSX_SOLVER_LEARNED = 2       /* flexible PCG with the learned preconditioner, Jacobi-PCG fallback */
 };
 
+/* Initial guess of the pressure solve for fsx_set_solver_x0 */
+enum fsx_x0_policy {
+  FSX_X0_ZERO = 0,      /* x0 = 0, the CPU path (MKL RCI CG, BCH:4002-4004); the default */
+  FSX_X0_PREVIOUS = 1   /* the previous solution, the GPU path (CUSP keeps Pdot, CU:268, 425-442; Q6) */
+};
+
 /* Velocity mass for fsx_set_mass */
 enum fsx_mass_mode {
   FSX_MASS_LUMPED = 0,      /* row-sum lumped M, the reference's update (BCH:3262-3295, 3781-3783, 3960-3962) */
@@ -142,6 +148,14 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
  * shape tables, node values, Jacobians and Gauss-point sums in FP32, the
  * element result subtracted from the FP64 R1. No reference counterpart. */
 int fsx_set_convection_precision(fsx_ctx* ctx, int bits);
+/* Initial guess of the pressure solves (fsx_step2, fsx_solve_pressure).
+ * FSX_X0_PREVIOUS starts from the Pdot already held (zero after
+ * fsx_init_state, as the reference zeroes Pdot_d once): r0 = b - Z x0 in
+ * FP64, the selected solver on Z e = r0 from zero with its relative
+ * tolerance scaled by |b|/|r0|, x = x0 + e, so the stop test is the cold
+ * solve's |b - Z x| <= rtol |b|; an x0 already within it takes no
+ * iteration. */
+int fsx_set_solver_x0(fsx_ctx* ctx, int policy);
 /* enqueue solver iterations as replayed HIP graphs (1) or as individual
  * launches (0, the default: on the 64^3 cavity both run within 5% of each
  * other); results are identical. No reference counterpart (the CUSP and MKL

@@ -69,6 +69,7 @@ struct Case {
   // solver options (reference: rtol 1e-6 equivalent, maxit 1000)
   double pcg_rtol = 1e-8;
   int pcg_maxit = 1000;
+  bool pcg_warm = false;          // x0 = previous Pdot (the GPU path, CU:425-442) instead of 0
   // velocity mass: 0 lumped (the reference, BCH:3262-3295), 1 consistent M
   // solved by Jacobi-PCG for the three components (no reference counterpart)
   int mass_mode = 0;
@@ -667,9 +668,11 @@ void gtmv(const Case& c, const double* v, const double* x, double* y, double alp
 }
 
 // Textbook Jacobi-PCG (BCH:3981-4055 restated): x0 = 0, stop when
-// ||r||_2 <= rtol * ||b||_2 or after maxit iterations.
+// ||r||_2 <= rtol * ||b||_2 or after maxit iterations. With warm, x0 is the
+// x passed in (the reference's GPU path keeps the previous Pdot, CU:425-442)
+// and r0 = b - A x0; the stop test stays relative to ||b||.
 int pcg(const std::vector<int>& rp, const std::vector<int>& ci, const std::vector<double>& v, int n,
-        const double* b, double* x, double rtol, int maxit, double* relres) {
+        const double* b, double* x, double rtol, int maxit, double* relres, bool warm = false) {
   std::vector<double> dinv(n), r(n), z(n), p(n), q(n);
   for (int i = 0; i < n; i++) {
     double d = 0.0;
@@ -677,13 +680,21 @@ int pcg(const std::vector<int>& rp, const std::vector<int>& ci, const std::vecto
     dinv[i] = 1.0 / d;
   }
   double bb = 0.0;
-  for (int i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; bb += b[i] * b[i]; }
+  if (warm) {
+    csrmv(rp, ci, v.data(), x, r.data(), n, 1.0, 0.0);
+    for (int i = 0; i < n; i++) { r[i] = b[i] - r[i]; bb += b[i] * b[i]; }
+  } else {
+    for (int i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; bb += b[i] * b[i]; }
+  }
   double bnorm = std::sqrt(bb);
   if (bnorm == 0.0) { *relres = 0.0; return 0; }
+  double rr0 = 0.0;
+  for (int i = 0; i < n; i++) rr0 += r[i] * r[i];
+  if (std::sqrt(rr0) <= rtol * bnorm) { *relres = std::sqrt(rr0) / bnorm; return 0; }
   double rz = 0.0;
   for (int i = 0; i < n; i++) { z[i] = dinv[i] * r[i]; p[i] = z[i]; rz += r[i] * z[i]; }
   int it = 0;
-  double rnorm = bnorm;
+  double rnorm = std::sqrt(rr0);
   while (it < maxit) {
     csrmv(rp, ci, v.data(), p.data(), q.data(), n, 1.0, 0.0);
     double pq = 0.0;
@@ -832,7 +843,8 @@ void step2(Case& c, int iter) {
     gtmv(c, Gs[d]->data(), dummy.data() + (size_t)d*NN, c.R2.data(), 1.0, d == 0 ? 0.0 : 1.0);
   if (c.zeroP > 0) c.R2[c.zeroP] = 0.0;                                  // BCH:4180-4184
   auto t0 = std::chrono::steady_clock::now();
-  c.last_pcg_iters = pcg(c.Zrow, c.Zcol, c.Zval, c.NNp, c.R2.data(), c.Pdot.data(), c.pcg_rtol, c.pcg_maxit, &c.last_pcg_relres);
+  c.last_pcg_iters = pcg(c.Zrow, c.Zcol, c.Zval, c.NNp, c.R2.data(), c.Pdot.data(), c.pcg_rtol, c.pcg_maxit, &c.last_pcg_relres,
+                         c.pcg_warm);
   c.t_pcg_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   c.total_pcg_iters += c.last_pcg_iters;
   c.n_solves++;
@@ -1041,6 +1053,14 @@ int orc_run(void* h, int nsteps, double* stats) {
 }
 
 int orc_step1(void* h, int iter) { step1(*(Case*)h, iter); return 0; }
+
+// pressure PCG initial guess: 0 zero (BCH:4002-4004), 1 the previous Pdot (CU:425-442)
+int orc_set_pcg_x0(void* h, int policy) {
+  Case& c = *(Case*)h;
+  if (policy < 0 || policy > 1) { c.err = "orc_set_pcg_x0: bad policy"; return 1; }
+  c.pcg_warm = policy == 1;
+  return 0;
+}
 
 // velocity mass mode (0 lumped, 1 consistent with Jacobi-PCG per component)
 int orc_set_mass(void* h, int mode, double rtol, int maxit) {

@@ -57,6 +57,7 @@ def lib():
         L.orc_spmv.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double]
         L.orc_timings.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.orc_set_pcg_x0.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_mass_stats.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -157,6 +158,11 @@ class Oracle:
         stats = np.zeros((nsteps, 8))
         done = self.L.orc_run(self.h, nsteps, _dp(stats))
         return stats[:done]
+
+    def set_pcg_x0(self, policy):
+        """Pressure PCG initial guess: 0 zero (the CPU path), 1 the previous
+        Pdot (the GPU path's warm start)."""
+        self._check(self.L.orc_set_pcg_x0(self.h, policy), "set_pcg_x0")
 
     def set_mass(self, mode, rtol=1e-10, maxit=1000):
         """Velocity mass: 0 lumped (the reference), 1 consistent M solved by

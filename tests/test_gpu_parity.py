@@ -394,3 +394,27 @@ def test_fp32_convection(fsx_lib):
     s.run(3)
     assert rel(s.download("Un"), u64) < 1e-6 and rel(s.download("Pn"), p64) < 1e-5
     s.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_warm_start(fsx_lib, oracle_lib, mode):
+    """Warm-started pressure solves (FSX_X0_PREVIOUS, the reference GPU path's
+    x0) in the FP64 (mode 0), mixed (1) and learned (2) solvers against the
+    oracle's warm-started PCG: fields after 4 steps to 1e-6, every solve at
+    its stop test, and, for FP64, per-step PCG iteration counts within 2%."""
+    say(f"warm start, solver mode {mode}: cavity 16^3, 4 steps")
+    s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(16), NGP=27, pcg_rtol=1e-10)
+    if mode == 2:
+        s.unet_setup((17, 17, 17), seed=3)
+    s.set_solver(mode, 1e-10, 5000)
+    s.set_solver_x0(fsx_lib.X0_PREVIOUS)
+    o.set_pcg_x0(1)
+    st = s.run(4)
+    so = o.run(4)
+    assert all(x["relres"] <= 1e-10 for x in st)
+    assert rel(s.download("Un"), o.d("Un")) < 1e-6
+    assert rel(s.download("Pn"), o.d("Pn")) < 1e-6
+    if mode == 0:
+        for x, v in zip(st, so[:, 1]):
+            assert abs(x["pcg"] - v) <= 0.02 * v + 2, (x["pcg"], v)
+    s.close()

@@ -429,3 +429,23 @@ def test_consistent_mass_step(oracle_lib):
         ad = np.linalg.solve(Mbc, R3[c * NN:(c + 1) * NN])
         assert np.allclose(acc[c * NN:(c + 1) * NN], ad, rtol=1e-9, atol=1e-9 * np.abs(ad).max())
     assert np.allclose(o.d("Unp1"), o.d("UnpHalf") + dt * acc, rtol=0, atol=1e-15)
+
+
+def test_warm_start_pcg(oracle_lib):
+    """Pressure PCG from the previous Pdot (the reference's GPU path keeps it,
+    CU:425-442): the same fields as the cold start (both to rtol 1e-12), fewer
+    iterations in total, and each solve's true residual |R2 - Z Pdot| within
+    rtol |R2| (recomputed here from the oracle's Z)."""
+    cold = oracle_lib.case_from_mesh(meshgen.cavity(4), NGP=27, pcg_rtol=1e-12, pcg_maxit=5000)
+    warm = oracle_lib.case_from_mesh(meshgen.cavity(4), NGP=27, pcg_rtol=1e-12, pcg_maxit=5000)
+    warm.set_pcg_x0(1)
+    sc = cold.run(3)
+    sw = warm.run(3)
+    for a in ("Un", "Pn"):
+        x, y = warm.d(a), cold.d(a)
+        assert np.linalg.norm(x - y) <= 1e-9 * np.linalg.norm(y), a
+    assert sw[:, 1].sum() < sc[:, 1].sum()
+    n = warm.sizes()["NNp"]
+    Z = csr_dense(warm.i("Zrow"), warm.i("Zcol"), warm.d("Z"), (n, n))
+    r2, pd = warm.d("R2"), warm.d("Pdot")
+    assert np.linalg.norm(r2 - Z @ pd) <= 1.01e-12 * np.linalg.norm(r2)
