@@ -214,7 +214,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--probe", action="store_true",
-                    help="only set up the case, run one FP64 time step, time the Z SpMV kernels and run a 256 MiB "
+                    help="only set up the case, run one FP64 time step and one with FP32 matrices and convection, time "
+                         "the step kernels and the Z SpMV kernels and run a 256 MiB "
                          "calibration copy (the workload of the rocprofv3 --pmc traffic runs)")
     ap.add_argument("--x0", default="previous", choices=("zero", "previous"),
                     help="initial guess of every pressure solve, in all modes: the previous Pdot (the reference's "
@@ -246,6 +247,15 @@ def main():
         solver.sync()
         for kid in (fsx.KERNEL_STEP1, fsx.KERNEL_KACC, fsx.KERNEL_STEP2_RHS, fsx.KERNEL_STEP3):
             solver.bench_kernel(kid, reps=3)
+        # the same with the mixed forms' FP32 matrices and convection
+        solver.set_matrix_precision(32)
+        solver.set_convection_precision(32)
+        solver.run(1)
+        solver.sync()
+        for kid in (fsx.KERNEL_STEP1, fsx.KERNEL_KACC, fsx.KERNEL_STEP2_RHS, fsx.KERNEL_STEP3):
+            solver.bench_kernel(kid, reps=3)
+        solver.set_matrix_precision(64)
+        solver.set_convection_precision(64)
         ms32, b32 = solver.bench_spmv(precision=1, reps=20)
         ms64, b64 = solver.bench_spmv(precision=0, reps=20)
         copy_bytes = 256 << 20
