@@ -705,6 +705,7 @@ int fsx_init_state(fsx_ctx* ctx) {
   const long long n3 = 3LL * ctx->NN;
   ctx->kacc_pending = false;
   ctx->gpn_valid = false;
+  drop_records(ctx);
   for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
                     ctx->KAcc, ctx->R1})
     FSX_TRY(launch_fill(ctx, v, 0.0, n3));
@@ -736,7 +737,7 @@ int fsx_begin_step(fsx_ctx* ctx) {
   FSX_TRY(launch_copy(ctx, ctx->UnpHalf_prev, ctx->Un, 3LL * ctx->NN));    // BCH:2872-2876
   FSX_TRY(launch_fill(ctx, ctx->Acc_prev, 0.0, 3LL * ctx->NN));
   FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pn, ctx->NNp));
-  return FSX_OK;
+  return records_begin_step(ctx);
 }
 
 int fsx_convection(fsx_ctx* ctx) { FSX_TRY(need_step(ctx)); return launch_convection(ctx); }
@@ -783,6 +784,10 @@ int fsx_next_iteration(fsx_ctx* ctx) {
   // loop still moves them into Un and Pn (BCH:3070-3077), so they are copied.
   std::swap(ctx->UnpHalf_prev, ctx->UnpHalf);
   std::swap(ctx->Acc_prev, ctx->Acc);
+  std::swap(ctx->Urec_prev, ctx->Urec);   // their FP32 shadows go with them
+  std::swap(ctx->Arec_prev, ctx->Arec);
+  std::swap(ctx->urec_prev_ok, ctx->urec_ok);
+  std::swap(ctx->arec_prev_ok, ctx->arec_ok);
   FSX_TRY(launch_copy(ctx, ctx->Unp1_prev, ctx->Unp1, 3LL * ctx->NN));
   FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pnp1, ctx->NNp));
   return FSX_OK;
@@ -928,6 +933,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   FSX_TRY(settle_kacc(ctx));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
   ctx->gpn_valid = false;
+  drop_records(ctx);   // the FP32 shadows may no longer match the vectors
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   if (kind == 3) {
     const size_t NN = ctx->NN;

@@ -145,6 +145,14 @@ struct fsx_ctx {
   float *Kf_mem = nullptr, *Gf_mem = nullptr, *GTf_mem = nullptr;   // their allocations
   int mat_bits = 64;
   int conv_bits = 64;            // arithmetic of the convection kernel (fsx_set_convection_precision)
+  // FP32 shadows of UnpHalf and Acc (and of their "prev" buffers), 4 floats
+  // per node (the 4th unused), written beside the FP64 vectors by step1 and
+  // step3 at matrix precision 32 with lumped mass. step1 gathers its K
+  // operands from the prev shadows while their flags say they hold the FP64
+  // prev vectors rounded; any other writer of those vectors clears the flags.
+  float *Urec = nullptr, *Urec_prev = nullptr, *Arec = nullptr, *Arec_prev = nullptr;
+  float* rec_mem = nullptr;
+  bool urec_ok = false, urec_prev_ok = false, arec_ok = false, arec_prev_ok = false;
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
@@ -264,6 +272,10 @@ int launch_assemble_static(fsx_ctx* ctx);
 int launch_build_Z(fsx_ctx* ctx);
 int launch_convection(fsx_ctx* ctx);
 int launch_step1(fsx_ctx* ctx, int iter);
+int records_begin_step(fsx_ctx* ctx);
+inline void drop_records(fsx_ctx* ctx) {
+  ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = false;
+}
 int launch_step2_rhs(fsx_ctx* ctx, int iter);
 int launch_pnp1(fsx_ctx* ctx);
 int launch_step3(fsx_ctx* ctx, int iter);

@@ -67,7 +67,11 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // three. Pn is fixed over the inner iterations of a time step, so G Pn is the
 // same in each: GMODE 1 forms it and keeps it in GPn, GMODE 2 reads it back
 // (24 bytes per row instead of G's 28 per nonzero), GMODE 0 only forms it.
-template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE, class VT>
+// With REC the K operands are gathered from the FP32 shadows Urp (and Arp)
+// of UnpHalf_prev (and Acc_prev), one 16-byte load per column and vector
+// instead of 24 bytes; the products and sums stay FP64. A non-null Uout
+// receives UnpHalf rounded to FP32, the shadow the next inner iteration reads.
+template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE, class VT, bool REC>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, ColSrc mcs,
                                                   const VT* __restrict__ K, const int* __restrict__ Grow,
                                                   const int* __restrict__ Gcol, const VT* __restrict__ G,
@@ -76,7 +80,8 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
                                                   const double* __restrict__ MdInv, const unsigned char* __restrict__ bc,
                                                   double* __restrict__ UnpHalf, double dt,
                                                   const double* __restrict__ Accp, double* __restrict__ KAcc,
-                                                  double* __restrict__ GPn) {
+                                                  double* __restrict__ GPn, const float4* __restrict__ Urp,
+                                                  const float4* __restrict__ Arp, float* __restrict__ Uout) {
   int l;
   const long long r = my_row<LPR>(&l);
   double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
@@ -94,9 +99,18 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
       }
 #pragma unroll
       for (int j = 0; j < UK; j++) {
-        const size_t x = 3 * (size_t)c[j];
-        k0 += a[j] * Uprev[x]; k1 += a[j] * Uprev[x + 1]; k2 += a[j] * Uprev[x + 2];
-        if constexpr (KACC) { a0 += a[j] * Accp[x]; a1 += a[j] * Accp[x + 1]; a2 += a[j] * Accp[x + 2]; }
+        if constexpr (REC) {
+          const float4 u = Urp[c[j]];
+          k0 += a[j] * (double)u.x; k1 += a[j] * (double)u.y; k2 += a[j] * (double)u.z;
+          if constexpr (KACC) {
+            const float4 v = Arp[c[j]];
+            a0 += a[j] * (double)v.x; a1 += a[j] * (double)v.y; a2 += a[j] * (double)v.z;
+          }
+        } else {
+          const size_t x = 3 * (size_t)c[j];
+          k0 += a[j] * Uprev[x]; k1 += a[j] * Uprev[x + 1]; k2 += a[j] * Uprev[x + 2];
+          if constexpr (KACC) { a0 += a[j] * Accp[x]; a1 += a[j] * Accp[x + 1]; a2 += a[j] * Accp[x + 2]; }
+        }
       }
     }
     if constexpr (GMODE != 2) {
@@ -137,7 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
     double R = (R1[i] - kk) - gg;
     if (bc[r]) R = 0.0;
     R1[i] = R;
-    UnpHalf[i] = Un[i] + dt * R * MdInv[r];
+    const double uh = Un[i] + dt * R * MdInv[r];
+    UnpHalf[i] = uh;
+    if (Uout) Uout[4 * (size_t)r + l] = (float)uh;
     if constexpr (KACC) KAcc[i] = l == 0 ? a0 : (l == 1 ? a1 : a2);
   }
 }
@@ -155,14 +171,22 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   // G Pn: formed and kept by the first step1 after Pn changed, read back after
   const bool gload = ctx->gpn_valid;
   const bool f32 = ctx->Kf != nullptr;   // FP32 matrix values (fsx_set_matrix_precision)
-#define STEP1_LAUNCH_T(KA, KP, GM, VT, KV, GV)                                                                        \
-  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, mcs, \
-                     KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,           \
+  // FP32 operand shadows: written with lumped mass (the consistent update
+  // rewrites UnpHalf afterwards), read when they hold the prev vectors
+  const bool wrec = f32 && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT;
+  const bool rrec = wrec && ctx->urec_prev_ok && (!kacc || ctx->arec_prev_ok);
+  const float4* urp = reinterpret_cast<const float4*>(ctx->Urec_prev);
+  const float4* arp = reinterpret_cast<const float4*>(ctx->Arec_prev);
+  float* uout = wrec ? ctx->Urec : nullptr;
+#define STEP1_LAUNCH_T(KA, KP, GM, VT, RC, KV, GV)                                                                    \
+  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT, RC>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,  \
+                     mcs, KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,      \
                      ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr,       \
-                     ctx->GPn)
+                     ctx->GPn, urp, arp, uout)
 #define STEP1_LAUNCH(KA, KP, GM) \
-  do { if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, ctx->Kf, ctx->Gf); \
-       else STEP1_LAUNCH_T(KA, KP, GM, double, ctx->K, ctx->G); } while (0)
+  do { if (rrec) STEP1_LAUNCH_T(KA, KP, GM, float, true, ctx->Kf, ctx->Gf); \
+       else if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, false, ctx->Kf, ctx->Gf); \
+       else STEP1_LAUNCH_T(KA, KP, GM, double, false, ctx->K, ctx->G); } while (0)
 #define STEP1_G(KA, KP) \
   do { if (gload) STEP1_LAUNCH(KA, KP, 2); else STEP1_LAUNCH(KA, KP, 1); } while (0)
   if (mcs.row_pat) {
@@ -175,6 +199,7 @@ int launch_step1(fsx_ctx* ctx, int iter) {
 #undef STEP1_LAUNCH_T
   FSX_TRY(launch_check(ctx, "k_step1"));
   ctx->gpn_valid = true;
+  ctx->urec_ok = wrec;
   if (ctx->mass_mode == FSX_MASS_CONSISTENT) return mass_update_step1(ctx);
   return FSX_OK;
 }
@@ -190,17 +215,43 @@ __global__ void k_d2f(long long n, const double* __restrict__ a, float* __restri
 // kernels read the FP64 values.
 int refresh_matrix_copies(fsx_ctx* ctx) {
   ctx->Kf = ctx->Gf = ctx->GTf = nullptr;
+  drop_records(ctx);
   if (ctx->mat_bits != 32 || !ctx->assembled) return FSX_OK;
   if (!ctx->Kf_mem) {
     FSX_TRY(alloc(ctx, &ctx->Kf_mem, (size_t)ctx->nnzM));
     FSX_TRY(alloc(ctx, &ctx->Gf_mem, 3 * (size_t)ctx->nnzG));
     FSX_TRY(alloc(ctx, &ctx->GTf_mem, 3 * (size_t)ctx->nnzG));
   }
+  if (!ctx->rec_mem) {
+    const size_t n4 = 4 * (size_t)ctx->NN;
+    FSX_TRY(alloc(ctx, &ctx->rec_mem, 4 * n4));
+    ctx->Urec = ctx->rec_mem; ctx->Urec_prev = ctx->rec_mem + n4;
+    ctx->Arec = ctx->rec_mem + 2 * n4; ctx->Arec_prev = ctx->rec_mem + 3 * n4;
+  }
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzM, ctx->K, ctx->Kf_mem);
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->G, ctx->Gf_mem);
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->GT, ctx->GTf_mem);
   FSX_TRY(launch_check(ctx, "k_d2f"));
   ctx->Kf = ctx->Kf_mem; ctx->Gf = ctx->Gf_mem; ctx->GTf = ctx->GTf_mem;
+  return FSX_OK;
+}
+
+// FP32 shadows of a node-major velocity vector: 4 floats per node, the 4th 0
+__global__ void k_rec(long long nn, const double* __restrict__ v, float4* __restrict__ rec) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x)
+    rec[i] = make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], 0.0f);
+}
+
+// The shadows of UnpHalf_prev = Un and Acc_prev = 0 that fsx_begin_step sets
+// (only where step1 and step3 keep the shadows up to date)
+int records_begin_step(fsx_ctx* ctx) {
+  drop_records(ctx);
+  if (!ctx->Kf || !ctx->rec_mem || ctx->mass_mode == FSX_MASS_CONSISTENT) return FSX_OK;
+  hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
+                     reinterpret_cast<float4*>(ctx->Urec_prev));
+  FSX_TRY(launch_check(ctx, "k_rec"));
+  FSX_TRY(launch_fill(ctx, reinterpret_cast<double*>(ctx->Arec_prev), 0.0, 2LL * ctx->NN));   // 16 zero bytes per node
+  ctx->urec_prev_ok = ctx->arec_prev_ok = true;
   return FSX_OK;
 }
 
@@ -373,7 +424,8 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
                                                   const double* __restrict__ KAcc, int use_kacc,
                                                   const unsigned char* __restrict__ bc, const double* __restrict__ MdInv,
                                                   const double* __restrict__ UnpHalf, double* __restrict__ Acc,
-                                                  double* __restrict__ Unp1, double dt, double* __restrict__ Rout) {
+                                                  double* __restrict__ Unp1, double dt, double* __restrict__ Rout,
+                                                  float* __restrict__ Aout) {
   int l;
   const long long r = my_row<LPR>(&l);
   double g0 = 0, g1 = 0, g2 = 0;
@@ -408,6 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
     if (Rout) { Rout[i] = R; return; }   // consistent mass: the caller solves for Acc
     const double a = R * MdInv[r];
     Acc[i] = a;
+    if (Aout) Aout[4 * (size_t)r + l] = (float)a;   // the FP32 shadow step1 reads
     Unp1[i] = UnpHalf[i] + dt * a;
   }
 }
@@ -415,14 +468,17 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
 static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
   // 64^3: 8x2 0.253 ms, 8x4 0.266, 4x2 0.290, 4x8 0.298, 4x4 0.307, 16x2 0.312, 8x8 0.319, 16x4 0.343
   constexpr int LPR = 8;
+  // Acc's FP32 shadow, kept where step1 reads it (lumped mass, FP32 matrices)
+  float* aout = (ctx->Gf && ctx->rec_mem && !Rout) ? ctx->Arec : nullptr;
   if (ctx->Gf)
     hipLaunchKernelGGL((k_step3<LPR, 2, float>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                        ctx->Grow, ctx->Gcol, ctx->Gf, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
-                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout);
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout);
   else
     hipLaunchKernelGGL((k_step3<LPR, 2, double>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                        ctx->Grow, ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
-                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout);
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout);
+  ctx->arec_ok = aout != nullptr;
 }
 
 int launch_step3(fsx_ctx* ctx, int iter) {
@@ -1123,6 +1179,15 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP3) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
   const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG;
   const double vb = ctx->Kf ? 4.0 : 8.0;   // bytes per stored matrix value
+  if (kernel == FSX_KERNEL_STEP1 && ctx->Kf && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT) {
+    // the FP32 shadows of UnpHalf_prev and Acc_prev, as the inner iterations have them
+    hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->UnpHalf_prev,
+                       reinterpret_cast<float4*>(ctx->Urec_prev));
+    hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Acc_prev,
+                       reinterpret_cast<float4*>(ctx->Arec_prev));
+    FSX_TRY(launch_check(ctx, "k_rec"));
+    ctx->urec_prev_ok = ctx->arec_prev_ok = true;
+  }
   // K's columns: a pattern start per row plus the dictionary, or a column per entry
   const double kcols = ctx->Mrowpat ? 4.0 * NN + 4.0 * (double)ctx->Mpatlen : 4.0 * nK;
   auto launch = [&]() {
@@ -1135,20 +1200,26 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   };
   double bytes = 0;
   switch (kernel) {
-    case FSX_KERNEL_STEP1:   // K (col + value), G Pn (kept: 3 doubles per row), Uprev, R1 r/w, Un, MdInv, bc,
-                             // UnpHalf, Acc_prev, KAcc: the form of the second and later inner iterations
-      bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
-              24.0 * NN + 24.0 * NN + 24.0 * NN;
+    case FSX_KERNEL_STEP1: {  // K (col + value), G Pn (kept: 3 doubles per row), Uprev, R1 r/w, Un, MdInv, bc,
+                              // UnpHalf, Acc_prev, KAcc: the form of the second and later inner iterations;
+                              // with the FP32 shadows Uprev and Acc_prev are 16 bytes per node and
+                              // UnpHalf's shadow is written (12 bytes)
+      const bool wrec = ctx->Kf && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT;
+      const bool rrec = wrec && ctx->urec_prev_ok && ctx->arec_prev_ok;
+      const double vin = rrec ? 16.0 : 24.0;
+      bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + vin * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
+              24.0 * NN + vin * NN + 24.0 * NN + (wrec ? 12.0 * NN : 0.0);
       break;
+    }
     case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
       bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2
       bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
       break;
-    default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc, Unp1
+    default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc (+ its FP32 shadow), Unp1
       bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +
-              24.0 * NN;
+              24.0 * NN + (ctx->Gf && ctx->rec_mem ? 12.0 * NN : 0.0);
       break;
   }
   FSX_TRY(settle_kacc(ctx));

@@ -36,6 +36,11 @@
 //     colour order but no race is possible even where the colouring is
 //     invalid (the i=0 reset quirk);
 //   * SpMVs use the full symmetric Z instead of MKL's upper-triangle symv.
+//
+// Options with no reference counterpart, off by default, that model the
+// device's other forms for the tests: consistent-mass updates (orc_set_mass),
+// the warm-started PCG (orc_set_pcg_x0, the reference GPU path's x0) and
+// FP32-rounded K operands (orc_set_k_operands).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -70,6 +75,10 @@ struct Case {
   double pcg_rtol = 1e-8;
   int pcg_maxit = 1000;
   bool pcg_warm = false;          // x0 = previous Pdot (the GPU path, CU:425-442) instead of 0
+  // K's operands UnpHalf_prev and Acc_prev rounded to FP32 before the K
+  // products of step1 and K*Acc (lumped mass only): a model of the device's
+  // FP32 operand option for tests, no reference counterpart
+  bool k_operands_f32 = false;
   // velocity mass: 0 lumped (the reference, BCH:3262-3295), 1 consistent M
   // solved by Jacobi-PCG for the three components (no reference counterpart)
   int mass_mode = 0;
@@ -809,13 +818,22 @@ void mass_solve(Case& c, const double* R, double* a) {
   }
 }
 
+// the operand of a K product: v itself, or v rounded to FP32 (k_operands_f32)
+std::vector<double> k_operand(const Case& c, const std::vector<double>& v) {
+  if (!c.k_operands_f32 || c.mass_mode != 0) return v;
+  std::vector<double> r(v.size());
+  for (size_t i = 0; i < v.size(); i++) r[i] = (double)(float)v[i];
+  return r;
+}
+
 // BCH:3692-3801
 void step1(Case& c, int iter) {
   (void)iter;
   const int NN = c.NN;
   convection(c, c.Un.data(), c.UnpHalf_prev.data(), c.R1.data());
+  const std::vector<double> uk = k_operand(c, c.UnpHalf_prev);
   for (int d = 0; d < 3; d++) {
-    csrmv(c.Mrow, c.Mcol, c.Kval.data(), c.UnpHalf_prev.data() + (size_t)d*NN, c.R1.data() + (size_t)d*NN, NN, -1.0, 1.0);
+    csrmv(c.Mrow, c.Mcol, c.Kval.data(), uk.data() + (size_t)d*NN, c.R1.data() + (size_t)d*NN, NN, -1.0, 1.0);
   }
   const std::vector<double>* Gs[3] = {&c.G1, &c.G2, &c.G3};
   for (int d = 0; d < 3; d++)
@@ -892,8 +910,9 @@ int timeStep(Case& c) {
     c.Unp1_prev = c.Unp1;
     c.Acc_prev = c.Acc;
     c.Pnp1_prev = c.Pnp1;
+    const std::vector<double> ak = k_operand(c, c.Acc_prev);
     for (int d = 0; d < 3; d++)                                          // BCH:2981-3035
-      csrmv(c.Mrow, c.Mcol, c.Kval.data(), c.Acc_prev.data() + (size_t)d*NN, c.KAcc.data() + (size_t)d*NN, NN, 1.0, 0.0);
+      csrmv(c.Mrow, c.Mcol, c.Kval.data(), ak.data() + (size_t)d*NN, c.KAcc.data() + (size_t)d*NN, NN, 1.0, 0.0);
   }
   c.last_inner_iters = std::min(iter, c.maxIter);
   double oneOverdt = 1.0000000000000000 / c.dt;                         // BCH:3048-3062
@@ -1059,6 +1078,14 @@ int orc_set_pcg_x0(void* h, int policy) {
   Case& c = *(Case*)h;
   if (policy < 0 || policy > 1) { c.err = "orc_set_pcg_x0: bad policy"; return 1; }
   c.pcg_warm = policy == 1;
+  return 0;
+}
+
+// K's operands in FP64 (0) or rounded to FP32 (1); see Case::k_operands_f32
+int orc_set_k_operands(void* h, int f32) {
+  Case& c = *(Case*)h;
+  if (f32 < 0 || f32 > 1) { c.err = "orc_set_k_operands: 0 or 1"; return 1; }
+  c.k_operands_f32 = f32 == 1;
   return 0;
 }
 

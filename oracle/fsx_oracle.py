@@ -58,6 +58,7 @@ def lib():
         L.orc_timings.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.orc_set_pcg_x0.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_k_operands.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_mass_stats.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -163,6 +164,12 @@ class Oracle:
         """Pressure PCG initial guess: 0 zero (the CPU path), 1 the previous
         Pdot (the GPU path's warm start)."""
         self._check(self.L.orc_set_pcg_x0(self.h, policy), "set_pcg_x0")
+
+    def set_k_operands(self, f32):
+        """Round the K products' operands UnpHalf_prev and Acc_prev to FP32
+        (1) or not (0): a model of the device's FP32 operand shadows, lumped
+        mass only."""
+        self._check(self.L.orc_set_k_operands(self.h, int(f32)), "set_k_operands")
 
     def set_mass(self, mode, rtol=1e-10, maxit=1000):
         """Velocity mass: 0 lumped (the reference), 1 consistent M solved by

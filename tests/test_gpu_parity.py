@@ -338,15 +338,17 @@ def test_reference_sized_cases(fsx_lib, oracle_lib, case):
 
 def test_fp32_matrix_copies(fsx_lib, oracle_lib):
     """Matrix precision 32 (the mixed forms' FP32 assembly: the step kernels
-    read K, G, G^T rounded to FP32, with FP64 products, sums and vectors)
-    against the oracle whose K and G values are rounded to FP32 the same way
-    after set-up (Z and the lumped masses stay FP64 in both): fields after
-    3 steps agree to 1e-9; and against the FP64 run, the stated accuracy of
-    the form: velocity within 1e-6, pressure within 1e-5."""
+    read K, G, G^T rounded to FP32, and step1 gathers K's operands
+    UnpHalf_prev and Acc_prev from FP32 shadows, with FP64 products, sums and
+    vectors) against the oracle whose K and G values and K operands are
+    rounded to FP32 the same way (Z and the lumped masses stay FP64 in both):
+    fields after 3 steps agree to 1e-9; and against the FP64 run, the stated
+    accuracy of the form: velocity within 1e-6, pressure within 1e-5."""
     say("matrix precision 32: cavity 8^3, 3 steps")
     s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(8), NGP=27, pcg_rtol=1e-12)
     for name in ("K", "G1", "G2", "G3"):
         o.set_d(name, o.d(name).astype(np.float32).astype(np.float64))
+    o.set_k_operands(1)
     s.run(3)
     u64, p64 = s.download("Un"), s.download("Pn")
     s.reset(mode=0)

@@ -449,3 +449,35 @@ def test_warm_start_pcg(oracle_lib):
     Z = csr_dense(warm.i("Zrow"), warm.i("Zcol"), warm.d("Z"), (n, n))
     r2, pd = warm.d("R2"), warm.d("Pdot")
     assert np.linalg.norm(r2 - Z @ pd) <= 1.01e-12 * np.linalg.norm(r2)
+
+
+def test_k_operands_f32(oracle_lib):
+    """K operands rounded to FP32 (the oracle's model of the device's FP32
+    shadows of UnpHalf_prev and Acc_prev): step1's R1 moves by exactly
+    -K (fl32(v) - v) per component, v = UnpHalf_prev, except at BC rows,
+    which step1 zeroes; with consistent mass the option does nothing."""
+    c = oracle_lib.case_from_mesh(meshgen.cavity(3), NGP=8)
+    NN = c.sizes()["NN"]
+    v = np.random.default_rng(5).standard_normal(3 * NN)
+    c.set_d("UnpHalf_prev", v)
+    c.step1(1)
+    r0 = c.d("R1").copy()
+    c.set_k_operands(1)
+    c.set_d("UnpHalf_prev", v)
+    c.step1(1)
+    r1 = c.d("R1").copy()
+    K = csr_dense(c.i("Mrow"), c.i("Mcol"), c.d("K"), (NN, NN))
+    dv = (v.astype(np.float32).astype(np.float64) - v).reshape(3, NN)
+    want = -(K @ dv.T).T.reshape(-1)
+    for n in c.i("BCnodes"):
+        want[[n, n + NN, n + 2 * NN]] = 0.0
+    assert np.linalg.norm(want) > 0
+    assert np.linalg.norm((r1 - r0) - want) <= 1e-6 * np.linalg.norm(want)
+    c.set_mass(1, 1e-12, 1000)
+    c.set_d("UnpHalf_prev", v)
+    c.step1(1)
+    rm1 = c.d("R1").copy()
+    c.set_k_operands(0)
+    c.set_d("UnpHalf_prev", v)
+    c.step1(1)
+    assert np.array_equal(rm1, c.d("R1"))
