@@ -208,7 +208,9 @@ def main():
     ap.add_argument("--ngp", type=int, default=27)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--maxit", type=int, default=20000)
-    ap.add_argument("--modes", default="fp64,mixed,learned")
+    ap.add_argument("--modes", default="fp64,mixed,learned",
+                    help="solver modes to run, of fp64, mixed, learned and direct (band Cholesky of Z, the MATLAB "
+                         "prototype's pressure solve)")
     ap.add_argument("--train-iters", type=int, default=400)
     ap.add_argument("--train-capture-steps", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-4)
@@ -265,7 +267,8 @@ def main():
                           "copy_GBs": 2 * copy_bytes / (msc * 1e-3) / 1e9}), flush=True)
         solver.close()
         return
-    modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED, "learned": fsx.SOLVER_LEARNED}
+    modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED, "learned": fsx.SOLVER_LEARNED,
+             "direct": fsx.SOLVER_DIRECT}
     res = {}
     train = None
     for name in args.modes.split(","):
@@ -275,8 +278,12 @@ def main():
             log(f"  {train}")
         log(f"mode {name}: {args.warmup} warmup + {args.steps} timed steps")
         res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit,
-                             matrix_bits=64 if name == "fp64" else 32, x0=args.x0)
+                             matrix_bits=64 if name in ("fp64", "direct") else 32, x0=args.x0)
         r = res[name]
+        if name == "direct":   # the factor is formed in the first warm-up step
+            w, fms, fbytes = solver.direct_stats()
+            r["extra"] = {"half_bandwidth": w, "factor_ms": round(fms, 1), "band_GB": round(fbytes / 1e9, 2)}
+            log(f"  direct: {r['extra']}")
         log(f"  {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms, "
             f"max relres {r['max_relres']:.2e}")
     head = next((m for m in ("learned", "mixed") if m in res), list(res)[0])
@@ -332,7 +339,7 @@ def main():
                     for k, v in kern.items()},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 4), "pcg_iters": v["pcg_iters"],
                       "inner_iters": v["inner_iters"], "solve_ms": round(v["solve_ms"], 2),
-                      "max_relres": v["max_relres"]} for k, v in res.items()},
+                      "max_relres": v["max_relres"], **v.get("extra", {})} for k, v in res.items()},
         "spmv_fp64": {"launch_ms": round(ms64, 5), "achieved_GBs": round(bytes64 / (ms64 * 1e-3) / 1e9, 1)},
         "z_row_patterns": npat,
     }

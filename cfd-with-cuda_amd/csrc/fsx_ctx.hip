@@ -278,7 +278,7 @@ int fsx_set_graphs(fsx_ctx* ctx, int enable) {
 
 int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit) {
   if (!ctx) return FSX_EINVAL;
-  if (mode < FSX_SOLVER_FP64_JACOBI || mode > FSX_SOLVER_LEARNED) return fail(ctx, FSX_EINVAL, "unknown solver mode");
+  if (mode < FSX_SOLVER_FP64_JACOBI || mode > FSX_SOLVER_DIRECT) return fail(ctx, FSX_EINVAL, "unknown solver mode");
   if (mode == FSX_SOLVER_LEARNED && !ctx->unet) return fail(ctx, FSX_ESTATE, "learned mode needs fsx_unet_setup first");
   if (!(rtol >= 0) || maxit < 1) return fail(ctx, FSX_EINVAL, "rtol >= 0 and maxit >= 1 required");
   ctx->solver_mode = mode; ctx->rtol = rtol; ctx->maxit = maxit;
@@ -696,6 +696,7 @@ int fsx_build_Z(fsx_ctx* ctx) {
   FSX_TRY(launch_build_Z(ctx));
   FSX_TRY(build_z_forms(ctx));
   ctx->z_built = true;
+  ctx->band_ready = false;   // a direct solve factors the new Z
   return FSX_OK;
 }
 
@@ -934,6 +935,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
   ctx->gpn_valid = false;
   drop_records(ctx);   // the FP32 shadows may no longer match the vectors
+  if (id == FSX_VAL_Z) ctx->band_ready = false;
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   if (kind == 3) {
     const size_t NN = ctx->NN;

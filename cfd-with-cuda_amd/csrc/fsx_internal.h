@@ -152,6 +152,20 @@ struct fsx_ctx {
   // prev vectors rounded; any other writer of those vectors clears the flags.
   float *Urec = nullptr, *Urec_prev = nullptr, *Arec = nullptr, *Arec_prev = nullptr;
   float* rec_mem = nullptr;
+  // direct pressure solve (FSX_SOLVER_DIRECT): Cholesky factor of Z as a
+  // lower band in band order, entry (i, j) at i*band_W + (j - i + band_w)
+  double* band = nullptr;
+  double* band_up = nullptr;     // its transpose as an upper band, entry (j, i) at j*band_W + (i - j)
+  double* band_inv = nullptr;    // inverses of the 1024x1024 diagonal blocks, then their transposes
+  double* band_vec = nullptr;    // 3 band_n: the right-hand side, y and x, in band order
+  double* band_part = nullptr;   // partial sums of the triangular-solve steps
+  int* band_perm = nullptr;      // band order -> pressure id
+  int* band_iperm = nullptr;     // pressure id -> band order
+  int* band_flag = nullptr;      // set at a non-positive pivot
+  long long band_W = 0;          // row stride
+  int band_w = -1, band_n = 0;   // half-bandwidth, order padded to a multiple of 256
+  bool band_ready = false;       // the factor is of the current Z
+  double band_factor_ms = 0.0;
   bool urec_ok = false, urec_prev_ok = false, arec_ok = false, arec_prev_ok = false;
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
@@ -273,6 +287,8 @@ int launch_build_Z(fsx_ctx* ctx);
 int launch_convection(fsx_ctx* ctx);
 int launch_step1(fsx_ctx* ctx, int iter);
 int records_begin_step(fsx_ctx* ctx);
+int direct_factor(fsx_ctx* ctx);
+int direct_apply(fsx_ctx* ctx, const double* b, double* x);
 inline void drop_records(fsx_ctx* ctx) {
   ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = false;
 }

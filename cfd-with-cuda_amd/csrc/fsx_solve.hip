@@ -1041,14 +1041,6 @@ int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, do
 
 int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres, int truncate_at);
 
-// the selected solver from x = 0
-static int solve_cold(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
-  if (ctx->solver_mode == FSX_SOLVER_LEARNED && ctx->unet) return solve_learned(ctx, b, x, iters, relres, 0);
-  if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)
-    return solve_mixed(ctx, b, x, iters, relres);
-  return solve_fp64(ctx, b, x, iters, relres);
-}
-
 // |b|^2 (slot 0) and |r|^2 (slot 1)
 __global__ __launch_bounds__(kBlock) void k_sumsq2(int n, const double* __restrict__ b, const double* __restrict__ r,
                                                    double* __restrict__ partials) {
@@ -1058,6 +1050,37 @@ __global__ __launch_bounds__(kBlock) void k_sumsq2(int n, const double* __restri
     v[1] += r[i] * r[i];
   }
   write_partials<2>(v, partials);
+}
+
+// Cholesky of Z (the MATLAB prototype's pressure step: factor once,
+// ML:1876-1885, then two triangular solves, ML:2021-2030), then the FP64
+// residual |b - Z x| / |b|, which must meet rtol like the iterative solvers'
+static int solve_direct(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  const int n = ctx->NNp;
+  FSX_TRY(direct_factor(ctx));
+  FSX_TRY(direct_apply(ctx, b, x));
+  launch_zresidual(ctx, x, b, ctx->w0r);
+  hipLaunchKernelGGL(k_sumsq2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->w0r, ctx->partials);
+  FSX_TRY(launch_check(ctx, "k_sumsq2"));
+  double h[2 * kRedGrid];
+  FSX_HIP(hipMemcpyAsync(h, ctx->partials, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  double bb = 0.0, rr = 0.0;
+  for (int i = 0; i < kRedGrid; i++) { bb += h[i]; rr += h[kRedGrid + i]; }
+  *iters = 0;
+  *relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+  if (!(rr == rr)) return fail(ctx, FSX_ENUMERIC, "direct solve residual is NaN/Inf");
+  if (*relres > ctx->rtol) return fail(ctx, FSX_ENOCONV, "direct solve: residual above rtol");
+  return FSX_OK;
+}
+
+// the selected solver from x = 0
+static int solve_cold(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  if (ctx->solver_mode == FSX_SOLVER_DIRECT) return solve_direct(ctx, b, x, iters, relres);
+  if (ctx->solver_mode == FSX_SOLVER_LEARNED && ctx->unet) return solve_learned(ctx, b, x, iters, relres, 0);
+  if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)
+    return solve_mixed(ctx, b, x, iters, relres);
+  return solve_fp64(ctx, b, x, iters, relres);
 }
 
 // Warm start from the solution already in x (the reference's GPU path hands
@@ -1096,7 +1119,8 @@ static int solve_warm(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   int rc;
   timing_begin(ctx);
-  if (ctx->x0_policy == FSX_X0_PREVIOUS) rc = solve_warm(ctx, b, x, iters, relres);
+  // (a direct solve does not use an initial guess)
+  if (ctx->x0_policy == FSX_X0_PREVIOUS && ctx->solver_mode != FSX_SOLVER_DIRECT) rc = solve_warm(ctx, b, x, iters, relres);
   else rc = solve_cold(ctx, b, x, iters, relres);
   if (rc == FSX_OK || rc == FSX_ENOCONV) timing_end(ctx, *iters);
   ctx->last_relres = *relres;

@@ -21,7 +21,7 @@ _vp = ctypes.c_void_p
 
 FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTATE, FSX_ENOCONV, FSX_ENOMEM, FSX_ENUMERIC = range(7)
 CSR_M, CSR_G, CSR_Z = 0, 1, 2
-SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
+SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_DIRECT = 0, 1, 2, 3
 KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3 = 0, 1, 2, 3
 MASS_LUMPED, MASS_CONSISTENT = 0, 1
 X0_ZERO, X0_PREVIOUS = 0, 1
@@ -36,7 +36,7 @@ DEVICE_SYMBOLS = (
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
     "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision",
-    "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
@@ -114,6 +114,8 @@ def lib():
         "fsx_set_matrix_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_convection_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_solver_x0": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_direct_stats": ([_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "fsx_mass_stats": ([_vp, _I, _D, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "fsx_bench_copy": ([_vp, ctypes.c_size_t, ctypes.c_int, _D], ctypes.c_int),
         "fsx_z_patterns": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
@@ -419,6 +421,16 @@ class Solver:
         """X0_ZERO (the CPU path) or X0_PREVIOUS (warm start from the previous
         Pdot, the reference's GPU path)."""
         self.check(self.L.fsx_set_solver_x0(self.ctx, policy), "set_solver_x0")
+
+    def direct_stats(self):
+        """(half-bandwidth, factorization ms, band bytes) of the direct
+        solver's Cholesky factor of Z; half-bandwidth -1 before the first
+        direct solve."""
+        w = ctypes.c_int()
+        ms = ctypes.c_double()
+        by = ctypes.c_double()
+        self.check(self.L.fsx_direct_stats(self.ctx, ctypes.byref(w), ctypes.byref(ms), ctypes.byref(by)), "direct_stats")
+        return w.value, ms.value, by.value
 
     def set_convection_precision(self, bits):
         """64 (the reference) or 32: the convection residual's element

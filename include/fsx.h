@@ -77,7 +77,10 @@ enum fsx_vec_id {
 enum fsx_solver_mode {
   FSX_SOLVER_FP64_JACOBI = 0,  /* FP64 Jacobi-PCG, x0 = 0, ||r|| <= rtol ||b|| */
   FSX_SOLVER_MIXED_IR = 1,     /* FP32 inner Jacobi-PCG, FP64 residual + refinement, same stop test */
-  FSX_SOLVER_LEARNED = 2       /* flexible PCG with the learned preconditioner, Jacobi-PCG fallback */
+  FSX_SOLVER_LEARNED = 2,      /* flexible PCG with the learned preconditioner, Jacobi-PCG fallback */
+  FSX_SOLVER_DIRECT = 3        /* Cholesky of Z factored once, two triangular solves per step (the MATLAB
+                                  prototype's pressure step, ML:1876-1885, 2021-2030); rtol bounds the
+                                  FP64 residual it must meet */
 };
 
 /* Initial guess of the pressure solve for fsx_set_solver_x0 */
@@ -156,6 +159,13 @@ int fsx_set_convection_precision(fsx_ctx* ctx, int bits);
  * solve's |b - Z x| <= rtol |b|; an x0 already within it takes no
  * iteration. */
 int fsx_set_solver_x0(fsx_ctx* ctx, int policy);
+/* FSX_SOLVER_DIRECT's factor: half-bandwidth of Z in the band order used
+ * (-1 before the first direct solve), wall time of the factorization (ms)
+ * and the bytes of the band. The factor is formed at the first direct solve
+ * after Z changes (ML:1876-1885 factors once before the time loop); Z's band
+ * must fit in 64 GiB (FSX_ENOMEM otherwise). No counterpart in the
+ * reference's C++/CUDA code. */
+int fsx_direct_stats(fsx_ctx* ctx, int* half_bandwidth, double* factor_ms, double* band_bytes);
 /* enqueue solver iterations as replayed HIP graphs (1) or as individual
  * launches (0, the default: on the 64^3 cavity both run within 5% of each
  * other); results are identical. No reference counterpart (the CUSP and MKL

@@ -39,8 +39,10 @@
 //
 // Options with no reference counterpart, off by default, that model the
 // device's other forms for the tests: consistent-mass updates (orc_set_mass),
-// the warm-started PCG (orc_set_pcg_x0, the reference GPU path's x0) and
-// FP32-rounded K operands (orc_set_k_operands).
+// the warm-started PCG (orc_set_pcg_x0, the reference GPU path's x0),
+// FP32-rounded K operands (orc_set_k_operands) and the direct Cholesky
+// pressure solve of the MATLAB prototype (orc_set_direct, ML:1876-1885,
+// 2021-2030).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -79,6 +81,11 @@ struct Case {
   // products of step1 and K*Acc (lumped mass only): a model of the device's
   // FP32 operand option for tests, no reference counterpart
   bool k_operands_f32 = false;
+  // pressure solve by Cholesky of Z instead of PCG (the MATLAB prototype's
+  // step, ML:1876-1885, 2021-2030); the factor is kept until Z changes
+  bool pcg_direct = false;
+  int chol_w = -1;                // half-bandwidth of the kept factor, -1: none
+  std::vector<double> chol_L;     // lower band, row i at i*(w+1), column j at j - i + w
   // velocity mass: 0 lumped (the reference, BCH:3262-3295), 1 consistent M
   // solved by Jacobi-PCG for the three components (no reference counterpart)
   int mass_mode = 0;
@@ -848,6 +855,73 @@ void step1(Case& c, int iter) {
   for (size_t i = 0; i < (size_t)3*NN; i++) c.UnpHalf[i] = c.Un[i] + c.dt * c.R1[i] * c.MdInv[i];
 }
 
+// Cholesky of Z in its own (natural) order, Z = L L^T with L stored as a
+// lower band of half-bandwidth w = max |i - j| over Z's pattern (a band
+// factor keeps no fill outside the band). The prototype factors Z with an
+// AMD ordering (ML:1876-1885); the order changes the fill, not the solution.
+// Returns false when a pivot is not positive.
+bool chol_factor(Case& c) {
+  const int n = c.NNp;
+  int w = 0;
+  for (int i = 0; i < n; i++)
+    for (int k = c.Zrow[i]; k < c.Zrow[i+1]; k++) w = std::max(w, std::abs(i - c.Zcol[k]));
+  const size_t W = (size_t)w + 1;
+  std::vector<double> L((size_t)n * W, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int k = c.Zrow[i]; k < c.Zrow[i+1]; k++)
+      if (c.Zcol[k] <= i) L[i * W + (c.Zcol[k] - i + w)] = c.Zval[k];
+  for (int j = 0; j < n; j++) {                          // left-looking, column j
+    double* Lj = L.data() + j * W;
+    double s = Lj[w];
+    for (int k = std::max(0, j - w); k < j; k++) s -= Lj[k - j + w] * Lj[k - j + w];
+    if (!(s > 0.0)) { c.chol_w = -1; return false; }
+    const double d = std::sqrt(s);
+    Lj[w] = d;
+    const int iend = std::min(n - 1, j + w);
+#pragma omp parallel for schedule(static)
+    for (int i = j + 1; i <= iend; i++) {
+      double* Li = L.data() + i * W;
+      double t = Li[j - i + w];
+      for (int k = std::max(0, i - w); k < j; k++) t -= Li[k - i + w] * Lj[k - j + w];
+      Li[j - i + w] = t / d;
+    }
+  }
+  c.chol_L.swap(L);
+  c.chol_w = w;
+  return true;
+}
+
+// x = Z^-1 b with the kept factor: L y = b, then L^T x = y (ML:2021-2030)
+void chol_solve(const Case& c, const double* b, double* x) {
+  const int n = c.NNp, w = c.chol_w;
+  const size_t W = (size_t)w + 1;
+  const double* L = c.chol_L.data();
+  std::vector<double> y(n);
+  for (int i = 0; i < n; i++) {
+    double t = b[i];
+    for (int k = std::max(0, i - w); k < i; k++) t -= L[i * W + (k - i + w)] * y[k];
+    y[i] = t / L[i * W + w];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double t = y[i];
+    for (int k = i + 1; k <= std::min(n - 1, i + w); k++) t -= L[k * W + (i - k + w)] * x[k];
+    x[i] = t / L[i * W + w];
+  }
+}
+
+// the direct pressure solve: factor on first use, solve, |b - Z x| / |b|
+int direct_solve(Case& c, const double* b, double* x, double* relres) {
+  if (c.chol_w < 0 && !chol_factor(c)) { c.err = "Cholesky of Z met a non-positive pivot"; return 1; }
+  chol_solve(c, b, x);
+  const int n = c.NNp;
+  std::vector<double> r(n);
+  csrmv(c.Zrow, c.Zcol, c.Zval.data(), x, r.data(), n, 1.0, 0.0);
+  double bb = 0.0, rr = 0.0;
+  for (int i = 0; i < n; i++) { bb += b[i] * b[i]; rr += (b[i] - r[i]) * (b[i] - r[i]); }
+  *relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+  return 0;
+}
+
 // BCH:3808-3905
 void step2(Case& c, int iter) {
   const int NN = c.NN;
@@ -861,8 +935,14 @@ void step2(Case& c, int iter) {
     gtmv(c, Gs[d]->data(), dummy.data() + (size_t)d*NN, c.R2.data(), 1.0, d == 0 ? 0.0 : 1.0);
   if (c.zeroP > 0) c.R2[c.zeroP] = 0.0;                                  // BCH:4180-4184
   auto t0 = std::chrono::steady_clock::now();
-  c.last_pcg_iters = pcg(c.Zrow, c.Zcol, c.Zval, c.NNp, c.R2.data(), c.Pdot.data(), c.pcg_rtol, c.pcg_maxit, &c.last_pcg_relres,
-                         c.pcg_warm);
+  if (c.pcg_direct) {
+    c.last_pcg_iters = 0;
+    if (direct_solve(c, c.R2.data(), c.Pdot.data(), &c.last_pcg_relres) != 0)
+      for (int i = 0; i < c.NNp; i++) c.Pdot[i] = NAN;
+  } else {
+    c.last_pcg_iters = pcg(c.Zrow, c.Zcol, c.Zval, c.NNp, c.R2.data(), c.Pdot.data(), c.pcg_rtol, c.pcg_maxit,
+                           &c.last_pcg_relres, c.pcg_warm);
+  }
   c.t_pcg_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   c.total_pcg_iters += c.last_pcg_iters;
   c.n_solves++;
@@ -1044,7 +1124,9 @@ int orc_get_d(void* h, const char* name, double* dst) {
 }
 int orc_set_d(void* h, const char* name, const double* src) {
   auto* v = dvec(*(Case*)h, name); if (!v) return 1;
-  std::memcpy(v->data(), src, v->size() * sizeof(double)); return 0;
+  std::memcpy(v->data(), src, v->size() * sizeof(double));
+  if (v == &((Case*)h)->Zval) ((Case*)h)->chol_w = -1;   // the kept factor is of the old Z
+  return 0;
 }
 int orc_get_i(void* h, const char* name, int* dst) {
   auto* v = ivec(*(Case*)h, name); if (!v) return 1;
@@ -1080,6 +1162,20 @@ int orc_set_pcg_x0(void* h, int policy) {
   c.pcg_warm = policy == 1;
   return 0;
 }
+
+// pressure solve: 0 Jacobi-PCG (the reference), 1 Cholesky of Z (Case::pcg_direct)
+int orc_set_direct(void* h, int on) {
+  Case& c = *(Case*)h;
+  if (on < 0 || on > 1) { c.err = "orc_set_direct: 0 or 1"; return 1; }
+  c.pcg_direct = on == 1;
+  return 0;
+}
+// x = Z^-1 b by the kept (or a new) Cholesky factor; relres = |b - Z x| / |b|
+int orc_direct_solve(void* h, const double* b, double* x, double* relres) {
+  return direct_solve(*(Case*)h, b, x, relres);
+}
+// half-bandwidth of the kept factor (-1 when none)
+int orc_chol_bandwidth(void* h) { return ((Case*)h)->chol_w; }
 
 // K's operands in FP64 (0) or rounded to FP32 (1); see Case::k_operands_f32
 int orc_set_k_operands(void* h, int f32) {

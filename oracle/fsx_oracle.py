@@ -59,6 +59,9 @@ def lib():
         L.orc_set_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.orc_set_pcg_x0.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_k_operands.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_direct.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_direct_solve.argtypes = [ctypes.c_void_p, _D, _D, _D]
+        L.orc_chol_bandwidth.argtypes = [ctypes.c_void_p]
         L.orc_mass_stats.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -164,6 +167,22 @@ class Oracle:
         """Pressure PCG initial guess: 0 zero (the CPU path), 1 the previous
         Pdot (the GPU path's warm start)."""
         self._check(self.L.orc_set_pcg_x0(self.h, policy), "set_pcg_x0")
+
+    def set_direct(self, on):
+        """Pressure solve by Cholesky of Z (1, the MATLAB prototype's step)
+        or Jacobi-PCG (0, the reference)."""
+        self._check(self.L.orc_set_direct(self.h, int(on)), "set_direct")
+
+    def direct_solve(self, b):
+        """x = Z^-1 b by the Cholesky factor of Z; returns (x, |b - Zx|/|b|)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        rel = np.zeros(1)
+        self._check(self.L.orc_direct_solve(self.h, _dp(b), _dp(x), _dp(rel)), "direct_solve")
+        return x, float(rel[0])
+
+    def chol_bandwidth(self):
+        return self.L.orc_chol_bandwidth(self.h)
 
     def set_k_operands(self, f32):
         """Round the K products' operands UnpHalf_prev and Acc_prev to FP32

@@ -481,3 +481,28 @@ def test_k_operands_f32(oracle_lib):
     c.set_d("UnpHalf_prev", v)
     c.step1(1)
     assert np.array_equal(rm1, c.d("R1"))
+
+
+def test_direct_pressure_solve(oracle_lib):
+    """The oracle's Cholesky pressure solve (the MATLAB prototype's step,
+    ML:1876-1885, 2021-2030) against a dense LAPACK solve of its own Z on a
+    4^3 cavity; its half-bandwidth is the largest |i - j| of Z's pattern; a
+    time step with it agrees with one whose PCG runs to rtol 1e-13."""
+    c = oracle_lib.case_from_mesh(meshgen.cavity(4), NGP=8)
+    n = c.sizes()["NNp"]
+    Z = csr_dense(c.i("Zrow"), c.i("Zcol"), c.d("Z"), (n, n))
+    b = np.random.default_rng(11).standard_normal(n)
+    x, rel = c.direct_solve(b)
+    want = np.linalg.solve(Z, b)
+    assert np.linalg.norm(x - want) <= 1e-11 * np.linalg.norm(want)
+    assert rel < 1e-13
+    rows = np.repeat(np.arange(n), np.diff(c.i("Zrow")))
+    assert c.chol_bandwidth() == np.abs(rows - c.i("Zcol")).max()
+    a = oracle_lib.case_from_mesh(meshgen.cavity(4), NGP=8, pcg_rtol=1e-13, pcg_maxit=5000)
+    d = oracle_lib.case_from_mesh(meshgen.cavity(4), NGP=8, pcg_rtol=1e-13, pcg_maxit=5000)
+    d.set_direct(1)
+    a.run(2)
+    sd = d.run(2)
+    assert (sd[:, 1] == 0).all()
+    for name in ("Un", "Pn"):
+        assert np.linalg.norm(d.d(name) - a.d(name)) <= 1e-10 * np.linalg.norm(a.d(name)), name
