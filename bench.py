@@ -16,9 +16,11 @@ trajectory is run with the plain FP64 Jacobi-PCG, with the mixed-precision
 form (FP32 assembly of K, G, G^T and the convection, FP32 inner PCG, FP64
 refinement residual) and with learned+mixed
 (flexible PCG in FP32 preconditioned by Jacobi plus the trained multilevel
-network, FP64 reliable updates and stop test, Jacobi-PCG fallback); `value`
-is the headline (learned+mixed) mode, the other rates, the pressure-solve
-speed-up over FP64 and the field errors against the FP64 run sit beside it.
+network, FP64 reliable updates and stop test, Jacobi-PCG fallback), and with
+the direct solve (band Cholesky of Z factored once in the first warm-up
+step, the MATLAB prototype's pressure step); `value` is the headline
+(learned+mixed) mode, the other rates, the pressure-solve speed-up over FP64
+and the field errors against the FP64 run sit beside it.
 The network is set up, fed with the solver's own pressure solves and
 trained before the timed region (train_s in the output).
 
@@ -38,9 +40,11 @@ METRIC = "timesteps/s, 64^3 hex27 cavity: learned+mixed vs plain FP64 PCG at 1e-
 
 def step_kernels(vt):
     """rocprofv3 names of the step's row kernels for matrix values of type vt
-    ("double", or "float" in the mixed forms): step1 with K*Acc_prev fused
-    and G*Pn kept (inner iterations >= 2), step2's right-hand side, step3."""
-    return (f"k_step1<8, 2, 2, true, true, 2, {vt}>", f"k_step2_rhs<16, 8, {vt}>", f"k_step3<8, 2, {vt}>")
+    ("double", or "float" in the mixed forms, whose step1 also gathers K's
+    operands from their FP32 shadows): step1 with K*Acc_prev fused and G*Pn
+    kept (inner iterations >= 2), step2's right-hand side, step3."""
+    rec = "true" if vt == "float" else "false"
+    return (f"k_step1<8, 2, 2, true, true, 2, {vt}, {rec}>", f"k_step2_rhs<16, 8, {vt}>", f"k_step3<8, 2, {vt}>")
 
 
 def zspmv_name(prec, npat, ncls):
@@ -208,7 +212,7 @@ def main():
     ap.add_argument("--ngp", type=int, default=27)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--maxit", type=int, default=20000)
-    ap.add_argument("--modes", default="fp64,mixed,learned",
+    ap.add_argument("--modes", default="fp64,mixed,learned,direct",
                     help="solver modes to run, of fp64, mixed, learned and direct (band Cholesky of Z, the MATLAB "
                          "prototype's pressure solve)")
     ap.add_argument("--train-iters", type=int, default=400)

@@ -13,8 +13,9 @@
 // band in place. The 1024x1024 diagonal blocks of L are then inverted
 // (k_band_inv) and L is also kept transposed as an upper band
 // (k_band_transpose), so that both triangular solves read rows. Each solve
-// runs in 1024-row steps: the off-diagonal band part as row sums in four
-// slices a row (k_band_rowpart, 1024 wavefronts), then the block's inverse
+// runs in 1024-row steps: the off-diagonal band part as row sums, a
+// workgroup a row with its four wavefronts on four slices of the columns
+// (k_band_rowpart), then the block's inverse, a wavefront a row
 // (k_band_apply_inv). Every sum has a fixed order, so the solve is
 // deterministic.
 #include <algorithm>
@@ -209,18 +210,20 @@ __global__ void k_band_scatter(int n, const int* __restrict__ perm, const double
   if (k < n) x[perm[k]] = y[k];
 }
 
-// One triangular-solve step's off-diagonal part, for the kSB rows R0 + r:
-// part[q][r] = sum over the q-th of kQ slices of the row's columns of
-// M(R0 + r, j) v[j], a wavefront per (row, slice), 8 loads in flight a lane.
+// One triangular-solve step's right-hand side for the kSB rows R0 + r:
+// t[r] = rhs[R0 + r] - sum_j M(R0 + r, j) v[j] over the row's off-diagonal
+// band columns, a workgroup per row, its kQ = 4 wavefronts on four slices of
+// the columns (8 loads in flight a lane), summed in LDS in slice order.
 // Forward (UPPER false): M = L, columns [max(0, i - w), R0), L(i, j) at
 // i*W + (j - i + w). Backward (UPPER true): M = U, columns
 // [R0 + kSB, min(np, i + w + 1)), U(i, j) at i*W + (j - i).
 template <bool UPPER>
-__global__ __launch_bounds__(256) void k_band_rowpart(const double* __restrict__ M, long long W, int w, int np, int R0,
-                                                      const double* __restrict__ v, double* __restrict__ part) {
-  const int lane = threadIdx.x & 63;
-  const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int r = pair / kQ, q = pair % kQ;
+__global__ __launch_bounds__(64 * kQ) void k_band_rowpart(const double* __restrict__ M, long long W, int w, int np,
+                                                          int R0, const double* __restrict__ v,
+                                                          const double* __restrict__ rhs, double* __restrict__ t) {
+  __shared__ double ws[kQ];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int r = blockIdx.x;
   const long long i = (long long)R0 + r;
   long long lo, hi;
   if (UPPER) { lo = (long long)R0 + kSB; hi = i + w + 1 < np ? i + w + 1 : np; }
@@ -245,43 +248,44 @@ __global__ __launch_bounds__(256) void k_band_rowpart(const double* __restrict__
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane == 0) part[q * kSB + r] = acc;
+  if (lane == 0) ws[q] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double p = rhs[i];
+#pragma unroll
+    for (int k = 0; k < kQ; k++) p -= ws[k];
+    t[r] = p;
+  }
 }
 
-// The step's diagonal block: t[s] = rhs[R0 + s] - sum_q part[q][s], then
-// out[R0 + r] = sum_s X(r, s) t[s], X the block's inverse (forward: Linv,
-// nonzero for s <= r) or its transpose (backward: nonzero for s >= r), rows
-// contiguous; 16 rows a workgroup, 16 lanes a row
-template <bool UPPER>
-__global__ __launch_bounds__(256) void k_band_apply_inv(const double* __restrict__ part, const double* __restrict__ X,
-                                                        const double* __restrict__ rhs, int R0, double* __restrict__ out) {
-  __shared__ double t[kSB];
-  for (int s = threadIdx.x; s < kSB; s += 256) {
-    double p = rhs[R0 + s];
-#pragma unroll
-    for (int q = 0; q < kQ; q++) p -= part[q * kSB + s];
-    t[s] = p;
-  }
+// The step's diagonal block: out[R0 + r] = sum_s X(r, s) t[s], X the block's
+// inverse (forward: Linv, nonzero for s <= r) or its transpose (backward:
+// nonzero for s >= r), rows contiguous; a wavefront per row
+__global__ __launch_bounds__(256) void k_band_apply_inv(const double* __restrict__ t, const double* __restrict__ X,
+                                                        int upper, int R0, double* __restrict__ out) {
+  __shared__ double ts[kSB];
+  for (int s = threadIdx.x; s < kSB; s += 256) ts[s] = t[s];
   __syncthreads();
-  const int r = blockIdx.x * 16 + threadIdx.x / 16, l = threadIdx.x % 16;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   const double* xr = X + (long long)r * kSB;
-  const int s_lo = UPPER ? r : 0, s_hi = UPPER ? kSB : r + 1;
+  const int s_lo = upper ? r : 0, s_hi = upper ? kSB : r + 1;
   double acc = 0.0;
-  for (int s0 = s_lo; s0 < s_hi; s0 += 8 * 16) {
-    double m[8], y[8];
+  for (int s0 = s_lo; s0 < s_hi; s0 += 4 * 64) {
+    double m[4], y[4];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int s = s0 + u * 16 + l;
+    for (int u = 0; u < 4; u++) {
+      const int s = s0 + u * 64 + lane;
       const bool ok = s < s_hi;
       m[u] = ok ? xr[s] : 0.0;
-      y[u] = ok ? t[s] : 0.0;
+      y[u] = ok ? ts[s] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 8; u++) acc += m[u] * y[u];
+    for (int u = 0; u < 4; u++) acc += m[u] * y[u];
   }
 #pragma unroll
-  for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (l == 0) out[R0 + r] = acc;
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) out[R0 + r] = acc;
 }
 
 // half-bandwidth of Z's pattern under the order iperm (pressure id -> band row)
@@ -426,19 +430,16 @@ int direct_apply(fsx_ctx* ctx, const double* b, double* x) {
   double* xx = bb + 2 * (size_t)np;
   const double* inv_rm = ctx->band_inv;
   const double* inv_cm = ctx->band_inv + (size_t)np * kSB;
-  const dim3 gpart(kSB * kQ / 4), gapply(kSB / 16);
+  double* t = ctx->band_part;
+  const dim3 gpart(kSB), gapply(kSB / 4);
   hipLaunchKernelGGL(k_band_gather, dim3((np + 255) / 256), dim3(256), 0, ctx->stream, n, np, ctx->band_perm, b, bb);
   for (int R0 = 0; R0 < np; R0 += kSB) {
-    hipLaunchKernelGGL(k_band_rowpart<false>, gpart, dim3(256), 0, ctx->stream, ctx->band, W, w, np, R0, y,
-                       ctx->band_part);
-    hipLaunchKernelGGL(k_band_apply_inv<false>, gapply, dim3(256), 0, ctx->stream, ctx->band_part,
-                       inv_rm + (size_t)R0 * kSB, bb, R0, y);
+    hipLaunchKernelGGL(k_band_rowpart<false>, gpart, dim3(64 * kQ), 0, ctx->stream, ctx->band, W, w, np, R0, y, bb, t);
+    hipLaunchKernelGGL(k_band_apply_inv, gapply, dim3(256), 0, ctx->stream, t, inv_rm + (size_t)R0 * kSB, 0, R0, y);
   }
   for (int R0 = np - kSB; R0 >= 0; R0 -= kSB) {
-    hipLaunchKernelGGL(k_band_rowpart<true>, gpart, dim3(256), 0, ctx->stream, ctx->band_up, W, w, np, R0, xx,
-                       ctx->band_part);
-    hipLaunchKernelGGL(k_band_apply_inv<true>, gapply, dim3(256), 0, ctx->stream, ctx->band_part,
-                       inv_cm + (size_t)R0 * kSB, y, R0, xx);
+    hipLaunchKernelGGL(k_band_rowpart<true>, gpart, dim3(64 * kQ), 0, ctx->stream, ctx->band_up, W, w, np, R0, xx, y, t);
+    hipLaunchKernelGGL(k_band_apply_inv, gapply, dim3(256), 0, ctx->stream, t, inv_cm + (size_t)R0 * kSB, 1, R0, xx);
   }
   hipLaunchKernelGGL(k_band_scatter, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->band_perm, xx, x);
   return launch_check(ctx, "direct solve");

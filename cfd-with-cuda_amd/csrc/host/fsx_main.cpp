@@ -13,7 +13,7 @@
 
 static void usage() {
   std::fprintf(stderr,
-               "usage: fsx_solver [file.inp | cavity N | duct] [--ngp 8|27] [--steps K] [--mode fp64|mixed]\n"
+               "usage: fsx_solver [file.inp | cavity N | duct] [--ngp 8|27] [--steps K] [--mode fp64|mixed|direct]\n"
                "                  [--rtol R] [--re RE] [--dt DT]\n"
                "       without a mesh argument the .inp named in ProblemName.txt is read\n");
 }
@@ -50,7 +50,11 @@ int main(int argc, char** argv) {
     auto next = [&]() { return a + 1 < argc ? argv[++a] : (char*)"0"; };
     if (s == "--steps") steps = std::atoi(next());
     else if (s == "--ngp") ngp = std::atoi(next());
-    else if (s == "--mode") mode = std::strcmp(next(), "mixed") == 0 ? FSX_SOLVER_MIXED_IR : FSX_SOLVER_FP64_JACOBI;
+    else if (s == "--mode") {
+      const char* m = next();
+      mode = std::strcmp(m, "mixed") == 0 ? FSX_SOLVER_MIXED_IR
+             : std::strcmp(m, "direct") == 0 ? FSX_SOLVER_DIRECT : FSX_SOLVER_FP64_JACOBI;
+    }
     else if (s == "--rtol") rtol = std::atof(next());
     else if (s == "--re") re = std::atof(next());
     else if (s == "--dt") dt = std::atof(next());

@@ -34,6 +34,7 @@ def clean(name):
         args = re.match(r"I((?:Li-?\d+E)+)E", name[m.end() + n:])
         targs = re.findall(r"Li(-?\d+)E", args.group(1)) if args else []
         return f"fsx::{kern}" + (f"<{', '.join(targs)}>" if targs else "")
+    name = name.replace("(anonymous namespace)::", "")
     return re.sub(r"\(.*", "", name).replace("void ", "")
 
 
