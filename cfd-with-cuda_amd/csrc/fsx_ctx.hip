@@ -478,10 +478,11 @@ static int build_z_forms(fsx_ctx* ctx) {
   }
   for (fsx::IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) iter_graph_reset(*g);
   for (void* p : {(void*)ctx->Zr_row, (void*)ctx->Zr_val, (void*)ctx->Zr_col, (void*)ctx->Zr_pat, (void*)ctx->Zr_off,
-                  (void*)ctx->Zc_ent, (void*)ctx->Zc_chunk, (void*)ctx->Zc_rows})
+                  (void*)ctx->Zc_ent, (void*)ctx->Zc_chunk, (void*)ctx->Zc_rows, (void*)ctx->Zc_delta,
+                  (void*)ctx->Zc_doff})
     if (p) dev_free(ctx, p);
   ctx->Zr_row = nullptr; ctx->Zr_val = nullptr; ctx->Zr_col = nullptr; ctx->Zr_pat = nullptr; ctx->Zr_off = nullptr;
-  ctx->Zc_ent = nullptr; ctx->Zc_chunk = nullptr; ctx->Zc_rows = nullptr;
+  ctx->Zc_ent = nullptr; ctx->Zc_chunk = nullptr; ctx->Zc_rows = nullptr; ctx->Zc_delta = nullptr; ctx->Zc_doff = nullptr;
   ctx->Zr_npat = 0; ctx->Zr_patlen = 0; ctx->Zc_n = 0; ctx->Zc_len = 0; ctx->Zc_nchunk = 0;
   const long long nnz2 = rp2[n];
   ctx->nnzZr = nnz2;
@@ -531,6 +532,32 @@ static int build_z_forms(fsx_ctx* ctx) {
     FSX_HIP(hipMemcpy(ctx->Zc_chunk, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
     FSX_HIP(hipMemcpy(ctx->Zc_rows, rows.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
     ctx->Zc_nchunk = (int)chunks.size();
+    // the FP64 corrections: a class entry k is entry k of each member row
+    // (same column offsets in the same order); rows that differ keep no split
+    std::vector<long long> doff(chunks.size() + 1, 0);
+    for (size_t c = 0; c < chunks.size(); c++) doff[c + 1] = doff[c] + 64LL * chunks[c].w;
+    std::vector<float> delta((size_t)doff.back(), 0.0f);
+    bool same = true;
+    for (size_t c = 0; c < chunks.size() && same; c++) {
+      const int4 ch = chunks[c];
+      for (int l = 0; l < ch.y && same; l++) {
+        const int r = rows[ch.x + l];
+        if (rp2[r + 1] - rp2[r] != ch.w) { same = false; break; }
+        for (int k = 0; k < ch.w; k++) {
+          const int kk = rp2[r] + k;
+          if (ci2[kk] - r != co[ch.z + k]) { same = false; break; }
+          const double v = v2[kk];
+          delta[(size_t)doff[c] + (size_t)k * 64 + l] = (float)(v - (double)(float)v);
+        }
+      }
+    }
+    if (same) {
+      FSX_TRY(alloc(ctx, &ctx->Zc_delta, std::max<size_t>(1, delta.size())));
+      FSX_TRY(alloc(ctx, &ctx->Zc_doff, chunks.size()));
+      if (!delta.empty())
+        FSX_HIP(hipMemcpy(ctx->Zc_delta, delta.data(), delta.size() * sizeof(float), hipMemcpyHostToDevice));
+      FSX_HIP(hipMemcpy(ctx->Zc_doff, doff.data(), chunks.size() * sizeof(long long), hipMemcpyHostToDevice));
+    }
   }
   return FSX_OK;
 }

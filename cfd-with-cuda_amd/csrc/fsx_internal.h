@@ -202,6 +202,11 @@ struct fsx_ctx {
   // (first index into Zc_rows, row count, class entry start, entry count)
   int4* Zc_chunk = nullptr; int Zc_nchunk = 0;
   int* Zc_rows = nullptr;
+  // FP64 Z on the classes: each row's values minus their FP32 rounding, as
+  // FP32 (Z64 = class value + delta to within 2^-48 of the value), chunk c's
+  // block at Zc_doff[c] laid out entry-major, 64 lanes a row (the FP64
+  // residuals of the mixed and learned solvers read these instead of Zr_val)
+  float* Zc_delta = nullptr; long long* Zc_doff = nullptr;
   // lumped mass and BCs (device, NN-long; the reference replicates them x3)
   double* MdInv = nullptr; double* MdOrigInv = nullptr; double* Md = nullptr;
   unsigned char* bcflag = nullptr;

@@ -896,10 +896,58 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restric
   if (i < n && l == 0) rh[i] = b[i] - s;
 }
 
+// rh = b - Z xh in FP64 on the row classes: each value is the class's FP32
+// value plus the row's FP32 correction (Z64 to within 2^-48 of each value),
+// a wavefront per chunk of rows of one class, a lane per row
+template <int U>
+__global__ __launch_bounds__(kBlock) void k_zcls_residual(int nchunk, const int4* __restrict__ chunk,
+                                                          const int* __restrict__ rows, const int2* __restrict__ ent,
+                                                          const float* __restrict__ delta,
+                                                          const long long* __restrict__ doff,
+                                                          const double* __restrict__ xh, const double* __restrict__ b,
+                                                          double* __restrict__ rh) {
+  const int w = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (w >= nchunk) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int4 c = chunk[w];
+  const int rs = __builtin_amdgcn_readfirstlane(c.x), len = __builtin_amdgcn_readfirstlane(c.y);
+  const int es = __builtin_amdgcn_readfirstlane(c.z), el = __builtin_amdgcn_readfirstlane(c.w);
+  const int r = rows[rs + (lane < len ? lane : 0)];
+  const int2* __restrict__ e = ent + es;
+  const float* __restrict__ dl = delta + doff[w] + lane;
+  double s = 0.0;
+  int k = 0;
+  for (; k + U <= el; k += U) {
+    int2 t[U];
+    double v[U];
+    float d[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) t[j] = e[k + j];
+#pragma unroll
+    for (int j = 0; j < U; j++) { v[j] = xh[r + t[j].x]; d[j] = dl[(k + j) * 64]; }
+#pragma unroll
+    for (int j = 0; j < U; j++) s += ((double)__int_as_float(t[j].y) + (double)d[j]) * v[j];
+  }
+  for (; k < el; k++) {
+    const int2 t = e[k];
+    s += ((double)__int_as_float(t.y) + (double)dl[k * 64]) * xh[r + t.x];
+  }
+  if (lane < len) rh[r] = b[r] - s;
+}
+
 static void launch_zresidual(fsx_ctx* ctx, const double* xh, const double* b, double* rh) {
   constexpr int LPR = PcgCfg<double>::LPR, U = PcgCfg<double>::U;
   const ColSrc cs = zr_cols(ctx);
   const int n = ctx->NNp;
+  // the mixed and learned solvers' residuals on the row classes with FP64
+  // corrections (half the bytes of the FP64 values); the FP64 and direct
+  // solvers keep the FP64 values
+  if (ctx->Zc_delta && (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)) {
+    const int wpb = kBlock / kWave;
+    hipLaunchKernelGGL(k_zcls_residual<kClsU>, dim3((ctx->Zc_nchunk + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream,
+                       ctx->Zc_nchunk, ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, ctx->Zc_delta, ctx->Zc_doff, xh, b, rh);
+    return;
+  }
   if (cs.row_pat)
     hipLaunchKernelGGL((k_residual<8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, ctx->Zr_row, cs,
                        ctx->Zr_val, xh, b, rh);
