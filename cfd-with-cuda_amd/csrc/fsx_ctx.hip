@@ -816,6 +816,7 @@ int fsx_next_iteration(fsx_ctx* ctx) {
   std::swap(ctx->Arec_prev, ctx->Arec);
   std::swap(ctx->urec_prev_ok, ctx->urec_ok);
   std::swap(ctx->arec_prev_ok, ctx->arec_ok);
+  ctx->drec_ok = false;   // step1 of the next inner iteration writes it again
   FSX_TRY(launch_copy(ctx, ctx->Unp1_prev, ctx->Unp1, 3LL * ctx->NN));
   FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pnp1, ctx->NNp));
   return FSX_OK;

@@ -167,6 +167,11 @@ struct fsx_ctx {
   bool band_ready = false;       // the factor is of the current Z
   double band_factor_ms = 0.0;
   bool urec_ok = false, urec_prev_ok = false, arec_ok = false, arec_prev_ok = false;
+  // step2's operand d = UnpHalf / dt^2 - [KAcc formed] MdOrigInv KAcc as an
+  // FP32 shadow (same layout), written by step1 next to UnpHalf; valid while
+  // drec_ok, with the KAcc term iff drec_kacc
+  float* Drec = nullptr;
+  bool drec_ok = false, drec_kacc = false;
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
@@ -295,7 +300,7 @@ int records_begin_step(fsx_ctx* ctx);
 int direct_factor(fsx_ctx* ctx);
 int direct_apply(fsx_ctx* ctx, const double* b, double* x);
 inline void drop_records(fsx_ctx* ctx) {
-  ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = false;
+  ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = ctx->drec_ok = false;
 }
 int launch_step2_rhs(fsx_ctx* ctx, int iter);
 int launch_pnp1(fsx_ctx* ctx);

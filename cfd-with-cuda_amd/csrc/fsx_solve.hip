@@ -70,7 +70,9 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // With REC the K operands are gathered from the FP32 shadows Urp (and Arp)
 // of UnpHalf_prev (and Acc_prev), one 16-byte load per column and vector
 // instead of 24 bytes; the products and sums stay FP64. A non-null Uout
-// receives UnpHalf rounded to FP32, the shadow the next inner iteration reads.
+// receives UnpHalf rounded to FP32, the shadow the next inner iteration reads,
+// and a non-null Dout step2's operand UnpHalf / dt^2 (- MdOrigInv KAcc with
+// KACC) rounded to FP32.
 template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE, class VT, bool REC>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, ColSrc mcs,
                                                   const VT* __restrict__ K, const int* __restrict__ Grow,
@@ -81,7 +83,9 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
                                                   double* __restrict__ UnpHalf, double dt,
                                                   const double* __restrict__ Accp, double* __restrict__ KAcc,
                                                   double* __restrict__ GPn, const float4* __restrict__ Urp,
-                                                  const float4* __restrict__ Arp, float* __restrict__ Uout) {
+                                                  const float4* __restrict__ Arp, float* __restrict__ Uout,
+                                                  const double* __restrict__ MdOrigInv, double o2,
+                                                  float* __restrict__ Dout) {
   int l;
   const long long r = my_row<LPR>(&l);
   double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
@@ -154,7 +158,13 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
     const double uh = Un[i] + dt * R * MdInv[r];
     UnpHalf[i] = uh;
     if (Uout) Uout[4 * (size_t)r + l] = (float)uh;
-    if constexpr (KACC) KAcc[i] = l == 0 ? a0 : (l == 1 ? a1 : a2);
+    double ka = 0.0;
+    if constexpr (KACC) { ka = l == 0 ? a0 : (l == 1 ? a1 : a2); KAcc[i] = ka; }
+    if (Dout) {   // step2's operand, as k_step2_rhs forms it
+      double d = o2 * uh;
+      if constexpr (KACC) d = d - MdOrigInv[r] * ka;
+      Dout[4 * (size_t)r + l] = (float)d;
+    }
   }
 }
 
@@ -178,11 +188,13 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   const float4* urp = reinterpret_cast<const float4*>(ctx->Urec_prev);
   const float4* arp = reinterpret_cast<const float4*>(ctx->Arec_prev);
   float* uout = wrec ? ctx->Urec : nullptr;
+  float* dout = wrec ? ctx->Drec : nullptr;
+  const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
 #define STEP1_LAUNCH_T(KA, KP, GM, VT, RC, KV, GV)                                                                    \
   hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT, RC>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,  \
                      mcs, KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,      \
                      ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr,       \
-                     ctx->GPn, urp, arp, uout)
+                     ctx->GPn, urp, arp, uout, ctx->MdOrigInv, o2, dout)
 #define STEP1_LAUNCH(KA, KP, GM) \
   do { if (rrec) STEP1_LAUNCH_T(KA, KP, GM, float, true, ctx->Kf, ctx->Gf); \
        else if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, false, ctx->Kf, ctx->Gf); \
@@ -200,6 +212,8 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   FSX_TRY(launch_check(ctx, "k_step1"));
   ctx->gpn_valid = true;
   ctx->urec_ok = wrec;
+  ctx->drec_ok = wrec;
+  ctx->drec_kacc = kacc;
   if (ctx->mass_mode == FSX_MASS_CONSISTENT) return mass_update_step1(ctx);
   return FSX_OK;
 }
@@ -224,9 +238,10 @@ int refresh_matrix_copies(fsx_ctx* ctx) {
   }
   if (!ctx->rec_mem) {
     const size_t n4 = 4 * (size_t)ctx->NN;
-    FSX_TRY(alloc(ctx, &ctx->rec_mem, 4 * n4));
+    FSX_TRY(alloc(ctx, &ctx->rec_mem, 5 * n4));
     ctx->Urec = ctx->rec_mem; ctx->Urec_prev = ctx->rec_mem + n4;
     ctx->Arec = ctx->rec_mem + 2 * n4; ctx->Arec_prev = ctx->rec_mem + 3 * n4;
+    ctx->Drec = ctx->rec_mem + 4 * n4;
   }
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzM, ctx->K, ctx->Kf_mem);
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->G, ctx->Gf_mem);
@@ -307,6 +322,7 @@ int launch_kacc(fsx_ctx* ctx) {
                          ctx->Acc_prev, ctx->KAcc);
   }
   ctx->kacc_pending = false;
+  ctx->drec_ok = false;   // step2's FP32 operand holds the KAcc before this one
   return launch_check(ctx, "k_k3");
 }
 
@@ -319,12 +335,14 @@ int settle_kacc(fsx_ctx* ctx) { return ctx->kacc_pending ? launch_kacc(ctx) : FS
 // R2[p] = sum_c G_c^T d_c, d_c = UnpHalf_c / dt^2 - [iter>1] MdOrigInv KAcc_c,
 // summed as (c0 + c1) + c2 like the three csrmv calls of BCH:3864-3868;
 // R2[p0] = 0 for p0 > 0 (BCH:4180-4184).
-template <int LPR, int U, class VT>
+// With DREC the operand d is gathered as step1's FP32 shadow of it (16 bytes
+// a column instead of 48 plus MdOrigInv); sums stay FP64.
+template <int LPR, int U, class VT, bool DREC>
 __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int* __restrict__ GTrow,
                                                       const int* __restrict__ GTcol, const VT* __restrict__ GT,
                                                       const double* __restrict__ UnpHalf, const double* __restrict__ MdOrigInv,
                                                       const double* __restrict__ KAcc, int use_kacc, double o2,
-                                                      int zeroP, double* __restrict__ R2) {
+                                                      int zeroP, double* __restrict__ R2, const float4* __restrict__ Drec) {
   int l;
   const long long p = my_row<LPR>(&l);
   double s0 = 0, s1 = 0, s2 = 0;
@@ -345,13 +363,18 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int i = c[j];
-        const size_t x = 3 * (size_t)i;
-        double d0 = o2 * UnpHalf[x], d1 = o2 * UnpHalf[x + 1], d2 = o2 * UnpHalf[x + 2];
-        if (use_kacc) {
-          const double m = MdOrigInv[i];
-          d0 = d0 - m * KAcc[x]; d1 = d1 - m * KAcc[x + 1]; d2 = d2 - m * KAcc[x + 2];
+        if constexpr (DREC) {
+          const float4 d = Drec[i];
+          s0 += a0[j] * (double)d.x; s1 += a1[j] * (double)d.y; s2 += a2[j] * (double)d.z;
+        } else {
+          const size_t x = 3 * (size_t)i;
+          double d0 = o2 * UnpHalf[x], d1 = o2 * UnpHalf[x + 1], d2 = o2 * UnpHalf[x + 2];
+          if (use_kacc) {
+            const double m = MdOrigInv[i];
+            d0 = d0 - m * KAcc[x]; d1 = d1 - m * KAcc[x + 1]; d2 = d2 - m * KAcc[x + 2];
+          }
+          s0 += a0[j] * d0; s1 += a1[j] * d1; s2 += a2[j] * d2;
         }
-        s0 += a0[j] * d0; s1 += a1[j] * d1; s2 += a2[j] * d2;
       }
     }
   }
@@ -363,14 +386,22 @@ int launch_step2_rhs(fsx_ctx* ctx, int iter) {
   // 64^3: 16x8 0.307 ms, 16x4 0.310, 32x4 0.316, 16x2 0.312, 8x4 0.344, 8x8 0.346, 64x2 0.372, 4x8 0.429
   constexpr int LPR = 16;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
-  if (ctx->GTf)
-    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
-                       ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
-                       iter != 1 ? 1 : 0, o2, ctx->zeroP, ctx->R2);
+  const int use_kacc = iter != 1 ? 1 : 0;
+  // step1's FP32 shadow of d, when it is of the current UnpHalf and KAcc term
+  const bool drec = ctx->GTf && ctx->drec_ok && ctx->drec_kacc == (use_kacc == 1);
+  const float4* dr = reinterpret_cast<const float4*>(ctx->Drec);
+  if (drec)
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, true>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream,
+                       ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
+                       use_kacc, o2, ctx->zeroP, ctx->R2, dr);
+  else if (ctx->GTf)
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, false>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream,
+                       ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
+                       use_kacc, o2, ctx->zeroP, ctx->R2, dr);
   else
-    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, double>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NNp,
-                       ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GT, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
-                       iter != 1 ? 1 : 0, o2, ctx->zeroP, ctx->R2);
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, double, false>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream,
+                       ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GT, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
+                       use_kacc, o2, ctx->zeroP, ctx->R2, dr);
   return launch_check(ctx, "k_step2_rhs");
 }
 
@@ -1275,19 +1306,23 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
     case FSX_KERNEL_STEP1: {  // K (col + value), G Pn (kept: 3 doubles per row), Uprev, R1 r/w, Un, MdInv, bc,
                               // UnpHalf, Acc_prev, KAcc: the form of the second and later inner iterations;
                               // with the FP32 shadows Uprev and Acc_prev are 16 bytes per node and
-                              // UnpHalf's shadow is written (12 bytes)
+                              // the shadows of UnpHalf and of step2's operand are written (12 + 12
+                              // bytes; the latter reads MdOrigInv, 8)
       const bool wrec = ctx->Kf && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT;
       const bool rrec = wrec && ctx->urec_prev_ok && ctx->arec_prev_ok;
       const double vin = rrec ? 16.0 : 24.0;
       bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + vin * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
-              24.0 * NN + vin * NN + 24.0 * NN + (wrec ? 12.0 * NN : 0.0);
+              24.0 * NN + vin * NN + 24.0 * NN + (wrec ? 32.0 * NN : 0.0);
       break;
     }
     case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
       bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
       break;
-    case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2
-      bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
+    case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2; or G^T, d's FP32 shadow, R2
+      if (ctx->GTf && ctx->drec_ok && ctx->drec_kacc)
+        bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 16.0 * NN + 8.0 * NNp;
+      else
+        bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
       break;
     default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc (+ its FP32 shadow), Unp1
       bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +

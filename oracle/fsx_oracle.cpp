@@ -40,7 +40,7 @@
 // Options with no reference counterpart, off by default, that model the
 // device's other forms for the tests: consistent-mass updates (orc_set_mass),
 // the warm-started PCG (orc_set_pcg_x0, the reference GPU path's x0),
-// FP32-rounded K operands (orc_set_k_operands) and the direct Cholesky
+// FP32-rounded operands (orc_set_f32_operands) and the direct Cholesky
 // pressure solve of the MATLAB prototype (orc_set_direct, ML:1876-1885,
 // 2021-2030).
 #include <cmath>
@@ -77,10 +77,11 @@ struct Case {
   double pcg_rtol = 1e-8;
   int pcg_maxit = 1000;
   bool pcg_warm = false;          // x0 = previous Pdot (the GPU path, CU:425-442) instead of 0
-  // K's operands UnpHalf_prev and Acc_prev rounded to FP32 before the K
-  // products of step1 and K*Acc (lumped mass only): a model of the device's
-  // FP32 operand option for tests, no reference counterpart
-  bool k_operands_f32 = false;
+  // FP32-rounded operands (lumped mass only): K's operands UnpHalf_prev and
+  // Acc_prev before the K products of step1 and K*Acc, and step2's operand
+  // UnpHalf / dt^2 - MdOrigInv KAcc before the G^T products: a model of the
+  // device's FP32 operand shadows for tests, no reference counterpart
+  bool f32_operands = false;
   // pressure solve by Cholesky of Z instead of PCG (the MATLAB prototype's
   // step, ML:1876-1885, 2021-2030); the factor is kept until Z changes
   bool pcg_direct = false;
@@ -825,9 +826,9 @@ void mass_solve(Case& c, const double* R, double* a) {
   }
 }
 
-// the operand of a K product: v itself, or v rounded to FP32 (k_operands_f32)
+// the operand of a K product: v itself, or v rounded to FP32 (f32_operands)
 std::vector<double> k_operand(const Case& c, const std::vector<double>& v) {
-  if (!c.k_operands_f32 || c.mass_mode != 0) return v;
+  if (!c.f32_operands || c.mass_mode != 0) return v;
   std::vector<double> r(v.size());
   for (size_t i = 0; i < v.size(); i++) r[i] = (double)(float)v[i];
   return r;
@@ -930,6 +931,8 @@ void step2(Case& c, int iter) {
   for (size_t i = 0; i < (size_t)3*NN; i++) dummy[i] = o2 * c.UnpHalf[i];
   if (iter != 1)
     for (size_t i = 0; i < (size_t)3*NN; i++) dummy[i] = dummy[i] - c.MdOrigInv[i] * c.KAcc[i];
+  if (c.f32_operands && c.mass_mode == 0)                              // the device's FP32 shadow of it
+    for (size_t i = 0; i < (size_t)3*NN; i++) dummy[i] = (double)(float)dummy[i];
   const std::vector<double>* Gs[3] = {&c.G1, &c.G2, &c.G3};
   for (int d = 0; d < 3; d++)
     gtmv(c, Gs[d]->data(), dummy.data() + (size_t)d*NN, c.R2.data(), 1.0, d == 0 ? 0.0 : 1.0);
@@ -1177,11 +1180,12 @@ int orc_direct_solve(void* h, const double* b, double* x, double* relres) {
 // half-bandwidth of the kept factor (-1 when none)
 int orc_chol_bandwidth(void* h) { return ((Case*)h)->chol_w; }
 
-// K's operands in FP64 (0) or rounded to FP32 (1); see Case::k_operands_f32
-int orc_set_k_operands(void* h, int f32) {
+// step1's K operands and step2's G^T operand in FP64 (0) or rounded to FP32
+// (1); see Case::f32_operands
+int orc_set_f32_operands(void* h, int f32) {
   Case& c = *(Case*)h;
-  if (f32 < 0 || f32 > 1) { c.err = "orc_set_k_operands: 0 or 1"; return 1; }
-  c.k_operands_f32 = f32 == 1;
+  if (f32 < 0 || f32 > 1) { c.err = "orc_set_f32_operands: 0 or 1"; return 1; }
+  c.f32_operands = f32 == 1;
   return 0;
 }
 

@@ -58,7 +58,7 @@ def lib():
         L.orc_timings.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_mass.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.orc_set_pcg_x0.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_set_k_operands.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_f32_operands.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_direct.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_direct_solve.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.orc_chol_bandwidth.argtypes = [ctypes.c_void_p]
@@ -184,11 +184,11 @@ class Oracle:
     def chol_bandwidth(self):
         return self.L.orc_chol_bandwidth(self.h)
 
-    def set_k_operands(self, f32):
-        """Round the K products' operands UnpHalf_prev and Acc_prev to FP32
-        (1) or not (0): a model of the device's FP32 operand shadows, lumped
-        mass only."""
-        self._check(self.L.orc_set_k_operands(self.h, int(f32)), "set_k_operands")
+    def set_f32_operands(self, f32):
+        """Round step1's K operands UnpHalf_prev and Acc_prev and step2's G^T
+        operand UnpHalf/dt^2 - MdOrigInv KAcc to FP32 (1) or not (0): a model
+        of the device's FP32 operand shadows, lumped mass only."""
+        self._check(self.L.orc_set_f32_operands(self.h, int(f32)), "set_f32_operands")
 
     def set_mass(self, mode, rtol=1e-10, maxit=1000):
         """Velocity mass: 0 lumped (the reference), 1 consistent M solved by

@@ -348,7 +348,7 @@ def test_fp32_matrix_copies(fsx_lib, oracle_lib):
     s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(8), NGP=27, pcg_rtol=1e-12)
     for name in ("K", "G1", "G2", "G3"):
         o.set_d(name, o.d(name).astype(np.float32).astype(np.float64))
-    o.set_k_operands(1)
+    o.set_f32_operands(1)
     s.run(3)
     u64, p64 = s.download("Un"), s.download("Pn")
     s.reset(mode=0)

@@ -451,18 +451,21 @@ def test_warm_start_pcg(oracle_lib):
     assert np.linalg.norm(r2 - Z @ pd) <= 1.01e-12 * np.linalg.norm(r2)
 
 
-def test_k_operands_f32(oracle_lib):
-    """K operands rounded to FP32 (the oracle's model of the device's FP32
-    shadows of UnpHalf_prev and Acc_prev): step1's R1 moves by exactly
-    -K (fl32(v) - v) per component, v = UnpHalf_prev, except at BC rows,
-    which step1 zeroes; with consistent mass the option does nothing."""
-    c = oracle_lib.case_from_mesh(meshgen.cavity(3), NGP=8)
+def test_f32_operands(oracle_lib):
+    """Operands rounded to FP32 (the oracle's model of the device's FP32
+    shadows): step1's R1 moves by exactly -K (fl32(v) - v) per component,
+    v = UnpHalf_prev, except at BC rows, which step1 zeroes; step2's R2 moves
+    by exactly sum_d G_d^T (fl32(e_d) - e_d), e = UnpHalf / dt^2 - MdOrigInv
+    KAcc, except at the pinned pressure row; with consistent mass the option
+    does nothing."""
+    dt = 0.01
+    c = oracle_lib.case_from_mesh(meshgen.cavity(3), NGP=8, dt=dt)
     NN = c.sizes()["NN"]
     v = np.random.default_rng(5).standard_normal(3 * NN)
     c.set_d("UnpHalf_prev", v)
     c.step1(1)
     r0 = c.d("R1").copy()
-    c.set_k_operands(1)
+    c.set_f32_operands(1)
     c.set_d("UnpHalf_prev", v)
     c.step1(1)
     r1 = c.d("R1").copy()
@@ -473,11 +476,31 @@ def test_k_operands_f32(oracle_lib):
         want[[n, n + NN, n + 2 * NN]] = 0.0
     assert np.linalg.norm(want) > 0
     assert np.linalg.norm((r1 - r0) - want) <= 1e-6 * np.linalg.norm(want)
+    # step2: R2 = sum_d G_d^T e_d
+    rng = np.random.default_rng(6)
+    u, k = rng.standard_normal(3 * NN), rng.standard_normal(3 * NN)
+    r2 = []
+    for f in (0, 1):
+        c.set_f32_operands(f)
+        c.set_d("UnpHalf", u)
+        c.set_d("KAcc", k)
+        c.step2(2)
+        r2.append(c.d("R2").copy())
+    NNp = c.sizes()["NNp"]
+    e = u / dt**2 - c.d("MdOrigInv") * k
+    de = (e.astype(np.float32).astype(np.float64) - e).reshape(3, NN)
+    want2 = sum(csr_dense(c.i("Grow"), c.i("Gcol"), c.d(f"G{d + 1}"), (NN, NNp)).T @ de[d] for d in range(3))
+    pinned = (r2[0] == 0.0) & (r2[1] == 0.0)   # the zero-pressure row (BCH:4180-4184)
+    assert pinned.sum() <= 1
+    want2[pinned] = 0.0
+    assert np.linalg.norm(want2) > 0
+    assert np.linalg.norm((r2[1] - r2[0]) - want2) <= 1e-6 * np.linalg.norm(want2)
+    c.set_f32_operands(1)
     c.set_mass(1, 1e-12, 1000)
     c.set_d("UnpHalf_prev", v)
     c.step1(1)
     rm1 = c.d("R1").copy()
-    c.set_k_operands(0)
+    c.set_f32_operands(0)
     c.set_d("UnpHalf_prev", v)
     c.step1(1)
     assert np.array_equal(rm1, c.d("R1"))
