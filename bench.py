@@ -43,8 +43,11 @@ def step_kernels(vt):
     ("double", or "float" in the mixed forms, whose step1 also gathers K's
     operands from their FP32 shadows): step1 with K*Acc_prev fused and G*Pn
     kept (inner iterations >= 2), step2's right-hand side, step3."""
-    rec = "true" if vt == "float" else "false"
-    return (f"k_step1<8, 2, 2, true, true, 2, {vt}, {rec}>", f"k_step2_rhs<16, 8, {vt}>", f"k_step3<8, 2, {vt}>")
+    if vt == "float":
+        return ("k_step1<8, 2, 2, true, true, 2, float, true>", "k_step2_rhs<32, 1, float, true>",
+                "k_step3<4, 1, float>")
+    return ("k_step1<8, 2, 2, true, true, 2, double, false>", "k_step2_rhs<16, 8, double, false>",
+            "k_step3<8, 2, double>")
 
 
 def zspmv_name(prec, npat, ncls):

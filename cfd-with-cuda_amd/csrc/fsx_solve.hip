@@ -383,15 +383,18 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
 }
 
 int launch_step2_rhs(fsx_ctx* ctx, int iter) {
-  // 64^3: 16x8 0.307 ms, 16x4 0.310, 32x4 0.316, 16x2 0.312, 8x4 0.344, 8x8 0.346, 64x2 0.372, 4x8 0.429
+  // FP64 operands, 64^3: 16x8 0.307 ms, 16x4 0.310, 32x4 0.316, 16x2 0.312, 8x4 0.344, 8x8 0.346, 64x2 0.372,
+  // 4x8 0.429
   constexpr int LPR = 16;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
   const int use_kacc = iter != 1 ? 1 : 0;
   // step1's FP32 shadow of d, when it is of the current UnpHalf and KAcc term
   const bool drec = ctx->GTf && ctx->drec_ok && ctx->drec_kacc == (use_kacc == 1);
   const float4* dr = reinterpret_cast<const float4*>(ctx->Drec);
+  // with the FP32 operand, 32 lanes x 1 load a row: 122.7 us at 64^3 (32x2
+  // 131.6, 16x1 126.4, 16x2 134.3, 32x4 139.6, 16x4 142.8, 16x8 161.0)
   if (drec)
-    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, true>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream,
+    hipLaunchKernelGGL((k_step2_rhs<32, 1, float, true>), dim3(rows_grid(ctx->NNp, 32)), dim3(kBlock), 0, ctx->stream,
                        ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
                        use_kacc, o2, ctx->zeroP, ctx->R2, dr);
   else if (ctx->GTf)
@@ -497,12 +500,15 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
 }
 
 static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
-  // 64^3: 8x2 0.253 ms, 8x4 0.266, 4x2 0.290, 4x8 0.298, 4x4 0.307, 16x2 0.312, 8x8 0.319, 16x4 0.343
+  // FP64 G values, 64^3: 8x2 0.253 ms, 8x4 0.266, 4x2 0.290, 4x8 0.298, 4x4 0.307, 16x2 0.312, 8x8 0.319,
+  // 16x4 0.343
   constexpr int LPR = 8;
   // Acc's FP32 shadow, kept where step1 reads it (lumped mass, FP32 matrices)
   float* aout = (ctx->Gf && ctx->rec_mem && !Rout) ? ctx->Arec : nullptr;
+  // FP32 G values: 4 lanes x 1 load a row, 225.7 us at 64^3 (4x2 237.5, 8x1
+  // 245.9, 8x2 252.6, 4x3 254.6, 8x4 261.7, 16x1 347.0)
   if (ctx->Gf)
-    hipLaunchKernelGGL((k_step3<LPR, 2, float>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+    hipLaunchKernelGGL((k_step3<4, 1, float>), dim3(rows_grid(ctx->NN, 4)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                        ctx->Grow, ctx->Gcol, ctx->Gf, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
                        ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout);
   else
