@@ -37,6 +37,8 @@ struct SolveScalars {
   int stop;           // 0 running, 1 converged, 2 maxit, 3 NaN/Inf, 5 paused for a reliable update
   int maxit, pad;
   int active_v[2];    // iteration j ran its update kernel (stop was 0 then)
+  double rp_v[2];     // learned solver: r.p entering iteration j sits in rp_v[j & 1]
+  double pq_v[2];     // learned solver: p.Zp of iteration j
 };
 
 // SolveScalars as published to host-coherent pinned memory by k_publish:
@@ -230,6 +232,7 @@ struct fsx_ctx {
   double *w0r = nullptr, *w0e = nullptr;   // warm start: b - Z x0 and the correction
   int x0_policy = FSX_X0_ZERO;
   float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 PCG
+  double* fpq = nullptr; int fpq_n = 0;   // learned solver: partial sums of p.Zp from its SpMV
   double* partials = nullptr;    // 8 * kMaxGrid, per-block partial sums of the reduction kernels
   fsx::SolveScalars* sc = nullptr;
   fsx::StepScalars* st = nullptr;

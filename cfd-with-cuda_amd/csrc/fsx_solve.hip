@@ -625,11 +625,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
 // interior of a structured grid). With XCD, consecutive blocks of chunks go
 // to one XCD (blocks are dealt to the 8 XCDs round robin), so each XCD's L2
 // holds one slab of p instead of all of it.
+// With pq non-null each wavefront also writes its chunk's p.q (FP64 sum of
+// FP32 products) to pq[chunk]: the flexible CG takes p.Zp from these.
 template <int U, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __restrict__ chunk,
                                                       const int* __restrict__ rows, const int2* __restrict__ ent,
                                                       const float* __restrict__ p, float* __restrict__ q,
-                                                      const SolveScalars* __restrict__ sc) {
+                                                      const SolveScalars* __restrict__ sc, double* __restrict__ pq) {
   if (sc && sc->stop) return;
   int b = blockIdx.x;
   if constexpr (XCD) b = (b % 8) * (gridDim.x / 8) + b / 8;   // gridDim.x is a multiple of 8
@@ -658,6 +660,10 @@ __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __
     s += __int_as_float(t.y) * p[r + t.x];
   }
   if (lane < len) q[r] = s;
+  if (pq) {
+    const double v = group_sum<kWave>(lane < len ? (double)p[r] * (double)s : 0.0);
+    if (lane == 0) pq[w] = v;
+  }
 }
 
 // The solvers' Z (structural zeros dropped): FP64 rows through the pattern
@@ -676,7 +682,7 @@ static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc)
       const int wpb = kBlock / kWave;
       const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
       hipLaunchKernelGGL((k_zcls_spmv<kClsU, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
-                         ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc);
+                         ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, nullptr);
       return;
     }
   }
@@ -1085,6 +1091,33 @@ int solve_fp64_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, doubl
 
 int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc) {
   launch_zspmv<float>(ctx, p, q, sc);
+  return FSX_OK;
+}
+
+// p.q partials of the flexible CG: one per block of a fixed 256-block grid
+__global__ __launch_bounds__(kBlock) void k_fpq_dot(int n, const float* __restrict__ p, const float* __restrict__ q,
+                                                    double* __restrict__ pq, const SolveScalars* __restrict__ sc) {
+  if (sc->stop) return;
+  double v[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += (double)p[i] * (double)q[i];
+  write_partials<1>(v, pq);
+}
+
+// q = Z p in FP32 with the p.q partials in pq (*npq of them): from the
+// row-class SpMV itself (one per chunk), or from a separate pass when Z has
+// no row classes
+int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc, double* pq, int* npq) {
+  if (ctx->Zc_chunk) {
+    const int wpb = kBlock / kWave;
+    const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
+    hipLaunchKernelGGL((k_zcls_spmv<kClsU, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
+                       ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, pq);
+    *npq = ctx->Zc_nchunk;
+    return FSX_OK;
+  }
+  launch_zspmv<float>(ctx, p, q, sc);
+  hipLaunchKernelGGL(k_fpq_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, p, q, pq, sc);
+  *npq = kRedGrid;
   return FSX_OK;
 }
 

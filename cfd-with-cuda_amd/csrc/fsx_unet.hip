@@ -786,61 +786,68 @@ int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fs
 // direction Z-orthogonalised against the previous one, beta = -z.Zp / p.Zp.
 // Both stay exact when the preconditioner varies from one application to the
 // next (BF16 rounding, a scale taken from the residual) and when a reliable
-// update replaces r. Partial-sum slots (kRedGrid doubles each):
-//   0 p.q, 1 r.p (k_fpcg_dot); 2 r.r (k_fpcg_update, k_fpcg_load_r);
-//   3 r.z, 4 z.q (k_conv0T).
+// update replaces r. p.Zp comes from the SpMV's own partial sums (one per
+// row-class chunk); r.p is carried as a scalar: r.p_new = r.z + beta r.p_old
+// (bilinear, so exact up to rounding), with r.p_old summed by the update
+// kernel and, after a reliable update replaces r, by k_fpcg_load_r.
+// Partial-sum slots (kRedGrid doubles each): 1 r.p_old, 2 r.r
+// (k_fpcg_update, k_fpcg_load_r); 3 r.z, 4 z.q (k_conv0T).
 namespace fsx {
 
-constexpr int kSlotRR = 2;
+constexpr int kSlotRP = 1;
 
-__global__ __launch_bounds__(kBlock) void k_fpcg_dot(int n, const float* __restrict__ r, const float* __restrict__ p,
-                                                     const float* __restrict__ q, double* __restrict__ partials,
-                                                     const SolveScalars* __restrict__ sc) {
-  if (sc->stop) return;
-  double v[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
-    v[0] += (double)p[i] * (double)q[i];
-    v[1] += (double)r[i] * (double)p[i];
-  }
-  write_partials<2>(v, partials);
+// the npq partial sums of p.Zp in a fixed order; every thread gets the total
+__device__ __forceinline__ double sum_pq(const double* __restrict__ pq, int npq) {
+  __shared__ double red[kBlock / kWave];
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < npq; i += kBlock) v[0] += pq[i];
+  block_sum_all<1>(v, red);
+  return v[0];
 }
 
 // alpha = r.p / p.q (0 when p is not a descent direction, which
-// k_fpcg_direction then reports); x += alpha p, r -= alpha q; r.r partials
+// k_fpcg_direction then reports); x += alpha p, r -= alpha q; partials of
+// r.p (the direction before its update) and r.r
 __global__ __launch_bounds__(kBlock) void k_fpcg_update(int j, int n, float* __restrict__ x, float* __restrict__ r,
                                                         const float* __restrict__ p, const float* __restrict__ q,
-                                                        double* __restrict__ partials, SolveScalars* sc) {
+                                                        double* __restrict__ partials, const double* __restrict__ pq,
+                                                        int npq, SolveScalars* sc) {
   const int stop = sc->stop;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->active_v[j & 1] = stop ? 0 : 1;
   if (stop) return;
-  double s[2];
-  read_partials<2>(partials, s);
-  const float a = (s[0] > 0.0 && s[1] > 0.0) ? (float)(s[1] / s[0]) : 0.0f;
-  double v[1] = {0.0};
+  const double pqv = sum_pq(pq, npq), rp = sc->rp_v[j & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq_v[j & 1] = pqv;
+  const float a = (pqv > 0.0 && rp > 0.0) ? (float)(rp / pqv) : 0.0f;
+  double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
-    x[i] += a * p[i];
+    const float pi = p[i];
+    x[i] += a * pi;
     const float ri = r[i] - a * q[i];
     r[i] = ri;
-    v[0] += (double)ri * (double)ri;
+    v[0] += (double)ri * (double)pi;
+    v[1] += (double)ri * (double)ri;
   }
-  write_partials<1>(v, partials + kSlotRR * kRedGrid);
+  write_partials<2>(v, partials + kSlotRP * kRedGrid);
 }
 
-// p = z - (z.q / p.q) p; stop tests (block 0): 3 NaN/Inf, 6 loss of descent
-// (p.Zp, r.p or r.z not positive), 1 converged (FP32 residual; the FP64
-// check decides), 5 pause for a reliable update, 2 iteration cap.
+// p = z - (z.q / p.q) p and the next r.p; stop tests (block 0): 3 NaN/Inf,
+// 6 loss of descent (p.Zp, r.p or r.z not positive), 1 converged (FP32
+// residual; the FP64 check decides), 5 pause for a reliable update, 2
+// iteration cap.
 __global__ __launch_bounds__(kBlock) void k_fpcg_direction(int j, int n, const float* __restrict__ z, float* __restrict__ p,
                                                            const double* __restrict__ partials, double delta2,
                                                            SolveScalars* sc) {
   if (!sc->active_v[j & 1]) return;
-  double s[5];
-  read_partials<5>(partials, s);
-  const double pq = s[0], rp = s[1], rr = s[2], rz = s[3], zq = s[4];
-  const bool bad = !(pq == pq) || !(rp == rp) || !(rr == rr) || !(rz == rz) || !(zq == zq);
+  double s[4];
+  read_partials<4>(partials + kSlotRP * kRedGrid, s);
+  const double rpo = s[0], rr = s[1], rz = s[2], zq = s[3];
+  const double pq = sc->pq_v[j & 1], rp = sc->rp_v[j & 1];
+  const bool bad = !(pq == pq) || !(rp == rp) || !(rr == rr) || !(rz == rz) || !(zq == zq) || !(rpo == rpo);
   const bool lost = !(pq > 0.0) || !(rp > 0.0) || !(rz > 0.0);
   const float be = (bad || lost) ? 0.f : (float)(-zq / pq);
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) p[i] = z[i] + be * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rp_v[(j + 1) & 1] = rz + (double)be * rpo;
     const int it = sc->iters + 1;
     sc->iters = it;
     sc->rr = rr;
@@ -852,21 +859,37 @@ __global__ __launch_bounds__(kBlock) void k_fpcg_direction(int j, int n, const f
   }
 }
 
-__global__ void k_copy_ff(int n, const float* __restrict__ a, float* __restrict__ b) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
+// first direction p = z; r.p = r.z (slot 3) enters iteration 0
+__global__ __launch_bounds__(kBlock) void k_fpcg_start(int n, const float* __restrict__ z, float* __restrict__ p,
+                                                       const double* __restrict__ partials, SolveScalars* sc) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = z[i];
+  if (blockIdx.x == 0) {
+    double rz[1];
+    read_partials<1>(partials + 3 * kRedGrid, rz);
+    if (threadIdx.x == 0) sc->rp_v[0] = rz[0];
+  }
 }
 
-// r = float(rh), x_lo = 0, r.r partials
+// r.p of a replaced residual (slot 1, from k_fpcg_load_r) enters iteration j
+__global__ __launch_bounds__(kBlock) void k_fpcg_set_rp(int j, const double* __restrict__ partials, SolveScalars* sc) {
+  double rp[1];
+  read_partials<1>(partials + kSlotRP * kRedGrid, rp);
+  if (threadIdx.x == 0) sc->rp_v[j & 1] = rp[0];
+}
+
+// r = float(rh), x_lo = 0; partials of r.p (the direction kept) and r.r
 __global__ __launch_bounds__(kBlock) void k_fpcg_load_r(int n, const double* __restrict__ rh, float* __restrict__ r,
-                                                        float* __restrict__ x, double* __restrict__ partials) {
-  double v[1] = {0.0};
+                                                        float* __restrict__ x, const float* __restrict__ p,
+                                                        double* __restrict__ partials) {
+  double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const float ri = (float)rh[i];
     r[i] = ri;
     x[i] = 0.0f;
-    v[0] += (double)ri * (double)ri;
+    v[0] += (double)ri * (double)p[i];
+    v[1] += (double)ri * (double)ri;
   }
-  write_partials<1>(v, partials + kSlotRR * kRedGrid);
+  write_partials<2>(v, partials + kSlotRP * kRedGrid);
 }
 
 __global__ void k_zero_f(int n, float* __restrict__ a) {
@@ -931,6 +954,7 @@ int precondition(fsx_ctx* ctx, fsx_unet* u, const SolveScalars* sc) {
 namespace fsx {
 
 int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc);
+int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc, double* pq, int* npq);
 int launch_fold_residual(fsx_ctx* ctx, double* x, const double* b, int* stop_out, double* rr_hi_out);
 int solve_mixed_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
 
@@ -952,22 +976,33 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
   hipLaunchKernelGGL(k_learned_init, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, max_learned);
   FSX_TRY(read_scalars(ctx));
   if (ctx->h_sc->stop == 1) { *iters = 0; *relres = 0.0; return FSX_OK; }   // b = 0
-  // start: r = b, z = M r, p = z
-  hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx, ctx->partials);
+  // the SpMV's p.Zp partial sums: one per row-class chunk, or per block of
+  // a 256-block grid
+  const int npq_max = std::max(ctx->Zc_chunk ? ctx->Zc_nchunk : 0, kRedGrid);
+  if (ctx->fpq_n < npq_max) {
+    if (ctx->fpq) dev_free(ctx, ctx->fpq);
+    ctx->fpq = nullptr;
+    ctx->fpq_n = 0;
+    FSX_TRY(alloc(ctx, &ctx->fpq, (size_t)npq_max));
+    ctx->fpq_n = npq_max;
+    iter_graph_reset(ctx->graph_learned);   // the captured iterations hold the old buffer
+  }
+  // start: r = b, z = M r, p = z, r.p = r.z
+  hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx, ctx->fp,
+                     ctx->partials);
   hipLaunchKernelGGL(k_zero_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fq);
   FSX_TRY(precondition(ctx, u, nullptr));
-  hipLaunchKernelGGL(k_copy_ff, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fz, ctx->fp);
+  hipLaunchKernelGGL(k_fpcg_start, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fz, ctx->fp, ctx->partials, ctx->sc);
   FSX_TRY(launch_check(ctx, "fpcg start"));
   int j = 0;
   const int seg = std::min(max_learned, segment_guess(ctx->last_learned_iters, delta, ctx->rtol, 2));
   int guess = seg;
   int fallback = 0;
   auto issue = [&](int jj) -> int {
-    FSX_TRY(launch_pcg_spmv_f32(ctx, ctx->fp, ctx->fq, ctx->sc));
-    hipLaunchKernelGGL(k_fpcg_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->fr, ctx->fp, ctx->fq, ctx->partials,
-                       ctx->sc);
+    int npq = 0;
+    FSX_TRY(launch_pcg_spmv_pq_f32(ctx, ctx->fp, ctx->fq, ctx->sc, ctx->fpq, &npq));
     hipLaunchKernelGGL(k_fpcg_update, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, jj, n, ctx->fx, ctx->fr, ctx->fp,
-                       ctx->fq, ctx->partials, ctx->sc);
+                       ctx->fq, ctx->partials, ctx->fpq, npq, ctx->sc);
     FSX_TRY(precondition(ctx, u, ctx->sc));
     hipLaunchKernelGGL(k_fpcg_direction, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, jj, n, ctx->fz, ctx->fp,
                        ctx->partials, delta * delta, ctx->sc);
@@ -999,8 +1034,9 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
     if (st == 1) break;                                     // converged in FP64
     if (stop == 6 || stop == 2 || st == 2) { fallback = stop == 6 ? 6 : 2; break; }
     // replace r by the FP64 residual and keep p: the next alpha uses r.p
-    hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx,
+    hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx, ctx->fp,
                        ctx->partials);
+    hipLaunchKernelGGL(k_fpcg_set_rp, dim3(1), dim3(kBlock), 0, ctx->stream, j, ctx->partials, ctx->sc);
     guess = seg;
   }
   const int learned_iters = ctx->h_sc->iters;
