@@ -462,3 +462,42 @@ def test_direct_pressure_solve(fsx_lib, oracle_lib, case):
     x2, _, rel2, rc2 = s.solve_pressure(b)
     assert rc2 == fsx_lib.FSX_ENOCONV and rel2 < 1e-12 and rel(x2, x) < 1e-13
     s.close()
+
+
+def test_fp32_forms_switch_and_consistent_mass(fsx_lib, oracle_lib):
+    """The FP32 operand shadows follow every change of form: matrix precision
+    32 for 2 steps, 64 for 1, 32 for 1 more (each switch drops the shadows and
+    the next step rebuilds them) against the oracle with its K, G and operand
+    roundings switched the same way, fields to 1e-9; and with consistent mass,
+    where the shadows stay off (the mass solve rewrites UnpHalf), against the
+    oracle with only K and G rounded, fields to 1e-8 (mass solves at 1e-12)."""
+    say("FP32 forms switched mid-run: cavity 8^3, 2 + 1 + 1 steps")
+    s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(8), NGP=27, pcg_rtol=1e-12)
+    k64 = {name: o.d(name).copy() for name in ("K", "G1", "G2", "G3")}
+
+    def oracle_form(f32):
+        for name, v in k64.items():
+            o.set_d(name, v.astype(np.float32).astype(np.float64) if f32 else v)
+        o.set_f32_operands(1 if f32 else 0)
+
+    for f32, n in ((True, 2), (False, 1), (True, 1)):
+        s.set_matrix_precision(32 if f32 else 64)
+        oracle_form(f32)
+        s.run(n)
+        o.run(n)
+        assert rel(s.download("Un"), o.d("Un")) < 1e-9
+        assert rel(s.download("Pn"), o.d("Pn")) < 1e-9
+    s.close()
+    say("FP32 matrices with consistent mass: cavity 6^3, 3 steps")
+    s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(6), NGP=27, pcg_rtol=1e-12)
+    for name in ("K", "G1", "G2", "G3"):
+        o.set_d(name, o.d(name).astype(np.float32).astype(np.float64))
+    o.set_f32_operands(1)   # no effect with consistent mass, as on the device
+    s.set_matrix_precision(32)
+    s.set_mass(fsx_lib.MASS_CONSISTENT, rtol=1e-12, maxit=2000)
+    o.set_mass(1, rtol=1e-12, maxit=2000)
+    s.run(3)
+    o.run(3)
+    assert rel(s.download("Un"), o.d("Un")) < 1e-8
+    assert rel(s.download("Pn"), o.d("Pn")) < 1e-8
+    s.close()
