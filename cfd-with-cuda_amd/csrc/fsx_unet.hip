@@ -973,9 +973,9 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
   FSX_TRY(launch_fill(ctx, x, 0.0, n));
   FSX_TRY(launch_copy(ctx, ctx->wh, b, n));
   hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->partials);
+  // (b = 0 sets the stop flag here: every iteration is then a no-op and the
+  // first FP64 check finds the zero residual; no read-back before the start)
   hipLaunchKernelGGL(k_learned_init, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, max_learned);
-  FSX_TRY(read_scalars(ctx));
-  if (ctx->h_sc->stop == 1) { *iters = 0; *relres = 0.0; return FSX_OK; }   // b = 0
   // the SpMV's p.Zp partial sums: one per row-class chunk, or per block of
   // a 256-block grid
   const int npq_max = std::max(ctx->Zc_chunk ? ctx->Zc_nchunk : 0, kRedGrid);
