@@ -147,8 +147,9 @@ struct fsx_ctx {
   float *Kf_mem = nullptr, *Gf_mem = nullptr, *GTf_mem = nullptr;   // their allocations
   int mat_bits = 64;
   int conv_bits = 64;            // arithmetic of the convection kernel (fsx_set_convection_precision)
-  // FP32 shadows of UnpHalf and Acc (and of their "prev" buffers), 4 floats
-  // per node (the 4th unused), written beside the FP64 vectors by step1 and
+  // FP32 shadows of UnpHalf and Acc (and of their "prev" buffers), interleaved
+  // 8 floats a node (UnpHalf's 3 at Urec + 8i, Acc's at Arec = Urec + 4; the
+  // 4th of each unused), written beside the FP64 vectors by step1 and
   // step3 at matrix precision 32 with lumped mass. step1 gathers its K
   // operands from the prev shadows while their flags say they hold the FP64
   // prev vectors rounded; any other writer of those vectors clears the flags.

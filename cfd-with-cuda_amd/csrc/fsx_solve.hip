@@ -69,7 +69,8 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // (24 bytes per row instead of G's 28 per nonzero), GMODE 0 only forms it.
 // With REC the K operands are gathered from the FP32 shadows Urp (and Arp)
 // of UnpHalf_prev (and Acc_prev), one 16-byte load per column and vector
-// instead of 24 bytes; the products and sums stay FP64. A non-null Uout
+// instead of 24 bytes, the two of a column side by side in one 32-byte
+// record (node i at 8i floats); the products and sums stay FP64. A non-null Uout
 // receives UnpHalf rounded to FP32, the shadow the next inner iteration reads,
 // and a non-null Dout step2's operand UnpHalf / dt^2 (- MdOrigInv KAcc with
 // KACC) rounded to FP32.
@@ -104,10 +105,10 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
 #pragma unroll
       for (int j = 0; j < UK; j++) {
         if constexpr (REC) {
-          const float4 u = Urp[c[j]];
+          const float4 u = Urp[2 * (size_t)c[j]];
           k0 += a[j] * (double)u.x; k1 += a[j] * (double)u.y; k2 += a[j] * (double)u.z;
           if constexpr (KACC) {
-            const float4 v = Arp[c[j]];
+            const float4 v = Arp[2 * (size_t)c[j]];
             a0 += a[j] * (double)v.x; a1 += a[j] * (double)v.y; a2 += a[j] * (double)v.z;
           }
         } else {
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
     R1[i] = R;
     const double uh = Un[i] + dt * R * MdInv[r];
     UnpHalf[i] = uh;
-    if (Uout) Uout[4 * (size_t)r + l] = (float)uh;
+    if (Uout) Uout[8 * (size_t)r + l] = (float)uh;
     double ka = 0.0;
     if constexpr (KACC) { ka = l == 0 ? a0 : (l == 1 ? a1 : a2); KAcc[i] = ka; }
     if (Dout) {   // step2's operand, as k_step2_rhs forms it
@@ -239,8 +240,10 @@ int refresh_matrix_copies(fsx_ctx* ctx) {
   if (!ctx->rec_mem) {
     const size_t n4 = 4 * (size_t)ctx->NN;
     FSX_TRY(alloc(ctx, &ctx->rec_mem, 5 * n4));
-    ctx->Urec = ctx->rec_mem; ctx->Urec_prev = ctx->rec_mem + n4;
-    ctx->Arec = ctx->rec_mem + 2 * n4; ctx->Arec_prev = ctx->rec_mem + 3 * n4;
+    // UnpHalf's and Acc's shadows interleaved, 8 floats a node (u v w - a b c -),
+    // so that step1's two gathers of a column fall in one 32-byte run
+    ctx->Urec = ctx->rec_mem; ctx->Arec = ctx->rec_mem + 4;
+    ctx->Urec_prev = ctx->rec_mem + 2 * n4; ctx->Arec_prev = ctx->rec_mem + 2 * n4 + 4;
     ctx->Drec = ctx->rec_mem + 4 * n4;
   }
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzM, ctx->K, ctx->Kf_mem);
@@ -251,10 +254,12 @@ int refresh_matrix_copies(fsx_ctx* ctx) {
   return FSX_OK;
 }
 
-// FP32 shadows of a node-major velocity vector: 4 floats per node, the 4th 0
+// FP32 shadow of a node-major velocity vector (zeros for a null v): node i
+// at rec[2 i], in the interleaved 8-float records
 __global__ void k_rec(long long nn, const double* __restrict__ v, float4* __restrict__ rec) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x)
-    rec[i] = make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], 0.0f);
+    rec[2 * i] = v ? make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], 0.0f)
+                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // The shadows of UnpHalf_prev = Un and Acc_prev = 0 that fsx_begin_step sets
@@ -264,8 +269,9 @@ int records_begin_step(fsx_ctx* ctx) {
   if (!ctx->Kf || !ctx->rec_mem || ctx->mass_mode == FSX_MASS_CONSISTENT) return FSX_OK;
   hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
                      reinterpret_cast<float4*>(ctx->Urec_prev));
+  hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, nullptr,
+                     reinterpret_cast<float4*>(ctx->Arec_prev));
   FSX_TRY(launch_check(ctx, "k_rec"));
-  FSX_TRY(launch_fill(ctx, reinterpret_cast<double*>(ctx->Arec_prev), 0.0, 2LL * ctx->NN));   // 16 zero bytes per node
   ctx->urec_prev_ok = ctx->arec_prev_ok = true;
   return FSX_OK;
 }
@@ -494,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
     if (Rout) { Rout[i] = R; return; }   // consistent mass: the caller solves for Acc
     const double a = R * MdInv[r];
     Acc[i] = a;
-    if (Aout) Aout[4 * (size_t)r + l] = (float)a;   // the FP32 shadow step1 reads
+    if (Aout) Aout[8 * (size_t)r + l] = (float)a;   // the FP32 shadow step1 reads
     Unp1[i] = UnpHalf[i] + dt * a;
   }
 }
