@@ -65,6 +65,7 @@ def log(msg):
 
 
 X0_POLICY = {"zero": 0, "previous": 1}
+GRAPHS = 0   # --graphs
 
 
 def run_mode(solver, mode, steps, warmup, rtol, maxit, matrix_bits=64, x0="zero"):
@@ -73,6 +74,7 @@ def run_mode(solver, mode, steps, warmup, rtol, maxit, matrix_bits=64, x0="zero"
     pressure solves' initial guess (zero, or the previous Pdot)."""
     import numpy as np
     solver.reset(mode=mode, rtol=rtol, maxit=maxit)
+    solver.set_graphs(GRAPHS)
     solver.set_solver_x0(X0_POLICY[x0])
     solver.set_matrix_precision(matrix_bits)
     solver.set_convection_precision(matrix_bits)
@@ -231,9 +233,13 @@ def main():
                          "GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
     ap.add_argument("--workload", default="cavity", choices=("cavity", "duct"),
                     help="duct: the bent-duct case with consistent-mass updates, printed as a line of its own")
+    ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
+                    help="enqueue the solver iterations as replayed HIP graphs (1) or as individual launches (0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pcg-iters", type=int, default=30)
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = args.graphs
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the path is single-GPU, "
               f"see the sharding design in DESIGN.md)", file=sys.stderr)

@@ -67,6 +67,23 @@ __device__ __forceinline__ void write_partials(double (&v)[N], double* partials)
     for (int n = 0; n < N; n++) partials[n * kRedGrid + blockIdx.x] = v[n];
 }
 
+// n partial sums in a fixed order (thread t adds entries t, t + kBlock, ...,
+// then the block tree), for blocks of kBlock threads; every thread receives
+// the total. For n = kRedGrid it equals read_partials<1>.
+__device__ __forceinline__ double sum_parts(const double* __restrict__ parts, int n) {
+  __shared__ double red[kBlock / kWave];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};   // four loads in flight a lane
+  int i = threadIdx.x;
+  for (; i + 3 * kBlock < n; i += 4 * kBlock) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] += parts[i + u * kBlock];
+  }
+  for (int u = 0; i < n; i += kBlock, u++) a[u] += parts[i];
+  double v[1] = {(a[0] + a[1]) + (a[2] + a[3])};
+  block_sum_all<1>(v, red);
+  return v[0];
+}
+
 // partials[n * kRedGrid + b], b < kRedGrid, summed in a fixed order by a
 // block of BS >= kRedGrid threads (threads past kRedGrid add exact zeros, so
 // the sum does not depend on BS).

@@ -301,6 +301,7 @@ int launch_build_Z(fsx_ctx* ctx);
 int launch_convection(fsx_ctx* ctx);
 int launch_step1(fsx_ctx* ctx, int iter);
 int records_begin_step(fsx_ctx* ctx);
+int ensure_fpq(fsx_ctx* ctx);
 int direct_factor(fsx_ctx* ctx);
 int direct_apply(fsx_ctx* ctx, const double* b, double* x);
 inline void drop_records(fsx_ctx* ctx) {

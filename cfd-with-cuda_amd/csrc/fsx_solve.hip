@@ -631,8 +631,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
 // interior of a structured grid). With XCD, consecutive blocks of chunks go
 // to one XCD (blocks are dealt to the 8 XCDs round robin), so each XCD's L2
 // holds one slab of p instead of all of it.
-// With pq non-null each wavefront also writes its chunk's p.q (FP64 sum of
-// FP32 products) to pq[chunk]: the flexible CG takes p.Zp from these.
+// pq[b] = the block's sum of v over its lanes, wavefronts in order (every
+// wavefront of the block calls it once)
+__device__ __forceinline__ void block_pq(double v, int b, double* __restrict__ pq) {
+  __shared__ double wsum[kBlock / kWave];
+  v = group_sum<kWave>(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlock / kWave; i++) t += wsum[i];
+    pq[b] = t;
+  }
+}
+
+// With pq non-null each block also writes its chunks' p.q (FP64 sum of FP32
+// products, its wavefronts in order) to pq[block]: the solvers take p.Zp
+// from these instead of a separate dot pass.
 template <int U, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __restrict__ chunk,
                                                       const int* __restrict__ rows, const int2* __restrict__ ent,
@@ -642,7 +657,10 @@ __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __
   int b = blockIdx.x;
   if constexpr (XCD) b = (b % 8) * (gridDim.x / 8) + b / 8;   // gridDim.x is a multiple of 8
   const int w = b * (kBlock / kWave) + threadIdx.x / kWave;
-  if (w >= nchunk) return;
+  if (w >= nchunk) {
+    if (pq) block_pq(0.0, b, pq);
+    return;
+  }
   const int lane = threadIdx.x & (kWave - 1);
   const int4 c = chunk[w];
   const int rs = __builtin_amdgcn_readfirstlane(c.x), len = __builtin_amdgcn_readfirstlane(c.y);
@@ -666,10 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __
     s += __int_as_float(t.y) * p[r + t.x];
   }
   if (lane < len) q[r] = s;
-  if (pq) {
-    const double v = group_sum<kWave>(lane < len ? (double)p[r] * (double)s : 0.0);
-    if (lane == 0) pq[w] = v;
-  }
+  if (pq) block_pq(lane < len ? (double)p[r] * (double)s : 0.0, b, pq);
 }
 
 // The solvers' Z (structural zeros dropped): FP64 rows through the pattern
@@ -779,16 +794,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dot(int n, const T* __restrict__
 }
 
 template <class T>
+// p.q from npq partial sums: k_pcg_dot's kRedGrid (slot 0 of partials) or
+// the FP32 row-class SpMV's one per chunk
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int j, int n, T* __restrict__ x, T* __restrict__ r,
                                                        T* __restrict__ z, const T* __restrict__ p, const T* __restrict__ q,
                                                        const T* __restrict__ dinv, double* __restrict__ partials,
-                                                       SolveScalars* sc) {
+                                                       SolveScalars* sc, const double* __restrict__ pqp, int npq) {
   const int stop = sc->stop;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->active_v[j & 1] = stop ? 0 : 1;
   if (stop) return;
-  double pq[1];
-  read_partials<1>(partials, pq);
-  const T a = (T)(sc->rz_v[j & 1] / pq[0]);
+  const double pq = sum_parts(pqp, npq);
+  const T a = (T)(sc->rz_v[j & 1] / pq);
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     x[i] += a * p[i];
@@ -828,15 +844,44 @@ __global__ __launch_bounds__(kBlock) void k_pcg_direction(int j, int n, const T*
   }
 }
 
+int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc, double* pq, int* npq);
+
+// The FP32 SpMV's p.q partial sums (one per row-class chunk, or per block of
+// a 256-block grid); the captured solver iterations hold the buffer, so they
+// are dropped when it is reallocated.
+int ensure_fpq(fsx_ctx* ctx) {
+  const int need = std::max(ctx->Zc_chunk ? ctx->Zc_nchunk : 0, kRedGrid);   // >= the SpMV's block count
+  if (ctx->fpq_n >= need) return FSX_OK;
+  if (ctx->fpq) dev_free(ctx, ctx->fpq);
+  ctx->fpq = nullptr;
+  ctx->fpq_n = 0;
+  FSX_TRY(alloc(ctx, &ctx->fpq, (size_t)need));
+  ctx->fpq_n = need;
+  for (IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) iter_graph_reset(*g);
+  return FSX_OK;
+}
+
 template <class T>
 static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* dinv, T* x, T* r, T* z, T* p, T* q,
                                  int kind, double delta2) {
   constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
   const int n = ctx->NNp;
-  launch_zspmv<T>(ctx, p, q, ctx->sc);
-  hipLaunchKernelGGL(k_pcg_dot<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, p, q, ctx->partials, ctx->sc);
+  const double* pqp = ctx->partials;
+  int npq = kRedGrid;
+  bool fused = false;
+  if constexpr (sizeof(T) == 4) {
+    if (ctx->Zc_chunk && ctx->fpq) {   // the FP32 row-class SpMV writes p.q per chunk: no separate dot pass
+      (void)launch_pcg_spmv_pq_f32(ctx, p, q, ctx->sc, ctx->fpq, &npq);
+      pqp = ctx->fpq;
+      fused = true;
+    }
+  }
+  if (!fused) {
+    launch_zspmv<T>(ctx, p, q, ctx->sc);
+    hipLaunchKernelGGL(k_pcg_dot<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, p, q, ctx->partials, ctx->sc);
+  }
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, x, r, z, p, q, dinv,
-                     ctx->partials, ctx->sc);
+                     ctx->partials, ctx->sc, pqp, npq);
   if (kind == 0)
     hipLaunchKernelGGL((k_pcg_direction<T, 0>), dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, z, p, ctx->partials,
                        delta2, ctx->sc);
@@ -1047,6 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_replace_finish(int j, const doubl
 static int solve_mixed(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   const int n = ctx->NNp;
   const double delta = 1e-2;
+  FSX_TRY(ensure_fpq(ctx));
   FSX_TRY(launch_fill(ctx, x, 0.0, n));
   hipLaunchKernelGGL(k_pcg_init<float>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->Zdinvf, ctx->fx, ctx->fr,
                      ctx->fz, ctx->fp, ctx->partials);
@@ -1118,7 +1164,7 @@ int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveSc
     const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
     hipLaunchKernelGGL((k_zcls_spmv<kClsU, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
                        ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, pq);
-    *npq = ctx->Zc_nchunk;
+    *npq = (int)blocks;
     return FSX_OK;
   }
   launch_zspmv<float>(ctx, p, q, sc);

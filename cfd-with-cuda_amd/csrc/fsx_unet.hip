@@ -796,15 +796,6 @@ namespace fsx {
 
 constexpr int kSlotRP = 1;
 
-// the npq partial sums of p.Zp in a fixed order; every thread gets the total
-__device__ __forceinline__ double sum_pq(const double* __restrict__ pq, int npq) {
-  __shared__ double red[kBlock / kWave];
-  double v[1] = {0.0};
-  for (int i = threadIdx.x; i < npq; i += kBlock) v[0] += pq[i];
-  block_sum_all<1>(v, red);
-  return v[0];
-}
-
 // alpha = r.p / p.q (0 when p is not a descent direction, which
 // k_fpcg_direction then reports); x += alpha p, r -= alpha q; partials of
 // r.p (the direction before its update) and r.r
@@ -815,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_fpcg_update(int j, int n, float* __r
   const int stop = sc->stop;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->active_v[j & 1] = stop ? 0 : 1;
   if (stop) return;
-  const double pqv = sum_pq(pq, npq), rp = sc->rp_v[j & 1];
+  const double pqv = sum_parts(pq, npq), rp = sc->rp_v[j & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq_v[j & 1] = pqv;
   const float a = (pqv > 0.0 && rp > 0.0) ? (float)(rp / pqv) : 0.0f;
   double v[2] = {0.0, 0.0};
@@ -976,17 +967,7 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
   // (b = 0 sets the stop flag here: every iteration is then a no-op and the
   // first FP64 check finds the zero residual; no read-back before the start)
   hipLaunchKernelGGL(k_learned_init, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, max_learned);
-  // the SpMV's p.Zp partial sums: one per row-class chunk, or per block of
-  // a 256-block grid
-  const int npq_max = std::max(ctx->Zc_chunk ? ctx->Zc_nchunk : 0, kRedGrid);
-  if (ctx->fpq_n < npq_max) {
-    if (ctx->fpq) dev_free(ctx, ctx->fpq);
-    ctx->fpq = nullptr;
-    ctx->fpq_n = 0;
-    FSX_TRY(alloc(ctx, &ctx->fpq, (size_t)npq_max));
-    ctx->fpq_n = npq_max;
-    iter_graph_reset(ctx->graph_learned);   // the captured iterations hold the old buffer
-  }
+  FSX_TRY(ensure_fpq(ctx));
   // start: r = b, z = M r, p = z, r.p = r.z
   hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->fr, ctx->fx, ctx->fp,
                      ctx->partials);
