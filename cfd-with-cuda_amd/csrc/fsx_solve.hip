@@ -789,6 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dot(int n, const T* __restrict__
                                                     double* __restrict__ partials, const SolveScalars* __restrict__ sc) {
   if (sc->stop) return;
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += (double)p[i] * (double)q[i];
   write_partials<1>(v, partials);
 }
@@ -806,6 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int j, int n, T* __restri
   const double pq = sum_parts(pqp, npq);
   const T a = (T)(sc->rz_v[j & 1] / pq);
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     x[i] += a * p[i];
     const T ri = r[i] - a * q[i];
@@ -831,6 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_direction(int j, int n, const T*
   const double rz_old = sc->rz_v[j & 1];
   const bool bad = !(s[0] == s[0]) || !(s[1] == s[1]) || !(rz_old == rz_old) || rz_old == 0.0;
   const T be = bad ? T(0) : (T)(s[1] / rz_old);
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) p[i] = z[i] + be * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const int it = sc->iters + 1;

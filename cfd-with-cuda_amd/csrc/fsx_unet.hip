@@ -810,6 +810,7 @@ __global__ __launch_bounds__(kBlock) void k_fpcg_update(int j, int n, float* __r
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq_v[j & 1] = pqv;
   const float a = (pqv > 0.0 && rp > 0.0) ? (float)(rp / pqv) : 0.0f;
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const float pi = p[i];
     x[i] += a * pi;
@@ -836,6 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_fpcg_direction(int j, int n, const f
   const bool bad = !(pq == pq) || !(rp == rp) || !(rr == rr) || !(rz == rz) || !(zq == zq) || !(rpo == rpo);
   const bool lost = !(pq > 0.0) || !(rp > 0.0) || !(rz > 0.0);
   const float be = (bad || lost) ? 0.f : (float)(-zq / pq);
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) p[i] = z[i] + be * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->rp_v[(j + 1) & 1] = rz + (double)be * rpo;
