@@ -622,15 +622,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
   if (r < n && l == 0) q[r] = s;
 }
 
-// q = Z p on the FP32 row classes: row r sums the entries of its class,
-// (offset o, value a) pairs, as a * p[r + o], in entry order. The class table
-// is a few tens of KB and stays in cache, so a launch moves little more than
-// the row list, p and q. One wavefront per chunk of up to 64 rows of one
-// class: the entries are wave-uniform and come through scalar loads, and
-// entry k's gathers p[r + o_k] follow the chunk's rows (contiguous for the
-// interior of a structured grid). With XCD, consecutive blocks of chunks go
-// to one XCD (blocks are dealt to the 8 XCDs round robin), so each XCD's L2
-// holds one slab of p instead of all of it.
 // pq[b] = the block's sum of v over its lanes, wavefronts in order (every
 // wavefront of the block calls it once)
 __device__ __forceinline__ void block_pq(double v, int b, double* __restrict__ pq) {
@@ -645,6 +636,17 @@ __device__ __forceinline__ void block_pq(double v, int b, double* __restrict__ p
   }
 }
 
+// q = Z p on the FP32 row classes: row r sums the entries of its class,
+// (offset o, value a) pairs, as a * p[r + o], in entry order. The class table
+// is a few tens of KB and stays in cache, so a launch moves little more than
+// the row list, p and q. One wavefront per chunk of up to 64 rows of one
+// class: the entries are wave-uniform and come through scalar loads, and
+// entry k's gathers p[r + o_k] follow the chunk's rows (contiguous for the
+// interior of a structured grid). With XCD, consecutive blocks of chunks go
+// to one XCD (blocks are dealt to the 8 XCDs round robin), so each XCD's L2
+// holds one slab of p instead of all of it; on the 64^3 cavity the launch
+// order measured faster (9.05 us against 9.4 with the remap; U = 4, 16, 32
+// with the remap: 10.1, 9.3, 10.1), so the solvers launch it without.
 // With pq non-null each block also writes its chunks' p.q (FP64 sum of FP32
 // products, its wavefronts in order) to pq[block]: the solvers take p.Zp
 // from these instead of a separate dot pass.
@@ -702,7 +704,7 @@ static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc)
     if (ctx->Zc_chunk) {
       const int wpb = kBlock / kWave;
       const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
-      hipLaunchKernelGGL((k_zcls_spmv<kClsU, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
+      hipLaunchKernelGGL((k_zcls_spmv<kClsU, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
                          ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, nullptr);
       return;
     }
@@ -1165,7 +1167,7 @@ int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveSc
   if (ctx->Zc_chunk) {
     const int wpb = kBlock / kWave;
     const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
-    hipLaunchKernelGGL((k_zcls_spmv<kClsU, true>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
+    hipLaunchKernelGGL((k_zcls_spmv<kClsU, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
                        ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, pq);
     *npq = (int)blocks;
     return FSX_OK;
