@@ -365,8 +365,10 @@ int direct_factor(fsx_ctx* ctx) {
                                  " GB, above the 64 GiB limit");
   std::vector<int> perm(n);
   for (int r = 0; r < n; r++) perm[iperm[r]] = r;
-  // (re)allocate when the shape changed
-  if (!ctx->band || ctx->band_n != np || ctx->band_W != W) {
+  // (re)allocate when the shape changed, or after an allocation failed part way
+  const bool have_all = ctx->band && ctx->band_up && ctx->band_inv && ctx->band_vec && ctx->band_part &&
+                        ctx->band_perm && ctx->band_iperm && ctx->band_flag;
+  if (!have_all || ctx->band_n != np || ctx->band_W != W || ctx->band_nn != n) {
     for (void* p : {(void*)ctx->band, (void*)ctx->band_up, (void*)ctx->band_inv, (void*)ctx->band_vec,
                     (void*)ctx->band_part, (void*)ctx->band_perm, (void*)ctx->band_iperm, (void*)ctx->band_flag})
       if (p) dev_free(ctx, p);
@@ -381,7 +383,7 @@ int direct_factor(fsx_ctx* ctx) {
     FSX_TRY(alloc(ctx, &ctx->band_iperm, (size_t)n));
     FSX_TRY(alloc(ctx, &ctx->band_flag, 1));
   }
-  ctx->band_n = np; ctx->band_W = W; ctx->band_w = w;
+  ctx->band_n = np; ctx->band_W = W; ctx->band_w = w; ctx->band_nn = n;   // the shape the buffers are sized for
   FSX_HIP(hipMemcpy(ctx->band_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
   FSX_HIP(hipMemcpy(ctx->band_iperm, iperm.data(), n * sizeof(int), hipMemcpyHostToDevice));
   const int zero = 0;

@@ -166,7 +166,8 @@ struct fsx_ctx {
   int* band_iperm = nullptr;     // pressure id -> band order
   int* band_flag = nullptr;      // set at a non-positive pivot
   long long band_W = 0;          // row stride
-  int band_w = -1, band_n = 0;   // half-bandwidth, order padded to a multiple of 256
+  int band_w = -1, band_n = 0;   // half-bandwidth, order padded to a multiple of 1024
+  int band_nn = 0;               // NNp the permutations are sized for
   bool band_ready = false;       // the factor is of the current Z
   double band_factor_ms = 0.0;
   bool urec_ok = false, urec_prev_ok = false, arec_ok = false, arec_prev_ok = false;
