@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_init(int NN, const double* __re
                                                       double* __restrict__ z, double* __restrict__ p,
                                                       double* __restrict__ partials) {
   double v[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock) {
     const double di = dinv[i];
 #pragma unroll
@@ -119,6 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_dot(int NN, const double* __res
                                                      double* __restrict__ partials, const MassScalars* __restrict__ sc) {
   if (sc->stop) return;
   double v[3] = {0, 0, 0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock)
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] += p[3 * (size_t)i + c] * q[3 * (size_t)i + c];
@@ -138,6 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_update(int j, int NN, double* _
 #pragma unroll
   for (int c = 0; c < 3; c++) a[c] = pq[c] > 0.0 ? sc->rz[j & 1][c] / pq[c] : 0.0;
   double v[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock) {
     const double di = dinv[i];
 #pragma unroll
@@ -168,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_direction(int j, int NN, const 
     const double old = sc->rz[j & 1][c];
     be[c] = old != 0.0 ? s[3 + c] / old : 0.0;
   }
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock)
 #pragma unroll
     for (int c = 0; c < 3; c++) {

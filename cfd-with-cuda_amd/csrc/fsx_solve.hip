@@ -421,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_pnp1(int NNp, const double* __restri
                                                  const double* __restrict__ Pprev, double* __restrict__ Pnp1, double dt,
                                                  double* __restrict__ partials) {
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NNp; i += kRedGrid * kBlock) {
     const double p = Pn[i] + dt * Pdot[i];
     Pnp1[i] = p;
@@ -434,6 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_pnp1(int NNp, const double* __restri
 __global__ __launch_bounds__(kBlock) void k_vnorms(long long n, const double* __restrict__ u, const double* __restrict__ uprev,
                                                    double* __restrict__ partials) {
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)kRedGrid * kBlock) {
     const double a = u[i], d = a - uprev[i];
     v[0] += a * a; v[1] += d * d;
@@ -543,6 +545,7 @@ int launch_step3(fsx_ctx* ctx, int iter) {
 __global__ __launch_bounds__(kBlock) void k_maxacc(long long n, const double* __restrict__ Unp1, const double* __restrict__ Un,
                                                    double oneOverdt, double* __restrict__ partials) {
   double m = 0.0;
+#pragma unroll 4   // independent elements: their loads in flight together
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)kRedGrid * kBlock)
     m = fmax(m, fabs((Unp1[i] - Un[i]) * oneOverdt));
   m = block_max_all(m);
@@ -762,6 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __rest
                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                      T* __restrict__ p, double* __restrict__ partials) {
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const T bi = (T)b[i];
     const T zi = dinv[i] * bi;
@@ -961,6 +965,7 @@ static int solve_fp64(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
 // search direction is kept, so the Krylov information is not lost as in a
 // restarted refinement, and the iteration count stays that of FP64 CG.
 __global__ __launch_bounds__(kBlock) void k_mx_fold(int n, double* __restrict__ xh, float* __restrict__ xl) {
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     xh[i] += (double)xl[i];
     xl[i] = 0.0f;
@@ -1057,6 +1062,7 @@ static void launch_zresidual(fsx_ctx* ctx, const double* xh, const double* b, do
 
 __global__ __launch_bounds__(kBlock) void k_sumsq(int n, const double* __restrict__ a, double* __restrict__ partials) {
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += a[i] * a[i];
   write_partials<1>(v, partials);
 }
@@ -1079,6 +1085,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_replace(int n, const double* __re
                                                        float* __restrict__ r, float* __restrict__ z,
                                                        double* __restrict__ partials) {
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const float ri = (float)rh[i];
     const float zi = dinv[i] * ri;
@@ -1156,6 +1163,7 @@ __global__ __launch_bounds__(kBlock) void k_fpq_dot(int n, const float* __restri
                                                     double* __restrict__ pq, const SolveScalars* __restrict__ sc) {
   if (sc->stop) return;
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += (double)p[i] * (double)q[i];
   write_partials<1>(v, pq);
 }
@@ -1220,6 +1228,7 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
 __global__ __launch_bounds__(kBlock) void k_sumsq2(int n, const double* __restrict__ b, const double* __restrict__ r,
                                                    double* __restrict__ partials) {
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     v[0] += b[i] * b[i];
     v[1] += r[i] * r[i];

@@ -501,6 +501,7 @@ __global__ void k_scale_f(long long n, float* __restrict__ a, float s) {
 __global__ __launch_bounds__(kBlock) void k_loss(int n, const float* __restrict__ y, const float* __restrict__ tgt, double tt,
                                                  float* __restrict__ gy, double* __restrict__ partials) {
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const float d = y[i] - tgt[i];
     gy[i] = (float)(2.0 * d / tt);
@@ -514,6 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_loss(int n, const float* __restrict_
 __global__ __launch_bounds__(kBlock) void k_eloss(int n, const float* __restrict__ q, double tt, float* __restrict__ gy,
                                                   double* __restrict__ partials) {
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const float d = gy[i];
     v[0] += (double)d * (double)q[i];
@@ -529,6 +531,7 @@ __global__ void k_sub_f(int n, const float* __restrict__ a, const float* __restr
 __global__ __launch_bounds__(kBlock) void k_dot_f(int n, const float* __restrict__ a, const float* __restrict__ b,
                                                   double* __restrict__ partials) {
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += (double)a[i] * (double)b[i];
   write_partials<1>(v, partials);
 }
@@ -875,6 +878,7 @@ __global__ __launch_bounds__(kBlock) void k_fpcg_load_r(int n, const double* __r
                                                         float* __restrict__ x, const float* __restrict__ p,
                                                         double* __restrict__ partials) {
   double v[2] = {0.0, 0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const float ri = (float)rh[i];
     r[i] = ri;
@@ -891,6 +895,7 @@ __global__ void k_zero_f(int n, float* __restrict__ a) {
 
 __global__ __launch_bounds__(kBlock) void k_sumsq_u(int n, const double* __restrict__ a, double* __restrict__ partials) {
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += a[i] * a[i];
   write_partials<1>(v, partials);
 }
@@ -919,6 +924,7 @@ __global__ __launch_bounds__(kBlock) void k_make_sample(int n, const double* __r
   read_partials<1>(partials, rr);
   const double inv_s = rr[0] > 0.0 ? 1.0 / std::sqrt(rr[0] / n) : 0.0;
   double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
     const double r = rh[i], e = xs[i] - xk[i];
     in[i] = (float)(r * inv_s);
