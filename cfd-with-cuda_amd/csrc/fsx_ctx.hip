@@ -817,6 +817,7 @@ int fsx_next_iteration(fsx_ctx* ctx) {
   std::swap(ctx->urec_prev_ok, ctx->urec_ok);
   std::swap(ctx->arec_prev_ok, ctx->arec_ok);
   ctx->drec_ok = false;   // step1 of the next inner iteration writes it again
+  ctx->ucmp_ok = false;   // UnpHalf_prev is no longer Un
   FSX_TRY(launch_copy(ctx, ctx->Unp1_prev, ctx->Unp1, 3LL * ctx->NN));
   FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pnp1, ctx->NNp));
   return FSX_OK;
@@ -843,6 +844,7 @@ int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
   std::swap(ctx->Un, ctx->Unp1);                                           // BCH:3070-3077
   std::swap(ctx->Pn, ctx->Pnp1);
   ctx->gpn_valid = false;                                                  // G Pn changes with Pn
+  ctx->ucmp_ok = false;                                                    // and Un's shadow with Un
   return FSX_OK;
 }
 

@@ -176,6 +176,10 @@ struct fsx_ctx {
   // drec_ok, with the KAcc term iff drec_kacc
   float* Drec = nullptr;
   bool drec_ok = false, drec_kacc = false;
+  // compact FP32 shadow of Un (4 floats a node), the first inner iteration's
+  // K operand (UnpHalf_prev = Un then); valid until the next inner iteration
+  float* Ucmp = nullptr;
+  bool ucmp_ok = false;
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
@@ -306,7 +310,7 @@ int ensure_fpq(fsx_ctx* ctx);
 int direct_factor(fsx_ctx* ctx);
 int direct_apply(fsx_ctx* ctx, const double* b, double* x);
 inline void drop_records(fsx_ctx* ctx) {
-  ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = ctx->drec_ok = false;
+  ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = ctx->drec_ok = ctx->ucmp_ok = false;
 }
 int launch_step2_rhs(fsx_ctx* ctx, int iter);
 int launch_pnp1(fsx_ctx* ctx);

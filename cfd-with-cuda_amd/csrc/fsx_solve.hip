@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
                                                   double* __restrict__ GPn, const float4* __restrict__ Urp,
                                                   const float4* __restrict__ Arp, float* __restrict__ Uout,
                                                   const double* __restrict__ MdOrigInv, double o2,
-                                                  float* __restrict__ Dout) {
+                                                  float* __restrict__ Dout, int us) {
   int l;
   const long long r = my_row<LPR>(&l);
   double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
 #pragma unroll
       for (int j = 0; j < UK; j++) {
         if constexpr (REC) {
-          const float4 u = Urp[2 * (size_t)c[j]];
+          const float4 u = Urp[(size_t)us * c[j]];
           k0 += a[j] * (double)u.x; k1 += a[j] * (double)u.y; k2 += a[j] * (double)u.z;
           if constexpr (KACC) {
             const float4 v = Arp[2 * (size_t)c[j]];
@@ -186,7 +186,11 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   // rewrites UnpHalf afterwards), read when they hold the prev vectors
   const bool wrec = f32 && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT;
   const bool rrec = wrec && ctx->urec_prev_ok && (!kacc || ctx->arec_prev_ok);
-  const float4* urp = reinterpret_cast<const float4*>(ctx->Urec_prev);
+  // the first inner iteration (no K Acc) reads the compact shadow of Un:
+  // the interleaved records would bring Acc's unused half along
+  const bool cmp = rrec && !kacc && ctx->ucmp_ok;
+  const float4* urp = reinterpret_cast<const float4*>(cmp ? ctx->Ucmp : ctx->Urec_prev);
+  const int us = cmp ? 1 : 2;   // float4 records a node
   const float4* arp = reinterpret_cast<const float4*>(ctx->Arec_prev);
   float* uout = wrec ? ctx->Urec : nullptr;
   float* dout = wrec ? ctx->Drec : nullptr;
@@ -195,7 +199,7 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT, RC>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,  \
                      mcs, KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,      \
                      ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr,       \
-                     ctx->GPn, urp, arp, uout, ctx->MdOrigInv, o2, dout)
+                     ctx->GPn, urp, arp, uout, ctx->MdOrigInv, o2, dout, us)
 #define STEP1_LAUNCH(KA, KP, GM) \
   do { if (rrec) STEP1_LAUNCH_T(KA, KP, GM, float, true, ctx->Kf, ctx->Gf); \
        else if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, false, ctx->Kf, ctx->Gf); \
@@ -239,12 +243,13 @@ int refresh_matrix_copies(fsx_ctx* ctx) {
   }
   if (!ctx->rec_mem) {
     const size_t n4 = 4 * (size_t)ctx->NN;
-    FSX_TRY(alloc(ctx, &ctx->rec_mem, 5 * n4));
+    FSX_TRY(alloc(ctx, &ctx->rec_mem, 6 * n4));
     // UnpHalf's and Acc's shadows interleaved, 8 floats a node (u v w - a b c -),
     // so that step1's two gathers of a column fall in one 32-byte run
     ctx->Urec = ctx->rec_mem; ctx->Arec = ctx->rec_mem + 4;
     ctx->Urec_prev = ctx->rec_mem + 2 * n4; ctx->Arec_prev = ctx->rec_mem + 2 * n4 + 4;
     ctx->Drec = ctx->rec_mem + 4 * n4;
+    ctx->Ucmp = ctx->rec_mem + 5 * n4;
   }
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzM, ctx->K, ctx->Kf_mem);
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->G, ctx->Gf_mem);
@@ -255,11 +260,11 @@ int refresh_matrix_copies(fsx_ctx* ctx) {
 }
 
 // FP32 shadow of a node-major velocity vector (zeros for a null v): node i
-// at rec[2 i], in the interleaved 8-float records
-__global__ void k_rec(long long nn, const double* __restrict__ v, float4* __restrict__ rec) {
+// at rec[stride i] (stride 2 in the interleaved 8-float records, 1 compact)
+__global__ void k_rec(long long nn, const double* __restrict__ v, float4* __restrict__ rec, int stride) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x)
-    rec[2 * i] = v ? make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], 0.0f)
-                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    rec[stride * i] = v ? make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], 0.0f)
+                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // The shadows of UnpHalf_prev = Un and Acc_prev = 0 that fsx_begin_step sets
@@ -268,11 +273,13 @@ int records_begin_step(fsx_ctx* ctx) {
   drop_records(ctx);
   if (!ctx->Kf || !ctx->rec_mem || ctx->mass_mode == FSX_MASS_CONSISTENT) return FSX_OK;
   hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
-                     reinterpret_cast<float4*>(ctx->Urec_prev));
+                     reinterpret_cast<float4*>(ctx->Urec_prev), 2);
   hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, nullptr,
-                     reinterpret_cast<float4*>(ctx->Arec_prev));
+                     reinterpret_cast<float4*>(ctx->Arec_prev), 2);
+  hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
+                     reinterpret_cast<float4*>(ctx->Ucmp), 1);
   FSX_TRY(launch_check(ctx, "k_rec"));
-  ctx->urec_prev_ok = ctx->arec_prev_ok = true;
+  ctx->urec_prev_ok = ctx->arec_prev_ok = ctx->ucmp_ok = true;
   return FSX_OK;
 }
 
@@ -1390,9 +1397,9 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   if (kernel == FSX_KERNEL_STEP1 && ctx->Kf && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT) {
     // the FP32 shadows of UnpHalf_prev and Acc_prev, as the inner iterations have them
     hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->UnpHalf_prev,
-                       reinterpret_cast<float4*>(ctx->Urec_prev));
+                       reinterpret_cast<float4*>(ctx->Urec_prev), 2);
     hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Acc_prev,
-                       reinterpret_cast<float4*>(ctx->Arec_prev));
+                       reinterpret_cast<float4*>(ctx->Arec_prev), 2);
     FSX_TRY(launch_check(ctx, "k_rec"));
     ctx->urec_prev_ok = ctx->arec_prev_ok = true;
   }
