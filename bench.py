@@ -241,8 +241,8 @@ def main():
     global GRAPHS
     GRAPHS = args.graphs
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the path is single-GPU, "
-              f"see the sharding design in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the solver core is built "
+              f"and measured for one GPU; sharding by mesh partitions is designed, not built)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import fsx
