@@ -53,7 +53,7 @@ def step_kernels(vt):
 def zspmv_name(prec, npat, ncls):
     """rocprofv3 name of the Z SpMV kernel the solvers run in this form."""
     if prec and ncls > 0:
-        return "k_zcls_spmv<8, false>"
+        return "k_zcls_spmv<8, false, 2>"
     if npat > 0:   # the row-pattern form runs 8 lanes x 16 loads per row in both precisions
         return "k_pcg_spmv<float, 8, 16, true>" if prec else "k_pcg_spmv<double, 8, 16, true>"
     return "k_pcg_spmv<float, 32, 4, false>" if prec else "k_pcg_spmv<double, 16, 8, false>"

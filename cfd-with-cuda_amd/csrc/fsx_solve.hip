@@ -660,43 +660,56 @@ __device__ __forceinline__ void block_pq(double v, int b, double* __restrict__ p
 // With pq non-null each block also writes its chunks' p.q (FP64 sum of FP32
 // products, its wavefronts in order) to pq[block]: the solvers take p.Zp
 // from these instead of a separate dot pass.
-template <int U, bool XCD>
+template <int U, bool XCD, int S>
 __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __restrict__ chunk,
                                                       const int* __restrict__ rows, const int2* __restrict__ ent,
                                                       const float* __restrict__ p, float* __restrict__ q,
                                                       const SolveScalars* __restrict__ sc, double* __restrict__ pq) {
+  static_assert((kBlock / kWave) % S == 0, "a chunk's wavefronts share a block");
   if (sc && sc->stop) return;
   int b = blockIdx.x;
   if constexpr (XCD) b = (b % 8) * (gridDim.x / 8) + b / 8;   // gridDim.x is a multiple of 8
-  const int w = b * (kBlock / kWave) + threadIdx.x / kWave;
-  if (w >= nchunk) {
-    if (pq) block_pq(0.0, b, pq);
-    return;
-  }
+  const int wv = threadIdx.x / kWave;
+  const int wg = __builtin_amdgcn_readfirstlane(b * (kBlock / kWave) + wv);
+  const int w = wg / S, h = wg % S;   // chunk, and this wavefront's share of its entries
   const int lane = threadIdx.x & (kWave - 1);
-  const int4 c = chunk[w];
-  const int rs = __builtin_amdgcn_readfirstlane(c.x), len = __builtin_amdgcn_readfirstlane(c.y);
-  const int es = __builtin_amdgcn_readfirstlane(c.z), el = __builtin_amdgcn_readfirstlane(c.w);
-  const int r = rows[rs + (lane < len ? lane : 0)];
-  const int2* __restrict__ e = ent + es;
   float s = 0.0f;
-  int k = 0;
-  for (; k + U <= el; k += U) {
-    int2 t[U];
-    float v[U];
+  int r = 0, len = 0;
+  if (w < nchunk) {
+    const int4 c = chunk[w];
+    const int rs = __builtin_amdgcn_readfirstlane(c.x);
+    len = __builtin_amdgcn_readfirstlane(c.y);
+    const int es = __builtin_amdgcn_readfirstlane(c.z), el = __builtin_amdgcn_readfirstlane(c.w);
+    r = rows[rs + (lane < len ? lane : 0)];
+    const int2* __restrict__ e = ent + es;
+    const int k1 = el * (h + 1) / S;
+    int k = el * h / S;
+    for (; k + U <= k1; k += U) {
+      int2 t[U];
+      float v[U];
 #pragma unroll
-    for (int j = 0; j < U; j++) t[j] = e[k + j];
+      for (int j = 0; j < U; j++) t[j] = e[k + j];
 #pragma unroll
-    for (int j = 0; j < U; j++) v[j] = p[r + t[j].x];
+      for (int j = 0; j < U; j++) v[j] = p[r + t[j].x];
 #pragma unroll
-    for (int j = 0; j < U; j++) s += __int_as_float(t[j].y) * v[j];
+      for (int j = 0; j < U; j++) s += __int_as_float(t[j].y) * v[j];
+    }
+    for (; k < k1; k++) {
+      const int2 t = e[k];
+      s += __int_as_float(t.y) * p[r + t.x];
+    }
   }
-  for (; k < el; k++) {
-    const int2 t = e[k];
-    s += __int_as_float(t.y) * p[r + t.x];
+  if constexpr (S > 1) {
+    // the chunk's S partial sums, added in wavefront order by its first one
+    __shared__ float part[kBlock / kWave][kWave];
+    part[wv][lane] = s;
+    __syncthreads();
+    if (h == 0)
+      for (int j = 1; j < S; j++) s += part[wv + j][lane];
   }
-  if (lane < len) q[r] = s;
-  if (pq) block_pq(lane < len ? (double)p[r] * (double)s : 0.0, b, pq);
+  const bool own = h == 0 && w < nchunk && lane < len;
+  if (own) q[r] = s;
+  if (pq) block_pq(own ? (double)p[r] * (double)s : 0.0, b, pq);
 }
 
 // The solvers' Z (structural zeros dropped): FP64 rows through the pattern
@@ -704,6 +717,13 @@ __global__ __launch_bounds__(kBlock) void k_zcls_spmv(int nchunk, const int4* __
 static ColSrc zr_cols(fsx_ctx* ctx) { return ColSrc{ctx->Zr_col, ctx->Zr_pat, ctx->Zr_off}; }
 
 constexpr int kClsU = 8;
+constexpr int kClsS = 2;   // wavefronts per chunk of the row-class SpMV
+
+// blocks of the row-class SpMV (a multiple of 8, one p.q partial each)
+static unsigned zcls_blocks(const fsx_ctx* ctx) {
+  const long long wpb = kBlock / kWave;
+  return (unsigned)(((long long)ctx->Zc_nchunk * kClsS + 8 * wpb - 1) / (8 * wpb) * 8);
+}
 
 // q = Z p: FP64 on the solvers' rows; FP32 on the row classes, or on the
 // FP32 copy of the full pattern when the rows do not compress
@@ -712,9 +732,8 @@ static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc)
   const int n = ctx->NNp;
   if constexpr (sizeof(T) == 4) {
     if (ctx->Zc_chunk) {
-      const int wpb = kBlock / kWave;
-      const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
-      hipLaunchKernelGGL((k_zcls_spmv<kClsU, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
+      hipLaunchKernelGGL((k_zcls_spmv<kClsU, false, kClsS>), dim3(zcls_blocks(ctx)), dim3(kBlock), 0, ctx->stream,
+                         ctx->Zc_nchunk,
                          ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, nullptr);
       return;
     }
@@ -866,7 +885,7 @@ int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveSc
 // a 256-block grid); the captured solver iterations hold the buffer, so they
 // are dropped when it is reallocated.
 int ensure_fpq(fsx_ctx* ctx) {
-  const int need = std::max(ctx->Zc_chunk ? ctx->Zc_nchunk : 0, kRedGrid);   // >= the SpMV's block count
+  const int need = std::max(ctx->Zc_chunk ? (int)zcls_blocks(ctx) : 0, kRedGrid);   // >= the SpMV's block count
   if (ctx->fpq_n >= need) return FSX_OK;
   if (ctx->fpq) dev_free(ctx, ctx->fpq);
   ctx->fpq = nullptr;
@@ -1180,9 +1199,8 @@ __global__ __launch_bounds__(kBlock) void k_fpq_dot(int n, const float* __restri
 // no row classes
 int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScalars* sc, double* pq, int* npq) {
   if (ctx->Zc_chunk) {
-    const int wpb = kBlock / kWave;
-    const unsigned blocks = (unsigned)((ctx->Zc_nchunk + 8 * wpb - 1) / (8 * wpb) * 8);
-    hipLaunchKernelGGL((k_zcls_spmv<kClsU, false>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
+    const unsigned blocks = zcls_blocks(ctx);
+    hipLaunchKernelGGL((k_zcls_spmv<kClsU, false, kClsS>), dim3(blocks), dim3(kBlock), 0, ctx->stream, ctx->Zc_nchunk,
                        ctx->Zc_chunk, ctx->Zc_rows, ctx->Zc_ent, p, q, sc, pq);
     *npq = (int)blocks;
     return FSX_OK;
