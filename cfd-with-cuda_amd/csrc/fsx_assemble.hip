@@ -158,12 +158,16 @@ int launch_assemble_static(fsx_ctx* ctx) {
                        ctx->Grow, ctx->Gcol, ctx->G);
     FSX_TRY(launch_check(ctx, "k_assemble"));
   }
+  FSX_TRY(launch_lump(ctx));
+  hipLaunchKernelGGL(k_gt_values, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzG, ctx->GTpos, ctx->G, ctx->GT);
+  return launch_check(ctx, "k_gt_values");
+}
+
+int launch_lump(fsx_ctx* ctx) {
   const int g = std::max(1, std::min((ctx->NN + kBlock - 1) / kBlock, 4096));
   hipLaunchKernelGGL(k_lump, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow, ctx->M, ctx->bcflag, ctx->Md,
                      ctx->MdOrigInv, ctx->MdInv);
-  FSX_TRY(launch_check(ctx, "k_lump"));
-  hipLaunchKernelGGL(k_gt_values, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzG, ctx->GTpos, ctx->G, ctx->GT);
-  return launch_check(ctx, "k_gt_values");
+  return launch_check(ctx, "k_lump");
 }
 
 // ------------------------------------------------------------ Z
@@ -228,6 +232,15 @@ int launch_build_Z(fsx_ctx* ctx) {
   const int g = std::max(1, std::min((ctx->NNp + kBlock - 1) / kBlock, 4096));
   hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z, ctx->Zf,
                      ctx->Zdinv, ctx->Zdinvf, ctx->zeroP);
+  return launch_check(ctx, "k_z_finish");
+}
+
+// The Jacobi inverses and the FP32 copy of Z values that already carry the
+// penalty pin (an uploaded Z): k_z_finish without pinning again.
+int launch_z_refresh(fsx_ctx* ctx) {
+  const int g = std::max(1, std::min((ctx->NNp + kBlock - 1) / kBlock, 4096));
+  hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z, ctx->Zf,
+                     ctx->Zdinv, ctx->Zdinvf, -1);
   return launch_check(ctx, "k_z_finish");
 }
 

@@ -979,8 +979,22 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
     for (size_t k = 0; k < n; k++) t[ctx->posM[k]] = host[k];
     FSX_HIP(hipMemcpy(p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
     if (id == FSX_VAL_K) FSX_TRY(refresh_matrix_copies(ctx));
+    if (id == FSX_VAL_M) {
+      // the lumped masses follow M (Z keeps the lumped mass it was built
+      // from until fsx_build_Z runs again)
+      FSX_TRY(launch_lump(ctx));
+      ctx->mass_dinv_ready = false;
+    }
   } else {
     FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
+    if (id == FSX_VAL_Z) {
+      // the uploaded values carry the penalty pin; every form the solvers
+      // read (Jacobi inverses, the FP32 copy, the rows without structural
+      // zeros, the row classes) is rebuilt from them
+      FSX_TRY(launch_z_refresh(ctx));
+      FSX_TRY(build_z_forms(ctx));
+      ctx->z_built = true;
+    }
   }
   return FSX_OK;
 }

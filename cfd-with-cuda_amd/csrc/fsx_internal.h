@@ -302,7 +302,9 @@ int row_grid(long long nrows, int rows_per_block);
 
 // kernel entry points implemented in the .hip files (host-side launchers)
 int launch_assemble_static(fsx_ctx* ctx);
+int launch_lump(fsx_ctx* ctx);
 int launch_build_Z(fsx_ctx* ctx);
+int launch_z_refresh(fsx_ctx* ctx);
 int launch_convection(fsx_ctx* ctx);
 int launch_step1(fsx_ctx* ctx, int iter);
 int records_begin_step(fsx_ctx* ctx);

@@ -3,6 +3,7 @@
 // time loop in the order of the reference's timeLoop (BCH:2815-3120).
 #include "fsx_host_internal.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -95,14 +96,16 @@ extern "C" int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats) {
     s->timeN++;
     s->t += s->dt;
     DRV_CHECK(fsx_begin_step(s->ctx));                                      // BCH:2865-2881
-    int iter, pcg_total = 0, it = 0;
-    double rel = 0.0;
+    int iter, pcg_total = 0, it = 0, noconv = 0;
+    double rel = 0.0, rel_max = 0.0;
     for (iter = 1; iter <= o.maxIter; iter++) {                            // BCH:2885
       DRV_CHECK(fsx_convection(s->ctx));                                     // step1 (BCH:3702)
       DRV_CHECK(fsx_step1(s->ctx, iter));
       int rc = fsx_step2(s->ctx, iter, &it, &rel);                           // BCH:2898
       if (rc != FSX_OK && rc != FSX_ENOCONV) { s->err = fsx_last_error(s->ctx); return -rc; }
+      if (rc == FSX_ENOCONV) noconv++;                                       // counted, not hidden
       pcg_total += it;
+      rel_max = std::max(rel_max, rel);
       DRV_CHECK(fsx_step3(s->ctx, iter));                                    // BCH:2911
       int conv = 0;
       double ru, rp;
@@ -128,6 +131,9 @@ extern "C" int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats) {
       std::memcpy(st.mon, mon, sizeof mon);
       st.wall_s = wall;
       st.pcg_relres = rel;
+      st.pcg_noconv = noconv;
+      st.pad_ = 0;
+      st.pcg_relres_max = rel_max;
     }
     if (steady) { s->steady = 1; return n + 1; }
   }

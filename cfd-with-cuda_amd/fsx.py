@@ -59,7 +59,8 @@ class Options(ctypes.Structure):
 class StepStats(ctypes.Structure):
     _fields_ = [("step", ctypes.c_int), ("inner_iters", ctypes.c_int), ("pcg_iters", ctypes.c_int),
                 ("steady", ctypes.c_int), ("time", ctypes.c_double), ("max_acc", ctypes.c_double),
-                ("mon", ctypes.c_double * 4), ("wall_s", ctypes.c_double), ("pcg_relres", ctypes.c_double)]
+                ("mon", ctypes.c_double * 4), ("wall_s", ctypes.c_double), ("pcg_relres", ctypes.c_double),
+                ("pcg_noconv", ctypes.c_int), ("pad_", ctypes.c_int), ("pcg_relres_max", ctypes.c_double)]
 
 
 class Timing(ctypes.Structure):
@@ -383,7 +384,8 @@ class Solver:
         if n < 0:
             raise FsxError(-n, self.L.fsxd_error(self.h).decode())
         return [dict(step=s.step, inner=s.inner_iters, pcg=s.pcg_iters, steady=s.steady, time=s.time,
-                     max_acc=s.max_acc, mon=list(s.mon), wall_s=s.wall_s, relres=s.pcg_relres) for s in stats[:n]]
+                     max_acc=s.max_acc, mon=list(s.mon), wall_s=s.wall_s, relres=s.pcg_relres,
+                     noconv=s.pcg_noconv, relres_max=s.pcg_relres_max) for s in stats[:n]]
 
     def reset(self, mode=None, rtol=None, maxit=None):
         """Back to the initial fields; optionally switch the pressure solver."""

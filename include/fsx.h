@@ -226,6 +226,12 @@ int fsx_monitor(fsx_ctx* ctx, int node, double* out4);
 
 /* ---------------------------------------------------------------- data */
 int fsx_download(fsx_ctx* ctx, int id, double* host, size_t count);
+/* State vectors and K, M, Z values (not G or the lumped masses). A K upload
+ * refreshes the FP32 copy the step kernels read; an M upload lumps it again
+ * (Md, MdOrigInv, MdInv: BCH:3262-3295; Z keeps the lumped mass it was built
+ * from until fsx_build_Z runs again); a Z upload (values with the penalty
+ * pin, as fsx_download returns them) rebuilds every form the pressure
+ * solvers read and makes Z current without fsx_build_Z. */
 int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count);
 int fsx_sync(fsx_ctx* ctx);
 /* solve Z x = b on the context's Z with the current solver settings; b and
@@ -307,11 +313,19 @@ int fsx_timing_get(fsx_ctx* ctx, fsx_timing* out);
  * z = dinv r + U(r), falling back to Jacobi-PCG when the flexible recurrence
  * stalls. The pressure nodes must be numbered as a structured nx*ny*nz grid,
  * x fastest (the corner numbering of the structured generators), with
- * nx, ny, nz = 4m+1. */
+ * nx, ny, nz = 8m+1 (m >= 1: three node-aligned coarsenings, 65 -> 33 -> 17
+ * -> 9 nodes per axis on the 64^3 cavity). The network is
+ * J^T J with J = three stride-2 3x3x3 convolutions C0 (1 -> 16 channels),
+ * C1 (16 -> 32), C2 (32 -> 64), each followed by zeroing the boundary nodes
+ * of its output grid; the transpose pass reuses the same weights. */
 int fsx_unet_setup(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed);
-/* number of FP32 parameters: layers in order in, d1, l1a, d2, l2a, l2b, l1b,
- * l1c, out; layer weights [cout][kp] with kp = 27*cin rounded up to 32 and
- * k = tap*cin + ci, tap = dx+1 + 3(dy+1) + 9(dz+1); padding entries are 0 */
+/* number of FP32 parameters (70144 = 16*32 + 32*448 + 64*864): the three
+ * layers in order C0, C1, C2, each stored as weights [cout][kp] with
+ * kp = 27*cin rounded up to 32 (C0 32, C1 448, C2 864), entry k = tap*cin + ci
+ * of row co, tap = (dx+1) + 3(dy+1) + 9(dz+1) for the input node
+ * 2o + (dx, dy, dz) of output node o; C0 starts at offset 0, C1 at 512, C2 at
+ * 14848; padding entries (k >= 27*cin) are 0. The weights are grid-size
+ * independent: a set trained on one 8m+1 grid applies to any other. */
 int fsx_unet_params(fsx_ctx* ctx, long long* n);
 int fsx_unet_get_weights(fsx_ctx* ctx, float* host, size_t n);
 int fsx_unet_set_weights(fsx_ctx* ctx, const float* host, size_t n);

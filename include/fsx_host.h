@@ -93,6 +93,10 @@ typedef struct fsxd_step_stats {
   double mon[4];      /* u, v, w, p at the monitor node */
   double wall_s;      /* host wall time of the step (device synchronised) */
   double pcg_relres;  /* of the last solve */
+  int pcg_noconv;     /* the step's solves that stopped at the iteration cap (FSX_ENOCONV; their
+                         partial answer is used, as the reference does, BCH:4041-4044) */
+  int pad_;
+  double pcg_relres_max;  /* largest relative residual of the step's solves */
 } fsxd_step_stats;
 
 void fsxd_default_options(fsxd_options* o);
