@@ -136,10 +136,10 @@ __global__ void k_publish(const SolveScalars* __restrict__ sc, Published* pub, u
 // (a copy plus hipStreamSynchronize costs tens of microseconds of idle GPU
 // per solver decision). The stream is queried every few thousand spins so
 // that a failed kernel or a lost publication ends the wait with an error.
-int read_scalars(fsx_ctx* ctx) {
+int read_scalars(fsx_ctx* ctx, const SolveScalars* sc) {
   static_assert(sizeof(SolveScalars) % sizeof(unsigned) == 0, "SolveScalars is copied as 32-bit words");
   const unsigned seq = ++ctx->pub_seq;
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kWave), 0, ctx->stream, ctx->sc, ctx->d_pub, seq);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kWave), 0, ctx->stream, sc ? sc : ctx->sc, ctx->d_pub, seq);
   FSX_TRY(launch_check(ctx, "k_publish"));
   volatile unsigned* flag = &ctx->h_pub->seq;
   for (long spin = 1; *flag != seq; spin++) {
@@ -194,6 +194,7 @@ int fsx_create(int device, fsx_ctx** out) {
   if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_st, sizeof(StepScalars)) != hipSuccess) rc = FSX_ENOMEM;
   if (rc == FSX_OK && hipHostMalloc((void**)&ctx->h_mon, 4 * sizeof(double)) != hipSuccess) rc = FSX_ENOMEM;
   if (rc == FSX_OK && (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)) rc = FSX_EHIP;
+  if (rc == FSX_OK) rc = scratch_setup(ctx);   // the device is current here
   if (rc == FSX_OK) {
     if (hipMemsetAsync(ctx->sc, 0, sizeof(SolveScalars), ctx->stream) != hipSuccess ||
         hipMemsetAsync(ctx->st, 0, sizeof(StepScalars), ctx->stream) != hipSuccess)

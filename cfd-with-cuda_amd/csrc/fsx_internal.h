@@ -268,6 +268,9 @@ struct fsx_ctx {
   int last_learned_iters = 0;
   int last_fallback = 0;
   void (*capture_hook)(fsx_ctx*, const double* b, const double* x) = nullptr;
+  // the scratch array of fsx_solve_pressure_noalloc on this device (its
+  // device address, from hipGetSymbolAddress at fsx_create)
+  void* scratch = nullptr;
   // iteration graphs: FP64 PCG, mixed inner PCG, learned flexible PCG
   bool use_graphs = false;      // measured on par with individual launches on this path
   fsx::IterGraph graph_fp64, graph_mixed, graph_learned;
@@ -330,8 +333,12 @@ int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double a
 int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);
 int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi);
 void unet_free(fsx_ctx* ctx);
-// *ctx->h_sc <- the solve scalars after all work enqueued so far
-int read_scalars(fsx_ctx* ctx);
+// *ctx->h_sc <- the solve scalars (sc, or the context's own when null) after
+// all work enqueued so far
+int read_scalars(fsx_ctx* ctx, const SolveScalars* sc = nullptr);
+// the device address of the allocation-free entry point's scratch array on
+// the context's device (once per device, at fsx_create)
+int scratch_setup(fsx_ctx* ctx);
 void iter_graph_reset(IterGraph& g);
 
 // Enqueue iterations j .. j + count - 1 (rounded up to whole graphs when

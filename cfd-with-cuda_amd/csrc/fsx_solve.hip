@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace fsx {
 
@@ -896,33 +898,36 @@ int ensure_fpq(fsx_ctx* ctx) {
   return FSX_OK;
 }
 
+// One iteration on the solve scalars sc and the reduction partials
+// (3 kRedGrid doubles); the solvers pass the context's own, the
+// allocation-free solve its scratch.
 template <class T>
 static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* dinv, T* x, T* r, T* z, T* p, T* q,
-                                 int kind, double delta2) {
+                                 int kind, double delta2, SolveScalars* sc, double* partials) {
   constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
   const int n = ctx->NNp;
-  const double* pqp = ctx->partials;
+  const double* pqp = partials;
   int npq = kRedGrid;
   bool fused = false;
   if constexpr (sizeof(T) == 4) {
     if (ctx->Zc_chunk && ctx->fpq) {   // the FP32 row-class SpMV writes p.q per chunk: no separate dot pass
-      (void)launch_pcg_spmv_pq_f32(ctx, p, q, ctx->sc, ctx->fpq, &npq);
+      (void)launch_pcg_spmv_pq_f32(ctx, p, q, sc, ctx->fpq, &npq);
       pqp = ctx->fpq;
       fused = true;
     }
   }
   if (!fused) {
-    launch_zspmv<T>(ctx, p, q, ctx->sc);
-    hipLaunchKernelGGL(k_pcg_dot<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, p, q, ctx->partials, ctx->sc);
+    launch_zspmv<T>(ctx, p, q, sc);
+    hipLaunchKernelGGL(k_pcg_dot<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, p, q, partials, sc);
   }
   hipLaunchKernelGGL(k_pcg_update<T>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, x, r, z, p, q, dinv,
-                     ctx->partials, ctx->sc, pqp, npq);
+                     partials, sc, pqp, npq);
   if (kind == 0)
-    hipLaunchKernelGGL((k_pcg_direction<T, 0>), dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, z, p, ctx->partials,
-                       delta2, ctx->sc);
+    hipLaunchKernelGGL((k_pcg_direction<T, 0>), dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, z, p, partials,
+                       delta2, sc);
   else
-    hipLaunchKernelGGL((k_pcg_direction<T, 1>), dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, z, p, ctx->partials,
-                       delta2, ctx->sc);
+    hipLaunchKernelGGL((k_pcg_direction<T, 1>), dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, n, z, p, partials,
+                       delta2, sc);
 }
 
 // Enqueues iterations j0, j0+1, ... in chunks (the first sized from the
@@ -936,7 +941,7 @@ static int pcg_iterate(fsx_ctx* ctx, int j0, const T* dinv, T* x, T* r, T* z, T*
   IterGraph& G = sizeof(T) == 8 ? ctx->graph_fp64 : ctx->graph_mixed;
   const unsigned long long key = (unsigned long long)(uintptr_t)x;   // the only caller-owned buffer
   auto issue = [&](int jj) -> int {
-    pcg_launch_iteration<T>(ctx, jj, dinv, x, r, z, p, q, kind, delta2);
+    pcg_launch_iteration<T>(ctx, jj, dinv, x, r, z, p, q, kind, delta2, ctx->sc, ctx->partials);
     return FSX_OK;
   };
   for (;;) {
@@ -1516,4 +1521,87 @@ extern "C" int fsx_bench_copy(fsx_ctx* ctx, size_t bytes, int reps, double* ms_p
   dev_free(ctx, b);
   *ms_per_launch = t / reps;
   return rc;
+}
+
+// ------------------------------------------------------------ allocation-free pressure solve
+// fsx_solve_pressure_noalloc runs the FP64 Jacobi-PCG of fsx_solve_pressure
+// with everything beyond the caller's b, x and the context's Z held in one
+// file-scope device array: the solve scalars, the reduction partials and the
+// Krylov vectors r, z, p, q. Each device that loads the library holds its own
+// copy of the array until the process ends; its device address is taken once
+// per device by scratch_setup (at fsx_create, with the device current), and
+// only that address is ever handed to kernels (the array's name on the host
+// is a stand-in, not a device address).
+namespace fsx {
+
+constexpr long long kNoallocMaxRows = 1LL << 20;              // pressure nodes (NNp) the scratch holds
+constexpr size_t kScratchHead = 256;                          // SolveScalars, padded
+constexpr size_t kScratchPartials = 8 * (size_t)kRedGrid;     // doubles of reduction partials
+constexpr size_t kScratchBytes = kScratchHead + 8 * kScratchPartials + 4 * 8 * (size_t)kNoallocMaxRows;
+static_assert(sizeof(SolveScalars) <= kScratchHead, "solve scalars fit the scratch head");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int scratch_setup(fsx_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return fail(ctx, FSX_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ctx, FSX_EHIP, std::string("hipGetSymbolAddress(scratch) -> ") + hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = p;
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  return FSX_OK;
+}
+
+}  // namespace fsx
+
+extern "C" int fsx_solve_pressure_noalloc(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
+  using namespace fsx;
+  if (!ctx || !b || !x) return FSX_EINVAL;
+  if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
+  if (!ctx->scratch) return fail(ctx, FSX_ESTATE, "scratch address not set up for this device");
+  const int n = ctx->NNp;
+  if ((long long)n > kNoallocMaxRows)
+    return fail(ctx, FSX_EINVAL, "fsx_solve_pressure_noalloc: NNp " + std::to_string(n) + " exceeds the scratch maximum of " +
+                                     std::to_string(kNoallocMaxRows) + " pressure nodes");
+  unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+  SolveScalars* sc = reinterpret_cast<SolveScalars*>(base);
+  double* partials = reinterpret_cast<double*>(base + kScratchHead);
+  double* r = partials + kScratchPartials;
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  // k_pcg_init writes x, r, z, p and the partials, k_pcg_init_finish every
+  // scalar the iterations read: nothing of a previous call is read
+  hipLaunchKernelGGL(k_pcg_init<double>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->Zdinv, x, r, z, p,
+                     partials);
+  hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sc, ctx->rtol, ctx->maxit);
+  FSX_TRY(launch_check(ctx, "k_pcg_init (noalloc)"));
+  int j = 0;
+  int chunk = segment_guess(std::min(ctx->last_iters, ctx->maxit), 1.0, ctx->rtol, 4);
+  for (;;) {
+    for (int c = 0; c < chunk; c++)
+      pcg_launch_iteration<double>(ctx, j + c, ctx->Zdinv, x, r, z, p, q, 0, 0.0, sc, partials);
+    j += chunk;
+    FSX_TRY(launch_check(ctx, "PCG iterations (noalloc)"));
+    FSX_TRY(read_scalars(ctx, sc));
+    if (ctx->h_sc->stop) break;
+    if (j > ctx->maxit + 64) return fail(ctx, FSX_ESTATE, "PCG did not terminate");
+    chunk = 4;
+  }
+  const SolveScalars& s = *ctx->h_sc;
+  if (iters) *iters = s.iters;
+  if (relres) *relres = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  if (s.stop == 3) return fail(ctx, FSX_ENUMERIC, "PCG met NaN/Inf");
+  if (s.stop == 2) { ctx->err = "PCG reached maxit"; return FSX_ENOCONV; }
+  return FSX_OK;
 }

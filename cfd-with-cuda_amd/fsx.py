@@ -35,7 +35,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision",
+    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
@@ -108,6 +108,7 @@ def lib():
         "fsx_upload_vec": ([_vp, ctypes.c_int, _D, ctypes.c_size_t], ctypes.c_int),
         "fsx_sync": ([_vp], ctypes.c_int),
         "fsx_solve_pressure": ([_vp, _D, _D, _I, _D], ctypes.c_int),
+        "fsx_solve_pressure_noalloc": ([_vp, _vp, _vp, _I, _D], ctypes.c_int),
         "fsx_spmv": ([_vp, ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "fsx_bench_spmv": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_bench_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
@@ -519,6 +520,17 @@ class Solver:
         if rc not in (FSX_OK, FSX_ENOCONV):
             self.check(rc, "solve_pressure")
         return x, it.value, rel.value, rc
+
+    def solve_pressure_noalloc(self, b_dev, x_dev):
+        """The allocation-free FP64 Jacobi-PCG on device arrays of NNp doubles
+        (raw device addresses b_dev, x_dev, ready for the context's stream);
+        returns (iterations, relative residual, status)."""
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        rc = self.L.fsx_solve_pressure_noalloc(self.ctx, _vp(b_dev), _vp(x_dev), ctypes.byref(it), ctypes.byref(rel))
+        if rc not in (FSX_OK, FSX_ENOCONV):
+            self.check(rc, "solve_pressure_noalloc")
+        return it.value, rel.value, rc
 
     def spmv(self, which, x, y=None, alpha=1.0, beta=0.0):
         """which: K, G1..G3, Z, G1T..G3T; Zsolver64 / Zsolver32 apply Z as the

@@ -237,6 +237,21 @@ int fsx_sync(fsx_ctx* ctx);
 /* solve Z x = b on the context's Z with the current solver settings; b and
  * x are host arrays of NNp doubles (a test and tooling entry point). */
 int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
+/* The FP64 Jacobi-PCG of fsx_solve_pressure (the plain solver, x0 = 0, the
+ * context's rtol and maxit, whatever solver mode and x0 policy are set) on
+ * DEVICE arrays b and x of NNp doubles, allocating nothing: its solve
+ * scalars, reduction partials and Krylov vectors live in a file-scope device
+ * array of the library (one per GPU, held until the process ends), so the
+ * call makes no hipMalloc/hipFree and no allocating library call. x equals
+ * what fsx_solve_pressure returns in that configuration. Limit: NNp <= 2^20
+ * pressure nodes (about 32 MiB of scratch); larger Z is refused with
+ * FSX_EINVAL. Every call on one GPU shares that array: calls that overlap on
+ * one GPU (from two contexts or threads) race; nothing of a previous call is
+ * read. b and x must be ready for, and are read back after, the context's
+ * stream (the call returns once the solve has finished). No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid it). */
+int fsx_solve_pressure_noalloc(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
 /* y = alpha * op(v) + beta * y on the context's matrices for host vectors:
  * which = 0 K (NN), 1..3 G_d (NN <- NNp), 4 Z (NNp), 5..7 G_d^T (NNp <- NN).
  * Mirrors the reference's mkl_dcsrmv / cusparseDcsrmv calls. which = 8 and
