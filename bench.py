@@ -120,6 +120,7 @@ def train_learned(solver, args):
     solver.set_matrix_precision(32)
     solver.set_convection_precision(32)
     n1 = args.n + 1
+    solver.unet_form(args.learned_form)
     solver.unet_setup((n1, n1, n1), seed=args.seed)
     solver.unet_capture(64)
     solver.run(args.train_capture_steps)
@@ -223,6 +224,9 @@ def main():
     ap.add_argument("--train-iters", type=int, default=400)
     ap.add_argument("--train-capture-steps", type=int, default=3)
     ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--learned-form", type=int, default=1, choices=(0, 1),
+                    help="preconditioner of the learned mode: 1 two-level (Jacobi smoothing around the network, "
+                         "tiled Z products), 0 additive (dinv r + network)")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--probe", action="store_true",
                     help="only set up the case, run one FP64 time step and one with FP32 matrices and convection, time "

@@ -320,6 +320,22 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
     }
   }
   ctx->NE = NE; ctx->NN = NN; ctx->NNp = NNp;
+  // Structured pressure grid: every element's corner ids differ by 1, nx and
+  // nx*ny along its x, y and z edges (local corners 0-1, 0-3, 0-4, BCH:2312-
+  // 2343), as the structured generators number them (x fastest). The Z
+  // products of the solvers then stage the grid through LDS tiles.
+  ctx->pg_nx = ctx->pg_ny = ctx->pg_nz = 0;
+  {
+    const int nx = LtoGnode[3] - LtoGnode[0], nxy = LtoGnode[4] - LtoGnode[0];
+    bool ok = nx > 1 && nxy > nx && nxy % nx == 0 && NNp % nxy == 0;
+    for (int e = 0; ok && e < NE; e++) {
+      const int* L = LtoGnode + (size_t)e * 27;
+      const int c = L[0];
+      ok = L[1] - c == 1 && L[2] - c == 1 + nx && L[3] - c == nx && L[4] - c == nxy && L[5] - c == 1 + nxy &&
+           L[6] - c == 1 + nx + nxy && L[7] - c == nx + nxy;
+    }
+    if (ok) { ctx->pg_nx = nx; ctx->pg_ny = nxy / nx; ctx->pg_nz = NNp / nxy; }
+  }
   // internal velocity numbering: the nodes in the order of their coordinates
   // (z, y, x), each node placed by the trilinear map of the first element
   // that holds it, coordinates compared on a grid of 1e-9 of the mesh extent.
@@ -506,7 +522,11 @@ static int build_z_forms(fsx_ctx* ctx) {
   }
   std::vector<int> cls, cp, co;
   std::vector<unsigned> cv;
-  if (build_row_dict(n, rp2.data(), ci2.data(), f2.data(), 65535, 1 << 20, cls, cp, co, &cv)) {
+  const bool classes = build_row_dict(n, rp2.data(), ci2.data(), f2.data(), 65535, 1 << 20, cls, cp, co, &cv);
+  // the classes staged through LDS tiles of a structured pressure grid
+  if (classes) FSX_TRY(build_tile_plan(ctx, cls, cp, co, cv));
+  else FSX_TRY(build_tile_plan(ctx, {}, {}, {}, {}));
+  if (classes) {
     std::vector<int2> ent(co.size());
     for (size_t k = 0; k < co.size(); k++) ent[k] = make_int2(co[k], (int)cv[k]);
     FSX_TRY(alloc(ctx, &ctx->Zc_ent, std::max<size_t>(1, ent.size())));

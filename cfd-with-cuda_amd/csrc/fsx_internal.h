@@ -39,6 +39,8 @@ struct SolveScalars {
   int active_v[2];    // iteration j ran its update kernel (stop was 0 then)
   double rp_v[2];     // learned solver: r.p entering iteration j sits in rp_v[j & 1]
   double pq_v[2];     // learned solver: p.Zp of iteration j
+  double rp;          // tiled learned solver: r.p of the current iteration (written by its direction kernel)
+  double pq_prev;     // tiled learned solver: p.Zp of the last update (0 before the first)
 };
 
 // SolveScalars as published to host-coherent pinned memory by k_publish:
@@ -48,6 +50,49 @@ struct Published {
   unsigned seq;
   unsigned pad[3];
 };
+
+// The pressure operator Z staged through LDS on a structured nx*ny*nz grid
+// (x fastest): a tile is every x of ty rows in y and tz in z (at most 1024
+// rows, one a thread), held in LDS with a halo of 2 nodes ((nx + 4) x (ty +
+// 4) x (tz + 4) values, zero outside the grid), since Z couples nodes up to 2
+// apart per axis. A tile's rows are sorted by FP32 row class; row t of tile b
+// is rows[b * rstride + t] = (grid node, LDS index | own index << 16, global
+// class entry start, entry count | tile entry start << 8), its entries
+// (LDS offset, FP32 value bits) sit at ent + global start and, for the
+// wavefronts whose rows mix classes, in the tile's entry block
+// tent[b * estride ...] that the kernel stages in LDS.
+struct TilePlan {
+  int nx, ny, nz;
+  int ty, tz, lx, ly, lz;
+  int tiles_y, ntiles, npart;
+  int rstride, estride;    // rows and entries reserved per tile
+  const int4* rows;
+  const int2* ent;
+  const int2* tent;
+};
+
+// Vectors of one tiled kernel of the learned solver's iteration (the mode
+// decides which are read and written; see k_tile in fsx_stencil.hip).
+struct TileVecs {
+  const float* z;     // kTileDir: the preconditioned residual
+  const float* pin;   // kTileDir: the last direction; kTileSmooth, kTilePre: the direction for the r.p partial
+  float* pout;        // kTileDir: the new direction
+  float* q;           // kTileDir writes Z p; kTileSmooth and kTilePost read it
+  const float* rin;   // kTileSmooth: the residual before the update; kTilePre, kTilePost: the residual
+  float* rout;        // kTileSmooth: the updated residual
+  float* x;           // kTileSmooth: the FP32 iterate
+  float* r1;          // kTileSmooth, kTilePre: r - Z (omega dinv r), the network's input
+  const float* z2;    // kTilePost: omega dinv r + the network's correction
+  float* zout;        // kTilePost: z
+  const float* dinv;  // FP32 Jacobi inverse
+  float omega;        // Jacobi smoothing weight
+  double* part;       // kTileSlots slots of TilePlan::npart partial sums
+  SolveScalars* sc;
+  int j, maxit;       // iteration index (kTileDir) and flexible-iteration cap
+  double delta2;      // reliable-update threshold on |r|^2 / |r_anchor|^2
+};
+enum { kTileDir = 0, kTileSmooth = 1, kTilePre = 2, kTilePost = 3 };
+constexpr int kTileSlots = 5;   // p.q, r.r, r.p, r.z, z.q
 
 // Scalars of the time step (norms for the convergence checks, max |dU|/dt).
 struct StepScalars {
@@ -197,6 +242,14 @@ struct fsx_ctx {
   int* Zrowpat = nullptr; int* Zpatoff = nullptr;
   int Znpat = 0; long long Zpatlen = 0;
   int zrow_max = 0;
+  // structured pressure grid (x fastest), detected at fsx_upload_mesh; 0 when
+  // the pressure numbering is not a grid (e.g. a permuted mesh)
+  int pg_nx = 0, pg_ny = 0, pg_nz = 0;
+  // the FP32 row classes of Z staged through LDS tiles of that grid (null
+  // when the grid is not structured or Z is not a radius-2 stencil on it)
+  fsx::TilePlan tile{};
+  void* tile_mem[3] = {nullptr, nullptr, nullptr};
+  double* tile_part = nullptr;   // 5 slots of tile.npart partial sums
   // Z without its structural zeros (entries below kPruneRel of the row's
   // largest magnitude), built by fsx_build_Z for the solvers:
   //  * FP64 rows Zr_row / Zr_val, columns through a pattern dictionary
@@ -265,6 +318,18 @@ struct fsx_ctx {
   // learned preconditioner
   struct fsx_unet* unet = nullptr;
   int learned_maxit = 400;       // flexible-CG iterations before the Jacobi fallback
+  // preconditioner form (fsx_unet_form): 1 the two-level multiplicative form
+  // on the LDS tiles (default where the tile plan exists), 0 the additive form
+  int learned_form = 1;
+  float learned_omega = 0.8f;    // Jacobi smoothing weight of the two-level form
+  float* fr2 = nullptr;          // the tiled flexible CG's second residual and direction buffers,
+  float* fp2 = nullptr;          // the network's input r1 and output z2 (NNp each)
+  float* fr1 = nullptr;
+  float* fz2 = nullptr;
+  // flexible iterations of each segment (between reliable updates) of the
+  // last tiled learned solve: the next solve enqueues that many before it
+  // reads the scalars back
+  int learned_seg[16] = {0};
   int last_learned_iters = 0;
   int last_fallback = 0;
   void (*capture_hook)(fsx_ctx*, const double* b, const double* x) = nullptr;
@@ -332,6 +397,11 @@ int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double*
 int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta);
 int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);
 int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi);
+int build_tile_plan(fsx_ctx* ctx, const std::vector<int>& cls, const std::vector<int>& cp, const std::vector<int>& co,
+                    const std::vector<unsigned>& cv);
+int launch_tile(fsx_ctx* ctx, int mode, const TileVecs& v);
+int launch_warm_residual(fsx_ctx* ctx, const double* x, const double* b, double* rh);
+int launch_tile_rescan(fsx_ctx* ctx, const double* rh, float* r, const float* p);
 void unet_free(fsx_ctx* ctx);
 // *ctx->h_sc <- the solve scalars (sc, or the context's own when null) after
 // all work enqueued so far

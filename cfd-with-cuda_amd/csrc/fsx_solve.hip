@@ -1253,6 +1253,8 @@ int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, do
 }
 
 int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres, int truncate_at);
+int solve_learned_tiled(fsx_ctx* ctx, const double* b, double* x, bool warm, int* iters, double* relres,
+                        int truncate_at);
 
 // |b|^2 (slot 0) and |r|^2 (slot 1)
 __global__ __launch_bounds__(kBlock) void k_sumsq2(int n, const double* __restrict__ b, const double* __restrict__ r,
@@ -1264,6 +1266,15 @@ __global__ __launch_bounds__(kBlock) void k_sumsq2(int n, const double* __restri
     v[1] += r[i] * r[i];
   }
   write_partials<2>(v, partials);
+}
+
+// rh = b - Z x in FP64 and the partial sums of |b|^2 (slot 0) and |rh|^2
+// (slot 1) of ctx->partials, with no read-back (the tiled learned solver's
+// warm start takes them on the device)
+int launch_warm_residual(fsx_ctx* ctx, const double* x, const double* b, double* rh) {
+  launch_zresidual(ctx, x, b, rh);
+  hipLaunchKernelGGL(k_sumsq2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, b, rh, ctx->partials);
+  return launch_check(ctx, "warm residual");
 }
 
 // Cholesky of Z (the MATLAB prototype's pressure step: factor once,
@@ -1288,9 +1299,16 @@ static int solve_direct(fsx_ctx* ctx, const double* b, double* x, int* iters, do
   return FSX_OK;
 }
 
+// the learned solver in its tiled two-level form (a structured pressure grid
+// whose Z rows compress into classes, and the multiplicative form selected)
+static bool learned_tiled(const fsx_ctx* ctx) {
+  return ctx->solver_mode == FSX_SOLVER_LEARNED && ctx->unet && ctx->tile.ntiles > 0 && ctx->learned_form == 1;
+}
+
 // the selected solver from x = 0
 static int solve_cold(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   if (ctx->solver_mode == FSX_SOLVER_DIRECT) return solve_direct(ctx, b, x, iters, relres);
+  if (learned_tiled(ctx)) return solve_learned_tiled(ctx, b, x, false, iters, relres, 0);
   if (ctx->solver_mode == FSX_SOLVER_LEARNED && ctx->unet) return solve_learned(ctx, b, x, iters, relres, 0);
   if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)
     return solve_mixed(ctx, b, x, iters, relres);
@@ -1333,8 +1351,10 @@ static int solve_warm(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   int rc;
   timing_begin(ctx);
-  // (a direct solve does not use an initial guess)
-  if (ctx->x0_policy == FSX_X0_PREVIOUS && ctx->solver_mode != FSX_SOLVER_DIRECT) rc = solve_warm(ctx, b, x, iters, relres);
+  // (a direct solve does not use an initial guess; the tiled learned solver
+  // starts from x on the device)
+  if (ctx->x0_policy == FSX_X0_PREVIOUS && learned_tiled(ctx)) rc = solve_learned_tiled(ctx, b, x, true, iters, relres, 0);
+  else if (ctx->x0_policy == FSX_X0_PREVIOUS && ctx->solver_mode != FSX_SOLVER_DIRECT) rc = solve_warm(ctx, b, x, iters, relres);
   else rc = solve_cold(ctx, b, x, iters, relres);
   if (rc == FSX_OK || rc == FSX_ENOCONV) timing_end(ctx, *iters);
   ctx->last_relres = *relres;

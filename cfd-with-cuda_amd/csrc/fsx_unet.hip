@@ -324,11 +324,12 @@ __global__ __launch_bounds__(kBlock) void k_conv0(Grid3 gf, Grid3 g0, const floa
 // 512 x 1 7.0 us, 512 x 2 7.5, 256 x 4 7.9, 256 x 2 8.2, 1024 x 2 10.3.
 constexpr int kWideBlock = 512;
 
+// With omega (the two-level form) z = omega dinv r + y and no partials.
 __global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const float* __restrict__ h, float gain,
                                                    float* __restrict__ y, const float* __restrict__ r,
                                                    const float* __restrict__ dinv, const float* __restrict__ q,
                                                    float* __restrict__ z, double* __restrict__ partials,
-                                                   const SolveScalars* __restrict__ sc) {
+                                                   const SolveScalars* __restrict__ sc, float omega) {
   if (sc && sc->stop) return;   // sc is null on a start, which always runs
   double acc2[2] = {0.0, 0.0};
   // block b owns nodes [b per, (b + 1) per); a thread takes kNodes of them
@@ -362,7 +363,7 @@ __global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const
       const unsigned o = (oz[c] * (unsigned)g0.ny + oy[b]) * (unsigned)g0.nx + ox[a];
       hv[u][s8] = h[(size_t)o * 32 + (tx[a] + 3 * ty[b] + 9 * tz[c])];
     }
-    if (z) { rv[u] = r[f]; dv[u] = dinv[f]; qv[u] = q[f]; }
+    if (z) { rv[u] = r[f]; dv[u] = dinv[f]; qv[u] = partials ? q[f] : 0.f; }
   }
 #pragma unroll
   for (int u = 0; u < kNodes; u++) {
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const
     for (int s8 = 0; s8 < 8; s8++) acc += ((s8 & ~mk[u]) ? 0.f : 1.f) * hv[u][s8];
     const float yv = gain * acc;
     if (z) {
-      const float zi = dv[u] * rv[u] + yv;
+      const float zi = omega * dv[u] * rv[u] + yv;
       z[f] = zi;
       acc2[0] += (double)rv[u] * (double)zi;
       acc2[1] += (double)zi * (double)qv[u];
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(kWideBlock) void k_conv0T(Grid3 g0, Grid3 gf, const
     }
   }
   }
-  if (z) write_partials<2, kWideBlock>(acc2, partials + 3 * kRedGrid);
+  if (z && partials) write_partials<2, kWideBlock>(acc2, partials + 3 * kRedGrid);
 }
 
 // ------------------------------------------------------------ weight gradients
@@ -660,14 +661,32 @@ int apply_net(fsx_ctx* ctx, fsx_unet* u, Acts& S, const float* x, float* y, cons
                out == kOutNone ? nullptr : u->h0));
   if (out == kOutY)
     hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, u->h0, u->gain, y, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, sc);
+                       nullptr, nullptr, nullptr, nullptr, sc, 1.0f);
   else if (out == kOutZ)
     hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, u->h0, u->gain, nullptr,
-                       x, ctx->Zdinvf, ctx->fq, ctx->fz, ctx->partials, sc);
+                       x, ctx->Zdinvf, ctx->fq, ctx->fz, ctx->partials, sc, 1.0f);
   return launch_check(ctx, "network application");
 }
 
 int forward(fsx_ctx* ctx, fsx_unet* u, const float* x, float* y) { return apply_net(ctx, u, u->A, x, y, nullptr); }
+
+// The two-level form's coarse correction: the network on the pre-smoothed
+// residual r1, its output added to the pre-smoothing of r:
+// z2 = omega dinv r + gain J^T J r1.
+int apply_net_mult(fsx_ctx* ctx, fsx_unet* u, const float* r1, const float* r, float* z2, const SolveScalars* sc) {
+  const Layer* L = u->L.data();
+  Acts& S = u->A;
+  const long long n0 = u->g0.n();
+  hipLaunchKernelGGL(k_conv0, dim3(blocks_for(n0 * 16, kBlock)), dim3(kBlock), 0, ctx->stream, u->gf, u->g0, r1, L[0].wr,
+                     S.f0, S.f0b, sc);
+  FSX_TRY(conv(ctx, 16, 32, 2, u->g0, u->g1, S.f0b, L[1].w16, S.f1, S.f1b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 32, 64, 2, u->g1, u->g2, S.f1b, L[2].w16, S.f2, S.f2b, nullptr, 1, sc));
+  FSX_TRY(conv(ctx, 64, 32, 3, u->g2, u->g1, S.f2b, L[2].wt16, S.g1, S.g1b, S.f1, 1, sc));
+  FSX_TRY(conv(ctx, 32, 16, 3, u->g1, u->g0, S.g1b, L[1].wt16, S.g0, nullptr, S.f0, 1, sc, L[0].wr, u->h0));
+  hipLaunchKernelGGL(k_conv0T, dim3(kRedGrid), dim3(kWideBlock), 0, ctx->stream, u->g0, u->gf, u->h0, u->gain, nullptr, r,
+                     ctx->Zdinvf, nullptr, z2, nullptr, sc, ctx->learned_omega);
+  return launch_check(ctx, "network application (two-level)");
+}
 
 // Parameter gradient of <gy, y> after forward() of x on set A (accumulates
 // into u->g): with y = gain J^T J x, <gy, y> = gain <J gy, J x>, whose
@@ -934,6 +953,38 @@ __global__ __launch_bounds__(kBlock) void k_make_sample(int n, const double* __r
   write_partials<1>(v, partials + kRedGrid);
 }
 
+// the two-level form's sample: with t = omega dinv r (rh) and r1 = r - Z t,
+// in = r1 / s, t' = (x* - x_k - t) / s: the network, applied to the
+// pre-smoothed residual, is trained towards the error that the smoothing
+// leaves; slot 1 gets the error energy (x* - x_k).r / s^2 (slot 0 holds |r|^2)
+__global__ __launch_bounds__(kBlock) void k_make_sample2(int n, const double* __restrict__ rh, const double* __restrict__ r1,
+                                                         const double* __restrict__ t, const double* __restrict__ xs,
+                                                         const double* __restrict__ xk, double* __restrict__ partials,
+                                                         float* __restrict__ in, float* __restrict__ tgt) {
+  double rr[1];
+  read_partials<1>(partials, rr);
+  const double inv_s = rr[0] > 0.0 ? 1.0 / std::sqrt(rr[0] / n) : 0.0;
+  double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const double e = xs[i] - xk[i];
+    in[i] = (float)(r1[i] * inv_s);
+    tgt[i] = (float)((e - t[i]) * inv_s);
+    v[0] += e * rh[i] * inv_s * inv_s;
+  }
+  write_partials<1>(v, partials + kRedGrid);
+}
+
+__global__ void k_scaled_dinv(int n, double w, const double* __restrict__ dinv, const double* __restrict__ r,
+                              double* __restrict__ t) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) t[i] = w * dinv[i] * r[i];
+}
+
+__global__ void k_stop_clear(SolveScalars* sc) {
+  sc->stop = 0;
+  sc->pq_prev = 0.0;
+}
+
 }  // namespace fsx
 
 namespace {
@@ -1065,6 +1116,167 @@ int solve_learned(fsx_ctx* ctx, const double* b, double* x, int* iters, double* 
 }
 
 int solve_fp64_entry(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
+
+// FP32 |b|^2 and |r0|^2 (slots 0 and 1) -> the scalars of a tiled solve: the
+// stop threshold rtol^2 |b|^2 is the cold solve's even when the solve starts
+// from x0 (r0 = b - Z x0); an r0 already within it stops at once.
+__global__ __launch_bounds__(kBlock) void k_tiled_init(const double* __restrict__ partials, SolveScalars* sc, double rtol,
+                                                       int maxit) {
+  double s[2];
+  read_partials<2>(partials, s);
+  if (threadIdx.x == 0) {
+    sc->bb = s[0];
+    sc->rr = s[1]; sc->rr_anchor = s[1]; sc->rr_hi = s[1];
+    sc->rtol2bb = rtol * rtol * s[0];
+    sc->iters = 0; sc->maxit = maxit;
+    sc->stop = s[1] <= sc->rtol2bb ? 1 : (s[1] == s[1] ? 0 : 3);
+    sc->active_v[0] = 0; sc->active_v[1] = 0;
+    sc->rp = 0.0; sc->pq_prev = 0.0;
+  }
+}
+
+// Flexible CG with the two-level preconditioner
+//   z = S r + C r1 + S (r - Z (S r + C r1)),  r1 = r - Z S r,
+// S = omega D^-1 (Jacobi smoothing) and C = gain J^T J (the network as the
+// coarse correction); I - M Z = (I - S Z)(I - C Z)(I - S Z), so M is SPD
+// whenever the smoothing converges (omega < 2 / lambda_max(D^-1 Z)), for any
+// network weights. One iteration is nine launches: the direction with q = Z p
+// (k_tile kTileDir), the update with the pre-smoothing (kTileSmooth), the
+// network's five convolutions and its last layer forming z2, and the
+// post-smoothing with z (kTilePost); the three Z products read a vector
+// staged in LDS. The recurrence is FCG(1) in FP32 as in solve_learned, with
+// FP64 reliable updates (every 1e-4 of residual reduction here) and the same
+// FP64 stop test; a reliable update replaces r and keeps p, q and z, whose
+// partial sums with the new r are rescanned (k_tile_rescan), so the next
+// direction is Z-orthogonalised against the kept one as usual. With warm, x
+// holds the initial guess and r0 = b - Z x is formed on the device.
+int solve_learned_tiled(fsx_ctx* ctx, const double* b, double* x, bool warm, int* iters, double* relres,
+                        int truncate_at) {
+  fsx_unet* u = ctx->unet;
+  const int n = ctx->NNp;
+  // reliable updates every 1e-4 of FP32 residual reduction: the recurrence
+  // keeps about 1e-7 of the residual it started the segment from, so the
+  // FP64 check decides convergence long before FP32 rounding does
+  const double delta = 1e-4;
+  const int max_learned = truncate_at > 0 ? truncate_at : std::min(ctx->maxit, ctx->learned_maxit);
+  for (float** v : {&ctx->fr2, &ctx->fp2, &ctx->fr1, &ctx->fz2})
+    if (!*v) FSX_TRY(alloc(ctx, v, (size_t)n));
+  float* R[2] = {ctx->fr, ctx->fr2};
+  float* P[2] = {ctx->fp, ctx->fp2};
+  if (warm) {
+    FSX_TRY(launch_warm_residual(ctx, x, b, ctx->wh));
+  } else {
+    FSX_TRY(launch_fill(ctx, x, 0.0, n));
+    FSX_TRY(launch_copy(ctx, ctx->wh, b, n));
+    hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->partials);
+    hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->partials + kRedGrid);
+  }
+  hipLaunchKernelGGL(k_tiled_init, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, ctx->sc, ctx->rtol, max_learned);
+  // the direction before the first (p[1]) and q zero; r_0 = float(wh), x_lo = 0
+  hipLaunchKernelGGL(k_zero_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, P[1]);
+  hipLaunchKernelGGL(k_zero_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fq);
+  hipLaunchKernelGGL(k_fpcg_load_r, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, R[0], ctx->fx, P[1],
+                     ctx->partials);
+  FSX_TRY(launch_check(ctx, "tiled learned start"));
+  TileVecs tv{};
+  tv.dinv = ctx->Zdinvf;
+  tv.omega = ctx->learned_omega;
+  tv.part = ctx->tile_part;
+  tv.sc = ctx->sc;
+  tv.maxit = max_learned;
+  tv.delta2 = delta * delta;
+  // z for the residual r[j & 1] (a start or a replaced residual)
+  auto precondition_at = [&](int j) -> int {
+    TileVecs a = tv;
+    a.rin = R[j & 1]; a.pin = P[(j + 1) & 1]; a.r1 = ctx->fr1;
+    FSX_TRY(launch_tile(ctx, kTilePre, a));
+    FSX_TRY(apply_net_mult(ctx, u, ctx->fr1, R[j & 1], ctx->fz2, ctx->sc));
+    TileVecs c = tv;
+    c.rin = R[j & 1]; c.z2 = ctx->fz2; c.zout = ctx->fz; c.q = ctx->fq;
+    return launch_tile(ctx, kTilePost, c);
+  };
+  auto issue = [&](int j) -> int {
+    TileVecs d = tv;
+    d.z = ctx->fz; d.pin = P[(j + 1) & 1]; d.pout = P[j & 1]; d.q = ctx->fq; d.j = j;
+    FSX_TRY(launch_tile(ctx, kTileDir, d));
+    TileVecs m = tv;
+    m.pin = P[j & 1]; m.q = ctx->fq; m.rin = R[j & 1]; m.rout = R[(j + 1) & 1]; m.x = ctx->fx; m.r1 = ctx->fr1;
+    FSX_TRY(launch_tile(ctx, kTileSmooth, m));
+    FSX_TRY(apply_net_mult(ctx, u, ctx->fr1, R[(j + 1) & 1], ctx->fz2, ctx->sc));
+    TileVecs c = tv;
+    c.rin = R[(j + 1) & 1]; c.z2 = ctx->fz2; c.zout = ctx->fz; c.q = ctx->fq;
+    return launch_tile(ctx, kTilePost, c);
+  };
+  FSX_TRY(precondition_at(0));
+  int j = 0;
+  int fallback = 0;
+  // each segment enqueues as many iterations as the same segment of the last
+  // solve took (4 the first time), then one at a time: iterations past the
+  // stop are no-op launches, a read-back idles the device, and consecutive
+  // pressure solves converge alike
+  const bool keep_segs = truncate_at == 0;
+  for (int segi = 0;; segi++) {
+    const int j_seg = j;
+    const int known = segi < 16 ? ctx->learned_seg[segi] : 0;
+    int guess = std::max(1, std::min(known > 0 ? known : 4, max_learned - j + 1));
+    int issued = 0;
+    for (;;) {
+      for (int c = 0; c < guess; c++) FSX_TRY(issue(j + c));
+      j += guess;
+      issued += guess;
+      FSX_TRY(read_scalars(ctx));
+      if (ctx->h_sc->stop) break;
+      if (issued > max_learned + 64) return fail(ctx, FSX_ESTATE, "tiled FPCG did not terminate");
+      guess = 1;
+    }
+    const int stop = ctx->h_sc->stop;
+    j = ctx->h_sc->iters;
+    if (keep_segs && segi < 16) ctx->learned_seg[segi] = std::max(1, j - j_seg);
+    // reliable update: fold x_lo into x, FP64 residual wh = b - Z x, FP64 stop test
+    int st = 0;
+    double rr_hi = 0;
+    FSX_TRY(launch_fold_residual(ctx, x, b, &st, &rr_hi));
+    if (st == 3 || stop == 3) return fail(ctx, FSX_ENUMERIC, "learned solve met NaN/Inf");
+    if (st == 1) break;                                     // converged in FP64
+    if (stop == 6 || stop == 2 || st == 2) { fallback = stop == 6 ? 6 : 2; break; }
+    // replace r by the FP64 residual; keep p, q and z (rescan its partials)
+    FSX_TRY(launch_tile_rescan(ctx, ctx->wh, R[j & 1], P[(j + 1) & 1]));
+  }
+  const int learned_iters = ctx->h_sc->iters;
+  const double bb = ctx->h_sc->bb, rh = ctx->h_sc->rr_hi;
+  if (truncate_at > 0) {
+    *iters = learned_iters;
+    *relres = bb > 0 ? std::sqrt(rh / bb) : 0.0;
+    return FSX_OK;
+  }
+  ctx->last_learned_iters = learned_iters;
+  ctx->last_fallback = fallback;
+  if (fallback) {
+    // Jacobi-PCG (mixed) on the remaining FP64 residual wh, to the same
+    // absolute threshold rtol ||b||
+    int it2 = 0;
+    double rel2 = 0;
+    const double saved_rtol = ctx->rtol;
+    ctx->rtol = rh > 0 ? std::min(1.0, saved_rtol * std::sqrt(bb / rh)) : saved_rtol;
+    FSX_TRY(launch_copy(ctx, ctx->wr, ctx->wh, n));        // the mixed solver overwrites wh
+    const int rc = solve_mixed_entry(ctx, ctx->wr, ctx->wz, &it2, &rel2);
+    ctx->rtol = saved_rtol;
+    if (rc != FSX_OK && rc != FSX_ENOCONV) return rc;
+    FSX_TRY(launch_add(ctx, x, ctx->wz, n));
+    int st = 0;
+    double rr2 = 0;
+    FSX_TRY(launch_residual_only(ctx, x, b, &st, &rr2));
+    *iters = learned_iters + it2;
+    *relres = bb > 0 ? std::sqrt(rr2 / bb) : 0.0;
+    ctx->last_iters = *iters;
+    if (st != 1) { ctx->err = "learned solve and Jacobi fallback reached maxit"; return FSX_ENOCONV; }
+    return FSX_OK;
+  }
+  *iters = learned_iters;
+  *relres = bb > 0 ? std::sqrt(rh / bb) : 0.0;
+  ctx->last_iters = learned_iters;
+  return FSX_OK;
+}
 
 void unet_free(fsx_ctx* ctx) {
   delete ctx->unet;   // device buffers belong to the context's allocation list
@@ -1243,6 +1455,7 @@ int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsampl
   if (!ks || nk <= 0) return fail(ctx, FSX_EINVAL, "no truncation depths");
   if (u->ncap == 0) return fail(ctx, FSX_ESTATE, "no captured solves");
   const bool use_learned = flags & 1, append = (flags & 2) && u->nsamp > 0;
+  const bool two_level = ctx->learned_form == 1 && ctx->tile.ntiles > 0;
   const int n = ctx->NNp;
   const int start = append ? u->nsamp : 0;
   const int total = start + u->ncap * nk;
@@ -1288,8 +1501,9 @@ int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsampl
         ctx->last_iters = ks[q];
         int it = 0;
         double rel = 0;
-        const int rc = use_learned ? solve_learned(ctx, b, u->tmp_x, &it, &rel, ks[q])
-                                   : solve_fp64_entry(ctx, b, u->tmp_x, &it, &rel);
+        const int rc = !use_learned ? solve_fp64_entry(ctx, b, u->tmp_x, &it, &rel)
+                       : two_level ? solve_learned_tiled(ctx, b, u->tmp_x, false, &it, &rel, ks[q])
+                                   : solve_learned(ctx, b, u->tmp_x, &it, &rel, ks[q]);
         if (rc != FSX_OK && rc != FSX_ENOCONV) { restore(); return rc; }
       }
       int st = 0;
@@ -1298,8 +1512,17 @@ int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsampl
       // an iterate closer than 1e-5 ||b|| to the solution is compared with x*
       // that is itself only good to the solver's tolerance: no information
       if (!(rr >= 1e-10 * bb)) continue;
-      hipLaunchKernelGGL(k_make_sample, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, xs, u->tmp_x, ctx->Zdinv,
-                         ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
+      if (two_level) {
+        // t = omega dinv r, r1 = r - Z t (|r|^2 stays in slot 0)
+        hipLaunchKernelGGL(k_scaled_dinv, dim3(256), dim3(kBlock), 0, ctx->stream, n, (double)ctx->learned_omega, ctx->Zdinv,
+                           ctx->wh, ctx->w0e);
+        FSX_TRY(launch_warm_residual(ctx, ctx->w0e, ctx->wh, ctx->w0r));
+        hipLaunchKernelGGL(k_make_sample2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->w0r, ctx->w0e, xs,
+                           u->tmp_x, ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
+      } else {
+        hipLaunchKernelGGL(k_make_sample, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, xs, u->tmp_x, ctx->Zdinv,
+                           ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
+      }
       FSX_TRY(launch_check(ctx, "k_make_sample"));
       FSX_TRY(sum_partials_host(ctx, &u->s_tt[s], 1));
       if (!(u->s_tt[s] > 0)) { restore(); return fail(ctx, FSX_ENUMERIC, "a training sample has no error energy"); }
@@ -1343,6 +1566,50 @@ int fsx_unet_train(fsx_ctx* ctx, int steps, int batch, float lr, unsigned long l
   }
   if (loss_out) *loss_out = tail_n ? tail / tail_n : 0.0;
   return fsx_sync(ctx);
+}
+
+int fsx_unet_form(fsx_ctx* ctx, int form, float omega) {
+  if (!ctx) return FSX_EINVAL;
+  if (form != 0 && form != 1) return fail(ctx, FSX_EINVAL, "form must be 0 (additive) or 1 (two-level)");
+  if (form == 1 && !(omega > 0.0f && omega < 1.0f)) return fail(ctx, FSX_EINVAL, "omega must lie in (0, 1)");
+  ctx->learned_form = form;
+  if (form == 1) ctx->learned_omega = omega;
+  return FSX_OK;
+}
+
+int fsx_unet_form_get(fsx_ctx* ctx, int* form_in_use, float* omega) {
+  if (!ctx) return FSX_EINVAL;
+  if (form_in_use) *form_in_use = (ctx->learned_form == 1 && ctx->tile.ntiles > 0) ? 1 : 0;
+  if (omega) *omega = ctx->learned_omega;
+  return FSX_OK;
+}
+
+int fsx_unet_precondition(fsx_ctx* ctx, const float* r, float* z) {
+  FSX_TRY(need_unet(ctx));
+  if (!r || !z) return FSX_EINVAL;
+  fsx_unet* u = ctx->unet;
+  const int n = ctx->NNp;
+  FSX_HIP(hipMemcpyAsync(ctx->fr, r, (size_t)n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_stop_clear, dim3(1), dim3(1), 0, ctx->stream, ctx->sc);
+  hipLaunchKernelGGL(k_zero_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fq);
+  if (ctx->learned_form == 1 && ctx->tile.ntiles > 0) {
+    for (float** v : {&ctx->fr2, &ctx->fp2, &ctx->fr1, &ctx->fz2})
+      if (!*v) FSX_TRY(alloc(ctx, v, (size_t)n));
+    hipLaunchKernelGGL(k_zero_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, ctx->fp2);
+    TileVecs a{};
+    a.dinv = ctx->Zdinvf; a.omega = ctx->learned_omega; a.part = ctx->tile_part; a.sc = ctx->sc;
+    a.rin = ctx->fr; a.pin = ctx->fp2; a.r1 = ctx->fr1;
+    FSX_TRY(launch_tile(ctx, kTilePre, a));
+    FSX_TRY(apply_net_mult(ctx, u, ctx->fr1, ctx->fr, ctx->fz2, ctx->sc));
+    TileVecs c = a;
+    c.rin = ctx->fr; c.z2 = ctx->fz2; c.zout = ctx->fz; c.q = ctx->fq;
+    FSX_TRY(launch_tile(ctx, kTilePost, c));
+  } else {
+    FSX_TRY(precondition(ctx, u, nullptr));
+  }
+  FSX_HIP(hipMemcpyAsync(z, ctx->fz, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  return FSX_OK;
 }
 
 int fsx_unet_limit(fsx_ctx* ctx, int learned_maxit) {

@@ -39,7 +39,7 @@ DEVICE_SYMBOLS = (
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
-    "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats",
+    "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats", "fsx_unet_form", "fsx_unet_form_get", "fsx_unet_precondition",
 )
 HOST_SYMBOLS = (
     "fsxh_mesh_cavity", "fsxh_mesh_duct", "fsxh_mesh_read_inp", "fsxh_mesh_write_inp", "fsxh_mesh_free",
@@ -136,6 +136,9 @@ def lib():
         "fsx_unet_prepare": ([_vp, _I, ctypes.c_int, ctypes.c_int, _I], ctypes.c_int),
         "fsx_unet_train": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _D], ctypes.c_int),
         "fsx_unet_limit": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_unet_form": ([_vp, ctypes.c_int, ctypes.c_float], ctypes.c_int),
+        "fsx_unet_form_get": ([_vp, _I, _F], ctypes.c_int),
+        "fsx_unet_precondition": ([_vp, _F, _F], ctypes.c_int),
         "fsx_unet_stats": ([_vp, _I, _I], ctypes.c_int),
         "fsxh_mesh_cavity": ([ctypes.c_int, ctypes.c_double, ctypes.c_double], _vp),
         "fsxh_mesh_duct": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -660,6 +663,21 @@ class Solver:
         loss = ctypes.c_double()
         self.check(self.L.fsx_unet_train(self.ctx, steps, batch, lr, seed, ctypes.byref(loss)), "unet_train")
         return loss.value
+
+    def unet_form(self, form, omega=0.8):
+        self.check(self.L.fsx_unet_form(self.ctx, form, omega), "unet_form")
+
+    def unet_form_get(self):
+        f = ctypes.c_int()
+        om = ctypes.c_float()
+        self.check(self.L.fsx_unet_form_get(self.ctx, ctypes.byref(f), ctypes.byref(om)), "unet_form_get")
+        return f.value, om.value
+
+    def unet_precondition(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        z = np.zeros_like(r)
+        self.check(self.L.fsx_unet_precondition(self.ctx, _fp(r), _fp(z)), "unet_precondition")
+        return z
 
     def unet_limit(self, learned_maxit):
         self.check(self.L.fsx_unet_limit(self.ctx, learned_maxit), "unet_limit")

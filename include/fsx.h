@@ -367,6 +367,23 @@ int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsampl
  * of the last 50 steps (the fraction of error energy the preconditioner
  * leaves) */
 int fsx_unet_train(fsx_ctx* ctx, int steps, int batch, float lr, unsigned long long seed, double* loss);
+/* Form of the preconditioner M that the flexible PCG applies:
+ * form 0, additive: z = dinv r + gain J^T J r;
+ * form 1, two-level (the default): z = S r + C r1 + S (r - Z (S r + C r1)),
+ * r1 = r - Z S r, with S = omega D^-1 (one Jacobi sweep) and C = gain J^T J
+ * the network as the coarse correction, so I - M Z = (I - S Z)(I - C Z)(I - S Z)
+ * and M is SPD for any weights while omega < 2 / lambda_max(D^-1 Z) (about 2
+ * on the cavity meshes: omega = 0.8 by default). Form 1 needs a structured
+ * pressure grid whose Z rows compress into classes (its three Z products read
+ * LDS-staged tiles of the grid); elsewhere the solver uses form 0. Training
+ * pairs (fsx_unet_prepare) follow the form set when they are made. omega is
+ * taken for form 1 only and must lie in (0, 1). No reference counterpart. */
+int fsx_unet_form(fsx_ctx* ctx, int form, float omega);
+/* the form the learned solver runs (1 only where the tiles exist) and omega */
+int fsx_unet_form_get(fsx_ctx* ctx, int* form_in_use, float* omega);
+/* z = M r for host arrays of NNp floats, with the form in use (a test and
+ * tooling entry point; overwrites the solver's work vectors) */
+int fsx_unet_precondition(fsx_ctx* ctx, const float* r, float* z);
 /* flexible-PCG iterations allowed before the Jacobi-PCG fallback (default 400) */
 int fsx_unet_limit(fsx_ctx* ctx, int learned_maxit);
 /* flexible-PCG iterations of the last learned solve and why it fell back to

@@ -255,3 +255,21 @@ def loss_grad(x, t, w, shape, gain=1.0, exact=True):
     tt = float(t @ t)
     d = y - t
     return float(d @ d) / tt, backward(2.0 * d / tt, c), y
+
+
+def precondition_additive(r, w, shape, gain, dinv, exact=True):
+    """The additive form the flexible PCG can apply: z = dinv r + gain J^T J r."""
+    return dinv * r + forward(r, w, shape, gain, exact)[0]
+
+
+def precondition_two_level(r, w, shape, gain, omega, dinv, zop, exact=True):
+    """The two-level form: S = omega diag(dinv) (one Jacobi sweep) before and
+    after the coarse correction C = gain J^T J (the network),
+
+        z1 = S r,  r1 = r - Z z1,  z2 = z1 + C r1,  z = z2 + S (r - Z z2),
+
+    so that I - M Z = (I - S Z)(I - C Z)(I - S Z). zop(v) applies Z."""
+    z1 = omega * dinv * r
+    r1 = r - zop(z1)
+    z2 = z1 + forward(r1, w, shape, gain, exact)[0]
+    return z2 + omega * dinv * (r - zop(z2))

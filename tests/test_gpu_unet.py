@@ -175,6 +175,82 @@ def test_learned_time_steps_match_jacobi(fsx_lib):
     s.close()
 
 
+def host_z(s):
+    import scipy.sparse as sp
+    n = s.sizes["NNp"]
+    return sp.csr_matrix((s.download("Z"), s.topo.get("Zcol"), s.topo.get("Zrow")), shape=(n, n))
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_two_level_preconditioner_matches_oracle(fsx_lib, N):
+    """z = M r of the two-level form (the tiled Z products around the
+    network) and of the additive form against the oracle's restatement in the
+    device arithmetic model, on the context's Z: 2e-3 normwise."""
+    say(f"two-level preconditioner: grid {N + 1}^3")
+    s = cavity_solver(fsx_lib, N)
+    g = (N + 1,) * 3
+    s.unet_setup(g, seed=13)
+    form, omega = s.unet_form_get()
+    assert form == 1 and abs(omega - 0.8) < 1e-7
+    w = s.unet_weights().astype(np.float64)
+    gain = s.unet_gain()
+    Z = host_z(s)
+    dinv = (1.0 / Z.diagonal()).astype(np.float32).astype(np.float64)
+    r = np.random.default_rng(N + 3).normal(size=(N + 1) ** 3).astype(np.float32)
+    z = s.unet_precondition(r)
+    ref = U.precondition_two_level(r.astype(np.float64), w, g, gain, float(np.float32(omega)), dinv, lambda v: Z @ v,
+                                   exact=False)
+    e = rel(z, ref)
+    say(f"  two-level rel err {e:.2e}")
+    assert e < 2e-3
+    s.unet_form(0)
+    assert s.unet_form_get()[0] == 0
+    za = s.unet_precondition(r)
+    ea = rel(za, U.precondition_additive(r.astype(np.float64), w, g, gain, dinv, exact=False))
+    say(f"  additive rel err {ea:.2e}")
+    assert ea < 2e-3
+    s.close()
+
+
+def test_two_level_solve_beats_additive_and_jacobi(fsx_lib):
+    """On a 16^3 cavity right-hand side with the untrained (multilevel)
+    network: the two-level flexible CG needs fewer iterations than the
+    additive one, which needs fewer than Jacobi-PCG; every solve meets
+    ||b - Z x|| <= 1e-8 ||b|| on the FP64 Z; a warm start from a nearby x0
+    takes fewer iterations and meets the same test."""
+    say("two-level solve: cavity 16^3, NGP 8")
+    s = cavity_solver(fsx_lib, 16, steps=2)
+    b = s.download("R2")
+    Z = host_z(s)
+    s.unet_setup((17, 17, 17), seed=7)
+    s.set_solver(1, 1e-8, 20000)
+    xj, itj, _, rc = s.solve_pressure(b)
+    assert rc == 0
+    res = {}
+    for form in (0, 1):
+        s.unet_form(form)
+        s.set_solver(2, 1e-8, 20000)
+        x, it, relres, rc = s.solve_pressure(b)
+        lit, fb = s.unet_stats()
+        tr = float(np.linalg.norm(b - Z @ x) / np.linalg.norm(b))
+        say(f"  form {form}: {it} it (flexible {lit}, fallback {fb}), device {relres:.2e}, host {tr:.2e}")
+        assert rc == 0 and fb == 0 and tr <= 1.0001e-8
+        assert rel(x, xj) < 1e-5
+        res[form] = it
+    say(f"  jacobi {itj}")
+    assert res[1] < res[0] < itj
+    # warm start (the previous solve's answer, perturbed) on the device
+    s.set_solver_x0(1)
+    s.upload("Pdot", xj * (1.0 + 1e-3))
+    s.set_solver(2, 1e-8, 20000)
+    b2 = s.download("R2")
+    x2, it2, rel2, rc2 = s.solve_pressure(b2)
+    tr2 = float(np.linalg.norm(b2 - Z @ x2) / np.linalg.norm(b2))
+    say(f"  warm start: {it2} it, host {tr2:.2e}")
+    assert rc2 == 0 and tr2 <= 1.0001e-8 and it2 < res[1]
+    s.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_graph_replay_matches_eager(fsx_lib, mode):
     """Solver iterations replayed from captured HIP graphs give bit-identical
@@ -185,6 +261,7 @@ def test_graph_replay_matches_eager(fsx_lib, mode):
     b = s.download("R2")
     if mode == 2:
         s.unet_setup((17, 17, 17), seed=5)
+        s.unet_form(0)   # graphs are the additive form's (the tiled form launches directly)
     s.set_solver(mode, 1e-8, 20000)
     s.set_graphs(False)
     xe, ite, _, rce = s.solve_pressure(b)
