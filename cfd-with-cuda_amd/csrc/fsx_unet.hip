@@ -525,6 +525,35 @@ __global__ __launch_bounds__(kBlock) void k_eloss(int n, const float* __restrict
   write_partials<1>(v, partials);
 }
 
+// The two-level loss: with d = y - t (the coarse correction's error, y = C r1
+// and t = e1 the error after pre-smoothing), the error left by the whole
+// preconditioner is f = (I - S Z) d, S = omega diag(dinv). These kernels form
+// f = d - S (Z d) in place of d, then f.(Z f) with ws = S (Z f), and last the
+// gradient 2 (I - Z S) Z f / tt = 2 (Z f - Z ws) / tt of f.Z f / tt.
+__global__ void k_smooth_sub_f(int n, float omega, const double* __restrict__ dinv, const float* __restrict__ zd,
+                               float* __restrict__ d) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    d[i] = d[i] - omega * (float)dinv[i] * zd[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_eloss2a(int n, float omega, const double* __restrict__ dinv,
+                                                    const float* __restrict__ f, const float* __restrict__ zf,
+                                                    float* __restrict__ ws, double* __restrict__ partials) {
+  double v[1] = {0.0};
+#pragma unroll 4   // independent elements: their loads in flight together
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) {
+    const float q = zf[i];
+    v[0] += (double)f[i] * (double)q;
+    ws[i] = omega * (float)dinv[i] * q;
+  }
+  write_partials<1>(v, partials);
+}
+
+__global__ void k_eloss2b(int n, const float* __restrict__ zf, double tt, float* __restrict__ gy) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    gy[i] = (float)(2.0 * ((double)zf[i] - (double)gy[i]) / tt);
+}
+
 __global__ void k_sub_f(int n, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ c) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = a[i] - b[i];
 }
@@ -571,19 +600,21 @@ struct fsx_unet {
   std::vector<Layer> L;     // C0, C1, C2
   size_t nparam = 0;        // padded parameter count (rows of stride kp)
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  float *x = nullptr, *y = nullptr, *tgt = nullptr, *gy = nullptr, *zq = nullptr;   // pressure grid
+  float *x = nullptr, *y = nullptr, *tgt = nullptr, *gy = nullptr, *zq = nullptr, *ws = nullptr;   // pressure grid
   Acts A, B;                // A: the preconditioner / the sample; B: the loss gradient (training)
   float* wpart = nullptr;
   float* h0 = nullptr;      // first layer's transpose taps on G0, 32 per node (27 used)
   // training set (normalised pairs, FP32)
-  int ncap = 0, cap_max = 0;
+  int ncap = 0, cap_max = 0, cap_limit = 0;   // captured, allocated, asked for
   double *cap_b = nullptr, *cap_x = nullptr;     // captured (R2, Pdot) pairs
   int nsamp = 0, samp_max = 0;
   float *s_in = nullptr, *s_tgt = nullptr;
   std::vector<double> s_tt;   // per-pair loss normaliser (error energy)
+  int s_kind = 1;             // loss of the pairs: 1 energy (additive form), 2 two-level energy
   int adam_t = 0;
   float gain = 1.0f;        // fixed output gain of the network
-  double* tmp_x = nullptr;  // truncated iterate while preparing samples
+  double* tmp_x = nullptr;  // truncated iterate (or drawn error) while preparing samples
+  double* zero_x = nullptr; // zeros: the iterate of a drawn error's pair
 };
 
 namespace {
@@ -790,7 +821,7 @@ int setup_unet(fsx_ctx* ctx, int nx, int ny, int nz, unsigned long long seed, fs
     u->gain = (float)(0.25 * dinv[dinv.size() / 2]);
   }
   const size_t nf = u->gf.n();
-  for (float** p : {&u->x, &u->y, &u->tgt, &u->gy, &u->zq}) FSX_TRY(ualloc(ctx, p, nf));
+  for (float** p : {&u->x, &u->y, &u->tgt, &u->gy, &u->zq, &u->ws}) FSX_TRY(ualloc(ctx, p, nf));
   FSX_TRY(alloc_acts(ctx, u.get(), u->A));
   FSX_TRY(alloc_acts(ctx, u.get(), u->B));
   FSX_TRY(ualloc(ctx, &u->wpart, (size_t)kWChunks * 64 * 27 * 64));
@@ -978,6 +1009,28 @@ __global__ __launch_bounds__(kBlock) void k_make_sample2(int n, const double* __
 __global__ void k_scaled_dinv(int n, double w, const double* __restrict__ dinv, const double* __restrict__ r,
                               double* __restrict__ t) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) t[i] = w * dinv[i] * r[i];
+}
+
+// e_i ~ N(0, 1), seeded by (seed, draw) and the node index (SplitMix64
+// finaliser for two uniforms, Box-Muller), 0 at the pinned pressure node
+__device__ inline unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_randn(int n, unsigned long long seed, int draw, int pin, double* __restrict__ e) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned long long k = mix64(seed + 0x9e3779b97f4a7c15ULL * (unsigned long long)(draw + 1)) + 2ULL * i;
+    const double u1 = ((mix64(k) >> 11) + 1) * 0x1.0p-53, u2 = (mix64(k + 1) >> 11) * 0x1.0p-53;
+    e[i] = i == pin ? 0.0 : std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+}
+
+// e <- e - omega dinv (Z e), one damped Jacobi sweep on the error
+__global__ void k_jacobi_err(int n, double omega, const double* __restrict__ dinv, const double* __restrict__ ze,
+                             double* __restrict__ e) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) e[i] -= omega * dinv[i] * ze[i];
 }
 
 __global__ void k_stop_clear(SolveScalars* sc) {
@@ -1292,7 +1345,7 @@ namespace {
 // stores (R2, Pdot) of every converged pressure solve while capacity lasts
 void capture_pair(fsx_ctx* ctx, const double* b, const double* x) {
   fsx_unet* u = ctx->unet;
-  if (!u || u->ncap >= u->cap_max) return;
+  if (!u || u->ncap >= u->cap_limit) return;
   const size_t n = ctx->NNp;
   (void)hipMemcpyAsync(u->cap_b + (size_t)u->ncap * n, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
   (void)hipMemcpyAsync(u->cap_x + (size_t)u->ncap * n, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
@@ -1326,10 +1379,20 @@ int train_sample(fsx_ctx* ctx, fsx_unet* u, int kind, const float* x, const floa
   FSX_TRY(forward(ctx, u, x, u->y));
   if (kind == 0) {
     hipLaunchKernelGGL(k_loss, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, u->y, t, tt, u->gy, ctx->partials);
-  } else {
+  } else if (kind == 1) {
     hipLaunchKernelGGL(k_sub_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, u->y, t, u->gy);
     spmv_f(ctx, u->gy, u->zq);
     hipLaunchKernelGGL(k_eloss, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, u->zq, tt, u->gy, ctx->partials);
+  } else {
+    const float om = ctx->learned_omega;
+    hipLaunchKernelGGL(k_sub_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, u->y, t, u->gy);
+    spmv_f(ctx, u->gy, u->zq);
+    hipLaunchKernelGGL(k_smooth_sub_f, dim3(256), dim3(kBlock), 0, ctx->stream, n, om, ctx->Zdinv, u->zq, u->gy);
+    spmv_f(ctx, u->gy, u->zq);
+    hipLaunchKernelGGL(k_eloss2a, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, om, ctx->Zdinv, u->gy, u->zq, u->ws,
+                       ctx->partials);
+    spmv_f(ctx, u->ws, u->gy);
+    hipLaunchKernelGGL(k_eloss2b, dim3(256), dim3(kBlock), 0, ctx->stream, n, u->zq, tt, u->gy);
   }
   FSX_TRY(launch_check(ctx, "unet loss"));
   FSX_TRY(backward(ctx, u, x));
@@ -1338,6 +1401,51 @@ int train_sample(fsx_ctx* ctx, fsx_unet* u, int kind, const float* x, const floa
     FSX_TRY(sum_partials_host(ctx, &d2));
     *loss = d2 / tt;
   }
+  return FSX_OK;
+}
+
+// Training pair s from the residual r = b - Z x_k in ctx->wh (|r|^2 partial
+// sums in slot 0 of ctx->partials) and the error xs - xk of the iterate, in
+// the additive (input r, target e - dinv r) or the two-level form (input
+// r1 = r - Z S r, target e - S r); the pair's error energy goes to s_tt[s].
+int make_pair(fsx_ctx* ctx, fsx_unet* u, bool two_level, const double* xs, const double* xk, int s) {
+  const int n = ctx->NNp;
+  if (two_level) {
+    // t = omega dinv r, r1 = r - Z t (|r|^2 stays in slot 0)
+    hipLaunchKernelGGL(k_scaled_dinv, dim3(256), dim3(kBlock), 0, ctx->stream, n, (double)ctx->learned_omega, ctx->Zdinv,
+                       ctx->wh, ctx->w0e);
+    FSX_TRY(launch_warm_residual(ctx, ctx->w0e, ctx->wh, ctx->w0r));
+    hipLaunchKernelGGL(k_make_sample2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->w0r, ctx->w0e, xs, xk,
+                       ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
+  } else {
+    hipLaunchKernelGGL(k_make_sample, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, xs, xk, ctx->Zdinv,
+                       ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
+  }
+  FSX_TRY(launch_check(ctx, "k_make_sample"));
+  FSX_TRY(sum_partials_host(ctx, &u->s_tt[s], 1));
+  if (!(u->s_tt[s] > 0)) return fail(ctx, FSX_ENUMERIC, "a training sample has no error energy");
+  return FSX_OK;
+}
+
+// room for `total` pairs, keeping the first `keep`
+int reserve_pairs(fsx_ctx* ctx, fsx_unet* u, int total, int keep) {
+  const size_t n = ctx->NNp;
+  if (total > u->samp_max) {
+    float *ni = nullptr, *nt = nullptr;
+    FSX_TRY(alloc(ctx, &ni, (size_t)total * n));
+    FSX_TRY(alloc(ctx, &nt, (size_t)total * n));
+    if (keep > 0) {
+      FSX_HIP(hipMemcpyAsync(ni, u->s_in, (size_t)keep * n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+      FSX_HIP(hipMemcpyAsync(nt, u->s_tgt, (size_t)keep * n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    FSX_HIP(hipStreamSynchronize(ctx->stream));
+    if (u->s_in) { dev_free(ctx, u->s_in); dev_free(ctx, u->s_tgt); }
+    u->s_in = ni;
+    u->s_tgt = nt;
+    u->samp_max = total;
+  }
+  u->s_tt.resize(total, 0.0);
+  if (!u->tmp_x) FSX_TRY(alloc(ctx, &u->tmp_x, n));
   return FSX_OK;
 }
 
@@ -1398,7 +1506,7 @@ int fsx_unet_apply(fsx_ctx* ctx, const float* x, float* y) {
 
 int fsx_unet_loss_grad(fsx_ctx* ctx, int kind, const float* x, const float* t, double* loss, float* grad) {
   FSX_TRY(need_unet(ctx));
-  if (kind != 0 && kind != 1) return fail(ctx, FSX_EINVAL, "loss kind must be 0 (L2) or 1 (energy)");
+  if (kind < 0 || kind > 2) return fail(ctx, FSX_EINVAL, "loss kind must be 0 (L2), 1 (energy) or 2 (two-level energy)");
   fsx_unet* u = ctx->unet;
   const size_t n = ctx->NNp;
   FSX_HIP(hipMemcpyAsync(u->x, x, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
@@ -1426,11 +1534,14 @@ int fsx_unet_capture(fsx_ctx* ctx, int capacity) {
   if (capacity <= 0) { ctx->capture_hook = nullptr; return FSX_OK; }
   if (capacity > u->cap_max) {
     if (u->cap_b) { dev_free(ctx, u->cap_b); dev_free(ctx, u->cap_x); }
+    u->cap_b = u->cap_x = nullptr;
+    u->cap_max = u->cap_limit = u->ncap = 0;
     FSX_TRY(alloc(ctx, &u->cap_b, (size_t)capacity * ctx->NNp));
     FSX_TRY(alloc(ctx, &u->cap_x, (size_t)capacity * ctx->NNp));
-    u->cap_max = capacity;
-    u->ncap = 0;
   }
+  u->cap_max = std::max(u->cap_max, capacity);
+  u->cap_limit = capacity;
+  u->ncap = 0;   // a new capture replaces the previous one
   ctx->capture_hook = capture_pair;
   return FSX_OK;
 }
@@ -1457,24 +1568,10 @@ int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsampl
   const bool use_learned = flags & 1, append = (flags & 2) && u->nsamp > 0;
   const bool two_level = ctx->learned_form == 1 && ctx->tile.ntiles > 0;
   const int n = ctx->NNp;
+  if (append && u->s_kind != (two_level ? 2 : 1))
+    return fail(ctx, FSX_ESTATE, "appended training pairs must be of the same preconditioner form");
   const int start = append ? u->nsamp : 0;
-  const int total = start + u->ncap * nk;
-  if (total > u->samp_max) {
-    float *ni = nullptr, *nt = nullptr;
-    FSX_TRY(alloc(ctx, &ni, (size_t)total * n));
-    FSX_TRY(alloc(ctx, &nt, (size_t)total * n));
-    if (start > 0) {
-      FSX_HIP(hipMemcpyAsync(ni, u->s_in, (size_t)start * n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-      FSX_HIP(hipMemcpyAsync(nt, u->s_tgt, (size_t)start * n * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    FSX_HIP(hipStreamSynchronize(ctx->stream));
-    if (u->s_in) { dev_free(ctx, u->s_in); dev_free(ctx, u->s_tgt); }
-    u->s_in = ni;
-    u->s_tgt = nt;
-    u->samp_max = total;
-  }
-  u->s_tt.resize(total, 0.0);
-  if (!u->tmp_x) FSX_TRY(alloc(ctx, &u->tmp_x, (size_t)n));
+  FSX_TRY(reserve_pairs(ctx, u, start + u->ncap * nk, start));
   const double saved_rtol = ctx->rtol;
   const int saved_maxit = ctx->maxit, saved_last = ctx->last_iters;
   const int saved_li = ctx->last_learned_iters, saved_fb = ctx->last_fallback;
@@ -1512,27 +1609,48 @@ int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsampl
       // an iterate closer than 1e-5 ||b|| to the solution is compared with x*
       // that is itself only good to the solver's tolerance: no information
       if (!(rr >= 1e-10 * bb)) continue;
-      if (two_level) {
-        // t = omega dinv r, r1 = r - Z t (|r|^2 stays in slot 0)
-        hipLaunchKernelGGL(k_scaled_dinv, dim3(256), dim3(kBlock), 0, ctx->stream, n, (double)ctx->learned_omega, ctx->Zdinv,
-                           ctx->wh, ctx->w0e);
-        FSX_TRY(launch_warm_residual(ctx, ctx->w0e, ctx->wh, ctx->w0r));
-        hipLaunchKernelGGL(k_make_sample2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->w0r, ctx->w0e, xs,
-                           u->tmp_x, ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
-      } else {
-        hipLaunchKernelGGL(k_make_sample, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, xs, u->tmp_x, ctx->Zdinv,
-                           ctx->partials, u->s_in + (size_t)s * n, u->s_tgt + (size_t)s * n);
-      }
-      FSX_TRY(launch_check(ctx, "k_make_sample"));
-      FSX_TRY(sum_partials_host(ctx, &u->s_tt[s], 1));
-      if (!(u->s_tt[s] > 0)) { restore(); return fail(ctx, FSX_ENUMERIC, "a training sample has no error energy"); }
+      const int rc = make_pair(ctx, u, two_level, xs, u->tmp_x, s);
+      if (rc != FSX_OK) { restore(); return rc; }
       s++;
     }
   }
   restore();
   if (s == 0) return fail(ctx, FSX_ESTATE, "no informative training pairs (all truncated iterates converged)");
   u->nsamp = s;
+  u->s_kind = two_level ? 2 : 1;
   if (nsamples) *nsamples = s;
+  return fsx_sync(ctx);
+}
+
+// Training pairs without solves: seeded errors e = (I - S Z)^sweeps e0, e0
+// i.i.d. normal per node (0 at the pinned node), r = Z e in FP64, made into
+// pairs of the current form like fsx_unet_prepare's and appended to them.
+int fsx_unet_prepare_random(fsx_ctx* ctx, int count, int sweeps, unsigned long long seed, int* nsamples) {
+  FSX_TRY(need_unet(ctx));
+  fsx_unet* u = ctx->unet;
+  if (count < 1 || sweeps < 0) return fail(ctx, FSX_EINVAL, "count >= 1 and sweeps >= 0 required");
+  const bool two_level = ctx->learned_form == 1 && ctx->tile.ntiles > 0;
+  if (u->nsamp > 0 && u->s_kind != (two_level ? 2 : 1))
+    return fail(ctx, FSX_ESTATE, "appended training pairs must be of the same preconditioner form");
+  const int n = ctx->NNp, start = u->nsamp;
+  FSX_TRY(reserve_pairs(ctx, u, start + count, start));
+  if (!u->zero_x) FSX_TRY(alloc(ctx, &u->zero_x, (size_t)n));
+  FSX_TRY(launch_fill(ctx, u->zero_x, 0.0, n));
+  const double om = ctx->learned_omega;
+  for (int p = 0; p < count; p++) {
+    hipLaunchKernelGGL(k_randn, dim3(256), dim3(kBlock), 0, ctx->stream, n, seed, p, ctx->zeroP, u->tmp_x);
+    for (int k = 0; k < sweeps; k++) {
+      FSX_TRY(spmv_host_op(ctx, 4, u->tmp_x, ctx->wh, 1.0, 0.0));
+      hipLaunchKernelGGL(k_jacobi_err, dim3(256), dim3(kBlock), 0, ctx->stream, n, om, ctx->Zdinv, ctx->wh, u->tmp_x);
+    }
+    FSX_TRY(spmv_host_op(ctx, 4, u->tmp_x, ctx->wh, 1.0, 0.0));
+    hipLaunchKernelGGL(k_sumsq_u, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, ctx->wh, ctx->partials);
+    FSX_TRY(launch_check(ctx, "random pair"));
+    FSX_TRY(make_pair(ctx, u, two_level, u->tmp_x, u->zero_x, start + p));
+  }
+  u->nsamp = start + count;
+  u->s_kind = two_level ? 2 : 1;
+  if (nsamples) *nsamples = u->nsamp;
   return fsx_sync(ctx);
 }
 
@@ -1554,7 +1672,7 @@ int fsx_unet_train(fsx_ctx* ctx, int steps, int batch, float lr, unsigned long l
     for (int b = 0; b < batch; b++) {
       const int idx = (int)(rng() % (unsigned long long)u->nsamp);
       double l = 0;
-      FSX_TRY(train_sample(ctx, u, 1, u->s_in + (size_t)idx * n, u->s_tgt + (size_t)idx * n, u->s_tt[idx], &l));
+      FSX_TRY(train_sample(ctx, u, u->s_kind, u->s_in + (size_t)idx * n, u->s_tgt + (size_t)idx * n, u->s_tt[idx], &l));
       lsum += l;
     }
     u->adam_t++;

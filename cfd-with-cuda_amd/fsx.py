@@ -38,7 +38,7 @@ DEVICE_SYMBOLS = (
     "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
-    "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare",
+    "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats", "fsx_unet_form", "fsx_unet_form_get", "fsx_unet_precondition",
 )
 HOST_SYMBOLS = (
@@ -134,6 +134,7 @@ def lib():
         "fsx_unet_capture": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_unet_captured": ([_vp, _I], ctypes.c_int),
         "fsx_unet_prepare": ([_vp, _I, ctypes.c_int, ctypes.c_int, _I], ctypes.c_int),
+        "fsx_unet_prepare_random": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _I], ctypes.c_int),
         "fsx_unet_train": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _D], ctypes.c_int),
         "fsx_unet_limit": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_unet_form": ([_vp, ctypes.c_int, ctypes.c_float], ctypes.c_int),
@@ -363,6 +364,7 @@ class Solver:
         self.ctx = _vp(self.L.fsxd_ctx(self.h))
         self.topo = Topology(handle=self.L.fsxd_topo(self.h), owner=False)
         self.sizes = self.topo.info()
+        self.mesh_params = mesh.params()
 
     def close(self):
         if getattr(self, "h", None):
@@ -390,6 +392,21 @@ class Solver:
         return [dict(step=s.step, inner=s.inner_iters, pcg=s.pcg_iters, steady=s.steady, time=s.time,
                      max_acc=s.max_acc, mon=list(s.mon), wall_s=s.wall_s, relres=s.pcg_relres,
                      noconv=s.pcg_noconv, relres_max=s.pcg_relres_max) for s in stats[:n]]
+
+    def set_case(self, viscosity, lid=1.0):
+        """Another viscosity and lid speed on the same mesh and context: the
+        BC values of the mesh scaled by `lid` are uploaded, the static
+        matrices are assembled again with the new viscosity (K scales with it;
+        M, G and Z do not change) and the state goes back to the initial
+        fields (fsx_init_state via fsxd_reset)."""
+        p = self.mesh_params
+        self.check(self.L.fsx_set_params(self.ctx, self.opt.NGP, p["density"], float(viscosity), p["dt"]), "set_params")
+        nodes = np.ascontiguousarray(self.topo.get("BCnodes"), dtype=np.int32)
+        vals = np.ascontiguousarray(self.topo.get("BCvalues"), dtype=np.float64) * float(lid)
+        self.check(self.L.fsx_upload_bc(self.ctx, nodes.size, _ip(nodes), _dp(vals), int(self.sizes["zeroP"])),
+                   "upload_bc")
+        self.check(self.L.fsx_assemble_static(self.ctx), "assemble_static")
+        self.reset()
 
     def reset(self, mode=None, rtol=None, maxit=None):
         """Back to the initial fields; optionally switch the pressure solver."""
@@ -657,6 +674,13 @@ class Solver:
         n = ctypes.c_int()
         flags = (1 if learned else 0) | (2 if append else 0)
         self.check(self.L.fsx_unet_prepare(self.ctx, _ip(k), k.size, flags, ctypes.byref(n)), "unet_prepare")
+        return n.value
+
+    def unet_prepare_random(self, count, sweeps=0, seed=0):
+        """count training pairs from seeded drawn errors, appended; returns the pair count"""
+        n = ctypes.c_int()
+        self.check(self.L.fsx_unet_prepare_random(self.ctx, int(count), int(sweeps), int(seed), ctypes.byref(n)),
+                   "unet_prepare_random")
         return n.value
 
     def unet_train(self, steps, batch=4, lr=1e-3, seed=0):

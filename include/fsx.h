@@ -349,23 +349,38 @@ int fsx_unet_gain(fsx_ctx* ctx, float set, float* get);
 /* y = U(x) (s = 1) for host arrays of NNp floats */
 int fsx_unet_apply(fsx_ctx* ctx, const float* x, float* y);
 /* loss of y = gain * U(x) against t and its parameter gradient (nparam
- * floats): kind 0 relative L2 |y - t|^2 / |t|^2, kind 1 relative energy norm
- * (y - t)' Z (y - t) / t' Z t on the FP32 Z (the training loss) */
+ * floats), all on the FP32 Z: kind 0 relative L2 |y - t|^2 / |t|^2, kind 1
+ * relative energy norm (y - t)' Z (y - t) / t' Z t (the additive form's
+ * training loss), kind 2 the energy norm after post-smoothing,
+ * f' Z f / t' Z t with f = (I - S Z)(y - t) and S = omega D^-1 of the
+ * two-level form (its training loss) */
 int fsx_unet_loss_grad(fsx_ctx* ctx, int kind, const float* x, const float* t, double* loss, float* grad);
 /* keep the right-hand sides and solutions of the next `capacity` converged
- * pressure solves (0 stops capturing) */
+ * pressure solves (0 stops capturing); a call with capacity > 0 drops what an
+ * earlier capture kept */
 int fsx_unet_capture(fsx_ctx* ctx, int capacity);
 int fsx_unet_captured(fsx_ctx* ctx, int* n);
 /* training pairs from the captured solves, one per truncation depth ks[q]:
- * the residual of the ks[q]-th iterate from x = 0 of FP64 Jacobi-PCG (flags
- * bit 0 clear) or of the learned flexible PCG (bit 0 set), with the error
- * that remains as target; bit 1 appends to the pairs already held, otherwise
- * they are replaced and the output gain is set */
+ * the residual r of the ks[q]-th iterate from x = 0 of FP64 Jacobi-PCG (flags
+ * bit 0 clear) or of the learned flexible PCG (bit 0 set) and the error e
+ * that remains, both scaled by 1 / rms(r). In the additive form the input is
+ * r and the target e - dinv r; in the two-level form the input is
+ * r1 = r - Z S r and the target e - S r (the error after pre-smoothing).
+ * Bit 1 appends to the pairs already held (of the same form), otherwise they
+ * are replaced. */
 int fsx_unet_prepare(fsx_ctx* ctx, const int* ks, int nk, int flags, int* nsamples);
-/* Adam on the energy-norm loss (y - t)' Z (y - t) normalised per pair by its
- * error energy, seeded mini-batches of prepared pairs; returns the mean loss
- * of the last 50 steps (the fraction of error energy the preconditioner
- * leaves) */
+/* `count` more training pairs, of the form in use, without solves: seeded
+ * errors e = (I - S Z)^sweeps e0 with e0 i.i.d. normal per node (0 at the
+ * pinned node), S = omega D^-1, and r = Z e; appended to the pairs already
+ * held. Drawn errors cover every mode of Z, so that training does not trade
+ * modes absent from the captured solves for the ones present. */
+int fsx_unet_prepare_random(fsx_ctx* ctx, int count, int sweeps, unsigned long long seed, int* nsamples);
+/* Adam on seeded mini-batches of the prepared pairs, normalised per pair by
+ * its error energy e' Z e: in the additive form the energy norm
+ * (y - t)' Z (y - t), in the two-level form the energy f' Z f of the error
+ * f = (I - S Z)(t - y) that the whole preconditioner leaves (kinds 1 and 2
+ * of fsx_unet_loss_grad). Returns the mean loss of the last 50 steps (the
+ * fraction of error energy that one application leaves). */
 int fsx_unet_train(fsx_ctx* ctx, int steps, int batch, float lr, unsigned long long seed, double* loss);
 /* Form of the preconditioner M that the flexible PCG applies:
  * form 0, additive: z = dinv r + gain J^T J r;

@@ -62,7 +62,7 @@ def test_unet_forward_matches_oracle(fsx_lib, N):
     s.close()
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_unet_gradient_matches_oracle(fsx_lib, kind):
     say(f"unet loss gradient: grid 9^3, loss kind {kind}")
     s = cavity_solver(fsx_lib, 8)
@@ -77,7 +77,7 @@ def test_unet_gradient_matches_oracle(fsx_lib, kind):
     loss, grad = s.unet_loss_grad(x, t, kind=kind)
     if kind == 0:
         loss_ref, grad_ref, _ = U.loss_grad(x, t, w, g, gain=gain, exact=False)
-    else:
+    elif kind == 1:
         # energy norm on the context's FP64 Z (the device trains on its FP32 copy)
         y, cache = U.forward(x, w, g, gain=gain, exact=False)
         t64 = t.astype(np.float64)
@@ -85,6 +85,28 @@ def test_unet_gradient_matches_oracle(fsx_lib, kind):
         zd = s.spmv("Z", y - t64)
         loss_ref = float((y - t64) @ zd) / tt
         grad_ref = U.backward(2.0 * zd / tt, cache)
+    else:
+        # energy after post-smoothing, f = (I - S Z)(y - t), S = omega D^-1:
+        # the loss of f' Z f is quadratic in y with gradient 2 (I - Z S) Z f
+        y, cache = U.forward(x, w, g, gain=gain, exact=False)
+        t64 = t.astype(np.float64)
+        Z = host_z(s)
+        dinv = (1.0 / Z.diagonal()).astype(np.float32).astype(np.float64)
+        sw = float(np.float32(s.unet_form_get()[1])) * dinv
+        tt = float(t64 @ (Z @ t64))
+        f = (y - t64) - sw * (Z @ (y - t64))
+        zf = Z @ f
+        loss_ref = float(f @ zf) / tt
+        grad_ref = U.backward(2.0 * (zf - Z @ (sw * zf)) / tt, cache)
+        # the same gradient by central differences of the loss along y
+        dy = np.random.default_rng(2).normal(size=y.size)
+
+        def lossy(yy):
+            ff = (yy - t64) - sw * (Z @ (yy - t64))
+            return float(ff @ (Z @ ff)) / tt
+        h = 1e-3 * np.linalg.norm(y) / np.linalg.norm(dy)
+        fd = (lossy(y + h * dy) - lossy(y - h * dy)) / (2 * h)
+        assert abs(fd - 2.0 * (zf - Z @ (sw * zf)) @ dy / tt) <= 1e-6 * abs(fd) + 1e-12
     say(f"  loss {loss:.6f} oracle {loss_ref:.6f}")
     assert abs(loss - loss_ref) <= 1e-3 * loss_ref
     offs, total = U.layer_offsets()
@@ -271,4 +293,26 @@ def test_graph_replay_matches_eager(fsx_lib, mode):
         assert rcg == rce == 0 and itg == ite, (rep, itg, ite)
         assert np.array_equal(xg, xe), rep
         s.spmv("Z", xg)   # eager work between replays
+    s.close()
+
+
+def test_random_pairs_append_and_train(fsx_lib):
+    """fsx_unet_prepare_random appends pairs of the form in use, training on
+    them gives a finite loss below the initial one, and pairs of another form
+    are refused."""
+    say("random training pairs: grid 17^3")
+    s = cavity_solver(fsx_lib, 16)
+    s.unet_setup((17, 17, 17), seed=5)
+    assert s.unet_prepare_random(6, sweeps=4, seed=1) == 6
+    assert s.unet_prepare_random(4, sweeps=0, seed=2) == 10
+    # the loss over the same 64 seeded draws of the pairs before and after
+    # training (an Adam step of lr 1e-12 leaves the weights as they are)
+    l0 = s.unet_train(1, batch=64, lr=1e-12, seed=0)
+    s.unet_train(100, batch=4, lr=1e-4, seed=1)
+    l1 = s.unet_train(1, batch=64, lr=1e-12, seed=0)
+    say(f"  loss {l0:.4f} -> {l1:.4f}")
+    assert np.isfinite(l1) and 0 < l1 < l0
+    s.unet_form(0)
+    with pytest.raises(fsx_lib.FsxError):
+        s.unet_prepare_random(2, sweeps=0, seed=3)
     s.close()
