@@ -261,54 +261,9 @@ def duct_workload(fsx, args, steps, warmup, cpu_steps):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--re", type=float, default=1000.0)
-    ap.add_argument("--dt", type=float, default=0.0025)
-    ap.add_argument("--cluster", type=float, default=0.0,
-                    help="sinh clustering of the cavity mesh towards the walls (the reference generator's, 0 uniform)")
-    ap.add_argument("--ngp", type=int, default=27)
-    ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--maxit", type=int, default=20000)
-    ap.add_argument("--modes", default="fp64,mixed,learned,learned_untrained,direct",
-                    help="solver modes to run, of fp64, mixed, learned, learned_untrained and direct")
-    ap.add_argument("--train-iters", type=int, default=800, help="Adam steps of each training round")
-    ap.add_argument("--train-rounds", type=int, default=2,
-                    help="on-policy training rounds after the first (each re-prepares pairs from the learned "
-                         "solver's own truncated iterates)")
-    ap.add_argument("--train-runs", type=int, default=6, help="seeded cavity runs the network is trained on")
-    ap.add_argument("--train-steps", type=int, default=3, help="time steps of each training run")
-    ap.add_argument("--train-seed", type=int, default=2024)
-    ap.add_argument("--lr", type=float, default=1e-4)
-    ap.add_argument("--learned-form", type=int, default=1, choices=(0, 1),
-                    help="preconditioner of the learned mode: 1 two-level (Jacobi smoothing around the network, "
-                         "tiled Z products), 0 additive (dinv r + network)")
-    ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--probe", action="store_true",
-                    help="only set up the case, run one FP64 time step and one with FP32 matrices and convection, time "
-                         "the step kernels and the Z SpMV kernels and run a 256 MiB "
-                         "calibration copy (the workload of the rocprofv3 --pmc traffic runs)")
-    ap.add_argument("--x0", default="previous", choices=("zero", "previous"),
-                    help="initial guess of every pressure solve, in all modes: the previous Pdot (the reference's "
-                         "GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
-    ap.add_argument("--workloads", default="cavity8,cavity32,duct",
-                    help="other configurations to run beside the headline (empty for none)")
-    ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
-                    help="enqueue the FP64 and mixed solver iterations as replayed HIP graphs (1) or as individual "
-                         "launches (0)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="whole 64^3 time steps the CPU baseline times")
-    args = ap.parse_args()
-    global GRAPHS
-    GRAPHS = args.graphs
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the solver core is built "
-              f"and measured for one GPU; sharding by mesh partitions is designed, not built)", file=sys.stderr)
-        sys.exit(2)
+def headline(args):
+    """The 64^3 cavity in every solver mode of --modes: the JSON line's headline
+    fields (None after --probe)."""
     import numpy as np
     import fsx
 
@@ -460,6 +415,64 @@ def main():
     else:
         out["cpu_baseline"] = None
     mesh.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--re", type=float, default=1000.0)
+    ap.add_argument("--dt", type=float, default=0.0025)
+    ap.add_argument("--cluster", type=float, default=0.0,
+                    help="sinh clustering of the cavity mesh towards the walls (the reference generator's, 0 uniform)")
+    ap.add_argument("--ngp", type=int, default=27)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--maxit", type=int, default=20000)
+    ap.add_argument("--modes", default="fp64,mixed,learned,learned_untrained,direct",
+                    help="solver modes to run, of fp64, mixed, learned, learned_untrained and direct (none: only the "
+                         "--workloads)")
+    ap.add_argument("--train-iters", type=int, default=800, help="Adam steps of each training round")
+    ap.add_argument("--train-rounds", type=int, default=2,
+                    help="on-policy training rounds after the first (each re-prepares pairs from the learned "
+                         "solver's own truncated iterates)")
+    ap.add_argument("--train-runs", type=int, default=6, help="seeded cavity runs the network is trained on")
+    ap.add_argument("--train-steps", type=int, default=3, help="time steps of each training run")
+    ap.add_argument("--train-seed", type=int, default=2024)
+    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--learned-form", type=int, default=1, choices=(0, 1),
+                    help="preconditioner of the learned mode: 1 two-level (Jacobi smoothing around the network, "
+                         "tiled Z products), 0 additive (dinv r + network)")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--probe", action="store_true",
+                    help="only set up the case, run one FP64 time step and one with FP32 matrices and convection, time "
+                         "the step kernels and the Z SpMV kernels and run a 256 MiB "
+                         "calibration copy (the workload of the rocprofv3 --pmc traffic runs)")
+    ap.add_argument("--x0", default="previous", choices=("zero", "previous"),
+                    help="initial guess of every pressure solve, in all modes: the previous Pdot (the reference's "
+                         "GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
+    ap.add_argument("--workloads", default="cavity8,cavity32,duct",
+                    help="other configurations to run beside the headline (empty for none)")
+    ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
+                    help="enqueue the FP64 and mixed solver iterations as replayed HIP graphs (1) or as individual "
+                         "launches (0)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="whole 64^3 time steps the CPU baseline times")
+    args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = args.graphs
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the solver core is built "
+              f"and measured for one GPU; sharding by mesh partitions is designed, not built)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import fsx
+
+    out = headline(args) if args.modes != "none" else {"metric": METRIC, "value": None}
+    if out is None:   # --probe
+        return
     work = {}
     for wname in [w for w in args.workloads.split(",") if w]:
         if wname == "cavity8":
