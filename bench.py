@@ -65,7 +65,8 @@ def step_kernels(vt):
 
 
 def zspmv_name(prec, npat, ncls):
-    """rocprofv3 name of the Z SpMV kernel the FP64 / mixed solvers run."""
+    """rocprofv3 name of the Z SpMV kernel the FP64 / mixed solvers run
+    (npat: the solvers' row-pattern count, ncls: the FP32 row classes)."""
     if prec and ncls > 0:
         return "k_zcls_spmv<8, false, 2>"
     if npat > 0:   # the row-pattern form runs 8 lanes x 16 loads per row in both precisions

@@ -122,6 +122,36 @@ __global__ __launch_bounds__(kWave) void k_assemble(int ncolor_elems, const int*
   }
 }
 
+// detJ_k w_k of every element at every Gauss point (the f_k of k_assemble,
+// in the same arithmetic), for the element-by-element mass products:
+// fac[k * NE + e]
+__global__ void k_mass_fac(int NE, int NGP, const int* __restrict__ P8, const double* __restrict__ xc, ShapeTables tab,
+                           double* __restrict__ fac) {
+  const long long n = (long long)NE * NGP;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int e = (int)(t / NGP), k = (int)(t % NGP);
+    double ec[8][3];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const int pn = P8[(size_t)e * 8 + m];
+#pragma unroll
+      for (int d = 0; d < 3; d++) ec[m][d] = xc[3 * (size_t)pn + d];
+    }
+    double iJ[3][3], det;
+    jacobian(ec, tab.dSp + k * 24, iJ, &det);
+    fac[(size_t)k * NE + e] = det * tab.w[k];
+  }
+}
+
+// for the elements whose corner pressure ids P8 lists (NE x 8, any order)
+int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac) {
+  const long long n = (long long)ctx->NE * ctx->NGP;
+  const unsigned g = (unsigned)std::max(1LL, std::min((n + kBlock - 1) / kBlock, 8192LL));
+  hipLaunchKernelGGL(k_mass_fac, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NE, ctx->NGP, P8, ctx->xc, ctx->tab,
+                     fac);
+  return launch_check(ctx, "k_mass_fac");
+}
+
 // Lumped mass: Md = row sums of M (BCH:3262-3266), MdOrigInv = 1/Md
 // (BCH:3287-3289), MdInv = 1/Md with BC rows 1 (BCH:3292-3295, 4121-4135).
 __global__ void k_lump(int NN, const int* __restrict__ Mrow, const double* __restrict__ Mv,
@@ -206,10 +236,11 @@ __global__ __launch_bounds__(kWave) void k_build_z(int NNp, const int* __restric
   for (int s = t; s < zn; s += kWave) Z[zb + s] = (z1[s] + z2[s]) + z3[s];
 }
 
-// Penalty pin (BCH:4167-4177, only for node > 0), Jacobi inverse diagonal
-// and the FP32 copy used by the mixed-precision inner solve.
+// Penalty pin (BCH:4167-4177, only for node > 0) and the Jacobi inverse
+// diagonal in FP64 and FP32 (the solvers' FP32 values of Z are made from
+// the pruned rows, fsx_build_Z).
 __global__ void k_z_finish(int NNp, const int* __restrict__ Zrow, const int* __restrict__ Zcol, double* __restrict__ Z,
-                           float* __restrict__ Zf, double* __restrict__ dinv, float* __restrict__ dinvf, int zeroP) {
+                           double* __restrict__ dinv, float* __restrict__ dinvf, int zeroP) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < NNp; p += gridDim.x * blockDim.x) {
     double d = 0.0;
     for (int k = Zrow[p]; k < Zrow[p + 1]; k++) {
@@ -218,7 +249,6 @@ __global__ void k_z_finish(int NNp, const int* __restrict__ Zrow, const int* __r
         if (zeroP > 0 && p == zeroP) { v = v * 1000.0; Z[k] = v; }
         d = v;
       }
-      Zf[k] = (float)v;
     }
     dinv[p] = 1.0 / d;
     dinvf[p] = (float)(1.0 / d);
@@ -230,16 +260,16 @@ int launch_build_Z(fsx_ctx* ctx) {
                      ctx->GTrow, ctx->GTcol, ctx->GT, ctx->Grow, ctx->Gcol, ctx->G, ctx->MdOrigInv);
   FSX_TRY(launch_check(ctx, "k_build_z"));
   const int g = std::max(1, std::min((ctx->NNp + kBlock - 1) / kBlock, 4096));
-  hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z, ctx->Zf,
+  hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z,
                      ctx->Zdinv, ctx->Zdinvf, ctx->zeroP);
   return launch_check(ctx, "k_z_finish");
 }
 
-// The Jacobi inverses and the FP32 copy of Z values that already carry the
-// penalty pin (an uploaded Z): k_z_finish without pinning again.
+// The Jacobi inverses of Z values that already carry the penalty pin (an
+// uploaded Z): k_z_finish without pinning again.
 int launch_z_refresh(fsx_ctx* ctx) {
   const int g = std::max(1, std::min((ctx->NNp + kBlock - 1) / kBlock, 4096));
-  hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z, ctx->Zf,
+  hipLaunchKernelGGL(k_z_finish, dim3(g), dim3(kBlock), 0, ctx->stream, ctx->NNp, ctx->Zrow, ctx->Zcol, ctx->Z,
                      ctx->Zdinv, ctx->Zdinvf, -1);
   return launch_check(ctx, "k_z_finish");
 }

@@ -494,11 +494,12 @@ static int build_z_forms(fsx_ctx* ctx) {
     rp2[r + 1] = (int)ci2.size();
   }
   for (fsx::IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) iter_graph_reset(*g);
-  for (void* p : {(void*)ctx->Zr_row, (void*)ctx->Zr_val, (void*)ctx->Zr_col, (void*)ctx->Zr_pat, (void*)ctx->Zr_off,
+  for (void* p : {(void*)ctx->Zr_row, (void*)ctx->Zr_val, (void*)ctx->Zr_valf, (void*)ctx->Zr_col, (void*)ctx->Zr_pat, (void*)ctx->Zr_off,
                   (void*)ctx->Zc_ent, (void*)ctx->Zc_chunk, (void*)ctx->Zc_rows, (void*)ctx->Zc_delta,
                   (void*)ctx->Zc_doff})
     if (p) dev_free(ctx, p);
-  ctx->Zr_row = nullptr; ctx->Zr_val = nullptr; ctx->Zr_col = nullptr; ctx->Zr_pat = nullptr; ctx->Zr_off = nullptr;
+  ctx->Zr_row = nullptr; ctx->Zr_val = nullptr; ctx->Zr_valf = nullptr; ctx->Zr_col = nullptr; ctx->Zr_pat = nullptr;
+  ctx->Zr_off = nullptr;
   ctx->Zc_ent = nullptr; ctx->Zc_chunk = nullptr; ctx->Zc_rows = nullptr; ctx->Zc_delta = nullptr; ctx->Zc_doff = nullptr;
   ctx->Zr_npat = 0; ctx->Zr_patlen = 0; ctx->Zc_n = 0; ctx->Zc_len = 0; ctx->Zc_nchunk = 0;
   const long long nnz2 = rp2[n];
@@ -507,6 +508,9 @@ static int build_z_forms(fsx_ctx* ctx) {
   FSX_TRY(alloc(ctx, &ctx->Zr_val, (size_t)std::max(1LL, nnz2)));
   FSX_HIP(hipMemcpy(ctx->Zr_row, rp2.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice));
   FSX_HIP(hipMemcpy(ctx->Zr_val, v2.data(), (size_t)nnz2 * sizeof(double), hipMemcpyHostToDevice));
+  FSX_TRY(alloc(ctx, &ctx->Zr_valf, (size_t)std::max(1LL, nnz2)));
+  static_assert(sizeof(unsigned) == sizeof(float), "FP32 bits");
+  FSX_HIP(hipMemcpy(ctx->Zr_valf, f2.data(), (size_t)nnz2 * sizeof(float), hipMemcpyHostToDevice));
   std::vector<int> rowpat, pp, po;
   if (build_patterns(n, rp2.data(), ci2.data(), 1 << 16, 1 << 22, rowpat, pp, po)) {
     for (int& id : rowpat) id = pp[id];
@@ -679,20 +683,7 @@ int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const i
       if (!diag) return fail(ctx, FSX_EINVAL, "Z pattern lacks a diagonal entry");
     }
     ctx->Zrow = drow; ctx->Zcol = dcol; ctx->nnzZ = nnz; ctx->zrow_max = rmax;
-    {
-      std::vector<int> rp, pp, po;
-      if (build_patterns(nrows, rowptr, col, 1 << 16, 1 << 22, rp, pp, po)) {
-        for (int& id : rp) id = pp[id];   // rows keep where their pattern starts
-        FSX_TRY(alloc(ctx, &ctx->Zrowpat, rp.size()));
-        FSX_TRY(alloc(ctx, &ctx->Zpatoff, std::max<size_t>(1, po.size())));
-        FSX_HIP(hipMemcpy(ctx->Zrowpat, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice));
-        if (!po.empty()) FSX_HIP(hipMemcpy(ctx->Zpatoff, po.data(), po.size() * sizeof(int), hipMemcpyHostToDevice));
-        ctx->Znpat = (int)pp.size() - 1;
-        ctx->Zpatlen = (long long)po.size();
-      }
-    }
     FSX_TRY(alloc(ctx, &ctx->Z, (size_t)nnz));
-    FSX_TRY(alloc(ctx, &ctx->Zf, (size_t)nnz));
     FSX_TRY(alloc(ctx, &ctx->Zdinv, (size_t)nrows));
     FSX_TRY(alloc(ctx, &ctx->Zdinvf, (size_t)nrows));
     ctx->have_Z = true;
@@ -735,6 +726,7 @@ int fsx_assemble_static(fsx_ctx* ctx) {
   ctx->gpn_valid = false;
   FSX_TRY(refresh_matrix_copies(ctx));
   ctx->mass_dinv_ready = false;
+  ctx->mass_m_uploaded = false;   // M is the elements' sum again
   return FSX_OK;
 }
 
@@ -759,6 +751,7 @@ int fsx_init_state(fsx_ctx* ctx) {
                     ctx->KAcc, ctx->R1})
     FSX_TRY(launch_fill(ctx, v, 0.0, n3));
   for (double* v : {ctx->Pn, ctx->Pnp1, ctx->Pnp1_prev, ctx->Pdot, ctx->R2}) FSX_TRY(launch_fill(ctx, v, 0.0, ctx->NNp));
+  if (ctx->mass_x) FSX_TRY(launch_fill(ctx, ctx->mass_x, 0.0, n3));   // the consistent step1's warm start
   // applyBC_initial (BCH:4062-4107): Un at BC nodes <- BC values
   if (ctx->nbc > 0) {
     std::vector<int> nodes(ctx->nbc);
@@ -1005,6 +998,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
       // from until fsx_build_Z runs again)
       FSX_TRY(launch_lump(ctx));
       ctx->mass_dinv_ready = false;
+      ctx->mass_m_uploaded = true;   // consistent solves apply these CSR values
     }
   } else {
     FSX_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));

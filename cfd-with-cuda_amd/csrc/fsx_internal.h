@@ -161,6 +161,23 @@ struct fsx_ctx {
   double* mass_dinv = nullptr;   // NN, 1 / M_ii, 0 at BC nodes
   double* mass_sc = nullptr;     // device MassScalars
   bool mass_dinv_ready = false;
+  // the element-by-element form of M (FSX_MASS_CONSISTENT): detJ w at every
+  // Gauss point of every element (NE x NGP), the node -> (element, local
+  // node) incidence in ascending element order, and the per-element products
+  // (NE x 27 x 3); built at the first consistent solve. M values uploaded by
+  // fsx_upload_vec no longer match the elements: until the next
+  // fsx_assemble_static the solves then apply the CSR values.
+  // The element form visits the elements sorted by their centre node's
+  // internal id (the nodes' coordinate order), so that a block's elements
+  // and a node's elements sit close together in memory: mass_L27 holds
+  // their velocity ids in that order, and the factors and products follow it.
+  int* mass_L27 = nullptr;
+  double* mass_fac = nullptr;
+  int* mass_inc_ptr = nullptr;
+  int* mass_inc = nullptr;
+  double* mass_ye = nullptr;
+  bool mass_m_uploaded = false;
+  bool mass_elems_ready = false;   // the tables above are built
   int mass_last_iters = 0;
   double mass_last_relres = 0.0;
   long long mass_total_iters = 0;
@@ -235,12 +252,7 @@ struct fsx_ctx {
   double* GT = nullptr;          // nnzG*3, transposed pattern
   int* Zrow = nullptr; int* Zcol = nullptr;
   double* Z = nullptr; double* Zdinv = nullptr;
-  float* Zf = nullptr; float* Zdinvf = nullptr;
-  // Z columns as a dictionary of row patterns: col = row + Zpatoff[Zrowpat[row] + k - Zrow[row]]
-  // (Zrowpat holds where the row's pattern starts; null when the rows do not
-  // compress, and the SpMVs then read Zcol)
-  int* Zrowpat = nullptr; int* Zpatoff = nullptr;
-  int Znpat = 0; long long Zpatlen = 0;
+  float* Zdinvf = nullptr;
   int zrow_max = 0;
   // structured pressure grid (x fastest), detected at fsx_upload_mesh; 0 when
   // the pressure numbering is not a grid (e.g. a permuted mesh)
@@ -256,9 +268,10 @@ struct fsx_ctx {
   //    (Zr_pat, Zr_off as above) or, when the rows do not compress, Zr_col;
   //  * the FP32 copy as row classes when its rows take few distinct lists of
   //    (column offset, FP32 value): a class's entries sit in Zc_ent as
-  //    (offset, float bits); null otherwise (the FP32 solvers then read Zf
-  //    on the full pattern).
+  //    (offset, float bits); null otherwise (the FP32 solvers then read
+  //    Zr_valf on the rows of the FP64 form).
   int* Zr_row = nullptr; double* Zr_val = nullptr; int* Zr_col = nullptr;
+  float* Zr_valf = nullptr;   // Zr_val rounded to FP32: the FP32 solvers' Z where the rows form no classes
   int* Zr_pat = nullptr; int* Zr_off = nullptr;
   long long nnzZr = 0; int Zr_npat = 0; long long Zr_patlen = 0;
   int2* Zc_ent = nullptr;
@@ -388,6 +401,7 @@ int launch_step3(fsx_ctx* ctx, int iter);
 int launch_kacc(fsx_ctx* ctx);
 int settle_kacc(fsx_ctx* ctx);
 int refresh_matrix_copies(fsx_ctx* ctx);
+int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);
 int mass_update_step1(fsx_ctx* ctx);
 int mass_update_step3(fsx_ctx* ctx);
 int launch_maxacc(fsx_ctx* ctx);

@@ -16,13 +16,23 @@
 // BC rows of q are zero and the Jacobi inverse is 0 there, so r, z, p and x
 // stay zero at BC nodes and M p equals M_bc p on the free rows.
 //
-// One iteration is four kernels (SpMV, p.q per component, update, direction)
-// with the scalars on the device, like the pressure PCG; the host polls the
-// stop flag after every kMassChunk iterations.
+// M is applied element by element (FSX_MASS_CONSISTENT): per element the
+// products y_e = M_e p_e = sum_k Sv_k f_k (Sv_k^T p_e) with f_k = detJ_k w_k,
+// the quadrature k_assemble sums M with, then per node the sum of its
+// elements' y_e rows in a fixed element order (deterministic, no atomics).
+// That reads NE x (27 ids + NGP factors + 81 products) instead of M's
+// values and columns: on the 16x16x160 bent duct 0.1 GB against 0.26 GB per
+// application. FSX_MASS_CONSISTENT_CSR applies the assembled CSR values.
+//
+// One iteration is four kernels (the two of the element form with p.q
+// fused into the second, or the CSR SpMV and p.q; update; direction) with
+// the scalars on the device, like the pressure PCG; the host polls the stop
+// flag after every kMassChunk iterations.
 #include "fsx_device.h"
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace fsx {
 
@@ -44,41 +54,68 @@ __global__ void k_mass_dinv(int NN, const int* __restrict__ Mrow, const int* __r
   }
 }
 
-// x = 0, r = b, z = dinv r, p = z; partials of r.z (slots 0-2) and b.b (3-5)
-__global__ __launch_bounds__(kBlock) void k_mass_init(int NN, const double* __restrict__ b, const double* __restrict__ dinv,
-                                                      double* __restrict__ x, double* __restrict__ r,
-                                                      double* __restrict__ z, double* __restrict__ p,
-                                                      double* __restrict__ partials) {
+// Warm start, first pass: with q = M x0, the partials of |b_c|^2 (slots
+// 16-18) and |b_c - q_c|^2 (19-21)
+__global__ __launch_bounds__(kBlock) void k_mass_warm_a(int NN, const double* __restrict__ b, const double* __restrict__ q,
+                                                        double* __restrict__ partials) {
   double v[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 4   // independent elements: their loads in flight together
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t k = 3 * (size_t)i + c;
+      const double bi = b[k], ri = bi - q[k];
+      v[c] += bi * bi;
+      v[3 + c] += ri * ri;
+    }
+  write_partials<6>(v, partials + 16 * kRedGrid);
+}
+
+// Second pass: component c keeps x0 when |b_c - M x0_c| < |b_c|, else starts
+// from 0; r = b - M x, z = dinv r, p = z; partials of r.z (0-2), b.b (3-5)
+// and r.r (6-8)
+__global__ __launch_bounds__(kBlock) void k_mass_warm_b(int NN, const double* __restrict__ b, const double* __restrict__ q,
+                                                        const double* __restrict__ dinv, double* __restrict__ x,
+                                                        double* __restrict__ r, double* __restrict__ z,
+                                                        double* __restrict__ p, double* __restrict__ partials) {
+  double s[6];
+  read_partials<6>(partials + 16 * kRedGrid, s);
+  bool warm[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) warm[c] = s[3 + c] < s[c];
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 2   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock) {
     const double di = dinv[i];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t k = 3 * (size_t)i + c;
-      const double bi = b[k], zi = di * bi;
-      x[k] = 0.0; r[k] = bi; z[k] = zi; p[k] = zi;
-      v[c] += bi * zi;
+      const double bi = b[k];
+      const double ri = warm[c] ? bi - q[k] : bi, zi = di * ri;
+      if (!warm[c]) x[k] = 0.0;
+      r[k] = ri; z[k] = zi; p[k] = zi;
+      v[c] += ri * zi;
       v[3 + c] += bi * bi;
+      v[6 + c] += ri * ri;
     }
   }
-  write_partials<6>(v, partials);
+  write_partials<9>(v, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_mass_init_finish(const double* __restrict__ partials, MassScalars* sc,
+__global__ __launch_bounds__(kBlock) void k_mass_warm_finish(const double* __restrict__ partials, MassScalars* sc,
                                                              double rtol, int maxit) {
-  double s[6];
-  read_partials<6>(partials, s);
+  double s[9];
+  read_partials<9>(partials, s);
   if (threadIdx.x == 0) {
-    bool zero = true;
+    bool done = true;
     for (int c = 0; c < 3; c++) {
       sc->rz[0][c] = s[c]; sc->rz[1][c] = 0.0;
-      sc->bb[c] = s[3 + c]; sc->rr[c] = s[3 + c];
-      zero = zero && s[3 + c] == 0.0;
+      sc->bb[c] = s[3 + c]; sc->rr[c] = s[6 + c];
+      done = done && s[6 + c] <= rtol * rtol * s[3 + c];
     }
     sc->rtol2 = rtol * rtol;
     sc->iters = 0; sc->maxit = maxit;
-    sc->stop = zero ? 1 : 0;
+    sc->stop = done ? 1 : 0;
   }
 }
 
@@ -88,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_mass_spmv(int NN, const int* __restr
                                                       const double* __restrict__ M, const unsigned char* __restrict__ bc,
                                                       const double* __restrict__ p, double* __restrict__ q,
                                                       const MassScalars* __restrict__ sc) {
-  if (sc->stop) return;
+  if (sc && sc->stop) return;
   const int l = threadIdx.x % LPR;
   const long long r = (long long)blockIdx.x * (kBlock / LPR) + threadIdx.x / LPR;
   double s0 = 0, s1 = 0, s2 = 0;
@@ -115,6 +152,76 @@ __global__ __launch_bounds__(kBlock) void k_mass_spmv(int NN, const int* __restr
   if (r < NN && l < 3) q[3 * (size_t)r + l] = bc[r] ? 0.0 : (l == 0 ? s0 : (l == 1 ? s1 : s2));
 }
 
+// The element form, first pass: y_e = M_e p_e, one thread per element and
+// velocity component. The thread holds its element's 27 values of p in
+// registers; for each Gauss point k it takes row k of the shape table (the
+// same for every thread: scalar loads), u = sum_j Sv[k][j] p_j, then adds
+// Sv[k][i] f_k u to y_i for the 27 nodes i. The element arrays are
+// transposed so that consecutive threads read consecutive words: ids
+// L[j * NE + e], factors fac[k * NE + e], products ye[(i * NE + e) * 3 + c].
+__global__ __launch_bounds__(kBlock) void k_mass_elem(int NE, int NGP, const int* __restrict__ L,
+                                                      const double* __restrict__ fac, const double* __restrict__ Sv,
+                                                      const double* __restrict__ p, double* __restrict__ ye,
+                                                      const MassScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  const long long t = blockIdx.x * (long long)kBlock + threadIdx.x;
+  if (t >= 3LL * NE) return;
+  const int e = (int)(t / 3), c = (int)(t % 3);
+  double pe[27], y[27];
+#pragma unroll
+  for (int j = 0; j < 27; j++) {
+    pe[j] = p[3 * (size_t)L[(size_t)j * NE + e] + c];
+    y[j] = 0.0;
+  }
+  for (int k = 0; k < NGP; k++) {
+    const double* row = Sv + k * 27;
+    double u = 0.0;
+#pragma unroll
+    for (int j = 0; j < 27; j++) u += row[j] * pe[j];
+    const double v = fac[(size_t)k * NE + e] * u;
+#pragma unroll
+    for (int i = 0; i < 27; i++) y[i] += row[i] * v;
+  }
+#pragma unroll
+  for (int i = 0; i < 27; i++) ye[((size_t)i * NE + e) * 3 + c] = y[i];
+}
+
+// Second pass: q_r = sum of y_e rows of the elements holding node r (0 at
+// BC rows); with DOT the per-block p.q partials per component at
+// partials[kGatherPq + c kGatherGrid + block] (the grid is kGatherGrid blocks)
+constexpr int kGatherGrid = 2048;
+constexpr int kGatherPq = 32 * kRedGrid;
+
+template <bool DOT>
+__global__ __launch_bounds__(kBlock) void k_mass_gather(int NN, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+                                                        const double* __restrict__ ye, const unsigned char* __restrict__ bc,
+                                                        const double* __restrict__ p, double* __restrict__ q,
+                                                        double* __restrict__ partials, const MassScalars* __restrict__ sc) {
+  if (sc && sc->stop) return;
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll 2   // independent rows: their loads in flight together
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < NN; r += gridDim.x * kBlock) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (!bc[r]) {
+      const int t1 = inc_ptr[r + 1];
+      for (int t = inc_ptr[r]; t < t1; t++) {
+        const double* y = ye + 3 * (size_t)inc[t];
+        s0 += y[0]; s1 += y[1]; s2 += y[2];
+      }
+    }
+    const size_t k = 3 * (size_t)r;
+    q[k] = s0; q[k + 1] = s1; q[k + 2] = s2;
+    if (DOT) { v[0] += p[k] * s0; v[1] += p[k + 1] * s1; v[2] += p[k + 2] * s2; }
+  }
+  if (DOT) {
+    __shared__ double red[3 * (kBlock / kWave)];
+    block_sum_all<3>(v, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < 3; c++) partials[kGatherPq + c * kGatherGrid + blockIdx.x] = v[c];
+  }
+}
+
 // p.q per component (slots 6-8)
 __global__ __launch_bounds__(kBlock) void k_mass_dot(int NN, const double* __restrict__ p, const double* __restrict__ q,
                                                      double* __restrict__ partials, const MassScalars* __restrict__ sc) {
@@ -127,15 +234,18 @@ __global__ __launch_bounds__(kBlock) void k_mass_dot(int NN, const double* __res
   write_partials<3>(v, partials + 6 * kRedGrid);
 }
 
-// alpha_c = r.z_c / p.q_c (0 for a component with no work left);
+// alpha_c = r.z_c / p.q_c (0 for a component with no work left), p.q_c the
+// sum of npq per-block partials at pqp + c npq;
 // x += alpha p, r -= alpha q, z = dinv r; partials of r.r (slots 0-2) and r.z (3-5)
 __global__ __launch_bounds__(kBlock) void k_mass_update(int j, int NN, double* __restrict__ x, double* __restrict__ r,
                                                         double* __restrict__ z, const double* __restrict__ p,
                                                         const double* __restrict__ q, const double* __restrict__ dinv,
+                                                        const double* __restrict__ pqp, int npq,
                                                         double* __restrict__ partials, const MassScalars* __restrict__ sc) {
   if (sc->stop) return;
   double pq[3];
-  read_partials<3>(partials + 6 * kRedGrid, pq);
+#pragma unroll
+  for (int c = 0; c < 3; c++) pq[c] = sum_parts(pqp + c * npq, npq);
   double a[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) a[c] = pq[c] > 0.0 ? sc->rz[j & 1][c] / pq[c] : 0.0;
@@ -201,7 +311,82 @@ __global__ void k_mass_axpy(long long n, const double* __restrict__ u, const dou
     y[i] = u[i] + dt * x[i];
 }
 
-// Solves M_bc x = b (3NN, node-major) to the context's mass tolerance.
+// The element form's tables (host-built once): the elements sorted by
+// their centre node's internal id, their velocity ids in that order, detJ w
+// per element and Gauss point, and the node -> (sorted element, local node)
+// incidence, ascending per node.
+static int mass_elements_setup(fsx_ctx* ctx) {
+  const int NE = ctx->NE, NN = ctx->NN;
+  for (void* q : {(void*)ctx->mass_L27, (void*)ctx->mass_fac, (void*)ctx->mass_inc_ptr, (void*)ctx->mass_inc,
+                  (void*)ctx->mass_ye})
+    dev_free(ctx, q);   // what an earlier, failed set-up left
+  ctx->mass_L27 = nullptr; ctx->mass_fac = nullptr; ctx->mass_inc_ptr = nullptr; ctx->mass_inc = nullptr;
+  ctx->mass_ye = nullptr;
+  std::vector<int> L((size_t)NE * 27), P((size_t)NE * 8);
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_HIP(hipMemcpy(L.data(), ctx->L27, L.size() * sizeof(int), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(P.data(), ctx->P8, P.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> order(NE);
+  for (int e = 0; e < NE; e++) order[e] = e;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return L[(size_t)a * 27 + 26] < L[(size_t)b * 27 + 26]; });
+  std::vector<int> Ls((size_t)NE * 27), Ps((size_t)NE * 8);   // Ls[i * NE + k]: node i of sorted element k
+  for (int k = 0; k < NE; k++) {
+    for (int i = 0; i < 27; i++) Ls[(size_t)i * NE + k] = L[(size_t)order[k] * 27 + i];
+    std::copy_n(P.begin() + (size_t)order[k] * 8, 8, Ps.begin() + (size_t)k * 8);
+  }
+  std::vector<int> ptr((size_t)NN + 1, 0), inc((size_t)NE * 27);
+  for (int v : Ls) ptr[(size_t)v + 1]++;
+  for (int r = 0; r < NN; r++) ptr[r + 1] += ptr[r];
+  std::vector<int> next(ptr.begin(), ptr.end() - 1);
+  for (int k = 0; k < NE; k++)   // entries i * NE + k, ascending k per node
+    for (int i = 0; i < 27; i++) inc[next[Ls[(size_t)i * NE + k]]++] = i * NE + k;
+  int* P8s = nullptr;
+  FSX_TRY(alloc(ctx, &ctx->mass_L27, Ls.size()));
+  FSX_TRY(alloc(ctx, &P8s, Ps.size()));
+  FSX_TRY(alloc(ctx, &ctx->mass_fac, (size_t)NE * ctx->NGP));
+  FSX_TRY(alloc(ctx, &ctx->mass_inc_ptr, ptr.size()));
+  FSX_TRY(alloc(ctx, &ctx->mass_inc, inc.size()));
+  FSX_TRY(alloc(ctx, &ctx->mass_ye, (size_t)NE * 81));
+  FSX_HIP(hipMemcpy(ctx->mass_L27, Ls.data(), Ls.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(P8s, Ps.data(), Ps.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->mass_inc_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->mass_inc, inc.data(), inc.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_TRY(launch_mass_fac(ctx, P8s, ctx->mass_fac));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, P8s);
+  ctx->mass_elems_ready = true;
+  return FSX_OK;
+}
+
+// q = M_bc p; with dot the p.q partials (slots 6-8). sc null: always runs.
+static void mass_apply(fsx_ctx* ctx, bool elements, const double* p, double* q, bool dot, const MassScalars* sc) {
+  const int NN = ctx->NN;
+  if (elements) {
+    hipLaunchKernelGGL(k_mass_elem, dim3((unsigned)((3LL * ctx->NE + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, ctx->NE, ctx->NGP, ctx->mass_L27, ctx->mass_fac, ctx->tab.Sv,
+                       p, ctx->mass_ye, sc);
+    if (dot)
+      hipLaunchKernelGGL(k_mass_gather<true>, dim3(kGatherGrid), dim3(kBlock), 0, ctx->stream, NN, ctx->mass_inc_ptr,
+                         ctx->mass_inc, ctx->mass_ye, ctx->bcflag, p, q, ctx->partials, sc);
+    else
+      hipLaunchKernelGGL(k_mass_gather<false>, dim3((unsigned)std::min((NN + kBlock - 1) / kBlock, 4096)), dim3(kBlock),
+                         0, ctx->stream, NN, ctx->mass_inc_ptr, ctx->mass_inc, ctx->mass_ye, ctx->bcflag, p, q,
+                         ctx->partials, sc);
+    return;
+  }
+  constexpr int LPR = 16, U = 4;
+  const ColSrc mcs{ctx->Mcol, ctx->Mrowpat, ctx->Mpatoff};
+  const unsigned grid = (unsigned)std::max(1, (NN + kBlock / LPR - 1) / (kBlock / LPR));
+  if (mcs.row_pat)
+    hipLaunchKernelGGL((k_mass_spmv<LPR, U, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, NN, ctx->Mrow, mcs,
+                       ctx->M, ctx->bcflag, p, q, sc);
+  else
+    hipLaunchKernelGGL((k_mass_spmv<LPR, U, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, NN, ctx->Mrow, mcs,
+                       ctx->M, ctx->bcflag, p, q, sc);
+  if (dot) hipLaunchKernelGGL(k_mass_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, p, q, ctx->partials, sc);
+}
+
+// Solves M_bc x = b (3NN, node-major) to the context's mass tolerance; x
+// holds the previous solution of the same update (the warm start).
 int mass_solve(fsx_ctx* ctx, const double* b, double* x) {
   const int NN = ctx->NN;
   if (!ctx->mass_dinv_ready) {
@@ -210,29 +395,25 @@ int mass_solve(fsx_ctx* ctx, const double* b, double* x) {
     FSX_TRY(launch_check(ctx, "k_mass_dinv"));
     ctx->mass_dinv_ready = true;
   }
+  const bool elements = ctx->mass_mode == FSX_MASS_CONSISTENT && !ctx->mass_m_uploaded;
+  if (elements && !ctx->mass_elems_ready) FSX_TRY(mass_elements_setup(ctx));
   MassScalars* sc = reinterpret_cast<MassScalars*>(ctx->mass_sc);
   double *r = ctx->mass_w, *z = r + 3 * (size_t)NN, *p = z + 3 * (size_t)NN, *q = p + 3 * (size_t)NN;
-  hipLaunchKernelGGL(k_mass_init, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, b, ctx->mass_dinv, x, r, z, p,
+  mass_apply(ctx, elements, x, q, false, nullptr);
+  hipLaunchKernelGGL(k_mass_warm_a, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, b, q, ctx->partials);
+  hipLaunchKernelGGL(k_mass_warm_b, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, b, q, ctx->mass_dinv, x, r, z, p,
                      ctx->partials);
-  hipLaunchKernelGGL(k_mass_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, sc, ctx->mass_rtol,
+  hipLaunchKernelGGL(k_mass_warm_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, sc, ctx->mass_rtol,
                      ctx->mass_maxit);
-  FSX_TRY(launch_check(ctx, "k_mass_init"));
-  constexpr int LPR = 16, U = 4;
-  const ColSrc mcs{ctx->Mcol, ctx->Mrowpat, ctx->Mpatoff};
-  const unsigned grid = (unsigned)std::max(1, (NN + kBlock / LPR - 1) / (kBlock / LPR));
+  FSX_TRY(launch_check(ctx, "mass PCG start"));
   MassScalars h{};
   int j = 0;
   for (;;) {
     for (int t = 0; t < kMassChunk; t++, j++) {
-      if (mcs.row_pat)
-        hipLaunchKernelGGL((k_mass_spmv<LPR, U, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, NN, ctx->Mrow, mcs,
-                           ctx->M, ctx->bcflag, p, q, sc);
-      else
-        hipLaunchKernelGGL((k_mass_spmv<LPR, U, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, NN, ctx->Mrow, mcs,
-                           ctx->M, ctx->bcflag, p, q, sc);
-      hipLaunchKernelGGL(k_mass_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, p, q, ctx->partials, sc);
+      mass_apply(ctx, elements, p, q, true, sc);
       hipLaunchKernelGGL(k_mass_update, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, x, r, z, p, q,
-                         ctx->mass_dinv, ctx->partials, sc);
+                         ctx->mass_dinv, ctx->partials + (elements ? kGatherPq : 6 * kRedGrid),
+                         elements ? kGatherGrid : kRedGrid, ctx->partials, sc);
       hipLaunchKernelGGL(k_mass_direction, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, z, p, ctx->partials, sc);
     }
     FSX_TRY(launch_check(ctx, "mass PCG"));
@@ -280,9 +461,10 @@ size_t mass_scalars_bytes() { return sizeof(MassScalars); }
 extern "C" int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit) {
   using namespace fsx;
   if (!ctx) return FSX_EINVAL;
-  if (mode != FSX_MASS_LUMPED && mode != FSX_MASS_CONSISTENT) return fail(ctx, FSX_EINVAL, "fsx_set_mass: unknown mode");
+  if (mode != FSX_MASS_LUMPED && mode != FSX_MASS_CONSISTENT && mode != FSX_MASS_CONSISTENT_CSR)
+    return fail(ctx, FSX_EINVAL, "fsx_set_mass: unknown mode");
   if (!(rtol >= 0) || maxit < 1) return fail(ctx, FSX_EINVAL, "fsx_set_mass: rtol >= 0 and maxit >= 1 required");
-  if (mode == FSX_MASS_CONSISTENT) {
+  if (mode != FSX_MASS_LUMPED) {
     if (!ctx->have_mesh) return fail(ctx, FSX_ESTATE, "fsx_set_mass: upload the mesh first");
     const size_t n3 = 3 * (size_t)ctx->NN;
     if (!ctx->mass_w) {
@@ -291,6 +473,7 @@ extern "C" int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit) {
       FSX_TRY(alloc(ctx, &ctx->mass_b, n3));
       FSX_TRY(alloc(ctx, &ctx->mass_dinv, (size_t)ctx->NN));
       FSX_TRY(alloc(ctx, &ctx->mass_sc, (mass_scalars_bytes() + 7) / 8));
+      FSX_TRY(launch_fill(ctx, ctx->mass_x, 0.0, (long long)n3));   // the first warm start is 0
     }
   }
   ctx->mass_mode = mode;

@@ -186,7 +186,7 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   const bool f32 = ctx->Kf != nullptr;   // FP32 matrix values (fsx_set_matrix_precision)
   // FP32 operand shadows: written with lumped mass (the consistent update
   // rewrites UnpHalf afterwards), read when they hold the prev vectors
-  const bool wrec = f32 && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT;
+  const bool wrec = f32 && ctx->rec_mem && ctx->mass_mode == FSX_MASS_LUMPED;
   const bool rrec = wrec && ctx->urec_prev_ok && (!kacc || ctx->arec_prev_ok);
   // the first inner iteration (no K Acc) reads the compact shadow of Un:
   // the interleaved records would bring Acc's unused half along
@@ -221,7 +221,7 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   ctx->urec_ok = wrec;
   ctx->drec_ok = wrec;
   ctx->drec_kacc = kacc;
-  if (ctx->mass_mode == FSX_MASS_CONSISTENT) return mass_update_step1(ctx);
+  if (ctx->mass_mode != FSX_MASS_LUMPED) return mass_update_step1(ctx);
   return FSX_OK;
 }
 
@@ -273,7 +273,7 @@ __global__ void k_rec(long long nn, const double* __restrict__ v, float4* __rest
 // (only where step1 and step3 keep the shadows up to date)
 int records_begin_step(fsx_ctx* ctx) {
   drop_records(ctx);
-  if (!ctx->Kf || !ctx->rec_mem || ctx->mass_mode == FSX_MASS_CONSISTENT) return FSX_OK;
+  if (!ctx->Kf || !ctx->rec_mem || ctx->mass_mode != FSX_MASS_LUMPED) return FSX_OK;
   hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
                      reinterpret_cast<float4*>(ctx->Urec_prev), 2);
   hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, nullptr,
@@ -536,7 +536,7 @@ static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
 }
 
 int launch_step3(fsx_ctx* ctx, int iter) {
-  if (ctx->mass_mode == FSX_MASS_CONSISTENT) {
+  if (ctx->mass_mode != FSX_MASS_LUMPED) {
     launch_step3_rows(ctx, iter, ctx->mass_b);
     FSX_TRY(launch_check(ctx, "k_step3"));
     const int rc = mass_update_step3(ctx);
@@ -728,7 +728,7 @@ static unsigned zcls_blocks(const fsx_ctx* ctx) {
 }
 
 // q = Z p: FP64 on the solvers' rows; FP32 on the row classes, or on the
-// FP32 copy of the full pattern when the rows do not compress
+// FP32 rounding of the same rows when they form no classes
 template <class T>
 static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc) {
   const int n = ctx->NNp;
@@ -741,10 +741,11 @@ static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc)
     }
   }
   constexpr int LPR = PcgCfg<T>::LPR, U = PcgCfg<T>::U;
+  // both precisions on the rows of the FP64 form (structural zeros dropped)
   const bool fp64 = sizeof(T) == 8;
-  const ColSrc cs = fp64 ? zr_cols(ctx) : ColSrc{ctx->Zcol, ctx->Zrowpat, ctx->Zpatoff};
-  const int* rp = fp64 ? ctx->Zr_row : ctx->Zrow;
-  const T* val = fp64 ? (const T*)(const void*)ctx->Zr_val : (const T*)(const void*)ctx->Zf;
+  const ColSrc cs = zr_cols(ctx);
+  const int* rp = ctx->Zr_row;
+  const T* val = fp64 ? (const T*)(const void*)ctx->Zr_val : (const T*)(const void*)ctx->Zr_valf;
   // the dictionary form uses 8 lanes x 16 loads per row in both precisions
   // (a sweep of 8x16, 16x8, 32x4, 64x2 on the 64^3 Z: 36.7, 56.2, 64.3,
   // 82.0 us in FP32; 44.5, 45.5, 53.4, 80.9 us in FP64): more independent
@@ -1405,7 +1406,7 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
   //  dictionary) + row pointers.
   //  FP32 row classes: the launch plan (16 bytes per chunk of rows, 4 per
   //  row) + the class table; otherwise
-  //  the FP32 copy on the full pattern like the FP64 form.
+  //  the FP32 copy on the rows of the FP64 form.
   if (precision == 0) {
     FSX_TRY(bench_spmv_t<double>(ctx, ctx->wp, ctx->wq, reps, &ms));
     const double nnz = (double)ctx->nnzZr;
@@ -1416,8 +1417,8 @@ extern "C" int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_
     if (ctx->Zc_chunk) {
       bytes = 16.0 * ctx->Zc_nchunk + 4.0 * n + 8.0 * ctx->Zc_len + 8.0 * n;
     } else {
-      const double nnz = (double)ctx->nnzZ;
-      const double cols = ctx->Zrowpat ? 4.0 * n + 4.0 * (ctx->Zpatlen + ctx->Znpat + 1) : 4.0 * nnz;
+      const double nnz = (double)ctx->nnzZr;
+      const double cols = ctx->Zr_pat ? 4.0 * n + 4.0 * (ctx->Zr_patlen + ctx->Zr_npat + 1) : 4.0 * nnz;
       bytes = 4.0 * nnz + cols + 4.0 * (n + 1) + 8.0 * n;
     }
   }
@@ -1437,7 +1438,7 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP3) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
   const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG;
   const double vb = ctx->Kf ? 4.0 : 8.0;   // bytes per stored matrix value
-  if (kernel == FSX_KERNEL_STEP1 && ctx->Kf && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT) {
+  if (kernel == FSX_KERNEL_STEP1 && ctx->Kf && ctx->rec_mem && ctx->mass_mode == FSX_MASS_LUMPED) {
     // the FP32 shadows of UnpHalf_prev and Acc_prev, as the inner iterations have them
     hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->UnpHalf_prev,
                        reinterpret_cast<float4*>(ctx->Urec_prev), 2);
@@ -1463,7 +1464,7 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
                               // with the FP32 shadows Uprev and Acc_prev are 16 bytes per node and
                               // the shadows of UnpHalf and of step2's operand are written (12 + 12
                               // bytes; the latter reads MdOrigInv, 8)
-      const bool wrec = ctx->Kf && ctx->rec_mem && ctx->mass_mode != FSX_MASS_CONSISTENT;
+      const bool wrec = ctx->Kf && ctx->rec_mem && ctx->mass_mode == FSX_MASS_LUMPED;
       const bool rrec = wrec && ctx->urec_prev_ok && ctx->arec_prev_ok;
       const double vin = rrec ? 16.0 : 24.0;
       bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + vin * NN + 48.0 * NN + 24.0 * NN + 8.0 * NN + 1.0 * NN +
@@ -1509,8 +1510,8 @@ extern "C" int fsx_z_forms(fsx_ctx* ctx, long long* nnz_solver, int* nclasses, l
 
 extern "C" int fsx_z_patterns(fsx_ctx* ctx, int* npatterns, long long* offsets) {
   if (!ctx) return FSX_EINVAL;
-  if (npatterns) *npatterns = ctx->Zrowpat ? ctx->Znpat : 0;
-  if (offsets) *offsets = ctx->Zrowpat ? ctx->Zpatlen : 0;
+  if (npatterns) *npatterns = ctx->Zr_pat ? ctx->Zr_npat : 0;
+  if (offsets) *offsets = ctx->Zr_pat ? ctx->Zr_patlen : 0;
   return FSX_OK;
 }
 

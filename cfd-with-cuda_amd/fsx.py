@@ -23,7 +23,7 @@ FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTATE, FSX_ENOCONV, FSX_ENOMEM, FSX_ENUMERIC 
 CSR_M, CSR_G, CSR_Z = 0, 1, 2
 SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_DIRECT = 0, 1, 2, 3
 KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3 = 0, 1, 2, 3
-MASS_LUMPED, MASS_CONSISTENT = 0, 1
+MASS_LUMPED, MASS_CONSISTENT, MASS_CONSISTENT_CSR = 0, 1, 2
 X0_ZERO, X0_PREVIOUS = 0, 1
 
 VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, R1=7, Pn=8, Pnp1=9,

@@ -92,7 +92,9 @@ enum fsx_x0_policy {
 /* Velocity mass for fsx_set_mass */
 enum fsx_mass_mode {
   FSX_MASS_LUMPED = 0,      /* row-sum lumped M, the reference's update (BCH:3262-3295, 3781-3783, 3960-3962) */
-  FSX_MASS_CONSISTENT = 1   /* consistent M, one 3-RHS Jacobi-PCG per update (no reference counterpart) */
+  FSX_MASS_CONSISTENT = 1,  /* consistent M, one 3-RHS Jacobi-PCG per update (no reference counterpart),
+                               M applied element by element */
+  FSX_MASS_CONSISTENT_CSR = 2   /* the same solves with M applied from its assembled CSR values */
 };
 
 /* ---------------------------------------------------------------- lifecycle */
@@ -129,10 +131,16 @@ int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit);
  * instead, M_bc being the assembled consistent mass with the rows and
  * columns of BC nodes replaced by the identity; the three velocity
  * components are solved together as one 3-right-hand-side Jacobi-PCG (one
- * pass over M per iteration, per-component scalars and stop test
- * ||r_c|| <= rtol ||b_c||, x0 = 0). The pressure operator Z keeps the lumped
- * MdOrigInv. No reference counterpart (the reference assembles M, then lumps
- * it, BCH:3195-3295). Needs the mesh uploaded. */
+ * application of M per iteration, per-component scalars and stop test
+ * ||r_c|| <= rtol ||b_c||). Each component starts from the update's previous
+ * solution (the last a, or Acc) when that leaves a smaller residual than 0,
+ * else from 0 (0 after fsx_init_state). M is applied element by element,
+ * y = sum_e M_e p_e from detJ w at the Gauss points, summed per node in a
+ * fixed element order; FSX_MASS_CONSISTENT_CSR applies the assembled CSR
+ * values instead (also what FSX_MASS_CONSISTENT does after M values are
+ * uploaded, until the next fsx_assemble_static). The pressure operator Z
+ * keeps the lumped MdOrigInv. No reference counterpart (the reference
+ * assembles M, then lumps it, BCH:3195-3295). Needs the mesh uploaded. */
 int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit);
 /* iterations (largest over the components is the count of the block
  * solve) and largest relative residual of the last mass solve, and the
@@ -270,9 +278,10 @@ int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, 
  * write of q. No reference counterpart (the reference prints wall times only,
  * BCH:4489-4518). */
 int fsx_bench_spmv(fsx_ctx* ctx, int precision, int reps, double* ms_per_launch, double* bytes_per_launch);
-/* Size of the dictionary of Z row patterns (distinct column-offset lists)
- * that the Z SpMVs read instead of a column index per entry; 0 patterns when
- * Z did not compress and the SpMVs read the CSR columns. No reference
+/* Size of the dictionary of row patterns (distinct column-offset lists) of
+ * the solvers' Z (structural zeros dropped, fsx_z_forms) that the Z SpMVs
+ * read instead of a column index per entry; 0 patterns when the rows did not
+ * compress and the SpMVs read CSR columns. No reference
  * counterpart (the reference's Z is plain CSR, CU:281-411). */
 int fsx_z_patterns(fsx_ctx* ctx, int* npatterns, long long* offsets);
 /* The solvers' forms of the built Z. Entries whose magnitude is below 2^-46
@@ -282,7 +291,7 @@ int fsx_z_patterns(fsx_ctx* ctx, int* npatterns, long long* offsets);
  * as row classes when its rows take at most 65535 distinct lists of (column
  * offset, FP32 value) with at most 2^20 entries in total: nclasses and
  * class_entries return their count and total length (0 when the FP32 solves
- * read the full FP32 copy). No reference counterpart (the reference solves
+ * read the FP32 rounding of the kept entries on their CSR rows). No reference counterpart (the reference solves
  * with the assembled CSR Z, CU:418-451, BCH:3981-4055). */
 int fsx_z_forms(fsx_ctx* ctx, long long* nnz_solver, int* nclasses, long long* class_entries);
 /* Times `reps` launches of a plain device copy of `bytes` (read once, written

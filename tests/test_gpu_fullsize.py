@@ -15,8 +15,9 @@ at this size:
   * after 3 time steps the mixed and learned+mixed runs agree with the FP64
     run to 1e-6 (velocity) and 1e-5 (pressure), the stated accuracy of the
     mixed forms.
-The oracle pins the same kernels at the sizes it finishes in seconds (the
-other GPU tests); this file shows they hold at the full workload's shape.
+The bent duct at its benchmark size is checked the same way (its own test
+below). The oracle pins the same kernels at the sizes it finishes in seconds
+(the other GPU tests); this file shows they hold at the full workloads' shape.
 """
 import numpy as np
 import pytest
@@ -111,3 +112,43 @@ def test_cavity64_fields_and_solves(fsx_lib, cavity):
     check_solve(fsx_lib.SOLVER_LEARNED, "learned")
     say(f"  learned vs FP64: velocity {rel(ul, u64):.2e}, pressure {rel(pl, p64):.2e}")
     assert rel(ul, u64) < 1e-6 and rel(pl, p64) < 1e-5
+
+
+def test_duct_full_size_fields_and_solves(fsx_lib):
+    """The bent duct at its benchmark size (16 x 16 cross-section elements,
+    40 + 40 + 80 along the axis, numbering permuted from a seed: 349,569
+    velocity and 46,529 pressure nodes), Re 200, dt 0.005, consistent mass:
+      * every pressure solve stops at 1e-8 and the last mass solve of each
+        run at its 1e-10, in the FP64 and the mixed modes;
+      * the element-by-element and the CSR application of M give the same
+        FP64 fields after 3 steps to 1e-9;
+      * the mixed run agrees with the FP64 run to 1e-6 (velocity) and 1e-5
+        (pressure) after 3 steps;
+      * the velocity stays zero on the walls and equals the parabolic inlet
+        profile at the inlet nodes (the BC rows of the consistent update)."""
+    mesh = fsx_lib.Mesh.duct(n=16, n_in=40, n_arc=40, n_out=80, seed=7)
+    mesh.set_params(NGP=27, dt=0.005, maxIter=3, tolerance=1e-3, converge=1e-6, viscosity=1.0 / 200)
+    say("full-size bent duct 16x16x160: setup")
+    s = fsx_lib.Solver(mesh, device=0, NGP=27, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=RTOL,
+                       pcg_maxit=20000, solver_mode=0)
+    assert s.sizes["NN"] == 349569 and s.sizes["NNp"] == 46529
+    nodes = s.topo.get("BCnodes")
+    vals = s.topo.get("BCvalues").reshape(-1, 3)
+    res = {}
+    for name, mass, mode, bits in (("fp64", fsx_lib.MASS_CONSISTENT, fsx_lib.SOLVER_FP64, 64),
+                                   ("fp64 csr", fsx_lib.MASS_CONSISTENT_CSR, fsx_lib.SOLVER_FP64, 64),
+                                   ("mixed", fsx_lib.MASS_CONSISTENT, fsx_lib.SOLVER_MIXED, 32)):
+        say(f"  {name}: 3 steps")
+        s.set_mass(mass, rtol=1e-10, maxit=2000)
+        res[name] = run_steps(fsx_lib, s, mode, bits)
+        it, rr, _ = s.mass_stats()
+        assert 0 < it < 2000 and rr <= 1e-10, (it, rr)
+        un = res[name][0].reshape(3, -1).T   # component-major, as the reference stores Un
+        assert np.allclose(un[nodes], vals, rtol=0, atol=1e-14)
+    a, b, m = res["fp64"], res["fp64 csr"], res["mixed"]
+    say(f"  element vs CSR mass: velocity {rel(a[0], b[0]):.2e}, pressure {rel(a[1], b[1]):.2e}")
+    assert rel(a[0], b[0]) < 1e-9 and rel(a[1], b[1]) < 1e-9
+    say(f"  mixed vs FP64: velocity {rel(m[0], a[0]):.2e}, pressure {rel(m[1], a[1]):.2e}")
+    assert rel(m[0], a[0]) < 1e-6 and rel(m[1], a[1]) < 1e-5
+    s.close()
+    mesh.close()
