@@ -220,6 +220,8 @@ def duct_workload(fsx, args, steps, warmup, cpu_steps):
     s.set_mass(fsx.MASS_CONSISTENT, rtol=1e-10, maxit=2000)
     res = {}
     for name, mode, bits in (("fp64", fsx.SOLVER_FP64, 64), ("mixed", fsx.SOLVER_MIXED, 32)):
+        if name not in args.duct_modes.split(","):
+            continue
         r = run_mode(s, mode, steps, warmup, args.rtol, args.maxit, matrix_bits=bits, x0=args.x0,
                      total=warmup + steps)
         m_it, m_rel, m_tot = s.mass_stats()
@@ -227,9 +229,10 @@ def duct_workload(fsx, args, steps, warmup, cpu_steps):
                      "inner_iters": r["inner_iters"], "solve_ms": round(r["solve_ms"], 2),
                      "max_relres": r["max_relres"], "mass_iters_last_solve": m_it, "mass_relres_last_solve": m_rel}
         log(f"  duct {name}: {res[name]}")
+    head = "mixed" if "mixed" in res else "fp64"
     out = {"workload": "90-degree bent square duct 16x16x160 hex27/hex8, permuted numbering, Re 200, dt 0.005, "
                        f"consistent mass, pressure rtol {args.rtol:g}",
-           "steps": steps, "warmup": warmup, "value": res["mixed"]["steps_per_s"], "mode": "mixed",
+           "steps": steps, "warmup": warmup, "value": res[head]["steps_per_s"], "mode": head,
            "NN": s.sizes["NN"], "NNp": s.sizes["NNp"], "nnz_M": s.sizes["nnzM"], "modes": res}
     s.close()
     if cpu_steps > 0 and not args.no_cpu:
@@ -314,9 +317,10 @@ def headline(args):
             solver.unet_set_weights(w_trained)
         elif name == "learned_untrained":
             solver.unet_set_weights(w_init)
-        log(f"mode {name}: {args.warmup} warm-up + {args.steps} timed steps, trajectory of {TRAJECTORY_STEPS}")
+        log(f"mode {name}: {args.warmup} warm-up + {args.steps} timed steps, trajectory of {args.trajectory}")
         res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit,
-                             matrix_bits=64 if name in ("fp64", "direct") else 32, x0=args.x0)
+                             matrix_bits=64 if name in ("fp64", "direct") else 32, x0=args.x0,
+                             total=max(args.trajectory, args.warmup + args.steps))
         r = res[name]
         if name == "direct":   # the factor is formed in the first warm-up step
             w, fms, fbytes = solver.direct_stats()
@@ -368,7 +372,7 @@ def headline(args):
                                (f" (sinh clustering {args.cluster:g})" if args.cluster else "") +
                                f", NGP {args.ngp}, Re {args.re:g}, dt {args.dt:g}, 3 inner iterations, "
                                f"pressure rtol {args.rtol:g}, pressure x0 {args.x0}, timed steps "
-                               f"{args.warmup + 1}-{args.warmup + args.steps} of {TRAJECTORY_STEPS}",
+                               f"{args.warmup + 1}-{args.warmup + args.steps} of {args.trajectory}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname,
@@ -459,6 +463,9 @@ def main():
     ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
                     help="enqueue the FP64 and mixed solver iterations as replayed HIP graphs (1) or as individual "
                          "launches (0)")
+    ap.add_argument("--duct-modes", default="fp64,mixed", help="pressure solver modes of the bent-duct workload")
+    ap.add_argument("--trajectory", type=int, default=TRAJECTORY_STEPS,
+                    help="steps of each mode's trajectory (the 64^3 configuration's 100; fewer for profiled runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3, help="whole 64^3 time steps the CPU baseline times")
     args = ap.parse_args()
@@ -481,8 +488,8 @@ def main():
             work[wname] = cavity_workload(fsx, 8, 100.0, 0.01, 50, 50, args,
                                           "lid-driven cavity 8^3 hex27/hex8, Re 100, dt 0.01, 50 steps")
         elif wname == "cavity32":
-            log("workload: 32^3 cavity, Re 400, dt 0.005, 200 steps (CPU: 20)")
-            work[wname] = cavity_workload(fsx, 32, 400.0, 0.005, 200, 20, args,
+            log("workload: 32^3 cavity, Re 400, dt 0.005, 200 steps (GPU and CPU)")
+            work[wname] = cavity_workload(fsx, 32, 400.0, 0.005, 200, 200, args,
                                           "lid-driven cavity 32^3 hex27/hex8, Re 400, dt 0.005, 200 steps")
         elif wname == "duct":
             work[wname] = duct_workload(fsx, args, steps=min(args.steps, 20), warmup=min(args.warmup, 5),
