@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libfsx.so")
+# FSX_LIB names another build of the library (the sanitizer build of `make sanitize`)
+LIB_PATH = os.environ.get("FSX_LIB") or os.path.join(HERE, "_build", "libfsx.so")
 
 _D = ctypes.POINTER(ctypes.c_double)
 _F = ctypes.POINTER(ctypes.c_float)

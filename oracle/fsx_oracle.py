@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libfsx_oracle.so")
+# FSX_ORACLE_LIB names another build (the sanitizer build of `make -C oracle sanitize`)
+_LIB_PATH = os.environ.get("FSX_ORACLE_LIB") or os.path.join(_HERE, "_build", "libfsx_oracle.so")
 _lib = None
 
 _D = ctypes.POINTER(ctypes.c_double)
