@@ -271,6 +271,12 @@ int fsx_set_convection_precision(fsx_ctx* ctx, int bits) {
   return FSX_OK;
 }
 
+int fsx_set_step1_window(fsx_ctx* ctx, int enable) {
+  if (!ctx) return FSX_EINVAL;
+  ctx->use_step1_window = enable != 0;
+  return FSX_OK;
+}
+
 int fsx_set_graphs(fsx_ctx* ctx, int enable) {
   if (!ctx) return FSX_EINVAL;
   ctx->use_graphs = enable != 0;
@@ -654,6 +660,28 @@ int fsx_upload_csr(fsx_ctx* ctx, int kind, int nrows, const int* rowptr, const i
       if (!po.empty()) FSX_HIP(hipMemcpy(ctx->Mpatoff, po.data(), po.size() * sizeof(int), hipMemcpyHostToDevice));
       ctx->Mnpat = (int)pp.size() - 1;
       ctx->Mpatlen = (long long)po.size();
+      // the LDS window's slots, when every offset is a (dx, dy, dz) step of
+      // at most 2 on the velocity grid of a structured mesh
+      if (ctx->pg_nx > 0 && !po.empty()) {
+        const long long vnx = 2LL * ctx->pg_nx - 1, vnxy = vnx * (2LL * ctx->pg_ny - 1);
+        std::vector<int> slot(po.size());
+        bool ok = vnx > 4;
+        for (size_t k = 0; ok && k < po.size(); k++) {
+          const long long o = po[k];
+          const long long dz = (o + 2 * vnxy * 3 + vnxy / 2) / vnxy - 6;   // nearest multiple of vnxy
+          const long long rz = o - dz * vnxy;
+          const long long dy = (rz + 2 * vnx * 3 + vnx / 2) / vnx - 6;
+          const long long dx = rz - dy * vnx;
+          ok = dz >= -2 && dz <= 2 && dy >= -2 && dy <= 2 && dx >= -2 && dx <= 2;
+          slot[k] = (int)(2 + dx + 36 * ((dz + 2) * 5 + (dy + 2)));
+        }
+        if (ok) {
+          FSX_TRY(alloc(ctx, &ctx->Mpatslot, slot.size()));
+          FSX_HIP(hipMemcpy(ctx->Mpatslot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice));
+          ctx->vnx = (int)vnx;
+          ctx->vnxy = vnxy;
+        }
+      }
     }
     ctx->have_M = true;
   } else if (kind == FSX_CSR_G) {

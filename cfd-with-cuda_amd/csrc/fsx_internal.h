@@ -246,6 +246,16 @@ struct fsx_ctx {
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
   int Mnpat = 0; long long Mpatlen = 0;
+  // step1's LDS window of operand records: when every K pattern offset is
+  // dz*vnxy + dy*vnx + dx with |dx|, |dy|, |dz| <= 2 for the velocity grid of
+  // a structured mesh (vnx = 2 pg_nx - 1 nodes a line), a block of rows r0..
+  // r0 + 31 reads its columns from 25 runs of 36 records staged in LDS, run
+  // (dy, dz) holding ids r0 - 2 + dy*vnx + dz*vnxy + [0, 36); Mpatslot holds
+  // each pattern entry's slot 2 + dx + 36 ((dz + 2) 5 + dy + 2), so that the
+  // entry of row r sits at slot (r - r0) + Mpatslot[..]. Null otherwise.
+  int* Mpatslot = nullptr;
+  int vnx = 0; long long vnxy = 0;
+  bool use_step1_window = false;   // fsx_set_step1_window (measured slower at 64^3)
   int* Grow = nullptr; int* Gcol = nullptr;
   double* G = nullptr;           // nnzG*3, (G1, G2, G3) interleaved per nonzero
   int* GTrow = nullptr; int* GTcol = nullptr; int* GTpos = nullptr;

@@ -76,7 +76,10 @@ __device__ __forceinline__ long long my_row(int* lane) {
 // receives UnpHalf rounded to FP32, the shadow the next inner iteration reads,
 // and a non-null Dout step2's operand UnpHalf / dt^2 (- MdOrigInv KAcc with
 // KACC) rounded to FP32.
-template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE, class VT, bool REC>
+// With WIN (REC and KPAT only) the block first stages the operand records of
+// its column window in LDS (25 runs of 36 nodes, see Mpatslot) and the rows
+// read them there; the products and their order are the same.
+template <int LPR, int UK, int UG, bool KACC, bool KPAT, int GMODE, class VT, bool REC, bool WIN = false>
 __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict__ Mrow, ColSrc mcs,
                                                   const VT* __restrict__ K, const int* __restrict__ Grow,
                                                   const int* __restrict__ Gcol, const VT* __restrict__ G,
@@ -88,9 +91,25 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
                                                   double* __restrict__ GPn, const float4* __restrict__ Urp,
                                                   const float4* __restrict__ Arp, float* __restrict__ Uout,
                                                   const double* __restrict__ MdOrigInv, double o2,
-                                                  float* __restrict__ Dout, int us) {
+                                                  float* __restrict__ Dout, int us, const int* __restrict__ pslot,
+                                                  int vnx, long long vnxy) {
+  static_assert(!WIN || (REC && KPAT), "the window reads FP32 records through the K patterns");
+  constexpr int kWinRows = kBlock / LPR, kWinLen = kWinRows + 4, kWinSlots = 25 * kWinLen;
+  __shared__ float4 win[WIN ? 2 * kWinSlots : 1];
   int l;
   const long long r = my_row<LPR>(&l);
+  const long long r0 = (long long)blockIdx.x * kWinRows;
+  if constexpr (WIN) {
+    for (int t = threadIdx.x; t < kWinSlots; t += kBlock) {
+      const int run = t / kWinLen, pos = t - run * kWinLen;
+      const long long id = r0 - 2 + pos + (long long)(run % 5 - 2) * vnx + (long long)(run / 5 - 2) * vnxy;
+      if (id >= 0 && id < NN) {
+        win[(size_t)us * t] = Urp[(size_t)us * id];
+        if (us == 2) win[2 * (size_t)t + 1] = Urp[2 * (size_t)id + 1];
+      }
+    }
+    __syncthreads();
+  }
   double k0 = 0, k1 = 0, k2 = 0, g0 = 0, g1 = 0, g2 = 0, a0 = 0, a1 = 0, a2 = 0;
   if (r < NN) {
     const int rs = Mrow[r], re = Mrow[r + 1];
@@ -101,12 +120,20 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
 #pragma unroll
       for (int j = 0; j < UK; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
-        c[j] = col_at<KPAT>(mcs, r, cb, kk);
+        if constexpr (WIN) c[j] = (int)(r - r0) + pslot[cb + kk];   // the entry's window slot
+        else c[j] = col_at<KPAT>(mcs, r, cb, kk);
         a[j] = k < re ? (double)K[kk] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < UK; j++) {
-        if constexpr (REC) {
+        if constexpr (WIN) {
+          const float4 u = win[(size_t)us * c[j]];
+          k0 += a[j] * (double)u.x; k1 += a[j] * (double)u.y; k2 += a[j] * (double)u.z;
+          if constexpr (KACC) {
+            const float4 v = win[2 * (size_t)c[j] + 1];
+            a0 += a[j] * (double)v.x; a1 += a[j] * (double)v.y; a2 += a[j] * (double)v.z;
+          }
+        } else if constexpr (REC) {
           const float4 u = Urp[(size_t)us * c[j]];
           k0 += a[j] * (double)u.x; k1 += a[j] * (double)u.y; k2 += a[j] * (double)u.z;
           if constexpr (KACC) {
@@ -197,15 +224,21 @@ int launch_step1(fsx_ctx* ctx, int iter) {
   float* uout = wrec ? ctx->Urec : nullptr;
   float* dout = wrec ? ctx->Drec : nullptr;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
-#define STEP1_LAUNCH_T(KA, KP, GM, VT, RC, KV, GV)                                                                    \
-  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT, RC>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN, ctx->Mrow,  \
-                     mcs, KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1, ctx->MdInv,      \
-                     ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr, KA ? ctx->KAcc : nullptr,       \
-                     ctx->GPn, urp, arp, uout, ctx->MdOrigInv, o2, dout, us)
+  // the LDS window: the K patterns decompose on the velocity grid, and the
+  // Acc records sit beside the UnpHalf ones (interleaved shadows)
+  const bool win = rrec && mcs.row_pat && ctx->Mpatslot && ctx->use_step1_window &&
+                   (!kacc || (us == 2 && arp == urp + 1));
+#define STEP1_LAUNCH_T(KA, KP, GM, VT, RC, KV, GV, WN)                                                                \
+  hipLaunchKernelGGL((k_step1<LPR, UK, 2, KA, KP, GM, VT, RC, WN>), grid, dim3(kBlock), 0, ctx->stream, ctx->NN,       \
+                     ctx->Mrow, mcs, KV, ctx->Grow, ctx->Gcol, GV, ctx->UnpHalf_prev, ctx->Pn, ctx->Un, ctx->R1,        \
+                     ctx->MdInv, ctx->bcflag, ctx->UnpHalf, ctx->dt, KA ? ctx->Acc_prev : nullptr,                     \
+                     KA ? ctx->KAcc : nullptr, ctx->GPn, urp, arp, uout, ctx->MdOrigInv, o2, dout, us, ctx->Mpatslot,  \
+                     ctx->vnx, ctx->vnxy)
 #define STEP1_LAUNCH(KA, KP, GM) \
-  do { if (rrec) STEP1_LAUNCH_T(KA, KP, GM, float, true, ctx->Kf, ctx->Gf); \
-       else if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, false, ctx->Kf, ctx->Gf); \
-       else STEP1_LAUNCH_T(KA, KP, GM, double, false, ctx->K, ctx->G); } while (0)
+  do { if (rrec && KP && win) STEP1_LAUNCH_T(KA, KP, GM, float, true, ctx->Kf, ctx->Gf, KP); \
+       else if (rrec) STEP1_LAUNCH_T(KA, KP, GM, float, true, ctx->Kf, ctx->Gf, false); \
+       else if (f32) STEP1_LAUNCH_T(KA, KP, GM, float, false, ctx->Kf, ctx->Gf, false); \
+       else STEP1_LAUNCH_T(KA, KP, GM, double, false, ctx->K, ctx->G, false); } while (0)
 #define STEP1_G(KA, KP) \
   do { if (gload) STEP1_LAUNCH(KA, KP, 2); else STEP1_LAUNCH(KA, KP, 1); } while (0)
   if (mcs.row_pat) {

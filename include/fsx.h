@@ -154,6 +154,13 @@ int fsx_mass_stats(fsx_ctx* ctx, int* last_iters, double* last_relres, long long
  * entry and 24 per G entry of device memory. No reference counterpart (the
  * reference is FP64 throughout; its `real` type is unused, BCH:130-134). */
 int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
+/* step1 with FP32 operand shadows on a structured mesh whose K patterns are
+ * steps of at most 2 on the velocity grid can stage each block's column
+ * window of shadow records in LDS (enable != 0) instead of gathering them
+ * from memory (0, the default: the faster form at 64^3); the products,
+ * their order and so the results are the same either way. No reference
+ * counterpart (a kernel-form switch for measurement and checks). */
+int fsx_set_step1_window(fsx_ctx* ctx, int enable);
 /* Arithmetic of the matrix-free convection residual (fsx_convection): 64
  * (default, the reference's) or 32, the mixed forms' FP32 element assembly:
  * shape tables, node values, Jacobians and Gauss-point sums in FP32, the

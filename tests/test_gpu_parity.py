@@ -324,6 +324,29 @@ def test_mass_element_form_matches_csr(fsx_lib, oracle_lib, case):
     s.close()
 
 
+@pytest.mark.parametrize("case", ["cavity16", "cavity12_sinh"])
+def test_step1_window_is_bitwise_the_gather_form(fsx_lib, oracle_lib, case):
+    """step1 with its column window staged in LDS (fsx_set_step1_window) reads
+    the same FP32 operand records and sums the same products in the same
+    order as the form that gathers them from memory: 3 time steps in the
+    mixed forms give bitwise the same fields."""
+    say(f"step1 LDS window: {case}, 3 steps, mixed forms")
+    mesh = fsx_lib.Mesh.cavity(16) if case == "cavity16" else fsx_lib.Mesh.cavity(12, cluster=2.0)
+    s, o = make_pair(fsx_lib, oracle_lib, mesh, NGP=27, mode=1, pcg_rtol=1e-8)
+    o.close()
+    out = []
+    for win in (1, 0):
+        s.reset(mode=1)
+        s.set_matrix_precision(32)
+        s.set_convection_precision(32)
+        s.set_step1_window(win)
+        s.run(3)
+        out.append((s.download("Un"), s.download("Pn"), s.download("KAcc")))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    s.close()
+
+
 def test_tecplot_restart_continues_run(fsx_lib, tmp_path):
     """Output and restart through the driver (createTecplot BCH:4249-4482,
     readRestartFile BCH:2795-2796): a run of 2 + 2 steps and a run of 2
