@@ -244,6 +244,17 @@ int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, doub
   ctx->tab.Sp = ctx->tab.dSv + NGP * 81;
   ctx->tab.dSp = ctx->tab.Sp + NGP * 8;
   ctx->tab.w = ctx->tab.dSp + NGP * 24;
+  {
+    std::vector<float> tf(t.begin(), t.end());
+    if (ctx->tabf_mem) dev_free(ctx, ctx->tabf_mem);
+    FSX_TRY(alloc(ctx, &ctx->tabf_mem, tf.size()));
+    FSX_HIP(hipMemcpy(ctx->tabf_mem, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->tabf.Sv = ctx->tabf_mem;
+    ctx->tabf.dSv = ctx->tabf.Sv + NGP * 27;
+    ctx->tabf.Sp = ctx->tabf.dSv + NGP * 81;
+    ctx->tabf.dSp = ctx->tabf.Sp + NGP * 8;
+    ctx->tabf.w = ctx->tabf.dSp + NGP * 24;
+  }
   ctx->NGP = NGP; ctx->density = density; ctx->viscosity = viscosity; ctx->dt = dt;
   ctx->have_params = true;
   return FSX_OK;
@@ -268,6 +279,12 @@ int fsx_set_convection_precision(fsx_ctx* ctx, int bits) {
   if (!ctx) return FSX_EINVAL;
   if (bits != 32 && bits != 64) return fail(ctx, FSX_EINVAL, "fsx_set_convection_precision: 32 or 64");
   ctx->conv_bits = bits;
+  return FSX_OK;
+}
+
+int fsx_set_convection_form(fsx_ctx* ctx, int elements) {
+  if (!ctx) return FSX_EINVAL;
+  ctx->conv_elements = elements != 0;
   return FSX_OK;
 }
 

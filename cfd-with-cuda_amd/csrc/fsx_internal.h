@@ -108,6 +108,15 @@ struct ShapeTables {
   double* dSp;
   double* w;
 };
+// the same tables rounded to FP32 (the FP32 element kernels read them as
+// scalar loads: every thread of a wavefront walks the same Gauss point)
+struct ShapeTablesF {
+  const float* Sv;
+  const float* dSv;
+  const float* Sp;
+  const float* dSp;
+  const float* w;
+};
 
 // A block of solver iterations captured once as a HIP graph (one per
 // starting parity; kernels see the iteration index only through j & 1) and
@@ -161,23 +170,30 @@ struct fsx_ctx {
   double* mass_dinv = nullptr;   // NN, 1 / M_ii, 0 at BC nodes
   double* mass_sc = nullptr;     // device MassScalars
   bool mass_dinv_ready = false;
-  // the element-by-element form of M (FSX_MASS_CONSISTENT): detJ w at every
-  // Gauss point of every element (NE x NGP), the node -> (element, local
-  // node) incidence in ascending element order, and the per-element products
-  // (NE x 27 x 3); built at the first consistent solve. M values uploaded by
-  // fsx_upload_vec no longer match the elements: until the next
-  // fsx_assemble_static the solves then apply the CSR values.
-  // The element form visits the elements sorted by their centre node's
-  // internal id (the nodes' coordinate order), so that a block's elements
-  // and a node's elements sit close together in memory: mass_L27 holds
-  // their velocity ids in that order, and the factors and products follow it.
-  int* mass_L27 = nullptr;
+  // The element-by-element products (the consistent mass, the FP32
+  // convection) visit the elements sorted by their centre node's internal id
+  // (the nodes' coordinate order), so that a block's elements and a node's
+  // elements sit close together in memory: el_L27 holds their velocity ids
+  // transposed ([i][e]), el_P8 their corners' pressure ids ([e][m]), and the
+  // node -> (element, local node) incidence lists the entries i * NE + e of
+  // each node in ascending e. Built at the first element-form product.
+  int* el_L27 = nullptr;
+  int* el_P8 = nullptr;
+  int* el_inc_ptr = nullptr;
+  int* el_inc = nullptr;
+  bool el_ready = false;
+  // The consistent mass's detJ w per element and Gauss point ([k][e]) and its
+  // per-element products ([i][e][3]). M values uploaded by fsx_upload_vec no
+  // longer match the elements: until the next fsx_assemble_static the solves
+  // then apply the CSR values.
   double* mass_fac = nullptr;
-  int* mass_inc_ptr = nullptr;
-  int* mass_inc = nullptr;
   double* mass_ye = nullptr;
+  bool mass_fac_ready = false;
   bool mass_m_uploaded = false;
-  bool mass_elems_ready = false;   // the tables above are built
+  // the FP32 convection's per-element products ([i][e][3]), and whether the
+  // FP32 convection runs element by element (else per colour)
+  float* conv_ye = nullptr;
+  bool conv_elements = true;
   int mass_last_iters = 0;
   double mass_last_relres = 0.0;
   long long mass_total_iters = 0;
@@ -329,6 +345,8 @@ struct fsx_ctx {
   int conv_grid = 0;             // blocks of the persistent convection kernel (0: not yet queried)
   int conv_grid_f = 0;           // the same for the FP32 convection
   double* tab_mem = nullptr;
+  fsx::ShapeTablesF tabf{};     // FP32 copy of tab
+  float* tabf_mem = nullptr;
   // accounting
   std::vector<void*> allocs;
   size_t bytes = 0;
@@ -412,6 +430,7 @@ int launch_kacc(fsx_ctx* ctx);
 int settle_kacc(fsx_ctx* ctx);
 int refresh_matrix_copies(fsx_ctx* ctx);
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);
+int elements_setup(fsx_ctx* ctx);
 int mass_update_step1(fsx_ctx* ctx);
 int mass_update_step3(fsx_ctx* ctx);
 int launch_maxacc(fsx_ctx* ctx);

@@ -311,17 +311,15 @@ __global__ void k_mass_axpy(long long n, const double* __restrict__ u, const dou
     y[i] = u[i] + dt * x[i];
 }
 
-// The element form's tables (host-built once): the elements sorted by
-// their centre node's internal id, their velocity ids in that order, detJ w
-// per element and Gauss point, and the node -> (sorted element, local node)
+// The element forms' tables (host-built once): the elements sorted by
+// their centre node's internal id, their velocity ids transposed, their
+// corners' pressure ids, and the node -> (sorted element, local node)
 // incidence, ascending per node.
-static int mass_elements_setup(fsx_ctx* ctx) {
+int elements_setup(fsx_ctx* ctx) {
   const int NE = ctx->NE, NN = ctx->NN;
-  for (void* q : {(void*)ctx->mass_L27, (void*)ctx->mass_fac, (void*)ctx->mass_inc_ptr, (void*)ctx->mass_inc,
-                  (void*)ctx->mass_ye})
+  for (void* q : {(void*)ctx->el_L27, (void*)ctx->el_P8, (void*)ctx->el_inc_ptr, (void*)ctx->el_inc})
     dev_free(ctx, q);   // what an earlier, failed set-up left
-  ctx->mass_L27 = nullptr; ctx->mass_fac = nullptr; ctx->mass_inc_ptr = nullptr; ctx->mass_inc = nullptr;
-  ctx->mass_ye = nullptr;
+  ctx->el_L27 = nullptr; ctx->el_P8 = nullptr; ctx->el_inc_ptr = nullptr; ctx->el_inc = nullptr;
   std::vector<int> L((size_t)NE * 27), P((size_t)NE * 8);
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   FSX_HIP(hipMemcpy(L.data(), ctx->L27, L.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -340,36 +338,38 @@ static int mass_elements_setup(fsx_ctx* ctx) {
   std::vector<int> next(ptr.begin(), ptr.end() - 1);
   for (int k = 0; k < NE; k++)   // entries i * NE + k, ascending k per node
     for (int i = 0; i < 27; i++) inc[next[Ls[(size_t)i * NE + k]]++] = i * NE + k;
-  int* P8s = nullptr;
-  FSX_TRY(alloc(ctx, &ctx->mass_L27, Ls.size()));
-  FSX_TRY(alloc(ctx, &P8s, Ps.size()));
-  FSX_TRY(alloc(ctx, &ctx->mass_fac, (size_t)NE * ctx->NGP));
-  FSX_TRY(alloc(ctx, &ctx->mass_inc_ptr, ptr.size()));
-  FSX_TRY(alloc(ctx, &ctx->mass_inc, inc.size()));
-  FSX_TRY(alloc(ctx, &ctx->mass_ye, (size_t)NE * 81));
-  FSX_HIP(hipMemcpy(ctx->mass_L27, Ls.data(), Ls.size() * sizeof(int), hipMemcpyHostToDevice));
-  FSX_HIP(hipMemcpy(P8s, Ps.data(), Ps.size() * sizeof(int), hipMemcpyHostToDevice));
-  FSX_HIP(hipMemcpy(ctx->mass_inc_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
-  FSX_HIP(hipMemcpy(ctx->mass_inc, inc.data(), inc.size() * sizeof(int), hipMemcpyHostToDevice));
-  FSX_TRY(launch_mass_fac(ctx, P8s, ctx->mass_fac));
-  FSX_HIP(hipStreamSynchronize(ctx->stream));
-  dev_free(ctx, P8s);
-  ctx->mass_elems_ready = true;
+  FSX_TRY(alloc(ctx, &ctx->el_L27, Ls.size()));
+  FSX_TRY(alloc(ctx, &ctx->el_P8, Ps.size()));
+  FSX_TRY(alloc(ctx, &ctx->el_inc_ptr, ptr.size()));
+  FSX_TRY(alloc(ctx, &ctx->el_inc, inc.size()));
+  FSX_HIP(hipMemcpy(ctx->el_L27, Ls.data(), Ls.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->el_P8, Ps.data(), Ps.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->el_inc_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->el_inc, inc.data(), inc.size() * sizeof(int), hipMemcpyHostToDevice));
+  ctx->el_ready = true;
   return FSX_OK;
+}
+
+// the consistent mass's factors and products on top of the element tables
+static int mass_elements_setup(fsx_ctx* ctx) {
+  if (!ctx->el_ready) FSX_TRY(elements_setup(ctx));
+  if (!ctx->mass_fac) FSX_TRY(alloc(ctx, &ctx->mass_fac, (size_t)ctx->NE * ctx->NGP));
+  if (!ctx->mass_ye) FSX_TRY(alloc(ctx, &ctx->mass_ye, (size_t)ctx->NE * 81));
+  return launch_mass_fac(ctx, ctx->el_P8, ctx->mass_fac);
 }
 
 // q = M_bc p; with dot the p.q partials (slots 6-8). sc null: always runs.
 static void mass_apply(fsx_ctx* ctx, bool elements, const double* p, double* q, bool dot, const MassScalars* sc) {
   const int NN = ctx->NN;
   if (elements) {
-    hipLaunchKernelGGL(k_mass_elem, dim3((unsigned)((3LL * ctx->NE + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, ctx->NE, ctx->NGP, ctx->mass_L27, ctx->mass_fac, ctx->tab.Sv,
+    hipLaunchKernelGGL(k_mass_elem, dim3((unsigned)((3LL * ctx->NE + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, ctx->NE, ctx->NGP, ctx->el_L27, ctx->mass_fac, ctx->tab.Sv,
                        p, ctx->mass_ye, sc);
     if (dot)
-      hipLaunchKernelGGL(k_mass_gather<true>, dim3(kGatherGrid), dim3(kBlock), 0, ctx->stream, NN, ctx->mass_inc_ptr,
-                         ctx->mass_inc, ctx->mass_ye, ctx->bcflag, p, q, ctx->partials, sc);
+      hipLaunchKernelGGL(k_mass_gather<true>, dim3(kGatherGrid), dim3(kBlock), 0, ctx->stream, NN, ctx->el_inc_ptr,
+                         ctx->el_inc, ctx->mass_ye, ctx->bcflag, p, q, ctx->partials, sc);
     else
       hipLaunchKernelGGL(k_mass_gather<false>, dim3((unsigned)std::min((NN + kBlock - 1) / kBlock, 4096)), dim3(kBlock),
-                         0, ctx->stream, NN, ctx->mass_inc_ptr, ctx->mass_inc, ctx->mass_ye, ctx->bcflag, p, q,
+                         0, ctx->stream, NN, ctx->el_inc_ptr, ctx->el_inc, ctx->mass_ye, ctx->bcflag, p, q,
                          ctx->partials, sc);
     return;
   }
@@ -396,7 +396,8 @@ int mass_solve(fsx_ctx* ctx, const double* b, double* x) {
     ctx->mass_dinv_ready = true;
   }
   const bool elements = ctx->mass_mode == FSX_MASS_CONSISTENT && !ctx->mass_m_uploaded;
-  if (elements && !ctx->mass_elems_ready) FSX_TRY(mass_elements_setup(ctx));
+  if (elements && !ctx->mass_fac_ready) FSX_TRY(mass_elements_setup(ctx));
+  if (elements) ctx->mass_fac_ready = true;
   MassScalars* sc = reinterpret_cast<MassScalars*>(ctx->mass_sc);
   double *r = ctx->mass_w, *z = r + 3 * (size_t)NN, *p = z + 3 * (size_t)NN, *q = p + 3 * (size_t)NN;
   mass_apply(ctx, elements, x, q, false, nullptr);

@@ -36,7 +36,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision", "fsx_set_step1_window",
+    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_convection_form",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
@@ -115,6 +115,7 @@ def lib():
         "fsx_bench_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_set_mass": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
         "fsx_set_step1_window": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_set_convection_form": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_matrix_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_convection_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_solver_x0": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -441,6 +442,9 @@ class Solver:
         """64 (the reference) or 32: FP32 copies of K, G, G^T in the step
         kernels, the mixed-precision forms' FP32 assembly."""
         self.check(self.L.fsx_set_matrix_precision(self.ctx, bits), "set_matrix_precision")
+
+    def set_convection_form(self, elements):
+        self.check(self.L.fsx_set_convection_form(self.ctx, int(bool(elements))), "set_convection_form")
 
     def set_step1_window(self, enable):
         self.check(self.L.fsx_set_step1_window(self.ctx, int(bool(enable))), "set_step1_window")

@@ -166,6 +166,13 @@ int fsx_set_step1_window(fsx_ctx* ctx, int enable);
  * shape tables, node values, Jacobians and Gauss-point sums in FP32, the
  * element result subtracted from the FP64 R1. No reference counterpart. */
 int fsx_set_convection_precision(fsx_ctx* ctx, int bits);
+/* Form of the FP32 convection: elements != 0 (the default) one pass over all
+ * elements, one thread per element and velocity component, the element rows
+ * summed per node in a fixed order; 0 one pass per colour of the element
+ * colouring with the rows added into R1 (the FP64 convection is always the
+ * latter). Both compute each element's rows in the same FP32 arithmetic. No
+ * reference counterpart (a kernel-form switch for measurement and checks). */
+int fsx_set_convection_form(fsx_ctx* ctx, int elements);
 /* Initial guess of the pressure solves (fsx_step2, fsx_solve_pressure).
  * FSX_X0_PREVIOUS starts from the Pdot already held (zero after
  * fsx_init_state, as the reference zeroes Pdot_d once): r0 = b - Z x0 in

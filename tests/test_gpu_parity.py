@@ -451,6 +451,16 @@ def test_fp32_convection(fsx_lib):
     e = rel(r32, r64)
     say(f"  one convection call: FP32 vs FP64 {e:.2e}")
     assert 0 < e < 1e-5
+    # the colour-by-colour FP32 form computes each element's rows in the same
+    # FP32 arithmetic; the FP64 sums over a node's elements run in another
+    # order, and the rows of neighbouring elements cancel to a residual about
+    # 1e5 times smaller than they are (measured 2.0e-11 normwise; bar 1e-9)
+    s.set_convection_form(0)
+    assert L.fsx_convection(ctx) == 0
+    rc = s.download("R1")
+    say(f"  element vs colour form: {rel(r32, rc):.2e}")
+    assert rel(r32, rc) < 1e-9
+    s.set_convection_form(1)
     s.set_convection_precision(64)
     s.reset(mode=0)
     s.run(3)
