@@ -1180,6 +1180,65 @@ int orc_direct_solve(void* h, const double* b, double* x, double* relres) {
 // half-bandwidth of the kept factor (-1 when none)
 int orc_chol_bandwidth(void* h) { return ((Case*)h)->chol_w; }
 
+// Symbolic Cholesky of the symmetric pattern (rowptr, col; both triangles,
+// diagonal included) under the order perm (perm[new] = old): the column
+// counts of L (diagonal included), summed into *nnzL, with sum_k c_k^2 in
+// *flops (the multiply-add count of the numeric factor, up to lower-order
+// terms) and the elimination tree's height. The elimination tree is built
+// with path compression (ancestor links); row i of L is the union of the
+// tree paths from the row's entries j < i up to i, so walking each path
+// until a node already marked for row i counts every entry of L once.
+// (The prototype orders Z with symamd before chol, ML:1880-1881; this is
+// the fill any order gives, for comparing orders.)
+int orc_symbolic_cholesky(int n, const int* rowptr, const int* col, const int* perm, long long* nnzL, double* flops,
+                          int* height) {
+  if (n <= 0 || !rowptr || !col || !perm) return 1;
+  std::vector<int> iperm(n, -1), parent(n, -1), anc(n, -1), mark(n, -1);
+  for (int i = 0; i < n; i++) {
+    if (perm[i] < 0 || perm[i] >= n || iperm[perm[i]] >= 0) return 2;
+    iperm[perm[i]] = i;
+  }
+  for (int i = 0; i < n; i++) {
+    const int o = perm[i];
+    for (int t = rowptr[o]; t < rowptr[o + 1]; t++) {
+      int k = iperm[col[t]];
+      while (k >= 0 && k < i) {
+        const int nx = anc[k];
+        anc[k] = i;
+        if (nx < 0) { parent[k] = i; break; }
+        k = nx;
+      }
+    }
+  }
+  std::vector<long long> cc(n, 1);
+  for (int i = 0; i < n; i++) {
+    mark[i] = i;
+    const int o = perm[i];
+    for (int t = rowptr[o]; t < rowptr[o + 1]; t++) {
+      int k = iperm[col[t]];
+      if (k >= i) continue;
+      while (mark[k] != i) {
+        mark[k] = i;
+        cc[k]++;
+        k = parent[k];
+      }
+    }
+  }
+  long long s = 0;
+  double f = 0.0;
+  for (int k = 0; k < n; k++) { s += cc[k]; f += (double)cc[k] * (double)cc[k]; }
+  std::vector<int> depth(n, 0);
+  int h = 0;
+  for (int k = n - 1; k >= 0; k--) {
+    depth[k] = parent[k] < 0 ? 1 : depth[parent[k]] + 1;
+    h = std::max(h, depth[k]);
+  }
+  if (nnzL) *nnzL = s;
+  if (flops) *flops = f;
+  if (height) *height = h;
+  return 0;
+}
+
 // step1's K operands and step2's G^T operand in FP64 (0) or rounded to FP32
 // (1); see Case::f32_operands
 int orc_set_f32_operands(void* h, int f32) {

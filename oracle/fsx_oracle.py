@@ -63,6 +63,7 @@ def lib():
         L.orc_set_direct.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_direct_solve.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.orc_chol_bandwidth.argtypes = [ctypes.c_void_p]
+        L.orc_symbolic_cholesky.argtypes = [ctypes.c_int, _I, _I, _I, ctypes.POINTER(ctypes.c_longlong), _D, _I]
         L.orc_mass_stats.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -255,6 +256,56 @@ def pcg_csr(rowptr, col, val, b, rtol=1e-8, maxit=5000):
     rel = ctypes.c_double()
     it = L.orc_pcg_csr(b.size, _ip(rowptr), _ip(col), _dp(val), _dp(b), _dp(x), rtol, maxit, ctypes.byref(rel))
     return x, it, rel.value
+
+
+def symbolic_cholesky(rowptr, col, perm):
+    """(nnz(L) with the diagonal, sum of squared column counts, elimination
+    tree height) of the Cholesky factor of the symmetric pattern under perm
+    (perm[new] = old)"""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    perm = np.ascontiguousarray(perm, dtype=np.int32)
+    nnz = ctypes.c_longlong()
+    fl = np.zeros(1)
+    h = np.zeros(1, dtype=np.int32)
+    rc = lib().orc_symbolic_cholesky(perm.size, _ip(rowptr), _ip(col), _ip(perm), ctypes.byref(nnz), _dp(fl), _ip(h))
+    if rc != 0:
+        raise ValueError(f"symbolic_cholesky: bad input ({rc})")
+    return nnz.value, float(fl[0]), int(h[0])
+
+
+def nested_dissection_grid(nx, ny, nz, radius=2, leaf=64):
+    """Nested-dissection order of an nx*ny*nz grid (x fastest) whose nodes
+    couple within `radius` per axis: each box is split across its longest
+    axis by a slab of `radius` planes, the two halves ordered first
+    (recursively), the slab last; boxes of at most `leaf` nodes keep the
+    grid order. Returns perm (perm[new] = old)."""
+    out = []
+
+    def rec(x0, x1, y0, y1, z0, z1):
+        dims = (x1 - x0, y1 - y0, z1 - z0)
+        if dims[0] * dims[1] * dims[2] <= leaf or max(dims) <= 2 * radius + 1:
+            for z in range(z0, z1):
+                for y in range(y0, y1):
+                    out.extend(range(z * nx * ny + y * nx + x0, z * nx * ny + y * nx + x1))
+            return
+        a = int(np.argmax(dims))
+        lo, hi = (x0, x1) if a == 0 else ((y0, y1) if a == 1 else (z0, z1))
+        m = (lo + hi - radius) // 2
+        box = [x0, x1, y0, y1, z0, z1]
+        left, right, sep = list(box), list(box), list(box)
+        left[2 * a + 1] = m
+        sep[2 * a], sep[2 * a + 1] = m, m + radius
+        right[2 * a] = m + radius
+        rec(*left)
+        rec(*right)
+        rec_sep = sep
+        for z in range(rec_sep[4], rec_sep[5]):
+            for y in range(rec_sep[2], rec_sep[3]):
+                out.extend(range(z * nx * ny + y * nx + rec_sep[0], z * nx * ny + y * nx + rec_sep[1]))
+
+    rec(0, nx, 0, ny, 0, nz)
+    return np.asarray(out, dtype=np.int32)
 
 
 def set_threads(n):

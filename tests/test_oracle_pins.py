@@ -529,3 +529,70 @@ def test_direct_pressure_solve(oracle_lib):
     assert (sd[:, 1] == 0).all()
     for name in ("Un", "Pn"):
         assert np.linalg.norm(d.d(name) - a.d(name)) <= 1e-10 * np.linalg.norm(a.d(name)), name
+
+
+def _grid_pattern(nx, ny, nz, radius):
+    """CSR of the (2 radius + 1)^3-point stencil on an nx*ny*nz grid (x fastest)"""
+    import scipy.sparse as sp
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    rows, cols = [], []
+    rr = range(-radius, radius + 1)
+    for dz in rr:
+        for dy in rr:
+            for dx in rr:
+                a = idx[max(0, -dz):nz - max(0, dz), max(0, -dy):ny - max(0, dy), max(0, -dx):nx - max(0, dx)]
+                b = idx[max(0, dz):nz - max(0, -dz) if dz < 0 else nz, :, :][:a.shape[0]]
+                b = idx[max(0, dz):max(0, dz) + a.shape[0], max(0, dy):max(0, dy) + a.shape[1],
+                        max(0, dx):max(0, dx) + a.shape[2]]
+                rows.append(a.ravel())
+                cols.append(b.ravel())
+    m = sp.csr_matrix((np.ones(sum(r.size for r in rows)), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    m.sum_duplicates()
+    m.sort_indices()
+    return m
+
+
+@pytest.mark.parametrize("order", ["grid", "nd", "random"])
+def test_symbolic_cholesky_counts_the_numeric_fill(oracle_lib, order):
+    """The oracle's symbolic Cholesky (column counts by row subtrees of the
+    elimination tree) against the structural nonzeros of a numeric Cholesky
+    of an SPD matrix with that pattern (random positive weights, diagonally
+    dominant, so no entry of L cancels to zero), on a 6x5x4 grid with the
+    pressure operator's radius-2 coupling, for three orders."""
+    m = _grid_pattern(6, 5, 4, 2)
+    n = m.shape[0]
+    rng = np.random.default_rng(3)
+    if order == "grid":
+        perm = np.arange(n, dtype=np.int32)
+    elif order == "nd":
+        perm = oracle_lib.nested_dissection_grid(6, 5, 4, radius=2, leaf=8)
+        assert np.array_equal(np.sort(perm), np.arange(n))
+    else:
+        perm = rng.permutation(n).astype(np.int32)
+    a = m.toarray()
+    w = np.triu(rng.uniform(0.1, 1.0, (n, n)), 1)
+    a = a * (w + w.T)
+    np.fill_diagonal(a, a.sum(axis=1) + 1.0)
+    ap = a[np.ix_(perm, perm)]
+    L = np.linalg.cholesky(ap)
+    nnz_num = int(np.count_nonzero(np.abs(L) > 1e-300))
+    nnz, flops, h = oracle_lib.symbolic_cholesky(m.indptr, m.indices, perm)
+    assert nnz == nnz_num
+    cc = np.count_nonzero(np.abs(L) > 1e-300, axis=0)
+    assert flops == float(np.sum(cc.astype(np.float64) ** 2))
+    assert 1 <= h <= n
+
+
+def test_nested_dissection_fills_less_than_the_band(oracle_lib):
+    """On a 17^3 grid with radius-2 coupling the nested-dissection order's
+    factor is smaller and cheaper than the grid (band) order's, whose column
+    counts are its half-bandwidth (2 nx ny + 2 nx + 2) plus one."""
+    nx = 17
+    m = _grid_pattern(nx, nx, nx, 2)
+    n = m.shape[0]
+    band = oracle_lib.symbolic_cholesky(m.indptr, m.indices, np.arange(n, dtype=np.int32))
+    nd = oracle_lib.symbolic_cholesky(m.indptr, m.indices, oracle_lib.nested_dissection_grid(nx, nx, nx))
+    w = 2 * nx * nx + 2 * nx + 2
+    assert band[0] <= n * (w + 1)
+    assert nd[0] < 0.75 * band[0] and nd[1] < 0.6 * band[1]   # measured 0.68 and 0.56
