@@ -188,6 +188,8 @@ int launch_assemble_static(fsx_ctx* ctx) {
                        ctx->Grow, ctx->Gcol, ctx->G);
     FSX_TRY(launch_check(ctx, "k_assemble"));
   }
+  ctx->k_viscosity = ctx->viscosity;   // the element form of step1 applies this K
+  ctx->k_uploaded = false;
   FSX_TRY(launch_lump(ctx));
   hipLaunchKernelGGL(k_gt_values, dim3(4096), dim3(kBlock), 0, ctx->stream, ctx->nnzG, ctx->GTpos, ctx->G, ctx->GT);
   return launch_check(ctx, "k_gt_values");
@@ -578,8 +580,19 @@ static int launch_convection_t(fsx_ctx* ctx, int* grid_cache) {
   return launch_check(ctx, "k_convection");
 }
 
-// FP64 arithmetic, or FP32 (fsx_set_convection_precision, the mixed forms)
+// FP64 arithmetic, or FP32 (fsx_set_convection_precision, the mixed forms).
+// When the next step1 can run element by element, the FP32 convection is
+// only recorded here and formed in that pass (launch_step1).
 int launch_convection(fsx_ctx* ctx) {
+  if (step1_elements_ok(ctx)) {
+    ctx->conv_pending = true;
+    return FSX_OK;
+  }
+  return launch_convection_now(ctx);
+}
+
+int launch_convection_now(fsx_ctx* ctx) {
+  ctx->conv_pending = false;
   // the FP32 element form writes every entry of R1 (BCH:3541-3545's zeroing included)
   if (ctx->conv_bits == 32 && ctx->conv_elements) return launch_convection_elements(ctx);
   FSX_TRY(launch_fill(ctx, ctx->R1, 0.0, 3LL * ctx->NN));                  // BCH:3541-3545

@@ -255,6 +255,7 @@ int fsx_set_params(fsx_ctx* ctx, int NGP, double density, double viscosity, doub
     ctx->tabf.dSp = ctx->tabf.Sp + NGP * 8;
     ctx->tabf.w = ctx->tabf.dSp + NGP * 24;
   }
+  FSX_TRY(elem_tables_setup(ctx, t.data(), NGP));
   ctx->NGP = NGP; ctx->density = density; ctx->viscosity = viscosity; ctx->dt = dt;
   ctx->have_params = true;
   return FSX_OK;
@@ -291,6 +292,13 @@ int fsx_set_convection_form(fsx_ctx* ctx, int elements) {
 int fsx_set_step1_window(fsx_ctx* ctx, int enable) {
   if (!ctx) return FSX_EINVAL;
   ctx->use_step1_window = enable != 0;
+  return FSX_OK;
+}
+
+int fsx_set_step1_elements(fsx_ctx* ctx, int enable) {
+  if (!ctx) return FSX_EINVAL;
+  FSX_TRY(settle_deferred(ctx));
+  ctx->use_step1_elements = enable != 0;
   return FSX_OK;
 }
 
@@ -790,6 +798,7 @@ int fsx_init_state(fsx_ctx* ctx) {
   if (!ctx->have_mesh || !ctx->have_bc) return fail(ctx, FSX_ESTATE, "fsx_init_state needs mesh and BCs");
   const long long n3 = 3LL * ctx->NN;
   ctx->kacc_pending = false;
+  ctx->conv_pending = false;
   ctx->gpn_valid = false;
   drop_records(ctx);
   for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
@@ -820,7 +829,7 @@ static int need_step(fsx_ctx* ctx) {
 
 int fsx_begin_step(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   FSX_TRY(launch_copy(ctx, ctx->UnpHalf_prev, ctx->Un, 3LL * ctx->NN));    // BCH:2872-2876
   FSX_TRY(launch_fill(ctx, ctx->Acc_prev, 0.0, 3LL * ctx->NN));
   FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pn, ctx->NNp));
@@ -832,7 +841,7 @@ int fsx_step1(fsx_ctx* ctx, int iter) { FSX_TRY(need_step(ctx)); return launch_s
 
 int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres) {
   FSX_TRY(need_step(ctx));
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   FSX_TRY(launch_step2_rhs(ctx, iter));
   int it = 0;
   double rel = 0.0;
@@ -846,7 +855,7 @@ int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres) {
 
 int fsx_step3(fsx_ctx* ctx, int iter) {
   FSX_TRY(need_step(ctx));
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   return launch_step3(ctx, iter);
 }
 
@@ -864,7 +873,7 @@ int fsx_check_convergence(fsx_ctx* ctx, double tol, int* converged, double* rel_
 
 int fsx_next_iteration(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   // BCH:2964-2972. UnpHalf and Acc are rewritten by step1/step3 before they
   // are read again, so their "prev" copies are pointer swaps. Unp1 and Pnp1
   // must keep their values: when this was the last inner iteration the time
@@ -884,7 +893,7 @@ int fsx_next_iteration(fsx_ctx* ctx) {
 
 // KAcc = K Acc_prev: recorded here and formed by the next fsx_step1 in its
 // pass over K (k_step1 with KACC); any other entry point that reads KAcc or
-// changes Acc_prev first forms it on its own (settle_kacc).
+// changes Acc_prev first forms it on its own (settle_deferred).
 int fsx_kacc(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
   ctx->kacc_pending = true;
@@ -893,7 +902,7 @@ int fsx_kacc(fsx_ctx* ctx) {
 
 int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
   FSX_TRY(need_step(ctx));
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   FSX_TRY(launch_maxacc(ctx));
   FSX_HIP(hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(StepScalars), hipMemcpyDeviceToHost, ctx->stream));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -903,7 +912,7 @@ int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
   std::swap(ctx->Un, ctx->Unp1);                                           // BCH:3070-3077
   std::swap(ctx->Pn, ctx->Pnp1);
   ctx->gpn_valid = false;                                                  // G Pn changes with Pn
-  ctx->ucmp_ok = false;                                                    // and Un's shadow with Un
+  ctx->ucmp_ok = ctx->ucmp_un = false;                                     // and Un's shadow with Un
   return FSX_OK;
 }
 
@@ -982,7 +991,7 @@ int fsx_download(fsx_ctx* ctx, int id, double* host, size_t count) {
   double* p; size_t n; int kind;
   FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_download: count mismatch (expected " + std::to_string(n) + ")");
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   const size_t NN = ctx->NN;
   const std::vector<int>& ip = ctx->viperm;
@@ -1020,7 +1029,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   FSX_TRY(vec_info(ctx, id, &p, &n, &kind));
   if (kind != 0 && kind != 3 && kind != 4)
     return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
   ctx->gpn_valid = false;
   drop_records(ctx);   // the FP32 shadows may no longer match the vectors
@@ -1037,7 +1046,10 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
     std::vector<double> t(n);
     for (size_t k = 0; k < n; k++) t[ctx->posM[k]] = host[k];
     FSX_HIP(hipMemcpy(p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    if (id == FSX_VAL_K) FSX_TRY(refresh_matrix_copies(ctx));
+    if (id == FSX_VAL_K) {
+      FSX_TRY(refresh_matrix_copies(ctx));
+      ctx->k_uploaded = true;   // step1 applies these CSR values until the next fsx_assemble_static
+    }
     if (id == FSX_VAL_M) {
       // the lumped masses follow M (Z keeps the lumped mass it was built
       // from until fsx_build_Z runs again)
@@ -1068,7 +1080,7 @@ int fsx_sync(fsx_ctx* ctx) {
 int fsx_solve_pressure(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   if (!ctx || !b || !x) return FSX_EINVAL;
   if (!ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   FSX_HIP(hipMemcpyAsync(ctx->R2, b, (size_t)ctx->NNp * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   int it = 0;
   double rel = 0;
@@ -1085,7 +1097,7 @@ int fsx_spmv(fsx_ctx* ctx, int which, const double* x, double* y, double alpha, 
   if (!ctx || !x || !y) return FSX_EINVAL;
   if (!ctx->assembled) return fail(ctx, FSX_ESTATE, "assemble first");
   if ((which == 4 || which == 8 || which == 9) && !ctx->z_built) return fail(ctx, FSX_ESTATE, "build Z first");
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   if ((which == 8 || which == 9) && !(alpha == 1.0 && beta == 0.0))
     return fail(ctx, FSX_EINVAL, "fsx_spmv: the solvers' Z forms take alpha = 1, beta = 0");
   size_t nx, ny;

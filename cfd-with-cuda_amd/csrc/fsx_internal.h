@@ -156,7 +156,7 @@ struct fsx_ctx {
   int maxit = 5000;
   bool have_mesh = false, have_M = false, have_G = false, have_Z = false, have_bc = false, have_params = false;
   bool assembled = false, z_built = false, state_ready = false;
-  bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_kacc)
+  bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_deferred)
   bool gpn_valid = false;        // GPn holds G Pn for the current Pn
   double* GPn = nullptr;         // 3NN: G Pn, node-major, kept over the inner iterations of a step
   // velocity mass (fsx_set_mass): lumped (the reference) or consistent, M_bc
@@ -258,6 +258,22 @@ struct fsx_ctx {
   // K operand (UnpHalf_prev = Un then); valid until the next inner iteration
   float* Ucmp = nullptr;
   bool ucmp_ok = false;
+  bool ucmp_un = false;          // Ucmp holds Un rounded (set with the step's shadows, cleared when Un changes)
+  // step1 element by element on the matrix cores (fsx_elem.hip): the
+  // convection, K UnpHalf_prev and K Acc_prev from the element forms, then a
+  // node pass. fsx_convection only records the convection (conv_pending)
+  // when the next step1 can take this form; anything else that reads R1 or
+  // changes the state first forms it on its own (settle_deferred).
+  bool use_step1_elements = true;  // fsx_set_step1_elements
+  bool conv_pending = false;
+  float* ek_tab = nullptr;       // packed MFMA A operands per Gauss-point tile, then dSp and w per point
+  int ek_ntiles = 0;
+  int ek_grid = 0;               // blocks of the element kernel (0: 8 per CU)
+  float* ek_ye = nullptr;        // per-element rows [i][e][6]: conv + K UnpHalf_prev, K Acc_prev
+  float* ek_xl = nullptr;        // corner coordinates relative to corner 0, [e][8][3] (sorted elements)
+  bool ek_xl_ready = false;
+  double k_viscosity = 0.0;      // the viscosity K was assembled with
+  bool k_uploaded = false;       // K values from fsx_upload_vec (no element form)
   // M/K columns as a dictionary of row patterns (as for Z below); null when
   // the rows do not compress
   int* Mrowpat = nullptr; int* Mpatoff = nullptr;
@@ -422,12 +438,17 @@ int direct_factor(fsx_ctx* ctx);
 int direct_apply(fsx_ctx* ctx, const double* b, double* x);
 inline void drop_records(fsx_ctx* ctx) {
   ctx->urec_ok = ctx->urec_prev_ok = ctx->arec_ok = ctx->arec_prev_ok = ctx->drec_ok = ctx->ucmp_ok = false;
+  ctx->ucmp_un = false;
 }
 int launch_step2_rhs(fsx_ctx* ctx, int iter);
 int launch_pnp1(fsx_ctx* ctx);
 int launch_step3(fsx_ctx* ctx, int iter);
 int launch_kacc(fsx_ctx* ctx);
-int settle_kacc(fsx_ctx* ctx);
+int settle_deferred(fsx_ctx* ctx);
+int launch_convection_now(fsx_ctx* ctx);
+int elem_tables_setup(fsx_ctx* ctx, const double* tables, int NGP);
+bool step1_elements_ok(const fsx_ctx* ctx);
+int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload);
 int refresh_matrix_copies(fsx_ctx* ctx);
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);
 int elements_setup(fsx_ctx* ctx);

@@ -347,6 +347,7 @@ int elements_setup(fsx_ctx* ctx) {
   FSX_HIP(hipMemcpy(ctx->el_inc_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
   FSX_HIP(hipMemcpy(ctx->el_inc, inc.data(), inc.size() * sizeof(int), hipMemcpyHostToDevice));
   ctx->el_ready = true;
+  ctx->ek_xl_ready = false;   // the element kernels' corner table follows the new order
   return FSX_OK;
 }
 

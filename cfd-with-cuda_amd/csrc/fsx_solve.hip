@@ -200,6 +200,20 @@ __global__ __launch_bounds__(kBlock) void k_step1(int NN, const int* __restrict_
 
 int launch_step1(fsx_ctx* ctx, int iter) {
   (void)iter;
+  if (ctx->conv_pending) {
+    // the convection recorded by fsx_convection: formed here with the K
+    // products element by element, or on its own when that form no longer applies
+    if (step1_elements_ok(ctx)) {
+      const bool kacc = ctx->kacc_pending;
+      FSX_TRY(launch_step1_elements(ctx, kacc, ctx->gpn_valid));
+      ctx->conv_pending = ctx->kacc_pending = false;
+      ctx->gpn_valid = true;
+      ctx->urec_ok = ctx->drec_ok = true;
+      ctx->drec_kacc = kacc;
+      return FSX_OK;
+    }
+    FSX_TRY(launch_convection_now(ctx));
+  }
   // 8 lanes x 2 K entries in flight per row. Fused with K*Acc at 64^3
   // (node-major vectors): 8x2 1.064 ms, 16x2 1.057, 8x4 1.077, 16x4 1.092,
   // 8x8 1.150, 8x16 1.217, 16x8 1.224, 4x16 1.352, 32x2 1.287
@@ -314,7 +328,7 @@ int records_begin_step(fsx_ctx* ctx) {
   hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
                      reinterpret_cast<float4*>(ctx->Ucmp), 1);
   FSX_TRY(launch_check(ctx, "k_rec"));
-  ctx->urec_prev_ok = ctx->arec_prev_ok = ctx->ucmp_ok = true;
+  ctx->urec_prev_ok = ctx->arec_prev_ok = ctx->ucmp_ok = ctx->ucmp_un = true;
   return FSX_OK;
 }
 
@@ -375,9 +389,14 @@ int launch_kacc(fsx_ctx* ctx) {
 }
 
 // KAcc = K Acc_prev is deferred from fsx_kacc to the next step1, which forms
-// it in its own pass over K; every other entry point that could read KAcc or
-// change Acc_prev first settles a pending product with k_k3.
-int settle_kacc(fsx_ctx* ctx) { return ctx->kacc_pending ? launch_kacc(ctx) : FSX_OK; }
+// it in its own pass over K (or over the elements); every other entry point
+// that could read KAcc or change Acc_prev first settles a pending product
+// with k_k3. A convection recorded for the element form of step1 is formed
+// the same way (into R1) before anything else reads R1 or changes the state.
+int settle_deferred(fsx_ctx* ctx) {
+  if (ctx->conv_pending) FSX_TRY(launch_convection_now(ctx));
+  return ctx->kacc_pending ? launch_kacc(ctx) : FSX_OK;
+}
 
 // ------------------------------------------------------------ step 2 RHS
 // R2[p] = sum_c G_c^T d_c, d_c = UnpHalf_c / dt^2 - [iter>1] MdOrigInv KAcc_c,
@@ -1521,7 +1540,7 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
               24.0 * NN + (ctx->Gf && ctx->rec_mem ? 12.0 * NN : 0.0);
       break;
   }
-  FSX_TRY(settle_kacc(ctx));
+  FSX_TRY(settle_deferred(ctx));
   for (int w = 0; w < 2; w++) launch();
   FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int r = 0; r < reps; r++) launch();

@@ -36,7 +36,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_convection_form",
+    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
@@ -115,6 +115,7 @@ def lib():
         "fsx_bench_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _D, _D], ctypes.c_int),
         "fsx_set_mass": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
         "fsx_set_step1_window": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_set_step1_elements": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_convection_form": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_matrix_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_convection_precision": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -448,6 +449,11 @@ class Solver:
 
     def set_step1_window(self, enable):
         self.check(self.L.fsx_set_step1_window(self.ctx, int(bool(enable))), "set_step1_window")
+
+    def set_step1_elements(self, enable):
+        """step1 of the mixed forms from the element forms of the convection
+        and K on the matrix cores (the default) or from K's CSR values."""
+        self.check(self.L.fsx_set_step1_elements(self.ctx, int(bool(enable))), "set_step1_elements")
 
     def set_solver_x0(self, policy):
         """X0_ZERO (the CPU path) or X0_PREVIOUS (warm start from the previous

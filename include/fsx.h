@@ -161,6 +161,19 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
  * their order and so the results are the same either way. No reference
  * counterpart (a kernel-form switch for measurement and checks). */
 int fsx_set_step1_window(fsx_ctx* ctx, int enable);
+/* step1 of the mixed forms element by element (enable != 0, the default):
+ * when the convection and the matrices are FP32, the convection is in its
+ * element form, the mass is lumped and K is the assembled one (not uploaded
+ * through fsx_upload_vec), fsx_convection only records the convection and
+ * the next fsx_step1 forms it together with K UnpHalf_prev and the pending
+ * K Acc_prev from the element forms (FP32 matrix-core products of the shape
+ * tables with the FP32 shadows of Un, UnpHalf_prev and Acc_prev, the node
+ * sums FP64) instead of the CSR products of K; then the same update of R1,
+ * UnpHalf and KAcc as the CSR form, equal to FP32 rounding. Any other entry
+ * point that reads R1 or changes the state first forms a recorded
+ * convection on its own. 0: the CSR products. No reference counterpart
+ * (the reference applies the assembled K, BCH:3746-3783). */
+int fsx_set_step1_elements(fsx_ctx* ctx, int enable);
 /* Arithmetic of the matrix-free convection residual (fsx_convection): 64
  * (default, the reference's) or 32, the mixed forms' FP32 element assembly:
  * shape tables, node values, Jacobians and Gauss-point sums in FP32, the

@@ -339,6 +339,7 @@ def test_step1_window_is_bitwise_the_gather_form(fsx_lib, oracle_lib, case):
         s.reset(mode=1)
         s.set_matrix_precision(32)
         s.set_convection_precision(32)
+        s.set_step1_elements(0)   # the CSR form, whose gathers the window replaces
         s.set_step1_window(win)
         s.run(3)
         out.append((s.download("Un"), s.download("Pn"), s.download("KAcc")))
