@@ -265,7 +265,7 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits) {
   if (!ctx) return FSX_EINVAL;
   if (bits != 32 && bits != 64) return fail(ctx, FSX_EINVAL, "fsx_set_matrix_precision: 32 or 64");
   ctx->mat_bits = bits;
-  ctx->gpn_valid = false;   // G Pn is formed again from the values now in use
+  ctx->gpn_valid = ctx->gpn_next_valid = false;   // G Pn is formed again from the values now in use
   return refresh_matrix_copies(ctx);
 }
 
@@ -426,7 +426,7 @@ int fsx_upload_mesh(fsx_ctx* ctx, int NE, int NN, int NNp, const int* LtoGnode, 
   // state vectors
   const size_t n3 = (size_t)3 * NN;
   for (double** v : {&ctx->Un, &ctx->Unp1, &ctx->Unp1_prev, &ctx->UnpHalf, &ctx->UnpHalf_prev, &ctx->Acc, &ctx->Acc_prev,
-                     &ctx->KAcc, &ctx->R1, &ctx->GPn})
+                     &ctx->KAcc, &ctx->R1, &ctx->GPn, &ctx->GPn_next})
     FSX_TRY(alloc(ctx, v, n3));
   for (double** v : {&ctx->Pn, &ctx->Pnp1, &ctx->Pnp1_prev, &ctx->Pdot, &ctx->R2, &ctx->wr, &ctx->wz, &ctx->wp, &ctx->wq,
                      &ctx->wh, &ctx->w0r, &ctx->w0e})
@@ -776,7 +776,7 @@ int fsx_assemble_static(fsx_ctx* ctx) {
     return fail(ctx, FSX_ESTATE, "fsx_assemble_static needs mesh, M and G patterns, BCs and parameters");
   FSX_TRY(launch_assemble_static(ctx));
   ctx->assembled = true;
-  ctx->gpn_valid = false;
+  ctx->gpn_valid = ctx->gpn_next_valid = false;
   FSX_TRY(refresh_matrix_copies(ctx));
   ctx->mass_dinv_ready = false;
   ctx->mass_m_uploaded = false;   // M is the elements' sum again
@@ -799,7 +799,7 @@ int fsx_init_state(fsx_ctx* ctx) {
   const long long n3 = 3LL * ctx->NN;
   ctx->kacc_pending = false;
   ctx->conv_pending = false;
-  ctx->gpn_valid = false;
+  ctx->gpn_valid = ctx->gpn_next_valid = false;
   drop_records(ctx);
   for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
                     ctx->KAcc, ctx->R1})
@@ -842,6 +842,7 @@ int fsx_step1(fsx_ctx* ctx, int iter) { FSX_TRY(need_step(ctx)); return launch_s
 int fsx_step2(fsx_ctx* ctx, int iter, int* pcg_iters, double* relres) {
   FSX_TRY(need_step(ctx));
   FSX_TRY(settle_deferred(ctx));
+  ctx->gpn_next_valid = false;   // Pnp1 changes
   FSX_TRY(launch_step2_rhs(ctx, iter));
   int it = 0;
   double rel = 0.0;
@@ -911,7 +912,10 @@ int fsx_end_step(fsx_ctx* ctx, double converge, int* steady, double* max_acc) {
   if (steady) *steady = (m > converge) ? 0 : 1;                            // BCH:3059-3061
   std::swap(ctx->Un, ctx->Unp1);                                           // BCH:3070-3077
   std::swap(ctx->Pn, ctx->Pnp1);
-  ctx->gpn_valid = false;                                                  // G Pn changes with Pn
+  // G Pn changes with Pn: the G Pnp1 that step3 carried becomes it
+  if (ctx->gpn_next_valid) std::swap(ctx->GPn, ctx->GPn_next);
+  ctx->gpn_valid = ctx->gpn_next_valid;
+  ctx->gpn_next_valid = false;
   ctx->ucmp_ok = ctx->ucmp_un = false;                                     // and Un's shadow with Un
   return FSX_OK;
 }
@@ -1031,7 +1035,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
     return fail(ctx, FSX_EINVAL, "fsx_upload_vec: only state vectors and K/M/Z values");
   FSX_TRY(settle_deferred(ctx));
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
-  ctx->gpn_valid = false;
+  ctx->gpn_valid = ctx->gpn_next_valid = false;
   drop_records(ctx);   // the FP32 shadows may no longer match the vectors
   if (id == FSX_VAL_Z) ctx->band_ready = false;
   FSX_HIP(hipStreamSynchronize(ctx->stream));

@@ -159,6 +159,8 @@ struct fsx_ctx {
   bool kacc_pending = false;     // fsx_kacc deferred to the next step1 (settle_deferred)
   bool gpn_valid = false;        // GPn holds G Pn for the current Pn
   double* GPn = nullptr;         // 3NN: G Pn, node-major, kept over the inner iterations of a step
+  double* GPn_next = nullptr;    // 3NN: G Pnp1 from step3 (G Pn + dt G Pdot), the next step's G Pn
+  bool gpn_next_valid = false;   // GPn_next holds G Pnp1 for the current Pnp1
   // velocity mass (fsx_set_mass): lumped (the reference) or consistent, M_bc
   // solved by the 3-right-hand-side Jacobi-PCG of fsx_mass.hip
   int mass_mode = FSX_MASS_LUMPED;

@@ -528,7 +528,8 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
                                                   const unsigned char* __restrict__ bc, const double* __restrict__ MdInv,
                                                   const double* __restrict__ UnpHalf, double* __restrict__ Acc,
                                                   double* __restrict__ Unp1, double dt, double* __restrict__ Rout,
-                                                  float* __restrict__ Aout) {
+                                                  float* __restrict__ Aout, const double* __restrict__ GPn,
+                                                  double* __restrict__ GPnext) {
   int l;
   const long long r = my_row<LPR>(&l);
   double g0 = 0, g1 = 0, g2 = 0;
@@ -557,6 +558,9 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
   if (r < NN && l < 3) {
     const double gg = l == 0 ? g0 : (l == 1 ? g1 : g2);
     const size_t i = 3 * (size_t)r + l;
+    // G Pnp1 = G Pn + dt G Pdot (Pnp1 = Pn + dt Pdot, k_pnp1): the next
+    // time step's G Pn when this is the step's last inner iteration
+    if (GPnext) GPnext[i] = GPn[i] + dt * gg;
     double R = -dt * gg;
     if (use_kacc) R = R - dt * KAcc[i];
     if (bc[r]) R = 0.0;
@@ -574,17 +578,24 @@ static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
   constexpr int LPR = 8;
   // Acc's FP32 shadow, kept where step1 reads it (lumped mass, FP32 matrices)
   float* aout = (ctx->Gf && ctx->rec_mem && !Rout) ? ctx->Arec : nullptr;
+  // with the element form of step1 (FP32 G, lumped mass), G Pnp1 is carried
+  // to the next time step from the G Pdot formed here instead of a pass over
+  // G at its first step1
+  const bool carry = ctx->Gf && !Rout && ctx->gpn_valid && ctx->use_step1_elements && ctx->GPn_next;
+  const double* gpn = carry ? ctx->GPn : nullptr;
+  double* gnext = carry ? ctx->GPn_next : nullptr;
   // FP32 G values: 4 lanes x 1 load a row, 225.7 us at 64^3 (4x2 237.5, 8x1
   // 245.9, 8x2 252.6, 4x3 254.6, 8x4 261.7, 16x1 347.0)
   if (ctx->Gf)
     hipLaunchKernelGGL((k_step3<4, 1, float>), dim3(rows_grid(ctx->NN, 4)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                        ctx->Grow, ctx->Gcol, ctx->Gf, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
-                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout);
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout, gpn, gnext);
   else
     hipLaunchKernelGGL((k_step3<LPR, 2, double>), dim3(rows_grid(ctx->NN, LPR)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                        ctx->Grow, ctx->Gcol, ctx->G, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
-                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout);
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout, nullptr, nullptr);
   ctx->arec_ok = aout != nullptr;
+  ctx->gpn_next_valid = carry;
 }
 
 int launch_step3(fsx_ctx* ctx, int iter) {

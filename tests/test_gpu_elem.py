@@ -165,3 +165,30 @@ def test_step1_elements_deferral(fsx_lib):
     assert np.array_equal(r_up, r_csr) and np.array_equal(u_up, u_csr)
     s.close()
     mesh.close()
+
+
+@pytest.mark.parametrize("case", ["cavity10", "duct"])
+def test_carried_gpn_equals_formed(fsx_lib, case):
+    """With the element form, step3 carries G Pnp1 = G Pn + dt G Pdot into the
+    next time step instead of a pass over G at its first step1 (k_pnp1 forms
+    Pnp1 = Pn + dt Pdot). After 2 steps, the next step's first R1 with the
+    carried G Pn and with G Pn formed afresh (the matrix copies refreshed,
+    which drops the carried value) agree to FP64 rounding (bar 1e-13
+    normwise)."""
+    s, mesh = make(fsx_lib, case)
+    out = []
+    for fresh in (False, True):
+        s.reset(mode=0)
+        forms(s, 32, 1)
+        s.run(2)
+        if fresh:
+            s.set_matrix_precision(32)
+        s.begin_step()
+        s.convection()
+        s.step1(1)
+        out.append(s.download("R1"))
+    e = rel(out[0], out[1])
+    say(f"  {case}: carried vs formed G Pn, R1 {e:.2e}")
+    assert e < 1e-13
+    s.close()
+    mesh.close()
