@@ -50,6 +50,7 @@ sys.path.insert(0, os.path.join(ROOT, "cfd-with-cuda_amd"))
 METRIC = "timesteps/s, 64^3 hex27 cavity: learned+mixed vs plain FP64 PCG at 1e-8 residual"
 TRAJECTORY_STEPS = 100   # the 64^3 configuration's steps
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK = 157.3   # TFLOP/s, dense
 
 
 def step_kernels(vt):
@@ -62,6 +63,18 @@ def step_kernels(vt):
                 "k_step3<4, 1, float>")
     return ("k_step1<8, 2, 2, true, true, 2, double, false>", "k_step2_rhs<16, 8, double, false>",
             "k_step3<8, 2, double>")
+
+
+ELEM_KERNELS = ("k_elem_step1<true>", "k_step1_nodes<true, 2>")
+
+
+def elem_flops(ne, ngp):
+    """Useful FP32 flops of one k_elem_step1<true> launch: 2 x the multiply-adds
+    of its contractions at the real Gauss points and nodes, per element and
+    point: field set 0 (Jacobian 8x9, reference gradients 27x9, u0 27x3, K back
+    27x9, convection back 27x3) and set 1 (Jacobian, gradients, K back)."""
+    macs = (72 + 243 + 81 + 243 + 81) + (72 + 243 + 243)
+    return 2.0 * macs * ngp * ne
 
 
 def zspmv_name(prec, npat, ncls):
@@ -289,7 +302,8 @@ def headline(args):
         solver.set_convection_precision(32)
         solver.run(1)
         solver.sync()
-        for kid in (fsx.KERNEL_STEP1, fsx.KERNEL_KACC, fsx.KERNEL_STEP2_RHS, fsx.KERNEL_STEP3):
+        for kid in (fsx.KERNEL_STEP1, fsx.KERNEL_KACC, fsx.KERNEL_STEP2_RHS, fsx.KERNEL_STEP3,
+                    fsx.KERNEL_STEP1_ELEMENTS, fsx.KERNEL_STEP1_NODES):
             solver.bench_kernel(kid, reps=3)
         solver.set_matrix_precision(64)
         solver.set_convection_precision(64)
@@ -342,11 +356,17 @@ def headline(args):
     solver.set_matrix_precision(32 if prec else 64)   # the headline mode's matrices
     knames = step_kernels("float" if prec else "double")
     kern = {}
-    for kid, kname, per_step in ((fsx.KERNEL_STEP1, knames[0], steps_inner),
-                                 (fsx.KERNEL_STEP2_RHS, knames[1], steps_inner),
-                                 (fsx.KERNEL_STEP3, knames[2], steps_inner)):
+    # step1 of the mixed forms runs element by element (its K Acc_prev form in
+    # inner iterations >= 2): its two passes instead of the CSR k_step1
+    step1 = ((fsx.KERNEL_STEP1_ELEMENTS, ELEM_KERNELS[0], steps_inner - 1),
+             (fsx.KERNEL_STEP1_NODES, ELEM_KERNELS[1], steps_inner - 1)) if prec else \
+        ((fsx.KERNEL_STEP1, knames[0], steps_inner),)
+    for kid, kname, per_step in step1 + ((fsx.KERNEL_STEP2_RHS, knames[1], steps_inner),
+                                         (fsx.KERNEL_STEP3, knames[2], steps_inner)):
         ms_k, by_k = solver.bench_kernel(kid, reps=20)
         kern[kname] = dict(launch_ms=ms_k, bytes=by_k, ms_per_step=ms_k * per_step)
+    if prec:
+        kern[ELEM_KERNELS[0]]["flops"] = elem_flops(solver.sizes["NE"], args.ngp)
     ms64, bytes64 = solver.bench_spmv(precision=0, reps=100)
     npat, _ = solver.z_patterns()
     _, ncls, _ = solver.z_forms()
@@ -354,7 +374,19 @@ def headline(args):
         kern[zspmv_name(0, npat, ncls)] = dict(launch_ms=ms64, bytes=bytes64,
                                                ms_per_step=0.0)   # not a kernel of the headline mode
     kname = max(kern, key=lambda k: kern[k]["ms_per_step"])
-    achieved = kern[kname]["bytes"] / (kern[kname]["launch_ms"] * 1e-3) / 1e9
+    kd = kern[kname]
+    if "flops" in kd:   # the element pass is bound by the matrix cores (FP32 MFMA)
+        roof = {"bound": "mfma", "kernel": kname,
+                "achieved": round(kd["flops"] / (kd["launch_ms"] * 1e-3) / 1e12, 2), "peak": FP32_MFMA_PEAK,
+                "unit": "TFLOP/s", "flops_per_launch": kd["flops"]}
+    else:
+        roof = {"bound": "hbm", "kernel": kname,
+                "achieved": round(kd["bytes"] / (kd["launch_ms"] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s"}
+    roof["frac"] = round(roof["achieved"] / roof["peak"], 4)
+    roof["traffic"] = measured_traffic(kname)
+    roof["algorithmic_bytes_per_launch"] = kd["bytes"]
+    roof["launch_ms"] = round(kd["launch_ms"], 5)
     out = {
         "metric": METRIC,
         "value": round(h["steps_per_s"], 4),
@@ -375,14 +407,14 @@ def headline(args):
                                f"{args.warmup + 1}-{args.warmup + args.steps} of {args.trajectory}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": kname,
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
-                     "algorithmic_bytes_per_launch": kern[kname]["bytes"],
-                     "launch_ms": round(kern[kname]["launch_ms"], 5)},
+        "roofline": roof,
         "kernels": {k: {"launch_ms": round(v["launch_ms"], 5), "algorithmic_bytes": v["bytes"],
                         "achieved_GBs": round(v["bytes"] / (v["launch_ms"] * 1e-3) / 1e9, 1),
-                        "ms_per_step": round(v["ms_per_step"], 4), "traffic": measured_traffic(k)}
+                        "ms_per_step": round(v["ms_per_step"], 4), "traffic": measured_traffic(k),
+                        **({"flops": v["flops"],
+                            "achieved_TFLOPs": round(v["flops"] / (v["launch_ms"] * 1e-3) / 1e12, 2),
+                            "frac_fp32_mfma": round(v["flops"] / (v["launch_ms"] * 1e-3) / 1e12 / FP32_MFMA_PEAK, 4)}
+                           if "flops" in v else {})}
                     for k, v in kern.items()},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 4), "pcg_iters": v["pcg_iters"],
                       "inner_iters": v["inner_iters"], "solve_ms": round(v["solve_ms"], 2),

@@ -462,8 +462,10 @@ bool step1_elements_ok(const fsx_ctx* ctx) {
          27LL * 6 * ctx->NE < (1LL << 32) && 2LL * ctx->NN < (1LL << 32);   // the kernel's 32-bit offsets
 }
 
-int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload) {
-  const int NE = ctx->NE, NN = ctx->NN;
+// The element pass: k_elem_step1 writes every element's rows of the
+// convection plus K UnpHalf_prev (and K Acc_prev with kacc) into ek_ye.
+int launch_elem_pass(fsx_ctx* ctx, bool kacc) {
+  const int NE = ctx->NE;
   if (!ctx->el_ready) FSX_TRY(elements_setup(ctx));
   if (!ctx->ek_ye) FSX_TRY(alloc(ctx, &ctx->ek_ye, (size_t)NE * 27 * 6));
   if (!ctx->ek_xl_ready) {
@@ -491,7 +493,14 @@ int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload) {
   else
     hipLaunchKernelGGL(k_elem_step1<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, NE, ctx->NGP, ctx->ek_ntiles,
                        nb, ctx->el_L27, ctx->ek_xl, ctx->ek_tab, nu, ucmp, urp, arp, ctx->ek_ye, ys);
-  FSX_TRY(launch_check(ctx, "k_elem_step1"));
+  return launch_check(ctx, "k_elem_step1");
+}
+
+// The node pass: k_step1_nodes sums each node's element rows from ek_ye and
+// applies step1's update (gload: G Pn kept from an earlier pass, else formed).
+int launch_nodes_pass(fsx_ctx* ctx, bool kacc, bool gload) {
+  const int NN = ctx->NN;
+  const int ys = kacc ? 6 : 3;
   const double o2 = 1.0000000000000000 / (ctx->dt * ctx->dt);
   const dim3 ng1((unsigned)((NN + kBlock - 1) / kBlock)), ng8((unsigned)((NN + kBlock / 8 - 1) / (kBlock / 8)));
 #define NODES_LAUNCH(KA, GM)                                                                                          \
@@ -506,6 +515,11 @@ int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload) {
   }
 #undef NODES_LAUNCH
   return launch_check(ctx, "k_step1_nodes");
+}
+
+int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload) {
+  FSX_TRY(launch_elem_pass(ctx, kacc));
+  return launch_nodes_pass(ctx, kacc, gload);
 }
 
 }  // namespace fsx

@@ -451,6 +451,8 @@ int launch_convection_now(fsx_ctx* ctx);
 int elem_tables_setup(fsx_ctx* ctx, const double* tables, int NGP);
 bool step1_elements_ok(const fsx_ctx* ctx);
 int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload);
+int launch_elem_pass(fsx_ctx* ctx, bool kacc);
+int launch_nodes_pass(fsx_ctx* ctx, bool kacc, bool gload);
 int refresh_matrix_copies(fsx_ctx* ctx);
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);
 int elements_setup(fsx_ctx* ctx);

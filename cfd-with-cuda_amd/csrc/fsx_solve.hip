@@ -1501,8 +1501,27 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
   using namespace fsx;
   if (!ctx || reps < 1 || !ms_per_launch) return FSX_EINVAL;
   if (!ctx->K || !ctx->G || !ctx->GT) return fail(ctx, FSX_ESTATE, "fsx_bench_kernel: assemble first");
-  if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP3) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
-  const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG;
+  if (kernel < FSX_KERNEL_STEP1 || kernel > FSX_KERNEL_STEP1_NODES) return fail(ctx, FSX_EINVAL, "fsx_bench_kernel: kernel id");
+  const double NN = ctx->NN, NNp = ctx->NNp, nK = (double)ctx->nnzM, nG = (double)ctx->nnzG, NE = ctx->NE;
+  const bool elems = kernel == FSX_KERNEL_STEP1_ELEMENTS || kernel == FSX_KERNEL_STEP1_NODES;
+  if (elems) {
+    // the FP32 shadows the element form reads, as an inner iteration >= 2 has
+    // them: UnpHalf_prev, Acc_prev and Un
+    if (ctx->rec_mem) {
+      hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN,
+                         ctx->UnpHalf_prev, reinterpret_cast<float4*>(ctx->Urec_prev), 2);
+      hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Acc_prev,
+                         reinterpret_cast<float4*>(ctx->Arec_prev), 2);
+      hipLaunchKernelGGL(k_rec, dim3(ew_grid(ctx->NN)), dim3(kBlock), 0, ctx->stream, (long long)ctx->NN, ctx->Un,
+                         reinterpret_cast<float4*>(ctx->Ucmp), 1);
+      FSX_TRY(launch_check(ctx, "k_rec"));
+      ctx->urec_prev_ok = ctx->arec_prev_ok = ctx->ucmp_un = true;
+    }
+    ctx->kacc_pending = true;
+    if (!step1_elements_ok(ctx))
+      return fail(ctx, FSX_ESTATE, "fsx_bench_kernel: the element form of step1 needs FP32 matrices and convection, "
+                                   "lumped mass and the assembled K");
+  }
   const double vb = ctx->Kf ? 4.0 : 8.0;   // bytes per stored matrix value
   if (kernel == FSX_KERNEL_STEP1 && ctx->Kf && ctx->rec_mem && ctx->mass_mode == FSX_MASS_LUMPED) {
     // the FP32 shadows of UnpHalf_prev and Acc_prev, as the inner iterations have them
@@ -1520,6 +1539,12 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
       case FSX_KERNEL_STEP1: ctx->kacc_pending = true; ctx->gpn_valid = true; (void)launch_step1(ctx, 2); break;
       case FSX_KERNEL_KACC: (void)launch_kacc(ctx); break;
       case FSX_KERNEL_STEP2_RHS: (void)launch_step2_rhs(ctx, 2); break;
+      case FSX_KERNEL_STEP1_ELEMENTS: (void)launch_elem_pass(ctx, true); break;
+      case FSX_KERNEL_STEP1_NODES:
+        ctx->gpn_valid = true;
+        (void)launch_nodes_pass(ctx, true, true);
+        ctx->urec_ok = ctx->drec_ok = ctx->drec_kacc = true;   // as step1 leaves them for step2
+        break;
       default: launch_step3_rows(ctx, 2); break;
     }
   };
@@ -1546,12 +1571,24 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
       else
         bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
       break;
+    case FSX_KERNEL_STEP1_ELEMENTS:   // per element its 27 node ids, corner offsets (24 floats) and rows
+                                      // (27 x 6 floats); per node the FP32 records of Un (16 bytes) and of
+                                      // UnpHalf_prev with Acc_prev (32)
+      bytes = (27.0 * 4 + 24.0 * 4 + 27.0 * 6 * 4) * NE + 48.0 * NN;
+      break;
+    case FSX_KERNEL_STEP1_NODES:      // the incidence (pointer, 27 entries an element), the element rows;
+                                      // G Pn, Un, MdInv, bc, MdOrigInv read; R1, UnpHalf, KAcc written, the
+                                      // shadows of UnpHalf and of step2's operand (12 + 12 bytes)
+      bytes = 4.0 * (NN + 1) + (27.0 * 4 + 27.0 * 6 * 4) * NE + (24.0 + 24.0 + 8.0 + 1.0 + 8.0) * NN +
+              (24.0 + 24.0 + 24.0 + 12.0 + 12.0) * NN;
+      break;
     default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc (+ its FP32 shadow), Unp1
       bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +
               24.0 * NN + (ctx->Gf && ctx->rec_mem ? 12.0 * NN : 0.0);
       break;
   }
   FSX_TRY(settle_deferred(ctx));
+  if (kernel == FSX_KERNEL_STEP1_NODES) FSX_TRY(launch_elem_pass(ctx, true));   // the rows the node pass sums
   for (int w = 0; w < 2; w++) launch();
   FSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int r = 0; r < reps; r++) launch();

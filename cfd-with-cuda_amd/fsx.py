@@ -23,7 +23,7 @@ _vp = ctypes.c_void_p
 FSX_OK, FSX_EINVAL, FSX_EHIP, FSX_ESTATE, FSX_ENOCONV, FSX_ENOMEM, FSX_ENUMERIC = range(7)
 CSR_M, CSR_G, CSR_Z = 0, 1, 2
 SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_DIRECT = 0, 1, 2, 3
-KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3 = 0, 1, 2, 3
+KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3, KERNEL_STEP1_ELEMENTS, KERNEL_STEP1_NODES = 0, 1, 2, 3, 4, 5
 MASS_LUMPED, MASS_CONSISTENT, MASS_CONSISTENT_CSR = 0, 1, 2
 X0_ZERO, X0_PREVIOUS = 0, 1
 
@@ -492,7 +492,9 @@ class Solver:
 
     def bench_kernel(self, kernel, reps=20):
         """(ms per launch, algorithmic bytes per launch) of one step row kernel
-        (KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3); clobbers
+        (KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3, and the two
+        passes of step1's element form, KERNEL_STEP1_ELEMENTS and
+        KERNEL_STEP1_NODES); clobbers
         the step vectors (reset() before stepping again)."""
         ms = ctypes.c_double()
         by = ctypes.c_double()

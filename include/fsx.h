@@ -332,7 +332,14 @@ enum fsx_kernel_id {
   FSX_KERNEL_STEP1 = 0,      /* K 3-RHS + G 3-block SpMV with the step1 update (step1GPUpart, CU:946-973) */
   FSX_KERNEL_KACC = 1,       /* K 3-RHS SpMV (calculate_KtimesAcc_prevGPU, CU:1067-1077) */
   FSX_KERNEL_STEP2_RHS = 2,  /* G^T 3-block SpMV forming R2 (step2GPU, CU:980-1011) */
-  FSX_KERNEL_STEP3 = 3       /* G 3-block SpMV with the step3 update (step3GPU, CU:1029-1060) */
+  FSX_KERNEL_STEP3 = 3,      /* G 3-block SpMV with the step3 update (step3GPU, CU:1029-1060) */
+  /* step1 of the mixed forms element by element (fsx_set_step1_elements):
+   * its element pass (convection, K UnpHalf_prev and K Acc_prev of every
+   * element on the matrix cores) and its node pass (the element rows summed
+   * per node, step1's update); replace step1GPUpart's K and G products,
+   * CU:946-973 */
+  FSX_KERNEL_STEP1_ELEMENTS = 4,
+  FSX_KERNEL_STEP1_NODES = 5
 };
 /* Times `reps` back-to-back launches of one of those kernels (inner
  * iteration >= 2 form, with KAcc) on the context's stream with HIP events:
