@@ -22,7 +22,9 @@ the configuration, so that `trajectory` reports all 100 steps as well:
            two-level preconditioner (Jacobi smoothing around the trained
            network), FP64 reliable updates and stop test, Jacobi fallback;
   learned_untrained  the same network at its initial (multilevel) weights;
-  direct   band Cholesky of Z factored once (the MATLAB prototype's step).
+  direct   sparse Cholesky of Z factored once (the MATLAB prototype's step):
+           nested-dissection order, multifrontal factor;
+  direct_band  (not run by default) the same with Z factored as a band.
 The network is trained before any timing on the pressure solves of seeded
 cavity runs (lid speed U[0.5, 2], Re U[100, 1000]) on the same mesh, never on
 the timed trajectory; its weights are saved to a file and loaded back for the
@@ -317,7 +319,7 @@ def headline(args):
         solver.close()
         return
     modes = {"fp64": fsx.SOLVER_FP64, "mixed": fsx.SOLVER_MIXED, "learned": fsx.SOLVER_LEARNED,
-             "learned_untrained": fsx.SOLVER_LEARNED, "direct": fsx.SOLVER_DIRECT}
+             "learned_untrained": fsx.SOLVER_LEARNED, "direct": fsx.SOLVER_DIRECT, "direct_band": fsx.SOLVER_DIRECT}
     names = args.modes.split(",")
     res = {}
     train = None
@@ -331,15 +333,23 @@ def headline(args):
             solver.unet_set_weights(w_trained)
         elif name == "learned_untrained":
             solver.unet_set_weights(w_init)
+        if name in ("direct", "direct_band"):
+            solver.set_direct_order(fsx.DIRECT_ND if name == "direct" else fsx.DIRECT_BAND)
         log(f"mode {name}: {args.warmup} warm-up + {args.steps} timed steps, trajectory of {args.trajectory}")
         res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit,
-                             matrix_bits=64 if name in ("fp64", "direct") else 32, x0=args.x0,
+                             matrix_bits=64 if name in ("fp64", "direct", "direct_band") else 32, x0=args.x0,
                              total=max(args.trajectory, args.warmup + args.steps))
         r = res[name]
-        if name == "direct":   # the factor is formed in the first warm-up step
+        if name in ("direct", "direct_band"):   # the factor is formed in the first warm-up step
             w, fms, fbytes = solver.direct_stats()
-            r["extra"] = {"half_bandwidth": w, "factor_ms": round(fms, 1), "band_GB": round(fbytes / 1e9, 2)}
-            log(f"  direct: {r['extra']}")
+            if name == "direct":
+                _, ffl, nfr, depth = solver.direct_info()
+                r["extra"] = {"order": "nested dissection", "factor_ms": round(fms, 1),
+                              "factor_GB": round(fbytes / 1e9, 3), "factor_flops": ffl, "fronts": nfr, "depth": depth}
+            else:
+                r["extra"] = {"order": "band", "half_bandwidth": w, "factor_ms": round(fms, 1),
+                              "band_GB": round(fbytes / 1e9, 2)}
+            log(f"  {name}: {r['extra']}")
         log(f"  timed {r['steps_per_s']:.3f} steps/s, PCG iters {r['pcg_iters']}, solve {r['solve_ms']:.1f} ms; "
             f"trajectory {r['traj']['steps_per_s']:.3f} steps/s, solve {r['traj']['solve_ms']:.1f} ms, "
             f"iters {r['traj']['pcg_iters']}, max relres {r['traj']['max_relres']:.2e}")
@@ -469,7 +479,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--modes", default="fp64,mixed,learned,learned_untrained,direct",
-                    help="solver modes to run, of fp64, mixed, learned, learned_untrained and direct (none: only the "
+                    help="solver modes to run, of fp64, mixed, learned, learned_untrained, direct and direct_band (none: only the "
                          "--workloads)")
     ap.add_argument("--train-iters", type=int, default=800, help="Adam steps of each training round")
     ap.add_argument("--train-rounds", type=int, default=2,

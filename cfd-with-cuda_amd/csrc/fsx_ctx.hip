@@ -213,6 +213,7 @@ void fsx_destroy(fsx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   fsx::unet_free(ctx);
+  fsx::mf_free(ctx);
   for (fsx::IterGraph* g : {&ctx->graph_fp64, &ctx->graph_mixed, &ctx->graph_learned}) fsx::iter_graph_reset(*g);
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
@@ -790,6 +791,7 @@ int fsx_build_Z(fsx_ctx* ctx) {
   FSX_TRY(build_z_forms(ctx));
   ctx->z_built = true;
   ctx->band_ready = false;   // a direct solve factors the new Z
+  fsx::mf_invalidate(ctx);
   return FSX_OK;
 }
 
@@ -1037,7 +1039,10 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   if (count != n) return fail(ctx, FSX_EINVAL, "fsx_upload_vec: count mismatch");
   ctx->gpn_valid = ctx->gpn_next_valid = false;
   drop_records(ctx);   // the FP32 shadows may no longer match the vectors
-  if (id == FSX_VAL_Z) ctx->band_ready = false;
+  if (id == FSX_VAL_Z) {
+    ctx->band_ready = false;
+    mf_invalidate(ctx);
+  }
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   if (kind == 3) {
     const size_t NN = ctx->NN;

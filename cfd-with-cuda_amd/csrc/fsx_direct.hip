@@ -451,8 +451,28 @@ int direct_apply(fsx_ctx* ctx, const double* b, double* x) {
 
 extern "C" int fsx_direct_stats(fsx_ctx* ctx, int* half_bandwidth, double* factor_ms, double* band_bytes) {
   if (!ctx) return FSX_EINVAL;
+  if (ctx->direct_order == FSX_DIRECT_ND) {
+    double lb = 0.0, ms = 0.0;
+    fsx::mf_info(ctx, &lb, nullptr, nullptr, nullptr, &ms);
+    if (half_bandwidth) *half_bandwidth = -1;
+    if (factor_ms) *factor_ms = ms;
+    if (band_bytes) *band_bytes = lb;   // L's front columns
+    return FSX_OK;
+  }
   if (half_bandwidth) *half_bandwidth = ctx->band_ready ? ctx->band_w : -1;
   if (factor_ms) *factor_ms = ctx->band_ready ? ctx->band_factor_ms : 0.0;
   if (band_bytes) *band_bytes = ctx->band_ready ? 8.0 * (double)ctx->band_n * (double)ctx->band_W : 0.0;   // L's band
   return FSX_OK;
+}
+
+extern "C" int fsx_set_direct_order(fsx_ctx* ctx, int order) {
+  if (!ctx) return FSX_EINVAL;
+  if (order != FSX_DIRECT_BAND && order != FSX_DIRECT_ND) return fsx::fail(ctx, FSX_EINVAL, "fsx_set_direct_order: unknown order");
+  ctx->direct_order = order;
+  return FSX_OK;
+}
+
+extern "C" int fsx_direct_info(fsx_ctx* ctx, double* factor_bytes, double* factor_flops, int* fronts, int* depth) {
+  if (!ctx) return FSX_EINVAL;
+  return fsx::mf_info(ctx, factor_bytes, factor_flops, fronts, depth, nullptr);
 }

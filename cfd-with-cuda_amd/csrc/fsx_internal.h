@@ -12,6 +12,7 @@
 #include "../../include/fsx.h"
 
 namespace fsx {
+struct MfState;   // fsx_sparse.hip
 
 constexpr int kWave = 64;         // CDNA wavefront width
 constexpr int kBlock = 256;       // default block size for row kernels
@@ -376,6 +377,9 @@ struct fsx_ctx {
   double last_relres = 0.0;
   // learned preconditioner
   struct fsx_unet* unet = nullptr;
+  // multifrontal direct solve in a nested-dissection order (fsx_sparse.hip)
+  fsx::MfState* mf = nullptr;
+  int direct_order = FSX_DIRECT_ND;
   int learned_maxit = 400;       // flexible-CG iterations before the Jacobi fallback
   // preconditioner form (fsx_unet_form): 1 the two-level multiplicative form
   // on the LDS tiles (default where the tile plan exists), 0 the additive form
@@ -452,6 +456,11 @@ int elem_tables_setup(fsx_ctx* ctx, const double* tables, int NGP);
 bool step1_elements_ok(const fsx_ctx* ctx);
 int launch_step1_elements(fsx_ctx* ctx, bool kacc, bool gload);
 int launch_elem_pass(fsx_ctx* ctx, bool kacc);
+int mf_factor(fsx_ctx* ctx);
+int mf_apply(fsx_ctx* ctx, const double* b, double* x);
+void mf_free(fsx_ctx* ctx);
+void mf_invalidate(fsx_ctx* ctx);
+int mf_info(const fsx_ctx* ctx, double* l_bytes, double* flops, int* fronts, int* depth, double* factor_ms);
 int launch_nodes_pass(fsx_ctx* ctx, bool kacc, bool gload);
 int refresh_matrix_copies(fsx_ctx* ctx);
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);

@@ -1349,8 +1349,13 @@ int launch_warm_residual(fsx_ctx* ctx, const double* x, const double* b, double*
 // residual |b - Z x| / |b|, which must meet rtol like the iterative solvers'
 static int solve_direct(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   const int n = ctx->NNp;
-  FSX_TRY(direct_factor(ctx));
-  FSX_TRY(direct_apply(ctx, b, x));
+  if (ctx->direct_order == FSX_DIRECT_ND) {
+    FSX_TRY(mf_factor(ctx));
+    FSX_TRY(mf_apply(ctx, b, x));
+  } else {
+    FSX_TRY(direct_factor(ctx));
+    FSX_TRY(direct_apply(ctx, b, x));
+  }
   launch_zresidual(ctx, x, b, ctx->w0r);
   hipLaunchKernelGGL(k_sumsq2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->w0r, ctx->partials);
   FSX_TRY(launch_check(ctx, "k_sumsq2"));

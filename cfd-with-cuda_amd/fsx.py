@@ -26,6 +26,7 @@ SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_DIRECT = 0, 1, 2, 3
 KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3, KERNEL_STEP1_ELEMENTS, KERNEL_STEP1_NODES = 0, 1, 2, 3, 4, 5
 MASS_LUMPED, MASS_CONSISTENT, MASS_CONSISTENT_CSR = 0, 1, 2
 X0_ZERO, X0_PREVIOUS = 0, 1
+DIRECT_BAND, DIRECT_ND = 0, 1
 
 VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, R1=7, Pn=8, Pnp1=9,
            Pnp1_prev=10, Pdot=11, R2=12, MdInv=13, MdOrigInv=14, K=15, M=16, G1=17, G2=18, G3=19, Z=20)
@@ -37,7 +38,7 @@ DEVICE_SYMBOLS = (
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
     "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
-    "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
+    "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_set_direct_order", "fsx_direct_info", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
     "fsx_unet_train", "fsx_unet_limit", "fsx_unet_stats", "fsx_unet_form", "fsx_unet_form_get", "fsx_unet_precondition",
@@ -48,6 +49,7 @@ HOST_SYMBOLS = (
     "fsxh_topology", "fsxh_topo_free", "fsxh_topo_info", "fsxh_topo_len", "fsxh_topo_get",
     "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_reset", "fsxd_ctx", "fsxd_topo", "fsxd_error",
     "fsxd_setup_times", "fsxh_write_tecplot", "fsxh_read_restart", "fsxd_write_tecplot", "fsxd_read_restart",
+    "fsxh_nd_order",
 )
 
 
@@ -175,6 +177,9 @@ def lib():
         "fsxh_read_restart": ([_vp, ctypes.c_char_p, _D, _D], ctypes.c_int),
         "fsxd_write_tecplot": ([_vp, ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
         "fsxd_read_restart": ([_vp, ctypes.c_char_p], ctypes.c_int),
+        "fsxh_nd_order": ([ctypes.c_int, _I, _I, _D, ctypes.c_int, _I, _D], ctypes.c_int),
+        "fsx_set_direct_order": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_direct_info": ([_vp, _D, _D, _I, _I], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -195,6 +200,21 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(_I)
+
+
+def nd_order(rp, ci, xyz, leaf=64):
+    """The direct solve's nested-dissection order of a symmetric pattern
+    (fsxh_nd_order): (perm new -> old, dict of nnz_l, flops, fronts, depth)."""
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = rp.size - 1
+    perm = np.zeros(n, np.int32)
+    st = np.zeros(4)
+    rc = lib().fsxh_nd_order(n, _ip(rp), _ip(ci), _dp(xyz), int(leaf), _ip(perm), _dp(st))
+    if rc != FSX_OK:
+        raise FsxError(rc, "fsxh_nd_order: inconsistent pattern or coordinates")
+    return perm, dict(nnz_l=st[0], flops=st[1], fronts=int(st[2]), depth=int(st[3]))
 
 
 class FsxError(RuntimeError):
@@ -469,6 +489,22 @@ class Solver:
         by = ctypes.c_double()
         self.check(self.L.fsx_direct_stats(self.ctx, ctypes.byref(w), ctypes.byref(ms), ctypes.byref(by)), "direct_stats")
         return w.value, ms.value, by.value
+
+    def set_direct_order(self, order):
+        """DIRECT_ND (the default: nested dissection, multifrontal factor) or
+        DIRECT_BAND (band Cholesky) for the direct pressure solve."""
+        self.check(self.L.fsx_set_direct_order(self.ctx, order), "set_direct_order")
+
+    def direct_info(self):
+        """(factor bytes, factor flops, fronts, depth) of the nested-dissection
+        factor; depth -1 before one is formed."""
+        by = ctypes.c_double()
+        fl = ctypes.c_double()
+        nf = ctypes.c_int()
+        dp = ctypes.c_int()
+        self.check(self.L.fsx_direct_info(self.ctx, ctypes.byref(by), ctypes.byref(fl), ctypes.byref(nf),
+                                          ctypes.byref(dp)), "direct_info")
+        return by.value, fl.value, nf.value, dp.value
 
     def set_convection_precision(self, bits):
         """64 (the reference) or 32: the convection residual's element

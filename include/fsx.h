@@ -194,13 +194,31 @@ int fsx_set_convection_form(fsx_ctx* ctx, int elements);
  * solve's |b - Z x| <= rtol |b|; an x0 already within it takes no
  * iteration. */
 int fsx_set_solver_x0(fsx_ctx* ctx, int policy);
-/* FSX_SOLVER_DIRECT's factor: half-bandwidth of Z in the band order used
- * (-1 before the first direct solve), wall time of the factorization (ms)
- * and the bytes of the band. The factor is formed at the first direct solve
- * after Z changes (ML:1876-1885 factors once before the time loop); Z's band
- * must fit in 64 GiB (FSX_ENOMEM otherwise). No counterpart in the
- * reference's C++/CUDA code. */
+/* Order and factor form of FSX_SOLVER_DIRECT (fsx_set_direct_order) */
+enum fsx_direct_order {
+  FSX_DIRECT_BAND = 0,  /* Z as a band in the narrower of the device's numbering and reverse
+                           Cuthill-McKee, factored in place (no fill outside the band) */
+  FSX_DIRECT_ND = 1     /* the default: a fill-reducing nested-dissection order (the role of the
+                           prototype's symamd, ML:1880-1881) and a multifrontal factor, one dense
+                           front per separator, the fronts of a tree level factored together */
+};
+/* Selects the direct solve's order and factor (FSX_DIRECT_ND by default);
+ * a change takes effect at the next direct solve, which factors Z in the
+ * new form. No counterpart in the reference's C++/CUDA code. */
+int fsx_set_direct_order(fsx_ctx* ctx, int order);
+/* FSX_SOLVER_DIRECT's factor: half-bandwidth of Z in the band order (-1
+ * unless the band factor is the current one), wall time of the current
+ * factorization (ms) and the bytes of its factor (the band, or L's columns
+ * of every front). The factor is formed at the first direct solve after Z
+ * changes (ML:1876-1885 factors once before the time loop); the band must
+ * fit in 64 GiB and the fronts in the free device memory (FSX_ENOMEM
+ * otherwise). No counterpart in the reference's C++/CUDA code. */
 int fsx_direct_stats(fsx_ctx* ctx, int* half_bandwidth, double* factor_ms, double* band_bytes);
+/* The nested-dissection factor's shape (zeros and depth -1 before one is
+ * formed): bytes of L's front columns, factor flops (sum of squared column
+ * counts, as fsxh_nd_order reports them), number of fronts and depth of the
+ * separator tree. No reference counterpart. */
+int fsx_direct_info(fsx_ctx* ctx, double* factor_bytes, double* factor_flops, int* fronts, int* depth);
 /* enqueue solver iterations as replayed HIP graphs (1) or as individual
  * launches (0, the default: on the 64^3 cavity both run within 5% of each
  * other); results are identical. No reference counterpart (the CUSP and MKL

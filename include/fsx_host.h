@@ -131,6 +131,20 @@ int fsxd_write_tecplot(fsxd_solver* s, const char* title, const char* path);
  * fsxd_reset, then Un and Pn from the file */
 int fsxd_read_restart(fsxd_solver* s, const char* path);
 
+/* ------------------------------------------------------------ direct-solve order
+ * The fill-reducing order of FSX_SOLVER_DIRECT's sparse factor (the role of
+ * the MATLAB prototype's symamd, ML:1880-1881): nested dissection of the n
+ * pressure nodes (coordinates xyz, 3 a node) by recursive coordinate
+ * bisection with vertex separators from the symmetric pattern (rp, ci) of
+ * Z; node sets of at most `leaf` nodes end the recursion. perm (n entries,
+ * may be NULL) receives the order, new index -> pressure id, in postorder of
+ * the separator tree; stats4 (may be NULL) the entries of L with dense
+ * fronts (pivot blocks' lower triangles included), the factor flops (sum of
+ * squared column counts), the number of fronts and the tree's depth. The
+ * device factor computes this same order. FSX_EINVAL for an inconsistent
+ * pattern or a non-finite coordinate. */
+int fsxh_nd_order(int n, const int* rp, const int* ci, const double* xyz, int leaf, int* perm, double* stats4);
+
 #ifdef __cplusplus
 }
 #endif

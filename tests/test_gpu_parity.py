@@ -497,27 +497,31 @@ def test_warm_start(fsx_lib, oracle_lib, mode):
     s.close()
 
 
+@pytest.mark.parametrize("order", ["band", "nd"])
 @pytest.mark.parametrize("case", ["cavity16", "duct"])
-def test_direct_pressure_solve(fsx_lib, oracle_lib, case):
-    """FSX_SOLVER_DIRECT (band Cholesky of Z on the device, the MATLAB
-    prototype's pressure step, ML:1876-1885, 2021-2030) against the oracle's
-    Cholesky solve: one solve of a random right-hand side to 1e-10 with an
-    FP64 residual below 1e-12, and fields after 3 time steps to 1e-9 (both
-    solves exact up to rounding); the device's band is never wider than the
-    oracle's natural order. The 16^3 cavity keeps the pressure grid's order
-    (half-bandwidth 614, padding rows up to a multiple of 256); the permuted
-    duct is reordered by reverse Cuthill-McKee. An rtol below what FP64 can
-    reach returns FSX_ENOCONV with the solution."""
+def test_direct_pressure_solve(fsx_lib, oracle_lib, case, order):
+    """FSX_SOLVER_DIRECT (Cholesky of Z on the device, the MATLAB prototype's
+    pressure step, ML:1876-1885, 2021-2030) against the oracle's Cholesky
+    solve: one solve of a random right-hand side to 1e-10 with an FP64
+    residual below 1e-12, and fields after 3 time steps to 1e-9 (both solves
+    exact up to rounding). Band: never wider than the oracle's natural order;
+    the 16^3 cavity keeps the pressure grid's order (half-bandwidth 614,
+    padding rows up to a multiple of 256), the permuted duct is reordered by
+    reverse Cuthill-McKee. Nested dissection (the default, the role of the
+    prototype's symamd): a multifrontal factor whose fronts hold the entry
+    count fsxh_nd_order reports, smaller than the band's. An rtol below what
+    FP64 can reach returns FSX_ENOCONV with the solution."""
     if case == "duct":
-        say("direct pressure solve: permuted bent duct n=4, 3 steps")
+        say(f"direct pressure solve ({order}): permuted bent duct n=4, 3 steps")
         mesh = fsx_lib.Mesh.duct(n=4, n_in=4, n_arc=6, n_out=5, seed=5)
         kw = dict(NGP=27, viscosity=1.0 / 200, dt=0.005)
     else:
-        say("direct pressure solve: cavity 16^3 NGP 8, 3 steps")
+        say(f"direct pressure solve ({order}): cavity 16^3 NGP 8, 3 steps")
         mesh = fsx_lib.Mesh.cavity(16)
         kw = dict(NGP=8)
     s, o = make_pair(fsx_lib, oracle_lib, mesh, pcg_rtol=1e-10, **kw)
     s.set_solver(fsx_lib.SOLVER_DIRECT, 1e-10, 1)
+    s.set_direct_order(fsx_lib.DIRECT_ND if order == "nd" else fsx_lib.DIRECT_BAND)
     o.set_direct(1)
     n = s.sizes["NNp"]
     b = np.random.default_rng(3).standard_normal(n)
@@ -526,8 +530,18 @@ def test_direct_pressure_solve(fsx_lib, oracle_lib, case):
     xo, _ = o.direct_solve(b)
     assert rel(x, xo) < 1e-10
     w, ms, nbytes = s.direct_stats()
-    assert 0 < w <= o.chol_bandwidth() and ms > 0 and nbytes >= 8.0 * n * (w + 1)
-    say(f"  half-bandwidth {w} (oracle's natural order {o.chol_bandwidth()}), factor {ms:.1f} ms")
+    if order == "band":
+        assert 0 < w <= o.chol_bandwidth() and ms > 0 and nbytes >= 8.0 * n * (w + 1)
+        say(f"  half-bandwidth {w} (oracle's natural order {o.chol_bandwidth()}), factor {ms:.1f} ms")
+    else:
+        t = fsx_lib.Topology(mesh)
+        _, st = fsx_lib.nd_order(t.get("Zrow"), t.get("Zcol"), np.asarray(t.get("coord")).reshape(-1, 3)[:n])
+        lb, fl, nf, depth = s.direct_info()
+        assert w == -1 and ms > 0 and nbytes == lb and nf == st["fronts"] and depth == st["depth"]
+        assert st["nnz_l"] * 8.0 <= lb and fl == st["flops"]
+        if case == "cavity16":   # against the band's storage in the grid order
+            assert lb < 8.0 * n * (o.chol_bandwidth() + 1)
+        say(f"  nested dissection: {nf} fronts, depth {depth}, L {lb / 1e6:.1f} MB, factor {ms:.1f} ms")
     st = s.run(3)
     o.run(3)
     assert all(v["pcg"] == 0 and v["relres"] < 1e-10 for v in st)
