@@ -20,17 +20,22 @@
 // the workspace for the parent level. The 128x128 diagonal blocks of every
 // front's pivot block are inverted once (k_mf_inv).
 //
+// After the levels, every front's pivot block is inverted explicitly, X =
+// inv(L11): its 128x128 diagonal blocks first (k_mf_inv), then block row by
+// block row, right-looking, from the identity (k_mf_xrow applies the
+// diagonal block's inverse to the block row, k_mf_xupd subtracts its
+// product with L's block column from the rows below), all fronts together.
+//
 // Solve. Forward, from the deepest level: a front's vector is its pivots'
 // right-hand side plus its children's update vectors (k_mf_fwd_init, one
-// workgroup a front, children in slot order); its pivot block is applied in
-// 128-row steps, right-looking: every workgroup of a step forms y_s =
-// inv(L_ss) w_s itself and subtracts L(r, s) y_s from its 64 rows below the
-// block (k_mf_fwd_step); what the boundary rows hold at the end is the
-// front's update vector. Backward, from the roots: the boundary columns,
-// y_j -= L(B, j)^T x_B (k_mf_bwd_bound), then the blocks from the last,
-// x_s = inv(L_ss)^T y_s and y_j -= L(s, j)^T x_s for the columns j before
-// the block (k_mf_bwd_step). Every sum has a fixed order: the solve is
-// deterministic.
+// workgroup a front, children in slot order); y = X w_P (k_mf_fwd_x, column
+// chunks of 256 in separate workgroups, their partial sums kept), then the
+// update vector u = w_B - L21 y (k_mf_fwd_l21 sums y from those partials and
+// forms L21's partial sums by column chunk, k_mf_fwd_reduce adds them).
+// Backward, from the roots: y_P -= L21^T x_B (k_mf_bwd_bound, a wavefront a
+// column), then x_P = X^T y_P (k_mf_bwd_x, a wavefront a column of X). Each
+// level is six launches whatever its fronts' sizes, and every sum has a
+// fixed order: the solve is deterministic.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -59,6 +64,9 @@ struct MfFront {
   long long woff;     // its vector in the forward-solve workspace
   long long boff;     // its boundary rows (new numbering) in the index store
   long long coff[2];  // its children's rows in this front (nb of the child each)
+  long long xoff;     // X = inv(L11), npiv x npiv column-major, in the inverse store
+  long long poff;     // partial sums of X w by 256-column chunk (ceil(npiv / 256) x npiv)
+  long long qoff;     // partial sums of L21 y by 256-column chunk (ceil(npiv / 256) x nb)
   int m, npiv, p0, nb;
   int child[2];       // device indices of the children (-1: none)
 };
@@ -67,7 +75,7 @@ struct MfFront {
 
 struct MfLevel {
   int f0 = 0, nf = 0;           // device fronts [f0, f0 + nf)
-  int maxpiv = 0, maxm = 0;
+  int maxpiv = 0, maxm = 0, maxnb = 0;
   int maxnb_child[2] = {0, 0};  // largest boundary of a child in each slot
   long long ws = 0;             // doubles of the level workspace
   long long a0 = 0, na = 0;     // range of the level's Z entries in the assembly list
@@ -77,7 +85,7 @@ struct MfState {
   std::vector<MfLevel> lev;     // by depth (0: roots)
   std::vector<int> hm, hpiv;    // per device front
   int nfront = 0, depth = 0, n = 0;
-  double nnz_l = 0, flops = 0, l_entries = 0, factor_ms = 0;
+  double nnz_l = 0, flops = 0, l_entries = 0, x_entries = 0, factor_ms = 0;
   MfFront* F = nullptr;
   int* cmap = nullptr;
   int* bidx = nullptr;
@@ -86,10 +94,15 @@ struct MfState {
   int* asrc = nullptr;
   double* ws[2] = {nullptr, nullptr};
   double* L = nullptr;
-  double* invc = nullptr;       // column-major inverses (forward)
-  double* invr = nullptr;       // row-major inverses (backward)
+  double* invc = nullptr;       // inverses of the 128x128 diagonal blocks, column-major
   int2* itask = nullptr;        // (front, block) of each inverse
   int ninv = 0;
+  double* Xs = nullptr;         // every front's X = inv(L11)
+  double* P = nullptr;          // partial sums of the forward solve
+  int* xlist = nullptr;         // fronts by descending block count (the inversion's steps)
+  std::vector<int> xcnt;        // fronts with more than s blocks, s = 0, 1, ...
+  std::vector<int> xorder;      // the host copy of xlist
+  int maxpiv = 0;
   double* W = nullptr;
   double* Y = nullptr;
   double* X = nullptr;
@@ -261,19 +274,15 @@ __global__ __launch_bounds__(256) void k_mf_store(const MfFront* __restrict__ F,
   for (int r = j + threadIdx.x; r < T.m; r += blockDim.x) d[r] = s[r];
 }
 
-// X = inverse of diagonal block s of a front's pivot block (kb x kb,
-// lower triangular; identity past kb): lane c forms column c by forward
-// substitution; written row-major (invr, X(r, c) at r*128 + c) and
-// column-major (invc, at c*128 + r)
+// inverse of diagonal block s of a front's pivot block (kb x kb, lower
+// triangular; identity past kb): lane c forms column c by forward
+// substitution, column-major (invc, X(r, c) at c*128 + r)
 __global__ __launch_bounds__(kSB) void k_mf_inv(const MfFront* __restrict__ F, const int2* __restrict__ task,
-                                                const double* __restrict__ L, double* __restrict__ invc,
-                                                double* __restrict__ invr) {
+                                                const double* __restrict__ L, double* __restrict__ invc) {
   const int2 tk = task[blockIdx.x];
   const MfFront T = F[tk.x];
   const int s = tk.y, R0 = s * kSB, kb = min(kSB, T.npiv - R0);
-  const long long ob = T.ioff + (long long)s * kSB * kSB;
-  double* __restrict__ Xr = invr + ob;
-  double* __restrict__ Xc = invc + ob;
+  double* __restrict__ Xc = invc + T.ioff + (long long)s * kSB * kSB;
   const int c = threadIdx.x;
   const double* __restrict__ Lb = L + T.loff + (long long)R0 * T.m + R0;   // L(R0 + r, R0 + q) at q*m + r
   for (int r = 0; r < kSB; r++) {
@@ -286,11 +295,104 @@ __global__ __launch_bounds__(kSB) void k_mf_inv(const MfFront* __restrict__ F, c
       v = 1.0 / Lb[(long long)r * T.m + r];
     } else {
       double acc = 0.0;
-      for (int q = c; q < r; q++) acc += Lb[(long long)q * T.m + r] * Xr[q * kSB + c];
+      for (int q = c; q < r; q++) acc += Lb[(long long)q * T.m + r] * Xc[c * kSB + q];
       v = -acc / Lb[(long long)r * T.m + r];
     }
-    Xr[r * kSB + c] = v;
     Xc[c * kSB + r] = v;
+  }
+}
+
+// ones on every front's diagonal of X (the store is zeroed before)
+__global__ void k_mf_xdiag(const MfFront* __restrict__ F, double* __restrict__ Xs) {
+  const MfFront T = F[blockIdx.x];
+  for (int c = threadIdx.x; c < T.npiv; c += blockDim.x) Xs[T.xoff + (long long)c * T.npiv + c] = 1.0;
+}
+
+// block row s of X, columns [0, min(npiv, 128(s+1))): X_s = inv(L_ss) X_s
+// (the rows hold what the earlier steps left of the identity), a 64-column
+// chunk a workgroup: lane (i, h) forms row i for 32 of its columns
+__global__ __launch_bounds__(256) void k_mf_xrow(const MfFront* __restrict__ F, const int* __restrict__ list, int s,
+                                                 const double* __restrict__ invc, double* __restrict__ Xs) {
+  __shared__ double Ws[kSB][65];
+  const MfFront T = F[list[blockIdx.y]];
+  const int R0 = s * kSB;
+  if (T.npiv <= R0) return;
+  const int kb = min(kSB, T.npiv - R0);
+  const int ncols = min(T.npiv, R0 + kSB);
+  const int c0 = 64 * blockIdx.x;
+  if (c0 >= ncols) return;
+  const int t = threadIdx.x;
+  double* __restrict__ Xf = Xs + T.xoff;   // X(r, c) at c*npiv + r
+  for (int e = t; e < kSB * 64; e += 256) {
+    const int c = e / kSB, i = e % kSB;
+    Ws[i][c] = (i < kb && c0 + c < ncols) ? Xf[(long long)(c0 + c) * T.npiv + R0 + i] : 0.0;
+  }
+  __syncthreads();
+  const int i = t & (kSB - 1), h = t >> 7;
+  if (i >= kb) return;
+  double acc[32];
+#pragma unroll
+  for (int c = 0; c < 32; c++) acc[c] = 0.0;
+  const double* __restrict__ inv = invc + T.ioff + (long long)s * kSB * kSB + i;   // inv(i, j) at j*128 + i
+  for (int j = 0; j <= i; j++) {
+    const double a = inv[(long long)j * kSB];
+#pragma unroll
+    for (int c = 0; c < 32; c++) acc[c] += a * Ws[j][32 * h + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 32; c++) {
+    const int col = c0 + 32 * h + c;
+    if (col < ncols) Xf[(long long)col * T.npiv + R0 + i] = acc[c];
+  }
+}
+
+// rows [R1, npiv) of X, R1 = 128(s+1), columns [0, R1): X(r, c) -= sum_k
+// L(r, R0 + k) X(R0 + k, c) over the 128 columns of block s, one 64x64 tile
+// a workgroup, 4x4 outputs a lane
+__global__ __launch_bounds__(256) void k_mf_xupd(const MfFront* __restrict__ F, const int* __restrict__ list, int s,
+                                                 const double* __restrict__ L, double* __restrict__ Xs) {
+  __shared__ double A[kPanel * kTileLd];   // L(r0 + a, R0 + kc + k) at k*kTileLd + a
+  __shared__ double B[kPanel * kTileLd];   // X(R0 + kc + k, c0 + c) at k*kTileLd + c
+  const MfFront T = F[list[blockIdx.y]];
+  const int R0 = s * kSB, R1 = R0 + kSB;
+  if (T.npiv <= R1) return;
+  const int ct = R1 / 64;
+  const long long I = blockIdx.x / ct, J = blockIdx.x % ct;
+  const long long r0 = R1 + 64 * I, c0 = 64 * J;
+  if (r0 >= T.npiv) return;
+  const int t = threadIdx.x, ta = t / 16, tb = t % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) acc[x][y] = 0.0;
+  for (int kc = 0; kc < kSB; kc += kPanel) {
+    for (int e = t; e < 64 * kPanel; e += 256) {
+      const int k = e / 64, a = e % 64;
+      const long long r = r0 + a;
+      A[k * kTileLd + a] = r < T.npiv ? L[T.loff + (long long)(R0 + kc + k) * T.m + r] : 0.0;
+      B[a * kTileLd + k] = Xs[T.xoff + (c0 + k) * (long long)T.npiv + R0 + kc + a];
+    }
+    __syncthreads();
+    for (int k = 0; k < kPanel; k++) {
+      double pa[4], pb[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) { pa[q] = A[k * kTileLd + 4 * ta + q]; pb[q] = B[k * kTileLd + 4 * tb + q]; }
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] += pa[x] * pb[y];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int y = 0; y < 4; y++) {
+    const long long c = c0 + 4 * tb + y;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const long long r = r0 + 4 * ta + x;
+      if (r < T.npiv) Xs[T.xoff + c * T.npiv + r] -= acc[x][y];
+    }
   }
 }
 
@@ -324,55 +426,95 @@ __global__ __launch_bounds__(256) void k_mf_fwd_init(const MfFront* __restrict__
   }
 }
 
-// forward step s of the level's fronts: y_s = inv(L_ss) w_s (every
-// workgroup forms it; chunk 0 stores it in Y), then w_r -= L(r, s) y_s for
-// this workgroup's 64 rows r below the block, the block's columns split
-// over the four wavefronts and their sums added in a fixed order
-__global__ __launch_bounds__(256) void k_mf_fwd_step(const MfFront* __restrict__ F, int f0, int s,
-                                                     const double* __restrict__ L, const double* __restrict__ invc,
-                                                     double* __restrict__ W, double* __restrict__ Y) {
-  __shared__ double wsh[kSB];
-  __shared__ double part[2][kSB];
+// partial sums of y = X w_P: workgroup (row chunk, column chunk cc, front)
+// sums columns [256 cc, 256 cc + 256) of its 64 rows, 64 columns a
+// wavefront, the wavefronts' sums added in a fixed order; rows r >= 256 cc
+// keep P(cc, r) (X is lower triangular: the later chunks of a row are zero)
+__global__ __launch_bounds__(256) void k_mf_fwd_x(const MfFront* __restrict__ F, int f0, const double* __restrict__ W,
+                                                  const double* __restrict__ Xs, double* __restrict__ P) {
   __shared__ double red[4][64];
-  const MfFront T = F[f0 + blockIdx.y];
-  const int R0 = s * kSB;
-  if (T.npiv <= R0) return;
-  const int kb = min(kSB, T.npiv - R0);
-  const long long rb = (long long)R0 + kb + 64LL * blockIdx.x;
-  if (blockIdx.x > 0 && rb >= T.m) return;
-  const int t = threadIdx.x;
-  if (t < kSB) wsh[t] = t < kb ? W[T.woff + R0 + t] : 0.0;
-  __syncthreads();
-  {
-    // y_i = sum_j X(i, j) w_j: lane i of half h sums j in [64h, 64h + 64)
-    const int i = t & (kSB - 1), h = t >> 7;
-    const double* __restrict__ X = invc + T.ioff + (long long)s * kSB * kSB + (long long)(64 * h) * kSB + i;
-    double v[64];
-#pragma unroll
-    for (int j = 0; j < 64; j++) v[j] = X[(long long)j * kSB];
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < 64; j++) acc += v[j] * wsh[64 * h + j];
-    part[h][i] = acc;
-  }
-  __syncthreads();
-  if (t < kSB) wsh[t] = part[0][t] + part[1][t];   // y (zero past kb: X is the identity there, w zero)
-  __syncthreads();
-  if (blockIdx.x == 0 && t < kb) Y[T.p0 + R0 + t] = wsh[t];
-  const int lane = t & 63, q = t >> 6;
-  const long long r = rb + lane;
+  const MfFront T = F[f0 + blockIdx.z];
+  const int r0 = 64 * blockIdx.x, cc = blockIdx.y, j0 = 256 * cc;
+  if (r0 >= T.npiv || j0 > r0 + 63) return;
+  const int t = threadIdx.x, lane = t & 63, q = t >> 6;
+  const int r = r0 + lane;
+  const int ja = j0 + 64 * q, jb = min(min(ja + 64, T.npiv), r0 + 64);
   double acc = 0.0;
-  if (r < T.m) {
-    const double* __restrict__ Lc = L + T.loff + (long long)(R0 + 32 * q) * T.m + r;
-    double v[32];
+  if (r < T.npiv) {
+    const double* __restrict__ Xc = Xs + T.xoff + r;
+    const double* __restrict__ w = W + T.woff;
+    for (int j = ja; j < jb; j += 16) {
+      double a[16], x[16];
 #pragma unroll
-    for (int j = 0; j < 32; j++) v[j] = 32 * q + j < kb ? Lc[(long long)j * T.m] : 0.0;
+      for (int u = 0; u < 16; u++) {
+        const int jj = j + u;
+        const bool ok = jj < jb;
+        a[u] = ok ? Xc[(long long)jj * T.npiv] : 0.0;
+        x[u] = ok ? w[jj] : 0.0;
+      }
 #pragma unroll
-    for (int j = 0; j < 32; j++) acc += v[j] * wsh[32 * q + j];
+      for (int u = 0; u < 16; u++) acc += a[u] * x[u];
+    }
   }
   red[q][lane] = acc;
   __syncthreads();
-  if (q == 0 && r < T.m) W[T.woff + r] -= (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  if (q == 0 && r < T.npiv && r >= j0)
+    P[T.poff + (long long)cc * T.npiv + r] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+// y of the chunk's 256 columns from the partial sums (chunks in order; chunk
+// row 0 stores them in Y), then partial sums of L21 y: workgroup (row
+// chunk, column chunk cc, front) over its 64 boundary rows, kept in Q(cc, r)
+__global__ __launch_bounds__(256) void k_mf_fwd_l21(const MfFront* __restrict__ F, int f0, const double* __restrict__ P,
+                                                    const double* __restrict__ L, double* __restrict__ Y,
+                                                    double* __restrict__ Q) {
+  __shared__ double ys[256];
+  __shared__ double red[4][64];
+  const MfFront T = F[f0 + blockIdx.z];
+  const int cc = blockIdx.y, j0 = 256 * cc;
+  if (j0 >= T.npiv) return;
+  const int r0 = 64 * blockIdx.x;
+  if (blockIdx.x > 0 && r0 >= T.nb) return;
+  const int t = threadIdx.x;
+  {
+    const int j = j0 + t;
+    double y = 0.0;
+    if (j < T.npiv)
+      for (int c = 0; c <= j / 256; c++) y += P[T.poff + (long long)c * T.npiv + j];
+    ys[t] = y;
+    if (blockIdx.x == 0 && j < T.npiv) Y[T.p0 + j] = y;
+  }
+  __syncthreads();
+  const int lane = t & 63, q = t >> 6;
+  const int r = r0 + lane;
+  const int ja = j0 + 64 * q, jb = min(ja + 64, T.npiv);
+  double acc = 0.0;
+  if (r < T.nb) {
+    const double* __restrict__ Lc = L + T.loff + T.npiv + r;
+    for (int j = ja; j < jb; j += 16) {
+      double a[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) a[u] = j + u < jb ? Lc[(long long)(j + u) * T.m] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; u++) acc += a[u] * ys[j + u - j0];   // (j + u - j0 < 256; a[u] = 0 past jb)
+    }
+  }
+  red[q][lane] = acc;
+  __syncthreads();
+  if (q == 0 && r < T.nb)
+    Q[T.qoff + (long long)cc * T.nb + r] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+// the update vector: w_B -= sum over the column chunks of Q (in order)
+__global__ __launch_bounds__(256) void k_mf_fwd_reduce(const MfFront* __restrict__ F, int f0, const double* __restrict__ Q,
+                                                       double* __restrict__ W) {
+  const MfFront T = F[f0 + blockIdx.y];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= T.nb) return;
+  const int ncc = (T.npiv + 255) / 256;
+  double acc = 0.0;
+  for (int c = 0; c < ncc; c++) acc += Q[T.qoff + (long long)c * T.nb + r];
+  W[T.woff + T.npiv + r] -= acc;
 }
 
 // y_j -= sum_r L(npiv + r, j) x(B_r) for the pivot columns j, a wavefront a
@@ -404,59 +546,31 @@ __global__ __launch_bounds__(256) void k_mf_bwd_bound(const MfFront* __restrict_
   if (lane == 0) Y[T.p0 + j] -= acc;
 }
 
-// backward step s: x_s = inv(L_ss)^T y_s (every workgroup forms it; chunk 0
-// stores it in X), then y_j -= sum_i L(R0 + i, j) x_i for this workgroup's
-// 64 columns j < R0, 16 a wavefront
-__global__ __launch_bounds__(256) void k_mf_bwd_step(const MfFront* __restrict__ F, int f0, int s,
-                                                     const double* __restrict__ L, const double* __restrict__ invr,
-                                                     double* __restrict__ Y, double* __restrict__ Xo) {
-  __shared__ double ysh[kSB];
-  __shared__ double part[2][kSB];
+// x_j = sum_{i >= j} X(i, j) y_i, a wavefront a column of X
+__global__ __launch_bounds__(256) void k_mf_bwd_x(const MfFront* __restrict__ F, int f0, const double* __restrict__ Xs,
+                                                  const double* __restrict__ Y, double* __restrict__ Xo) {
   const MfFront T = F[f0 + blockIdx.y];
-  const int R0 = s * kSB;
-  if (T.npiv <= R0) return;
-  const int kb = min(kSB, T.npiv - R0);
-  const int c0 = 64 * blockIdx.x;
-  if (blockIdx.x > 0 && c0 >= R0) return;
-  const int t = threadIdx.x;
-  if (t < kSB) ysh[t] = t < kb ? Y[T.p0 + R0 + t] : 0.0;
-  __syncthreads();
-  {
-    // x_i = sum_j X(j, i) y_j: lane i of half h sums j in [64h, 64h + 64)
-    const int i = t & (kSB - 1), h = t >> 7;
-    const double* __restrict__ X = invr + T.ioff + (long long)s * kSB * kSB + (long long)(64 * h) * kSB + i;
-    double v[64];
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= T.npiv) return;
+  const double* __restrict__ Xc = Xs + T.xoff + (long long)j * T.npiv;
+  const double* __restrict__ y = Y + T.p0;
+  double acc = 0.0;
+  for (int i0 = j; i0 < T.npiv; i0 += 4 * 64) {
+    double a[4], v[4];
 #pragma unroll
-    for (int j = 0; j < 64; j++) v[j] = X[(long long)j * kSB];
-    double acc = 0.0;
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * 64 + lane;
+      const bool ok = i < T.npiv;
+      a[u] = ok ? Xc[i] : 0.0;
+      v[u] = ok ? y[i] : 0.0;
+    }
 #pragma unroll
-    for (int j = 0; j < 64; j++) acc += v[j] * ysh[64 * h + j];
-    part[h][i] = acc;
-  }
-  __syncthreads();
-  if (t < kSB) ysh[t] = part[0][t] + part[1][t];
-  __syncthreads();
-  if (blockIdx.x == 0 && t < kb) Xo[T.p0 + R0 + t] = ysh[t];
-  const int lane = t & 63, q = t >> 6;
-  const double x0 = ysh[lane], x1 = ysh[64 + lane];
-  const bool ok0 = lane < kb, ok1 = 64 + lane < kb;
-  double a0[16], a1[16];
-#pragma unroll
-  for (int u = 0; u < 16; u++) {
-    const int j = c0 + 16 * q + u;
-    const bool okj = j < R0;
-    const double* __restrict__ Lc = L + T.loff + (long long)j * T.m + R0;
-    a0[u] = okj && ok0 ? Lc[lane] : 0.0;
-    a1[u] = okj && ok1 ? Lc[64 + lane] : 0.0;
+    for (int u = 0; u < 4; u++) acc += a[u] * v[u];
   }
 #pragma unroll
-  for (int u = 0; u < 16; u++) {
-    double acc = a0[u] * x0 + a1[u] * x1;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    const int j = c0 + 16 * q + u;
-    if (lane == 0 && j < R0) Y[T.p0 + j] -= acc;
-  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) Xo[T.p0 + j] = acc;
 }
 
 template <class Fn>
@@ -470,7 +584,7 @@ void mf_free(fsx_ctx* ctx) {
   MfState* S = ctx->mf;
   if (!S) return;
   for (void* p : {(void*)S->F, (void*)S->cmap, (void*)S->bidx, (void*)S->perm, (void*)S->adest, (void*)S->asrc,
-                  (void*)S->ws[0], (void*)S->ws[1], (void*)S->L, (void*)S->invc, (void*)S->invr, (void*)S->itask,
+                  (void*)S->ws[0], (void*)S->ws[1], (void*)S->L, (void*)S->invc, (void*)S->itask, (void*)S->Xs, (void*)S->P, (void*)S->xlist,
                   (void*)S->W, (void*)S->Y, (void*)S->X, (void*)S->bp, (void*)S->flag})
     if (p) dev_free(ctx, p);
   delete S;
@@ -514,7 +628,7 @@ static int mf_plan(fsx_ctx* ctx) {
   std::vector<int> cmap, bidx;
   S->hm.assign(nf, 0);
   S->hpiv.assign(nf, 0);
-  long long loff = 0, ioff = 0, woff = 0;
+  long long loff = 0, ioff = 0, woff = 0, xoff = 0, poff = 0;
   std::vector<int2> itask;
   for (int t = 0; t < nf; t++) {
     const NdFront& f = P.fronts[t];
@@ -532,10 +646,30 @@ static int mf_plan(fsx_ctx* ctx) {
     g.ioff = ioff; ioff += (long long)nblk * kSB * kSB;
     for (int s = 0; s < nblk; s++) itask.push_back(make_int2(did[t], s));
     g.woff = woff; woff += g.m;
+    g.xoff = xoff; xoff += (long long)g.npiv * g.npiv;
+    const long long ncc = (g.npiv + 255) / 256;
+    g.poff = poff; poff += ncc * g.npiv;
+    g.qoff = poff; poff += ncc * g.nb;
+    L.maxnb = std::max(L.maxnb, g.nb);
     g.boff = (long long)bidx.size();
     bidx.insert(bidx.end(), f.bnd.begin(), f.bnd.end());
     S->hm[did[t]] = g.m;
     S->hpiv[did[t]] = g.npiv;
+  }
+  // the inversion's steps: fronts by descending block count
+  {
+    std::vector<int> order(nf);
+    for (int t = 0; t < nf; t++) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return S->hpiv[a] > S->hpiv[b]; });
+    S->xcnt.clear();
+    const int maxblk = nf ? (S->hpiv[order[0]] + kSB - 1) / kSB : 0;
+    for (int st = 0; st < maxblk; st++) {
+      int c = 0;
+      while (c < nf && S->hpiv[order[c]] > st * kSB) c++;
+      S->xcnt.push_back(c);
+    }
+    S->maxpiv = nf ? S->hpiv[order[0]] : 0;
+    S->xorder = order;
   }
   // children's rows in their parents
   for (int t = 0; t < nf; t++) {
@@ -605,7 +739,7 @@ static int mf_plan(fsx_ctx* ctx) {
   long long wsmax = 1;
   for (const MfLevel& L : S->lev) wsmax = std::max(wsmax, L.ws);
   S->l_entries = (double)loff;
-  const double need = 8.0 * (2.0 * wsmax + (double)loff + 2.0 * (double)ioff + (double)woff + 4.0 * n) +
+  const double need = 8.0 * (2.0 * wsmax + (double)loff + (double)ioff + (double)xoff + (double)poff + (double)woff + 4.0 * n) +
                       12.0 * (double)adest.size() + 4.0 * (double)(cmap.size() + bidx.size());
   size_t freeb = 0, totalb = 0;
   FSX_HIP(hipMemGetInfo(&freeb, &totalb));
@@ -622,7 +756,10 @@ static int mf_plan(fsx_ctx* ctx) {
   FSX_TRY(alloc(ctx, &S->ws[1], (size_t)wsmax));
   FSX_TRY(alloc(ctx, &S->L, std::max<size_t>(1, (size_t)loff)));
   FSX_TRY(alloc(ctx, &S->invc, std::max<size_t>(1, (size_t)ioff)));
-  FSX_TRY(alloc(ctx, &S->invr, std::max<size_t>(1, (size_t)ioff)));
+  FSX_TRY(alloc(ctx, &S->Xs, std::max<size_t>(1, (size_t)xoff)));
+  FSX_TRY(alloc(ctx, &S->P, std::max<size_t>(1, (size_t)poff)));
+  FSX_TRY(alloc(ctx, &S->xlist, std::max<size_t>(1, (size_t)nf)));
+  S->x_entries = (double)xoff;
   FSX_TRY(alloc(ctx, &S->itask, std::max<size_t>(1, itask.size())));
   FSX_TRY(alloc(ctx, &S->W, std::max<size_t>(1, (size_t)woff)));
   FSX_TRY(alloc(ctx, &S->Y, (size_t)n));
@@ -639,6 +776,7 @@ static int mf_plan(fsx_ctx* ctx) {
     FSX_HIP(hipMemcpy(S->asrc, asrc.data(), asrc.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   if (!itask.empty()) FSX_HIP(hipMemcpy(S->itask, itask.data(), itask.size() * sizeof(int2), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(S->xlist, S->xorder.data(), (size_t)nf * sizeof(int), hipMemcpyHostToDevice));
   return FSX_OK;
 }
 
@@ -698,9 +836,24 @@ int mf_factor(fsx_ctx* ctx) {
     FSX_TRY(launch_check(ctx, "k_mf_store"));
     cur ^= 1;
   }
+  // X = inv(L11) of every front: the diagonal blocks' inverses, then block
+  // rows from the identity
   if (S->ninv > 0)
-    hipLaunchKernelGGL(k_mf_inv, dim3(S->ninv), dim3(kSB), 0, ctx->stream, S->F, S->itask, S->L, S->invc, S->invr);
+    hipLaunchKernelGGL(k_mf_inv, dim3(S->ninv), dim3(kSB), 0, ctx->stream, S->F, S->itask, S->L, S->invc);
   FSX_TRY(launch_check(ctx, "k_mf_inv"));
+  FSX_TRY(launch_fill(ctx, S->Xs, 0.0, (long long)S->x_entries));
+  if (S->nfront > 0) hipLaunchKernelGGL(k_mf_xdiag, dim3(S->nfront), dim3(256), 0, ctx->stream, S->F, S->Xs);
+  for (int st = 0; st < (int)S->xcnt.size(); st++) {
+    const int R1 = (st + 1) * kSB;
+    hipLaunchKernelGGL(k_mf_xrow, dim3((unsigned)(2 * (st + 1)), S->xcnt[st]), dim3(256), 0, ctx->stream, S->F,
+                       S->xlist, st, S->invc, S->Xs);
+    if (st + 1 < (int)S->xcnt.size()) {
+      const long long rt = (S->maxpiv - R1 + 63) / 64;
+      hipLaunchKernelGGL(k_mf_xupd, dim3((unsigned)(rt * (R1 / 64)), S->xcnt[st + 1]), dim3(256), 0, ctx->stream,
+                         S->F, S->xlist, st, S->L, S->Xs);
+    }
+  }
+  FSX_TRY(launch_check(ctx, "k_mf_xrow/k_mf_xupd"));
   int flag = 0;
   FSX_HIP(hipMemcpyAsync(&flag, S->flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   FSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -718,36 +871,34 @@ int mf_apply(fsx_ctx* ctx, const double* b, double* x) {
   hipLaunchKernelGGL(k_mf_permute, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, S->perm, b, S->bp);
   for (int d = S->depth; d >= 0; d--) {
     const MfLevel& L = S->lev[d];
-    over_y(L.nf, [&](int y0, int ny) {
-      hipLaunchKernelGGL(k_mf_fwd_init, dim3(ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0, S->cmap, S->bp, S->W);
+    over_y(L.nf, [&](int z0, int nz) {
+      hipLaunchKernelGGL(k_mf_fwd_init, dim3(nz), dim3(256), 0, ctx->stream, S->F, L.f0 + z0, S->cmap, S->bp, S->W);
     });
-    for (int s = 0; s * kSB < L.maxpiv; s++) {
-      long long chunks = 1;
-      for (int t = L.f0; t < L.f0 + L.nf; t++) {
-        if (S->hpiv[t] <= s * kSB) continue;
-        const long long kb = std::min(kSB, S->hpiv[t] - s * kSB), rows = S->hm[t] - s * kSB - kb;
-        chunks = std::max(chunks, (rows + 63) / 64);
-      }
-      over_y(L.nf, [&](int y0, int ny) {
-        hipLaunchKernelGGL(k_mf_fwd_step, dim3((unsigned)chunks, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0, s,
-                           S->L, S->invc, S->W, S->Y);
+    if (L.maxpiv > 0) {
+      const unsigned rch = (unsigned)((L.maxpiv + 63) / 64), cch = (unsigned)((L.maxpiv + 255) / 256);
+      const unsigned bch = (unsigned)std::max(1, (L.maxnb + 63) / 64);
+      over_y(L.nf, [&](int z0, int nz) {
+        hipLaunchKernelGGL(k_mf_fwd_x, dim3(rch, cch, nz), dim3(256), 0, ctx->stream, S->F, L.f0 + z0, S->W, S->Xs, S->P);
+        hipLaunchKernelGGL(k_mf_fwd_l21, dim3(bch, cch, nz), dim3(256), 0, ctx->stream, S->F, L.f0 + z0, S->P, S->L,
+                           S->Y, S->P);
       });
     }
+    if (L.maxnb > 0 && L.maxpiv > 0)
+      over_y(L.nf, [&](int z0, int nz) {
+        hipLaunchKernelGGL(k_mf_fwd_reduce, dim3((unsigned)((L.maxnb + 255) / 256), nz), dim3(256), 0, ctx->stream,
+                           S->F, L.f0 + z0, S->P, S->W);
+      });
   }
   for (int d = 0; d <= S->depth; d++) {
     const MfLevel& L = S->lev[d];
     if (L.maxpiv == 0) continue;
     over_y(L.nf, [&](int y0, int ny) {
-      hipLaunchKernelGGL(k_mf_bwd_bound, dim3((L.maxpiv + 3) / 4, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0,
-                         S->L, S->bidx, S->X, S->Y);
+      if (L.maxnb > 0)
+        hipLaunchKernelGGL(k_mf_bwd_bound, dim3((L.maxpiv + 3) / 4, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0,
+                           S->L, S->bidx, S->X, S->Y);
+      hipLaunchKernelGGL(k_mf_bwd_x, dim3((L.maxpiv + 3) / 4, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0, S->Xs,
+                         S->Y, S->X);
     });
-    for (int s = (L.maxpiv - 1) / kSB; s >= 0; s--) {
-      const unsigned chunks = (unsigned)std::max(1, (s * kSB + 63) / 64);
-      over_y(L.nf, [&](int y0, int ny) {
-        hipLaunchKernelGGL(k_mf_bwd_step, dim3(chunks, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0, s, S->L,
-                           S->invr, S->Y, S->X);
-      });
-    }
   }
   hipLaunchKernelGGL(k_mf_unpermute, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, S->perm, S->X, x);
   return launch_check(ctx, "multifrontal solve");
