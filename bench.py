@@ -237,6 +237,7 @@ def duct_workload(fsx, args, steps, warmup, cpu_steps):
     for name, mode, bits in (("fp64", fsx.SOLVER_FP64, 64), ("mixed", fsx.SOLVER_MIXED, 32)):
         if name not in args.duct_modes.split(","):
             continue
+        s.set_mass_precision(bits)   # the mixed forms' mass solves: FP32 segments, FP64 residuals
         r = run_mode(s, mode, steps, warmup, args.rtol, args.maxit, matrix_bits=bits, x0=args.x0,
                      total=warmup + steps)
         m_it, m_rel, m_tot = s.mass_stats()

@@ -173,6 +173,13 @@ struct fsx_ctx {
   double* mass_dinv = nullptr;   // NN, 1 / M_ii, 0 at BC nodes
   double* mass_sc = nullptr;     // device MassScalars
   bool mass_dinv_ready = false;
+  int mass_bits = 64;            // 32: FP32 segments with FP64 residuals (element form)
+  float* mass_w32 = nullptr;     // r, z, p, q, d (5 x 3NN) of the FP32 segments
+  float* mass_fac32 = nullptr;   // FP32 detJ w per element and Gauss point
+  float* mass_ye32 = nullptr;    // FP32 element products
+  float* mass_dinv32 = nullptr;  // FP32 Jacobi inverse
+  bool mass32_ready = false;     // the FP32 copies match mass_fac and mass_dinv
+  int mass_seg_iters[4] = {0, 0, 0, 0};   // iterations of the last mixed solve's segments
   // The element-by-element products (the consistent mass, the FP32
   // convection) visit the elements sorted by their centre node's internal id
   // (the nodes' coordinate order), so that a block's elements and a node's

@@ -28,6 +28,14 @@
 // fused into the second, or the CSR SpMV and p.q; update; direction) with
 // the scalars on the device, like the pressure PCG; the host polls the stop
 // flag after every kMassChunk iterations.
+//
+// Mixed precision (fsx_set_mass_precision 32, element form): the same
+// iteration in FP32 (vectors, element products, factors and shape table) on
+// the correction M_bc d = r, in segments: a segment starts from the FP64
+// residual r = b - M_bc x, runs until every component's FP32 residual has
+// fallen by 1e-4 (or below the FP64 target), then x += d and the residual is
+// formed again in FP64 (the FP64 element form), whose per-component test
+// ||r_c|| <= rtol ||b_c|| decides convergence, as in the FP64 solve.
 #include "fsx_device.h"
 
 #include <algorithm>
@@ -40,8 +48,13 @@ struct MassScalars {
   double rz[2][3];      // r.z per component, by iteration parity
   double bb[3], rr[3];  // |b_c|^2, |r_c|^2
   double rtol2;
-  int iters, maxit, stop;   // stop: 1 converged, 2 maxit, 3 NaN
+  double rr0[3];        // |r_c|^2 at a mixed segment's start
+  double delta2;        // a mixed segment ends at |r_c|^2 <= delta2 rr0_c (0: FP64 solve)
+  int iters, maxit, stop;   // stop: 1 converged (or the segment done), 2 maxit, 3 NaN
 };
+
+constexpr double kMassSegment = 1e-6;   // residual reduction of a mixed segment
+constexpr int kMassSegs = 4;            // segments whose lengths the next solve reuses
 
 constexpr int kMassChunk = 8;
 
@@ -114,6 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_mass_warm_finish(const double* __res
       done = done && s[6 + c] <= rtol * rtol * s[3 + c];
     }
     sc->rtol2 = rtol * rtol;
+    sc->delta2 = 0.0;
+    for (int c = 0; c < 3; c++) sc->rr0[c] = 0.0;
     sc->iters = 0; sc->maxit = maxit;
     sc->stop = done ? 1 : 0;
   }
@@ -159,26 +174,28 @@ __global__ __launch_bounds__(kBlock) void k_mass_spmv(int NN, const int* __restr
 // Sv[k][i] f_k u to y_i for the 27 nodes i. The element arrays are
 // transposed so that consecutive threads read consecutive words: ids
 // L[j * NE + e], factors fac[k * NE + e], products ye[(i * NE + e) * 3 + c].
+// (T = float: the mixed solve's FP32 form, the same operations in FP32)
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_mass_elem(int NE, int NGP, const int* __restrict__ L,
-                                                      const double* __restrict__ fac, const double* __restrict__ Sv,
-                                                      const double* __restrict__ p, double* __restrict__ ye,
+                                                      const T* __restrict__ fac, const T* __restrict__ Sv,
+                                                      const T* __restrict__ p, T* __restrict__ ye,
                                                       const MassScalars* __restrict__ sc) {
   if (sc && sc->stop) return;
   const long long t = blockIdx.x * (long long)kBlock + threadIdx.x;
   if (t >= 3LL * NE) return;
   const int e = (int)(t / 3), c = (int)(t % 3);
-  double pe[27], y[27];
+  T pe[27], y[27];
 #pragma unroll
   for (int j = 0; j < 27; j++) {
     pe[j] = p[3 * (size_t)L[(size_t)j * NE + e] + c];
-    y[j] = 0.0;
+    y[j] = T(0);
   }
   for (int k = 0; k < NGP; k++) {
-    const double* row = Sv + k * 27;
-    double u = 0.0;
+    const T* row = Sv + k * 27;
+    T u = T(0);
 #pragma unroll
     for (int j = 0; j < 27; j++) u += row[j] * pe[j];
-    const double v = fac[(size_t)k * NE + e] * u;
+    const T v = fac[(size_t)k * NE + e] * u;
 #pragma unroll
     for (int i = 0; i < 27; i++) y[i] += row[i] * v;
   }
@@ -192,26 +209,28 @@ __global__ __launch_bounds__(kBlock) void k_mass_elem(int NE, int NGP, const int
 constexpr int kGatherGrid = 2048;
 constexpr int kGatherPq = 32 * kRedGrid;
 
-template <bool DOT>
+template <bool DOT, class T>
 __global__ __launch_bounds__(kBlock) void k_mass_gather(int NN, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
-                                                        const double* __restrict__ ye, const unsigned char* __restrict__ bc,
-                                                        const double* __restrict__ p, double* __restrict__ q,
+                                                        const T* __restrict__ ye, const unsigned char* __restrict__ bc,
+                                                        const T* __restrict__ p, T* __restrict__ q,
                                                         double* __restrict__ partials, const MassScalars* __restrict__ sc) {
   if (sc && sc->stop) return;
   double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll 2   // independent rows: their loads in flight together
   for (int r = blockIdx.x * kBlock + threadIdx.x; r < NN; r += gridDim.x * kBlock) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    T s0 = T(0), s1 = T(0), s2 = T(0);
     if (!bc[r]) {
       const int t1 = inc_ptr[r + 1];
       for (int t = inc_ptr[r]; t < t1; t++) {
-        const double* y = ye + 3 * (size_t)inc[t];
+        const T* y = ye + 3 * (size_t)inc[t];
         s0 += y[0]; s1 += y[1]; s2 += y[2];
       }
     }
     const size_t k = 3 * (size_t)r;
     q[k] = s0; q[k + 1] = s1; q[k + 2] = s2;
-    if (DOT) { v[0] += p[k] * s0; v[1] += p[k + 1] * s1; v[2] += p[k + 2] * s2; }
+    if (DOT) {
+      v[0] += (double)p[k] * (double)s0; v[1] += (double)p[k + 1] * (double)s1; v[2] += (double)p[k + 2] * (double)s2;
+    }
   }
   if (DOT) {
     __shared__ double red[3 * (kBlock / kWave)];
@@ -237,31 +256,32 @@ __global__ __launch_bounds__(kBlock) void k_mass_dot(int NN, const double* __res
 // alpha_c = r.z_c / p.q_c (0 for a component with no work left), p.q_c the
 // sum of npq per-block partials at pqp + c npq;
 // x += alpha p, r -= alpha q, z = dinv r; partials of r.r (slots 0-2) and r.z (3-5)
-__global__ __launch_bounds__(kBlock) void k_mass_update(int j, int NN, double* __restrict__ x, double* __restrict__ r,
-                                                        double* __restrict__ z, const double* __restrict__ p,
-                                                        const double* __restrict__ q, const double* __restrict__ dinv,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mass_update(int j, int NN, T* __restrict__ x, T* __restrict__ r,
+                                                        T* __restrict__ z, const T* __restrict__ p,
+                                                        const T* __restrict__ q, const T* __restrict__ dinv,
                                                         const double* __restrict__ pqp, int npq,
                                                         double* __restrict__ partials, const MassScalars* __restrict__ sc) {
   if (sc->stop) return;
   double pq[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) pq[c] = sum_parts(pqp + c * npq, npq);
-  double a[3];
+  T a[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) a[c] = pq[c] > 0.0 ? sc->rz[j & 1][c] / pq[c] : 0.0;
+  for (int c = 0; c < 3; c++) a[c] = (T)(pq[c] > 0.0 ? sc->rz[j & 1][c] / pq[c] : 0.0);
   double v[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock) {
-    const double di = dinv[i];
+    const T di = dinv[i];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t k = 3 * (size_t)i + c;
       x[k] += a[c] * p[k];
-      const double ri = r[k] - a[c] * q[k];
-      const double zi = di * ri;
+      const T ri = r[k] - a[c] * q[k];
+      const T zi = di * ri;
       r[k] = ri; z[k] = zi;
-      v[c] += ri * ri;
-      v[3 + c] += ri * zi;
+      v[c] += (double)ri * (double)ri;
+      v[3 + c] += (double)ri * (double)zi;
     }
   }
   write_partials<6>(v, partials);
@@ -269,17 +289,18 @@ __global__ __launch_bounds__(kBlock) void k_mass_update(int j, int NN, double* _
 
 // beta_c = r.z_c(new) / r.z_c(old); p = z + beta p; block 0 records the scalars
 // and the stop test: every component at ||r_c|| <= rtol ||b_c||
-__global__ __launch_bounds__(kBlock) void k_mass_direction(int j, int NN, const double* __restrict__ z,
-                                                           double* __restrict__ p, const double* __restrict__ partials,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mass_direction(int j, int NN, const T* __restrict__ z,
+                                                           T* __restrict__ p, const double* __restrict__ partials,
                                                            MassScalars* sc) {
   if (sc->stop) return;
   double s[6];
   read_partials<6>(partials, s);
-  double be[3];
+  T be[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const double old = sc->rz[j & 1][c];
-    be[c] = old != 0.0 ? s[3 + c] / old : 0.0;
+    be[c] = (T)(old != 0.0 ? s[3 + c] / old : 0.0);
   }
 #pragma unroll 4   // independent elements: their loads in flight together
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock)
@@ -294,13 +315,103 @@ __global__ __launch_bounds__(kBlock) void k_mass_direction(int j, int NN, const 
       sc->rz[(j + 1) & 1][c] = s[3 + c];
       sc->rr[c] = s[c];
       bad = bad || !(s[c] == s[c]) || !(s[3 + c] == s[3 + c]);
-      done = done && s[c] <= sc->rtol2 * sc->bb[c];
+      done = done && s[c] <= fmax(sc->rtol2 * sc->bb[c], sc->delta2 * sc->rr0[c]);
     }
     const int it = sc->iters + 1;
     sc->iters = it;
     if (bad) sc->stop = 3;
     else if (done) sc->stop = 1;
     else if (it >= sc->maxit) sc->stop = 2;
+  }
+}
+
+// ---- mixed precision: FP32 segments on the correction d, FP64 residuals
+
+// A segment's start from the FP64 residual r64: r = float(r64), d = 0,
+// z = dinv r, p = z; partials of r.z (slots 0-2) and r.r (3-5)
+__global__ __launch_bounds__(kBlock) void k_mass_seg_start(int NN, const double* __restrict__ r64,
+                                                           const float* __restrict__ dinv, float* __restrict__ r,
+                                                           float* __restrict__ z, float* __restrict__ p,
+                                                           float* __restrict__ d, double* __restrict__ partials) {
+  double v[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock) {
+    const float di = dinv[i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t k = 3 * (size_t)i + c;
+      const float ri = (float)r64[k], zi = di * ri;
+      r[k] = ri; z[k] = zi; p[k] = zi; d[k] = 0.0f;
+      v[c] += (double)ri * (double)zi;
+      v[3 + c] += (double)ri * (double)ri;
+    }
+  }
+  write_partials<6>(v, partials);
+}
+
+// the segment's scalars: r.z for iteration j's parity, the start residual
+// rr0 and the segment's reduction target; stop at once if every component
+// already meets the FP64 target
+__global__ __launch_bounds__(kBlock) void k_mass_seg_finish(int j, const double* __restrict__ partials, MassScalars* sc,
+                                                            double delta) {
+  double s[6];
+  read_partials<6>(partials, s);
+  if (threadIdx.x == 0) {
+    bool done = true;
+    for (int c = 0; c < 3; c++) {
+      sc->rz[j & 1][c] = s[c];
+      sc->rr0[c] = s[3 + c];
+      sc->rr[c] = s[3 + c];
+      done = done && s[3 + c] <= sc->rtol2 * sc->bb[c];
+    }
+    sc->delta2 = delta * delta;
+    sc->stop = done ? 1 : 0;
+  }
+}
+
+// b = float(a), rounded to nearest
+__global__ void k_mass_round(long long n, const double* __restrict__ a, float* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// x += d (the FP64 iterate takes the segment's correction)
+__global__ void k_mass_fold(long long n, const float* __restrict__ d, double* __restrict__ x) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] += (double)d[i];
+}
+
+// r64 = b - q (q = M_bc x, zero at BC rows, where b is zero too); partials
+// of |r_c|^2 (slots 0-2)
+__global__ __launch_bounds__(kBlock) void k_mass_res64(int NN, const double* __restrict__ b, const double* __restrict__ q,
+                                                       double* __restrict__ r, double* __restrict__ partials) {
+  double v[3] = {0, 0, 0};
+#pragma unroll 4
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < NN; i += kRedGrid * kBlock)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t k = 3 * (size_t)i + c;
+      const double ri = b[k] - q[k];
+      r[k] = ri;
+      v[c] += ri * ri;
+    }
+  write_partials<3>(v, partials);
+}
+
+// the FP64 stop test on the formed residual: stop 1 when every component
+// meets ||r_c|| <= rtol ||b_c||, else 0 (a new segment follows), 3 on NaN,
+// 2 at the iteration cap
+__global__ __launch_bounds__(kBlock) void k_mass_res_finish(const double* __restrict__ partials, MassScalars* sc) {
+  double s[3];
+  read_partials<3>(partials, s);
+  if (threadIdx.x == 0) {
+    bool done = true, bad = false;
+    for (int c = 0; c < 3; c++) {
+      sc->rr[c] = s[c];
+      bad = bad || !(s[c] == s[c]);
+      done = done && s[c] <= sc->rtol2 * sc->bb[c];
+    }
+    sc->stop = bad ? 3 : (done ? 1 : (sc->iters >= sc->maxit ? 2 : 0));
   }
 }
 
@@ -363,14 +474,14 @@ static int mass_elements_setup(fsx_ctx* ctx) {
 static void mass_apply(fsx_ctx* ctx, bool elements, const double* p, double* q, bool dot, const MassScalars* sc) {
   const int NN = ctx->NN;
   if (elements) {
-    hipLaunchKernelGGL(k_mass_elem, dim3((unsigned)((3LL * ctx->NE + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, ctx->NE, ctx->NGP, ctx->el_L27, ctx->mass_fac, ctx->tab.Sv,
-                       p, ctx->mass_ye, sc);
+    hipLaunchKernelGGL(k_mass_elem<double>, dim3((unsigned)((3LL * ctx->NE + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, ctx->NE, ctx->NGP, ctx->el_L27, ctx->mass_fac, ctx->tab.Sv, p, ctx->mass_ye, sc);
     if (dot)
-      hipLaunchKernelGGL(k_mass_gather<true>, dim3(kGatherGrid), dim3(kBlock), 0, ctx->stream, NN, ctx->el_inc_ptr,
-                         ctx->el_inc, ctx->mass_ye, ctx->bcflag, p, q, ctx->partials, sc);
+      hipLaunchKernelGGL((k_mass_gather<true, double>), dim3(kGatherGrid), dim3(kBlock), 0, ctx->stream, NN,
+                         ctx->el_inc_ptr, ctx->el_inc, ctx->mass_ye, ctx->bcflag, p, q, ctx->partials, sc);
     else
-      hipLaunchKernelGGL(k_mass_gather<false>, dim3((unsigned)std::min((NN + kBlock - 1) / kBlock, 4096)), dim3(kBlock),
-                         0, ctx->stream, NN, ctx->el_inc_ptr, ctx->el_inc, ctx->mass_ye, ctx->bcflag, p, q,
+      hipLaunchKernelGGL((k_mass_gather<false, double>), dim3((unsigned)std::min((NN + kBlock - 1) / kBlock, 4096)),
+                         dim3(kBlock), 0, ctx->stream, NN, ctx->el_inc_ptr, ctx->el_inc, ctx->mass_ye, ctx->bcflag, p, q,
                          ctx->partials, sc);
     return;
   }
@@ -386,6 +497,86 @@ static void mass_apply(fsx_ctx* ctx, bool elements, const double* p, double* q, 
   if (dot) hipLaunchKernelGGL(k_mass_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, p, q, ctx->partials, sc);
 }
 
+// The statistics and return code of a finished solve from its scalars h
+static int mass_finish(fsx_ctx* ctx, const MassScalars& h) {
+  double rel = 0.0;
+  for (int c = 0; c < 3; c++) rel = std::max(rel, h.bb[c] > 0.0 ? std::sqrt(h.rr[c] / h.bb[c]) : 0.0);
+  ctx->mass_last_iters = h.iters;
+  ctx->mass_last_relres = rel;
+  ctx->mass_total_iters += h.iters;
+  if (h.stop == 3) return fail(ctx, FSX_ENUMERIC, "mass PCG met NaN/Inf");
+  if (h.stop == 2) { ctx->err = "mass PCG reached maxit"; return FSX_ENOCONV; }
+  return FSX_OK;
+}
+
+// The mixed form after the FP64 warm start (x, the FP64 residual r64 and the
+// scalars set): FP32 segments on M_bc d = r64, each folded into x and
+// checked on the FP64 residual of the FP64 element form.
+static int mass_solve_mixed(fsx_ctx* ctx, const double* b, double* x, double* r64, double* q64, MassScalars* sc) {
+  const int NN = ctx->NN, NE = ctx->NE;
+  const size_t n3 = 3 * (size_t)NN;
+  if (!ctx->mass_w32) {
+    FSX_TRY(alloc(ctx, &ctx->mass_w32, 5 * n3));
+    FSX_TRY(alloc(ctx, &ctx->mass_fac32, (size_t)NE * ctx->NGP));
+    FSX_TRY(alloc(ctx, &ctx->mass_ye32, (size_t)NE * 81));
+    FSX_TRY(alloc(ctx, &ctx->mass_dinv32, (size_t)NN));
+    ctx->mass32_ready = false;
+  }
+  if (!ctx->mass32_ready) {   // the FP32 factors and Jacobi inverse, rounded to nearest
+    hipLaunchKernelGGL(k_mass_round, dim3(4096), dim3(kBlock), 0, ctx->stream, (long long)NE * ctx->NGP, ctx->mass_fac,
+                       ctx->mass_fac32);
+    hipLaunchKernelGGL(k_mass_round, dim3(4096), dim3(kBlock), 0, ctx->stream, (long long)NN, ctx->mass_dinv,
+                       ctx->mass_dinv32);
+    FSX_TRY(launch_check(ctx, "mass FP32 copies"));
+    ctx->mass32_ready = true;
+  }
+  float *r = ctx->mass_w32, *z = r + n3, *p = z + n3, *q = p + n3, *d = q + n3;
+  MassScalars h{};
+  int j = 0;
+  const unsigned eg = (unsigned)((3LL * NE + kBlock - 1) / kBlock);
+  // (a warm start that already meets the target stops the first segment at
+  // once: k_mass_seg_finish tests the FP64 residual)
+  for (int seg = 0; seg == 0 || !h.stop; seg++) {
+    hipLaunchKernelGGL(k_mass_seg_start, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, r64, ctx->mass_dinv32, r, z, p,
+                       d, ctx->partials);
+    hipLaunchKernelGGL(k_mass_seg_finish, dim3(1), dim3(kBlock), 0, ctx->stream, j, ctx->partials, sc, kMassSegment);
+    // the first chunk as long as the same segment of the last solve, then
+    // kMassChunk iterations between polls (iterations past the stop are no-ops)
+    const int it0 = h.iters;
+    int chunk = std::max(kMassChunk, seg < kMassSegs ? ctx->mass_seg_iters[seg] : 0);
+    for (;;) {
+      for (int t = 0; t < chunk; t++, j++) {
+        hipLaunchKernelGGL(k_mass_elem<float>, dim3(eg), dim3(kBlock), 0, ctx->stream, NE, ctx->NGP, ctx->el_L27,
+                           ctx->mass_fac32, ctx->tabf.Sv, p, ctx->mass_ye32, sc);
+        hipLaunchKernelGGL((k_mass_gather<true, float>), dim3(kGatherGrid), dim3(kBlock), 0, ctx->stream, NN,
+                           ctx->el_inc_ptr, ctx->el_inc, ctx->mass_ye32, ctx->bcflag, p, q, ctx->partials, sc);
+        hipLaunchKernelGGL(k_mass_update<float>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, d, r, z, p, q,
+                           ctx->mass_dinv32, ctx->partials + kGatherPq, kGatherGrid, ctx->partials, sc);
+        hipLaunchKernelGGL(k_mass_direction<float>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, z, p,
+                           ctx->partials, sc);
+      }
+      FSX_TRY(launch_check(ctx, "mixed mass PCG"));
+      FSX_HIP(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+      FSX_HIP(hipStreamSynchronize(ctx->stream));
+      if (h.stop) break;
+      if (j > 2 * ctx->mass_maxit + 64) return fail(ctx, FSX_ESTATE, "mixed mass PCG did not terminate");
+      chunk = kMassChunk;
+    }
+    if (seg < kMassSegs) ctx->mass_seg_iters[seg] = h.iters - it0;
+    if (h.stop == 3) break;
+    // x += d, then the FP64 residual and stop test
+    hipLaunchKernelGGL(k_mass_fold, dim3((unsigned)std::min<long long>(((long long)n3 + kBlock - 1) / kBlock, 4096)),
+                       dim3(kBlock), 0, ctx->stream, (long long)n3, d, x);
+    mass_apply(ctx, true, x, q64, false, nullptr);
+    hipLaunchKernelGGL(k_mass_res64, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, NN, b, q64, r64, ctx->partials);
+    hipLaunchKernelGGL(k_mass_res_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partials, sc);
+    FSX_TRY(launch_check(ctx, "mixed mass residual"));
+    FSX_HIP(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    FSX_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return mass_finish(ctx, h);
+}
+
 // Solves M_bc x = b (3NN, node-major) to the context's mass tolerance; x
 // holds the previous solution of the same update (the warm start).
 int mass_solve(fsx_ctx* ctx, const double* b, double* x) {
@@ -395,6 +586,7 @@ int mass_solve(fsx_ctx* ctx, const double* b, double* x) {
                        ctx->Mrow, ctx->Mcol, ctx->M, ctx->bcflag, ctx->mass_dinv);
     FSX_TRY(launch_check(ctx, "k_mass_dinv"));
     ctx->mass_dinv_ready = true;
+    ctx->mass32_ready = false;   // the FP32 copy follows
   }
   const bool elements = ctx->mass_mode == FSX_MASS_CONSISTENT && !ctx->mass_m_uploaded;
   if (elements && !ctx->mass_fac_ready) FSX_TRY(mass_elements_setup(ctx));
@@ -410,28 +602,27 @@ int mass_solve(fsx_ctx* ctx, const double* b, double* x) {
   FSX_TRY(launch_check(ctx, "mass PCG start"));
   MassScalars h{};
   int j = 0;
+  if (elements && ctx->mass_bits == 32) return mass_solve_mixed(ctx, b, x, r, q, sc);
+  // the first chunk as long as the last solve, then kMassChunk iterations
+  // between polls (iterations past the stop are no-ops)
+  int chunk = std::max(kMassChunk, ctx->mass_last_iters);
   for (;;) {
-    for (int t = 0; t < kMassChunk; t++, j++) {
+    for (int t = 0; t < chunk; t++, j++) {
       mass_apply(ctx, elements, p, q, true, sc);
-      hipLaunchKernelGGL(k_mass_update, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, x, r, z, p, q,
+      hipLaunchKernelGGL(k_mass_update<double>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, x, r, z, p, q,
                          ctx->mass_dinv, ctx->partials + (elements ? kGatherPq : 6 * kRedGrid),
                          elements ? kGatherGrid : kRedGrid, ctx->partials, sc);
-      hipLaunchKernelGGL(k_mass_direction, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, z, p, ctx->partials, sc);
+      hipLaunchKernelGGL(k_mass_direction<double>, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, j, NN, z, p,
+                         ctx->partials, sc);
     }
     FSX_TRY(launch_check(ctx, "mass PCG"));
     FSX_HIP(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     FSX_HIP(hipStreamSynchronize(ctx->stream));
     if (h.stop) break;
-    if (j > ctx->mass_maxit + 2 * kMassChunk) return fail(ctx, FSX_ESTATE, "mass PCG did not terminate");
+    if (j > 2 * ctx->mass_maxit + 64) return fail(ctx, FSX_ESTATE, "mass PCG did not terminate");
+    chunk = kMassChunk;
   }
-  double rel = 0.0;
-  for (int c = 0; c < 3; c++) rel = std::max(rel, h.bb[c] > 0.0 ? std::sqrt(h.rr[c] / h.bb[c]) : 0.0);
-  ctx->mass_last_iters = h.iters;
-  ctx->mass_last_relres = rel;
-  ctx->mass_total_iters += h.iters;
-  if (h.stop == 3) return fail(ctx, FSX_ENUMERIC, "mass PCG met NaN/Inf");
-  if (h.stop == 2) { ctx->err = "mass PCG reached maxit"; return FSX_ENOCONV; }
-  return FSX_OK;
+  return mass_finish(ctx, h);
 }
 
 // step1's consistent-mass update: R1 is final, UnpHalf = Un + dt M_bc^-1 R1
@@ -481,6 +672,13 @@ extern "C" int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit) {
   ctx->mass_mode = mode;
   ctx->mass_rtol = rtol;
   ctx->mass_maxit = maxit;
+  return FSX_OK;
+}
+
+extern "C" int fsx_set_mass_precision(fsx_ctx* ctx, int bits) {
+  if (!ctx) return FSX_EINVAL;
+  if (bits != 64 && bits != 32) return fsx::fail(ctx, FSX_EINVAL, "fsx_set_mass_precision: 64 or 32");
+  ctx->mass_bits = bits;
   return FSX_OK;
 }
 

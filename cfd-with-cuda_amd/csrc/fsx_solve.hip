@@ -669,10 +669,24 @@ template <class T> struct PcgCfg;
 template <> struct PcgCfg<double> { static constexpr int LPR = 16, U = 8; };
 template <> struct PcgCfg<float> { static constexpr int LPR = 32, U = 4; };
 
+// pq[b] = the block's sum of v over its lanes, wavefronts in order (every
+// wavefront of the block calls it once)
+__device__ __forceinline__ void block_pq(double v, int b, double* __restrict__ pq) {
+  __shared__ double wsum[kBlock / kWave];
+  v = group_sum<kWave>(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kBlock / kWave; i++) t += wsum[i];
+    pq[b] = t;
+  }
+}
+
 template <class T, int LPR, int U, bool PAT>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restrict__ rp, ColSrc cs,
                                                      const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
-                                                     const SolveScalars* __restrict__ sc) {
+                                                     const SolveScalars* __restrict__ sc, double* __restrict__ pq) {
   if (sc && sc->stop) return;
   int l;
   const long long r = my_row<LPR>(&l);
@@ -695,21 +709,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int* __restric
   }
   if constexpr (sizeof(T) == 8) s = group_sum<LPR>(s); else s = group_sumf<LPR>(s);
   if (r < n && l == 0) q[r] = s;
+  // with pq: the block's p.q partial (its rows in order), so that no
+  // separate dot pass reads p and q again
+  if (pq) block_pq(r < n && l == 0 ? (double)p[r] * (double)s : 0.0, blockIdx.x, pq);
 }
 
-// pq[b] = the block's sum of v over its lanes, wavefronts in order (every
-// wavefront of the block calls it once)
-__device__ __forceinline__ void block_pq(double v, int b, double* __restrict__ pq) {
-  __shared__ double wsum[kBlock / kWave];
-  v = group_sum<kWave>(v);
-  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < kBlock / kWave; i++) t += wsum[i];
-    pq[b] = t;
-  }
-}
 
 // q = Z p on the FP32 row classes: row r sums the entries of its class,
 // (offset o, value a) pairs, as a * p[r + o], in entry order. The class table
@@ -793,7 +797,8 @@ static unsigned zcls_blocks(const fsx_ctx* ctx) {
 // q = Z p: FP64 on the solvers' rows; FP32 on the row classes, or on the
 // FP32 rounding of the same rows when they form no classes
 template <class T>
-static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc) {
+static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc, double* pq = nullptr,
+                         int* npq = nullptr) {
   const int n = ctx->NNp;
   if constexpr (sizeof(T) == 4) {
     if (ctx->Zc_chunk) {
@@ -815,10 +820,11 @@ static void launch_zspmv(fsx_ctx* ctx, const T* p, T* q, const SolveScalars* sc)
   // loads in flight per lane hide the per-row pattern lookup
   if (cs.row_pat)
     hipLaunchKernelGGL((k_pcg_spmv<T, 8, 16, true>), dim3(rows_grid(n, 8)), dim3(kBlock), 0, ctx->stream, n, rp, cs, val,
-                       p, q, sc);
+                       p, q, sc, pq);
   else
     hipLaunchKernelGGL((k_pcg_spmv<T, LPR, U, false>), dim3(rows_grid(n, LPR)), dim3(kBlock), 0, ctx->stream, n, rp, cs,
-                       val, p, q, sc);
+                       val, p, q, sc, pq);
+  if (npq) *npq = (int)(cs.row_pat ? rows_grid(n, 8) : rows_grid(n, LPR));
 }
 
 __global__ void k_to_f32(int n, const double* __restrict__ a, float* __restrict__ b) {
@@ -951,7 +957,8 @@ int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveSc
 // a 256-block grid); the captured solver iterations hold the buffer, so they
 // are dropped when it is reallocated.
 int ensure_fpq(fsx_ctx* ctx) {
-  const int need = std::max(ctx->Zc_chunk ? (int)zcls_blocks(ctx) : 0, kRedGrid);   // >= the SpMV's block count
+  // >= the SpMV's block count: row-class chunks, or CSR rows (32 lanes a row at most)
+  const int need = std::max({ctx->Zc_chunk ? (int)zcls_blocks(ctx) : 0, kRedGrid, (int)rows_grid(ctx->NNp, 32)});
   if (ctx->fpq_n >= need) return FSX_OK;
   if (ctx->fpq) dev_free(ctx, ctx->fpq);
   ctx->fpq = nullptr;
@@ -974,7 +981,7 @@ static void pcg_launch_iteration(fsx_ctx* ctx, int j, const T* dinv, T* x, T* r,
   int npq = kRedGrid;
   bool fused = false;
   if constexpr (sizeof(T) == 4) {
-    if (ctx->Zc_chunk && ctx->fpq) {   // the FP32 row-class SpMV writes p.q per chunk: no separate dot pass
+    if (ctx->fpq) {   // the FP32 SpMV (row classes or CSR rows) writes p.q per block: no separate dot pass
       (void)launch_pcg_spmv_pq_f32(ctx, p, q, sc, ctx->fpq, &npq);
       pqp = ctx->fpq;
       fused = true;
@@ -1256,15 +1263,6 @@ int launch_pcg_spmv_f32(fsx_ctx* ctx, const float* p, float* q, const SolveScala
   return FSX_OK;
 }
 
-// p.q partials of the flexible CG: one per block of a fixed 256-block grid
-__global__ __launch_bounds__(kBlock) void k_fpq_dot(int n, const float* __restrict__ p, const float* __restrict__ q,
-                                                    double* __restrict__ pq, const SolveScalars* __restrict__ sc) {
-  if (sc->stop) return;
-  double v[1] = {0.0};
-#pragma unroll 4   // independent elements: their loads in flight together
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRedGrid * kBlock) v[0] += (double)p[i] * (double)q[i];
-  write_partials<1>(v, pq);
-}
 
 // q = Z p in FP32 with the p.q partials in pq (*npq of them): from the
 // row-class SpMV itself (one per chunk), or from a separate pass when Z has
@@ -1277,9 +1275,7 @@ int launch_pcg_spmv_pq_f32(fsx_ctx* ctx, const float* p, float* q, const SolveSc
     *npq = (int)blocks;
     return FSX_OK;
   }
-  launch_zspmv<float>(ctx, p, q, sc);
-  hipLaunchKernelGGL(k_fpq_dot, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, p, q, pq, sc);
-  *npq = kRedGrid;
+  launch_zspmv<float>(ctx, p, q, sc, pq, npq);   // the block partials of p.q from the CSR SpMV itself
   return FSX_OK;
 }
 

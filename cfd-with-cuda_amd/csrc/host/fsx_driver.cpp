@@ -87,6 +87,17 @@ extern "C" void fsxd_free(fsxd_solver* s) {
   delete s;
 }
 
+// fsxd_run reports a failed call as the negative status (its positive
+// returns count the steps done)
+#define RUN_CHECK(call)                                                             \
+  do {                                                                              \
+    int _rc = (call);                                                               \
+    if (_rc != FSX_OK) {                                                            \
+      s->err = std::string(#call) + ": " + (s->ctx ? fsx_last_error(s->ctx) : ""); \
+      return -_rc;                                                                  \
+    }                                                                               \
+  } while (0)
+
 // BCH:2859-3118, one iteration of the while loop per time step.
 extern "C" int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats) {
   const fsxd_options& o = s->opt;
@@ -95,30 +106,30 @@ extern "C" int fsxd_run(fsxd_solver* s, int nsteps, fsxd_step_stats* stats) {
     double t0 = now_s();
     s->timeN++;
     s->t += s->dt;
-    DRV_CHECK(fsx_begin_step(s->ctx));                                      // BCH:2865-2881
+    RUN_CHECK(fsx_begin_step(s->ctx));                                      // BCH:2865-2881
     int iter, pcg_total = 0, it = 0, noconv = 0;
     double rel = 0.0, rel_max = 0.0;
     for (iter = 1; iter <= o.maxIter; iter++) {                            // BCH:2885
-      DRV_CHECK(fsx_convection(s->ctx));                                     // step1 (BCH:3702)
-      DRV_CHECK(fsx_step1(s->ctx, iter));
+      RUN_CHECK(fsx_convection(s->ctx));                                     // step1 (BCH:3702)
+      RUN_CHECK(fsx_step1(s->ctx, iter));
       int rc = fsx_step2(s->ctx, iter, &it, &rel);                           // BCH:2898
       if (rc != FSX_OK && rc != FSX_ENOCONV) { s->err = fsx_last_error(s->ctx); return -rc; }
       if (rc == FSX_ENOCONV) noconv++;                                       // counted, not hidden
       pcg_total += it;
       rel_max = std::max(rel_max, rel);
-      DRV_CHECK(fsx_step3(s->ctx, iter));                                    // BCH:2911
+      RUN_CHECK(fsx_step3(s->ctx, iter));                                    // BCH:2911
       int conv = 0;
       double ru, rp;
-      DRV_CHECK(fsx_check_convergence(s->ctx, o.tolerance, &conv, &ru, &rp));  // BCH:2925
+      RUN_CHECK(fsx_check_convergence(s->ctx, o.tolerance, &conv, &ru, &rp));  // BCH:2925
       if (conv) break;
-      DRV_CHECK(fsx_next_iteration(s->ctx));                                 // BCH:2928-2932
-      if (iter < o.maxIter) DRV_CHECK(fsx_kacc(s->ctx));                     // BCH:2982 (unused after the last iteration)
+      RUN_CHECK(fsx_next_iteration(s->ctx));                                 // BCH:2928-2932
+      if (iter < o.maxIter) RUN_CHECK(fsx_kacc(s->ctx));                     // BCH:2982 (unused after the last iteration)
     }
     int steady = 0;
     double maxAcc = 0.0;
-    DRV_CHECK(fsx_end_step(s->ctx, o.converge, &steady, &maxAcc));         // BCH:3046-3077
+    RUN_CHECK(fsx_end_step(s->ctx, o.converge, &steady, &maxAcc));         // BCH:3046-3077
     double mon[4] = {0, 0, 0, 0};
-    DRV_CHECK(fsx_monitor(s->ctx, s->topo->monPoint, mon));                // BCH:3088
+    RUN_CHECK(fsx_monitor(s->ctx, s->topo->monPoint, mon));                // BCH:3088
     double wall = now_s() - t0;
     if (stats) {
       fsxd_step_stats& st = stats[n];

@@ -37,7 +37,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
+    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_mass_precision", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_set_direct_order", "fsx_direct_info", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
@@ -179,6 +179,7 @@ def lib():
         "fsxd_read_restart": ([_vp, ctypes.c_char_p], ctypes.c_int),
         "fsxh_nd_order": ([ctypes.c_int, _I, _I, _D, ctypes.c_int, _I, _D], ctypes.c_int),
         "fsx_set_direct_order": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_set_mass_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_direct_info": ([_vp, _D, _D, _I, _I], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -489,6 +490,11 @@ class Solver:
         by = ctypes.c_double()
         self.check(self.L.fsx_direct_stats(self.ctx, ctypes.byref(w), ctypes.byref(ms), ctypes.byref(by)), "direct_stats")
         return w.value, ms.value, by.value
+
+    def set_mass_precision(self, bits):
+        """64 (default) or 32: the consistent-mass solves in FP32 segments
+        with FP64 residuals and stop test (the element form only)."""
+        self.check(self.L.fsx_set_mass_precision(self.ctx, bits), "set_mass_precision")
 
     def set_direct_order(self, order):
         """DIRECT_ND (the default: nested dissection, multifrontal factor) or

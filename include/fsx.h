@@ -142,6 +142,14 @@ int fsx_set_solver(fsx_ctx* ctx, int mode, double rtol, int maxit);
  * keeps the lumped MdOrigInv. No reference counterpart (the reference
  * assembles M, then lumps it, BCH:3195-3295). Needs the mesh uploaded. */
 int fsx_set_mass(fsx_ctx* ctx, int mode, double rtol, int maxit);
+/* Arithmetic of the consistent-mass solves (element form): 64 (default) or
+ * 32, the mixed forms': FP32 Jacobi-PCG segments on the correction, each
+ * starting from the FP64 residual b - M_bc x and ending at a 1e-4 reduction
+ * of its FP32 residual, then x += d and the FP64 residual, whose
+ * per-component test ||r_c|| <= rtol ||b_c|| decides convergence as in the
+ * FP64 solve. The CSR form (FSX_MASS_CONSISTENT_CSR, or M values uploaded)
+ * stays FP64. No reference counterpart (the reference lumps M). */
+int fsx_set_mass_precision(fsx_ctx* ctx, int bits);
 /* iterations (largest over the components is the count of the block
  * solve) and largest relative residual of the last mass solve, and the
  * iterations of all mass solves so far; any pointer may be NULL */
