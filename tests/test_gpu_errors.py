@@ -154,3 +154,26 @@ def test_z_and_m_uploads_rebuild_solver_forms(fsx_lib):
     assert np.array_equal(s.download("MdOrigInv"), md0)
     s.close()
     mesh.close()
+
+
+@pytest.mark.parametrize("mass", ["lumped", "consistent"])
+def test_time_loop_reports_failures(fsx_lib, mass):
+    """A failure inside the time loop reaches the caller: with NaN in Un the
+    first solve that meets it fails with FSX_ENUMERIC, and fsxd_run returns
+    that as a negative status (the Python run() raises it) instead of a step
+    count; the context then runs clean steps after a reset."""
+    say(f"time loop failure: cavity 8^3, {mass} mass, NaN in Un")
+    mesh, s = make(fsx_lib, MODES["fp64"])
+    if mass == "consistent":
+        s.set_mass(fsx_lib.MASS_CONSISTENT, rtol=1e-10, maxit=500)
+    un = s.download("Un")
+    un[len(un) // 2] = np.nan
+    s.upload("Un", un)
+    with pytest.raises(fsx_lib.FsxError) as e:
+        s.run(1)
+    assert e.value.code == fsx_lib.FSX_ENUMERIC, str(e.value)
+    s.reset(mode=MODES["fp64"])
+    st = s.run(2)
+    assert len(st) == 2 and all(x["pcg"] > 0 and x["relres"] <= 1e-10 for x in st)
+    assert np.isfinite(s.download("Un")).all()
+    s.close()
