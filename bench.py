@@ -364,7 +364,8 @@ def headline(args):
     # roofline entry is the one with the most device time per step in the
     # headline mode
     steps_inner = h["inner_iters"] / args.steps
-    solver.set_matrix_precision(32 if prec else 64)   # the headline mode's matrices
+    solver.set_matrix_precision(32 if prec else 64)   # the headline mode's matrices and convection
+    solver.set_convection_precision(32 if prec else 64)
     knames = step_kernels("float" if prec else "double")
     kern = {}
     # step1 of the mixed forms runs element by element (its K Acc_prev form in
