@@ -28,12 +28,13 @@
 //
 // Solve. Forward, from the deepest level: a front's vector is its pivots'
 // right-hand side plus its children's update vectors (k_mf_fwd_init, one
-// workgroup a front, children in slot order); y = X w_P (k_mf_fwd_x, column
+// workgroup a front, children in slot order; on levels of few fronts
+// k_mf_fwd_base and k_mf_fwd_child per slot, a lane an entry); y = X w_P (k_mf_fwd_x, column
 // chunks of 256 in separate workgroups, their partial sums kept), then the
 // update vector u = w_B - L21 y (k_mf_fwd_l21 sums y from those partials and
 // forms L21's partial sums by column chunk, k_mf_fwd_reduce adds them).
-// Backward, from the roots: y_P -= L21^T x_B (k_mf_bwd_bound, a wavefront a
-// column), then x_P = X^T y_P (k_mf_bwd_x, a wavefront a column of X). Each
+// Backward, from the roots: y_P -= L21^T x_B (k_mf_bwd_bound_lds, a wavefront a
+// column, x_B staged in LDS), then x_P = X^T y_P (k_mf_bwd_x, a wavefront a column of X). Each
 // level is six launches whatever its fronts' sizes, and every sum has a
 // fixed order: the solve is deterministic.
 #include <algorithm>
@@ -55,6 +56,8 @@ constexpr int kPanel = 64;      // factor panel width
 constexpr int kSB = 128;        // rows of a triangular-solve step (inverted diagonal blocks)
 constexpr int kTileLd = 66;     // LDS row length of the trailing-update tiles
 constexpr int kMaxGridY = 65535;
+constexpr size_t kMaxBoundLds = 128 * 1024;   // LDS bytes of a front's staged boundary values
+constexpr int kWideLevel = 64;                 // levels of at most this many fronts start their vectors wide
 
 // A front as the kernels read it (offsets in doubles or ints)
 struct MfFront {
@@ -426,6 +429,25 @@ __global__ __launch_bounds__(256) void k_mf_fwd_init(const MfFront* __restrict__
   }
 }
 
+// The same in three launches for levels of few, large fronts: the vectors'
+// start (b on the pivots, zeros on the boundary), then each child slot's
+// update vectors added, a 256-entry chunk a workgroup
+__global__ __launch_bounds__(256) void k_mf_fwd_base(const MfFront* __restrict__ F, int f0, const double* __restrict__ bp,
+                                                     double* __restrict__ W) {
+  const MfFront T = F[f0 + blockIdx.y];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < T.m) W[T.woff + k] = k < T.npiv ? bp[T.p0 + k] : 0.0;
+}
+__global__ __launch_bounds__(256) void k_mf_fwd_child(const MfFront* __restrict__ F, int f0, int slot,
+                                                      const int* __restrict__ cmap, double* __restrict__ W) {
+  const MfFront T = F[f0 + blockIdx.y];
+  const int c = T.child[slot];
+  if (c < 0) return;
+  const MfFront C = F[c];
+  const int a = blockIdx.x * 256 + threadIdx.x;
+  if (a < C.nb) W[T.woff + cmap[T.coff[slot] + a]] += W[C.woff + C.npiv + a];
+}
+
 // partial sums of y = X w_P: workgroup (row chunk, column chunk cc, front)
 // sums columns [256 cc, 256 cc + 256) of its 64 rows, 64 columns a
 // wavefront, the wavefronts' sums added in a fixed order; rows r >= 256 cc
@@ -537,6 +559,42 @@ __global__ __launch_bounds__(256) void k_mf_bwd_bound(const MfFront* __restrict_
       const bool ok = r < T.nb;
       a[u] = ok ? Lc[r] : 0.0;
       x[u] = ok ? X[bi[r]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc += a[u] * x[u];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) Y[T.p0 + j] -= acc;
+}
+
+// The same with the front's boundary values x(B) staged in LDS once per
+// workgroup of 16 wavefronts (16 columns), so that the column loop reads
+// L21 and LDS only (the boundary rows' values are gathered once, not once a
+// column); dynamic LDS of the level's largest boundary
+__global__ __launch_bounds__(1024) void k_mf_bwd_bound_lds(const MfFront* __restrict__ F, int f0,
+                                                           const double* __restrict__ L, const int* __restrict__ bidx,
+                                                           const double* __restrict__ X, double* __restrict__ Y) {
+  extern __shared__ double xb[];
+  const MfFront T = F[f0 + blockIdx.y];
+  const int j0 = blockIdx.x * 16;
+  if (j0 >= T.npiv || T.nb == 0) return;
+  const int* __restrict__ bi = bidx + T.boff;
+  for (int r = threadIdx.x; r < T.nb; r += 1024) xb[r] = X[bi[r]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int j = j0 + (threadIdx.x >> 6);
+  if (j >= T.npiv) return;
+  const double* __restrict__ Lc = L + T.loff + (long long)j * T.m + T.npiv;
+  double acc = 0.0;
+  for (int r0 = 0; r0 < T.nb; r0 += 4 * 64) {
+    double a[4], x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = r0 + u * 64 + lane;
+      const bool ok = r < T.nb;
+      a[u] = ok ? Lc[r] : 0.0;
+      x[u] = ok ? xb[r] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) acc += a[u] * x[u];
@@ -871,9 +929,21 @@ int mf_apply(fsx_ctx* ctx, const double* b, double* x) {
   hipLaunchKernelGGL(k_mf_permute, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, S->perm, b, S->bp);
   for (int d = S->depth; d >= 0; d--) {
     const MfLevel& L = S->lev[d];
-    over_y(L.nf, [&](int z0, int nz) {
-      hipLaunchKernelGGL(k_mf_fwd_init, dim3(nz), dim3(256), 0, ctx->stream, S->F, L.f0 + z0, S->cmap, S->bp, S->W);
-    });
+    if (L.nf <= kWideLevel) {   // few, large fronts: every entry its own lane
+      hipLaunchKernelGGL(k_mf_fwd_base, dim3((unsigned)((L.maxm + 255) / 256), L.nf), dim3(256), 0, ctx->stream, S->F,
+                         L.f0, S->bp, S->W);
+      if (d < S->depth)
+        for (int slot = 0; slot < 2; slot++) {
+          const int nbc = S->lev[d + 1].maxnb_child[slot];
+          if (nbc > 0)
+            hipLaunchKernelGGL(k_mf_fwd_child, dim3((unsigned)((nbc + 255) / 256), L.nf), dim3(256), 0, ctx->stream,
+                               S->F, L.f0, slot, S->cmap, S->W);
+        }
+    } else {
+      over_y(L.nf, [&](int z0, int nz) {
+        hipLaunchKernelGGL(k_mf_fwd_init, dim3(nz), dim3(256), 0, ctx->stream, S->F, L.f0 + z0, S->cmap, S->bp, S->W);
+      });
+    }
     if (L.maxpiv > 0) {
       const unsigned rch = (unsigned)((L.maxpiv + 63) / 64), cch = (unsigned)((L.maxpiv + 255) / 256);
       const unsigned bch = (unsigned)std::max(1, (L.maxnb + 63) / 64);
@@ -893,7 +963,10 @@ int mf_apply(fsx_ctx* ctx, const double* b, double* x) {
     const MfLevel& L = S->lev[d];
     if (L.maxpiv == 0) continue;
     over_y(L.nf, [&](int y0, int ny) {
-      if (L.maxnb > 0)
+      if (L.maxnb > 0 && (size_t)L.maxnb * sizeof(double) <= kMaxBoundLds)
+        hipLaunchKernelGGL(k_mf_bwd_bound_lds, dim3((L.maxpiv + 15) / 16, ny), dim3(1024),
+                           (size_t)L.maxnb * sizeof(double), ctx->stream, S->F, L.f0 + y0, S->L, S->bidx, S->X, S->Y);
+      else if (L.maxnb > 0)
         hipLaunchKernelGGL(k_mf_bwd_bound, dim3((L.maxpiv + 3) / 4, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0,
                            S->L, S->bidx, S->X, S->Y);
       hipLaunchKernelGGL(k_mf_bwd_x, dim3((L.maxpiv + 3) / 4, ny), dim3(256), 0, ctx->stream, S->F, L.f0 + y0, S->Xs,
