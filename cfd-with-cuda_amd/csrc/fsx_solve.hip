@@ -1583,10 +1583,13 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
       bytes = 4.0 * (NN + 1) + (27.0 * 4 + 27.0 * 6 * 4) * NE + (24.0 + 24.0 + 8.0 + 1.0 + 8.0) * NN +
               (24.0 + 24.0 + 24.0 + 12.0 + 12.0) * NN;
       break;
-    default:                 // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc (+ its FP32 shadow), Unp1
+    default: {               // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc (+ its FP32 shadow), Unp1; with the
+                             // element form of step1 also G Pn read and G Pnp1 written (the carry)
+      const bool carry = ctx->Gf && ctx->gpn_valid && ctx->use_step1_elements && ctx->GPn_next;
       bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +
-              24.0 * NN + (ctx->Gf && ctx->rec_mem ? 12.0 * NN : 0.0);
+              24.0 * NN + (ctx->Gf && ctx->rec_mem ? 12.0 * NN : 0.0) + (carry ? 48.0 * NN : 0.0);
       break;
+    }
   }
   FSX_TRY(settle_deferred(ctx));
   if (kernel == FSX_KERNEL_STEP1_NODES) FSX_TRY(launch_elem_pass(ctx, true));   // the rows the node pass sums
