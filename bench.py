@@ -55,11 +55,15 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK = 157.3   # TFLOP/s, dense
 
 
-def step_kernels(vt):
+def step_kernels(vt, classes=False):
     """rocprofv3 names of the step's row kernels for matrix values of type vt
     ("double", or "float" in the mixed forms, whose step1 also gathers K's
     operands from their FP32 shadows): step1 with K*Acc_prev fused and G*Pn
-    kept (inner iterations >= 2), step2's right-hand side, step3."""
+    kept (inner iterations >= 2), step2's right-hand side, step3; `classes`:
+    the mixed forms read G and G^T as row classes."""
+    if vt == "float" and classes:
+        return ("k_step1<8, 2, 2, true, true, 2, float, true>", "k_step2_rhs<16, 4, float, true, true>",
+                "k_step3<4, 2, float, true>")
     if vt == "float":
         return ("k_step1<8, 2, 2, true, true, 2, float, true>", "k_step2_rhs<32, 1, float, true>",
                 "k_step3<4, 1, float>")
@@ -366,7 +370,8 @@ def headline(args):
     steps_inner = h["inner_iters"] / args.steps
     solver.set_matrix_precision(32 if prec else 64)   # the headline mode's matrices and convection
     solver.set_convection_precision(32 if prec else 64)
-    knames = step_kernels("float" if prec else "double")
+    gcls = solver.g_class_info()
+    knames = step_kernels("float" if prec else "double", classes=bool(prec and gcls[0] > 0))
     kern = {}
     # step1 of the mixed forms runs element by element (its K Acc_prev form in
     # inner iterations >= 2): its two passes instead of the CSR k_step1
@@ -418,7 +423,8 @@ def headline(args):
                                f"pressure rtol {args.rtol:g}, pressure x0 {args.x0}, timed steps "
                                f"{args.warmup + 1}-{args.warmup + args.steps} of {args.trajectory}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
-                   "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU"},
+                   "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU",
+                   "g_classes": dict(zip(("G", "G_entries", "GT", "GT_entries"), gcls)) if prec else None},
         "roofline": roof,
         "kernels": {k: {"launch_ms": round(v["launch_ms"], 5), "algorithmic_bytes": v["bytes"],
                         "achieved_GBs": round(v["bytes"] / (v["launch_ms"] * 1e-3) / 1e9, 1),

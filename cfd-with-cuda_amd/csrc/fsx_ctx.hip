@@ -270,6 +270,23 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits) {
   return refresh_matrix_copies(ctx);
 }
 
+int fsx_set_matrix_classes(fsx_ctx* ctx, int on) {
+  if (!ctx) return FSX_EINVAL;
+  ctx->use_g_classes = on != 0;
+  ctx->gpn_valid = ctx->gpn_next_valid = false;
+  return refresh_matrix_copies(ctx);
+}
+
+int fsx_g_class_info(fsx_ctx* ctx, int* classes, long long* entries, int* t_classes, long long* t_entries) {
+  if (!ctx) return FSX_EINVAL;
+  const bool on = g_classes_on(ctx);
+  if (classes) *classes = on ? ctx->gc_ncls : 0;
+  if (entries) *entries = on ? ctx->gc_nent : 0;
+  if (t_classes) *t_classes = on ? ctx->gtc_ncls : 0;
+  if (t_entries) *t_entries = on ? ctx->gtc_nent : 0;
+  return FSX_OK;
+}
+
 int fsx_set_solver_x0(fsx_ctx* ctx, int policy) {
   if (!ctx) return FSX_EINVAL;
   if (policy != FSX_X0_ZERO && policy != FSX_X0_PREVIOUS) return fail(ctx, FSX_EINVAL, "fsx_set_solver_x0: unknown policy");
@@ -777,6 +794,7 @@ int fsx_assemble_static(fsx_ctx* ctx) {
     return fail(ctx, FSX_ESTATE, "fsx_assemble_static needs mesh, M and G patterns, BCs and parameters");
   FSX_TRY(launch_assemble_static(ctx));
   ctx->assembled = true;
+  ctx->gc_built = false;   // G's classes are formed again from the new values
   ctx->gpn_valid = ctx->gpn_next_valid = false;
   FSX_TRY(refresh_matrix_copies(ctx));
   ctx->mass_dinv_ready = false;

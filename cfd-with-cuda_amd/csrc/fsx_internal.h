@@ -233,6 +233,18 @@ struct fsx_ctx {
   // precision is 32 (fsx_set_matrix_precision); null at precision 64
   float* Kf = nullptr; float* Gf = nullptr; float* GTf = nullptr;
   float *Kf_mem = nullptr, *Gf_mem = nullptr, *GTf_mem = nullptr;   // their allocations
+  // G and G^T as row classes (fsx_gclass.h) beside the FP32 copies, where the
+  // rows compress and fsx_set_matrix_classes allows them; null otherwise.
+  // step3 and step2's right-hand side read these instead of Gf / GTf.
+  int2* Gc_row = nullptr;      // per row of G: (first column, class entry start << 8 | length)
+  float4* Gc_ent = nullptr;    // per class entry: (column offset bits, G1, G2, G3)
+  int* GTc_info = nullptr;     // per row of G^T: class entry start << 8 | length
+  int* GTc_col = nullptr;      // G^T's columns in class order, row p from GTrow[p]
+  float4* GTc_val = nullptr;   // per class entry: (G1, G2, G3, 0)
+  int gc_ncls = 0, gtc_ncls = 0;
+  long long gc_nent = 0, gtc_nent = 0;
+  bool use_g_classes = true;
+  bool gc_built = false;       // the classes (or their absence) are of the current assembly of G
   int mat_bits = 64;
   int conv_bits = 64;            // arithmetic of the convection kernel (fsx_set_convection_precision)
   // FP32 shadows of UnpHalf and Acc (and of their "prev" buffers), interleaved
@@ -470,6 +482,8 @@ void mf_invalidate(fsx_ctx* ctx);
 int mf_info(const fsx_ctx* ctx, double* l_bytes, double* flops, int* fronts, int* depth, double* factor_ms);
 int launch_nodes_pass(fsx_ctx* ctx, bool kacc, bool gload);
 int refresh_matrix_copies(fsx_ctx* ctx);
+// whether step3 and step2's right-hand side read G and G^T as row classes
+inline bool g_classes_on(const fsx_ctx* ctx) { return ctx->Gf && ctx->GTf && ctx->Gc_row && ctx->use_g_classes; }
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);
 int elements_setup(fsx_ctx* ctx);
 int mass_update_step1(fsx_ctx* ctx);

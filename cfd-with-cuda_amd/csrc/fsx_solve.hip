@@ -13,11 +13,14 @@
 // a fixed order, in every block (fsx_device.h). Results are bitwise
 // reproducible.
 #include "fsx_device.h"
+#include "fsx_gclass.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 namespace fsx {
 
@@ -278,9 +281,78 @@ __global__ void k_d2f(long long n, const double* __restrict__ a, float* __restri
     b[i] = (float)a[i];
 }
 
+// G and G^T as row classes (fsx_gclass.h) from the FP64 assembly, when the
+// matrix precision is 32 and the rows compress. A component below 2^-24 of
+// its row's largest magnitude is dropped: on congruent elements such
+// components are the rounding residue of contributions that cancel, and
+// their differences alone would keep rows that are otherwise equal apart.
+// The drop is below FP32's resolution of the row's largest term.
+constexpr double kGClassPrune = 5.9604644775390625e-08;   // 2^-24
+constexpr int kGClassMax = 65535;
+constexpr long long kGClassMaxEnt = 1LL << 22;   // class entry starts fit 23 bits
+constexpr int kGClassMaxLen = 255;               // row lengths fit 8 bits
+
+static int build_g_classes(fsx_ctx* ctx) {
+  // built once per assembly of G, when first wanted; kept while the
+  // precision or the switch changes (g_classes_on decides their use)
+  if (!ctx->use_g_classes || !ctx->Gf || ctx->gc_built) return FSX_OK;
+  for (void* p : {(void*)ctx->Gc_row, (void*)ctx->Gc_ent, (void*)ctx->GTc_info, (void*)ctx->GTc_col, (void*)ctx->GTc_val})
+    if (p) dev_free(ctx, p);
+  ctx->Gc_row = nullptr; ctx->Gc_ent = nullptr; ctx->GTc_info = nullptr; ctx->GTc_col = nullptr; ctx->GTc_val = nullptr;
+  ctx->gc_ncls = ctx->gtc_ncls = 0;
+  ctx->gc_nent = ctx->gtc_nent = 0;
+  ctx->gc_built = true;
+  const int nn = ctx->NN, np = ctx->NNp;
+  const long long nnz = ctx->nnzG;
+  std::vector<int> rp(nn + 1), ci(nnz), trp(np + 1);
+  std::vector<double> g(3 * (size_t)nnz);
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_HIP(hipMemcpy(rp.data(), ctx->Grow, (size_t)(nn + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(ci.data(), ctx->Gcol, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(g.data(), ctx->G, 3 * (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost));
+  FSX_HIP(hipMemcpy(trp.data(), ctx->GTrow, (size_t)(np + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  GClassForm F;
+  // the form pays where the class lists are a small part of the operator:
+  // at most an eighth of its entries
+  const int rc = g_class_form(nn, np, rp.data(), ci.data(), g.data(), kGClassPrune, kGClassMax,
+                              std::min(kGClassMaxEnt, nnz / 8), kGClassMaxLen, &F);
+  if (rc < 0) return fail(ctx, FSX_EINVAL, "G's pattern is inconsistent");
+  if (rc > 0 || F.trow_ptr != trp) return FSX_OK;   // no compression: the step kernels read Gf / GTf
+  std::vector<int2> grow(nn);
+  for (int r = 0; r < nn; r++)
+    grow[r] = make_int2(F.row_base[r], (F.cls_ptr[F.row_cls[r]] << 8) | (rp[r + 1] - rp[r]));
+  const size_t ne = F.ent_off.size();
+  std::vector<float4> gent(ne);
+  for (size_t t = 0; t < ne; t++) {
+    float off;
+    std::memcpy(&off, &F.ent_off[t], sizeof off);
+    gent[t] = make_float4(off, F.ent_val[3 * t], F.ent_val[3 * t + 1], F.ent_val[3 * t + 2]);
+  }
+  std::vector<int> tinfo(np);
+  for (int p = 0; p < np; p++) tinfo[p] = (F.tcls_ptr[F.trow_cls[p]] << 8) | (trp[p + 1] - trp[p]);
+  const size_t nte = F.tent_val.size() / 3;
+  std::vector<float4> tval(nte);
+  for (size_t t = 0; t < nte; t++) tval[t] = make_float4(F.tent_val[3 * t], F.tent_val[3 * t + 1], F.tent_val[3 * t + 2], 0.0f);
+  FSX_TRY(alloc(ctx, &ctx->Gc_row, (size_t)nn));
+  FSX_TRY(alloc(ctx, &ctx->Gc_ent, std::max<size_t>(1, ne)));
+  FSX_TRY(alloc(ctx, &ctx->GTc_info, (size_t)np));
+  FSX_TRY(alloc(ctx, &ctx->GTc_col, std::max<size_t>(1, (size_t)nnz)));
+  FSX_TRY(alloc(ctx, &ctx->GTc_val, std::max<size_t>(1, nte)));
+  FSX_HIP(hipMemcpy(ctx->Gc_row, grow.data(), (size_t)nn * sizeof(int2), hipMemcpyHostToDevice));
+  if (ne) FSX_HIP(hipMemcpy(ctx->Gc_ent, gent.data(), ne * sizeof(float4), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(ctx->GTc_info, tinfo.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+  if (nnz) FSX_HIP(hipMemcpy(ctx->GTc_col, F.tcol.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+  if (nte) FSX_HIP(hipMemcpy(ctx->GTc_val, tval.data(), nte * sizeof(float4), hipMemcpyHostToDevice));
+  ctx->gc_ncls = (int)F.cls_ptr.size() - 1;
+  ctx->gc_nent = (long long)ne;
+  ctx->gtc_ncls = (int)F.tcls_ptr.size() - 1;
+  ctx->gtc_nent = (long long)nte;
+  return FSX_OK;
+}
+
 // The step kernels' FP32 copies of K, G and G^T (rounded to nearest) at
-// matrix precision 32, after assembly or a K upload; at precision 64 the
-// kernels read the FP64 values.
+// matrix precision 32, after assembly or a K upload, and G's row classes
+// where they form; at precision 64 the kernels read the FP64 values.
 int refresh_matrix_copies(fsx_ctx* ctx) {
   ctx->Kf = ctx->Gf = ctx->GTf = nullptr;
   drop_records(ctx);
@@ -305,7 +377,7 @@ int refresh_matrix_copies(fsx_ctx* ctx) {
   hipLaunchKernelGGL(k_d2f, dim3(4096), dim3(kBlock), 0, ctx->stream, 3 * ctx->nnzG, ctx->GT, ctx->GTf_mem);
   FSX_TRY(launch_check(ctx, "k_d2f"));
   ctx->Kf = ctx->Kf_mem; ctx->Gf = ctx->Gf_mem; ctx->GTf = ctx->GTf_mem;
-  return FSX_OK;
+  return build_g_classes(ctx);
 }
 
 // FP32 shadow of a node-major velocity vector (zeros for a null v): node i
@@ -404,17 +476,22 @@ int settle_deferred(fsx_ctx* ctx) {
 // R2[p0] = 0 for p0 > 0 (BCH:4180-4184).
 // With DREC the operand d is gathered as step1's FP32 shadow of it (16 bytes
 // a column instead of 48 plus MdOrigInv); sums stay FP64.
-template <int LPR, int U, class VT, bool DREC>
+template <int LPR, int U, class VT, bool DREC, bool CLS = false>
 __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int* __restrict__ GTrow,
                                                       const int* __restrict__ GTcol, const VT* __restrict__ GT,
                                                       const double* __restrict__ UnpHalf, const double* __restrict__ MdOrigInv,
                                                       const double* __restrict__ KAcc, int use_kacc, double o2,
-                                                      int zeroP, double* __restrict__ R2, const float4* __restrict__ Drec) {
+                                                      int zeroP, double* __restrict__ R2, const float4* __restrict__ Drec,
+                                                      const int* __restrict__ GTc_info = nullptr,
+                                                      const float4* __restrict__ GTc_val = nullptr) {
   int l;
   const long long p = my_row<LPR>(&l);
   double s0 = 0, s1 = 0, s2 = 0;
   if (p < NNp) {
     const int rs = GTrow[p], re = GTrow[p + 1];
+    // class form: the row's columns in its class's order (GTcol is then the
+    // class-ordered column array), values from the class entries
+    const int cs = CLS ? (GTc_info[p] >> 8) - rs : 0;
     for (int base = rs + l; base < re; base += U * LPR) {
       int c[U];
       double a0[U], a1[U], a2[U];
@@ -423,9 +500,16 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
         c[j] = GTcol[kk];
         const bool ok = k < re;
-        a0[j] = ok ? (double)GT[3 * (size_t)kk] : 0.0;
-        a1[j] = ok ? (double)GT[3 * (size_t)kk + 1] : 0.0;
-        a2[j] = ok ? (double)GT[3 * (size_t)kk + 2] : 0.0;
+        if constexpr (CLS) {
+          const float4 v = GTc_val[cs + kk];
+          a0[j] = ok ? (double)v.x : 0.0;
+          a1[j] = ok ? (double)v.y : 0.0;
+          a2[j] = ok ? (double)v.z : 0.0;
+        } else {
+          a0[j] = ok ? (double)GT[3 * (size_t)kk] : 0.0;
+          a1[j] = ok ? (double)GT[3 * (size_t)kk + 1] : 0.0;
+          a2[j] = ok ? (double)GT[3 * (size_t)kk + 2] : 0.0;
+        }
       }
 #pragma unroll
       for (int j = 0; j < U; j++) {
@@ -460,10 +544,21 @@ int launch_step2_rhs(fsx_ctx* ctx, int iter) {
   const float4* dr = reinterpret_cast<const float4*>(ctx->Drec);
   // with the FP32 operand, 32 lanes x 1 load a row: 122.7 us at 64^3 (32x2
   // 131.6, 16x1 126.4, 16x2 134.3, 32x4 139.6, 16x4 142.8, 16x8 161.0)
-  if (drec)
+  const bool cls = g_classes_on(ctx);
+  // G^T as row classes with the FP32 operand: 16 lanes x 4 entries a round,
+  // 54.2 us at 64^3 (16x2 55.8, 8x4 56.1, 32x2 64.1, 32x1 76.1, 64x2 107.5)
+  if (drec && cls)
+    hipLaunchKernelGGL((k_step2_rhs<16, 4, float, true, true>), dim3(rows_grid(ctx->NNp, 16)), dim3(kBlock), 0,
+                       ctx->stream, ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTc_col, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv,
+                       ctx->KAcc, use_kacc, o2, ctx->zeroP, ctx->R2, dr, ctx->GTc_info, ctx->GTc_val);
+  else if (drec)
     hipLaunchKernelGGL((k_step2_rhs<32, 1, float, true>), dim3(rows_grid(ctx->NNp, 32)), dim3(kBlock), 0, ctx->stream,
                        ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
                        use_kacc, o2, ctx->zeroP, ctx->R2, dr);
+  else if (cls)
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, false, true>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0,
+                       ctx->stream, ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTc_col, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv,
+                       ctx->KAcc, use_kacc, o2, ctx->zeroP, ctx->R2, dr, ctx->GTc_info, ctx->GTc_val);
   else if (ctx->GTf)
     hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, false>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0, ctx->stream,
                        ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
@@ -521,7 +616,7 @@ int launch_pnp1(fsx_ctx* ctx) {
 // ------------------------------------------------------------ step 3
 // R3_c = -dt G_c Pdot - [iter>1] dt KAcc_c; BC rows 0; Acc = R3 MdInv;
 // Unp1 = UnpHalf + dt Acc (BCH:3912-3974).
-template <int LPR, int U, class VT>
+template <int LPR, int U, class VT, bool CLS = false>
 __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict__ Grow, const int* __restrict__ Gcol,
                                                   const VT* __restrict__ G, const double* __restrict__ Pdot,
                                                   const double* __restrict__ KAcc, int use_kacc,
@@ -529,11 +624,44 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
                                                   const double* __restrict__ UnpHalf, double* __restrict__ Acc,
                                                   double* __restrict__ Unp1, double dt, double* __restrict__ Rout,
                                                   float* __restrict__ Aout, const double* __restrict__ GPn,
-                                                  double* __restrict__ GPnext) {
+                                                  double* __restrict__ GPnext, const int2* __restrict__ Gc_row = nullptr,
+                                                  const float4* __restrict__ Gc_ent = nullptr) {
   int l;
   const long long r = my_row<LPR>(&l);
+  const bool out = r < NN && l < 3;
+  const size_t i = 3 * (size_t)r + l;
+  // the epilogue's operands, loaded before the row's gather so that their
+  // latency overlaps it
+  double gpn = 0.0, kacc = 0.0, uh = 0.0, m = 0.0;
+  bool isbc = false;
+  if (out) {
+    if (GPnext) gpn = GPn[i];
+    if (use_kacc) kacc = KAcc[i];
+    isbc = bc[r] != 0;
+    if (!Rout) { m = MdInv[r]; uh = UnpHalf[i]; }
+  }
   double g0 = 0, g1 = 0, g2 = 0;
-  if (r < NN) {
+  if (CLS && r < NN) {
+    // class form: entry k of the row is class entry start + k, at column
+    // base + the entry's offset; the same lanes take the same entries as
+    // the CSR loop below
+    const int2 info = Gc_row[r];
+    const int cs = info.y >> 8, len = info.y & 255;
+    for (int base = l; base < len; base += U * LPR) {
+      float4 e[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const int k = base + j * LPR;
+        e[j] = Gc_ent[cs + (k < len ? k : len - 1)];
+        if (k >= len) e[j].y = e[j].z = e[j].w = 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const double p = Pdot[info.x + __float_as_int(e[j].x)];
+        g0 += (double)e[j].y * p; g1 += (double)e[j].z * p; g2 += (double)e[j].w * p;
+      }
+    }
+  } else if (r < NN) {
     const int rs = Grow[r], re = Grow[r + 1];
     for (int base = rs + l; base < re; base += U * LPR) {
       int c[U];
@@ -555,20 +683,19 @@ __global__ __launch_bounds__(kBlock) void k_step3(int NN, const int* __restrict_
     }
   }
   g0 = group_sum<LPR>(g0); g1 = group_sum<LPR>(g1); g2 = group_sum<LPR>(g2);
-  if (r < NN && l < 3) {
+  if (out) {
     const double gg = l == 0 ? g0 : (l == 1 ? g1 : g2);
-    const size_t i = 3 * (size_t)r + l;
     // G Pnp1 = G Pn + dt G Pdot (Pnp1 = Pn + dt Pdot, k_pnp1): the next
     // time step's G Pn when this is the step's last inner iteration
-    if (GPnext) GPnext[i] = GPn[i] + dt * gg;
+    if (GPnext) GPnext[i] = gpn + dt * gg;
     double R = -dt * gg;
-    if (use_kacc) R = R - dt * KAcc[i];
-    if (bc[r]) R = 0.0;
+    if (use_kacc) R = R - dt * kacc;
+    if (isbc) R = 0.0;
     if (Rout) { Rout[i] = R; return; }   // consistent mass: the caller solves for Acc
-    const double a = R * MdInv[r];
+    const double a = R * m;
     Acc[i] = a;
     if (Aout) Aout[8 * (size_t)r + l] = (float)a;   // the FP32 shadow step1 reads
-    Unp1[i] = UnpHalf[i] + dt * a;
+    Unp1[i] = uh + dt * a;
   }
 }
 
@@ -586,7 +713,13 @@ static void launch_step3_rows(fsx_ctx* ctx, int iter, double* Rout = nullptr) {
   double* gnext = carry ? ctx->GPn_next : nullptr;
   // FP32 G values: 4 lanes x 1 load a row, 225.7 us at 64^3 (4x2 237.5, 8x1
   // 245.9, 8x2 252.6, 4x3 254.6, 8x4 261.7, 16x1 347.0)
-  if (ctx->Gf)
+  // G as row classes: 4 lanes x 2 entries a round, 142.5 us at 64^3 (4x3
+  // 142.0, 4x4 147.9, 4x1 153.8, 8x2 191.1, 8x1 199.7)
+  if (g_classes_on(ctx))
+    hipLaunchKernelGGL((k_step3<4, 2, float, true>), dim3(rows_grid(ctx->NN, 4)), dim3(kBlock), 0, ctx->stream, ctx->NN,
+                       ctx->Grow, ctx->Gcol, ctx->Gf, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
+                       ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout, gpn, gnext, ctx->Gc_row, ctx->Gc_ent);
+  else if (ctx->Gf)
     hipLaunchKernelGGL((k_step3<4, 1, float>), dim3(rows_grid(ctx->NN, 4)), dim3(kBlock), 0, ctx->stream, ctx->NN,
                        ctx->Grow, ctx->Gcol, ctx->Gf, ctx->Pdot, ctx->KAcc, iter != 1 ? 1 : 0, ctx->bcflag, ctx->MdInv,
                        ctx->UnpHalf, ctx->Acc, ctx->Unp1, ctx->dt, Rout, aout, gpn, gnext);
@@ -1566,12 +1699,16 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
     case FSX_KERNEL_KACC:    // K, Acc_prev, KAcc
       bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
       break;
-    case FSX_KERNEL_STEP2_RHS:   // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2; or G^T, d's FP32 shadow, R2
+    case FSX_KERNEL_STEP2_RHS: {  // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2; or G^T, d's FP32 shadow,
+                                  // R2. G^T as row classes: its columns, each row's class word and the class
+                                  // values (16 bytes an entry, once) in place of the values
+      const double gt = g_classes_on(ctx) ? 4.0 * nG + 4.0 * NNp + 16.0 * (double)ctx->gtc_nent : (4.0 + 3.0 * vb) * nG;
       if (ctx->GTf && ctx->drec_ok && ctx->drec_kacc)
-        bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 16.0 * NN + 8.0 * NNp;
+        bytes = gt + 4.0 * (NNp + 1) + 16.0 * NN + 8.0 * NNp;
       else
-        bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
+        bytes = gt + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
       break;
+    }
     case FSX_KERNEL_STEP1_ELEMENTS:   // per element its 27 node ids, corner offsets (24 floats) and rows
                                       // (27 x 6 floats); per node the FP32 records of Un (16 bytes) and of
                                       // UnpHalf_prev with Acc_prev (32)
@@ -1584,9 +1721,12 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
               (24.0 + 24.0 + 24.0 + 12.0 + 12.0) * NN;
       break;
     default: {               // G, Pdot, KAcc, bc, MdInv, UnpHalf, Acc (+ its FP32 shadow), Unp1; with the
-                             // element form of step1 also G Pn read and G Pnp1 written (the carry)
+                             // element form of step1 also G Pn read and G Pnp1 written (the carry). G as
+                             // row classes: each row's first column and class word (8 bytes) and the
+                             // class entries (16 bytes each, once) in place of its columns and values
       const bool carry = ctx->Gf && ctx->gpn_valid && ctx->use_step1_elements && ctx->GPn_next;
-      bytes = (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1) + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +
+      const double g = g_classes_on(ctx) ? 8.0 * NN + 16.0 * (double)ctx->gc_nent : (4.0 + 3.0 * vb) * nG + 4.0 * (NN + 1);
+      bytes = g + 8.0 * NNp + 24.0 * NN + 1.0 * NN + 8.0 * NN + 24.0 * NN + 24.0 * NN +
               24.0 * NN + (ctx->Gf && ctx->rec_mem ? 12.0 * NN : 0.0) + (carry ? 48.0 * NN : 0.0);
       break;
     }

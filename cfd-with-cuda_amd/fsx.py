@@ -37,7 +37,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_mass_precision", "fsx_set_matrix_precision", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
+    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_mass_precision", "fsx_set_matrix_precision", "fsx_set_matrix_classes", "fsx_g_class_info", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_set_direct_order", "fsx_direct_info", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
@@ -49,7 +49,7 @@ HOST_SYMBOLS = (
     "fsxh_topology", "fsxh_topo_free", "fsxh_topo_info", "fsxh_topo_len", "fsxh_topo_get",
     "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_reset", "fsxd_ctx", "fsxd_topo", "fsxd_error",
     "fsxd_setup_times", "fsxh_write_tecplot", "fsxh_read_restart", "fsxd_write_tecplot", "fsxd_read_restart",
-    "fsxh_nd_order",
+    "fsxh_nd_order", "fsxh_g_classes",
 )
 
 
@@ -181,6 +181,10 @@ def lib():
         "fsx_set_direct_order": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_set_mass_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_direct_info": ([_vp, _D, _D, _I, _I], ctypes.c_int),
+        "fsx_set_matrix_classes": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_g_class_info": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(ctypes.c_longlong)],
+                             ctypes.c_int),
+        "fsxh_g_classes": ([ctypes.c_int, ctypes.c_int, _I, _I, _D, ctypes.c_double, _I, _I, _I, _F], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -201,6 +205,30 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(_I)
+
+
+def g_classes(rp, ci, g3, ncols, prune_rel=2.0**-24):
+    """Row classes of G (fsxh_g_classes): G's CSR pattern (rp, ci) with
+    g3 = (G1, G2, G3) a row of an (nnz, 3) array over ncols columns. Returns
+    None when the rows do not compress, else a dict: counts (G's classes,
+    their entries, G^T's classes, theirs), row_cls, trow_cls and g3_f32, the
+    (nnz, 3) FP32 values the class form represents (NaN where G^T's classes
+    disagree with G's)."""
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    g3 = np.ascontiguousarray(g3, dtype=np.float64).reshape(-1, 3)
+    n = rp.size - 1
+    cnt = np.zeros(4, dtype=np.int32)
+    rc_ = np.zeros(n, dtype=np.int32)
+    tc = np.zeros(int(ncols), dtype=np.int32)
+    g32 = np.zeros(g3.shape, dtype=np.float32)
+    rc = lib().fsxh_g_classes(n, int(ncols), _ip(rp), _ip(ci), _dp(g3), float(prune_rel), _ip(cnt), _ip(rc_),
+                              _ip(tc), _fp(g32))
+    if rc != FSX_OK:
+        raise FsxError(rc, "fsxh_g_classes: inconsistent pattern")
+    if cnt[0] < 0:
+        return None
+    return dict(counts=tuple(int(c) for c in cnt), row_cls=rc_, trow_cls=tc, g3_f32=g32)
 
 
 def nd_order(rp, ci, xyz, leaf=64):
@@ -464,6 +492,20 @@ class Solver:
         """64 (the reference) or 32: FP32 copies of K, G, G^T in the step
         kernels, the mixed-precision forms' FP32 assembly."""
         self.check(self.L.fsx_set_matrix_precision(self.ctx, bits), "set_matrix_precision")
+
+    def set_matrix_classes(self, on):
+        """At matrix precision 32: G and G^T as row classes where the rows
+        compress (True, the default) or the FP32 CSR copies (False)."""
+        self.check(self.L.fsx_set_matrix_classes(self.ctx, int(bool(on))), "set_matrix_classes")
+
+    def g_class_info(self):
+        """(G's classes, their entries, G^T's classes, theirs) of the class
+        form in use; zeros when the step kernels read the CSR copies."""
+        a, c = ctypes.c_int(), ctypes.c_int()
+        b, d = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.L.fsx_g_class_info(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                           ctypes.byref(d)), "g_class_info")
+        return a.value, b.value, c.value, d.value
 
     def set_convection_form(self, elements):
         self.check(self.L.fsx_set_convection_form(self.ctx, int(bool(elements))), "set_convection_form")

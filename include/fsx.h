@@ -162,6 +162,23 @@ int fsx_mass_stats(fsx_ctx* ctx, int* last_iters, double* last_relres, long long
  * entry and 24 per G entry of device memory. No reference counterpart (the
  * reference is FP64 throughout; its `real` type is unused, BCH:130-134). */
 int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
+/* At matrix precision 32, whether step3 and step2's right-hand side read G
+ * and G^T as row classes (1, the default) where the rows compress: rows of
+ * G with the same column offsets from their first column and the same FP32
+ * values (components below 2^-24 of the row's largest magnitude dropped)
+ * share one class list, so each row costs its first column and a class
+ * reference instead of 16 bytes an entry; G^T's rows keep their columns and
+ * share their values by class (fsxh_g_classes builds the same classes on
+ * the host). The form is taken where the class lists hold at most an
+ * eighth of G's entries: on meshes of congruent elements (the uniform
+ * cavity: 64 classes of G, 125 of G^T, from 10^3 elements on); elsewhere
+ * (a stretched or curved mesh) the kernels read the FP32 CSR copies. 0
+ * keeps the CSR copies. No reference counterpart (the reference stores G
+ * as three CSR value arrays, BCH:3322-3513). */
+int fsx_set_matrix_classes(fsx_ctx* ctx, int on);
+/* The class form in use: G's classes and their entries, G^T's classes and
+ * theirs (all 0 when the kernels read the CSR copies); pointers may be NULL */
+int fsx_g_class_info(fsx_ctx* ctx, int* classes, long long* entries, int* t_classes, long long* t_entries);
 /* step1 with FP32 operand shadows on a structured mesh whose K patterns are
  * steps of at most 2 on the velocity grid can stage each block's column
  * window of shadow records in LDS (enable != 0) instead of gathering them

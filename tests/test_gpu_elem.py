@@ -192,3 +192,57 @@ def test_carried_gpn_equals_formed(fsx_lib, case):
     assert e < 1e-13
     s.close()
     mesh.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_g_classes_against_the_csr_copies(fsx_lib, case):
+    """At matrix precision 32, step2's right-hand side and step3 read G and
+    G^T as row classes where the rows compress: on the uniform cavity 64
+    classes of G and 125 of G^T; on the stretched cavity and the curved duct
+    the class lists would hold over an eighth of G's entries, the form is
+    not taken and the runs are the CSR copies' bit for bit. One inner
+    iteration's R2, Pdot and Acc against the CSR copies' within 1e-6
+    normwise (the same FP32 values summed in another order, residue
+    components below 2^-24 of their row's largest dropped), and the fields
+    after 3 steps within the mixed forms' bar of the FP64 run."""
+    s, mesh = make(fsx_lib, case)
+    out = {}
+    for on in (False, True):
+        s.reset(mode=0)
+        forms(s, 64, 0)
+        s.run(2)
+        forms(s, 32, 1)
+        s.set_matrix_classes(on)
+        info = s.g_class_info()
+        s.begin_step()
+        s.convection()
+        s.step1(1)
+        s.step2(1)
+        r2, pdot = s.download("R2"), s.download("Pdot")
+        s.step3(1)
+        acc = s.download("Acc")
+        s.reset(mode=0)
+        st = s.run(3)
+        assert all(x["relres"] <= 1e-10 for x in st)
+        out[on] = (info, r2, pdot, acc, s.download("Un"), s.download("Pn"))
+    s.reset(mode=0)
+    forms(s, 64, 0)
+    s.run(3)
+    ref = (s.download("Un"), s.download("Pn"))
+    s.set_matrix_classes(True)
+    assert out[False][0] == (0, 0, 0, 0)
+    if case == "cavity10":
+        assert out[True][0][0] == 64 and out[True][0][2] == 125, out[True][0]
+        for name, a, b in zip(("R2", "Pdot", "Acc"), out[True][1:4], out[False][1:4]):
+            e = rel(a, b)
+            say(f"  {case} {name}: classes vs CSR copies {e:.2e}")
+            assert e < 1e-6, (name, e)
+    else:
+        assert out[True][0] == (0, 0, 0, 0)
+        for a, b in zip(out[True][1:], out[False][1:]):
+            assert np.array_equal(a, b)
+    eu, ep = rel(out[True][4], ref[0]), rel(out[True][5], ref[1])
+    say(f"  {case} classes vs FP64 after 3 steps: velocity {eu:.2e}, pressure {ep:.2e}")
+    assert eu < 1e-6 and ep < 1e-5
+    s.close()
+    mesh.close()
