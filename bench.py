@@ -62,7 +62,7 @@ def step_kernels(vt, classes=False):
     kept (inner iterations >= 2), step2's right-hand side, step3; `classes`:
     the mixed forms read G and G^T as row classes."""
     if vt == "float" and classes:
-        return ("k_step1<8, 2, 2, true, true, 2, float, true>", "k_step2_rhs<16, 4, float, true, true>",
+        return ("k_step1<8, 2, 2, true, true, 2, float, true>", "k_step2_rhs<8, 4, float, true, 2>",
                 "k_step3<4, 2, float, true>")
     if vt == "float":
         return ("k_step1<8, 2, 2, true, true, 2, float, true>", "k_step2_rhs<32, 1, float, true>",

@@ -17,8 +17,10 @@ namespace fsx {
 // k-th entry) carries the tag cls_ptr[row_cls[r]] + k, the global index of
 // that entry in its class; a row of G^T lists its entries in ascending tag
 // order (ties by row), and rows with the same tag sequence share one class,
-// whose values are those of the tags' class entries. Its columns stay per
-// row (velocity numberings need not repeat), each row in its class's order.
+// whose values are those of the tags' class entries. Where the rows' column
+// offsets from their first (in that order) repeat too, they join the key
+// and the class entries hold them (`toffsets`, a velocity numbering in grid
+// order); otherwise the columns stay per row, each row in its class's order.
 struct GClassForm {
   // G
   std::vector<int> row_base;    // first column of each row
@@ -32,6 +34,8 @@ struct GClassForm {
   std::vector<int> tcol;        // G^T's columns, each row in its class's order
   std::vector<int> tcls_ptr;    // class c's entries [tcls_ptr[c], tcls_ptr[c+1])
   std::vector<float> tent_val;  // (G1, G2, G3) an entry
+  bool toffsets = false;        // the classes carry the column offsets:
+  std::vector<int> tent_off;    //   column offset from the row's first column, per class entry
 };
 
 // Builds the classes of G (nrows rows over ncols columns, CSR rp/ci with

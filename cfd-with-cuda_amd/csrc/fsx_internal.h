@@ -239,8 +239,9 @@ struct fsx_ctx {
   int2* Gc_row = nullptr;      // per row of G: (first column, class entry start << 8 | length)
   float4* Gc_ent = nullptr;    // per class entry: (column offset bits, G1, G2, G3)
   int* GTc_info = nullptr;     // per row of G^T: class entry start << 8 | length
-  int* GTc_col = nullptr;      // G^T's columns in class order, row p from GTrow[p]
-  float4* GTc_val = nullptr;   // per class entry: (G1, G2, G3, 0)
+  int* GTc_col = nullptr;      // G^T's columns in class order, row p from GTrow[p] (null with GTc_base)
+  int* GTc_base = nullptr;     // per row of G^T: its first column, where the classes carry the offsets
+  float4* GTc_val = nullptr;   // per class entry: (G1, G2, G3, column offset bits or 0)
   int gc_ncls = 0, gtc_ncls = 0;
   long long gc_nent = 0, gtc_nent = 0;
   bool use_g_classes = true;

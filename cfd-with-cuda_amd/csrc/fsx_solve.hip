@@ -296,9 +296,11 @@ static int build_g_classes(fsx_ctx* ctx) {
   // built once per assembly of G, when first wanted; kept while the
   // precision or the switch changes (g_classes_on decides their use)
   if (!ctx->use_g_classes || !ctx->Gf || ctx->gc_built) return FSX_OK;
-  for (void* p : {(void*)ctx->Gc_row, (void*)ctx->Gc_ent, (void*)ctx->GTc_info, (void*)ctx->GTc_col, (void*)ctx->GTc_val})
+  for (void* p : {(void*)ctx->Gc_row, (void*)ctx->Gc_ent, (void*)ctx->GTc_info, (void*)ctx->GTc_col, (void*)ctx->GTc_base,
+                  (void*)ctx->GTc_val})
     if (p) dev_free(ctx, p);
-  ctx->Gc_row = nullptr; ctx->Gc_ent = nullptr; ctx->GTc_info = nullptr; ctx->GTc_col = nullptr; ctx->GTc_val = nullptr;
+  ctx->Gc_row = nullptr; ctx->Gc_ent = nullptr; ctx->GTc_info = nullptr; ctx->GTc_col = nullptr; ctx->GTc_base = nullptr;
+  ctx->GTc_val = nullptr;
   ctx->gc_ncls = ctx->gtc_ncls = 0;
   ctx->gc_nent = ctx->gtc_nent = 0;
   ctx->gc_built = true;
@@ -328,21 +330,33 @@ static int build_g_classes(fsx_ctx* ctx) {
     std::memcpy(&off, &F.ent_off[t], sizeof off);
     gent[t] = make_float4(off, F.ent_val[3 * t], F.ent_val[3 * t + 1], F.ent_val[3 * t + 2]);
   }
-  std::vector<int> tinfo(np);
-  for (int p = 0; p < np; p++) tinfo[p] = (F.tcls_ptr[F.trow_cls[p]] << 8) | (trp[p + 1] - trp[p]);
+  std::vector<int> tinfo(np), tbase(F.toffsets ? np : 0);
+  for (int p = 0; p < np; p++) {
+    tinfo[p] = (F.tcls_ptr[F.trow_cls[p]] << 8) | (trp[p + 1] - trp[p]);
+    if (F.toffsets) tbase[p] = F.tcol[trp[p]];
+  }
   const size_t nte = F.tent_val.size() / 3;
   std::vector<float4> tval(nte);
-  for (size_t t = 0; t < nte; t++) tval[t] = make_float4(F.tent_val[3 * t], F.tent_val[3 * t + 1], F.tent_val[3 * t + 2], 0.0f);
+  for (size_t t = 0; t < nte; t++) {
+    float off = 0.0f;
+    if (F.toffsets) std::memcpy(&off, &F.tent_off[t], sizeof off);
+    tval[t] = make_float4(F.tent_val[3 * t], F.tent_val[3 * t + 1], F.tent_val[3 * t + 2], off);
+  }
   FSX_TRY(alloc(ctx, &ctx->Gc_row, (size_t)nn));
   FSX_TRY(alloc(ctx, &ctx->Gc_ent, std::max<size_t>(1, ne)));
   FSX_TRY(alloc(ctx, &ctx->GTc_info, (size_t)np));
-  FSX_TRY(alloc(ctx, &ctx->GTc_col, std::max<size_t>(1, (size_t)nnz)));
   FSX_TRY(alloc(ctx, &ctx->GTc_val, std::max<size_t>(1, nte)));
   FSX_HIP(hipMemcpy(ctx->Gc_row, grow.data(), (size_t)nn * sizeof(int2), hipMemcpyHostToDevice));
   if (ne) FSX_HIP(hipMemcpy(ctx->Gc_ent, gent.data(), ne * sizeof(float4), hipMemcpyHostToDevice));
   FSX_HIP(hipMemcpy(ctx->GTc_info, tinfo.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
-  if (nnz) FSX_HIP(hipMemcpy(ctx->GTc_col, F.tcol.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
   if (nte) FSX_HIP(hipMemcpy(ctx->GTc_val, tval.data(), nte * sizeof(float4), hipMemcpyHostToDevice));
+  if (F.toffsets) {
+    FSX_TRY(alloc(ctx, &ctx->GTc_base, (size_t)np));
+    FSX_HIP(hipMemcpy(ctx->GTc_base, tbase.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+  } else {
+    FSX_TRY(alloc(ctx, &ctx->GTc_col, std::max<size_t>(1, (size_t)nnz)));
+    if (nnz) FSX_HIP(hipMemcpy(ctx->GTc_col, F.tcol.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+  }
   ctx->gc_ncls = (int)F.cls_ptr.size() - 1;
   ctx->gc_nent = (long long)ne;
   ctx->gtc_ncls = (int)F.tcls_ptr.size() - 1;
@@ -476,7 +490,7 @@ int settle_deferred(fsx_ctx* ctx) {
 // R2[p0] = 0 for p0 > 0 (BCH:4180-4184).
 // With DREC the operand d is gathered as step1's FP32 shadow of it (16 bytes
 // a column instead of 48 plus MdOrigInv); sums stay FP64.
-template <int LPR, int U, class VT, bool DREC, bool CLS = false>
+template <int LPR, int U, class VT, bool DREC, int CLS = 0>
 __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int* __restrict__ GTrow,
                                                       const int* __restrict__ GTcol, const VT* __restrict__ GT,
                                                       const double* __restrict__ UnpHalf, const double* __restrict__ MdOrigInv,
@@ -488,24 +502,32 @@ __global__ __launch_bounds__(kBlock) void k_step2_rhs(int NNp, int NN, const int
   const long long p = my_row<LPR>(&l);
   double s0 = 0, s1 = 0, s2 = 0;
   if (p < NNp) {
-    const int rs = GTrow[p], re = GTrow[p + 1];
-    // class form: the row's columns in its class's order (GTcol is then the
-    // class-ordered column array), values from the class entries
-    const int cs = CLS ? (GTc_info[p] >> 8) - rs : 0;
+    // class forms (CLS 1, 2): values from the row's class entries; the
+    // columns from GTcol in class order (1), or the class entries' offsets
+    // from the row's first column, GTcol[p] (2)
+    int rs, re, cs = 0, c0 = 0;
+    if constexpr (CLS == 2) {
+      const int info = GTc_info[p];
+      rs = 0; re = info & 255; cs = info >> 8; c0 = GTcol[p];
+    } else {
+      rs = GTrow[p]; re = GTrow[p + 1];
+      if constexpr (CLS == 1) cs = (GTc_info[p] >> 8) - rs;
+    }
     for (int base = rs + l; base < re; base += U * LPR) {
       int c[U];
       double a0[U], a1[U], a2[U];
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const int k = base + j * LPR, kk = k < re ? k : re - 1;
-        c[j] = GTcol[kk];
         const bool ok = k < re;
-        if constexpr (CLS) {
+        if constexpr (CLS > 0) {
           const float4 v = GTc_val[cs + kk];
+          c[j] = CLS == 2 ? c0 + __float_as_int(v.w) : GTcol[kk];
           a0[j] = ok ? (double)v.x : 0.0;
           a1[j] = ok ? (double)v.y : 0.0;
           a2[j] = ok ? (double)v.z : 0.0;
         } else {
+          c[j] = GTcol[kk];
           a0[j] = ok ? (double)GT[3 * (size_t)kk] : 0.0;
           a1[j] = ok ? (double)GT[3 * (size_t)kk + 1] : 0.0;
           a2[j] = ok ? (double)GT[3 * (size_t)kk + 2] : 0.0;
@@ -545,18 +567,31 @@ int launch_step2_rhs(fsx_ctx* ctx, int iter) {
   // with the FP32 operand, 32 lanes x 1 load a row: 122.7 us at 64^3 (32x2
   // 131.6, 16x1 126.4, 16x2 134.3, 32x4 139.6, 16x4 142.8, 16x8 161.0)
   const bool cls = g_classes_on(ctx);
-  // G^T as row classes with the FP32 operand: 16 lanes x 4 entries a round,
-  // 54.2 us at 64^3 (16x2 55.8, 8x4 56.1, 32x2 64.1, 32x1 76.1, 64x2 107.5)
-  if (drec && cls)
-    hipLaunchKernelGGL((k_step2_rhs<16, 4, float, true, true>), dim3(rows_grid(ctx->NNp, 16)), dim3(kBlock), 0,
+  const bool off = cls && ctx->GTc_base;   // the classes carry the column offsets
+  // G^T as row classes with offsets, FP32 operand: 8 lanes x 4 entries a
+  // round, 38.7 us at 64^3 (8x8 39.9, 16x4 40.2, 16x2 41.9, 32x4 50.2, 32x2
+  // 53.4, 64x2 108.9)
+  if (drec && off)
+    hipLaunchKernelGGL((k_step2_rhs<8, 4, float, true, 2>), dim3(rows_grid(ctx->NNp, 8)), dim3(kBlock), 0, ctx->stream,
+                       ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTc_base, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
+                       use_kacc, o2, ctx->zeroP, ctx->R2, dr, ctx->GTc_info, ctx->GTc_val);
+  // G^T as row classes with per-row columns, FP32 operand: 16 lanes x 4
+  // entries a round, 54.2 us at 64^3 (16x2 55.8, 8x4 56.1, 32x2 64.1, 32x1
+  // 76.1, 64x2 107.5)
+  else if (drec && cls)
+    hipLaunchKernelGGL((k_step2_rhs<16, 4, float, true, 1>), dim3(rows_grid(ctx->NNp, 16)), dim3(kBlock), 0,
                        ctx->stream, ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTc_col, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv,
                        ctx->KAcc, use_kacc, o2, ctx->zeroP, ctx->R2, dr, ctx->GTc_info, ctx->GTc_val);
   else if (drec)
     hipLaunchKernelGGL((k_step2_rhs<32, 1, float, true>), dim3(rows_grid(ctx->NNp, 32)), dim3(kBlock), 0, ctx->stream,
                        ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTcol, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv, ctx->KAcc,
                        use_kacc, o2, ctx->zeroP, ctx->R2, dr);
+  else if (off)
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, false, 2>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0,
+                       ctx->stream, ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTc_base, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv,
+                       ctx->KAcc, use_kacc, o2, ctx->zeroP, ctx->R2, dr, ctx->GTc_info, ctx->GTc_val);
   else if (cls)
-    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, false, true>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_step2_rhs<LPR, 8, float, false, 1>), dim3(rows_grid(ctx->NNp, LPR)), dim3(kBlock), 0,
                        ctx->stream, ctx->NNp, ctx->NN, ctx->GTrow, ctx->GTc_col, ctx->GTf, ctx->UnpHalf, ctx->MdOrigInv,
                        ctx->KAcc, use_kacc, o2, ctx->zeroP, ctx->R2, dr, ctx->GTc_info, ctx->GTc_val);
   else if (ctx->GTf)
@@ -1700,13 +1735,16 @@ extern "C" int fsx_bench_kernel(fsx_ctx* ctx, int kernel, int reps, double* ms_p
       bytes = vb * nK + kcols + 4.0 * (NN + 1) + 24.0 * NN + 24.0 * NN;
       break;
     case FSX_KERNEL_STEP2_RHS: {  // G^T (col + 3 values), UnpHalf, MdOrigInv, KAcc, R2; or G^T, d's FP32 shadow,
-                                  // R2. G^T as row classes: its columns, each row's class word and the class
-                                  // values (16 bytes an entry, once) in place of the values
-      const double gt = g_classes_on(ctx) ? 4.0 * nG + 4.0 * NNp + 16.0 * (double)ctx->gtc_nent : (4.0 + 3.0 * vb) * nG;
+                                  // R2. G^T as row classes: each row's class word and the class entries (16
+                                  // bytes each, once), with the columns and row pointers, or each row's first
+                                  // column where the classes carry the offsets
+      const double gt = !g_classes_on(ctx) ? (4.0 + 3.0 * vb) * nG
+                        : (ctx->GTc_base ? 8.0 * NNp : 4.0 * nG + 4.0 * NNp + 4.0 * (NNp + 1)) + 16.0 * (double)ctx->gtc_nent;
+      const double rowp = g_classes_on(ctx) ? 0.0 : 4.0 * (NNp + 1);   // G^T's row pointers
       if (ctx->GTf && ctx->drec_ok && ctx->drec_kacc)
-        bytes = gt + 4.0 * (NNp + 1) + 16.0 * NN + 8.0 * NNp;
+        bytes = gt + rowp + 16.0 * NN + 8.0 * NNp;
       else
-        bytes = gt + 4.0 * (NNp + 1) + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
+        bytes = gt + rowp + 24.0 * NN + 8.0 * NN + 24.0 * NN + 8.0 * NNp;
       break;
     }
     case FSX_KERNEL_STEP1_ELEMENTS:   // per element its 27 node ids, corner offsets (24 floats) and rows

@@ -107,40 +107,85 @@ int g_class_form(int nrows, int ncols, const int* rp, const int* ci, const doubl
         ent[fill[ci[e]]++] = (tag << 32) | (unsigned)r;
       }
   }
-  ListDict td;
   F.trow_cls.assign(ncols, 0);
   F.tcol.resize(nnz);
   for (int p = 0; p < ncols; p++) {
     const int ps = F.trow_ptr[p], len = F.trow_ptr[p + 1] - ps;
     if (len > max_len) return 1;
     std::sort(ent.begin() + ps, ent.begin() + ps + len);
-    key.resize(len);
-    for (int k = 0; k < len; k++) {
-      key[k] = (unsigned)(ent[ps + k] >> 32);
-      F.tcol[ps + k] = (int)(ent[ps + k] & 0xffffffffLL);
-    }
-    F.trow_cls[p] = td.intern(key.data(), len);
-    if (td.count() > max_cls || (long long)td.store.size() > max_ent) return 1;
+    for (int k = 0; k < len; k++) F.tcol[ps + k] = (int)(ent[ps + k] & 0xffffffffLL);
   }
-  F.tcls_ptr.assign(td.ptr.begin(), td.ptr.end());
-  F.tent_val.resize(3 * td.store.size());
-  for (size_t t = 0; t < td.store.size(); t++)
-    for (int d = 0; d < 3; d++) F.tent_val[3 * t + d] = F.ent_val[3 * (size_t)td.store[t] + d];
+  // keyed by the tags (entries in tag order); then, where it makes no more
+  // classes, by (column offset from the row's first column, FP32 values)
+  // with the entries in column order, so that rows repeat their offsets
+  // wherever the numbering does
+  int ntag = -1;
+  std::vector<long long> oe;
+  for (int with_off = 0; with_off <= 1; with_off++) {
+    ListDict td;
+    bool fits = true;
+    const int w = with_off ? 4 : 1;
+    std::vector<int> cls(ncols), col;
+    if (with_off) col.resize(nnz);
+    for (int p = 0; p < ncols && fits; p++) {
+      const int ps = F.trow_ptr[p], len = F.trow_ptr[p + 1] - ps;
+      key.resize((size_t)w * len);
+      if (with_off) {
+        oe.resize(len);   // column << 32 | tag, ascending columns
+        for (int k = 0; k < len; k++) oe[k] = ((long long)F.tcol[ps + k] << 32) | (ent[ps + k] >> 32);
+        std::sort(oe.begin(), oe.end());
+        const int c0 = (int)(oe[0] >> 32);
+        for (int k = 0; k < len; k++) {
+          col[ps + k] = (int)(oe[k] >> 32);
+          const size_t tag = (size_t)(oe[k] & 0xffffffffLL);
+          key[4 * (size_t)k] = (unsigned)(col[ps + k] - c0);
+          for (int d = 0; d < 3; d++) key[4 * (size_t)k + 1 + d] = bits(F.ent_val[3 * tag + d]);
+        }
+      } else {
+        for (int k = 0; k < len; k++) key[k] = (unsigned)(ent[ps + k] >> 32);
+      }
+      cls[p] = td.intern(key.data(), w * len);
+      fits = td.count() <= max_cls && (long long)td.store.size() / w <= max_ent &&
+             (!with_off || td.count() <= ntag);
+    }
+    if (!fits) {
+      if (with_off) break;   // the tag-only classes stand
+      return 1;
+    }
+    if (!with_off) ntag = td.count();
+    else F.tcol.swap(col);
+    F.trow_cls.swap(cls);
+    const size_t ne = td.store.size() / w;
+    F.toffsets = with_off != 0;
+    F.tcls_ptr.resize(td.ptr.size());
+    for (size_t c = 0; c < td.ptr.size(); c++) F.tcls_ptr[c] = td.ptr[c] / w;
+    F.tent_val.resize(3 * ne);
+    F.tent_off.assign(with_off ? ne : 0, 0);
+    for (size_t t = 0; t < ne; t++) {
+      if (with_off) {
+        F.tent_off[t] = (int)td.store[4 * t];
+        for (int d = 0; d < 3; d++) std::memcpy(&F.tent_val[3 * t + d], &td.store[4 * t + 1 + d], sizeof(float));
+      } else {
+        for (int d = 0; d < 3; d++) F.tent_val[3 * t + d] = F.ent_val[3 * (size_t)td.store[t] + d];
+      }
+    }
+  }
   return 0;
 }
 
 }  // namespace fsx
 
 int fsxh_g_classes(int nrows, int ncols, const int* rp, const int* ci, const double* g3, double prune_rel,
-                   int* counts4, int* row_cls, int* trow_cls, float* g3_f32) {
+                   int* counts5, int* row_cls, int* trow_cls, float* g3_f32) {
   fsx::GClassForm F;
   const int rc = fsx::g_class_form(nrows, ncols, rp, ci, g3, prune_rel, 1 << 20, 1LL << 28, 1 << 20, &F);
   if (rc < 0) return FSX_EINVAL;
-  if (counts4) {
-    counts4[0] = rc == 0 ? (int)F.cls_ptr.size() - 1 : -1;
-    counts4[1] = rc == 0 ? (int)F.ent_off.size() : -1;
-    counts4[2] = rc == 0 ? (int)F.tcls_ptr.size() - 1 : -1;
-    counts4[3] = rc == 0 ? (int)F.tcls_ptr.back() : -1;
+  if (counts5) {
+    counts5[0] = rc == 0 ? (int)F.cls_ptr.size() - 1 : -1;
+    counts5[1] = rc == 0 ? (int)F.ent_off.size() : -1;
+    counts5[2] = rc == 0 ? (int)F.tcls_ptr.size() - 1 : -1;
+    counts5[3] = rc == 0 ? (int)F.tcls_ptr.back() : -1;
+    counts5[4] = rc == 0 ? (int)F.toffsets : -1;
   }
   if (rc != 0) return FSX_OK;
   if (row_cls) std::copy(F.row_cls.begin(), F.row_cls.end(), row_cls);
@@ -157,7 +202,9 @@ int fsxh_g_classes(int nrows, int ncols, const int* rp, const int* ci, const dou
     for (int p = 0; p < ncols; p++) {
       const int c0 = F.tcls_ptr[F.trow_cls[p]];
       for (int k = F.trow_ptr[p]; k < F.trow_ptr[p + 1]; k++) {
-        const int r = F.tcol[k];
+        // with offsets in the classes, the column the class form reads
+        const int r = F.toffsets ? F.tcol[F.trow_ptr[p]] + F.tent_off[c0 + k - F.trow_ptr[p]] : F.tcol[k];
+        if (r < 0 || r >= nrows) return FSX_EINVAL;
         const int* lo = ci + rp[r];
         const int* hi = ci + rp[r + 1];
         const int* at = std::lower_bound(lo, hi, p);

@@ -211,14 +211,14 @@ def g_classes(rp, ci, g3, ncols, prune_rel=2.0**-24):
     """Row classes of G (fsxh_g_classes): G's CSR pattern (rp, ci) with
     g3 = (G1, G2, G3) a row of an (nnz, 3) array over ncols columns. Returns
     None when the rows do not compress, else a dict: counts (G's classes,
-    their entries, G^T's classes, theirs), row_cls, trow_cls and g3_f32, the
-    (nnz, 3) FP32 values the class form represents (NaN where G^T's classes
-    disagree with G's)."""
+    their entries, G^T's classes, theirs, 1 if G^T's classes carry column
+    offsets), row_cls, trow_cls and g3_f32, the (nnz, 3) FP32 values the
+    class form represents (NaN where G^T's classes disagree with G's)."""
     rp = np.ascontiguousarray(rp, dtype=np.int32)
     ci = np.ascontiguousarray(ci, dtype=np.int32)
     g3 = np.ascontiguousarray(g3, dtype=np.float64).reshape(-1, 3)
     n = rp.size - 1
-    cnt = np.zeros(4, dtype=np.int32)
+    cnt = np.zeros(5, dtype=np.int32)
     rc_ = np.zeros(n, dtype=np.int32)
     tc = np.zeros(int(ncols), dtype=np.int32)
     g32 = np.zeros(g3.shape, dtype=np.float32)

@@ -167,14 +167,16 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
  * G with the same column offsets from their first column and the same FP32
  * values (components below 2^-24 of the row's largest magnitude dropped)
  * share one class list, so each row costs its first column and a class
- * reference instead of 16 bytes an entry; G^T's rows keep their columns and
- * share their values by class (fsxh_g_classes builds the same classes on
- * the host). The form is taken where the class lists hold at most an
- * eighth of G's entries: on meshes of congruent elements (the uniform
- * cavity: 64 classes of G, 125 of G^T, from 10^3 elements on); elsewhere
- * (a stretched or curved mesh) the kernels read the FP32 CSR copies. 0
- * keeps the CSR copies. No reference counterpart (the reference stores G
- * as three CSR value arrays, BCH:3322-3513). */
+ * reference instead of 16 bytes an entry. G^T's rows share their values by
+ * class too, and their column offsets where those repeat (the device
+ * numbers the velocity nodes in grid order on structured meshes), else
+ * they keep their columns (fsxh_g_classes builds the same classes on the
+ * host). The form is taken where the class lists hold at most an eighth of
+ * G's entries: on meshes of congruent elements (the uniform cavity: 64
+ * classes of G and 27 of G^T, from 10^3 elements on); elsewhere (a
+ * stretched or curved mesh) the kernels read the FP32 CSR copies. 0 keeps
+ * the CSR copies. No reference counterpart (the reference stores G as three
+ * CSR value arrays, BCH:3322-3513). */
 int fsx_set_matrix_classes(fsx_ctx* ctx, int on);
 /* The class form in use: G's classes and their entries, G^T's classes and
  * theirs (all 0 when the kernels read the CSR copies); pointers may be NULL */

@@ -153,16 +153,19 @@ int fsxh_nd_order(int n, const int* rp, const int* ci, const double* xyz, int le
  * below prune_rel times its row's largest magnitude counts as 0, the others
  * are rounded to FP32; rows with the same column offsets from their first
  * column and the same values share a class, and G^T's rows are classed by
- * the sequence of G's class entries they hold. counts4 (may be NULL)
- * receives the numbers of G's classes, their entries, G^T's classes and
- * theirs (all -1 beyond 2^20 classes or 2^28 entries; the device takes the
- * form only where the class entries are at most an eighth of G's); row_cls
+ * the sequence of G's class entries they hold, and by their column offsets
+ * from their first column too where those repeat (a velocity numbering in
+ * grid order: the classes then carry the offsets and no row stores its
+ * columns). counts5 (may be NULL) receives the numbers of G's classes,
+ * their entries, G^T's classes and theirs, and 1 when G^T's classes carry
+ * the offsets (all -1 beyond 2^20 classes or 2^28 entries; the device takes
+ * the form only where the class entries are at most an eighth of G's); row_cls
  * (nrows) and trow_cls (ncols) the classes; g3_f32 (3 nnz) the values the
  * class form represents, NaN where G^T's class disagrees with G's. The
  * device builds the same classes (with prune_rel 2^-24). FSX_EINVAL for an
  * inconsistent pattern. */
 int fsxh_g_classes(int nrows, int ncols, const int* rp, const int* ci, const double* g3, double prune_rel,
-                   int* counts4, int* row_cls, int* trow_cls, float* g3_f32);
+                   int* counts5, int* row_cls, int* trow_cls, float* g3_f32);
 
 #ifdef __cplusplus
 }

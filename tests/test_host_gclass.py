@@ -53,8 +53,9 @@ def test_g_classes_on_the_uniform_cavity(fsx_lib, oracle_lib, n):
     rp, ci, g3, nnp = oracle_g(oracle_lib, meshgen.cavity(n))
     f = fsx_lib.g_classes(rp, ci, g3, nnp)
     assert f is not None
-    ncls, nent, ntcls, ntent = f["counts"]
+    ncls, nent, ntcls, ntent, toff = f["counts"]
     assert (ncls, ntcls) == (64, 125), f["counts"]
+    assert toff == 0   # the reference's velocity numbering: G^T's rows keep their columns
     v32 = flushed_f32(rp, g3)
     assert np.array_equal(f["g3_f32"].view(np.uint32), v32.view(np.uint32))   # no NaN: G^T agrees
     keys = [(tuple(ci[rp[r]:rp[r + 1]] - ci[rp[r]]), v32[rp[r]:rp[r + 1]].tobytes()) for r in range(rp.size - 1)]
@@ -73,6 +74,28 @@ def test_g_classes_on_the_uniform_cavity(fsx_lib, oracle_lib, n):
         gty = np.bincount(ci, g3[:, d] * y[rows], minlength=nnp)
         cty = np.bincount(ci, f["g3_f32"][:, d].astype(np.float64) * y[rows], minlength=nnp)
         assert np.abs(cty - gty).max() <= 2e-7 * np.abs(g3[:, d]).max() * np.abs(y).max() * 125
+
+
+def test_g_transpose_classes_carry_offsets_in_grid_order(fsx_lib, oracle_lib):
+    """With the velocity nodes numbered in grid order (z, y, x), as the
+    device numbers them, G^T's rows repeat their column offsets too: keyed
+    by (offset, values) in column order they form 27 classes (below, inside
+    and above in each axis), which carry the offsets, and the form still
+    represents G exactly."""
+    o = oracle_lib.case_from_mesh(meshgen.cavity(6), NGP=27)
+    rp, ci = o.i("Grow"), o.i("Gcol")
+    g3 = np.stack([o.d("G1"), o.d("G2"), o.d("G3")], 1)
+    xyz = o.d("coord").reshape(-1, 3)
+    nnp = o.sizes()["NNp"]
+    o.close()
+    q = np.round(xyz * 1e9).astype(np.int64)   # mid-node coordinates carry rounding
+    order = np.lexsort((q[:, 0], q[:, 1], q[:, 2]))   # new row -> old row
+    lens = np.diff(rp)[order]
+    rp2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in order])
+    f = fsx_lib.g_classes(rp2, ci[idx], g3[idx], nnp)
+    assert f["counts"][2] == 27 and f["counts"][4] == 1, f["counts"]
+    assert np.array_equal(f["g3_f32"].view(np.uint32), flushed_f32(rp2, g3[idx]).view(np.uint32))
 
 
 def test_g_classes_grow_on_a_stretched_mesh(fsx_lib, oracle_lib):
