@@ -62,6 +62,12 @@ struct Published {
 // (LDS offset, FP32 value bits) sit at ent + global start and, for the
 // wavefronts whose rows mix classes, in the tile's entry block
 // tent[b * estride ...] that the kernel stages in LDS.
+//
+// On a tensor-product box grid the tiles apply Z as Kronecker products of 1D
+// stencils instead (fsx_ztensor.h): `coef` holds per axis and grid point
+// K's five coefficients (offsets -2..2) then M's, FP32, the x axis first,
+// and the tile's rows are in grid order; `zp` (-1: none) is the pinned
+// node, whose diagonal takes `pen` beyond the tensor form.
 struct TilePlan {
   int nx, ny, nz;
   int ty, tz, lx, ly, lz;
@@ -70,6 +76,10 @@ struct TilePlan {
   const int4* rows;
   const int2* ent;
   const int2* tent;
+  int tensor;              // 1: the Kronecker form (coef, zp, pen), 0: the row classes
+  const float* coef;
+  int zp;
+  float pen;
 };
 
 // Vectors of one tiled kernel of the learned solver's iteration (the mode
@@ -245,6 +255,8 @@ struct fsx_ctx {
   int gc_ncls = 0, gtc_ncls = 0;
   long long gc_nent = 0, gtc_nent = 0;
   bool use_g_classes = true;
+  bool use_z_tensor = true;    // the tiles' Kronecker form of Z where it holds (fsx_set_z_tensor)
+  double z_tensor_err = -1.0;  // its largest mismatch against Z's rows (-1: the grid is not a box grid)
   bool gc_built = false;       // the classes (or their absence) are of the current assembly of G
   int mat_bits = 64;
   int conv_bits = 64;            // arithmetic of the convection kernel (fsx_set_convection_precision)
@@ -498,6 +510,10 @@ int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);
 int launch_residual_only(fsx_ctx* ctx, double* x, const double* b, int* stop, double* rr_hi);
 int build_tile_plan(fsx_ctx* ctx, const std::vector<int>& cls, const std::vector<int>& cp, const std::vector<int>& co,
                     const std::vector<unsigned>& cv);
+struct ZTensor;
+// the tile plan of the Kronecker form (fsx_stencil.hip); FSX_OK with no
+// plan (ntiles 0) when a tile does not fit LDS
+int build_tile_plan_tensor(fsx_ctx* ctx, const ZTensor& zt);
 int launch_tile(fsx_ctx* ctx, int mode, const TileVecs& v);
 int launch_warm_residual(fsx_ctx* ctx, const double* x, const double* b, double* rh);
 int launch_tile_rescan(fsx_ctx* ctx, const double* rh, float* r, const float* p);

@@ -15,6 +15,7 @@
 // Every tile is one pass of at most 1024 rows, whatever its position, and
 // every global load of a launch is issued before the first wait.
 #include "fsx_device.h"
+#include "fsx_ztensor.h"
 
 #include <algorithm>
 #include <cstring>
@@ -163,6 +164,76 @@ int build_tile_plan(fsx_ctx* ctx, const std::vector<int>& cls, const std::vector
   return FSX_OK;
 }
 
+// LDS floats of a tile of the Kronecker form: the tile, the own rows' three
+// operands, the z pass's two products over (lx, ly, tz), the y pass's two
+// over (lx, ty, tz) and the tile's axis coefficients (10 a grid point)
+static size_t tensor_lds_floats(int nx, int ty, int tz) {
+  const size_t lx = nx + 2 * kTileHalo, ly = ty + 2 * kTileHalo, lz = tz + 2 * kTileHalo;
+  return lx * ly * lz + 3 * (size_t)nx * ty * tz + 2 * lx * ly * tz + 2 * lx * ty * tz + 10 * (size_t)(nx + ty + tz);
+}
+
+int build_tile_plan_tensor(fsx_ctx* ctx, const ZTensor& zt) {
+  for (void*& p : ctx->tile_mem)
+    if (p) { dev_free(ctx, p); p = nullptr; }
+  if (ctx->tile_part) { dev_free(ctx, ctx->tile_part); ctx->tile_part = nullptr; }
+  ctx->tile = TilePlan{};
+  const int nx = zt.n[0], ny = zt.n[1], nz = zt.n[2];
+  if (nx < 5 || ny < 5 || nz < 5 || nx > kTileThreads || (long long)nx * ny * nz != ctx->NNp) return FSX_OK;
+  // the most lines (ty x tz, ty >= tz) a tile of at most 1024 rows that fits LDS
+  int ty = 0, tz = 0, best = 0;
+  for (int a = 1; a <= std::min(ny, kTileThreads / nx); a++)
+    for (int c = 1; c <= std::min(a, nz); c++) {
+      const long long L = (long long)(nx + 2 * kTileHalo) * (a + 2 * kTileHalo) * (c + 2 * kTileHalo);
+      if ((long long)nx * a * c > kTileThreads || L > kTileSlotsMax * (long long)kTileThreads || L >= 65536) continue;
+      if (tensor_lds_floats(nx, a, c) * sizeof(float) > (size_t)kTileMaxLds) continue;
+      if (10 * (nx + a + c) > kTileThreads) continue;   // one coefficient a thread
+      if (a * c > best) { best = a * c; ty = a; tz = c; }
+    }
+  if (best == 0) return FSX_OK;
+  const int lx = nx + 2 * kTileHalo, ly = ty + 2 * kTileHalo, lz = tz + 2 * kTileHalo;
+  const int tiles_y = (ny + ty - 1) / ty, ntiles = tiles_y * ((nz + tz - 1) / tz);
+  const size_t rstride = kTileThreads;
+  // rows in grid order: (row, LDS index | own index << 16, 0, 1), 0 past the tile's rows
+  std::vector<int4> rows((size_t)ntiles * rstride, make_int4(0, 0, 0, 0));
+  for (int t = 0; t < ntiles; t++) {
+    const int ty0 = (t % tiles_y) * ty, tz0 = (t / tiles_y) * tz;
+    size_t q = 0;
+    for (int k = tz0; k < std::min(nz, tz0 + tz); k++)
+      for (int j = ty0; j < std::min(ny, ty0 + ty); j++)
+        for (int i = 0; i < nx; i++, q++) {
+          const int r = i + nx * (j + ny * k);
+          const int li = (i + kTileHalo) + lx * ((j - ty0 + kTileHalo) + ly * (k - tz0 + kTileHalo));
+          const int own = i + nx * ((j - ty0) + ty * (k - tz0));
+          rows[(size_t)t * rstride + q] = make_int4(r, li | (own << 16), 0, 1);
+        }
+  }
+  std::vector<float> coef(10 * (size_t)(nx + ny + nz));
+  size_t at = 0;
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < zt.n[d]; i++) {
+      for (int c = 0; c < 5; c++) coef[at++] = (float)zt.k[d][5 * (size_t)i + c];
+      for (int c = 0; c < 5; c++) coef[at++] = (float)zt.m[d][5 * (size_t)i + c];
+    }
+  int4* d_rows = nullptr; float* d_coef = nullptr;
+  FSX_TRY(alloc(ctx, &d_rows, rows.size()));
+  ctx->tile_mem[0] = d_rows;
+  FSX_TRY(alloc(ctx, &d_coef, coef.size()));
+  ctx->tile_mem[1] = d_coef;
+  FSX_HIP(hipMemcpy(d_rows, rows.data(), rows.size() * sizeof(int4), hipMemcpyHostToDevice));
+  FSX_HIP(hipMemcpy(d_coef, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice));
+  const int npart = (ntiles + 511) / 512 * 512;   // whole passes of 8 loads per lane of a wavefront
+  FSX_TRY(alloc(ctx, &ctx->tile_part, (size_t)kTileSlots * npart));
+  FSX_HIP(hipMemset(ctx->tile_part, 0, (size_t)kTileSlots * npart * sizeof(double)));
+  TilePlan& T = ctx->tile;
+  T.nx = nx; T.ny = ny; T.nz = nz;
+  T.ty = ty; T.tz = tz; T.lx = lx; T.ly = ly; T.lz = lz;
+  T.tiles_y = tiles_y; T.ntiles = ntiles; T.npart = npart;
+  T.rstride = (int)rstride; T.estride = 0;
+  T.rows = d_rows; T.ent = nullptr; T.tent = nullptr;
+  T.tensor = 1; T.coef = d_coef; T.zp = zt.zero_p; T.pen = (float)zt.pen;
+  return FSX_OK;
+}
+
 // ------------------------------------------------------------ tiled kernels
 // One block of 1024 threads per tile, one row a thread. MODE:
 //  kTileDir    (the iteration's first kernel) decides from the partial sums of
@@ -185,11 +256,21 @@ int build_tile_plan(fsx_ctx* ctx, const std::vector<int>& cls, const std::vector
 // loads; the operands the epilogue needs at a row come from LDS, where the
 // tile load put them. At most 64 VGPRs (8 wavefronts a SIMD), so two tiles
 // share a CU and every tile of the 65^3 grid is resident at once.
+//
+// TEN: Z as Kronecker products of 1D stencils (a tensor-product box grid):
+// Z = Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz applied to the tile by
+// passes along z (a = Mz t, b = Kz t over every x and y of the tile, own z),
+// along y (c = My a, de = Ky a + My b over every x, own y and z) and, per
+// row, along x (Kx c + Mx de): 45 multiply-adds a row instead of the 81 of
+// its class list, with each pass's products staged in LDS.
 struct TileGeom {
   int nx, ny, nz, ty, tz, lx, ly, lz, tiles_y, ntiles, npart, rstride, estride;
+  const float* coef;
+  int zp;
+  float pen;
 };
 
-template <int MODE, int TS>
+template <int MODE, int TS, bool TEN>
 __global__ __launch_bounds__(kTileThreads, 8) void k_tile(TileGeom T, const int4* __restrict__ rows,
                                                           const int2* __restrict__ ent, const int2* __restrict__ tent,
                                                           TileVecs A) {
@@ -203,18 +284,43 @@ __global__ __launch_bounds__(kTileThreads, 8) void k_tile(TileGeom T, const int4
   float* e1s = e0s + nown;        // direction, or dinv (kTilePost)
   float* e2s = e1s + nown;        // q (kTilePost), x (kTileSmooth)
   int2* tes = reinterpret_cast<int2*>(e2s + nown + ((L + 3 * nown) & 1));   // the tile's entry block (8-byte aligned)
+  // the Kronecker form's products and coefficients (TEN)
+  const int n1 = T.lx * T.ly * T.tz, n2 = T.lx * T.ty * T.tz;
+  float* a1s = e2s + nown;
+  float* b1s = a1s + n1;
+  float* c2s = b1s + n1;
+  float* d2s = c2s + n2;
+  float* cxs = d2s + n2;          // nx x 10
+  float* cys = cxs + 10 * T.nx;   // ty x 10
+  float* czs = cys + 10 * T.ty;   // tz x 10
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
   // --- one pass of loads: this thread's row, the tile's entry block, the
   // tile nodes and the own rows' operands, then the partial sums
   const int4 rw = rows[(size_t)b * T.rstride + threadIdx.x];
-  constexpr int NE = 4;   // entries of the block a thread copies (estride <= 4 x 1024)
-  int2 eb[NE];
-#pragma unroll
-  for (int u = 0; u < NE; u++) {
-    const int q = threadIdx.x + u * kTileThreads;
-    eb[u] = tent[(size_t)b * T.estride + (q < T.estride ? q : 0)];
-  }
   const int ty0 = (b % T.tiles_y) * T.ty, tz0 = (b / T.tiles_y) * T.tz;
+  constexpr int NE = TEN ? 1 : 4;   // entries of the block a thread copies (estride <= 4 x 1024)
+  int2 eb[NE];
+  float cf = 0.f;                   // TEN: this thread's axis coefficient for the LDS tables
+  const int ncf = 10 * (T.nx + T.ty + T.tz);
+  if constexpr (TEN) {
+    // coefficient q of the tile's tables: x rows, then the own y rows, then
+    // the own z rows (zero past the grid)
+    const int q = threadIdx.x;
+    if (q < ncf) {
+      const int row = q / 10, c = q - 10 * row;
+      int g = -1;
+      if (row < T.nx) g = row;
+      else if (row < T.nx + T.ty) { const int j = ty0 + row - T.nx; g = j < T.ny ? T.nx + j : -1; }
+      else { const int k = tz0 + row - T.nx - T.ty; g = k < T.nz ? T.nx + T.ny + k : -1; }
+      cf = g >= 0 ? T.coef[10 * g + c] : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+      const int q = threadIdx.x + u * kTileThreads;
+      eb[u] = tent[(size_t)b * T.estride + (q < T.estride ? q : 0)];
+    }
+  }
   int g[TS], own[TS];
   bool in[TS];
 #pragma unroll
@@ -312,10 +418,14 @@ __global__ __launch_bounds__(kTileThreads, 8) void k_tile(TileGeom T, const int4
       else if constexpr (MODE == kTilePost) { e0s[o] = vb[u]; e1s[o] = vc[u]; e2s[o] = vd[u]; }
     }
   }
+  if constexpr (TEN) {
+    if (threadIdx.x < ncf) cxs[threadIdx.x] = cf;   // ncf <= 1024 (nx + ty + tz <= 102)
+  } else {
 #pragma unroll
-  for (int u = 0; u < NE; u++) {
-    const int q = threadIdx.x + u * kTileThreads;
-    if (q < T.estride) tes[q] = eb[u];
+    for (int u = 0; u < NE; u++) {
+      const int q = threadIdx.x + u * kTileThreads;
+      if (q < T.estride) tes[q] = eb[u];
+    }
   }
   __syncthreads();
   // --- this thread's row
@@ -325,6 +435,51 @@ __global__ __launch_bounds__(kTileThreads, 8) void k_tile(TileGeom T, const int4
   const int centre = kTileHalo + T.lx * (kTileHalo + T.ly * kTileHalo);
   const int grow = rw.x, li = act ? (rw.y & 0xffff) : centre, o = rw.y >> 16;
   float acc = 0.f;
+  if constexpr (TEN) {
+    // z pass: every (x, y) of the tile at the own z
+    const int lxy = T.lx * T.ly;
+    for (int q = threadIdx.x; q < n1; q += kTileThreads) {
+      const int zo = q / lxy, xy = q - zo * lxy;
+      const float* src = tile + xy + lxy * zo;   // tile z = zo + d, d = 0..4
+      const float* cz = czs + 10 * zo;
+      float a = 0.f, bb = 0.f;
+#pragma unroll
+      for (int d = 0; d < 5; d++) {
+        const float t = src[d * lxy];
+        bb += cz[d] * t;
+        a += cz[5 + d] * t;
+      }
+      a1s[q] = a;
+      b1s[q] = bb;
+    }
+    __syncthreads();
+    // y pass: every x at the own y and z
+    const int lxt = T.lx * T.ty;
+    for (int q = threadIdx.x; q < n2; q += kTileThreads) {
+      const int zo = q / lxt, r2 = q - zo * lxt, yo = r2 / T.lx, x = r2 - yo * T.lx;
+      const int base = x + T.lx * (yo + T.ly * zo);   // products' y = yo + d, d = 0..4
+      const float* cy = cys + 10 * yo;
+      float c = 0.f, de = 0.f;
+#pragma unroll
+      for (int d = 0; d < 5; d++) {
+        const float a = a1s[base + d * T.lx], bb = b1s[base + d * T.lx];
+        c += cy[5 + d] * a;
+        de += cy[d] * a + cy[5 + d] * bb;
+      }
+      c2s[q] = c;
+      d2s[q] = de;
+    }
+    __syncthreads();
+    // x pass at this thread's row: own index o = x + nx (yo + ty zo)
+    if (act) {
+      const int yz = o / T.nx, x = o - yz * T.nx;
+      const int base = x + T.lx * yz;   // products' x = x + d, d = 0..4
+      const float* cx = cxs + 10 * x;
+#pragma unroll
+      for (int d = 0; d < 5; d++) acc += cx[d] * c2s[base + d] + cx[5 + d] * d2s[base + d];
+      if (grow == T.zp) acc += T.pen * tile[li];
+    }
+  } else {
   const int es0 = __builtin_amdgcn_readfirstlane(rw.z);
   const unsigned long long mixed = __ballot(act && rw.z != es0);
   if (mixed == 0) {
@@ -351,6 +506,7 @@ __global__ __launch_bounds__(kTileThreads, 8) void k_tile(TileGeom T, const int4
       const int2 t = e[k];
       acc += __int_as_float(t.y) * tile[li + t.x];
     }
+  }
   }
   double s0 = 0.0, s1 = 0.0;
   if (act) {
@@ -423,29 +579,37 @@ int launch_tile_rescan(fsx_ctx* ctx, const double* rh, float* r, const float* p)
   return launch_check(ctx, "k_tile_rescan");
 }
 
-template <int MODE>
+template <int MODE, bool TEN>
 static void launch_tile_ts(fsx_ctx* ctx, const TileGeom& T, const TileVecs& v, size_t lds) {
   const TilePlan& P = ctx->tile;
   const int L = T.lx * T.ly * T.lz;
   const dim3 grid(T.ntiles), block(kTileThreads);
   if (L <= 5 * kTileThreads)
-    hipLaunchKernelGGL((k_tile<MODE, 5>), grid, block, lds, ctx->stream, T, P.rows, P.ent, P.tent, v);
+    hipLaunchKernelGGL((k_tile<MODE, 5, TEN>), grid, block, lds, ctx->stream, T, P.rows, P.ent, P.tent, v);
   else if (L <= 9 * kTileThreads)
-    hipLaunchKernelGGL((k_tile<MODE, 9>), grid, block, lds, ctx->stream, T, P.rows, P.ent, P.tent, v);
+    hipLaunchKernelGGL((k_tile<MODE, 9, TEN>), grid, block, lds, ctx->stream, T, P.rows, P.ent, P.tent, v);
   else
-    hipLaunchKernelGGL((k_tile<MODE, kTileSlotsMax>), grid, block, lds, ctx->stream, T, P.rows, P.ent, P.tent, v);
+    hipLaunchKernelGGL((k_tile<MODE, kTileSlotsMax, TEN>), grid, block, lds, ctx->stream, T, P.rows, P.ent, P.tent, v);
+}
+
+template <int MODE>
+static void launch_tile_mode(fsx_ctx* ctx, const TileGeom& T, const TileVecs& v, size_t lds) {
+  if (ctx->tile.tensor) launch_tile_ts<MODE, true>(ctx, T, v, lds);
+  else launch_tile_ts<MODE, false>(ctx, T, v, lds);
 }
 
 int launch_tile(fsx_ctx* ctx, int mode, const TileVecs& v) {
   const TilePlan& P = ctx->tile;
-  const TileGeom T{P.nx, P.ny, P.nz, P.ty, P.tz, P.lx, P.ly, P.lz, P.tiles_y, P.ntiles, P.npart, P.rstride, P.estride};
+  const TileGeom T{P.nx, P.ny, P.nz, P.ty, P.tz, P.lx, P.ly, P.lz, P.tiles_y, P.ntiles, P.npart, P.rstride, P.estride,
+                   P.coef, P.zp, P.pen};
   const size_t L = (size_t)T.lx * T.ly * T.lz, nown = (size_t)T.nx * T.ty * T.tz;
-  const size_t lds = (L + 3 * nown + ((L + 3 * nown) & 1)) * sizeof(float) + (size_t)T.estride * sizeof(int2);
+  const size_t lds = P.tensor ? tensor_lds_floats(T.nx, T.ty, T.tz) * sizeof(float)
+                              : (L + 3 * nown + ((L + 3 * nown) & 1)) * sizeof(float) + (size_t)T.estride * sizeof(int2);
   switch (mode) {
-    case kTileDir: launch_tile_ts<kTileDir>(ctx, T, v, lds); break;
-    case kTileSmooth: launch_tile_ts<kTileSmooth>(ctx, T, v, lds); break;
-    case kTilePre: launch_tile_ts<kTilePre>(ctx, T, v, lds); break;
-    default: launch_tile_ts<kTilePost>(ctx, T, v, lds); break;
+    case kTileDir: launch_tile_mode<kTileDir>(ctx, T, v, lds); break;
+    case kTileSmooth: launch_tile_mode<kTileSmooth>(ctx, T, v, lds); break;
+    case kTilePre: launch_tile_mode<kTilePre>(ctx, T, v, lds); break;
+    default: launch_tile_mode<kTilePost>(ctx, T, v, lds); break;
   }
   return launch_check(ctx, "k_tile");
 }

@@ -37,7 +37,7 @@ DEVICE_SYMBOLS = (
     "fsx_upload_bc", "fsx_set_params", "fsx_set_solver", "fsx_set_graphs", "fsx_assemble_static", "fsx_build_Z", "fsx_init_state",
     "fsx_begin_step", "fsx_convection", "fsx_step1", "fsx_step2", "fsx_step3", "fsx_check_convergence",
     "fsx_next_iteration", "fsx_kacc", "fsx_end_step", "fsx_monitor", "fsx_download", "fsx_upload_vec", "fsx_sync",
-    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_mass_precision", "fsx_set_matrix_precision", "fsx_set_matrix_classes", "fsx_g_class_info", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
+    "fsx_solve_pressure", "fsx_solve_pressure_noalloc", "fsx_spmv", "fsx_bench_spmv", "fsx_bench_kernel", "fsx_bench_copy", "fsx_set_mass", "fsx_mass_stats", "fsx_set_mass_precision", "fsx_set_matrix_precision", "fsx_set_matrix_classes", "fsx_g_class_info", "fsx_set_z_tensor", "fsx_z_tensor_info", "fsx_set_step1_window", "fsx_set_step1_elements", "fsx_set_convection_form",
     "fsx_set_convection_precision", "fsx_set_solver_x0", "fsx_direct_stats", "fsx_set_direct_order", "fsx_direct_info", "fsx_z_patterns", "fsx_z_forms", "fsx_timing_reset", "fsx_timing_get",
     "fsx_unet_setup", "fsx_unet_params", "fsx_unet_get_weights", "fsx_unet_set_weights", "fsx_unet_gain",
     "fsx_unet_apply", "fsx_unet_loss_grad", "fsx_unet_capture", "fsx_unet_captured", "fsx_unet_prepare", "fsx_unet_prepare_random",
@@ -49,7 +49,7 @@ HOST_SYMBOLS = (
     "fsxh_topology", "fsxh_topo_free", "fsxh_topo_info", "fsxh_topo_len", "fsxh_topo_get",
     "fsxd_default_options", "fsxd_create", "fsxd_free", "fsxd_run", "fsxd_reset", "fsxd_ctx", "fsxd_topo", "fsxd_error",
     "fsxd_setup_times", "fsxh_write_tecplot", "fsxh_read_restart", "fsxd_write_tecplot", "fsxd_read_restart",
-    "fsxh_nd_order", "fsxh_g_classes",
+    "fsxh_nd_order", "fsxh_g_classes", "fsxh_z_tensor",
 )
 
 
@@ -182,9 +182,13 @@ def lib():
         "fsx_set_mass_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "fsx_direct_info": ([_vp, _D, _D, _I, _I], ctypes.c_int),
         "fsx_set_matrix_classes": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_set_z_tensor": ([_vp, ctypes.c_int], ctypes.c_int),
+        "fsx_z_tensor_info": ([_vp, _I, _D], ctypes.c_int),
         "fsx_g_class_info": ([_vp, _I, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(ctypes.c_longlong)],
                              ctypes.c_int),
         "fsxh_g_classes": ([ctypes.c_int, ctypes.c_int, _I, _I, _D, ctypes.c_double, _I, _I, _I, _F], ctypes.c_int),
+        "fsxh_z_tensor": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int, _I,
+                           _I, _D, ctypes.c_int, _D, _D], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -229,6 +233,26 @@ def g_classes(rp, ci, g3, ncols, prune_rel=2.0**-24):
     if cnt[0] < 0:
         return None
     return dict(counts=tuple(int(c) for c in cnt), row_cls=rc_, trow_cls=tc, g3_f32=g32)
+
+
+def z_tensor(shape, xc, ngp, rho, rp, ci, v, zero_p):
+    """Z of a tensor-product grid as Kronecker products of 1D stencils
+    (fsxh_z_tensor): shape (nx, ny, nz), corner coordinates xc (n, 3), Z's
+    rows (rp, ci, v). Returns dict(holds, err, pen, coef) with coef the
+    ((nx + ny + nz), 10) tables (K's five coefficients, then M's), None when
+    the coordinates do not separate."""
+    nx, ny, nz = (int(a) for a in shape)
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    xc = np.ascontiguousarray(xc, dtype=np.float64)
+    coef = np.zeros((nx + ny + nz, 10))
+    info = np.zeros(3)
+    rc = lib().fsxh_z_tensor(nx, ny, nz, _dp(xc), int(ngp), float(rho), rp.size - 1, _ip(rp), _ip(ci), _dp(v),
+                             int(zero_p), _dp(coef), _dp(info))
+    if rc != FSX_OK:
+        raise FsxError(rc, "fsxh_z_tensor: inconsistent sizes")
+    return dict(holds=bool(info[2]), err=float(info[0]), pen=float(info[1]), coef=coef if info[0] >= 0 else None)
 
 
 def nd_order(rp, ci, xyz, leaf=64):
@@ -497,6 +521,19 @@ class Solver:
         """At matrix precision 32: G and G^T as row classes where the rows
         compress (True, the default) or the FP32 CSR copies (False)."""
         self.check(self.L.fsx_set_matrix_classes(self.ctx, int(bool(on))), "set_matrix_classes")
+
+    def set_z_tensor(self, on):
+        """The learned solver's tiles apply Z as Kronecker products of 1D
+        stencils where that form holds (True, the default) or the FP32 row
+        classes (False)."""
+        self.check(self.L.fsx_set_z_tensor(self.ctx, int(bool(on))), "set_z_tensor")
+
+    def z_tensor_info(self):
+        """(in use, largest relative mismatch against Z's rows or -1)."""
+        a = ctypes.c_int()
+        e = ctypes.c_double()
+        self.check(self.L.fsx_z_tensor_info(self.ctx, ctypes.byref(a), ctypes.byref(e)), "z_tensor_info")
+        return bool(a.value), e.value
 
     def g_class_info(self):
         """(G's classes, their entries, G^T's classes, theirs) of the class

@@ -178,6 +178,21 @@ int fsx_set_matrix_precision(fsx_ctx* ctx, int bits);
  * the CSR copies. No reference counterpart (the reference stores G as three
  * CSR value arrays, BCH:3322-3513). */
 int fsx_set_matrix_classes(fsx_ctx* ctx, int on);
+/* Whether the learned solver's LDS-tiled Z products (there where Z's FP32
+ * rows form classes on a structured grid) apply Z as Kronecker products of
+ * 1D stencils (1, the default) where the pressure grid is a tensor-product
+ * box grid and that form reproduces Z's rows to 1e-10 of each row's
+ * largest magnitude (the pinned node's penalty added on its diagonal;
+ * fsxh_z_tensor builds the same tables on the host): 45 multiply-adds a
+ * row from three passes over the tile instead of the 81 entries of its
+ * FP32 row class. 0 keeps the row classes. Rebuilds the solvers' forms of Z
+ * when it is built. No reference counterpart (the reference applies the
+ * assembled CSR Z, CU:418-451). */
+int fsx_set_z_tensor(fsx_ctx* ctx, int on);
+/* in_use: 1 when the tiles apply the Kronecker form; err: its largest
+ * mismatch against Z's rows (-1 when the grid is not a box grid or Z is not
+ * built); pointers may be NULL */
+int fsx_z_tensor_info(fsx_ctx* ctx, int* in_use, double* err);
 /* The class form in use: G's classes and their entries, G^T's classes and
  * theirs (all 0 when the kernels read the CSR copies); pointers may be NULL */
 int fsx_g_class_info(fsx_ctx* ctx, int* classes, long long* entries, int* t_classes, long long* t_entries);

@@ -167,6 +167,27 @@ int fsxh_nd_order(int n, const int* rp, const int* ci, const double* xyz, int le
 int fsxh_g_classes(int nrows, int ncols, const int* rp, const int* ci, const double* g3, double prune_rel,
                    int* counts5, int* row_cls, int* trow_cls, float* g3_f32);
 
+/* ------------------------------------------------------------ Z on a tensor-product grid
+ * The learned solver's LDS-tiled Z products apply Z as Kronecker products
+ * of 1D stencils where the pressure grid (nx x ny x nz nodes, x fastest,
+ * corner coordinates xc, 3 a node) is a tensor-product box grid:
+ *   Z = Kx (x) My (x) Mz + Mx (x) Ky (x) Mz + Mx (x) My (x) Kz,
+ * K = D^T m^-1 D / rho^2 and M = B^T m^-1 B per axis, from the 1D quadratic
+ * velocity and linear pressure bases with the NGP rule's points per axis
+ * (D: derivative of the velocity basis against the pressure basis, B: the
+ * bases' product, m: the row-sum lumped 1D velocity mass), i.e. G_x =
+ * -(1/rho) D_x (x) B_y (x) B_z of BCH:3205-3207 and the lumped mass of
+ * BCH:3262-3278 factored. The form is checked against Z's rows (rp, ci, v
+ * of the n = nx ny nz nodes, structural zeros dropped) within 1e-10 of each
+ * row's largest magnitude, the pinned node zero_p's diagonal (its penalty)
+ * aside. coef ((nx + ny + nz) x 10, may be NULL) receives per axis and
+ * grid point K's five coefficients (offsets -2..2) then M's; info3 the
+ * largest relative mismatch (-1 when the coordinates do not separate), the
+ * pinned diagonal's difference and 1 when the form holds. The device builds
+ * the same tables. FSX_EINVAL for inconsistent sizes. */
+int fsxh_z_tensor(int nx, int ny, int nz, const double* xc, int ngp, double rho, int n, const int* rp, const int* ci,
+                  const double* v, int zero_p, double* coef, double* info3);
+
 #ifdef __cplusplus
 }
 #endif

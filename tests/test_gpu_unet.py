@@ -316,3 +316,58 @@ def test_random_pairs_append_and_train(fsx_lib):
     with pytest.raises(fsx_lib.FsxError):
         s.unet_prepare_random(2, sweeps=0, seed=3)
     s.close()
+
+
+@pytest.mark.parametrize("case", ["uniform", "sinh"])
+def test_tensor_tiles_against_class_tiles(fsx_lib, case):
+    """On a box grid the tiles apply Z as Kronecker products of 1D stencils
+    (fsx_set_z_tensor; the form reproduces Z's rows to 1e-12 here). The
+    two-level preconditioner with it against the one with the FP32 row
+    classes: 1e-5 normwise (FP32 sums in another order), and the learned
+    solve meets ||b - Z x|| <= 1e-8 ||b|| on the FP64 Z in the same number
+    of iterations give or take one. (The sinh-clustered 16^3 grid still
+    forms row classes; at 64^3 it does not, and there only the tensor form
+    gives the tiles.)"""
+    N = 16
+    mesh = fsx_lib.Mesh.cavity(N, cluster=2.0 if case == "sinh" else 0.0)
+    mesh.set_params(NGP=27, dt=0.01, maxIter=3, tolerance=1e-3, converge=1e-6, viscosity=0.01)
+    s = fsx_lib.Solver(mesh, device=0, NGP=27, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=1e-8,
+                       pcg_maxit=20000, solver_mode=1)
+    s.run(2)
+    b = s.download("R2")
+    Z = host_z(s)
+    s.unet_setup((N + 1,) * 3, seed=7)
+    used, err = s.z_tensor_info()
+    say(f"  {case}: tensor form in use {used}, row mismatch {err:.1e}")
+    assert used and 0.0 <= err < 1e-12
+    r = np.random.default_rng(5).normal(size=(N + 1) ** 3).astype(np.float32)
+    zt = s.unet_precondition(r)
+    s.set_solver(2, 1e-8, 20000)
+    xt, itt, _, rc = s.solve_pressure(b)
+    assert rc == 0 and float(np.linalg.norm(b - Z @ xt) / np.linalg.norm(b)) <= 1.0001e-8
+    s.set_z_tensor(False)
+    assert not s.z_tensor_info()[0]
+    s.set_solver(2, 1e-8, 20000)
+    xc, itc, _, rc = s.solve_pressure(b)
+    assert rc == 0 and float(np.linalg.norm(b - Z @ xc) / np.linalg.norm(b)) <= 1.0001e-8
+    say(f"  iterations: tensor tiles {itt}, class tiles {itc}")
+    zc = s.unet_precondition(r)
+    e = rel(zt, zc.astype(np.float64))
+    say(f"  preconditioner, tensor vs class tiles: {e:.2e}")
+    assert e < 1e-5
+    assert abs(itt - itc) <= 1
+    s.set_z_tensor(True)
+    assert s.z_tensor_info()[0]
+    s.close()
+    mesh.close()
+
+
+def test_tensor_form_not_on_the_duct(fsx_lib):
+    """The bent duct's pressure nodes form no box grid: no tensor form."""
+    mesh = fsx_lib.Mesh.duct(n=4, n_in=6, n_arc=6, n_out=8, seed=3)
+    mesh.set_params(NGP=27, dt=0.005, maxIter=3, tolerance=1e-3, converge=1e-6, viscosity=1.0 / 200)
+    s = fsx_lib.Solver(mesh, device=0, NGP=27, maxIter=3, tolerance=1e-3, converge=1e-6, pcg_rtol=1e-8,
+                       pcg_maxit=20000, solver_mode=1)
+    assert s.z_tensor_info() == (False, -1.0)
+    s.close()
+    mesh.close()
