@@ -4,10 +4,12 @@ Headline workload (the 64^3 cavity): lid-driven cavity on the unit cube,
 64^3 hex27-velocity / hex8-pressure elements (129^3 velocity nodes, 65^3
 pressure nodes), 27-point Gauss quadrature, Re = 1000 (viscosity 1/1000 with
 lid speed 1), dt = 0.0025, 3 inner iterations per step at tolerance 1e-3,
-pressure solved to a relative residual of 1e-8 from the previous Pdot in
-every mode (the reference's GPU path warm-starts its CG the same way; --x0
-zero gives its CPU path's cold start). Inputs are synthetic (generated mesh,
-zero initial fields plus the lid BC) and resident in HBM before timing starts.
+pressure solved to a relative residual of 1e-8, the FP64 mode from the
+previous Pdot (the reference's GPU path warm-starts its CG the same way; --x0
+zero gives its CPU path's cold start), the mixed and learned modes from the
+Z-norm projection of the solution onto the last 8 solutions (--x0-accel).
+Inputs are synthetic (generated mesh, zero initial fields plus the lid BC)
+and resident in HBM before timing starts.
 
 One "step" is one time step of the reference's timeLoop (BCH:2859-3118):
 convection residual, step1, step2 (assembly of R2 + the pressure solve),
@@ -97,7 +99,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-X0_POLICY = {"zero": 0, "previous": 1}
+X0_POLICY = {"zero": 0, "previous": 1, "projection": 2}
 GRAPHS = 0   # --graphs
 
 
@@ -242,8 +244,8 @@ def duct_workload(fsx, args, steps, warmup, cpu_steps):
         if name not in args.duct_modes.split(","):
             continue
         s.set_mass_precision(bits)   # the mixed forms' mass solves: FP32 segments, FP64 residuals
-        r = run_mode(s, mode, steps, warmup, args.rtol, args.maxit, matrix_bits=bits, x0=args.x0,
-                     total=warmup + steps)
+        r = run_mode(s, mode, steps, warmup, args.rtol, args.maxit, matrix_bits=bits,
+                     x0=args.x0 if name == "fp64" else args.x0_accel, total=warmup + steps)
         m_it, m_rel, m_tot = s.mass_stats()
         res[name] = {"steps_per_s": round(r["steps_per_s"], 4), "pcg_iters": r["pcg_iters"],
                      "inner_iters": r["inner_iters"], "solve_ms": round(r["solve_ms"], 2),
@@ -342,7 +344,8 @@ def headline(args):
             solver.set_direct_order(fsx.DIRECT_ND if name == "direct" else fsx.DIRECT_BAND)
         log(f"mode {name}: {args.warmup} warm-up + {args.steps} timed steps, trajectory of {args.trajectory}")
         res[name] = run_mode(solver, modes[name], args.steps, args.warmup, args.rtol, args.maxit,
-                             matrix_bits=64 if name in ("fp64", "direct", "direct_band") else 32, x0=args.x0,
+                             matrix_bits=64 if name in ("fp64", "direct", "direct_band") else 32,
+                             x0=args.x0 if name in ("fp64", "direct", "direct_band") else args.x0_accel,
                              total=max(args.trajectory, args.warmup + args.steps))
         r = res[name]
         if name in ("direct", "direct_band"):   # the factor is formed in the first warm-up step
@@ -420,7 +423,7 @@ def headline(args):
         "config": {"workload": f"lid-driven cavity {args.n}^3 hex27/hex8" +
                                (f" (sinh clustering {args.cluster:g})" if args.cluster else "") +
                                f", NGP {args.ngp}, Re {args.re:g}, dt {args.dt:g}, 3 inner iterations, "
-                               f"pressure rtol {args.rtol:g}, pressure x0 {args.x0}, timed steps "
+                               f"pressure rtol {args.rtol:g}, pressure x0 {args.x0} (fp64), {args.x0_accel} (mixed, learned), timed steps "
                                f"{args.warmup + 1}-{args.warmup + args.steps} of {args.trajectory}",
                    "mode": head, "NN": solver.sizes["NN"], "NNp": solver.sizes["NNp"],
                    "nnz_Z": solver.sizes["nnzZ"], "parallelism": "single GPU",
@@ -505,9 +508,12 @@ def main():
                     help="only set up the case, run one FP64 time step and one with FP32 matrices and convection, time "
                          "the step kernels and the Z SpMV kernels and run a 256 MiB "
                          "calibration copy (the workload of the rocprofv3 --pmc traffic runs)")
-    ap.add_argument("--x0", default="previous", choices=("zero", "previous"),
-                    help="initial guess of every pressure solve, in all modes: the previous Pdot (the reference's "
-                         "GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
+    ap.add_argument("--x0", default="previous", choices=("zero", "previous", "projection"),
+                    help="initial guess of the FP64 mode's pressure solves (and of the 8^3 and 32^3 workloads): the "
+                         "previous Pdot (the reference's GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
+    ap.add_argument("--x0-accel", default="projection", choices=("zero", "previous", "projection"),
+                    help="initial guess of the mixed and learned modes' pressure solves: the Z-norm projection onto "
+                         "the last 8 solutions (default), the previous Pdot or zero")
     ap.add_argument("--workloads", default="cavity8,cavity32,duct",
                     help="other configurations to run beside the headline (empty for none)")
     ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),

@@ -306,7 +306,8 @@ int fsx_g_class_info(fsx_ctx* ctx, int* classes, long long* entries, int* t_clas
 
 int fsx_set_solver_x0(fsx_ctx* ctx, int policy) {
   if (!ctx) return FSX_EINVAL;
-  if (policy != FSX_X0_ZERO && policy != FSX_X0_PREVIOUS) return fail(ctx, FSX_EINVAL, "fsx_set_solver_x0: unknown policy");
+  if (policy != FSX_X0_ZERO && policy != FSX_X0_PREVIOUS && policy != FSX_X0_PROJECTION)
+    return fail(ctx, FSX_EINVAL, "fsx_set_solver_x0: unknown policy");
   ctx->x0_policy = policy;
   return FSX_OK;
 }
@@ -847,6 +848,7 @@ int fsx_build_Z(fsx_ctx* ctx) {
   ctx->z_built = true;
   ctx->band_ready = false;   // a direct solve factors the new Z
   fsx::mf_invalidate(ctx);
+  fsx::proj_reset(ctx);      // the kept solutions are of the old Z
   return FSX_OK;
 }
 
@@ -858,6 +860,7 @@ int fsx_init_state(fsx_ctx* ctx) {
   ctx->conv_pending = false;
   ctx->gpn_valid = ctx->gpn_next_valid = false;
   drop_records(ctx);
+  fsx::proj_reset(ctx);
   for (double* v : {ctx->Un, ctx->Unp1, ctx->Unp1_prev, ctx->UnpHalf, ctx->UnpHalf_prev, ctx->Acc, ctx->Acc_prev,
                     ctx->KAcc, ctx->R1})
     FSX_TRY(launch_fill(ctx, v, 0.0, n3));
@@ -1097,6 +1100,7 @@ int fsx_upload_vec(fsx_ctx* ctx, int id, const double* host, size_t count) {
   if (id == FSX_VAL_Z) {
     ctx->band_ready = false;
     mf_invalidate(ctx);
+    proj_reset(ctx);
   }
   FSX_HIP(hipStreamSynchronize(ctx->stream));
   if (kind == 3) {

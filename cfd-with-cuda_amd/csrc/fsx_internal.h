@@ -379,6 +379,12 @@ struct fsx_ctx {
   // solver work (device)
   double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wq = nullptr, *wh = nullptr;    // FP64 PCG (+ FP64 residual)
   double *w0r = nullptr, *w0e = nullptr;   // warm start: b - Z x0 and the correction
+  // FSX_X0_PROJECTION: the last kProjMax solutions and right-hand sides (slot
+  // i at proj_x + i NNp), the window's Gram entries G[i][j] = x_i . b_j on
+  // the host, and the slot whose row and column of G are still to be formed
+  double *proj_x = nullptr, *proj_b = nullptr, *proj_part = nullptr;
+  int proj_n = 0, proj_next = 0, proj_pending = 0;   // pending: bit mask of slots
+  std::vector<double> proj_G;
   int x0_policy = FSX_X0_ZERO;
   float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 PCG
   double* fpq = nullptr; int fpq_n = 0;   // learned solver: partial sums of p.Zp from its SpMV
@@ -495,6 +501,8 @@ void mf_invalidate(fsx_ctx* ctx);
 int mf_info(const fsx_ctx* ctx, double* l_bytes, double* flops, int* fronts, int* depth, double* factor_ms);
 int launch_nodes_pass(fsx_ctx* ctx, bool kacc, bool gload);
 int refresh_matrix_copies(fsx_ctx* ctx);
+// empties the FSX_X0_PROJECTION window (fsx_solve.hip)
+void proj_reset(fsx_ctx* ctx);
 // whether step3 and step2's right-hand side read G and G^T as row classes
 inline bool g_classes_on(const fsx_ctx* ctx) { return ctx->Gf && ctx->GTf && ctx->Gc_row && ctx->use_g_classes; }
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);

@@ -1584,16 +1584,170 @@ static int solve_warm(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
   return rc;
 }
 
+// ------------------------------------------------------------ projection initial guess
+// FSX_X0_PROJECTION (P. F. Fischer, "Projection techniques for iterative
+// solution of Ax = b with successive right-hand sides", CMAME 163, 1998):
+// x0 = X a, the Z-norm best approximation of Z^-1 b in the span of the last
+// kProjMax solutions X, from (X^T Z X) a = X^T b. Z x_j is taken as the
+// right-hand side b_j it was solved for (they differ by that solve's
+// residual), so G[i][j] = x_i . b_j needs no product with Z; the Gram
+// system is solved on the host by a Cholesky factorization that drops the
+// vectors adding under 1e-10 of their own Z norm to the span (newest first).
+constexpr int kProjMax = 8;
+constexpr int kProjPairs = kProjMax + kProjMax * kProjMax;   // X^T b and every Gram entry at most
+struct ProjPairs {
+  const double* a[kProjPairs];
+  const double* b[kProjPairs];
+  int n;
+};
+struct ProjSum {
+  const double* x[kProjMax];
+  double c[kProjMax];
+  int n;
+};
+
+// part[k kRedGrid + block]: the block's share (a contiguous range) of a_k . b_k
+__global__ __launch_bounds__(kBlock) void k_proj_dots(int n, ProjPairs P, double* __restrict__ part) {
+  __shared__ double red[kBlock / kWave];
+  const int per = (n + kRedGrid - 1) / kRedGrid;
+  const int lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  for (int k = 0; k < P.n; k++) {
+    const double* __restrict__ a = P.a[k];
+    const double* __restrict__ bb = P.b[k];
+    double v[1] = {0.0};
+    for (int i = lo + threadIdx.x; i < hi; i += kBlock) v[0] += a[i] * bb[i];
+    block_sum_all<1>(v, red);
+    if (threadIdx.x == 0) part[k * kRedGrid + blockIdx.x] = v[0];
+    __syncthreads();   // red is reused by the next pair
+  }
+}
+
+// x = sum_j c_j x_j, in the order of j
+__global__ void k_proj_combine(int n, ProjSum S, double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int j = 0; j < S.n; j++) v += S.c[j] * S.x[j][i];
+    x[i] = v;
+  }
+}
+
+void proj_reset(fsx_ctx* ctx) {
+  ctx->proj_n = 0;
+  ctx->proj_next = 0;
+  ctx->proj_pending = 0;
+}
+
+static int proj_alloc(fsx_ctx* ctx) {
+  if (ctx->proj_x) return FSX_OK;
+  const size_t N = ctx->NNp;
+  FSX_TRY(alloc(ctx, &ctx->proj_x, kProjMax * N));
+  FSX_TRY(alloc(ctx, &ctx->proj_b, kProjMax * N));
+  FSX_TRY(alloc(ctx, &ctx->proj_part, (size_t)kProjPairs * kRedGrid));
+  ctx->proj_G.assign(kProjMax * kProjMax, 0.0);
+  proj_reset(ctx);
+  return FSX_OK;
+}
+
+// x0 into x (left as it is while the window is empty); proj_pending is the
+// bit mask of slots whose row and column of G are still to be formed
+static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
+  FSX_TRY(proj_alloc(ctx));
+  const int m = ctx->proj_n;
+  if (m == 0) return FSX_OK;
+  const size_t N = ctx->NNp;
+  const unsigned stale = (unsigned)ctx->proj_pending;
+  ProjPairs P{};
+  int k = 0;
+  int gi[kProjPairs], gj[kProjPairs];
+  for (int i = 0; i < m; i++) { P.a[k] = ctx->proj_x + i * N; P.b[k] = b; gi[k] = -1; gj[k] = i; k++; }
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < m; j++)
+      if (((stale >> i) & 1u) || ((stale >> j) & 1u)) {
+        P.a[k] = ctx->proj_x + i * N; P.b[k] = ctx->proj_b + j * N; gi[k] = i; gj[k] = j; k++;
+      }
+  P.n = k;
+  hipLaunchKernelGGL(k_proj_dots, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, (int)N, P, ctx->proj_part);
+  FSX_TRY(launch_check(ctx, "k_proj_dots"));
+  std::vector<double> h((size_t)k * kRedGrid);
+  FSX_HIP(hipMemcpyAsync(h.data(), ctx->proj_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  double d[kProjMax] = {0.0};
+  std::vector<double>& G = ctx->proj_G;
+  for (int q = 0; q < k; q++) {
+    double t = 0.0;
+    for (int blk = 0; blk < kRedGrid; blk++) t += h[(size_t)q * kRedGrid + blk];
+    if (!std::isfinite(t)) return FSX_OK;   // the guess is skipped; the solve meets its own test
+    if (gi[q] < 0) d[gj[q]] = t;
+    else G[gi[q] * kProjMax + gj[q]] = t;
+  }
+  ctx->proj_pending = 0;
+  // Cholesky of the symmetrised Gram matrix over the kept slots, newest first
+  int sel[kProjMax], ns = 0;
+  double L[kProjMax][kProjMax] = {};
+  for (int r = 0; r < m; r++) {
+    const int c = (ctx->proj_next - 1 - r + 2 * kProjMax) % kProjMax;
+    const double gcc = G[c * kProjMax + c];
+    if (!(gcc > 0.0)) continue;
+    double l[kProjMax], piv = gcc;
+    for (int q = 0; q < ns; q++) {
+      const int o = sel[q];
+      double v = 0.5 * (G[c * kProjMax + o] + G[o * kProjMax + c]);
+      for (int t = 0; t < q; t++) v -= L[q][t] * l[t];
+      l[q] = v / L[q][q];
+      piv -= l[q] * l[q];
+    }
+    if (!(piv > 1e-10 * gcc)) continue;
+    for (int q = 0; q < ns; q++) L[ns][q] = l[q];
+    L[ns][ns] = std::sqrt(piv);
+    sel[ns++] = c;
+  }
+  if (ns == 0) return FSX_OK;
+  double y[kProjMax], a[kProjMax];
+  for (int q = 0; q < ns; q++) {
+    double v = d[sel[q]];
+    for (int t = 0; t < q; t++) v -= L[q][t] * y[t];
+    y[q] = v / L[q][q];
+  }
+  for (int q = ns - 1; q >= 0; q--) {
+    double v = y[q];
+    for (int t = q + 1; t < ns; t++) v -= L[t][q] * a[t];
+    a[q] = v / L[q][q];
+  }
+  ProjSum S{};
+  S.n = ns;
+  for (int q = 0; q < ns; q++) { S.x[q] = ctx->proj_x + sel[q] * N; S.c[q] = a[q]; }
+  hipLaunchKernelGGL(k_proj_combine, dim3(ew_grid((long long)N)), dim3(kBlock), 0, ctx->stream, (int)N, S, x);
+  return launch_check(ctx, "k_proj_combine");
+}
+
+// the solution x of b into the window (the oldest slot when it is full)
+static int proj_record(fsx_ctx* ctx, const double* b, const double* x) {
+  FSX_TRY(proj_alloc(ctx));
+  const size_t N = ctx->NNp;
+  const int s = ctx->proj_next;
+  FSX_TRY(launch_copy(ctx, ctx->proj_x + s * N, x, (long long)N));
+  FSX_TRY(launch_copy(ctx, ctx->proj_b + s * N, b, (long long)N));
+  ctx->proj_next = (s + 1) % kProjMax;
+  ctx->proj_n = std::min(kProjMax, ctx->proj_n + 1);
+  ctx->proj_pending |= 1 << s;
+  return FSX_OK;
+}
+
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   int rc;
   timing_begin(ctx);
   // (a direct solve does not use an initial guess; the tiled learned solver
   // starts from x on the device)
-  if (ctx->x0_policy == FSX_X0_PREVIOUS && learned_tiled(ctx)) rc = solve_learned_tiled(ctx, b, x, true, iters, relres, 0);
-  else if (ctx->x0_policy == FSX_X0_PREVIOUS && ctx->solver_mode != FSX_SOLVER_DIRECT) rc = solve_warm(ctx, b, x, iters, relres);
+  const bool direct = ctx->solver_mode == FSX_SOLVER_DIRECT;
+  const bool proj = ctx->x0_policy == FSX_X0_PROJECTION && !direct;
+  const bool warm = (ctx->x0_policy == FSX_X0_PREVIOUS || proj) && !direct;
+  if (proj) FSX_TRY(proj_guess(ctx, b, x));
+  if (warm && learned_tiled(ctx)) rc = solve_learned_tiled(ctx, b, x, true, iters, relres, 0);
+  else if (warm) rc = solve_warm(ctx, b, x, iters, relres);
   else rc = solve_cold(ctx, b, x, iters, relres);
   if (rc == FSX_OK || rc == FSX_ENOCONV) timing_end(ctx, *iters);
   ctx->last_relres = *relres;
+  if (proj && rc == FSX_OK) FSX_TRY(proj_record(ctx, b, x));
   if (ctx->capture_hook && rc == FSX_OK) ctx->capture_hook(ctx, b, x);
   return rc;
 }

@@ -25,7 +25,7 @@ CSR_M, CSR_G, CSR_Z = 0, 1, 2
 SOLVER_FP64, SOLVER_MIXED, SOLVER_LEARNED, SOLVER_DIRECT = 0, 1, 2, 3
 KERNEL_STEP1, KERNEL_KACC, KERNEL_STEP2_RHS, KERNEL_STEP3, KERNEL_STEP1_ELEMENTS, KERNEL_STEP1_NODES = 0, 1, 2, 3, 4, 5
 MASS_LUMPED, MASS_CONSISTENT, MASS_CONSISTENT_CSR = 0, 1, 2
-X0_ZERO, X0_PREVIOUS = 0, 1
+X0_ZERO, X0_PREVIOUS, X0_PROJECTION = 0, 1, 2
 DIRECT_BAND, DIRECT_ND = 0, 1
 
 VEC = dict(Un=0, Unp1=1, Unp1_prev=2, UnpHalf=3, UnpHalf_prev=4, Acc=5, KAcc=6, R1=7, Pn=8, Pnp1=9,
@@ -556,8 +556,9 @@ class Solver:
         self.check(self.L.fsx_set_step1_elements(self.ctx, int(bool(enable))), "set_step1_elements")
 
     def set_solver_x0(self, policy):
-        """X0_ZERO (the CPU path) or X0_PREVIOUS (warm start from the previous
-        Pdot, the reference's GPU path)."""
+        """X0_ZERO (the CPU path), X0_PREVIOUS (warm start from the previous
+        Pdot, the reference's GPU path) or X0_PROJECTION (from the Z-norm
+        projection onto the last 8 solutions)."""
         self.check(self.L.fsx_set_solver_x0(self.ctx, policy), "set_solver_x0")
 
     def direct_stats(self):

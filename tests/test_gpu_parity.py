@@ -532,6 +532,44 @@ def test_warm_start(fsx_lib, oracle_lib, mode):
     s.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_projection_start(fsx_lib, oracle_lib, mode):
+    """FSX_X0_PROJECTION (x0 the Z-norm projection onto the last 8 solutions)
+    in the FP64, mixed and learned solvers against the oracle's
+    warm-started PCG: fields after 6 steps to 1e-6 and every solve at its
+    stop test; fewer iterations than FSX_X0_PREVIOUS over the same steps,
+    and after a reset (an empty window) one step about as many iterations as
+    the previous-solution start's."""
+    say(f"projection x0, solver mode {mode}: cavity 16^3, 6 steps")
+    s, o = make_pair(fsx_lib, oracle_lib, fsx_lib.Mesh.cavity(16), NGP=27, pcg_rtol=1e-10)
+    if mode == 2:
+        s.unet_setup((17, 17, 17), seed=3)
+    o.set_pcg_x0(1)
+    so = o.run(6)
+    its = {}
+    for pol in (fsx_lib.X0_PREVIOUS, fsx_lib.X0_PROJECTION):
+        s.reset(mode=mode, rtol=1e-10, maxit=5000)
+        s.set_solver_x0(pol)
+        st = s.run(6)
+        assert all(x["relres_max"] <= 1e-10 for x in st)
+        assert rel(s.download("Un"), o.d("Un")) < 1e-6
+        assert rel(s.download("Pn"), o.d("Pn")) < 1e-6
+        its[pol] = [x["pcg"] for x in st]
+    say(f"  iterations per step: previous {its[fsx_lib.X0_PREVIOUS]}, projection {its[fsx_lib.X0_PROJECTION]}")
+    assert sum(its[fsx_lib.X0_PROJECTION]) < sum(its[fsx_lib.X0_PREVIOUS])
+    s.reset(mode=mode, rtol=1e-10, maxit=5000)
+    s.set_solver_x0(fsx_lib.X0_PREVIOUS)
+    a = s.run(1)[0]["pcg"]
+    s.reset(mode=mode, rtol=1e-10, maxit=5000)
+    s.set_solver_x0(fsx_lib.X0_PROJECTION)
+    b1 = s.run(1)[0]["pcg"]
+    # one step is three solves: the first from zero in both, the next two
+    # from the last solution or its Z-norm best multiple (then the best
+    # combination of two): no worse in the Z norm, so about as many iterations
+    assert b1 <= a + 2, (b1, a)
+    s.close()
+
+
 @pytest.mark.parametrize("order", ["band", "nd"])
 @pytest.mark.parametrize("case", ["cavity16", "duct"])
 def test_direct_pressure_solve(fsx_lib, oracle_lib, case, order):
