@@ -7,7 +7,7 @@ lid speed 1), dt = 0.0025, 3 inner iterations per step at tolerance 1e-3,
 pressure solved to a relative residual of 1e-8, the FP64 mode from the
 previous Pdot (the reference's GPU path warm-starts its CG the same way; --x0
 zero gives its CPU path's cold start), the mixed and learned modes from the
-Z-norm projection of the solution onto the last 8 solutions (--x0-accel).
+Z-norm projection of the solution onto the last 16 solutions (--x0-accel).
 Inputs are synthetic (generated mesh, zero initial fields plus the lid BC)
 and resident in HBM before timing starts.
 
@@ -513,7 +513,7 @@ def main():
                          "previous Pdot (the reference's GPU path, CUDAcodes.cu:425-442; default) or zero (its CPU path)")
     ap.add_argument("--x0-accel", default="projection", choices=("zero", "previous", "projection"),
                     help="initial guess of the mixed and learned modes' pressure solves: the Z-norm projection onto "
-                         "the last 8 solutions (default), the previous Pdot or zero")
+                         "the last 16 solutions (default), the previous Pdot or zero")
     ap.add_argument("--workloads", default="cavity8,cavity32,duct",
                     help="other configurations to run beside the headline (empty for none)")
     ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),

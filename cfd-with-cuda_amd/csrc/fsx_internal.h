@@ -379,9 +379,10 @@ struct fsx_ctx {
   // solver work (device)
   double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wq = nullptr, *wh = nullptr;    // FP64 PCG (+ FP64 residual)
   double *w0r = nullptr, *w0e = nullptr;   // warm start: b - Z x0 and the correction
-  // FSX_X0_PROJECTION: the last kProjMax solutions and right-hand sides (slot
-  // i at proj_x + i NNp), the window's Gram entries G[i][j] = x_i . b_j on
-  // the host, and the slot whose row and column of G are still to be formed
+  // FSX_X0_PROJECTION: the last kProjMax solutions x_j and their products
+  // Z x_j (slot i at proj_x + i NNp, proj_b + i NNp), the window's Gram
+  // entries G[i][j] = x_i . Z x_j on the host, and the slots whose row and
+  // column of G are still to be formed
   double *proj_x = nullptr, *proj_b = nullptr, *proj_part = nullptr;
   int proj_n = 0, proj_next = 0, proj_pending = 0;   // pending: bit mask of slots
   std::vector<double> proj_G;

@@ -1588,38 +1588,57 @@ static int solve_warm(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
 // FSX_X0_PROJECTION (P. F. Fischer, "Projection techniques for iterative
 // solution of Ax = b with successive right-hand sides", CMAME 163, 1998):
 // x0 = X a, the Z-norm best approximation of Z^-1 b in the span of the last
-// kProjMax solutions X, from (X^T Z X) a = X^T b. Z x_j is taken as the
-// right-hand side b_j it was solved for (they differ by that solve's
-// residual), so G[i][j] = x_i . b_j needs no product with Z; the Gram
-// system is solved on the host by a Cholesky factorization that drops the
-// vectors adding under 1e-10 of their own Z norm to the span (newest first).
-constexpr int kProjMax = 8;
-constexpr int kProjPairs = kProjMax + kProjMax * kProjMax;   // X^T b and every Gram entry at most
-struct ProjPairs {
-  const double* a[kProjPairs];
-  const double* b[kProjPairs];
-  int n;
-};
+// kProjMax solutions X, from (X^T Z X) a = X^T b. Z x_j is kept beside x_j
+// (proj_b), from the solve's own final FP64 residual, so G[i][j] =
+// x_i . (Z x_j) needs no product with Z; the Gram system is solved on the
+// host by a Cholesky factorization that drops the vectors adding under
+// 1e-10 of their own Z norm to the span (newest first).
+// 16 kept solutions: 64^3 cavity, learned solve 48.4 ms over 20 steps (8:
+// 53.2 ms with Z x_j taken as b_j; 24: 51.1 ms, the longer dot and
+// combination passes outweighing the iterations saved)
+constexpr int kProjMax = 16;
+constexpr int kProjDots = 3 * kProjMax;   // X^T b and one slot's row and column of G
 struct ProjSum {
   const double* x[kProjMax];
   double c[kProjMax];
   int n;
 };
 
-// part[k kRedGrid + block]: the block's share (a contiguous range) of a_k . b_k
-__global__ __launch_bounds__(kBlock) void k_proj_dots(int n, ProjPairs P, double* __restrict__ part) {
-  __shared__ double red[kBlock / kWave];
+// One pass over the vectors: X^T b and, when a slot s was recorded since the
+// last guess (s >= 0), its row and column of G. Block shares of x_j . b
+// (part rows 0..m-1), x_s . Z x_j (m..2m-1) and x_j . Z x_s (2m..3m-1),
+// each block a contiguous range.
+__global__ __launch_bounds__(kBlock) void k_proj_dots1(int n, int m, int s, const double* __restrict__ X,
+                                                       const double* __restrict__ Bv, const double* __restrict__ b,
+                                                       double* __restrict__ part) {
+  __shared__ double red[3 * kProjMax * (kBlock / kWave)];
+  double acc[3 * kProjMax];
+#pragma unroll
+  for (int q = 0; q < 3 * kProjMax; q++) acc[q] = 0.0;
   const int per = (n + kRedGrid - 1) / kRedGrid;
   const int lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
-  for (int k = 0; k < P.n; k++) {
-    const double* __restrict__ a = P.a[k];
-    const double* __restrict__ bb = P.b[k];
-    double v[1] = {0.0};
-    for (int i = lo + threadIdx.x; i < hi; i += kBlock) v[0] += a[i] * bb[i];
-    block_sum_all<1>(v, red);
-    if (threadIdx.x == 0) part[k * kRedGrid + blockIdx.x] = v[0];
-    __syncthreads();   // red is reused by the next pair
+  const size_t sn = (size_t)(s < 0 ? 0 : s) * n;
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const double bi = b[i], xs = s < 0 ? 0.0 : X[sn + i], bs = s < 0 ? 0.0 : Bv[sn + i];
+#pragma unroll
+    for (int j = 0; j < kProjMax; j++) {
+      if (j < m) {
+        const double xj = X[(size_t)j * n + i], bj = Bv[(size_t)j * n + i];
+        acc[j] += xj * bi;
+        acc[kProjMax + j] += xs * bj;
+        acc[2 * kProjMax + j] += xj * bs;
+      }
+    }
   }
+  block_sum_all<3 * kProjMax>(acc, red);
+  if (threadIdx.x == 0)
+    for (int j = 0; j < m; j++) {
+      part[j * kRedGrid + blockIdx.x] = acc[j];
+      if (s >= 0) {
+        part[(m + j) * kRedGrid + blockIdx.x] = acc[kProjMax + j];
+        part[(2 * m + j) * kRedGrid + blockIdx.x] = acc[2 * kProjMax + j];
+      }
+    }
 }
 
 // x = sum_j c_j x_j, in the order of j
@@ -1642,31 +1661,31 @@ static int proj_alloc(fsx_ctx* ctx) {
   const size_t N = ctx->NNp;
   FSX_TRY(alloc(ctx, &ctx->proj_x, kProjMax * N));
   FSX_TRY(alloc(ctx, &ctx->proj_b, kProjMax * N));
-  FSX_TRY(alloc(ctx, &ctx->proj_part, (size_t)kProjPairs * kRedGrid));
+  FSX_TRY(alloc(ctx, &ctx->proj_part, (size_t)kProjDots * kRedGrid));
   ctx->proj_G.assign(kProjMax * kProjMax, 0.0);
   proj_reset(ctx);
   return FSX_OK;
 }
 
 // x0 into x (left as it is while the window is empty); proj_pending is the
-// bit mask of slots whose row and column of G are still to be formed
+// bit mask of the slot recorded since the last guess (at most one: each
+// record follows a guess, and a guess that fails empties the window)
 static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
   FSX_TRY(proj_alloc(ctx));
   const int m = ctx->proj_n;
   if (m == 0) return FSX_OK;
   const size_t N = ctx->NNp;
   const unsigned stale = (unsigned)ctx->proj_pending;
-  ProjPairs P{};
+  const int one = stale ? __builtin_ctz(stale) : -1;
   int k = 0;
-  int gi[kProjPairs], gj[kProjPairs];
-  for (int i = 0; i < m; i++) { P.a[k] = ctx->proj_x + i * N; P.b[k] = b; gi[k] = -1; gj[k] = i; k++; }
-  for (int i = 0; i < m; i++)
-    for (int j = 0; j < m; j++)
-      if (((stale >> i) & 1u) || ((stale >> j) & 1u)) {
-        P.a[k] = ctx->proj_x + i * N; P.b[k] = ctx->proj_b + j * N; gi[k] = i; gj[k] = j; k++;
-      }
-  P.n = k;
-  hipLaunchKernelGGL(k_proj_dots, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, (int)N, P, ctx->proj_part);
+  int gi[kProjDots], gj[kProjDots];
+  for (int j = 0; j < m; j++) { gi[k] = -1; gj[k] = j; k++; }
+  if (one >= 0) {
+    for (int j = 0; j < m; j++) { gi[k] = one; gj[k] = j; k++; }
+    for (int j = 0; j < m; j++) { gi[k] = j; gj[k] = one; k++; }
+  }
+  hipLaunchKernelGGL(k_proj_dots1, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, (int)N, m, one, ctx->proj_x,
+                     ctx->proj_b, b, ctx->proj_part);
   FSX_TRY(launch_check(ctx, "k_proj_dots"));
   std::vector<double> h((size_t)k * kRedGrid);
   FSX_HIP(hipMemcpyAsync(h.data(), ctx->proj_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1676,7 +1695,10 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
   for (int q = 0; q < k; q++) {
     double t = 0.0;
     for (int blk = 0; blk < kRedGrid; blk++) t += h[(size_t)q * kRedGrid + blk];
-    if (!std::isfinite(t)) return FSX_OK;   // the guess is skipped; the solve meets its own test
+    if (!std::isfinite(t)) {   // the guess is skipped (the solve meets its own test) and the window emptied
+      proj_reset(ctx);
+      return FSX_OK;
+    }
     if (gi[q] < 0) d[gj[q]] = t;
     else G[gi[q] * kProjMax + gj[q]] = t;
   }
@@ -1720,16 +1742,29 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
   return launch_check(ctx, "k_proj_combine");
 }
 
-// the solution x of b into the window (the oldest slot when it is full)
+__global__ void k_sub_d(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = a[i] - b[i];
+}
+
+// the solution x of b into the window (the oldest slot when it is full),
+// with Z x: b less the FP64 residual the mixed and learned solves leave in
+// wh for their final x; the FP64 solver's recurrence residual is not kept,
+// so there Z x is taken as b
 static int proj_record(fsx_ctx* ctx, const double* b, const double* x) {
   FSX_TRY(proj_alloc(ctx));
   const size_t N = ctx->NNp;
   const int s = ctx->proj_next;
   FSX_TRY(launch_copy(ctx, ctx->proj_x + s * N, x, (long long)N));
-  FSX_TRY(launch_copy(ctx, ctx->proj_b + s * N, b, (long long)N));
+  if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED) {
+    hipLaunchKernelGGL(k_sub_d, dim3(ew_grid((long long)N)), dim3(kBlock), 0, ctx->stream, (int)N, b, ctx->wh,
+                       ctx->proj_b + s * N);
+    FSX_TRY(launch_check(ctx, "k_sub_d"));
+  } else {
+    FSX_TRY(launch_copy(ctx, ctx->proj_b + s * N, b, (long long)N));
+  }
   ctx->proj_next = (s + 1) % kProjMax;
   ctx->proj_n = std::min(kProjMax, ctx->proj_n + 1);
-  ctx->proj_pending |= 1 << s;
+  ctx->proj_pending = 1 << s;
   return FSX_OK;
 }
 

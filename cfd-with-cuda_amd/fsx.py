@@ -558,7 +558,7 @@ class Solver:
     def set_solver_x0(self, policy):
         """X0_ZERO (the CPU path), X0_PREVIOUS (warm start from the previous
         Pdot, the reference's GPU path) or X0_PROJECTION (from the Z-norm
-        projection onto the last 8 solutions)."""
+        projection onto the last 16 solutions)."""
         self.check(self.L.fsx_set_solver_x0(self.ctx, policy), "set_solver_x0")
 
     def direct_stats(self):

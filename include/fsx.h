@@ -87,7 +87,7 @@ enum fsx_solver_mode {
 enum fsx_x0_policy {
   FSX_X0_ZERO = 0,      /* x0 = 0, the CPU path (MKL RCI CG, BCH:4002-4004); the default */
   FSX_X0_PREVIOUS = 1,  /* the previous solution, the GPU path (CUSP keeps Pdot, CU:268, 425-442; Q6) */
-  FSX_X0_PROJECTION = 2 /* the Z-norm projection onto the last 8 solutions (no reference counterpart) */
+  FSX_X0_PROJECTION = 2 /* the Z-norm projection onto the last 16 solutions (no reference counterpart) */
 };
 
 /* Velocity mass for fsx_set_mass */
@@ -236,15 +236,17 @@ int fsx_set_convection_form(fsx_ctx* ctx, int elements);
  * tolerance scaled by |b|/|r0|, x = x0 + e, so the stop test is the cold
  * solve's |b - Z x| <= rtol |b|; an x0 already within it takes no
  * iteration. FSX_X0_PROJECTION starts from the best approximation of the
- * solution, in the Z norm, in the span of the last 8 solutions (projection
+ * solution, in the Z norm, in the span of the last 16 solutions (projection
  * for successive right-hand sides, after P. F. Fischer, CMAME 163 (1998)):
- * x0 = X a with (X^T Z X) a = X^T b, Z x_j taken as the b_j it was solved
- * for (to the solve's residual), vectors that add less than 1e-10 of their
+ * x0 = X a with (X^T Z X) a = X^T b, Z x_j kept from each solve's final
+ * FP64 residual (taken as the b_j it was solved for in the FP64 solver,
+ * which keeps no FP64 residual), vectors that add less than 1e-10 of their
  * own Z norm to the span dropped; then as FSX_X0_PREVIOUS from x0. The
  * window restarts when Z is built or the state initialised; a solve that
  * ends at maxit is not added. Same stop test; on the 64^3 cavity its x0
- * leaves a 23 times smaller residual than the previous solution. Not for
- * the direct solver (it takes no initial guess). */
+ * leaves a residual about 300 times smaller than the previous solution's
+ * (32 times with 8 solutions). Not for the direct solver (it takes no
+ * initial guess). */
 int fsx_set_solver_x0(fsx_ctx* ctx, int policy);
 /* Order and factor form of FSX_SOLVER_DIRECT (fsx_set_direct_order) */
 enum fsx_direct_order {

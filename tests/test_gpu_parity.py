@@ -534,7 +534,7 @@ def test_warm_start(fsx_lib, oracle_lib, mode):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_projection_start(fsx_lib, oracle_lib, mode):
-    """FSX_X0_PROJECTION (x0 the Z-norm projection onto the last 8 solutions)
+    """FSX_X0_PROJECTION (x0 the Z-norm projection onto the last 16 solutions)
     in the FP64, mixed and learned solvers against the oracle's
     warm-started PCG: fields after 6 steps to 1e-6 and every solve at its
     stop test; fewer iterations than FSX_X0_PREVIOUS over the same steps,
