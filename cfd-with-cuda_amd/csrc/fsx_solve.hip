@@ -1604,41 +1604,38 @@ struct ProjSum {
   int n;
 };
 
-// One pass over the vectors: X^T b and, when a slot s was recorded since the
-// last guess (s >= 0), its row and column of G. Block shares of x_j . b
-// (part rows 0..m-1), x_s . Z x_j (m..2m-1) and x_j . Z x_s (2m..3m-1),
-// each block a contiguous range.
+// X^T b and, when a slot s was recorded since the last guess (s >= 0), its
+// row and column of G: block (chunk, j) sums its contiguous chunk of x_j . b
+// (part row j), x_s . Z x_j (m + j) and x_j . Z x_s (2m + j). (One block a
+// chunk for all j, 3m sums a thread, took 123 us a guess at 64^3.)
 __global__ __launch_bounds__(kBlock) void k_proj_dots1(int n, int m, int s, const double* __restrict__ X,
                                                        const double* __restrict__ Bv, const double* __restrict__ b,
                                                        double* __restrict__ part) {
-  __shared__ double red[3 * kProjMax * (kBlock / kWave)];
-  double acc[3 * kProjMax];
-#pragma unroll
-  for (int q = 0; q < 3 * kProjMax; q++) acc[q] = 0.0;
+  __shared__ double red[3 * (kBlock / kWave)];
+  const int j = blockIdx.y;
   const int per = (n + kRedGrid - 1) / kRedGrid;
   const int lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
-  const size_t sn = (size_t)(s < 0 ? 0 : s) * n;
+  const double* __restrict__ xj = X + (size_t)j * n;
+  const double* __restrict__ zj = Bv + (size_t)j * n;
+  const double* __restrict__ xs = X + (size_t)(s < 0 ? 0 : s) * n;
+  const double* __restrict__ zs = Bv + (size_t)(s < 0 ? 0 : s) * n;
+  double v[3] = {0.0, 0.0, 0.0};
   for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const double bi = b[i], xs = s < 0 ? 0.0 : X[sn + i], bs = s < 0 ? 0.0 : Bv[sn + i];
-#pragma unroll
-    for (int j = 0; j < kProjMax; j++) {
-      if (j < m) {
-        const double xj = X[(size_t)j * n + i], bj = Bv[(size_t)j * n + i];
-        acc[j] += xj * bi;
-        acc[kProjMax + j] += xs * bj;
-        acc[2 * kProjMax + j] += xj * bs;
-      }
+    const double x = xj[i];
+    v[0] += x * b[i];
+    if (s >= 0) {
+      v[1] += xs[i] * zj[i];
+      v[2] += x * zs[i];
     }
   }
-  block_sum_all<3 * kProjMax>(acc, red);
-  if (threadIdx.x == 0)
-    for (int j = 0; j < m; j++) {
-      part[j * kRedGrid + blockIdx.x] = acc[j];
-      if (s >= 0) {
-        part[(m + j) * kRedGrid + blockIdx.x] = acc[kProjMax + j];
-        part[(2 * m + j) * kRedGrid + blockIdx.x] = acc[2 * kProjMax + j];
-      }
+  block_sum_all<3>(v, red);
+  if (threadIdx.x == 0) {
+    part[j * kRedGrid + blockIdx.x] = v[0];
+    if (s >= 0) {
+      part[(m + j) * kRedGrid + blockIdx.x] = v[1];
+      part[(2 * m + j) * kRedGrid + blockIdx.x] = v[2];
     }
+  }
 }
 
 // x = sum_j c_j x_j, in the order of j
@@ -1684,7 +1681,7 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
     for (int j = 0; j < m; j++) { gi[k] = one; gj[k] = j; k++; }
     for (int j = 0; j < m; j++) { gi[k] = j; gj[k] = one; k++; }
   }
-  hipLaunchKernelGGL(k_proj_dots1, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, (int)N, m, one, ctx->proj_x,
+  hipLaunchKernelGGL(k_proj_dots1, dim3(kRedGrid, m), dim3(kBlock), 0, ctx->stream, (int)N, m, one, ctx->proj_x,
                      ctx->proj_b, b, ctx->proj_part);
   FSX_TRY(launch_check(ctx, "k_proj_dots"));
   std::vector<double> h((size_t)k * kRedGrid);
