@@ -137,24 +137,29 @@ __global__ void k_publish(const SolveScalars* __restrict__ sc, Published* pub, u
 // (a copy plus hipStreamSynchronize costs tens of microseconds of idle GPU
 // per solver decision). The stream is queried every few thousand spins so
 // that a failed kernel or a lost publication ends the wait with an error.
-int read_scalars(fsx_ctx* ctx, const SolveScalars* sc) {
-  static_assert(sizeof(SolveScalars) % sizeof(unsigned) == 0, "SolveScalars is copied as 32-bit words");
-  const unsigned seq = ++ctx->pub_seq;
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kWave), 0, ctx->stream, sc ? sc : ctx->sc, ctx->d_pub, seq);
-  FSX_TRY(launch_check(ctx, "k_publish"));
+int wait_published(fsx_ctx* ctx, unsigned seq, const char* what) {
   volatile unsigned* flag = &ctx->h_pub->seq;
   for (long spin = 1; *flag != seq; spin++) {
     if ((spin & 4095) == 0) {
       const hipError_t e = hipStreamQuery(ctx->stream);
       if (e != hipSuccess && e != hipErrorNotReady)
-        return fail(ctx, FSX_EHIP, std::string("solver scalars: ") + hipGetErrorString(e));
+        return fail(ctx, FSX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
       if (e == hipSuccess) {
         std::atomic_thread_fence(std::memory_order_seq_cst);
-        if (*flag != seq) return fail(ctx, FSX_EHIP, "solver scalars: publication not visible after the stream drained");
+        if (*flag != seq) return fail(ctx, FSX_EHIP, std::string(what) + ": publication not visible after the stream drained");
       }
     }
   }
   std::atomic_thread_fence(std::memory_order_acquire);
+  return FSX_OK;
+}
+
+int read_scalars(fsx_ctx* ctx, const SolveScalars* sc) {
+  static_assert(sizeof(SolveScalars) % sizeof(unsigned) == 0, "SolveScalars is copied as 32-bit words");
+  const unsigned seq = ++ctx->pub_seq;
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(kWave), 0, ctx->stream, sc ? sc : ctx->sc, ctx->d_pub, seq);
+  FSX_TRY(launch_check(ctx, "k_publish"));
+  FSX_TRY(wait_published(ctx, seq, "solver scalars"));
   std::memcpy(ctx->h_sc, const_cast<const SolveScalars*>(&ctx->h_pub->s), sizeof(SolveScalars));
   return FSX_OK;
 }

@@ -46,10 +46,16 @@ struct SolveScalars {
 
 // SolveScalars as published to host-coherent pinned memory by k_publish:
 // the host spins on seq instead of a copy plus a stream synchronisation.
+// FSX_X0_PROJECTION: solutions kept, and the dot products one guess forms
+// (X^T b and one kept slot's row and column of the Gram matrix)
+constexpr int kProjMax = 16;
+constexpr int kProjDots = 3 * kProjMax;
+
 struct Published {
   SolveScalars s;
   unsigned seq;
   unsigned pad[3];
+  double proj[kProjDots];   // the projection start's dot products (k_proj_publish)
 };
 
 // The pressure operator Z staged through LDS on a structured nx*ny*nz grid
@@ -386,6 +392,9 @@ struct fsx_ctx {
   double *proj_x = nullptr, *proj_b = nullptr, *proj_part = nullptr;
   int proj_n = 0, proj_next = 0, proj_pending = 0;   // pending: bit mask of slots
   std::vector<double> proj_G;
+  // the FP64 residual b - Z x0 the projection start formed beside x0 (the
+  // buffer the next warm start would compute it into), or null
+  double* warm_r = nullptr;
   int x0_policy = FSX_X0_ZERO;
   float *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr, *fx = nullptr;     // FP32 PCG
   double* fpq = nullptr; int fpq_n = 0;   // learned solver: partial sums of p.Zp from its SpMV
@@ -504,6 +513,8 @@ int launch_nodes_pass(fsx_ctx* ctx, bool kacc, bool gload);
 int refresh_matrix_copies(fsx_ctx* ctx);
 // empties the FSX_X0_PROJECTION window (fsx_solve.hip)
 void proj_reset(fsx_ctx* ctx);
+// host spin until the pinned publication carries sequence number seq
+int wait_published(fsx_ctx* ctx, unsigned seq, const char* what);
 // whether step3 and step2's right-hand side read G and G^T as row classes
 inline bool g_classes_on(const fsx_ctx* ctx) { return ctx->Gf && ctx->GTf && ctx->Gc_row && ctx->use_g_classes; }
 int launch_mass_fac(fsx_ctx* ctx, const int* P8, double* fac);

@@ -1503,7 +1503,8 @@ __global__ __launch_bounds__(kBlock) void k_sumsq2(int n, const double* __restri
 // (slot 1) of ctx->partials, with no read-back (the tiled learned solver's
 // warm start takes them on the device)
 int launch_warm_residual(fsx_ctx* ctx, const double* x, const double* b, double* rh) {
-  launch_zresidual(ctx, x, b, rh);
+  if (ctx->warm_r != rh) launch_zresidual(ctx, x, b, rh);   // else the projection start formed it
+  ctx->warm_r = nullptr;
   hipLaunchKernelGGL(k_sumsq2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, ctx->NNp, b, rh, ctx->partials);
   return launch_check(ctx, "warm residual");
 }
@@ -1559,7 +1560,8 @@ static int solve_cold(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
 // within the tolerance takes no iteration.
 static int solve_warm(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres) {
   const int n = ctx->NNp;
-  launch_zresidual(ctx, x, b, ctx->w0r);
+  if (ctx->warm_r != ctx->w0r) launch_zresidual(ctx, x, b, ctx->w0r);   // else the projection start formed it
+  ctx->warm_r = nullptr;
   hipLaunchKernelGGL(k_sumsq2, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, n, b, ctx->w0r, ctx->partials);
   FSX_TRY(launch_check(ctx, "k_sumsq2"));
   double h[2 * kRedGrid];
@@ -1593,13 +1595,12 @@ static int solve_warm(fsx_ctx* ctx, const double* b, double* x, int* iters, doub
 // x_i . (Z x_j) needs no product with Z; the Gram system is solved on the
 // host by a Cholesky factorization that drops the vectors adding under
 // 1e-10 of their own Z norm to the span (newest first).
-// 16 kept solutions: 64^3 cavity, learned solve 48.4 ms over 20 steps (8:
-// 53.2 ms with Z x_j taken as b_j; 24: 51.1 ms, the longer dot and
+// 16 kept solutions (kProjMax): 64^3 cavity, learned solve 48.4 ms over 20
+// steps (8: 53.2 ms with Z x_j taken as b_j; 24: 51.1 ms, the longer dot and
 // combination passes outweighing the iterations saved)
-constexpr int kProjMax = 16;
-constexpr int kProjDots = 3 * kProjMax;   // X^T b and one slot's row and column of G
 struct ProjSum {
   const double* x[kProjMax];
+  const double* z[kProjMax];   // Z x of each
   double c[kProjMax];
   int n;
 };
@@ -1638,12 +1639,35 @@ __global__ __launch_bounds__(kBlock) void k_proj_dots1(int n, int m, int s, cons
   }
 }
 
-// x = sum_j c_j x_j, in the order of j
-__global__ void k_proj_combine(int n, ProjSum S, double* __restrict__ x) {
+// The nq dot products from k_proj_dots1's partials (each summed by one
+// thread in block order) into the pinned publication, then its sequence
+// number: the host waits on it instead of a copy and a stream sync.
+__global__ __launch_bounds__(kBlock) void k_proj_publish(int nq, const double* __restrict__ part, Published* pub,
+                                                         unsigned seq) {
+  if (threadIdx.x < nq) {
+    const double* p = part + threadIdx.x * kRedGrid;
+    double t = 0.0;
+#pragma unroll 8
+    for (int blk = 0; blk < kRedGrid; blk++) t += p[blk];
+    reinterpret_cast<volatile double*>(pub->proj)[threadIdx.x] = t;
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) *reinterpret_cast<volatile unsigned*>(&pub->seq) = seq;
+}
+
+// x = sum_j c_j x_j and r = b - sum_j c_j Z x_j (= b - Z x to rounding), in
+// the order of j
+__global__ void k_proj_combine(int n, ProjSum S, const double* __restrict__ b, double* __restrict__ x,
+                               double* __restrict__ r) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double v = 0.0;
-    for (int j = 0; j < S.n; j++) v += S.c[j] * S.x[j][i];
+    double v = 0.0, z = 0.0;
+    for (int j = 0; j < S.n; j++) {
+      v += S.c[j] * S.x[j][i];
+      z += S.c[j] * S.z[j][i];
+    }
     x[i] = v;
+    r[i] = b[i] - z;
   }
 }
 
@@ -1664,10 +1688,12 @@ static int proj_alloc(fsx_ctx* ctx) {
   return FSX_OK;
 }
 
-// x0 into x (left as it is while the window is empty); proj_pending is the
-// bit mask of the slot recorded since the last guess (at most one: each
-// record follows a guess, and a guess that fails empties the window)
-static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
+// x0 into x (left as it is while the window is empty), and r = b - Z x0
+// into r, then ctx->warm_r = r (the warm start then skips that product);
+// proj_pending is the bit mask of the slot recorded since the last guess (at
+// most one: each record follows a guess, and a guess that fails empties the
+// window)
+static int proj_guess(fsx_ctx* ctx, const double* b, double* x, double* r) {
   FSX_TRY(proj_alloc(ctx));
   const int m = ctx->proj_n;
   if (m == 0) return FSX_OK;
@@ -1683,15 +1709,14 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
   }
   hipLaunchKernelGGL(k_proj_dots1, dim3(kRedGrid, m), dim3(kBlock), 0, ctx->stream, (int)N, m, one, ctx->proj_x,
                      ctx->proj_b, b, ctx->proj_part);
+  const unsigned seq = ++ctx->pub_seq;
+  hipLaunchKernelGGL(k_proj_publish, dim3(1), dim3(kBlock), 0, ctx->stream, k, ctx->proj_part, ctx->d_pub, seq);
   FSX_TRY(launch_check(ctx, "k_proj_dots"));
-  std::vector<double> h((size_t)k * kRedGrid);
-  FSX_HIP(hipMemcpyAsync(h.data(), ctx->proj_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  FSX_HIP(hipStreamSynchronize(ctx->stream));
+  FSX_TRY(wait_published(ctx, seq, "projection start"));
   double d[kProjMax] = {0.0};
   std::vector<double>& G = ctx->proj_G;
   for (int q = 0; q < k; q++) {
-    double t = 0.0;
-    for (int blk = 0; blk < kRedGrid; blk++) t += h[(size_t)q * kRedGrid + blk];
+    const double t = const_cast<const double*>(ctx->h_pub->proj)[q];
     if (!std::isfinite(t)) {   // the guess is skipped (the solve meets its own test) and the window emptied
       proj_reset(ctx);
       return FSX_OK;
@@ -1734,9 +1759,15 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x) {
   }
   ProjSum S{};
   S.n = ns;
-  for (int q = 0; q < ns; q++) { S.x[q] = ctx->proj_x + sel[q] * N; S.c[q] = a[q]; }
-  hipLaunchKernelGGL(k_proj_combine, dim3(ew_grid((long long)N)), dim3(kBlock), 0, ctx->stream, (int)N, S, x);
-  return launch_check(ctx, "k_proj_combine");
+  for (int q = 0; q < ns; q++) {
+    S.x[q] = ctx->proj_x + sel[q] * N;
+    S.z[q] = ctx->proj_b + sel[q] * N;
+    S.c[q] = a[q];
+  }
+  hipLaunchKernelGGL(k_proj_combine, dim3(ew_grid((long long)N)), dim3(kBlock), 0, ctx->stream, (int)N, S, b, x, r);
+  FSX_TRY(launch_check(ctx, "k_proj_combine"));
+  ctx->warm_r = r;
+  return FSX_OK;
 }
 
 __global__ void k_sub_d(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
@@ -1773,10 +1804,13 @@ int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double*
   const bool direct = ctx->solver_mode == FSX_SOLVER_DIRECT;
   const bool proj = ctx->x0_policy == FSX_X0_PROJECTION && !direct;
   const bool warm = (ctx->x0_policy == FSX_X0_PREVIOUS || proj) && !direct;
-  if (proj) FSX_TRY(proj_guess(ctx, b, x));
+  // the projection start also forms b - Z x0 where the warm start reads it
+  ctx->warm_r = nullptr;
+  if (proj) FSX_TRY(proj_guess(ctx, b, x, learned_tiled(ctx) ? ctx->wh : ctx->w0r));
   if (warm && learned_tiled(ctx)) rc = solve_learned_tiled(ctx, b, x, true, iters, relres, 0);
   else if (warm) rc = solve_warm(ctx, b, x, iters, relres);
   else rc = solve_cold(ctx, b, x, iters, relres);
+  ctx->warm_r = nullptr;
   if (rc == FSX_OK || rc == FSX_ENOCONV) timing_end(ctx, *iters);
   ctx->last_relres = *relres;
   if (proj && rc == FSX_OK) FSX_TRY(proj_record(ctx, b, x));
