@@ -608,14 +608,30 @@ static int build_z_forms(fsx_ctx* ctx) {
   // took twice the iterations, 51 against 25 a solve.)
   ctx->tile = TilePlan{};
   ctx->z_tensor_err = -1.0;
-  if (ctx->pg_nx > 0 && classes) {
+  if (ctx->zt64) dev_free(ctx, ctx->zt64);
+  ctx->zt64 = nullptr;
+  if (ctx->pg_nx > 0) {
     std::vector<double> xc(3 * (size_t)n);
     FSX_HIP(hipMemcpy(xc.data(), ctx->xc, xc.size() * sizeof(double), hipMemcpyDeviceToHost));
     ZTensor zt;
     if (z_tensor_tables(ctx->pg_nx, ctx->pg_ny, ctx->pg_nz, xc.data(), ctx->NGP, ctx->density, &zt) == 0) {
       const int held = z_tensor_check(n, rp2.data(), ci2.data(), v2.data(), ctx->zeroP, 1e-10, &zt);
       ctx->z_tensor_err = zt.err;
-      if (held == 0 && ctx->use_z_tensor) FSX_TRY(build_tile_plan_tensor(ctx, zt));
+      if (held == 0 && ctx->use_z_tensor) {
+        if (classes) FSX_TRY(build_tile_plan_tensor(ctx, zt));
+        // the FP64 tables for the mixed and learned FP64 residuals
+        std::vector<double> c64;
+        for (int d = 0; d < 3; d++)
+          for (int i = 0; i < zt.n[d]; i++) {
+            for (int c = 0; c < 5; c++) c64.push_back(zt.k[d][5 * (size_t)i + c]);
+            for (int c = 0; c < 5; c++) c64.push_back(zt.m[d][5 * (size_t)i + c]);
+          }
+        FSX_TRY(alloc(ctx, &ctx->zt64, c64.size()));
+        FSX_HIP(hipMemcpy(ctx->zt64, c64.data(), c64.size() * sizeof(double), hipMemcpyHostToDevice));
+        for (int d = 0; d < 3; d++) ctx->zt_n[d] = zt.n[d];
+        ctx->zt_zp = zt.zero_p;
+        ctx->zt_pen = zt.pen;
+      }
     }
   }
   if (ctx->tile.ntiles == 0) {

@@ -262,6 +262,13 @@ struct fsx_ctx {
   long long gc_nent = 0, gtc_nent = 0;
   bool use_g_classes = true;
   bool use_z_tensor = true;    // the tiles' Kronecker form of Z where it holds (fsx_set_z_tensor)
+  // Z's Kronecker form in FP64 for the mixed and learned solvers' FP64
+  // residuals (per axis and grid point K's then M's five coefficients, the
+  // x axis first), null where the form does not hold
+  double* zt64 = nullptr;
+  int zt_n[3] = {0, 0, 0};
+  int zt_zp = -1;
+  double zt_pen = 0.0;
   double z_tensor_err = -1.0;  // its largest mismatch against Z's rows (-1: the grid is not a box grid)
   bool gc_built = false;       // the classes (or their absence) are of the current assembly of G
   int mat_bits = 64;

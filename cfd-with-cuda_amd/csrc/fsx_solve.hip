@@ -1309,10 +1309,58 @@ __global__ __launch_bounds__(kBlock) void k_zcls_residual(int nchunk, const int4
   if (lane < len) rh[r] = b[r] - s;
 }
 
+// b - Z x in FP64 on a tensor-product box grid from Z's Kronecker form
+// (Kx My Mz + Mx Ky Mz + Mx My Kz, FP64 axis tables; the pinned node's
+// penalty on its diagonal): one row a thread, per (dy, dz) neighbour line
+// the K and M sums along x, then their weights. The tables are zero for
+// couplings past the grid's ends, so the loads clamp to the grid and carry
+// no branches.
+__global__ __launch_bounds__(kBlock) void k_zten_residual(int nx, int ny, int nz, const double* __restrict__ coef,
+                                                          int zp, double pen, const double* __restrict__ x,
+                                                          const double* __restrict__ b, double* __restrict__ r) {
+  const int id = blockIdx.x * kBlock + threadIdx.x;
+  const int nxy = nx * ny;
+  if (id >= nxy * nz) return;
+  const int k = id / nxy, rem = id - k * nxy, j = rem / nx, i = rem - j * nx;
+  const double* cx = coef + 10 * i;
+  const double* cy = coef + 10 * (nx + j);
+  const double* cz = coef + 10 * (nx + ny + k);
+  int xo[5];
+#pragma unroll
+  for (int d = 0; d < 5; d++) xo[d] = min(max(i + d - 2, 0), nx - 1);
+  double acc = 0.0;
+#pragma unroll
+  for (int dz = 0; dz < 5; dz++) {
+    const int kk = min(max(k + dz - 2, 0), nz - 1);
+#pragma unroll
+    for (int dy = 0; dy < 5; dy++) {
+      const int jj = min(max(j + dy - 2, 0), ny - 1);
+      const double* line = x + (size_t)kk * nxy + (size_t)jj * nx;
+      double tk = 0.0, tm = 0.0;
+#pragma unroll
+      for (int dx = 0; dx < 5; dx++) {
+        const double v = line[xo[dx]];
+        tk += cx[dx] * v;
+        tm += cx[5 + dx] * v;
+      }
+      acc += tk * (cy[5 + dy] * cz[5 + dz]) + tm * (cy[dy] * cz[5 + dz] + cy[5 + dy] * cz[dz]);
+    }
+  }
+  if (id == zp) acc += pen * x[id];
+  r[id] = b[id] - acc;
+}
+
 static void launch_zresidual(fsx_ctx* ctx, const double* xh, const double* b, double* rh) {
   constexpr int LPR = PcgCfg<double>::LPR, U = PcgCfg<double>::U;
   const ColSrc cs = zr_cols(ctx);
   const int n = ctx->NNp;
+  // the mixed and learned solvers' residuals from Z's Kronecker form where
+  // it holds (vectors only: 7 MB instead of the row classes' 100 MB)
+  if (ctx->zt64 && (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED)) {
+    hipLaunchKernelGGL(k_zten_residual, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->zt_n[0],
+                       ctx->zt_n[1], ctx->zt_n[2], ctx->zt64, ctx->zt_zp, ctx->zt_pen, xh, b, rh);
+    return;
+  }
   // the mixed and learned solvers' residuals on the row classes with FP64
   // corrections (half the bytes of the FP64 values); the FP64 and direct
   // solvers keep the FP64 values
