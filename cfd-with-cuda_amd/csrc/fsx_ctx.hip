@@ -911,9 +911,9 @@ static int need_step(fsx_ctx* ctx) {
 int fsx_begin_step(fsx_ctx* ctx) {
   FSX_TRY(need_step(ctx));
   FSX_TRY(settle_deferred(ctx));
-  FSX_TRY(launch_copy(ctx, ctx->UnpHalf_prev, ctx->Un, 3LL * ctx->NN));    // BCH:2872-2876
+  FSX_TRY(launch_copy2(ctx, ctx->UnpHalf_prev, ctx->Un, 3LL * ctx->NN, ctx->Pnp1_prev, ctx->Pn,
+                       ctx->NNp));    // BCH:2872-2876
   FSX_TRY(launch_fill(ctx, ctx->Acc_prev, 0.0, 3LL * ctx->NN));
-  FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pn, ctx->NNp));
   return records_begin_step(ctx);
 }
 
@@ -968,8 +968,7 @@ int fsx_next_iteration(fsx_ctx* ctx) {
   std::swap(ctx->arec_prev_ok, ctx->arec_ok);
   ctx->drec_ok = false;   // step1 of the next inner iteration writes it again
   ctx->ucmp_ok = false;   // UnpHalf_prev is no longer Un
-  FSX_TRY(launch_copy(ctx, ctx->Unp1_prev, ctx->Unp1, 3LL * ctx->NN));
-  FSX_TRY(launch_copy(ctx, ctx->Pnp1_prev, ctx->Pnp1, ctx->NNp));
+  FSX_TRY(launch_copy2(ctx, ctx->Unp1_prev, ctx->Unp1, 3LL * ctx->NN, ctx->Pnp1_prev, ctx->Pnp1, ctx->NNp));
   return FSX_OK;
 }
 

@@ -531,6 +531,7 @@ int mass_update_step3(fsx_ctx* ctx);
 int launch_maxacc(fsx_ctx* ctx);
 int launch_fill(fsx_ctx* ctx, double* p, double v, long long n);
 int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n);
+int launch_copy2(fsx_ctx* ctx, double* d1, const double* s1, long long n1, double* d2, const double* s2, long long n2);
 int pressure_solve(fsx_ctx* ctx, const double* b, double* x, int* iters, double* relres);
 int spmv_host_op(fsx_ctx* ctx, int which, const double* dx, double* dy, double alpha, double beta);
 int launch_add(fsx_ctx* ctx, double* x, const double* d, int n);

@@ -32,6 +32,15 @@ __global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, lon
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
+// two copies in one launch: d1 = s1 (n1 values), then d2 = s2 (n2 values)
+__global__ void k_copy2(double* __restrict__ d1, const double* __restrict__ s1, long long n1, double* __restrict__ d2,
+                        const double* __restrict__ s2, long long n2) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n1 + n2; i += (long long)gridDim.x * blockDim.x) {
+    if (i < n1) d1[i] = s1[i];
+    else d2[i - n1] = s2[i - n1];
+  }
+}
+
 static int ew_grid(long long n) {
   long long g = (n + kBlock - 1) / kBlock;
   return (int)std::max(1LL, std::min(g, 4096LL));
@@ -49,6 +58,10 @@ int launch_fill(fsx_ctx* ctx, double* p, double v, long long n) {
 int launch_copy(fsx_ctx* ctx, double* dst, const double* src, long long n) {
   hipLaunchKernelGGL(k_copy, dim3(ew_grid(n)), dim3(kBlock), 0, ctx->stream, dst, src, n);
   return launch_check(ctx, "k_copy");
+}
+int launch_copy2(fsx_ctx* ctx, double* d1, const double* s1, long long n1, double* d2, const double* s2, long long n2) {
+  hipLaunchKernelGGL(k_copy2, dim3(ew_grid(n1 + n2)), dim3(kBlock), 0, ctx->stream, d1, s1, n1, d2, s2, n2);
+  return launch_check(ctx, "k_copy2");
 }
 
 // Row of this lane group and the lane inside the group. Blocks take rows in
@@ -1687,17 +1700,23 @@ __global__ __launch_bounds__(kBlock) void k_proj_dots1(int n, int m, int s, cons
   }
 }
 
-// The nq dot products from k_proj_dots1's partials (each summed by one
-// thread in block order) into the pinned publication, then its sequence
-// number: the host waits on it instead of a copy and a stream sync.
-__global__ __launch_bounds__(kBlock) void k_proj_publish(int nq, const double* __restrict__ part, Published* pub,
-                                                         unsigned seq) {
-  if (threadIdx.x < nq) {
-    const double* p = part + threadIdx.x * kRedGrid;
+// The nq dot products from k_proj_dots1's partials (a wavefront per dot
+// product: each lane sums a strided quarter of its kRedGrid partials, then
+// the lanes fold by shuffles) into the pinned publication, then its
+// sequence number: the host waits on it instead of a copy and a stream sync.
+// (One thread per dot product summing its partials in a chain took 10 us.)
+constexpr int kPubBlock = 1024;
+__global__ __launch_bounds__(kPubBlock) void k_proj_publish(int nq, const double* __restrict__ part, Published* pub,
+                                                            unsigned seq) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int q = threadIdx.x / kWave; q < nq; q += kPubBlock / kWave) {
+    const double* p = part + q * kRedGrid;
     double t = 0.0;
-#pragma unroll 8
-    for (int blk = 0; blk < kRedGrid; blk++) t += p[blk];
-    reinterpret_cast<volatile double*>(pub->proj)[threadIdx.x] = t;
+#pragma unroll
+    for (int blk = lane; blk < kRedGrid; blk += kWave) t += p[blk];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) t += __shfl_down(t, o, kWave);
+    if (lane == 0) reinterpret_cast<volatile double*>(pub->proj)[q] = t;
   }
   __threadfence_system();
   __syncthreads();
@@ -1758,7 +1777,7 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x, double* r) {
   hipLaunchKernelGGL(k_proj_dots1, dim3(kRedGrid, m), dim3(kBlock), 0, ctx->stream, (int)N, m, one, ctx->proj_x,
                      ctx->proj_b, b, ctx->proj_part);
   const unsigned seq = ++ctx->pub_seq;
-  hipLaunchKernelGGL(k_proj_publish, dim3(1), dim3(kBlock), 0, ctx->stream, k, ctx->proj_part, ctx->d_pub, seq);
+  hipLaunchKernelGGL(k_proj_publish, dim3(1), dim3(kPubBlock), 0, ctx->stream, k, ctx->proj_part, ctx->d_pub, seq);
   FSX_TRY(launch_check(ctx, "k_proj_dots"));
   FSX_TRY(wait_published(ctx, seq, "projection start"));
   double d[kProjMax] = {0.0};
@@ -1818,8 +1837,13 @@ static int proj_guess(fsx_ctx* ctx, const double* b, double* x, double* r) {
   return FSX_OK;
 }
 
-__global__ void k_sub_d(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = a[i] - b[i];
+// one window slot: xs = x and zs = b - w (w = 0: zs = b)
+__global__ void k_proj_store(int n, const double* __restrict__ x, const double* __restrict__ b,
+                             const double* __restrict__ w, double* __restrict__ xs, double* __restrict__ zs) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    xs[i] = x[i];
+    zs[i] = w ? b[i] - w[i] : b[i];
+  }
 }
 
 // the solution x of b into the window (the oldest slot when it is full),
@@ -1830,14 +1854,10 @@ static int proj_record(fsx_ctx* ctx, const double* b, const double* x) {
   FSX_TRY(proj_alloc(ctx));
   const size_t N = ctx->NNp;
   const int s = ctx->proj_next;
-  FSX_TRY(launch_copy(ctx, ctx->proj_x + s * N, x, (long long)N));
-  if (ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED) {
-    hipLaunchKernelGGL(k_sub_d, dim3(ew_grid((long long)N)), dim3(kBlock), 0, ctx->stream, (int)N, b, ctx->wh,
-                       ctx->proj_b + s * N);
-    FSX_TRY(launch_check(ctx, "k_sub_d"));
-  } else {
-    FSX_TRY(launch_copy(ctx, ctx->proj_b + s * N, b, (long long)N));
-  }
+  const bool resid = ctx->solver_mode == FSX_SOLVER_MIXED_IR || ctx->solver_mode == FSX_SOLVER_LEARNED;
+  hipLaunchKernelGGL(k_proj_store, dim3(ew_grid((long long)N)), dim3(kBlock), 0, ctx->stream, (int)N, x, b,
+                     resid ? (const double*)ctx->wh : nullptr, ctx->proj_x + s * N, ctx->proj_b + s * N);
+  FSX_TRY(launch_check(ctx, "k_proj_store"));
   ctx->proj_next = (s + 1) % kProjMax;
   ctx->proj_n = std::min(kProjMax, ctx->proj_n + 1);
   ctx->proj_pending = 1 << s;
